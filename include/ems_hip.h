@@ -1,0 +1,159 @@
+/*
+ * ems_hip.h — MI355X (gfx950) EC-FDTD engine, C ABI.
+ *
+ * The drop-in boundary for openEMS's timestep loop.  An engine handle owns the
+ * device copies of the operator, the fields and every engine extension, and
+ * advances them with hand-written HIP kernels.  Host pointers passed in use
+ * the reference's layouts (N-I-J-K float32 for fields and coefficients);
+ * the device layout (structure of arrays, x fastest) stays internal.
+ *
+ * Reference interfaces replaced (paths relative to the openEMS tree):
+ *   ems_create            Operator::CreateEngine / Engine::Init           FDTD/operator.cpp:68-72, FDTD/engine.cpp:51-59
+ *   ems_set_excitation    Operator_Ext_Excitation -> Engine_Ext_Excitation FDTD/extensions/engine_ext_excitation.cpp:24-99
+ *   ems_add_upml          Operator_Ext_UPML -> Engine_Ext_UPML            FDTD/extensions/engine_ext_upml.cpp:24-234
+ *   ems_add_mur           Operator_Ext_Mur_ABC -> Engine_Ext_Mur_ABC      FDTD/extensions/engine_ext_mur_abc.cpp:28-178
+ *   ems_add_lorentz       Engine_Ext_LorentzMaterial / Engine_Ext_Dispersive
+ *                                                    FDTD/extensions/engine_ext_lorentzmaterial.cpp:79-173
+ *   ems_add_lumped_rlc    Engine_Ext_LumpedRLC                            FDTD/extensions/engine_ext_lumpedRLC.cpp:83-147
+ *   ems_iterate           Engine::IterateTS                              FDTD/engine.cpp:267-286
+ *   ems_get_numts         Engine::GetNumberOfTimesteps                   FDTD/engine.h:74
+ *   ems_get_volt/curr     Engine::GetVolt / GetCurr (bulk)                FDTD/engine.h:77-93
+ *   ems_set_volt/curr     Engine::SetVolt / SetCurr (bulk)                FDTD/engine.h:95-101
+ *   ems_add_probe         ProcessVoltage / ProcessCurrent / ProcessFieldProbe sampling
+ *                                                    Common/processvoltage.cpp:37-47, Common/processcurrent.cpp:96-171,
+ *                                                    Common/processfieldprobe.cpp:77-92
+ *   ems_add_fd_dump       ProcessFieldsFD::Process                       Common/processfields_fd.cpp:72-107
+ *   ems_energy            Engine_Interface_FDTD::CalcFastEnergy          FDTD/engine_interface_fdtd.cpp:302-347
+ *   ems_destroy           Engine::~Engine / Engine::Reset                FDTD/engine.cpp:45-108
+ *
+ * Conventions: every int-returning function returns EMS_OK (0) or a negative
+ * EMS_E_* code and never exits the process; ems_last_error() gives the
+ * message for the calling thread.  A handle is single-caller and not
+ * re-entrant.  Work is queued on the handle's HIP stream; ems_iterate returns
+ * after the queued steps have completed unless ems_set_async(h, 1) was called.
+ */
+#ifndef EMS_HIP_H
+#define EMS_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EMS_OK 0
+#define EMS_E_ARG (-1)
+#define EMS_E_STATE (-2)
+#define EMS_E_HIP (-3)
+#define EMS_E_NOMEM (-4)
+#define EMS_E_NODEV (-5)
+
+typedef struct ems_engine ems_engine;
+
+const char* ems_last_error(void);
+/* number of visible HIP devices (0 when no GPU is present) */
+int ems_device_count(void);
+
+/* numLines[3]; vv, vi, ii, iv: host float32 [3][Nx][Ny][Nz] (reference N-I-J-K).
+ * device: HIP device ordinal. */
+int ems_create(ems_engine** out, const unsigned numLines[3], const float* vv, const float* vi,
+               const float* ii, const float* iv, int device);
+
+/* Excitation signal (length samples, volt at n*dT, curr at (n+1/2)*dT),
+ * period in timesteps (0 when aperiodic) and the voltage / current cell lists:
+ * idx [3][K] (x row, y row, z row), dir [K], amp [K], delay [K] (timesteps). */
+int ems_set_excitation(ems_engine* h, const float* sig_volt, const float* sig_curr, unsigned length,
+                       int period_steps,
+                       unsigned nvolt, const unsigned* vidx, const unsigned short* vdir, const float* vamp,
+                       const unsigned* vdelay,
+                       unsigned ncurr, const unsigned* cidx, const unsigned short* cdir, const float* camp,
+                       const unsigned* cdelay);
+
+/* One UPML slab in creation order: start[3], num[3] (lines), six arrays
+ * [3][num0][num1][num2]: vv, vvfo, vvfn, ii, iifo, iifn. */
+int ems_add_upml(ems_engine* h, const unsigned start[3], const unsigned num[3], const float* vv,
+                 const float* vvfo, const float* vvfn, const float* ii, const float* iifo,
+                 const float* iifn);
+
+/* One Mur face in creation order: normal ny, face line, neighbour line,
+ * coefficients [N[ny+1]][N[ny+2]] for the two tangential components, and the
+ * first timestep at which the boundary is active. */
+int ems_add_mur(ems_engine* h, int ny, unsigned line, unsigned line_shift, const float* coeff_nyP,
+                const float* coeff_nyPP, unsigned start_ts);
+
+/* Lumped RLC elements (ADE form): pos [3][count], dir [count], and nine
+ * coefficient arrays [count]: i2v, ilv, vvd, vv2, vj1, vj2, ib0, b1, b2. */
+int ems_add_lumped_rlc(ems_engine* h, unsigned count, const unsigned* pos, const int* dir,
+                       const float* const coeffs[9]);
+
+/* Lorentz/Drude/Debye ADE of one dispersion order: cell list pos [3][count],
+ * coefficients [3][count] (v_int, v_ext, v_lor, i_int, i_ext, i_lor; the
+ * *_lor arrays may be NULL when the order has no Lorentz pole). */
+int ems_add_lorentz(ems_engine* h, unsigned count, const unsigned* pos, const float* v_int,
+                    const float* v_ext, const float* v_lor, const float* i_int, const float* i_ext,
+                    const float* i_lor, int volt_on, int curr_on);
+
+/* Finish set-up (extension ordering, device tables).  Called implicitly by
+ * the first ems_iterate. */
+int ems_init(ems_engine* h);
+
+int ems_iterate(ems_engine* h, unsigned nsteps);
+unsigned ems_get_numts(const ems_engine* h);
+int ems_set_async(ems_engine* h, int async);
+int ems_sync(ems_engine* h);
+
+/* whole-field transfers in the reference N-I-J-K layout, [3][Nx][Ny][Nz] */
+int ems_get_volt(ems_engine* h, float* out);
+int ems_get_curr(ems_engine* h, float* out);
+int ems_set_volt(ems_engine* h, const float* in);
+int ems_set_curr(ems_engine* h, const float* in);
+
+/* ---- probes (evaluated on the device at their sampling steps) ---- */
+#define EMS_PROBE_VOLTAGE 0   /* line integral of voltages, double accumulation   */
+#define EMS_PROBE_CURRENT 1   /* closed loop of currents, float accumulation       */
+#define EMS_PROBE_EFIELD 2    /* E = V / edge length at one node (3 components)    */
+#define EMS_PROBE_HFIELD 3    /* H = I / dual edge length at one node              */
+
+typedef struct ems_probe_desc {
+    int type;                     /* EMS_PROBE_*                                      */
+    unsigned nelem;               /* voltage/current: number of terms; points: 3       */
+    const unsigned* comp;         /* [nelem] field component of each term               */
+    const unsigned* pos;          /* [3][nelem] mesh index of each term                 */
+    const int* sign;              /* [nelem] +1 / -1 (voltage, current)                 */
+    const double* divisor;        /* [nelem] edge lengths (points), else NULL           */
+    unsigned interval;            /* time-domain sampling interval, 0 = none            */
+    unsigned fd_interval;         /* DFT sampling interval, 0 = none                    */
+    unsigned start_ts, stop_ts;   /* active window (Processing::SetProcessStartStopTime)*/
+    unsigned capacity;            /* maximum number of samples to record                */
+} ems_probe_desc;
+
+int ems_add_probe(ems_engine* h, const ems_probe_desc* d, int* id);
+/* number of recorded samples; with ts/values non-NULL copies them out:
+ * ts[count], values[count*3] (slot 0 used by voltage/current probes) */
+int ems_probe_results(ems_engine* h, int id, unsigned* count, unsigned* ts, double* values);
+
+/* ---- frequency-domain field dump (E field, node interpolation) ----
+ * box start/stop mesh indices, nfreq complex<float> factors per sample are
+ * supplied up front: factors[nsamples][nfreq][2] for samples taken at
+ * ts = k*fd_interval (k = 0..nsamples-1). */
+int ems_add_fd_dump(ems_engine* h, int field, const unsigned start[3], const unsigned stop[3],
+                    unsigned fd_interval, unsigned nfreq, unsigned nsamples, const float* factors,
+                    const double* const edge_len[3], int* id);
+/* out: [nfreq][3][nx][ny][nz][2] float32 (re, im), dump-box N-I-J-K order */
+int ems_fd_dump_results(ems_engine* h, int id, float* out);
+
+/* eps0 * sum(V^2) + mu0 * sum(I^2) over [0,N-1)^3 */
+int ems_energy(ems_engine* h, double* out);
+
+/* device time of the last ems_iterate call in milliseconds (HIP events) and
+ * the average duration of the voltage / current update kernels it launched */
+int ems_last_timing(ems_engine* h, double* total_ms, double* volt_ms, double* curr_ms);
+
+/* record HIP events around every voltage / current update kernel (adds one
+ * host synchronisation per step; used for kernel roofline measurements) */
+int ems_set_profiling(ems_engine* h, int enable);
+
+void ems_destroy(ems_engine* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EMS_HIP_H */

@@ -1,0 +1,122 @@
+/*
+ * ems_operator.h — host-side EC-FDTD operator builder (C ABI).
+ *
+ * Builds, from a CSXCAD-free scene of axis-aligned boxes, exactly the arrays
+ * the reference Operator hands to its engines:
+ *   - the float32 update coefficients vv, vi, ii, iv in the reference's
+ *     N-I-J-K layout (component, x, y, z; z fastest)
+ *     [Operator::vv_ptr/vi_ptr/ii_ptr/iv_ptr, FDTD/operator.h:362-367]
+ *   - the timestep [Operator::CalcTimestep, FDTD/operator.cpp:1882-2030]
+ *   - the excitation signal and cell lists
+ *     [Excitation, FDTD/excitation.cpp:95-276;
+ *      Operator_Ext_Excitation::BuildExtension, operator_ext_excitation.cpp:105-297]
+ *   - UPML slabs and coefficients [Operator_Ext_UPML, operator_ext_upml.cpp:69-445]
+ *   - Mur faces and coefficients [Operator_Ext_Mur_ABC, operator_ext_mur_abc.cpp:80-186]
+ *   - lumped parallel-RC elements folded into the operator
+ *     [Operator::Calc_LumpedElements, FDTD/operator.cpp:1586-1763]
+ *
+ * Replaces Operator::SetGeometryCSX + Operator::CalcECOperator
+ * (FDTD/operator.cpp:822-837, 986-1097) for box-only scenes.
+ * All functions return 0 on success and a negative EMSOP_E_* code on error;
+ * emsop_last_error() describes the last failure of the calling thread.
+ */
+#ifndef EMS_OPERATOR_H
+#define EMS_OPERATOR_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EMSOP_OK 0
+#define EMSOP_E_ARG (-1)
+#define EMSOP_E_STATE (-2)
+#define EMSOP_E_NUMERIC (-3)
+#define EMSOP_E_NOMEM (-4)
+
+typedef struct emsop emsop;
+
+/* signal types, Excitation::ExciteTypes (FDTD/excitation.h:29) */
+#define EMSOP_SIG_GAUSS 0
+#define EMSOP_SIG_SINUS 1
+#define EMSOP_SIG_DIRAC 2
+#define EMSOP_SIG_STEP 3
+
+const char* emsop_last_error(void);
+
+int emsop_create(emsop** out);
+void emsop_destroy(emsop* h);
+
+/* mesh lines in drawing units and the drawing unit in meters
+ * (CSRectGrid lines + delta unit, Operator::SetupCSXGrid FDTD/operator.cpp:793-820) */
+int emsop_set_grid(emsop* h, const double* x, unsigned nx, const double* y, unsigned ny,
+                   const double* z, unsigned nz, double delta_unit);
+int emsop_set_background(emsop* h, double eps_r, double mue_r, double kappa, double sigma);
+/* material box: per-direction eps_r, mue_r, kappa (S/m), sigma (Ohm/m) */
+int emsop_add_material(emsop* h, const double eps[3], const double mue[3], const double kappa[3],
+                       const double sigma[3], int priority, const double start[3], const double stop[3]);
+int emsop_add_metal(emsop* h, int priority, const double start[3], const double stop[3]);
+/* lumped element; parallel=1 for the R||C form folded into the operator */
+int emsop_add_lumped(emsop* h, int dir, double R, double C, double L, int caps, int parallel,
+                     int priority, const double start[3], const double stop[3]);
+/* excitation box: type 0 soft E, 1 hard E, 2 soft H, 3 hard H; val = per-direction amplitude */
+int emsop_add_excitation(emsop* h, int type, const double val[3], double delay, int priority,
+                         const double start[3], const double stop[3]);
+/* bc: 0 PEC, 1 PMC, 2 Mur, 3 PML, -1 none (openEMS::m_BC_type) */
+int emsop_set_boundary(emsop* h, const int bc[6], const unsigned pml_size[6], const double mur_vph[6]);
+int emsop_set_signal(emsop* h, int type, double f0, double fc);
+/* method: 1 or 3 (Rennings); forced_dt <= 0 means automatic; factor in (0,1] */
+int emsop_set_timestep(emsop* h, int method, double forced_dt, double factor);
+int emsop_set_cell_constant_material(emsop* h, int enable);
+/* threads used by the build (0 = all hardware threads) */
+int emsop_set_threads(emsop* h, int nthreads);
+
+/* run the operator build for a simulation of nrts timesteps */
+int emsop_build(emsop* h, unsigned nrts);
+
+/* ---- results (valid after emsop_build) ---- */
+int emsop_num_lines(const emsop* h, unsigned out[3]);
+double emsop_timestep(const emsop* h);
+unsigned emsop_nyquist(const emsop* h);
+double emsop_signal_period(const emsop* h);
+unsigned emsop_signal_length(const emsop* h);
+/* period in timesteps as Engine_Ext_Excitation computes it, 0 when aperiodic */
+int emsop_signal_period_steps(const emsop* h);
+const float* emsop_signal_volt(const emsop* h);
+const float* emsop_signal_curr(const emsop* h);
+/* which: 0 vv, 1 vi, 2 ii, 3 iv; [3][Nx][Ny][Nz] float32 */
+const float* emsop_coeff(const emsop* h, int which);
+/* which: 0 C, 1 G, 2 L, 3 R equivalent-circuit arrays (float32, N-I-J-K) */
+const float* emsop_ec(const emsop* h, int which);
+/* effective material of edge (ny, pos): out = eps*EPS0, kappa, mue*MUE0, sigma
+ * (Operator::Calc_EffMatPos, FDTD/operator.cpp:1566-1584) */
+int emsop_effmat_at(const emsop* h, int ny, const unsigned pos[3], double out[4]);
+
+unsigned emsop_exc_count(const emsop* h, int curr);
+/* idx: [3][K] uint32, dir: [K] uint16, amp [K] float32, delay [K] uint32 */
+int emsop_exc_get(const emsop* h, int curr, unsigned* idx, unsigned short* dir, float* amp, unsigned* delay);
+
+unsigned emsop_upml_count(const emsop* h);
+/* arrays: vv, vvfo, vvfn, ii, iifo, iifn, each [3][num0][num1][num2] */
+int emsop_upml_get(const emsop* h, unsigned k, unsigned start[3], unsigned num[3], const float* arrays[6]);
+
+unsigned emsop_mur_count(const emsop* h);
+/* coefficients [N[nyP]][N[nyPP]] for the two tangential components */
+int emsop_mur_get(const emsop* h, unsigned k, int* ny, unsigned* line, unsigned* line_shift,
+                  unsigned* start_ts, const float** coeff_nyP, const float** coeff_nyPP);
+
+/* mesh queries used by probe setup */
+double emsop_disc_line(const emsop* h, int n, unsigned pos, int dual);
+double emsop_edge_length(const emsop* h, int n, unsigned pos, int dual);
+/* SnapToMesh; returns 1 if inside on all axes */
+int emsop_snap_to_mesh(const emsop* h, const double coord[3], int dual, unsigned out[3], int inside[3]);
+/* SnapBox2Mesh (method 0): returns dimension or -2 outside */
+int emsop_snap_box(const emsop* h, const double start[3], const double stop[3], int dual,
+                   unsigned ustart[3], unsigned ustop[3], int start_in[3], int stop_in[3]);
+
+/* release the EC and effective-material arrays once extensions are built */
+void emsop_release_build_arrays(emsop* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EMS_OPERATOR_H */

@@ -1,0 +1,1413 @@
+// MI355X (gfx950) EC-FDTD engine: device state, update kernels, extension
+// kernels, probes and the C ABI declared in include/ems_hip.h.
+//
+// Device layout: every field / coefficient array is structure-of-arrays,
+// one float32 plane-major block per component, x fastest:
+//     a[c * cs + x + Nx * (y + Ny * z)]
+// with cs (component stride) rounded up to 64 elements.
+//
+// Arithmetic follows the reference element update exactly
+//     v *= vv;  v += vi * (((a - b) - c) + d)
+// (FDTD/engine.cpp:133-168), with separate rounding of every product and sum
+// (no contraction: -ffp-contract=off and the pragma below), so results are
+// bit-identical to the reference's basic engine on the same operator.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "ems_hip.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return fail(EMS_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr unsigned kBlock = 256;      // threads per block of the stencil kernels (4 wave64)
+constexpr unsigned kXCD = 8;          // accelerator complex dies on MI355X
+constexpr unsigned kMaxSlabs = 6;     // UPML slabs (Operator_Ext_UPML::Create_UPML creates at most 6)
+
+struct GridDesc {
+    unsigned Nx, Ny, Nz;
+    unsigned plane;     // Nx*Ny
+    unsigned long long cs;  // component stride
+    unsigned zdepth;    // z-sweep depth per block
+    unsigned nzc;       // number of z chunks
+    unsigned qtiles;    // ceil(plane / kBlock)
+};
+
+struct SlabDev {
+    int x0, x1, y0, y1, z0, z1;  // inclusive ranges
+    unsigned nx, ny;
+    unsigned long long cs;       // component stride inside the slab
+    float* c0;                   // vv / ii        [3][cs]
+    float* cfo;                  // vvfo / iifo
+    float* cfn;                  // vvfn / iifn
+    float* flux;                 // volt_flux / curr_flux
+};
+
+struct PmlArgs {
+    int n;
+    SlabDev s[kMaxSlabs];
+};
+
+// Workgroups are dispatched round-robin over the XCDs by linear block id;
+// remap so that each XCD owns one contiguous range of tiles (neighbouring
+// rows and z-chunks share that XCD's L2).
+__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb)
+{
+    const unsigned per = nb / kXCD, rem = nb % kXCD;
+    const unsigned xcd = b % kXCD, k = b / kXCD;
+    return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
+}
+
+// ---------------------------------------------------------------------------
+// voltage half step (Engine::UpdateVoltages, FDTD/engine.cpp:110-168) with the
+// UPML pre/post hooks of Engine_Ext_UPML (engine_ext_upml.cpp:52-124) fused in:
+//   pre : t = vv_u*V - vvfo*F;  V <- F
+//   main: V <- V*vv + vi*curl    (computed on the flux)
+//   post: F <- V;  V <- t + vvfn*F
+// The lower neighbour at index 0 is the cell itself (shift = bool(pos)).
+// ---------------------------------------------------------------------------
+template <bool PML>
+__global__ __launch_bounds__(kBlock) void k_volt(const GridDesc g, float* __restrict__ V, const float* __restrict__ I,
+                                                 const float* __restrict__ vv, const float* __restrict__ vi,
+                                                 const PmlArgs pml)
+{
+    const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
+    const unsigned qt = t % g.qtiles, zc = t / g.qtiles;
+    const unsigned q = qt * kBlock + threadIdx.x;
+    if (q >= g.plane) return;
+    const unsigned y = q / g.Nx, x = q - y * g.Nx;
+    const unsigned z0 = zc * g.zdepth;
+    const unsigned z1 = min(z0 + g.zdepth, g.Nz);
+    const unsigned long long cs = g.cs;
+    const unsigned xm = x ? 1u : 0u, ym = y ? g.Nx : 0u;
+    unsigned long long p = (unsigned long long)q + (unsigned long long)g.plane * z0;
+
+    const float* __restrict__ Ix = I;
+    const float* __restrict__ Iy = I + cs;
+    const float* __restrict__ Iz = I + 2 * cs;
+    // z-1 values carried in registers along the sweep (self at z = 0)
+    const unsigned long long pz = z0 ? p - g.plane : p;
+    float Ix_zm = Ix[pz], Iy_zm = Iy[pz];
+
+    // candidate UPML slabs of this (x, y) column
+    int cand0 = -1, cand1 = -1;
+    if (PML) {
+        for (int s = 0; s < pml.n; ++s) {
+            const SlabDev& S = pml.s[s];
+            if ((int)x >= S.x0 && (int)x <= S.x1 && (int)y >= S.y0 && (int)y <= S.y1) {
+                if (cand0 < 0)
+                    cand0 = s;
+                else if (cand1 < 0)
+                    cand1 = s;
+            }
+        }
+    }
+
+    for (unsigned z = z0; z < z1; ++z, p += g.plane) {
+        const float ix = Ix[p], iy = Iy[p], iz = Iz[p];
+        const float iz_ym = Iz[p - ym], ix_ym = Ix[p - ym];
+        const float iz_xm = Iz[p - xm], iy_xm = Iy[p - xm];
+        float curl[3];
+        curl[0] = iz - iz_ym - iy + Iy_zm;
+        curl[1] = ix - Ix_zm - iz + iz_xm;
+        curl[2] = iy - iy_xm - ix + ix_ym;
+        Ix_zm = ix;
+        Iy_zm = iy;
+
+        int slab = -1;
+        if (PML) {
+            if (cand0 >= 0 && (int)z >= pml.s[cand0].z0 && (int)z <= pml.s[cand0].z1)
+                slab = cand0;
+            else if (cand1 >= 0 && (int)z >= pml.s[cand1].z0 && (int)z <= pml.s[cand1].z1)
+                slab = cand1;
+        }
+        if (!PML || slab < 0) {
+#pragma unroll
+            for (int n = 0; n < 3; ++n) {
+                const unsigned long long i = n * cs + p;
+                float v = V[i];
+                v = v * vv[i];
+                v = v + vi[i] * curl[n];
+                V[i] = v;
+            }
+        } else {
+            const SlabDev& S = pml.s[slab];
+            const unsigned long long l = (unsigned long long)(x - S.x0) +
+                                         (unsigned long long)S.nx * ((y - S.y0) + (unsigned long long)S.ny * (z - S.z0));
+#pragma unroll
+            for (int n = 0; n < 3; ++n) {
+                const unsigned long long i = n * cs + p;
+                const unsigned long long li = n * S.cs + l;
+                const float v0 = V[i];
+                const float f0 = S.flux[li];
+                const float tpre = S.c0[li] * v0 - S.cfo[li] * f0;
+                float f = f0 * vv[i];
+                f = f + vi[i] * curl[n];
+                V[i] = tpre + S.cfn[li] * f;
+                S.flux[li] = f;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// current half step (Engine::UpdateCurrents, FDTD/engine.cpp:170-222), same
+// UPML fusion with the current flux (engine_ext_upml.cpp:135-234).  Updates
+// x < Nx-1, y < Ny-1, z < Nz-1.  Thread (0,0) of tile 0 advances the device
+// timestep counter (++numTS of Engine::IterateTS): no kernel of this launch
+// reads it, and every later kernel sees the new value.
+// ---------------------------------------------------------------------------
+template <bool PML>
+__global__ __launch_bounds__(kBlock) void k_curr(const GridDesc g, float* __restrict__ I, const float* __restrict__ V,
+                                                 const float* __restrict__ ii, const float* __restrict__ iv,
+                                                 const PmlArgs pml, unsigned* __restrict__ numts)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(numts, 1u);
+    const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
+    const unsigned qt = t % g.qtiles, zc = t / g.qtiles;
+    const unsigned q = qt * kBlock + threadIdx.x;
+    if (q >= g.plane) return;
+    const unsigned y = q / g.Nx, x = q - y * g.Nx;
+    if (x + 1 >= g.Nx || y + 1 >= g.Ny) return;
+    const unsigned z0 = zc * g.zdepth;
+    const unsigned z1 = min(z0 + g.zdepth, g.Nz - 1);
+    if (z0 >= z1) return;
+    const unsigned long long cs = g.cs;
+    const unsigned Nx = g.Nx;
+    unsigned long long p = (unsigned long long)q + (unsigned long long)g.plane * z0;
+
+    const float* __restrict__ Vx = V;
+    const float* __restrict__ Vy = V + cs;
+    const float* __restrict__ Vz = V + 2 * cs;
+    // values at z carried from the previous iteration's z+1 loads
+    float vx = Vx[p], vy = Vy[p];
+
+    int cand0 = -1, cand1 = -1;
+    if (PML) {
+        for (int s = 0; s < pml.n; ++s) {
+            const SlabDev& S = pml.s[s];
+            if ((int)x >= S.x0 && (int)x <= S.x1 && (int)y >= S.y0 && (int)y <= S.y1) {
+                if (cand0 < 0)
+                    cand0 = s;
+                else if (cand1 < 0)
+                    cand1 = s;
+            }
+        }
+    }
+
+    for (unsigned z = z0; z < z1; ++z, p += g.plane) {
+        const unsigned long long pzp = p + g.plane;
+        const float vz = Vz[p];
+        const float vx_zp = Vx[pzp], vy_zp = Vy[pzp];
+        const float vz_yp = Vz[p + Nx], vx_yp = Vx[p + Nx];
+        const float vz_xp = Vz[p + 1], vy_xp = Vy[p + 1];
+        float curl[3];
+        curl[0] = vz - vz_yp - vy + vy_zp;
+        curl[1] = vx - vx_zp - vz + vz_xp;
+        curl[2] = vy - vy_xp - vx + vx_yp;
+        vx = vx_zp;
+        vy = vy_zp;
+
+        int slab = -1;
+        if (PML) {
+            if (cand0 >= 0 && (int)z >= pml.s[cand0].z0 && (int)z <= pml.s[cand0].z1)
+                slab = cand0;
+            else if (cand1 >= 0 && (int)z >= pml.s[cand1].z0 && (int)z <= pml.s[cand1].z1)
+                slab = cand1;
+        }
+        if (!PML || slab < 0) {
+#pragma unroll
+            for (int n = 0; n < 3; ++n) {
+                const unsigned long long i = n * cs + p;
+                float c = I[i];
+                c = c * ii[i];
+                c = c + iv[i] * curl[n];
+                I[i] = c;
+            }
+        } else {
+            const SlabDev& S = pml.s[slab];
+            const unsigned long long l = (unsigned long long)(x - S.x0) +
+                                         (unsigned long long)S.nx * ((y - S.y0) + (unsigned long long)S.ny * (z - S.z0));
+#pragma unroll
+            for (int n = 0; n < 3; ++n) {
+                const unsigned long long i = n * cs + p;
+                const unsigned long long li = n * S.cs + l;
+                const float c0 = I[i];
+                const float f0 = S.flux[li];
+                const float tpre = S.c0[li] * c0 - S.cfo[li] * f0;
+                float f = f0 * ii[i];
+                f = f + iv[i] * curl[n];
+                I[i] = tpre + S.cfn[li] * f;
+                S.flux[li] = f;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// soft / hard source application (Engine_Ext_Excitation::Apply2Voltages /
+// Apply2Current, engine_ext_excitation.cpp:33-99).  numTS is read from the
+// device counter; the current-side source runs after the counter advanced,
+// hence ts_bias = -1 there.
+// ---------------------------------------------------------------------------
+__global__ void k_excite(float* __restrict__ F, unsigned long long cs, unsigned K, const unsigned* __restrict__ lin,
+                         const unsigned short* __restrict__ dir, const float* __restrict__ amp,
+                         const unsigned* __restrict__ delay, const float* __restrict__ sig, unsigned length,
+                         int period, const unsigned* __restrict__ numts, int ts_bias)
+{
+    const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const int ts = (int)*numts + ts_bias;
+    const int p = period > 0 ? period : ts + 1;
+    int e = ts - (int)delay[k];
+    e *= (e > 0);
+    e %= p;
+    e *= (e < (int)length);
+    const unsigned long long i = (unsigned long long)dir[k] * cs + lin[k];
+    F[i] = F[i] + amp[k] * sig[e];
+}
+
+// ---------------------------------------------------------------------------
+// Mur ABC (Engine_Ext_Mur_ABC, engine_ext_mur_abc.cpp:68-178)
+// ---------------------------------------------------------------------------
+struct MurDev {
+    unsigned na, nb;                      // points along nyP, nyPP
+    unsigned long long offP, offPP;       // component offsets nyP*cs, nyPP*cs
+    const unsigned* face;                 // [na*nb] linear index of the face point
+    const unsigned* shift;                // [na*nb] linear index of the neighbour point
+    const float* cP;
+    const float* cPP;
+    float* mP;
+    float* mPP;
+    unsigned start_ts;
+};
+
+__global__ void k_mur_pre(float* __restrict__ V, const MurDev m, const unsigned* __restrict__ numts)
+{
+    const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m.na * m.nb || *numts < m.start_ts) return;
+    const unsigned f = m.face[k], s = m.shift[k];
+    m.mP[k] = V[m.offP + s] - m.cP[k] * V[m.offP + f];
+    m.mPP[k] = V[m.offPP + s] - m.cPP[k] * V[m.offPP + f];
+}
+
+__global__ void k_mur_post(const float* __restrict__ V, const MurDev m, const unsigned* __restrict__ numts)
+{
+    const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m.na * m.nb || *numts < m.start_ts) return;
+    const unsigned s = m.shift[k];
+    m.mP[k] = m.mP[k] + m.cP[k] * V[m.offP + s];
+    m.mPP[k] = m.mPP[k] + m.cPP[k] * V[m.offPP + s];
+}
+
+__global__ void k_mur_apply(float* __restrict__ V, const MurDev m, const unsigned* __restrict__ numts)
+{
+    const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m.na * m.nb || *numts < m.start_ts) return;
+    const unsigned f = m.face[k];
+    V[m.offP + f] = m.mP[k];
+    V[m.offPP + f] = m.mPP[k];
+}
+
+// ---------------------------------------------------------------------------
+// lumped RLC (Engine_Ext_LumpedRLC, engine_ext_lumpedRLC.cpp:83-147).  The
+// reference rotates its three history buffers once per step (Vd in the pre
+// hook, J in the apply hook); after k+1 rotations logical slot i lives in
+// buffer (2(k+1) + i) mod 3, so the rotation is derived from numTS.
+// ---------------------------------------------------------------------------
+struct RlcDev {
+    unsigned count;
+    const unsigned long long* lin;  // component offset + linear index
+    const float* c[9];              // i2v, ilv, vvd, vv2, vj1, vj2, ib0, b1, b2
+    float* vd;                      // [3][count]
+    float* jn;                      // [3][count]
+    float* il;                      // [count]
+};
+
+__device__ __forceinline__ unsigned rlc_slot(unsigned ts, unsigned i) { return (2u * ((ts + 1u) % 3u) + i) % 3u; }
+
+__global__ void k_rlc_pre(const RlcDev r, const unsigned* __restrict__ numts)
+{
+    const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= r.count) return;
+    const unsigned ts = *numts;
+    const float* Vd1 = r.vd + rlc_slot(ts, 1) * r.count;
+    r.il[k] = r.il[k] + (r.c[0][k] * r.c[1][k]) * Vd1[k];
+}
+
+__global__ void k_rlc_apply(float* __restrict__ V, const RlcDev r, const unsigned* __restrict__ numts)
+{
+    const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= r.count) return;
+    const unsigned ts = *numts;
+    float* V0 = r.vd + rlc_slot(ts, 0) * r.count;
+    const float* V2 = r.vd + rlc_slot(ts, 2) * r.count;
+    float* J0 = r.jn + rlc_slot(ts, 0) * r.count;
+    const float* J1 = r.jn + rlc_slot(ts, 1) * r.count;
+    const float* J2 = r.jn + rlc_slot(ts, 2) * r.count;
+    float v = V[r.lin[k]];
+    v = r.c[2][k] * (v - r.il[k] + r.c[3][k] * V2[k] + r.c[4][k] * J1[k] + r.c[5][k] * J2[k]);
+    V0[k] = v;
+    J0[k] = r.c[6][k] * (v - V2[k]) - (r.c[7][k] * r.c[6][k]) * J1[k] - (r.c[8][k] * r.c[6][k]) * J2[k];
+    V[r.lin[k]] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Lorentz / Drude ADE (engine_ext_lorentzmaterial.cpp:79-173,
+// engine_ext_dispersive.cpp:76-132)
+// ---------------------------------------------------------------------------
+struct LorDev {
+    unsigned count;
+    const unsigned* lin;   // [count]
+    const float* cint;     // [3][count]
+    const float* cext;
+    const float* clor;     // nullptr: Drude/Debye form
+    float* ade;            // [3][count]
+    float* lor;            // [3][count]
+};
+
+__global__ void k_lor_pre(const float* __restrict__ F, unsigned long long cs, const LorDev d)
+{
+    const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.count) return;
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+        const unsigned j = n * d.count + k;
+        const float f = F[n * cs + d.lin[k]];
+        if (d.clor) {
+            d.lor[j] = d.lor[j] + d.clor[j] * d.ade[j];
+            float a = d.ade[j] * d.cint[j];
+            d.ade[j] = a + d.cext[j] * (f - d.lor[j]);
+        } else {
+            float a = d.ade[j] * d.cint[j];
+            d.ade[j] = a + d.cext[j] * f;
+        }
+    }
+}
+
+__global__ void k_lor_apply(float* __restrict__ F, unsigned long long cs, const LorDev d)
+{
+    const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.count) return;
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+        const unsigned long long i = n * cs + d.lin[k];
+        F[i] = F[i] - d.ade[n * d.count + k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// probes: one block per probe; the terms are gathered in parallel and summed
+// by one lane in the reference's order so the result is bit-identical
+// (double sum for voltages, float sum for currents).
+// ---------------------------------------------------------------------------
+struct ProbeDev {
+    int type;
+    unsigned nelem;
+    unsigned off;          // offset into the element tables
+    unsigned interval, fd_interval, start_ts, stop_ts, capacity;
+    unsigned* count;       // [1]
+    unsigned* ts_out;      // [capacity]
+    double* val_out;       // [capacity*3]
+};
+
+constexpr unsigned kProbeChunk = 2048;
+
+__global__ __launch_bounds__(kBlock) void k_probe(const float* __restrict__ V, const float* __restrict__ I,
+                                                  const ProbeDev* __restrict__ probes,
+                                                  const unsigned long long* __restrict__ elin,
+                                                  const signed char* __restrict__ esign,
+                                                  const double* __restrict__ ediv,
+                                                  const unsigned* __restrict__ numts)
+{
+    __shared__ float vals[kProbeChunk];
+    const ProbeDev P = probes[blockIdx.x];
+    const unsigned ts = *numts;
+    if (ts < P.start_ts || ts > P.stop_ts) return;
+    const bool due = (P.interval && ts % P.interval == 0) || (P.fd_interval && ts % P.fd_interval == 0);
+    if (!due) return;
+    const float* F = (P.type == EMS_PROBE_CURRENT || P.type == EMS_PROBE_HFIELD) ? I : V;
+    double acc_d = 0.0;
+    float acc_f = 0.0f;
+    double out[3] = {0.0, 0.0, 0.0};
+    for (unsigned base = 0; base < P.nelem; base += kProbeChunk) {
+        const unsigned m = min(kProbeChunk, P.nelem - base);
+        for (unsigned e = threadIdx.x; e < m; e += blockDim.x) vals[e] = F[elin[P.off + base + e]];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (unsigned e = 0; e < m; ++e) {
+                const unsigned j = P.off + base + e;
+                if (P.type == EMS_PROBE_VOLTAGE) {
+                    if (esign[j] > 0)
+                        acc_d = acc_d + (double)vals[e];
+                    else
+                        acc_d = acc_d - (double)vals[e];
+                } else if (P.type == EMS_PROBE_CURRENT) {
+                    if (esign[j] > 0)
+                        acc_f = acc_f + vals[e];
+                    else
+                        acc_f = acc_f - vals[e];
+                } else if (base + e < 3) {
+                    const double d = ediv[j];
+                    out[base + e] = d != 0.0 ? (double)vals[e] / d : 0.0;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const unsigned slot = *P.count;
+        if (slot < P.capacity) {
+            P.ts_out[slot] = ts;
+            if (P.type == EMS_PROBE_VOLTAGE) {
+                P.val_out[3 * slot] = acc_d;
+                P.val_out[3 * slot + 1] = 0.0;
+                P.val_out[3 * slot + 2] = 0.0;
+            } else if (P.type == EMS_PROBE_CURRENT) {
+                P.val_out[3 * slot] = (double)acc_f;
+                P.val_out[3 * slot + 1] = 0.0;
+                P.val_out[3 * slot + 2] = 0.0;
+            } else {
+                P.val_out[3 * slot] = out[0];
+                P.val_out[3 * slot + 1] = out[1];
+                P.val_out[3 * slot + 2] = out[2];
+            }
+            *P.count = slot + 1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// frequency-domain E dump with node interpolation (ProcessFieldsFD::Process,
+// processfields_fd.cpp:72-107; Engine_Interface_FDTD::GetRawInterpolatedField
+// NODE_INTERPOLATE, engine_interface_fdtd.cpp:63-124), complex<float> sums.
+// ---------------------------------------------------------------------------
+struct DumpDev {
+    unsigned s[3], n[3];          // box start and extent (mesh indices)
+    unsigned N[3];
+    unsigned fd_interval, nfreq, nsamples;
+    const float* factors;         // [nsamples][nfreq][2]
+    const double* el[3];          // primary edge lengths per axis
+    float* acc;                   // [nfreq][3][npts][2]
+};
+
+__global__ void k_fd_dump(const float* __restrict__ V, unsigned long long cs, unsigned Nx, unsigned plane, const DumpDev d,
+                          const unsigned* __restrict__ numts)
+{
+    const unsigned ts = *numts;
+    if (ts % d.fd_interval) return;
+    const unsigned ks = ts / d.fd_interval;
+    if (ks >= d.nsamples) return;
+    const unsigned npts = d.n[0] * d.n[1] * d.n[2];
+    const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npts) return;
+    // dump-box N-I-J-K order: z fastest
+    const unsigned lz = k % d.n[2];
+    const unsigned ly = (k / d.n[2]) % d.n[1];
+    const unsigned lx = k / (d.n[2] * d.n[1]);
+    const unsigned pos[3] = {d.s[0] + lx, d.s[1] + ly, d.s[2] + lz};
+    const unsigned long long base = pos[0] + (unsigned long long)Nx * pos[1] + (unsigned long long)plane * pos[2];
+    const unsigned long long stride[3] = {1ull, (unsigned long long)Nx, (unsigned long long)plane};
+    float E[3];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+        double out;
+        const unsigned pn = pos[n];
+        if (pn == d.N[n] - 1) {
+            const double delta = d.el[n][pn - 1];
+            const double raw = delta ? (double)V[n * cs + base - stride[n]] / delta : 0.0;
+            out = raw;
+        } else {
+            const double delta = d.el[n][pn];
+            out = delta ? (double)V[n * cs + base] / delta : 0.0;
+            if (delta == 0.0)
+                out = 0.0;
+            else if (pn != 0) {
+                const double deltaDown = d.el[n][pn - 1];
+                const double deltaRel = delta / (delta + deltaDown);
+                const double lo = deltaDown ? (double)V[n * cs + base - stride[n]] / deltaDown : 0.0;
+                out = out * (1.0 - deltaRel) + lo * deltaRel;
+            }
+        }
+        E[n] = (float)out;
+    }
+    for (unsigned f = 0; f < d.nfreq; ++f) {
+        const float er = d.factors[(ks * d.nfreq + f) * 2];
+        const float ei = d.factors[(ks * d.nfreq + f) * 2 + 1];
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            float* a = d.acc + (((unsigned long long)f * 3 + n) * npts + k) * 2;
+            a[0] = a[0] + E[n] * er;
+            a[1] = a[1] + E[n] * ei;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// energy (Engine_Interface_FDTD::CalcFastEnergy): per-block double partials
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_energy(const float* __restrict__ V, const float* __restrict__ I,
+                                                   unsigned long long cs, unsigned Nx, unsigned Ny, unsigned Nz,
+                                                   double* __restrict__ part)
+{
+    __shared__ double se[kBlock], sh[kBlock];
+    const unsigned long long plane = (unsigned long long)Nx * Ny;
+    const unsigned long long tot = plane * Nz;
+    double e = 0.0, h = 0.0;
+    for (unsigned long long p = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; p < tot;
+         p += (unsigned long long)gridDim.x * blockDim.x) {
+        const unsigned long long z = p / plane;
+        const unsigned long long r = p - z * plane;
+        const unsigned long long y = r / Nx, x = r - y * Nx;
+        if (x + 1 >= Nx || y + 1 >= Ny || z + 1 >= Nz) continue;
+        for (int n = 0; n < 3; ++n) {
+            const double v = V[n * cs + p], c = I[n * cs + p];
+            e += v * v;
+            h += c * c;
+        }
+    }
+    se[threadIdx.x] = e;
+    sh[threadIdx.x] = h;
+    __syncthreads();
+    for (unsigned s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            se[threadIdx.x] += se[threadIdx.x + s];
+            sh[threadIdx.x] += sh[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = se[0];
+        part[2 * blockIdx.x + 1] = sh[0];
+    }
+}
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+};
+
+void parallel_rows(size_t n, const std::function<void(size_t, size_t)>& fn);
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// engine state
+// ---------------------------------------------------------------------------
+struct ems_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    GridDesc g{};
+    unsigned N[3] = {0, 0, 0};
+    unsigned long long cs = 0;
+    float *V = nullptr, *I = nullptr;
+    float *vv = nullptr, *vi = nullptr, *ii = nullptr, *iv = nullptr;
+    unsigned* numts_dev = nullptr;
+    unsigned numTS = 0;
+    bool inited = false;
+    bool async = false;
+    std::vector<void*> allocs;
+
+    // excitation
+    struct Exc {
+        unsigned K = 0;
+        unsigned* lin = nullptr;
+        unsigned short* dir = nullptr;
+        float* amp = nullptr;
+        unsigned* delay = nullptr;
+    } exc[2];
+    float* sig[2] = {nullptr, nullptr};
+    unsigned sigLen = 0;
+    int period = 0;
+    bool has_exc = false;
+
+    // UPML
+    PmlArgs pmlV{}, pmlI{};
+    // Mur faces in creation order
+    std::vector<MurDev> mur;
+    // lumped RLC and dispersive lists
+    std::vector<RlcDev> rlc;
+    std::vector<LorDev> lor;
+    std::vector<int> lor_v, lor_i;
+
+    // probes
+    std::vector<ProbeDev> probes;
+    std::vector<unsigned long long> h_elin;
+    std::vector<signed char> h_esign;
+    std::vector<double> h_ediv;
+    ProbeDev* d_probes = nullptr;
+    unsigned long long* d_elin = nullptr;
+    signed char* d_esign = nullptr;
+    double* d_ediv = nullptr;
+    bool probes_dirty = false;
+    unsigned min_interval = 0;
+
+    // field dumps
+    std::vector<DumpDev> dumps;
+    std::vector<unsigned> dump_npts;
+
+    double* energy_part = nullptr;
+
+    // timing
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0, last_volt_ms = 0, last_curr_ms = 0;
+    bool profile = false;
+    std::vector<hipEvent_t> kev;
+
+    template <typename T>
+    int alloc(T** p, size_t n)
+    {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+        if (e != hipSuccess) return fail(EMS_E_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        allocs.push_back(q);
+        *p = (T*)q;
+        return 0;
+    }
+    unsigned long long lin(unsigned x, unsigned y, unsigned z) const
+    {
+        return x + (unsigned long long)N[0] * (y + (unsigned long long)N[1] * z);
+    }
+};
+
+namespace {
+
+void parallel_rows(size_t n, const std::function<void(size_t, size_t)>& fn)
+{
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n < 64 || nt == 1) {
+        fn(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    size_t per = (n + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+        size_t a = t * per, b = std::min(n, a + per);
+        if (a >= b) break;
+        th.emplace_back(fn, a, b);
+    }
+    for (auto& t : th) t.join();
+}
+
+// N-I-J-K host array -> device SoA (x fastest) staging buffer
+void nijk_to_soa(const float* src, float* dst, const unsigned* N, unsigned long long cs)
+{
+    const size_t rows = (size_t)3 * N[1] * N[2];
+    parallel_rows(rows, [&](size_t a, size_t b) {
+        for (size_t r = a; r < b; ++r) {
+            const unsigned n = (unsigned)(r / ((size_t)N[1] * N[2]));
+            const size_t yz = r % ((size_t)N[1] * N[2]);
+            const unsigned z = (unsigned)(yz / N[1]), y = (unsigned)(yz % N[1]);
+            float* d = dst + n * cs + (size_t)N[0] * (y + (size_t)N[1] * z);
+            for (unsigned x = 0; x < N[0]; ++x) d[x] = src[(((size_t)n * N[0] + x) * N[1] + y) * N[2] + z];
+        }
+    });
+}
+
+void soa_to_nijk(const float* src, float* dst, const unsigned* N, unsigned long long cs)
+{
+    const size_t rows = (size_t)3 * N[1] * N[2];
+    parallel_rows(rows, [&](size_t a, size_t b) {
+        for (size_t r = a; r < b; ++r) {
+            const unsigned n = (unsigned)(r / ((size_t)N[1] * N[2]));
+            const size_t yz = r % ((size_t)N[1] * N[2]);
+            const unsigned z = (unsigned)(yz / N[1]), y = (unsigned)(yz % N[1]);
+            const float* s = src + n * cs + (size_t)N[0] * (y + (size_t)N[1] * z);
+            for (unsigned x = 0; x < N[0]; ++x) dst[(((size_t)n * N[0] + x) * N[1] + y) * N[2] + z] = s[x];
+        }
+    });
+}
+
+int upload_nijk(ems_engine* h, float* dptr, const float* host)
+{
+    std::vector<float> stage(3 * h->cs, 0.0f);
+    nijk_to_soa(host, stage.data(), h->N, h->cs);
+    HIPCHK(hipMemcpy(dptr, stage.data(), stage.size() * sizeof(float), hipMemcpyHostToDevice));
+    return 0;
+}
+
+template <typename T>
+int upload(ems_engine* h, T** d, const T* src, size_t n)
+{
+    int rc = h->alloc(d, n);
+    if (rc) return rc;
+    if (n) HIPCHK(hipMemcpy(*d, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+unsigned choose_zdepth(const unsigned* N)
+{
+    // enough blocks to fill 256 CUs several times over; at least 4 planes per sweep
+    const unsigned plane = N[0] * N[1];
+    const unsigned qt = (plane + kBlock - 1) / kBlock;
+    unsigned d = 32;
+    while (d > 4 && (unsigned long long)qt * ((N[2] + d - 1) / d) < 256ull * 16) d /= 2;
+    return d;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* ems_last_error(void) { return g_err.c_str(); }
+
+int ems_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int ems_create(ems_engine** out, const unsigned numLines[3], const float* vv, const float* vi, const float* ii,
+               const float* iv, int device)
+{
+    if (!out || !numLines || !vv || !vi || !ii || !iv) return fail(EMS_E_ARG, "null argument");
+    *out = nullptr;
+    for (int n = 0; n < 3; ++n)
+        if (numLines[n] < 3) return fail(EMS_E_ARG, "at least 3 mesh lines per direction are required");
+    const unsigned long long plane = (unsigned long long)numLines[0] * numLines[1];
+    if (plane > 0xFFFFFFFFull || plane * numLines[2] > 0xFFFFFFFFull)
+        return fail(EMS_E_ARG, "mesh too large for 32-bit in-plane indexing");
+    int ndev = ems_device_count();
+    if (ndev <= 0) return fail(EMS_E_NODEV, "no HIP device available");
+    if (device < 0 || device >= ndev) return fail(EMS_E_ARG, "invalid device ordinal");
+    HIPCHK(hipSetDevice(device));
+    ems_engine* h = new ems_engine();
+    h->device = device;
+    for (int n = 0; n < 3; ++n) h->N[n] = numLines[n];
+    const unsigned long long tot = plane * numLines[2];
+    h->cs = (tot + 63) / 64 * 64;
+    GridDesc& g = h->g;
+    g.Nx = numLines[0];
+    g.Ny = numLines[1];
+    g.Nz = numLines[2];
+    g.plane = (unsigned)plane;
+    g.cs = h->cs;
+    g.zdepth = choose_zdepth(h->N);
+    g.nzc = (g.Nz + g.zdepth - 1) / g.zdepth;
+    g.qtiles = (g.plane + kBlock - 1) / kBlock;
+    int rc = 0;
+    auto bail = [&](int r) {
+        ems_destroy(h);
+        return r;
+    };
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(EMS_E_HIP, "stream creation failed"));
+    if ((rc = h->alloc(&h->V, 3 * h->cs)) || (rc = h->alloc(&h->I, 3 * h->cs)) || (rc = h->alloc(&h->vv, 3 * h->cs)) ||
+        (rc = h->alloc(&h->vi, 3 * h->cs)) || (rc = h->alloc(&h->ii, 3 * h->cs)) || (rc = h->alloc(&h->iv, 3 * h->cs)) ||
+        (rc = h->alloc(&h->numts_dev, 4)))
+        return bail(rc);
+    if (hipMemset(h->V, 0, 3 * h->cs * sizeof(float)) != hipSuccess ||
+        hipMemset(h->I, 0, 3 * h->cs * sizeof(float)) != hipSuccess ||
+        hipMemset(h->numts_dev, 0, 4 * sizeof(unsigned)) != hipSuccess)
+        return bail(fail(EMS_E_HIP, "hipMemset failed"));
+    if ((rc = upload_nijk(h, h->vv, vv)) || (rc = upload_nijk(h, h->vi, vi)) || (rc = upload_nijk(h, h->ii, ii)) ||
+        (rc = upload_nijk(h, h->iv, iv)))
+        return bail(rc);
+    hipEventCreate(&h->ev0);
+    hipEventCreate(&h->ev1);
+    *out = h;
+    return 0;
+}
+
+int ems_set_excitation(ems_engine* h, const float* sig_volt, const float* sig_curr, unsigned length, int period_steps,
+                       unsigned nvolt, const unsigned* vidx, const unsigned short* vdir, const float* vamp,
+                       const unsigned* vdelay, unsigned ncurr, const unsigned* cidx, const unsigned short* cdir,
+                       const float* camp, const unsigned* cdelay)
+{
+    if (!h) return fail(EMS_E_ARG, "null handle");
+    if (h->inited) return fail(EMS_E_STATE, "engine already initialised");
+    if (h->has_exc) return fail(EMS_E_STATE, "excitation already set");
+    if (length == 0 || !sig_volt || !sig_curr) return fail(EMS_E_ARG, "empty excitation signal");
+    HIPCHK(hipSetDevice(h->device));
+    int rc;
+    if ((rc = upload(h, &h->sig[0], sig_volt, length)) || (rc = upload(h, &h->sig[1], sig_curr, length))) return rc;
+    h->sigLen = length;
+    h->period = period_steps;
+    const unsigned Ks[2] = {nvolt, ncurr};
+    const unsigned* idx[2] = {vidx, cidx};
+    const unsigned short* dir[2] = {vdir, cdir};
+    const float* amp[2] = {vamp, camp};
+    const unsigned* del[2] = {vdelay, cdelay};
+    for (int c = 0; c < 2; ++c) {
+        const unsigned K = Ks[c];
+        h->exc[c].K = K;
+        if (!K) continue;
+        if (!idx[c] || !dir[c] || !amp[c] || !del[c]) return fail(EMS_E_ARG, "null excitation list");
+        std::vector<unsigned> lin(K);
+        for (unsigned k = 0; k < K; ++k) {
+            const unsigned x = idx[c][k], y = idx[c][K + k], z = idx[c][2 * K + k];
+            if (x >= h->N[0] || y >= h->N[1] || z >= h->N[2] || dir[c][k] > 2)
+                return fail(EMS_E_ARG, "excitation cell outside the mesh");
+            lin[k] = (unsigned)h->lin(x, y, z);
+        }
+        if ((rc = upload(h, &h->exc[c].lin, lin.data(), K)) || (rc = upload(h, &h->exc[c].dir, dir[c], K)) ||
+            (rc = upload(h, &h->exc[c].amp, amp[c], K)) || (rc = upload(h, &h->exc[c].delay, del[c], K)))
+            return rc;
+    }
+    h->has_exc = true;
+    return 0;
+}
+
+int ems_add_upml(ems_engine* h, const unsigned start[3], const unsigned num[3], const float* vv, const float* vvfo,
+                 const float* vvfn, const float* ii, const float* iifo, const float* iifn)
+{
+    if (!h || !start || !num || !vv || !vvfo || !vvfn || !ii || !iifo || !iifn) return fail(EMS_E_ARG, "null argument");
+    if (h->inited) return fail(EMS_E_STATE, "engine already initialised");
+    if (h->pmlV.n >= (int)kMaxSlabs) return fail(EMS_E_ARG, "at most 6 UPML slabs");
+    for (int n = 0; n < 3; ++n)
+        if (num[n] == 0 || start[n] + num[n] > h->N[n]) return fail(EMS_E_ARG, "UPML slab outside the mesh");
+    for (int s = 0; s < h->pmlV.n; ++s) {
+        const SlabDev& o = h->pmlV.s[s];
+        bool overlap = (int)start[0] <= o.x1 && (int)(start[0] + num[0] - 1) >= o.x0 && (int)start[1] <= o.y1 &&
+                       (int)(start[1] + num[1] - 1) >= o.y0 && (int)start[2] <= o.z1 && (int)(start[2] + num[2] - 1) >= o.z0;
+        if (overlap) return fail(EMS_E_ARG, "overlapping UPML slabs are not supported");
+    }
+    HIPCHK(hipSetDevice(h->device));
+    const unsigned long long scs = (unsigned long long)num[0] * num[1] * num[2];
+    // repack [3][nx][ny][nz] (z fastest) -> [3][nz][ny][nx] (x fastest)
+    auto repack = [&](const float* src) {
+        std::vector<float> d(3 * scs);
+        for (int n = 0; n < 3; ++n)
+            for (unsigned x = 0; x < num[0]; ++x)
+                for (unsigned y = 0; y < num[1]; ++y)
+                    for (unsigned z = 0; z < num[2]; ++z)
+                        d[n * scs + x + (unsigned long long)num[0] * (y + (unsigned long long)num[1] * z)] =
+                            src[(((size_t)n * num[0] + x) * num[1] + y) * num[2] + z];
+        return d;
+    };
+    SlabDev sv{}, si{};
+    for (SlabDev* s : {&sv, &si}) {
+        s->x0 = start[0];
+        s->y0 = start[1];
+        s->z0 = start[2];
+        s->x1 = start[0] + num[0] - 1;
+        s->y1 = start[1] + num[1] - 1;
+        s->z1 = start[2] + num[2] - 1;
+        s->nx = num[0];
+        s->ny = num[1];
+        s->cs = scs;
+    }
+    int rc;
+    std::vector<float> t;
+    t = repack(vv);
+    if ((rc = upload(h, &sv.c0, t.data(), t.size()))) return rc;
+    t = repack(vvfo);
+    if ((rc = upload(h, &sv.cfo, t.data(), t.size()))) return rc;
+    t = repack(vvfn);
+    if ((rc = upload(h, &sv.cfn, t.data(), t.size()))) return rc;
+    t = repack(ii);
+    if ((rc = upload(h, &si.c0, t.data(), t.size()))) return rc;
+    t = repack(iifo);
+    if ((rc = upload(h, &si.cfo, t.data(), t.size()))) return rc;
+    t = repack(iifn);
+    if ((rc = upload(h, &si.cfn, t.data(), t.size()))) return rc;
+    if ((rc = h->alloc(&sv.flux, 3 * scs)) || (rc = h->alloc(&si.flux, 3 * scs))) return rc;
+    HIPCHK(hipMemset(sv.flux, 0, 3 * scs * sizeof(float)));
+    HIPCHK(hipMemset(si.flux, 0, 3 * scs * sizeof(float)));
+    h->pmlV.s[h->pmlV.n++] = sv;
+    h->pmlI.s[h->pmlI.n++] = si;
+    return 0;
+}
+
+int ems_add_mur(ems_engine* h, int ny, unsigned line, unsigned line_shift, const float* coeff_nyP,
+                const float* coeff_nyPP, unsigned start_ts)
+{
+    if (!h || !coeff_nyP || !coeff_nyPP) return fail(EMS_E_ARG, "null argument");
+    if (h->inited) return fail(EMS_E_STATE, "engine already initialised");
+    if (ny < 0 || ny > 2 || line >= h->N[ny] || line_shift >= h->N[ny]) return fail(EMS_E_ARG, "bad Mur face");
+    HIPCHK(hipSetDevice(h->device));
+    const int nyP = (ny + 1) % 3, nyPP = (ny + 2) % 3;
+    MurDev m{};
+    m.na = h->N[nyP];
+    m.nb = h->N[nyPP];
+    m.offP = nyP * h->cs;
+    m.offPP = nyPP * h->cs;
+    m.start_ts = start_ts;
+    const size_t cnt = (size_t)m.na * m.nb;
+    std::vector<unsigned> face(cnt), shift(cnt);
+    for (unsigned i = 0; i < m.na; ++i)
+        for (unsigned j = 0; j < m.nb; ++j) {
+            unsigned p[3], s[3];
+            p[ny] = line;
+            s[ny] = line_shift;
+            p[nyP] = s[nyP] = i;
+            p[nyPP] = s[nyPP] = j;
+            face[(size_t)i * m.nb + j] = (unsigned)h->lin(p[0], p[1], p[2]);
+            shift[(size_t)i * m.nb + j] = (unsigned)h->lin(s[0], s[1], s[2]);
+        }
+    int rc;
+    unsigned *df, *dsh;
+    float *dcp, *dcpp;
+    if ((rc = upload(h, &df, face.data(), cnt)) || (rc = upload(h, &dsh, shift.data(), cnt)) ||
+        (rc = upload(h, &dcp, coeff_nyP, cnt)) || (rc = upload(h, &dcpp, coeff_nyPP, cnt)) ||
+        (rc = h->alloc(&m.mP, cnt)) || (rc = h->alloc(&m.mPP, cnt)))
+        return rc;
+    HIPCHK(hipMemset(m.mP, 0, cnt * sizeof(float)));
+    HIPCHK(hipMemset(m.mPP, 0, cnt * sizeof(float)));
+    m.face = df;
+    m.shift = dsh;
+    m.cP = dcp;
+    m.cPP = dcpp;
+    h->mur.push_back(m);
+    return 0;
+}
+
+int ems_add_lumped_rlc(ems_engine* h, unsigned count, const unsigned* pos, const int* dir, const float* const coeffs[9])
+{
+    if (!h || !pos || !dir || !coeffs) return fail(EMS_E_ARG, "null argument");
+    if (h->inited) return fail(EMS_E_STATE, "engine already initialised");
+    if (!count) return 0;
+    HIPCHK(hipSetDevice(h->device));
+    RlcDev r{};
+    r.count = count;
+    std::vector<unsigned long long> lin(count);
+    for (unsigned k = 0; k < count; ++k) {
+        const unsigned x = pos[k], y = pos[count + k], z = pos[2 * count + k];
+        if (x >= h->N[0] || y >= h->N[1] || z >= h->N[2] || dir[k] < 0 || dir[k] > 2)
+            return fail(EMS_E_ARG, "lumped element outside the mesh");
+        lin[k] = dir[k] * h->cs + h->lin(x, y, z);
+    }
+    int rc;
+    unsigned long long* dl;
+    if ((rc = upload(h, &dl, lin.data(), count))) return rc;
+    r.lin = dl;
+    for (int c = 0; c < 9; ++c) {
+        float* d;
+        if (!coeffs[c]) return fail(EMS_E_ARG, "null RLC coefficient array");
+        if ((rc = upload(h, &d, coeffs[c], count))) return rc;
+        r.c[c] = d;
+    }
+    if ((rc = h->alloc(&r.vd, 3 * count)) || (rc = h->alloc(&r.jn, 3 * count)) || (rc = h->alloc(&r.il, count)))
+        return rc;
+    HIPCHK(hipMemset(r.vd, 0, 3 * count * sizeof(float)));
+    HIPCHK(hipMemset(r.jn, 0, 3 * count * sizeof(float)));
+    HIPCHK(hipMemset(r.il, 0, count * sizeof(float)));
+    h->rlc.push_back(r);
+    return 0;
+}
+
+int ems_add_lorentz(ems_engine* h, unsigned count, const unsigned* pos, const float* v_int, const float* v_ext,
+                    const float* v_lor, const float* i_int, const float* i_ext, const float* i_lor, int volt_on,
+                    int curr_on)
+{
+    if (!h || !pos) return fail(EMS_E_ARG, "null argument");
+    if (h->inited) return fail(EMS_E_STATE, "engine already initialised");
+    if (!count) return 0;
+    HIPCHK(hipSetDevice(h->device));
+    std::vector<unsigned> lin(count);
+    for (unsigned k = 0; k < count; ++k) {
+        const unsigned x = pos[k], y = pos[count + k], z = pos[2 * count + k];
+        if (x >= h->N[0] || y >= h->N[1] || z >= h->N[2]) return fail(EMS_E_ARG, "dispersive cell outside the mesh");
+        lin[k] = (unsigned)h->lin(x, y, z);
+    }
+    int rc;
+    unsigned* dl;
+    if ((rc = upload(h, &dl, lin.data(), count))) return rc;
+    for (int side = 0; side < 2; ++side) {
+        const bool on = side ? curr_on : volt_on;
+        LorDev d{};
+        d.count = count;
+        d.lin = dl;
+        const float* ci = side ? i_int : v_int;
+        const float* ce = side ? i_ext : v_ext;
+        const float* cl = side ? i_lor : v_lor;
+        if (on) {
+            if (!ci || !ce) return fail(EMS_E_ARG, "null ADE coefficient array");
+            float *a, *b, *c = nullptr;
+            if ((rc = upload(h, &a, ci, 3 * count)) || (rc = upload(h, &b, ce, 3 * count))) return rc;
+            if (cl && (rc = upload(h, &c, cl, 3 * count))) return rc;
+            d.cint = a;
+            d.cext = b;
+            d.clor = c;
+            if ((rc = h->alloc(&d.ade, 3 * count)) || (rc = h->alloc(&d.lor, 3 * count))) return rc;
+            HIPCHK(hipMemset(d.ade, 0, 3 * count * sizeof(float)));
+            HIPCHK(hipMemset(d.lor, 0, 3 * count * sizeof(float)));
+        }
+        h->lor.push_back(d);
+        (side ? h->lor_i : h->lor_v).push_back(on ? 1 : 0);
+    }
+    return 0;
+}
+
+int ems_init(ems_engine* h)
+{
+    if (!h) return fail(EMS_E_ARG, "null handle");
+    h->inited = true;
+    return 0;
+}
+
+int ems_add_probe(ems_engine* h, const ems_probe_desc* d, int* id)
+{
+    if (!h || !d) return fail(EMS_E_ARG, "null argument");
+    if (d->type < 0 || d->type > 3) return fail(EMS_E_ARG, "unknown probe type");
+    if (d->nelem == 0 || !d->comp || !d->pos) return fail(EMS_E_ARG, "probe without terms");
+    if ((d->type == EMS_PROBE_EFIELD || d->type == EMS_PROBE_HFIELD) && (d->nelem != 3 || !d->divisor))
+        return fail(EMS_E_ARG, "field probes take three terms with divisors");
+    if ((d->type == EMS_PROBE_VOLTAGE || d->type == EMS_PROBE_CURRENT) && !d->sign)
+        return fail(EMS_E_ARG, "integral probes need term signs");
+    ProbeDev p{};
+    p.type = d->type;
+    p.nelem = d->nelem;
+    p.off = (unsigned)h->h_elin.size();
+    p.interval = d->interval;
+    p.fd_interval = d->fd_interval;
+    p.start_ts = d->start_ts;
+    p.stop_ts = d->stop_ts;
+    p.capacity = d->capacity;
+    for (unsigned e = 0; e < d->nelem; ++e) {
+        const unsigned x = d->pos[e], y = d->pos[d->nelem + e], z = d->pos[2 * d->nelem + e];
+        if (x >= h->N[0] || y >= h->N[1] || z >= h->N[2] || d->comp[e] > 2)
+            return fail(EMS_E_ARG, "probe term outside the mesh");
+        h->h_elin.push_back(d->comp[e] * h->cs + h->lin(x, y, z));
+        h->h_esign.push_back(d->sign ? (signed char)(d->sign[e] >= 0 ? 1 : -1) : 1);
+        h->h_ediv.push_back(d->divisor ? d->divisor[e] : 0.0);
+    }
+    HIPCHK(hipSetDevice(h->device));
+    int rc;
+    if ((rc = h->alloc(&p.count, 1)) || (rc = h->alloc(&p.ts_out, std::max(1u, p.capacity))) ||
+        (rc = h->alloc(&p.val_out, 3 * (size_t)std::max(1u, p.capacity))))
+        return rc;
+    HIPCHK(hipMemset(p.count, 0, sizeof(unsigned)));
+    h->probes.push_back(p);
+    h->probes_dirty = true;
+    if (id) *id = (int)h->probes.size() - 1;
+    return 0;
+}
+
+int ems_probe_results(ems_engine* h, int id, unsigned* count, unsigned* ts, double* values)
+{
+    if (!h || id < 0 || id >= (int)h->probes.size() || !count) return fail(EMS_E_ARG, "bad probe id");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const ProbeDev& p = h->probes[id];
+    unsigned c = 0;
+    HIPCHK(hipMemcpy(&c, p.count, sizeof(unsigned), hipMemcpyDeviceToHost));
+    *count = c;
+    if (ts && c) HIPCHK(hipMemcpy(ts, p.ts_out, c * sizeof(unsigned), hipMemcpyDeviceToHost));
+    if (values && c) HIPCHK(hipMemcpy(values, p.val_out, 3 * (size_t)c * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int ems_add_fd_dump(ems_engine* h, int field, const unsigned start[3], const unsigned stop[3], unsigned fd_interval,
+                    unsigned nfreq, unsigned nsamples, const float* factors, const double* const edge_len[3], int* id)
+{
+    if (!h || !start || !stop || !factors || !edge_len) return fail(EMS_E_ARG, "null argument");
+    if (field != 0) return fail(EMS_E_ARG, "only E-field frequency-domain dumps are supported");
+    if (fd_interval == 0 || nfreq == 0 || nsamples == 0) return fail(EMS_E_ARG, "empty frequency-domain dump");
+    DumpDev d{};
+    for (int n = 0; n < 3; ++n) {
+        if (start[n] > stop[n] || stop[n] >= h->N[n]) return fail(EMS_E_ARG, "dump box outside the mesh");
+        d.s[n] = start[n];
+        d.n[n] = stop[n] - start[n] + 1;
+        d.N[n] = h->N[n];
+    }
+    d.fd_interval = fd_interval;
+    d.nfreq = nfreq;
+    d.nsamples = nsamples;
+    HIPCHK(hipSetDevice(h->device));
+    int rc;
+    float* df;
+    if ((rc = upload(h, &df, factors, (size_t)nsamples * nfreq * 2))) return rc;
+    d.factors = df;
+    for (int n = 0; n < 3; ++n) {
+        double* de;
+        if ((rc = upload(h, &de, edge_len[n], h->N[n]))) return rc;
+        d.el[n] = de;
+    }
+    const size_t npts = (size_t)d.n[0] * d.n[1] * d.n[2];
+    if ((rc = h->alloc(&d.acc, (size_t)nfreq * 3 * npts * 2))) return rc;
+    HIPCHK(hipMemset(d.acc, 0, (size_t)nfreq * 3 * npts * 2 * sizeof(float)));
+    h->dumps.push_back(d);
+    h->dump_npts.push_back((unsigned)npts);
+    if (id) *id = (int)h->dumps.size() - 1;
+    return 0;
+}
+
+int ems_fd_dump_results(ems_engine* h, int id, float* out)
+{
+    if (!h || id < 0 || id >= (int)h->dumps.size() || !out) return fail(EMS_E_ARG, "bad dump id");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const DumpDev& d = h->dumps[id];
+    HIPCHK(hipMemcpy(out, d.acc, (size_t)d.nfreq * 3 * h->dump_npts[id] * 2 * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+static int sync_probe_tables(ems_engine* h)
+{
+    if (!h->probes_dirty) return 0;
+    int rc;
+    if ((rc = upload(h, &h->d_probes, h->probes.data(), h->probes.size())) ||
+        (rc = upload(h, &h->d_elin, h->h_elin.data(), h->h_elin.size())) ||
+        (rc = upload(h, &h->d_esign, h->h_esign.data(), h->h_esign.size())) ||
+        (rc = upload(h, &h->d_ediv, h->h_ediv.data(), h->h_ediv.size())))
+        return rc;
+    h->probes_dirty = false;
+    return 0;
+}
+
+static bool probe_due(const ems_engine* h, unsigned ts)
+{
+    for (const auto& p : h->probes) {
+        if (ts < p.start_ts || ts > p.stop_ts) continue;
+        if ((p.interval && ts % p.interval == 0) || (p.fd_interval && ts % p.fd_interval == 0)) return true;
+    }
+    return false;
+}
+
+static bool dump_due(const ems_engine* h, unsigned ts)
+{
+    for (const auto& d : h->dumps)
+        if (ts % d.fd_interval == 0 && ts / d.fd_interval < d.nsamples) return true;
+    return false;
+}
+
+static int launch_samples(ems_engine* h)
+{
+    // ProcessingArray::Process after the step: probes and dumps due at numTS
+    if (!h->probes.empty() && probe_due(h, h->numTS)) {
+        hipLaunchKernelGGL(k_probe, dim3((unsigned)h->probes.size()), dim3(kBlock), 0, h->stream, h->V, h->I,
+                           h->d_probes, h->d_elin, h->d_esign, h->d_ediv, h->numts_dev);
+    }
+    for (size_t k = 0; k < h->dumps.size(); ++k) {
+        const DumpDev& d = h->dumps[k];
+        if (h->numTS % d.fd_interval || h->numTS / d.fd_interval >= d.nsamples) continue;
+        const unsigned npts = h->dump_npts[k];
+        hipLaunchKernelGGL(k_fd_dump, dim3((npts + kBlock - 1) / kBlock), dim3(kBlock), 0, h->stream, h->V, h->cs,
+                           h->N[0], h->g.plane, d, h->numts_dev);
+    }
+    return 0;
+}
+
+// one timestep in the reference's hook order (Engine::IterateTS, FDTD/engine.cpp:267-286,
+// priorities: UPML > default (lumped RLC, Lorentz, Mur) > excitation)
+static void launch_step(ems_engine* h, bool time_kernels, double* vms, double* cms)
+{
+    const GridDesc& g = h->g;
+    const unsigned nblk = g.qtiles * g.nzc;
+    const unsigned T = 256;
+    // -- voltage pre hooks: Mur (creation order), Lorentz, lumped RLC; UPML pre is fused
+    for (const auto& m : h->mur)
+        hipLaunchKernelGGL(k_mur_pre, dim3((m.na * m.nb + T - 1) / T), dim3(T), 0, h->stream, h->V, m, h->numts_dev);
+    for (size_t k = 0; k < h->lor.size(); k += 2)
+        if (h->lor_v[k / 2])
+            hipLaunchKernelGGL(k_lor_pre, dim3((h->lor[k].count + T - 1) / T), dim3(T), 0, h->stream, h->V, h->cs, h->lor[k]);
+    for (const auto& r : h->rlc)
+        hipLaunchKernelGGL(k_rlc_pre, dim3((r.count + T - 1) / T), dim3(T), 0, h->stream, r, h->numts_dev);
+    if (time_kernels) hipEventRecord(h->kev[0], h->stream);
+    if (h->pmlV.n)
+        hipLaunchKernelGGL(k_volt<true>, dim3(nblk), dim3(kBlock), 0, h->stream, g, h->V, h->I, h->vv, h->vi, h->pmlV);
+    else
+        hipLaunchKernelGGL(k_volt<false>, dim3(nblk), dim3(kBlock), 0, h->stream, g, h->V, h->I, h->vv, h->vi, h->pmlV);
+    if (time_kernels) hipEventRecord(h->kev[1], h->stream);
+    // -- post hooks: Mur (reverse creation order, all read-only on V)
+    for (int k = (int)h->mur.size() - 1; k >= 0; --k) {
+        const auto& m = h->mur[k];
+        hipLaunchKernelGGL(k_mur_post, dim3((m.na * m.nb + T - 1) / T), dim3(T), 0, h->stream, h->V, m, h->numts_dev);
+    }
+    // -- apply hooks: lumped RLC, Lorentz, Mur (reverse creation order), excitation
+    for (const auto& r : h->rlc)
+        hipLaunchKernelGGL(k_rlc_apply, dim3((r.count + T - 1) / T), dim3(T), 0, h->stream, h->V, r, h->numts_dev);
+    for (size_t k = 0; k < h->lor.size(); k += 2)
+        if (h->lor_v[k / 2])
+            hipLaunchKernelGGL(k_lor_apply, dim3((h->lor[k].count + T - 1) / T), dim3(T), 0, h->stream, h->V, h->cs, h->lor[k]);
+    for (int k = (int)h->mur.size() - 1; k >= 0; --k) {
+        const auto& m = h->mur[k];
+        hipLaunchKernelGGL(k_mur_apply, dim3((m.na * m.nb + T - 1) / T), dim3(T), 0, h->stream, h->V, m, h->numts_dev);
+    }
+    if (h->exc[0].K) {
+        const auto& e = h->exc[0];
+        hipLaunchKernelGGL(k_excite, dim3((e.K + T - 1) / T), dim3(T), 0, h->stream, h->V, h->cs, e.K, e.lin, e.dir,
+                           e.amp, e.delay, h->sig[0], h->sigLen, h->period, h->numts_dev, 0);
+    }
+    // -- current half step: Lorentz pre, fused UPML + update, Lorentz apply, excitation
+    for (size_t k = 1; k < h->lor.size(); k += 2)
+        if (h->lor_i[k / 2])
+            hipLaunchKernelGGL(k_lor_pre, dim3((h->lor[k].count + T - 1) / T), dim3(T), 0, h->stream, h->I, h->cs, h->lor[k]);
+    if (time_kernels) hipEventRecord(h->kev[2], h->stream);
+    if (h->pmlI.n)
+        hipLaunchKernelGGL(k_curr<true>, dim3(nblk), dim3(kBlock), 0, h->stream, g, h->I, h->V, h->ii, h->iv, h->pmlI,
+                           h->numts_dev);
+    else
+        hipLaunchKernelGGL(k_curr<false>, dim3(nblk), dim3(kBlock), 0, h->stream, g, h->I, h->V, h->ii, h->iv, h->pmlI,
+                           h->numts_dev);
+    if (time_kernels) hipEventRecord(h->kev[3], h->stream);
+    for (size_t k = 1; k < h->lor.size(); k += 2)
+        if (h->lor_i[k / 2])
+            hipLaunchKernelGGL(k_lor_apply, dim3((h->lor[k].count + T - 1) / T), dim3(T), 0, h->stream, h->I, h->cs, h->lor[k]);
+    if (h->exc[1].K) {
+        const auto& e = h->exc[1];
+        hipLaunchKernelGGL(k_excite, dim3((e.K + T - 1) / T), dim3(T), 0, h->stream, h->I, h->cs, e.K, e.lin, e.dir,
+                           e.amp, e.delay, h->sig[1], h->sigLen, h->period, h->numts_dev, -1);
+    }
+    (void)vms;
+    (void)cms;
+}
+
+int ems_set_profiling(ems_engine* h, int enable)
+{
+    if (!h) return fail(EMS_E_ARG, "null handle");
+    h->profile = enable != 0;
+    if (h->profile && h->kev.empty()) {
+        h->kev.resize(4);
+        for (auto& e : h->kev) HIPCHK(hipEventCreate(&e));
+    }
+    return 0;
+}
+
+int ems_iterate(ems_engine* h, unsigned nsteps)
+{
+    if (!h) return fail(EMS_E_ARG, "null handle");
+    if (!h->inited) ems_init(h);
+    HIPCHK(hipSetDevice(h->device));
+    int rc = sync_probe_tables(h);
+    if (rc) return rc;
+    if (h->numTS == 0 && nsteps > 0) {
+        // Process() before the first step (openems.cpp:1424-1425)
+        launch_samples(h);
+    }
+    HIPCHK(hipEventRecord(h->ev0, h->stream));
+    double vsum = 0, csum = 0;
+    unsigned timed = 0;
+    for (unsigned s = 0; s < nsteps; ++s) {
+        const bool tk = h->profile;
+        launch_step(h, tk, nullptr, nullptr);
+        ++h->numTS;
+        launch_samples(h);
+        if (tk) {
+            HIPCHK(hipEventSynchronize(h->kev[3]));
+            float a = 0, b = 0;
+            hipEventElapsedTime(&a, h->kev[0], h->kev[1]);
+            hipEventElapsedTime(&b, h->kev[2], h->kev[3]);
+            vsum += a;
+            csum += b;
+            ++timed;
+        }
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(h->ev1, h->stream));
+    if (!h->async || h->profile) {
+        HIPCHK(hipEventSynchronize(h->ev1));
+        float ms = 0;
+        hipEventElapsedTime(&ms, h->ev0, h->ev1);
+        h->last_ms = ms;
+    }
+    if (timed) {
+        h->last_volt_ms = vsum / timed;
+        h->last_curr_ms = csum / timed;
+    }
+    return 0;
+}
+
+unsigned ems_get_numts(const ems_engine* h) { return h ? h->numTS : 0; }
+
+int ems_set_async(ems_engine* h, int async)
+{
+    if (!h) return fail(EMS_E_ARG, "null handle");
+    h->async = async != 0;
+    return 0;
+}
+
+int ems_sync(ems_engine* h)
+{
+    if (!h) return fail(EMS_E_ARG, "null handle");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+static int get_field(ems_engine* h, const float* d, float* out)
+{
+    if (!h || !out) return fail(EMS_E_ARG, "null argument");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    std::vector<float> stage(3 * h->cs);
+    HIPCHK(hipMemcpy(stage.data(), d, stage.size() * sizeof(float), hipMemcpyDeviceToHost));
+    soa_to_nijk(stage.data(), out, h->N, h->cs);
+    return 0;
+}
+
+static int set_field(ems_engine* h, float* d, const float* in)
+{
+    if (!h || !in) return fail(EMS_E_ARG, "null argument");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return upload_nijk(h, d, in);
+}
+
+int ems_get_volt(ems_engine* h, float* out) { return h ? get_field(h, h->V, out) : fail(EMS_E_ARG, "null handle"); }
+int ems_get_curr(ems_engine* h, float* out) { return h ? get_field(h, h->I, out) : fail(EMS_E_ARG, "null handle"); }
+int ems_set_volt(ems_engine* h, const float* in) { return h ? set_field(h, h->V, in) : fail(EMS_E_ARG, "null handle"); }
+int ems_set_curr(ems_engine* h, const float* in) { return h ? set_field(h, h->I, in) : fail(EMS_E_ARG, "null handle"); }
+
+int ems_energy(ems_engine* h, double* out)
+{
+    if (!h || !out) return fail(EMS_E_ARG, "null argument");
+    HIPCHK(hipSetDevice(h->device));
+    const unsigned nb = 1024;
+    if (!h->energy_part) {
+        int rc = h->alloc(&h->energy_part, 2 * nb);
+        if (rc) return rc;
+    }
+    double* part = h->energy_part;
+    hipLaunchKernelGGL(k_energy, dim3(nb), dim3(kBlock), 0, h->stream, h->V, h->I, h->cs, h->N[0], h->N[1], h->N[2], part);
+    std::vector<double> hp(2 * nb);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(hp.data(), part, hp.size() * sizeof(double), hipMemcpyDeviceToHost));
+    double e = 0, m = 0;
+    for (unsigned b = 0; b < nb; ++b) {
+        e += hp[2 * b];
+        m += hp[2 * b + 1];
+    }
+    *out = 8.85418781762e-12 * e + 1.256637062e-6 * m;
+    return 0;
+}
+
+int ems_last_timing(ems_engine* h, double* total_ms, double* volt_ms, double* curr_ms)
+{
+    if (!h) return fail(EMS_E_ARG, "null handle");
+    if (total_ms) *total_ms = h->last_ms;
+    if (volt_ms) *volt_ms = h->last_volt_ms;
+    if (curr_ms) *curr_ms = h->last_curr_ms;
+    return 0;
+}
+
+void ems_destroy(ems_engine* h)
+{
+    if (!h) return;
+    hipSetDevice(h->device);
+    if (h->stream) hipStreamSynchronize(h->stream);
+    for (void* p : h->allocs) hipFree(p);
+    if (h->ev0) hipEventDestroy(h->ev0);
+    if (h->ev1) hipEventDestroy(h->ev1);
+    for (auto& e : h->kev) hipEventDestroy(e);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+}
+
+}  // extern "C"

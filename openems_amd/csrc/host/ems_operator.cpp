@@ -1,0 +1,1470 @@
+// Host-side EC-FDTD operator builder for box scenes (see include/ems_operator.h).
+//
+// Every floating-point expression keeps the reference's operand order and
+// precision (float EC arrays, double intermediates), and the file is compiled
+// with -ffp-contract=off, so the float32 coefficients come out bit-identical
+// to the reference Operator for the same mesh and materials.
+//
+// Reference functions restated here (FDTD/operator.cpp unless noted):
+//   GetDiscLine/GetDiscDelta/GetEdgeLength      :144-213
+//   SnapToMeshLine/SnapToMesh/SnapBox2Mesh        :258-327
+//   Calc_ECOperatorPos                            :956-984
+//   CalcECOperator                                :986-1097
+//   ApplyElectricBC / ApplyMagneticBC             :1099-1187
+//   Calc_ECPos                                    :1189-1254
+//   GetRawDiscDelta                               :1256-1266
+//   AverageMatCellCenter / AverageMatQuarterCell  :1336-1564
+//   Calc_LumpedElements (parallel RC)             :1586-1763
+//   CalcTimestep_Var1 / Var3                      :1892-2030
+//   CalcPEC_Range                                 :2045-2083
+//   Excitation signals                            FDTD/excitation.cpp:150-276
+//   Operator_Ext_Excitation::BuildExtension       FDTD/extensions/operator_ext_excitation.cpp:105-297
+//   Operator_Ext_UPML::Create_UPML/CalcGradingKappa/BuildExtension
+//                                                 FDTD/extensions/operator_ext_upml.cpp:69-445
+//   Operator_Ext_Mur_ABC::BuildExtension          FDTD/extensions/operator_ext_mur_abc.cpp:80-186
+//   Engine_Ext_Mur_ABC late start                 FDTD/extensions/engine_ext_mur_abc.cpp:28-60
+//
+// Material lookup: the CSXCAD rule "highest priority primitive containing the
+// point wins" is evaluated with per-axis bitmasks over boxes ranked by
+// (priority, earlier insertion first): the winner at a point is the highest
+// rank set in mask_x & mask_y & mask_z.
+#include "ems_operator.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <limits>
+#include <new>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace {
+
+constexpr double kEPS0 = 8.85418781762e-12;  // tools/constants.h:23
+constexpr double kMUE0 = 1.256637062e-6;     // tools/constants.h:24
+constexpr double kC0 = 299792458;            // tools/constants.h:25
+constexpr double kZ0 = 376.730313461;        // tools/constants.h:26
+constexpr double kPI = 3.141592653589793238462643383279;  // tools/constants.h:27
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+struct Box {
+    double lo[3], hi[3];
+    int prio;
+    int order;
+};
+
+struct MatBox : Box {
+    double eps[3], mue[3], kappa[3], sigma[3];
+    bool metal;
+};
+
+struct LumpedBox : Box {
+    int dir;
+    double R, C, L;
+    bool caps, parallel;
+};
+
+struct ExcBox : Box {
+    int type;
+    double val[3];
+    double delay;
+};
+
+struct Slab {
+    unsigned start[3], num[3];
+    std::vector<float> a[6];  // vv, vvfo, vvfn, ii, iifo, iifn
+};
+
+struct MurFace {
+    int ny;
+    unsigned line, shift, start_ts;
+    std::vector<float> cP, cPP;
+};
+
+struct ExcList {
+    std::vector<unsigned> idx[3];
+    std::vector<unsigned short> dir;
+    std::vector<float> amp;
+    std::vector<unsigned> delay;
+};
+
+void parallel_for(unsigned n, int nthreads, const std::function<void(unsigned, unsigned)>& fn)
+{
+    unsigned nt = nthreads > 0 ? (unsigned)nthreads : std::max(1u, std::thread::hardware_concurrency());
+    nt = std::min(nt, std::max(1u, n));
+    if (nt <= 1) {
+        fn(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    unsigned per = n / nt, rem = n % nt, s = 0;
+    for (unsigned t = 0; t < nt; ++t) {
+        unsigned c = per + (t < rem ? 1 : 0);
+        th.emplace_back(fn, s, s + c);
+        s += c;
+    }
+    for (auto& t : th) t.join();
+}
+
+// ranked box set with per-axis membership masks
+template <typename B>
+struct Ranked {
+    std::vector<const B*> box;  // ascending rank
+    unsigned W = 0;             // 64-bit words per mask
+    void init(const std::vector<B>& all, std::function<bool(const B&)> keep)
+    {
+        box.clear();
+        for (const auto& b : all)
+            if (keep(b)) box.push_back(&b);
+        std::stable_sort(box.begin(), box.end(), [](const B* a, const B* b) {
+            if (a->prio != b->prio) return a->prio < b->prio;
+            return a->order > b->order;  // earlier insertion ranks higher among equals
+        });
+        W = (unsigned)((box.size() + 63) / 64);
+    }
+    // mask[i*W + w] for coordinates c[i] along axis n
+    std::vector<uint64_t> masks(int n, const std::vector<double>& c) const
+    {
+        std::vector<uint64_t> m((size_t)c.size() * W, 0);
+        for (size_t r = 0; r < box.size(); ++r) {
+            double a = std::min(box[r]->lo[n], box[r]->hi[n]);
+            double b = std::max(box[r]->lo[n], box[r]->hi[n]);
+            for (size_t i = 0; i < c.size(); ++i)
+                if (c[i] >= a && c[i] <= b) m[i * W + r / 64] |= (uint64_t)1 << (r % 64);
+        }
+        return m;
+    }
+    // highest rank set in the AND of three masks, -1 if none
+    inline int winner(const uint64_t* a, const uint64_t* b, const uint64_t* c) const
+    {
+        for (int w = (int)W - 1; w >= 0; --w) {
+            uint64_t x = a[w] & b[w] & c[w];
+            if (x) return w * 64 + 63 - __builtin_clzll(x);
+        }
+        return -1;
+    }
+};
+
+}  // namespace
+
+struct emsop {
+    // scene
+    std::vector<double> lines[3];
+    unsigned N[3] = {0, 0, 0};
+    double gd = 1;
+    double bg[4] = {1, 1, 0, 0};
+    std::vector<MatBox> mats;
+    std::vector<LumpedBox> lumped;
+    std::vector<ExcBox> excs;
+    int bc[6] = {0, 0, 0, 0, 0, 0};
+    unsigned pml[6] = {8, 8, 8, 8, 8, 8};
+    double vph[6] = {0, 0, 0, 0, 0, 0};
+    int sigType = -1;
+    double f0 = 0, fc = 0;
+    int tsMethod = 3;
+    double forcedDt = 0, tsFactor = 1;
+    bool cellConst = false;
+    int nthreads = 0;
+    bool built = false;
+    // derived mesh tables
+    std::vector<double> primD[3], dualL[3], dualD[3];
+    // results
+    double dT = 0;
+    unsigned nyquist = 0;
+    double period = 0;
+    unsigned sigLen = 0;
+    std::vector<float> sigV, sigC;
+    std::vector<float> coeff[4];  // vv vi ii iv
+    std::vector<float> ec[4];     // C G L R
+    // material sampling tables, one set per field component
+    Ranked<MatBox> matrk;
+    struct Samp {
+        std::vector<uint64_t> eps_n, P_up, P_dn, PP_up, PP_dn, mu_dn, mu_up, P_half, PP_half, dual_n, dual_P, dual_PP;
+    } samp[3];
+    ExcList exc[2];
+    std::vector<Slab> upml;
+    std::vector<MurFace> mur;
+
+    size_t size() const { return (size_t)N[0] * N[1] * N[2]; }
+    size_t idx(int n, unsigned x, unsigned y, unsigned z) const
+    {
+        return (((size_t)n * N[0] + x) * N[1] + y) * N[2] + z;
+    }
+    // GetDiscDelta(n,i,false)*gridDelta / GetDiscDelta(n,i,true)*gridDelta
+    double EL(int n, unsigned i, bool dual) const { return (dual ? dualD[n][i] : primD[n][i]) * gd; }
+    // GetNodeWidth(n, loc, true) for an int index that may be negative
+    double Wp(int n, int i) const { return i < 0 ? 0.0 : primD[n][i] * gd; }
+    // GetRawDiscDelta (operator.cpp:1256-1266)
+    double rawD(int n, int pos) const
+    {
+        const auto& l = lines[n];
+        if (pos < 0) return l[0] - l[1];
+        if (pos >= (int)N[n] - 1) return l[N[n] - 2] - l[N[n] - 1];
+        return l[pos + 1] - l[pos];
+    }
+
+    void mesh_tables()
+    {
+        for (int n = 0; n < 3; ++n) {
+            const auto& l = lines[n];
+            unsigned c = N[n];
+            primD[n].assign(c, 0);
+            dualL[n].assign(c, 0);
+            dualD[n].assign(c, 0);
+            for (unsigned i = 0; i < c; ++i) {
+                primD[n][i] = (i < c - 1) ? l[i + 1] - l[i] : l[i] - l[i - 1];
+                dualL[n][i] = (i < c - 1) ? 0.5 * (l[i] + l[i + 1]) : l[i] + 0.5 * (l[i] - l[i - 1]);
+            }
+            for (unsigned i = 0; i < c; ++i) dualD[n][i] = (i > 0) ? dualL[n][i] - dualL[n][i - 1] : l[1] - l[0];
+        }
+    }
+
+    unsigned snap_line(int n, double coord, bool& inside, bool dual) const
+    {
+        // SnapToMeshLine (operator.cpp:258-287)
+        inside = false;
+        const auto& l = lines[n];
+        if (coord < l[0]) return 0;
+        if (coord > l[N[n] - 1]) return N[n] - 1;
+        inside = true;
+        if (!dual) {
+            for (unsigned i = 0; i < N[n]; ++i)
+                if (coord <= dualL[n][i]) return i;
+        } else {
+            for (unsigned i = 1; i < N[n]; ++i)
+                if (coord <= l[i]) return i - 1;
+        }
+        return 0;
+    }
+
+    int snap_box(const double* start, const double* stop, bool dual, unsigned* us, unsigned* ue, bool* sin, bool* ein) const
+    {
+        // SnapBox2Mesh, SnapMethod 0 (operator.cpp:303-327)
+        double ls[3], le[3];
+        for (int n = 0; n < 3; ++n) {
+            ls[n] = std::fmin(start[n], stop[n]);
+            le[n] = std::fmax(start[n], stop[n]);
+            double mn = lines[n][0], mx = lines[n][N[n] - 1];
+            if ((ls[n] < mn && le[n] < mn) || (ls[n] > mx && le[n] > mx)) return -2;
+        }
+        int dim = 0;
+        for (int n = 0; n < 3; ++n) {
+            bool a, b;
+            us[n] = snap_line(n, ls[n], a, dual);
+            ue[n] = snap_line(n, le[n], b, dual);
+            if (sin) sin[n] = a;
+            if (ein) ein[n] = b;
+            if (ue[n] > us[n]) ++dim;
+        }
+        return dim;
+    }
+
+    int build(unsigned nrts);
+    void prepare_sampling();
+    double matval(int r, int kind, int ny) const;
+    void effmat_at(int ny, const unsigned* p, double* E) const;
+    int calc_ec();
+    double timestep_var3() const;
+    double timestep_var1() const;
+    void ec_to_coeff(int n, size_t i);
+    void apply_bc();
+    void calc_pec();
+    int lumped_rc();
+    int build_excitation();
+    void build_mur();
+    void build_upml();
+    int build_signal(unsigned nrts);
+    void grading(int n, int ny, unsigned p, const int* bcs, const unsigned* size, double& kv, double& ki) const;
+};
+
+// ---------------------------------------------------------------------------
+// effective materials and EC (AverageMatQuarterCell / AverageMatCellCenter / Calc_ECPos)
+// ---------------------------------------------------------------------------
+void emsop::prepare_sampling()
+{
+    matrk.init(mats, [](const MatBox& b) { return !b.metal; });
+    for (int ny = 0; ny < 3; ++ny) {
+        const int n = ny, nP = (ny + 1) % 3, nPP = (ny + 2) % 3;
+        Samp& S = samp[ny];
+        std::vector<double> c_eps_n(N[n]), c_mu_dn(N[n]), c_mu_up(N[n]);
+        std::vector<double> cPu(N[nP]), cPd(N[nP]), cPh(N[nP]), cPPu(N[nPP]), cPPd(N[nPP]), cPPh(N[nPP]);
+        for (unsigned i = 0; i < N[n]; ++i) {
+            c_eps_n[i] = lines[n][i] + rawD(n, i) * 0.5;
+            c_mu_dn[i] = lines[n][i] - rawD(n, (int)i - 1) * 0.25;
+            c_mu_up[i] = lines[n][i] + rawD(n, i) * 0.25;
+        }
+        for (unsigned i = 0; i < N[nP]; ++i) {
+            cPu[i] = lines[nP][i] + rawD(nP, i) * 0.25;
+            cPd[i] = lines[nP][i] - rawD(nP, (int)i - 1) * 0.25;
+            cPh[i] = lines[nP][i] + rawD(nP, i) * 0.5;
+        }
+        for (unsigned i = 0; i < N[nPP]; ++i) {
+            cPPu[i] = lines[nPP][i] + rawD(nPP, i) * 0.25;
+            cPPd[i] = lines[nPP][i] - rawD(nPP, (int)i - 1) * 0.25;
+            cPPh[i] = lines[nPP][i] + rawD(nPP, i) * 0.5;
+        }
+        S.eps_n = matrk.masks(n, c_eps_n);
+        S.P_up = matrk.masks(nP, cPu);
+        S.P_dn = matrk.masks(nP, cPd);
+        S.PP_up = matrk.masks(nPP, cPPu);
+        S.PP_dn = matrk.masks(nPP, cPPd);
+        S.mu_dn = matrk.masks(n, c_mu_dn);
+        S.mu_up = matrk.masks(n, c_mu_up);
+        S.P_half = matrk.masks(nP, cPh);
+        S.PP_half = matrk.masks(nPP, cPPh);
+        S.dual_n = matrk.masks(n, dualL[n]);
+        S.dual_P = matrk.masks(nP, dualL[nP]);
+        S.dual_PP = matrk.masks(nPP, dualL[nPP]);
+    }
+}
+
+double emsop::matval(int r, int kind, int ny) const
+{
+    // kind 0 eps, 1 kappa, 2 mue, 3 sigma (GetMaterial, operator.cpp:1281-1334)
+    if (r < 0) return kind == 0 ? bg[0] : kind == 1 ? bg[2] : kind == 2 ? bg[1] : bg[3];
+    const MatBox* b = matrk.box[r];
+    switch (kind) {
+    case 0: return b->eps[ny];
+    case 1: return b->kappa[ny];
+    case 2: return b->mue[ny];
+    default: return b->sigma[ny];
+    }
+}
+
+void emsop::effmat_at(int ny, const unsigned* p, double* E) const
+{
+    const int n = ny, nP = (ny + 1) % 3, nPP = (ny + 2) % 3;
+    const unsigned W = matrk.W;
+    const Samp& S = samp[ny];
+    const int in = p[n], iP = p[nP], iPP = p[nPP];
+    double E0, E1, E2, E3;
+    if (!cellConst) {
+        // AverageMatQuarterCell (operator.cpp:1447-1564)
+        const uint64_t* mn = &S.eps_n[(size_t)in * W];
+        int r0 = matrk.winner(mn, &S.P_up[(size_t)iP * W], &S.PP_up[(size_t)iPP * W]);
+        int r1 = matrk.winner(mn, &S.P_dn[(size_t)iP * W], &S.PP_up[(size_t)iPP * W]);
+        int r2 = matrk.winner(mn, &S.P_up[(size_t)iP * W], &S.PP_dn[(size_t)iPP * W]);
+        int r3 = matrk.winner(mn, &S.P_dn[(size_t)iP * W], &S.PP_dn[(size_t)iPP * W]);
+        double A0 = Wp(nP, iP) * Wp(nPP, iPP);
+        double A1 = Wp(nP, iP - 1) * Wp(nPP, iPP);
+        double A2 = Wp(nP, iP) * Wp(nPP, iPP - 1);
+        double A3 = Wp(nP, iP - 1) * Wp(nPP, iPP - 1);
+        double area = 0;
+        E0 = matval(r0, 0, ny) * A0;
+        E1 = matval(r0, 1, ny) * A0;
+        area += A0;
+        E0 += matval(r1, 0, ny) * A1;
+        E1 += matval(r1, 1, ny) * A1;
+        area += A1;
+        E0 += matval(r2, 0, ny) * A2;
+        E1 += matval(r2, 1, ny) * A2;
+        area += A2;
+        E0 += matval(r3, 0, ny) * A3;
+        E1 += matval(r3, 1, ny) * A3;
+        area += A3;
+        E0 *= kEPS0 / area;
+        E1 /= area;
+        const uint64_t* mPh = &S.P_half[(size_t)iP * W];
+        const uint64_t* mPPh = &S.PP_half[(size_t)iPP * W];
+        int rd = matrk.winner(&S.mu_dn[(size_t)in * W], mPh, mPPh);
+        int ru = matrk.winner(&S.mu_up[(size_t)in * W], mPh, mPPh);
+        double dny = Wp(n, in - 1);
+        E2 = dny / matval(rd, 2, ny);
+        double sg = matval(rd, 3, ny);
+        E3 = sg ? dny / sg : 0;
+        double length = dny;
+        dny = Wp(n, in);
+        E2 += dny / matval(ru, 2, ny);
+        sg = matval(ru, 3, ny);
+        if (sg)
+            E3 += dny / sg;
+        else
+            E3 = 0;
+        length += dny;
+        E2 = length * kMUE0 / E2;
+        if (E3) E3 = length / E3;
+    } else {
+        // AverageMatCellCenter (operator.cpp:1336-1445)
+        int lp[3] = {(int)p[0], (int)p[1], (int)p[2]};
+        double area = 0;
+        E0 = E1 = E2 = E3 = 0;
+        const int sh[4][2] = {{0, 0}, {-1, 0}, {0, -1}, {-1, -1}};
+        for (int s = 0; s < 4; ++s) {
+            lp[nP] = iP + sh[s][0];
+            lp[nPP] = iPP + sh[s][1];
+            if (lp[nP] < 0 || lp[nPP] < 0) continue;
+            int r = matrk.winner(&S.dual_n[(size_t)in * W], &S.dual_P[(size_t)lp[nP] * W], &S.dual_PP[(size_t)lp[nPP] * W]);
+            double A = Wp(nP, lp[nP]) * Wp(nPP, lp[nPP]);
+            E0 += matval(r, 0, ny) * A;
+            E1 += matval(r, 1, ny) * A;
+            area += A;
+        }
+        E0 *= kEPS0 / area;
+        E1 /= area;
+        double length = 0;
+        for (int s = -1; s <= 0; ++s) {
+            int ln = in + s;
+            if (ln < 0) continue;
+            int r = matrk.winner(&S.dual_n[(size_t)ln * W], &S.dual_P[(size_t)iP * W], &S.dual_PP[(size_t)iPP * W]);
+            double dny = Wp(n, ln);
+            E2 += dny / matval(r, 2, ny);
+            double sg = matval(r, 3, ny);
+            if (sg)
+                E3 += dny / sg;
+            else
+                E3 = 0;
+            length += dny;
+        }
+        E2 = length * kMUE0 / E2;
+        if (E3) E3 = length / E3;
+    }
+    E[0] = E0;
+    E[1] = E1;
+    E[2] = E2;
+    E[3] = E3;
+}
+
+int emsop::calc_ec()
+{
+    const size_t tot = size();
+    for (int k = 0; k < 4; ++k) ec[k].assign(3 * tot, 0.0f);
+    prepare_sampling();
+    std::vector<int> bad(N[0], 0);
+    for (int ny = 0; ny < 3; ++ny) {
+        const int nP = (ny + 1) % 3, nPP = (ny + 2) % 3;
+        parallel_for(N[0], nthreads, [&](unsigned x0, unsigned x1) {
+            unsigned p[3];
+            double E[4];
+            for (p[0] = x0; p[0] < x1; ++p[0])
+                for (p[1] = 0; p[1] < N[1]; ++p[1])
+                    for (p[2] = 0; p[2] < N[2]; ++p[2]) {
+                        effmat_at(ny, p, E);
+                        for (int k = 0; k < 4; ++k)
+                            if (std::isnan(E[k]) || std::isinf(E[k])) bad[p[0]] = 1;
+                        const size_t q = idx(ny, p[0], p[1], p[2]);
+                        // Calc_ECPos (operator.cpp:1189-1254)
+                        double delta = EL(ny, p[ny], false);
+                        double area = EL(nP, p[nP], true) * EL(nPP, p[nPP], true);
+                        if (delta) {
+                            ec[0][q] = (float)(E[0] * area / delta);
+                            ec[1][q] = (float)(E[1] * area / delta);
+                        }
+                        delta = EL(ny, p[ny], true);
+                        area = EL(nP, p[nP], false) * EL(nPP, p[nPP], false);
+                        if (delta) {
+                            ec[2][q] = (float)(E[2] * area / delta);
+                            ec[3][q] = (float)(E[3] * area / delta);
+                        }
+                    }
+        });
+    }
+    for (int b : bad)
+        if (b) return fail(EMSOP_E_NUMERIC, "an effective material parameter is not finite");
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// timestep (CalcTimestep_Var3 / Var1)
+// ---------------------------------------------------------------------------
+double emsop::timestep_var3() const
+{
+    const std::vector<float>& C = ec[0];
+    const std::vector<float>& L = ec[2];
+    std::vector<double> best(N[2], 1e200);
+    parallel_for(N[2], nthreads, [&](unsigned z0, unsigned z1) {
+        for (int n = 0; n < 3; ++n) {
+            const int nP = (n + 1) % 3, nPP = (n + 2) % 3;
+            unsigned p[3];
+            for (p[2] = z0; p[2] < z1; ++p[2])
+                for (p[1] = 0; p[1] < N[1]; ++p[1])
+                    for (p[0] = 0; p[0] < N[0]; ++p[0]) {
+                        // AdrOp with SetReflection2Cell: an out-of-range neighbour is the cell itself
+                        auto at = [&](int c, int dn, int dnP, int dnPP, int) -> size_t {
+                            int q[3] = {(int)p[0], (int)p[1], (int)p[2]};
+                            int off[3] = {0, 0, 0};
+                            off[n] += dn;
+                            off[nP] += dnP;
+                            off[nPP] += dnPP;
+                            for (int k = 0; k < 3; ++k) {
+                                int v = q[k] + off[k];
+                                if (v < 0 || v > (int)N[k] - 1) v = q[k];
+                                q[k] = v;
+                            }
+                            return idx(c, q[0], q[1], q[2]);
+                        };
+                        float wa = 1 / (L[at(nPP, 0, 0, 0, 0)] * C[at(n, 0, 1, 0, 0)]) + 1 / (L[at(nPP, 0, 0, 0, 0)] * C[at(n, 0, 0, 0, 0)]);
+                        double wqp = wa;
+                        float wb = 1 / (L[at(nP, 0, 0, 0, 0)] * C[at(n, 0, 0, 1, 0)]) + 1 / (L[at(nP, 0, 0, 0, 0)] * C[at(n, 0, 0, 0, 0)]);
+                        wqp += wb;
+                        float wc = 1 / (L[at(nPP, 0, -1, 0, 0)] * C[at(n, 0, 0, 0, 0)]) + 1 / (L[at(nPP, 0, -1, 0, 0)] * C[at(n, 0, -1, 0, 0)]);
+                        wqp += wc;
+                        float wd = 1 / (L[at(nP, 0, -1, -1, 0)] * C[at(n, 0, -1, 0, 0)]) + 1 / (L[at(nP, 0, -1, -1, 0)] * C[at(n, 0, -1, -1, 0)]);
+                        wqp += wd;
+                        double w4[4];
+                        w4[0] = 1 / (L[at(nPP, 0, 0, 0, 0)] * C[at(nP, 0, 0, 0, 0)]);
+                        w4[1] = 1 / (L[at(nPP, 0, -1, 0, 0)] * C[at(nP, 0, 0, 0, 0)]);
+                        w4[2] = 1 / (L[at(nP, 0, 0, 0, 0)] * C[at(nPP, 0, 0, 0, 0)]);
+                        w4[3] = 1 / (L[at(nP, 0, 0, -1, 0)] * C[at(nPP, 0, 0, 0, 0)]);
+                        double mn = w4[0];
+                        for (int k = 1; k < 4; ++k)
+                            if (w4[k] < mn) mn = w4[k];
+                        double wt1 = w4[0] + w4[1] + w4[2] + w4[3] - 2 * mn;
+                        w4[0] = 1 / (L[at(nPP, 0, 0, 0, 0)] * C[at(nP, 1, 0, 0, 0)]);
+                        w4[1] = 1 / (L[at(nPP, 0, -1, 0, 0)] * C[at(nP, 1, 0, 0, 0)]);
+                        w4[2] = 1 / (L[at(nP, 0, 0, 0, 0)] * C[at(nPP, 1, 0, 0, 0)]);
+                        w4[3] = 1 / (L[at(nP, 0, 0, -1, 0)] * C[at(nPP, 1, 0, 0, 0)]);
+                        mn = w4[0];
+                        for (int k = 1; k < 4; ++k)
+                            if (w4[k] < mn) mn = w4[k];
+                        double wt2 = w4[0] + w4[1] + w4[2] + w4[3] - 2 * mn;
+                        double w_total = wqp + wt1 + wt2;
+                        double newT = 2 / std::sqrt(w_total);
+                        if ((newT < best[p[2]]) && (newT > 0.0)) best[p[2]] = newT;
+                    }
+        }
+    });
+    double dt = 1e200;
+    for (double b : best)
+        if (b < dt) dt = b;
+    return dt;
+}
+
+double emsop::timestep_var1() const
+{
+    const std::vector<float>& C = ec[0];
+    const std::vector<float>& L = ec[2];
+    double dt = 1e200;
+    unsigned p[3];
+    for (int n = 0; n < 3; ++n) {
+        const int nP = (n + 1) % 3, nPP = (n + 2) % 3;
+        for (p[2] = 0; p[2] < N[2]; ++p[2])
+            for (p[1] = 0; p[1] < N[1]; ++p[1])
+                for (p[0] = 0; p[0] < N[0]; ++p[0]) {
+                    unsigned qPM[3] = {p[0], p[1], p[2]}, qPPM[3] = {p[0], p[1], p[2]};
+                    if (qPM[nP] > 0) --qPM[nP];
+                    if (qPPM[nPP] > 0) --qPPM[nPP];
+                    float s = (4 / L[idx(nP, p[0], p[1], p[2])] + 4 / L[idx(nP, qPPM[0], qPPM[1], qPPM[2])] +
+                               4 / L[idx(nPP, p[0], p[1], p[2])] + 4 / L[idx(nPP, qPM[0], qPM[1], qPM[2])]) /
+                              C[idx(n, p[0], p[1], p[2])];
+                    double newT = 2 / std::sqrt((double)s);
+                    if ((newT < dt) && (newT > 0.0)) dt = newT;
+                }
+    }
+    return dt;
+}
+
+// Calc_ECOperatorPos (operator.cpp:956-984)
+void emsop::ec_to_coeff(int n, size_t i)
+{
+    (void)n;
+    double C = ec[0][i], G = ec[1][i];
+    if (C > 0) {
+        coeff[0][i] = (float)((1.0 - dT * G / 2.0 / C) / (1.0 + dT * G / 2.0 / C));
+        coeff[1][i] = (float)((dT / C) / (1.0 + dT * G / 2.0 / C));
+    } else {
+        coeff[0][i] = 0;
+        coeff[1][i] = 0;
+    }
+    double L = ec[2][i], R = ec[3][i];
+    if (L > 0) {
+        coeff[2][i] = (float)((1.0 - dT * R / 2.0 / L) / (1.0 + dT * R / 2.0 / L));
+        coeff[3][i] = (float)((dT / L) / (1.0 + dT * R / 2.0 / L));
+    } else {
+        coeff[2][i] = 0;
+        coeff[3][i] = 0;
+    }
+}
+
+void emsop::apply_bc()
+{
+    // ApplyElectricBC with PEC except BC == -1 (operator.cpp:1042-1048, 1099-1137)
+    auto& vv = coeff[0];
+    auto& vi = coeff[1];
+    for (int n = 0; n < 3; ++n) {
+        const int nP = (n + 1) % 3, nPP = (n + 2) % 3;
+        unsigned p[3];
+        for (p[nP] = 0; p[nP] < N[nP]; ++p[nP])
+            for (p[nPP] = 0; p[nPP] < N[nPP]; ++p[nPP]) {
+                if (bc[2 * n] != -1) {
+                    p[n] = 0;
+                    for (int c : {nP, nPP}) {
+                        vv[idx(c, p[0], p[1], p[2])] = 0;
+                        vi[idx(c, p[0], p[1], p[2])] = 0;
+                    }
+                }
+                if (bc[2 * n + 1] != -1) {
+                    p[n] = N[n] - 1;
+                    for (int c : {n, nP, nPP}) {
+                        vv[idx(c, p[0], p[1], p[2])] = 0;
+                        vi[idx(c, p[0], p[1], p[2])] = 0;
+                    }
+                }
+            }
+    }
+}
+
+void emsop::calc_pec()
+{
+    // CalcPEC_Range (operator.cpp:2045-2083): MATERIAL|METAL lookup at the Yee edge centre
+    bool any_metal = false;
+    for (const auto& m : mats) any_metal |= m.metal;
+    if (!any_metal) return;
+    Ranked<MatBox> rk;
+    rk.init(mats, [](const MatBox&) { return true; });
+    const unsigned W = rk.W;
+    for (int n = 0; n < 3; ++n) {
+        std::vector<uint64_t> m[3];
+        for (int a = 0; a < 3; ++a) m[a] = rk.masks(a, a == n ? dualL[a] : lines[a]);
+        for (unsigned x = 0; x < N[0]; ++x)
+            for (unsigned y = 0; y < N[1]; ++y)
+                for (unsigned z = 0; z < N[2]; ++z) {
+                    int r = rk.winner(&m[0][(size_t)x * W], &m[1][(size_t)y * W], &m[2][(size_t)z * W]);
+                    if (r >= 0 && rk.box[r]->metal) {
+                        coeff[0][idx(n, x, y, z)] = 0;
+                        coeff[1][idx(n, x, y, z)] = 0;
+                    }
+                }
+    }
+}
+
+int emsop::lumped_rc()
+{
+    // Calc_LumpedElements (operator.cpp:1586-1763) for parallel-RC elements
+    for (const auto& le : lumped) {
+        if (!le.parallel || le.L > 0) continue;
+        double Cap = le.C, R = le.R;
+        if (Cap <= 0) Cap = NAN;
+        if (R < 0) R = NAN;
+        if (std::isnan(R) && std::isnan(Cap)) continue;
+        const int ny = le.dir;
+        if (ny < 0 || ny > 2) continue;
+        const int nyP = (ny + 1) % 3, nyPP = (ny + 2) % 3;
+        unsigned s[3], e[3];
+        int dim = snap_box(le.lo, le.hi, false, s, e, nullptr, nullptr);
+        if (dim <= 0) continue;
+        if (s[ny] == e[ny]) continue;
+        unsigned p[3];
+        double unitGC = 0;
+        for (p[ny] = s[ny]; p[ny] < e[ny]; ++p[ny]) {
+            double plane = 0;
+            for (p[nyP] = s[nyP]; p[nyP] <= e[nyP]; ++p[nyP])
+                for (p[nyPP] = s[nyPP]; p[nyPP] <= e[nyPP]; ++p[nyPP])
+                    plane += EL(nyP, p[nyP], true) * EL(nyPP, p[nyPP], true) / EL(ny, p[ny], false);
+            unitGC += 1 / plane;
+        }
+        unitGC = 1 / unitGC;
+        double kappa = 0, eps = 0;
+        if (R > 0) kappa = 1 / R / unitGC;
+        if (Cap > 0) {
+            eps = Cap / unitGC;
+            if (eps < kEPS0) Cap = 0;
+        }
+        for (p[ny] = s[ny]; p[ny] < e[ny]; ++p[ny])
+            for (p[nyP] = s[nyP]; p[nyP] <= e[nyP]; ++p[nyP])
+                for (p[nyPP] = s[nyPP]; p[nyPP] <= e[nyPP]; ++p[nyPP]) {
+                    size_t q = idx(ny, p[0], p[1], p[2]);
+                    double A = EL(nyP, p[nyP], true) * EL(nyPP, p[nyPP], true);
+                    if (Cap > 0) ec[0][q] = (float)(eps * A / EL(ny, p[ny], false));
+                    if (R > 0) ec[1][q] = (float)(kappa * A / EL(ny, p[ny], false));
+                    if (R == 0) {
+                        coeff[0][q] = 0;
+                        coeff[1][q] = 0;
+                    } else
+                        ec_to_coeff(ny, q);
+                }
+        if (le.caps) {
+            for (p[nyP] = s[nyP]; p[nyP] <= e[nyP]; ++p[nyP])
+                for (p[nyPP] = s[nyPP]; p[nyPP] <= e[nyPP]; ++p[nyPP])
+                    for (unsigned pl : {s[ny], e[ny]}) {
+                        p[ny] = pl;
+                        if (p[nyP] < e[nyP]) {
+                            coeff[0][idx(nyP, p[0], p[1], p[2])] = 0;
+                            coeff[1][idx(nyP, p[0], p[1], p[2])] = 0;
+                        }
+                        if (p[nyPP] < e[nyPP]) {
+                            coeff[0][idx(nyPP, p[0], p[1], p[2])] = 0;
+                            coeff[1][idx(nyPP, p[0], p[1], p[2])] = 0;
+                        }
+                    }
+        }
+    }
+    return 0;
+}
+
+void emsop_apply_magnetic_bc(emsop* h)
+{
+    // ApplyMagneticBC (operator.cpp:1139-1187): PMC where BC == 1, last current line always zero
+    auto& ii = h->coeff[2];
+    auto& iv = h->coeff[3];
+    const unsigned* N = h->N;
+    for (int n = 0; n < 3; ++n) {
+        const int nP = (n + 1) % 3, nPP = (n + 2) % 3;
+        unsigned p[3];
+        for (p[nP] = 0; p[nP] < N[nP]; ++p[nP])
+            for (p[nPP] = 0; p[nPP] < N[nPP]; ++p[nPP]) {
+                if (h->bc[2 * n] == 1) {
+                    p[n] = 0;
+                    for (int c : {n, nP, nPP}) {
+                        ii[h->idx(c, p[0], p[1], p[2])] = 0;
+                        iv[h->idx(c, p[0], p[1], p[2])] = 0;
+                    }
+                }
+                if (h->bc[2 * n + 1] == 1) {
+                    p[n] = N[n] - 2;
+                    for (int c : {nP, nPP}) {
+                        ii[h->idx(c, p[0], p[1], p[2])] = 0;
+                        iv[h->idx(c, p[0], p[1], p[2])] = 0;
+                    }
+                }
+                p[n] = N[n] - 1;
+                for (int c : {n, nP, nPP}) {
+                    ii[h->idx(c, p[0], p[1], p[2])] = 0;
+                    iv[h->idx(c, p[0], p[1], p[2])] = 0;
+                }
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// excitation extension (operator_ext_excitation.cpp:105-297), box primitives
+// ---------------------------------------------------------------------------
+int emsop::build_excitation()
+{
+    exc[0] = ExcList();
+    exc[1] = ExcList();
+    if (excs.empty()) return 0;
+    Ranked<ExcBox> rk;
+    rk.init(excs, [](const ExcBox&) { return true; });
+    const unsigned W = rk.W;
+    std::vector<uint64_t> mP[3], mD[3];
+    for (int a = 0; a < 3; ++a) {
+        mP[a] = rk.masks(a, lines[a]);
+        mD[a] = rk.masks(a, dualL[a]);
+    }
+    auto anyset = [&](const std::vector<uint64_t>& m, unsigned i) {
+        for (unsigned w = 0; w < W; ++w)
+            if (m[(size_t)i * W + w]) return true;
+        return false;
+    };
+    auto& vv = coeff[0];
+    auto& vi = coeff[1];
+    auto& ii = coeff[2];
+    auto& iv = coeff[3];
+    unsigned p[3];
+    for (p[2] = 0; p[2] < N[2]; ++p[2]) {
+        if (!anyset(mP[2], p[2]) && !anyset(mD[2], p[2])) continue;
+        for (p[1] = 0; p[1] < N[1]; ++p[1]) {
+            if (!anyset(mP[1], p[1]) && !anyset(mD[1], p[1])) continue;
+            for (p[0] = 0; p[0] < N[0]; ++p[0]) {
+                if (!anyset(mP[0], p[0]) && !anyset(mD[0], p[0])) continue;
+                // electric field excite
+                for (int n = 0; n < 3; ++n) {
+                    if (p[n] >= N[n] - 1) continue;  // GetYeeCoords(...,false) == false
+                    const std::vector<uint64_t>* m[3];
+                    for (int a = 0; a < 3; ++a) m[a] = (a == n) ? &mD[a] : &mP[a];
+                    int r = rk.winner(&(*m[0])[(size_t)p[0] * W], &(*m[1])[(size_t)p[1] * W], &(*m[2])[(size_t)p[2] * W]);
+                    if (r < 0) continue;
+                    const ExcBox* b = rk.box[r];
+                    if (b->type != 0 && b->type != 1) continue;
+                    double amp = b->val[n] * EL(n, p[n], false);
+                    if (amp != 0) {
+                        for (int a = 0; a < 3; ++a) exc[0].idx[a].push_back(p[a]);
+                        exc[0].dir.push_back((unsigned short)n);
+                        exc[0].amp.push_back((float)amp);
+                        exc[0].delay.push_back((unsigned)(b->delay / dT));
+                    }
+                    if (b->type == 1) {
+                        vv[idx(n, p[0], p[1], p[2])] = 0;
+                        vi[idx(n, p[0], p[1], p[2])] = 0;
+                    }
+                }
+                // magnetic field excite
+                for (int n = 0; n < 3; ++n) {
+                    if (p[0] >= N[0] - 1 || p[1] >= N[1] - 1 || p[2] >= N[2] - 1) continue;
+                    const std::vector<uint64_t>* m[3];
+                    for (int a = 0; a < 3; ++a) m[a] = (a == n) ? &mP[a] : &mD[a];
+                    int r = rk.winner(&(*m[0])[(size_t)p[0] * W], &(*m[1])[(size_t)p[1] * W], &(*m[2])[(size_t)p[2] * W]);
+                    if (r < 0) continue;
+                    const ExcBox* b = rk.box[r];
+                    if (b->type != 2 && b->type != 3) continue;
+                    double amp = b->val[n] * EL(n, p[n], true);
+                    if (amp != 0) {
+                        for (int a = 0; a < 3; ++a) exc[1].idx[a].push_back(p[a]);
+                        exc[1].dir.push_back((unsigned short)n);
+                        exc[1].amp.push_back((float)amp);
+                        exc[1].delay.push_back((unsigned)(b->delay / dT));
+                    }
+                    if (b->type == 3) {
+                        ii[idx(n, p[0], p[1], p[2])] = 0;
+                        iv[idx(n, p[0], p[1], p[2])] = 0;
+                    }
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Mur ABC (operator_ext_mur_abc.cpp:80-186)
+// ---------------------------------------------------------------------------
+void emsop::build_mur()
+{
+    mur.clear();
+    Ranked<MatBox> rk;
+    rk.init(mats, [](const MatBox& b) { return !b.metal; });
+    const unsigned W = rk.W;
+    for (int f = 0; f < 6; ++f) {
+        if (bc[f] != 2) continue;
+        MurFace F;
+        const int ny = f / 2;
+        const bool top = f % 2;
+        const int nyP = (ny + 1) % 3, nyPP = (ny + 2) % 3;
+        F.ny = ny;
+        F.line = top ? N[ny] - 1 : 0;
+        F.shift = top ? N[ny] - 2 : 1;
+        double delta = std::fabs(EL(ny, F.line, false));
+        std::vector<double> cn(1);
+        if (F.line == 0)
+            cn[0] = lines[ny][F.line] + delta / 2 / gd;
+        else
+            cn[0] = lines[ny][F.line] - delta / 2 / gd;
+        auto mN = rk.masks(ny, cn);
+        auto mA = rk.masks(nyP, lines[nyP]);
+        auto mB = rk.masks(nyPP, lines[nyPP]);
+        const unsigned na = N[nyP], nb = N[nyPP];
+        F.cP.assign((size_t)na * nb, 0);
+        F.cPP.assign((size_t)na * nb, 0);
+        for (unsigned i = 0; i < na; ++i)
+            for (unsigned j = 0; j < nb; ++j) {
+                const uint64_t* m3[3];
+                m3[ny] = &mN[0];
+                m3[nyP] = &mA[(size_t)i * W];
+                m3[nyPP] = &mB[(size_t)j * W];
+                int r = rk.winner(m3[0], m3[1], m3[2]);
+                double c0t;
+                size_t k = (size_t)i * nb + j;
+                if (r >= 0) {
+                    const MatBox* b = rk.box[r];
+                    double eps = b->eps[nyP], mue = b->mue[nyP];
+                    if (vph[f] > 0.0)
+                        c0t = vph[f] * dT;
+                    else
+                        c0t = kC0 * dT / std::sqrt(eps * mue);
+                    F.cP[k] = (float)((c0t - delta) / (c0t + delta));
+                    eps = b->eps[nyPP];
+                    mue = b->mue[nyPP];
+                    if (vph[f] > 0.0)
+                        c0t = vph[f] * dT;
+                    else
+                        c0t = kC0 * dT / std::sqrt(eps * mue);
+                    F.cPP[k] = (float)((c0t - delta) / (c0t + delta));
+                } else {
+                    if (vph[f] > 0.0)
+                        c0t = vph[f] * dT;
+                    else
+                        c0t = kC0 / std::sqrt(bg[0] * bg[1]) * dT;
+                    F.cP[k] = (float)((c0t - delta) / (c0t + delta));
+                    F.cPP[k] = F.cP[k];
+                }
+            }
+        // Engine_Ext_Mur_ABC: start late when a voltage excitation lies on the face
+        int maxDelay = -1;
+        for (size_t q = 0; q < exc[0].dir.size(); ++q)
+            if ((exc[0].dir[q] == nyP || exc[0].dir[q] == nyPP) && exc[0].idx[ny][q] == F.line)
+                if ((int)exc[0].delay[q] > maxDelay) maxDelay = (int)exc[0].delay[q];
+        F.start_ts = (maxDelay >= 0) ? (unsigned)(maxDelay + (int)sigLen + 10) : 0;
+        mur.push_back(std::move(F));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// UPML (operator_ext_upml.cpp:69-445)
+// ---------------------------------------------------------------------------
+static double upml_grading(double D, double dl, double Wd, double Z)
+{
+    // default grading function (operator_ext_upml.cpp:30) written out left to right
+    return -std::log(1e-6) * std::log(2.5) / (2 * dl * Z * (std::pow(2.5, Wd / dl) - 1)) * std::pow(2.5, D / dl);
+}
+
+void emsop::grading(int n, int ny, unsigned p, const int* bcs, const unsigned* size, double& kv, double& ki) const
+{
+    // one axis of CalcGradingKappa (operator_ext_upml.cpp:269-343)
+    kv = ki = 0;
+    const auto& L = lines[n];
+    if (p <= size[2 * n] && bcs[2 * n] == 3) {
+        double width = (L[size[2 * n]] - L[0]) * gd;
+        double depth = width - (L[p] - L[0]) * gd;
+        double el = EL(n, p, false);
+        if (n == ny) depth -= el / 2;
+        double dl = width / size[2 * n];
+        kv = depth > 0 ? upml_grading(depth, dl, width, kZ0) : 0;
+        if (n == ny) depth += el / 2;
+        if (n != ny) depth -= el / 2;
+        if (depth < 0) depth = 0;
+        ki = depth > 0 ? upml_grading(depth, dl, width, kZ0) : 0;
+    } else if (p >= N[n] - 1 - size[2 * n + 1] && bcs[2 * n + 1] == 3) {
+        double width = (L[N[n] - 1] - L[N[n] - size[2 * n + 1] - 1]) * gd;
+        double depth = width - (L[N[n] - 1] - L[p]) * gd;
+        double el = EL(n, p, false);
+        if (n == ny) depth += el / 2;
+        double dl = width / (size[2 * n]);  // the reference divides by the lower size here too
+        kv = depth > 0 ? upml_grading(depth, dl, width, kZ0) : 0;
+        if (n == ny) depth -= el / 2;
+        if (n != ny) depth += el / 2;
+        if (depth > width) depth = 0;
+        ki = depth > 0 ? upml_grading(depth, dl, width, kZ0) : 0;
+    }
+}
+
+void emsop::build_upml()
+{
+    upml.clear();
+    int B[6];
+    unsigned S[6];
+    for (int k = 0; k < 6; ++k) {
+        B[k] = bc[k];
+        S[k] = pml[k];
+    }
+    for (int n = 0; n < 3; ++n)
+        if (S[2 * n] * (B[2 * n] == 3) + S[2 * n + 1] * (B[2 * n + 1] == 3) >= N[n]) {
+            B[2 * n] = B[2 * n + 1] = 0;
+            S[2 * n] = S[2 * n + 1] = 0;
+        }
+    std::vector<std::pair<std::array<unsigned, 3>, std::array<unsigned, 3>>> ranges;
+    unsigned st[3] = {0, 0, 0}, sp[3] = {N[0] - 1, N[1] - 1, N[2] - 1};
+    auto push = [&]() { ranges.push_back({{st[0], st[1], st[2]}, {sp[0], sp[1], sp[2]}}); };
+    if (B[0] == 3) { st[0] = 0; sp[0] = S[0]; push(); }
+    if (B[1] == 3) { st[0] = N[0] - 1 - S[1]; sp[0] = N[0] - 1; push(); }
+    st[0] = (S[0] + 1) * (B[0] == 3);
+    sp[0] = N[0] - 1 - (S[0] + 1) * (B[1] == 3);
+    if (B[2] == 3) { st[1] = 0; sp[1] = S[2]; push(); }
+    if (B[3] == 3) { st[1] = N[1] - 1 - S[3]; sp[1] = N[1] - 1; push(); }
+    st[1] = (S[2] + 1) * (B[2] == 3);
+    sp[1] = N[1] - 1 - (S[3] + 1) * (B[3] == 3);
+    if (B[4] == 3) { st[2] = 0; sp[2] = S[4]; push(); }
+    if (B[5] == 3) { st[2] = N[2] - 1 - S[5]; sp[2] = N[2] - 1; push(); }
+
+    const double E2 = 2 * kEPS0;
+    auto& vv = coeff[0];
+    auto& vi = coeff[1];
+    auto& ii = coeff[2];
+    auto& iv = coeff[3];
+    for (auto& r : ranges) {
+        Slab s;
+        for (int k = 0; k < 3; ++k) {
+            s.start[k] = r.first[k];
+            s.num[k] = r.second[k] - r.first[k] + 1;
+        }
+        const size_t tot = (size_t)3 * s.num[0] * s.num[1] * s.num[2];
+        for (int k = 0; k < 6; ++k) s.a[k].assign(tot, 0.0f);
+        // per-axis grading tables: g[ny][axis][local] = (kv, ki)
+        std::vector<double> gv[3][3], gi[3][3];
+        for (int ny = 0; ny < 3; ++ny)
+            for (int a = 0; a < 3; ++a) {
+                gv[ny][a].resize(s.num[a]);
+                gi[ny][a].resize(s.num[a]);
+                for (unsigned l = 0; l < s.num[a]; ++l) grading(a, ny, l + s.start[a], B, S, gv[ny][a][l], gi[ny][a][l]);
+            }
+        auto li = [&](int n, unsigned x, unsigned y, unsigned z) {
+            return (((size_t)n * s.num[0] + x) * s.num[1] + y) * s.num[2] + z;
+        };
+        parallel_for(s.num[0], nthreads, [&](unsigned x0, unsigned x1) {
+            unsigned loc[3], p[3];
+            for (loc[0] = x0; loc[0] < x1; ++loc[0])
+                for (loc[1] = 0; loc[1] < s.num[1]; ++loc[1])
+                    for (loc[2] = 0; loc[2] < s.num[2]; ++loc[2]) {
+                        for (int k = 0; k < 3; ++k) p[k] = loc[k] + s.start[k];
+                        for (int n = 0; n < 3; ++n) {
+                            const int nP = (n + 1) % 3, nPP = (n + 2) % 3;
+                            double E[4];
+                            effmat_at(n, p, E);
+                            double kv[3], ki[3];
+                            for (int a = 0; a < 3; ++a) {
+                                kv[a] = gv[n][a][loc[a]];
+                                ki[a] = gi[n][a][loc[a]];
+                            }
+                            const size_t q = idx(n, p[0], p[1], p[2]);
+                            const size_t l = li(n, loc[0], loc[1], loc[2]);
+                            if (((kv[0] + kv[1] + kv[2]) != 0) && (E[1] < 1e3)) {
+                                if ((vv[q] + vi[q]) != 0) {
+                                    vv[q] = (float)((E2 - kv[nP] * dT) / (E2 + kv[nP] * dT));
+                                    vi[q] = (float)((E2 * dT) / (E2 + kv[nP] * dT) * EL(n, p[n], false) /
+                                                    (EL(nP, p[nP], true) * EL(nPP, p[nPP], true)));
+                                    s.a[0][l] = (float)((E2 - kv[nPP] * dT) / (E2 + kv[nPP] * dT));
+                                    s.a[2][l] = (float)((E2 + kv[n] * dT) / (E2 + kv[nPP] * dT) / E[0]);
+                                    s.a[1][l] = (float)((E2 - kv[n] * dT) / (E2 + kv[nPP] * dT) / E[0]);
+                                }
+                            } else {
+                                s.a[0][l] = vv[q];
+                                vv[q] = 0;
+                                s.a[1][l] = 0;
+                                s.a[2][l] = 1;
+                            }
+                            if ((ki[0] + ki[1] + ki[2]) != 0) {
+                                if ((ii[q] + iv[q]) != 0) {
+                                    ii[q] = (float)((E2 - ki[nP] * dT) / (E2 + ki[nP] * dT));
+                                    iv[q] = (float)((E2 * dT) / (E2 + ki[nP] * dT) * EL(n, p[n], true) /
+                                                    (EL(nP, p[nP], false) * EL(nPP, p[nPP], false)));
+                                    s.a[3][l] = (float)((E2 - ki[nPP] * dT) / (E2 + ki[nPP] * dT));
+                                    s.a[5][l] = (float)((E2 + ki[n] * dT) / (E2 + ki[nPP] * dT) / E[2]);
+                                    s.a[4][l] = (float)((E2 - ki[n] * dT) / (E2 + ki[nPP] * dT) / E[2]);
+                                }
+                            } else {
+                                s.a[3][l] = ii[q];
+                                ii[q] = 0;
+                                s.a[4][l] = 0;
+                                s.a[5][l] = 1;
+                            }
+                        }
+                    }
+        });
+        upml.push_back(std::move(s));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// excitation signal (FDTD/excitation.cpp:150-276)
+// ---------------------------------------------------------------------------
+static unsigned nyquist_num(double fmax, double dT)
+{
+    // CalcNyquistNum (tools/useful.cpp:30-36)
+    if (fmax == 0) return 0xFFFFFFFFu;
+    if (dT == 0) return 1;
+    double T0 = 1 / fmax;
+    return (unsigned)std::floor(T0 / 2 / dT);
+}
+
+int emsop::build_signal(unsigned nrts)
+{
+    sigV.clear();
+    sigC.clear();
+    switch (sigType) {
+    case EMSOP_SIG_GAUSS: {
+        sigLen = (unsigned)std::ceil(2.0 * 9.0 / (2.0 * kPI * fc) / dT);
+        if (sigLen > nrts) sigLen = nrts;
+        sigV.resize(sigLen);
+        sigC.resize(sigLen);
+        for (unsigned n = 0; n < sigLen; ++n) {
+            double t = n * dT;
+            sigV[n] = (float)(std::cos(2.0 * kPI * f0 * (t - 9.0 / (2.0 * kPI * fc))) * std::exp(-1 * std::pow(2.0 * kPI * fc * t / 3.0 - 3, 2)));
+            t += 0.5 * dT;
+            sigC[n] = (float)(std::cos(2.0 * kPI * f0 * (t - 9.0 / (2.0 * kPI * fc))) * std::exp(-1 * std::pow(2.0 * kPI * fc * t / 3.0 - 3, 2)));
+        }
+        nyquist = nyquist_num(f0 + fc, dT);
+        break;
+    }
+    case EMSOP_SIG_SINUS: {
+        sigLen = (unsigned)std::round(2.0 / f0 / dT);
+        sigV.assign(sigLen, 0.0f);
+        sigC.assign(sigLen, 0.0f);
+        for (unsigned n = 1; n < sigLen; ++n) {
+            double t = n * dT;
+            sigV[n] = (float)std::sin(2.0 * kPI * f0 * t);
+            t += 0.5 * dT;
+            sigC[n] = (float)std::sin(2.0 * kPI * f0 * t);
+        }
+        nyquist = nyquist_num(f0, dT);
+        break;
+    }
+    case EMSOP_SIG_DIRAC:
+        sigLen = 2;
+        sigV = {0.0f, 1.0f};
+        sigC = {0.0f, 1.0f};
+        nyquist = 1;
+        break;
+    case EMSOP_SIG_STEP:
+        sigLen = 2;
+        sigV = {1.0f, 1.0f};
+        sigC = {1.0f, 1.0f};
+        nyquist = 1;
+        break;
+    default:
+        return fail(EMSOP_E_STATE, "excitation signal not defined");
+    }
+    if (nyquist == 0) return fail(EMSOP_E_NUMERIC, "excitation setup failed (Nyquist number 0)");
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Operator::CalcECOperator (operator.cpp:986-1097)
+// ---------------------------------------------------------------------------
+int emsop::build(unsigned nrts)
+{
+    for (int n = 0; n < 3; ++n)
+        if (N[n] < 3) return fail(EMSOP_E_ARG, "at least 3 mesh lines per direction are required");
+    if (sigType < 0) return fail(EMSOP_E_STATE, "excitation signal not defined");
+    mesh_tables();
+    try {
+        int rc = calc_ec();
+        if (rc) return rc;
+        double dt_auto = (tsMethod == 3) ? timestep_var3() : timestep_var1();
+        if (forcedDt > 0)
+            dT = forcedDt;
+        else
+            dT = dt_auto;
+        if (dT == 0 || dT >= 1e200) return fail(EMSOP_E_NUMERIC, "timestep is zero");
+        dT *= tsFactor;
+        period = (sigType == EMSOP_SIG_SINUS) ? 1 / f0 : 0;
+        if (period > 0) {
+            unsigned TS = (unsigned)std::ceil(period / dT);
+            dT = period / TS;
+        }
+        const size_t tot = 3 * size();
+        for (int k = 0; k < 4; ++k) coeff[k].assign(tot, 0.0f);
+        parallel_for(3 * N[0], nthreads, [&](unsigned a, unsigned b) {
+            for (unsigned nx = a; nx < b; ++nx) {
+                int n = nx / N[0];
+                unsigned x = nx % N[0];
+                for (unsigned y = 0; y < N[1]; ++y)
+                    for (unsigned z = 0; z < N[2]; ++z) ec_to_coeff(n, idx(n, x, y, z));
+            }
+        });
+        apply_bc();
+        calc_pec();
+        lumped_rc();
+        emsop_apply_magnetic_bc(this);
+        rc = build_signal(nrts);
+        if (rc) return rc;
+        // extensions in the reference's creation order (openems.cpp:1185-1243)
+        build_excitation();
+        build_mur();
+        build_upml();
+    } catch (const std::bad_alloc&) {
+        return fail(EMSOP_E_NOMEM, "out of host memory while building the operator");
+    }
+    built = true;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* emsop_last_error(void) { return g_err.c_str(); }
+
+int emsop_create(emsop** out)
+{
+    if (!out) return fail(EMSOP_E_ARG, "null output pointer");
+    try {
+        *out = new emsop();
+    } catch (const std::bad_alloc&) {
+        return fail(EMSOP_E_NOMEM, "out of memory");
+    }
+    return 0;
+}
+
+void emsop_destroy(emsop* h) { delete h; }
+
+int emsop_set_grid(emsop* h, const double* x, unsigned nx, const double* y, unsigned ny, const double* z, unsigned nz,
+                   double delta_unit)
+{
+    if (!h || !x || !y || !z) return fail(EMSOP_E_ARG, "null argument");
+    if (nx < 3 || ny < 3 || nz < 3) return fail(EMSOP_E_ARG, "at least 3 mesh lines per direction are required");
+    if (delta_unit <= 0) return fail(EMSOP_E_ARG, "grid delta unit must be > 0");
+    const double* src[3] = {x, y, z};
+    unsigned cnt[3] = {nx, ny, nz};
+    for (int n = 0; n < 3; ++n) {
+        h->lines[n].assign(src[n], src[n] + cnt[n]);
+        for (unsigned i = 1; i < cnt[n]; ++i)
+            if (!(h->lines[n][i] > h->lines[n][i - 1])) return fail(EMSOP_E_ARG, "mesh lines must be strictly increasing");
+        h->N[n] = cnt[n];
+    }
+    h->gd = delta_unit;
+    h->built = false;
+    return 0;
+}
+
+int emsop_set_background(emsop* h, double eps_r, double mue_r, double kappa, double sigma)
+{
+    if (!h) return fail(EMSOP_E_ARG, "null handle");
+    h->bg[0] = eps_r;
+    h->bg[1] = mue_r;
+    h->bg[2] = kappa;
+    h->bg[3] = sigma;
+    return 0;
+}
+
+static int next_order(emsop* h) { return (int)(h->mats.size() + h->lumped.size() + h->excs.size()); }
+
+int emsop_add_material(emsop* h, const double eps[3], const double mue[3], const double kappa[3], const double sigma[3],
+                       int priority, const double start[3], const double stop[3])
+{
+    if (!h || !eps || !mue || !kappa || !sigma || !start || !stop) return fail(EMSOP_E_ARG, "null argument");
+    MatBox b;
+    for (int n = 0; n < 3; ++n) {
+        b.lo[n] = start[n];
+        b.hi[n] = stop[n];
+        b.eps[n] = eps[n];
+        b.mue[n] = mue[n];
+        b.kappa[n] = kappa[n];
+        b.sigma[n] = sigma[n];
+        if (!(mue[n] > 0) || !(eps[n] > 0)) return fail(EMSOP_E_ARG, "eps_r and mue_r must be > 0");
+    }
+    b.prio = priority;
+    b.order = next_order(h);
+    b.metal = false;
+    h->mats.push_back(b);
+    return 0;
+}
+
+int emsop_add_metal(emsop* h, int priority, const double start[3], const double stop[3])
+{
+    if (!h || !start || !stop) return fail(EMSOP_E_ARG, "null argument");
+    MatBox b;
+    for (int n = 0; n < 3; ++n) {
+        b.lo[n] = start[n];
+        b.hi[n] = stop[n];
+        b.eps[n] = b.mue[n] = 1;
+        b.kappa[n] = b.sigma[n] = 0;
+    }
+    b.prio = priority;
+    b.order = next_order(h);
+    b.metal = true;
+    h->mats.push_back(b);
+    return 0;
+}
+
+int emsop_add_lumped(emsop* h, int dir, double R, double C, double L, int caps, int parallel, int priority,
+                     const double start[3], const double stop[3])
+{
+    if (!h || !start || !stop) return fail(EMSOP_E_ARG, "null argument");
+    if (dir < 0 || dir > 2) return fail(EMSOP_E_ARG, "lumped element direction must be 0, 1 or 2");
+    LumpedBox b;
+    for (int n = 0; n < 3; ++n) {
+        b.lo[n] = start[n];
+        b.hi[n] = stop[n];
+    }
+    b.dir = dir;
+    b.R = R;
+    b.C = C;
+    b.L = L;
+    b.caps = caps != 0;
+    b.parallel = parallel != 0;
+    b.prio = priority;
+    b.order = next_order(h);
+    h->lumped.push_back(b);
+    return 0;
+}
+
+int emsop_add_excitation(emsop* h, int type, const double val[3], double delay, int priority, const double start[3],
+                         const double stop[3])
+{
+    if (!h || !val || !start || !stop) return fail(EMSOP_E_ARG, "null argument");
+    if (type < 0 || type > 3) return fail(EMSOP_E_ARG, "excitation type must be 0..3");
+    ExcBox b;
+    for (int n = 0; n < 3; ++n) {
+        b.lo[n] = start[n];
+        b.hi[n] = stop[n];
+        b.val[n] = val[n];
+    }
+    b.type = type;
+    b.delay = delay;
+    b.prio = priority;
+    b.order = next_order(h);
+    h->excs.push_back(b);
+    return 0;
+}
+
+int emsop_set_boundary(emsop* h, const int bc[6], const unsigned pml_size[6], const double mur_vph[6])
+{
+    if (!h || !bc) return fail(EMSOP_E_ARG, "null argument");
+    for (int k = 0; k < 6; ++k) {
+        if (bc[k] < -1 || bc[k] > 3) return fail(EMSOP_E_ARG, "boundary type must be -1..3");
+        h->bc[k] = bc[k];
+        if (pml_size) h->pml[k] = pml_size[k];
+        if (mur_vph) h->vph[k] = mur_vph[k];
+    }
+    return 0;
+}
+
+int emsop_set_signal(emsop* h, int type, double f0, double fc)
+{
+    if (!h) return fail(EMSOP_E_ARG, "null handle");
+    if (type < 0 || type > 3) return fail(EMSOP_E_ARG, "unknown signal type");
+    if (type == EMSOP_SIG_GAUSS && !(fc > 0)) return fail(EMSOP_E_ARG, "gaussian pulse needs fc > 0");
+    if (type == EMSOP_SIG_SINUS && !(f0 > 0)) return fail(EMSOP_E_ARG, "sinusoid needs f0 > 0");
+    h->sigType = type;
+    h->f0 = f0;
+    h->fc = fc;
+    return 0;
+}
+
+int emsop_set_timestep(emsop* h, int method, double forced_dt, double factor)
+{
+    if (!h) return fail(EMSOP_E_ARG, "null handle");
+    if (method != 1 && method != 3) return fail(EMSOP_E_ARG, "timestep method must be 1 or 3");
+    h->tsMethod = method;
+    h->forcedDt = forced_dt;
+    if (factor > 0 && factor <= 1) h->tsFactor = factor;  // Operator::SetTimestepFactor ignores others
+    return 0;
+}
+
+int emsop_set_cell_constant_material(emsop* h, int enable)
+{
+    if (!h) return fail(EMSOP_E_ARG, "null handle");
+    h->cellConst = enable != 0;
+    return 0;
+}
+
+int emsop_set_threads(emsop* h, int nthreads)
+{
+    if (!h) return fail(EMSOP_E_ARG, "null handle");
+    h->nthreads = nthreads;
+    return 0;
+}
+
+int emsop_build(emsop* h, unsigned nrts)
+{
+    if (!h) return fail(EMSOP_E_ARG, "null handle");
+    if (nrts == 0) return fail(EMSOP_E_ARG, "number of timesteps must be > 0");
+    return h->build(nrts);
+}
+
+int emsop_num_lines(const emsop* h, unsigned out[3])
+{
+    if (!h || !out) return fail(EMSOP_E_ARG, "null argument");
+    for (int n = 0; n < 3; ++n) out[n] = h->N[n];
+    return 0;
+}
+
+double emsop_timestep(const emsop* h) { return h ? h->dT : 0; }
+unsigned emsop_nyquist(const emsop* h) { return h ? h->nyquist : 0; }
+double emsop_signal_period(const emsop* h) { return h ? h->period : 0; }
+unsigned emsop_signal_length(const emsop* h) { return h ? h->sigLen : 0; }
+int emsop_signal_period_steps(const emsop* h) { return (h && h->period > 0) ? (int)(h->period / h->dT) : 0; }
+const float* emsop_signal_volt(const emsop* h) { return (h && h->built) ? h->sigV.data() : nullptr; }
+const float* emsop_signal_curr(const emsop* h) { return (h && h->built) ? h->sigC.data() : nullptr; }
+
+const float* emsop_coeff(const emsop* h, int which)
+{
+    if (!h || !h->built || which < 0 || which > 3 || h->coeff[which].empty()) return nullptr;
+    return h->coeff[which].data();
+}
+
+const float* emsop_ec(const emsop* h, int which)
+{
+    if (!h || !h->built || which < 0 || which > 3 || h->ec[which].empty()) return nullptr;
+    return h->ec[which].data();
+}
+
+int emsop_effmat_at(const emsop* h, int ny, const unsigned pos[3], double out[4])
+{
+    if (!h || !h->built || ny < 0 || ny > 2 || !pos || !out) return fail(EMSOP_E_ARG, "bad argument");
+    for (int n = 0; n < 3; ++n)
+        if (pos[n] >= h->N[n]) return fail(EMSOP_E_ARG, "position outside the mesh");
+    h->effmat_at(ny, pos, out);
+    return 0;
+}
+
+unsigned emsop_exc_count(const emsop* h, int curr)
+{
+    if (!h || !h->built) return 0;
+    return (unsigned)h->exc[curr ? 1 : 0].dir.size();
+}
+
+int emsop_exc_get(const emsop* h, int curr, unsigned* idx, unsigned short* dir, float* amp, unsigned* delay)
+{
+    if (!h || !h->built) return fail(EMSOP_E_STATE, "operator not built");
+    const ExcList& e = h->exc[curr ? 1 : 0];
+    const size_t K = e.dir.size();
+    for (size_t k = 0; k < K; ++k) {
+        if (idx)
+            for (int a = 0; a < 3; ++a) idx[a * K + k] = e.idx[a][k];
+        if (dir) dir[k] = e.dir[k];
+        if (amp) amp[k] = e.amp[k];
+        if (delay) delay[k] = e.delay[k];
+    }
+    return 0;
+}
+
+unsigned emsop_upml_count(const emsop* h) { return (h && h->built) ? (unsigned)h->upml.size() : 0; }
+
+int emsop_upml_get(const emsop* h, unsigned k, unsigned start[3], unsigned num[3], const float* arrays[6])
+{
+    if (!h || !h->built || k >= h->upml.size()) return fail(EMSOP_E_ARG, "bad UPML slab index");
+    const Slab& s = h->upml[k];
+    for (int n = 0; n < 3; ++n) {
+        if (start) start[n] = s.start[n];
+        if (num) num[n] = s.num[n];
+    }
+    if (arrays)
+        for (int a = 0; a < 6; ++a) arrays[a] = s.a[a].data();
+    return 0;
+}
+
+unsigned emsop_mur_count(const emsop* h) { return (h && h->built) ? (unsigned)h->mur.size() : 0; }
+
+int emsop_mur_get(const emsop* h, unsigned k, int* ny, unsigned* line, unsigned* line_shift, unsigned* start_ts,
+                  const float** cP, const float** cPP)
+{
+    if (!h || !h->built || k >= h->mur.size()) return fail(EMSOP_E_ARG, "bad Mur face index");
+    const MurFace& f = h->mur[k];
+    if (ny) *ny = f.ny;
+    if (line) *line = f.line;
+    if (line_shift) *line_shift = f.shift;
+    if (start_ts) *start_ts = f.start_ts;
+    if (cP) *cP = f.cP.data();
+    if (cPP) *cPP = f.cPP.data();
+    return 0;
+}
+
+double emsop_disc_line(const emsop* h, int n, unsigned pos, int dual)
+{
+    if (!h || n < 0 || n > 2 || pos >= h->N[n] || h->dualL[n].empty()) return 0;
+    return dual ? h->dualL[n][pos] : h->lines[n][pos];
+}
+
+double emsop_edge_length(const emsop* h, int n, unsigned pos, int dual)
+{
+    if (!h || n < 0 || n > 2 || pos >= h->N[n] || h->primD[n].empty()) return 0;
+    return h->EL(n, pos, dual != 0);
+}
+
+int emsop_snap_to_mesh(const emsop* h, const double coord[3], int dual, unsigned out[3], int inside[3])
+{
+    if (!h || !coord || !out || h->dualL[0].empty()) return fail(EMSOP_E_ARG, "bad argument or mesh not prepared");
+    int ok = 1;
+    for (int n = 0; n < 3; ++n) {
+        bool in;
+        out[n] = h->snap_line(n, coord[n], in, dual != 0);
+        if (inside) inside[n] = in;
+        ok &= in;
+    }
+    return ok;
+}
+
+int emsop_snap_box(const emsop* h, const double start[3], const double stop[3], int dual, unsigned us[3], unsigned ue[3],
+                   int start_in[3], int stop_in[3])
+{
+    if (!h || !start || !stop || !us || !ue || h->dualL[0].empty()) return fail(EMSOP_E_ARG, "bad argument or mesh not prepared");
+    bool si[3] = {false, false, false}, ei[3] = {false, false, false};
+    int dim = h->snap_box(start, stop, dual != 0, us, ue, si, ei);
+    for (int n = 0; n < 3; ++n) {
+        if (start_in) start_in[n] = si[n];
+        if (stop_in) stop_in[n] = ei[n];
+    }
+    return dim;
+}
+
+void emsop_release_build_arrays(emsop* h)
+{
+    if (!h) return;
+    for (int k = 0; k < 4; ++k) {
+        std::vector<float>().swap(h->ec[k]);
+    }
+}
+
+}  // extern "C"

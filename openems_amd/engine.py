@@ -1,0 +1,166 @@
+"""Engine_HIP: the MI355X engine behind include/ems_hip.h.
+
+Mirrors the reference's Engine API (FDTD/engine.h:41-101): ``IterateTS``,
+``GetNumberOfTimesteps`` and bulk field access, with the engine extensions
+(excitation, UPML, Mur) created from an :class:`~openems_amd.operator.Operator`
+in the reference's creation order.  There is no CPU path: without a GPU or
+without the native library construction raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._native import ProbeDesc, check, dp, fp, lib, ptr
+
+
+class EngineHIP:
+    def __init__(self, op, device=0):
+        L = lib()
+        if L.ems_device_count() <= 0:
+            raise RuntimeError("EngineHIP needs a HIP device; none is visible")
+        self.op = op
+        self.N = op.N
+        h = C.c_void_p()
+        n = (C.c_uint * 3)(*op.N)
+        check(L.ems_create(C.byref(h), n, op.coeff_ptr("vv"), op.coeff_ptr("vi"), op.coeff_ptr("ii"),
+                           op.coeff_ptr("iv"), int(device)))
+        self.h = h
+        self._keep = []
+        try:
+            self._extensions()
+        except Exception:
+            self.close()
+            raise
+
+    def _extensions(self):
+        L = lib()
+        op = self.op
+        sv, sc = op.signal()
+        ve, ce = op.excitation(False), op.excitation(True)
+        arrs = [sv, sc, ve["index"], ve["dir"], ve["amp"], ve["delay"], ce["index"], ce["dir"], ce["amp"], ce["delay"]]
+        arrs = [np.ascontiguousarray(a) for a in arrs]
+        self._keep += arrs
+        sv, sc, vi, vd, va, vdl, ci, cd, ca, cdl = arrs
+        check(L.ems_set_excitation(self.h, ptr(sv, C.c_float), ptr(sc, C.c_float), len(sv), op.period_steps,
+                                   len(vd), ptr(vi, C.c_uint), ptr(vd, C.c_ushort), ptr(va, C.c_float),
+                                   ptr(vdl, C.c_uint), len(cd), ptr(ci, C.c_uint), ptr(cd, C.c_ushort),
+                                   ptr(ca, C.c_float), ptr(cdl, C.c_uint)))
+        for f in op.mur_faces():
+            cP = np.ascontiguousarray(f["coeff_nyP"], np.float32)
+            cPP = np.ascontiguousarray(f["coeff_nyPP"], np.float32)
+            check(L.ems_add_mur(self.h, f["ny"], f["line"], f["line_shift"], ptr(cP, C.c_float), ptr(cPP, C.c_float),
+                                f["start_ts"]))
+        for s in op.upml_slabs():
+            st = (C.c_uint * 3)(*s["start"])
+            nm = (C.c_uint * 3)(*s["num"])
+            check(L.ems_add_upml(self.h, st, nm, *s["_ptrs"]))
+        check(L.ems_init(self.h))
+
+    # -- Engine API ------------------------------------------------------------
+    def IterateTS(self, n):
+        check(lib().ems_iterate(self.h, int(n)))
+        return True
+
+    iterate = IterateTS
+
+    def GetNumberOfTimesteps(self):
+        return lib().ems_get_numts(self.h)
+
+    @property
+    def numTS(self):
+        return self.GetNumberOfTimesteps()
+
+    def get_volt(self):
+        out = np.empty((3,) + tuple(self.N), np.float32)
+        check(lib().ems_get_volt(self.h, ptr(out, C.c_float)))
+        return out
+
+    def get_curr(self):
+        out = np.empty((3,) + tuple(self.N), np.float32)
+        check(lib().ems_get_curr(self.h, ptr(out, C.c_float)))
+        return out
+
+    def set_volt(self, a):
+        a = np.ascontiguousarray(a, np.float32)
+        check(lib().ems_set_volt(self.h, ptr(a, C.c_float)))
+
+    def set_curr(self, a):
+        a = np.ascontiguousarray(a, np.float32)
+        check(lib().ems_set_curr(self.h, ptr(a, C.c_float)))
+
+    def energy(self):
+        e = C.c_double()
+        check(lib().ems_energy(self.h, C.byref(e)))
+        return e.value
+
+    def sync(self):
+        check(lib().ems_sync(self.h))
+
+    def set_async(self, flag):
+        check(lib().ems_set_async(self.h, int(flag)))
+
+    def set_profiling(self, flag):
+        check(lib().ems_set_profiling(self.h, int(flag)))
+
+    def last_timing(self):
+        t, v, c = C.c_double(), C.c_double(), C.c_double()
+        check(lib().ems_last_timing(self.h, C.byref(t), C.byref(v), C.byref(c)))
+        return t.value, v.value, c.value
+
+    # -- probes and dumps ---------------------------------------------------------
+    def add_probe(self, ptype, comp, pos, sign, divisor, interval, fd_interval, start_ts, stop_ts, capacity):
+        comp = np.ascontiguousarray(comp, np.uint32)
+        pos = np.ascontiguousarray(pos, np.uint32)
+        sign = np.ascontiguousarray(sign if sign is not None else np.ones(len(comp)), np.int32)
+        div = np.ascontiguousarray(divisor if divisor is not None else np.zeros(len(comp)), np.float64)
+        self._keep += [comp, pos, sign, div]
+        d = ProbeDesc(int(ptype), len(comp), ptr(comp, C.c_uint), ptr(pos, C.c_uint), ptr(sign, C.c_int),
+                      ptr(div, C.c_double), int(interval), int(fd_interval), int(start_ts), int(stop_ts),
+                      int(capacity))
+        pid = C.c_int()
+        check(lib().ems_add_probe(self.h, C.byref(d), C.byref(pid)))
+        return pid.value
+
+    def probe_results(self, pid):
+        L = lib()
+        cnt = C.c_uint()
+        check(L.ems_probe_results(self.h, pid, C.byref(cnt), None, None))
+        n = cnt.value
+        ts = np.zeros(n, np.uint32)
+        vals = np.zeros((n, 3), np.float64)
+        if n:
+            check(L.ems_probe_results(self.h, pid, C.byref(cnt), ptr(ts, C.c_uint), ptr(vals, C.c_double)))
+        return ts, vals
+
+    def add_fd_dump(self, start, stop, fd_interval, factors):
+        L = lib()
+        factors = np.ascontiguousarray(factors, np.float32)   # [nsamples][nfreq][2]
+        ns, nf = factors.shape[0], factors.shape[1]
+        els = [np.ascontiguousarray(self.op.edge_lengths(n, False), np.float64) for n in range(3)]
+        self._keep += [factors] + els
+        elp = (dp * 3)(*[ptr(e, C.c_double) for e in els])
+        st = (C.c_uint * 3)(*start)
+        sp = (C.c_uint * 3)(*stop)
+        did = C.c_int()
+        check(L.ems_add_fd_dump(self.h, 0, st, sp, int(fd_interval), nf, ns, ptr(factors, C.c_float), elp,
+                                C.byref(did)))
+        shape = (nf, 3) + tuple(int(stop[n]) - int(start[n]) + 1 for n in range(3))
+        return did.value, shape
+
+    def fd_dump_results(self, did, shape):
+        out = np.zeros(shape + (2,), np.float32)
+        check(lib().ems_fd_dump_results(self.h, did, ptr(out, C.c_float)))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ems_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,279 @@
+"""Processing: probes and frequency-domain dumps evaluated by the engine.
+
+Host-side mirror of the reference's Processing classes.  The snapping and the
+order of the terms of every integral are fixed here, exactly as the
+reference's probe classes visit them, and handed to the device, which
+evaluates them at each sampling step (ems_add_probe); the time-domain rows and
+the running DFT are then assembled here in the reference's operation order.
+
+* ProcessVoltage            Common/processvoltage.cpp:37-47 ->
+                            Engine_Interface_FDTD::CalcVoltageIntegral (FDTD/engine_interface_fdtd.cpp:206-232)
+* ProcessCurrent            Common/processcurrent.cpp:42-171
+* ProcessFieldProbe         Common/processfieldprobe.cpp:77-92, GetRawField (FDTD/engine_interface_fdtd.cpp:263-300)
+* ProcessIntegral::Process  Common/processintegral.cpp:134-171 (TD rows, DFT)
+* Processing scheduling     Common/processing.cpp:82-170
+* ProcessFieldsFD           Common/processfields_fd.cpp:72-107 (E field, node interpolation)
+"""
+from __future__ import annotations
+
+import cmath
+import ctypes
+import ctypes.util
+import math
+
+import numpy as np
+
+_libm = ctypes.CDLL(ctypes.util.find_library("m") or "libm.so.6")
+_libm.cosf.restype = ctypes.c_float
+_libm.cosf.argtypes = [ctypes.c_float]
+_libm.sinf.restype = ctypes.c_float
+_libm.sinf.argtypes = [ctypes.c_float]
+
+PROBE_NAMES = {0: "voltage", 1: "current", 2: "E-field", 3: "H-field"}
+
+
+def nyquist_num(f, dT):
+    """CalcNyquistNum (tools/useful.cpp:30-36)."""
+    if f == 0:
+        return 0xFFFFFFFF
+    return int(math.floor(1 / f / 2 / dT))
+
+
+class Probe:
+    """One ProcessIntegral-family probe (type 0 voltage, 1 current, 2 E point, 3 H point)."""
+
+    def __init__(self, op, spec, nrts, oversampling=4):
+        self.op = op
+        self.spec = spec
+        self.name = spec.get("name", "")
+        self.ptype = int(spec["type"])
+        self.weight = float(spec.get("weight", 1.0))
+        self.dual = self.ptype in (1, 3)
+        nyq = op.nyquist
+        self.interval = max(1, nyq // int(oversampling))
+        self.fd_interval = 0
+        self.freqs = []
+        for f in spec.get("freqs", []):          # Processing::AddFrequency
+            nts = nyquist_num(f, op.dT)
+            if nts == 0:
+                continue
+            if self.fd_interval == 0:
+                self.fd_interval = nyq
+            if self.fd_interval > nts:
+                self.fd_interval = nts
+            self.freqs.append(float(f))
+        self.start_ts, self.stop_ts = 0, 0xFFFFFFFF
+        self.enabled = True
+        self._terms()
+        self.steps = [ts for ts in range(nrts + 1) if self.due(ts)]
+        self.pid = None
+
+    def due(self, ts):
+        if not self.enabled or ts < self.start_ts or ts > self.stop_ts:
+            return False
+        return ts % self.interval == 0 or (self.fd_interval != 0 and ts % self.fd_interval == 0)
+
+    def _terms(self):
+        op = self.op
+        s = self.spec
+        self.comp, self.pos, self.sign, self.div = [], [], [], None
+        if self.ptype == 0:
+            start, _ = op.snap_to_mesh(s["start"], False)
+            stop, _ = op.snap_to_mesh(s["stop"], False)
+            self.start, self.stop = start, stop
+            if sum(1 for n in range(3) if start[n] != stop[n]) != 1:
+                self.enabled = False
+                return
+            for n in range(3):
+                if start[n] < stop[n]:
+                    p = list(start)
+                    while p[n] < stop[n]:
+                        self._add(n, p, 1)
+                        p[n] += 1
+                elif start[n] > stop[n]:
+                    p = list(stop)
+                    while p[n] < start[n]:
+                        self._add(n, p, -1)
+                        p[n] += 1
+            return
+        dim, st, sp, sin, ein = op.snap_box(s["start"], s["stop"], self.dual)
+        if dim < 0:
+            self.enabled = False
+            return
+        self.start, self.stop, self.sin, self.ein = st, sp, sin, ein
+        if self.ptype == 1:
+            self._current_terms(s, dim)
+            return
+        self.div = []
+        for n in range(3):
+            self.comp.append(n)
+            self.pos.append(list(st))
+            self.sign.append(1)
+            self.div.append(op.edge_length(n, st[n], self.ptype == 3))
+
+    def _add(self, n, p, sg):
+        self.comp.append(n)
+        self.pos.append(list(p))
+        self.sign.append(sg)
+
+    def _current_terms(self, s, dim):
+        """ProcessCurrent::DefineStartStopCoord + CalcIntegral term order."""
+        op = self.op
+        st, sp = self.start, self.stop
+        nd = int(s.get("norm_dir", -1))
+        ds, de = s["start"], s["stop"]
+        if nd < 0 or nd > 2:
+            if dim != 2:
+                self.enabled = False
+                return
+            for n in range(3):
+                if sp[n] == st[n]:
+                    nd = n
+        else:
+            for n in range(3):
+                if n == nd:
+                    continue
+                if ds[n] == de[n]:
+                    if op.disc_line(n, st[n], True) > ds[n] and st[n] > 0:
+                        st[n] -= 1
+                    if op.disc_line(n, st[n], True) < ds[n] and sp[n] < op.N[n] - 1:
+                        sp[n] += 1
+        if st[nd] != sp[nd]:
+            self.enabled = False
+            return
+        si, ei = self.sin, self.ein
+        seq = []
+
+        def seg(n_comp, fixed, axis, lo, hi, sg):
+            for i in range(lo + 1, hi + 1):
+                p = list(fixed)
+                p[axis] = i
+                seq.append((n_comp, p, sg))
+        if nd == 0:
+            if ei[0] and si[2]:
+                seg(1, [sp[0], 0, st[2]], 1, st[1], sp[1], 1)
+            if ei[0] and ei[1]:
+                seg(2, [sp[0], sp[1], 0], 2, st[2], sp[2], 1)
+            if si[0] and ei[2]:
+                seg(1, [st[0], 0, sp[2]], 1, st[1], sp[1], -1)
+            if si[0] and si[1]:
+                seg(2, [st[0], st[1], 0], 2, st[2], sp[2], -1)
+        elif nd == 1:
+            if si[0] and si[1]:
+                seg(2, [st[0], st[1], 0], 2, st[2], sp[2], 1)
+            if ei[1] and ei[2]:
+                seg(0, [0, sp[1], sp[2]], 0, st[0], sp[0], 1)
+            if ei[0] and ei[1]:
+                seg(2, [sp[0], sp[1], 0], 2, st[2], sp[2], -1)
+            if si[1] and si[2]:
+                seg(0, [0, st[1], st[2]], 0, st[0], sp[0], -1)
+        else:
+            if si[1] and si[2]:
+                seg(0, [0, st[1], st[2]], 0, st[0], sp[0], 1)
+            if ei[0] and si[2]:
+                seg(1, [sp[0], 0, st[2]], 1, st[1], sp[1], 1)
+            if ei[1] and ei[2]:
+                seg(0, [0, sp[1], sp[2]], 0, st[0], sp[0], -1)
+            if si[0] and ei[2]:
+                seg(1, [st[0], 0, sp[2]], 1, st[1], sp[1], -1)
+        for n, p, sg in seq:
+            self._add(n, p, sg)
+        self.norm_dir = nd
+
+    # ---- engine registration and results -----------------------------------------
+    def register(self, eng):
+        if not self.enabled or not self.comp:
+            return
+        pos = np.array(self.pos, np.uint32).T.copy()     # [3][nelem]
+        self.pid = eng.add_probe(self.ptype, self.comp, pos, self.sign, self.div, self.interval,
+                                 self.fd_interval, self.start_ts, self.stop_ts, len(self.steps))
+
+    def collect(self, eng):
+        """Returns (td, fd): td rows (t, values*weight...), fd [nint][nfreq] complex."""
+        nint = 3 if self.ptype in (2, 3) else 1
+        if self.pid is None:
+            ts = np.array(self.steps, np.uint32)
+            vals = np.zeros((len(ts), 3))
+        else:
+            ts, vals = eng.probe_results(self.pid)
+        dT = self.op.dT
+        td = []
+        fd = np.zeros((nint, len(self.freqs)), np.complex128)
+        for k in range(len(ts)):
+            t_int = int(ts[k])
+            t = (t_int + (0.5 if self.dual else 0.0)) * dT   # GetTime(dualTime)
+            v = [float(vals[k, i]) for i in range(nint)]
+            if t_int % self.interval == 0:
+                td.append([t] + [x * self.weight for x in v])
+            if self.fd_interval and t_int % self.fd_interval == 0:
+                for j, f in enumerate(self.freqs):
+                    th = ((-2.0 * math.pi) * f) * t
+                    e = cmath.exp(complex(-0.0, th))
+                    for i, x in enumerate(v):
+                        a = x * self.weight
+                        re = ((a * e.real) * 2.0) * dT * float(self.fd_interval)
+                        im = ((a * e.imag) * 2.0) * dT * float(self.fd_interval)
+                        fd[i, j] += complex(re, im)
+        return np.array(td) if td else np.zeros((0, 1 + nint)), fd
+
+
+class FieldDumpFD:
+    """ProcessFieldsFD of the E field with node interpolation (dump type 10)."""
+
+    def __init__(self, op, spec, nrts):
+        self.op = op
+        self.spec = spec
+        self.name = spec.get("name", "")
+        dim, st, sp, _, _ = op.snap_box(spec["start"], spec["stop"], False)
+        # ProcessFields snaps with SnapMethod 1 (processfields.cpp:40) and orders start <= stop
+        lo = [min(a, b) for a, b in zip(spec["start"], spec["stop"])]
+        hi = [max(a, b) for a, b in zip(spec["start"], spec["stop"])]
+        for n in range(3):
+            if sp[n] > st[n]:
+                if op.disc_line(n, st[n]) > lo[n] and st[n] > 0:
+                    st[n] -= 1
+                if op.disc_line(n, sp[n]) < hi[n] and sp[n] < op.N[n] - 1:
+                    sp[n] += 1
+            if st[n] > sp[n]:
+                st[n], sp[n] = sp[n], st[n]
+        self.start, self.stop = st, sp
+        nyq = op.nyquist
+        self.fd_interval = 0
+        self.freqs = []
+        for f in spec.get("freqs", []):
+            nts = nyquist_num(f, op.dT)
+            if nts == 0:
+                continue
+            if self.fd_interval == 0:
+                self.fd_interval = nyq
+            if self.fd_interval > nts:
+                self.fd_interval = nts
+            self.freqs.append(float(f))
+        self.nsamples = nrts // self.fd_interval + 1 if self.fd_interval else 0
+        self.did = None
+
+    def factors(self):
+        """exp(-2 pi i f T) * 2 * dT * FD_Interval in complex<float>, per sample and frequency."""
+        dT = self.op.dT
+        out = np.zeros((self.nsamples, len(self.freqs), 2), np.float32)
+        scale = np.float32(dT * self.fd_interval)
+        for k in range(self.nsamples):
+            T = (k * self.fd_interval) * dT
+            for j, f in enumerate(self.freqs):
+                th = np.float32(((-2.0 * math.pi) * f) * T)
+                c = np.float32(_libm.cosf(th))
+                s = np.float32(_libm.sinf(th))
+                out[k, j, 0] = np.float32(c * np.float32(2)) * scale
+                out[k, j, 1] = np.float32(s * np.float32(2)) * scale
+        return out
+
+    def register(self, eng):
+        if not self.fd_interval:
+            return
+        self.did, self.shape = eng.add_fd_dump(self.start, self.stop, self.fd_interval, self.factors())
+
+    def collect(self, eng):
+        if self.did is None:
+            return None
+        raw = eng.fd_dump_results(self.did, self.shape)
+        return raw[..., 0].astype(np.float32) + 1j * raw[..., 1].astype(np.float32)

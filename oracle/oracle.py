@@ -1,0 +1,487 @@
+"""Oracle driver: runs the restated reference engines and probes on the CPU.
+
+TEST INFRASTRUCTURE ONLY (see oracle/operator_ref.py).  Loads
+``oracle/_build/liboracle_engine.so`` (built by ``make -C oracle``) and
+restates the probe processing of the reference:
+
+* ProcessVoltage / Engine_Interface_FDTD::CalcVoltageIntegral
+  (Common/processvoltage.cpp:37-47, FDTD/engine_interface_fdtd.cpp:206-232)
+* ProcessCurrent::CalcIntegral, float accumulation (Common/processcurrent.cpp:96-171)
+* ProcessFieldProbe, no interpolation (Common/processfieldprobe.cpp:77-92,
+  FDTD/engine_interface_fdtd.cpp:263-300)
+* ProcessIntegral::Process TD rows + running DFT (Common/processintegral.cpp:134-171)
+* Processing::CheckTimestep / sampling (Common/processing.cpp:82-132)
+* RunFDTD loop order: Process() at numTS=0, then after every step
+  (openems.cpp:1424-1441)
+* ProcessFieldsFD running complex<float> DFT of an E/H dump box
+  (Common/processfields_fd.cpp:72-107, Common/processfields.cpp:221-409)
+"""
+from __future__ import annotations
+
+import cmath
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+from .operator_ref import RefOperator, MUE0, EPS0
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+KIND = {"basic": 0, "basic_f64": 1, "sse": 2, "sse_compressed": 3, "multithreaded": 4}
+
+
+def build():
+    """Compile the oracle engine library (g++, no GPU needed)."""
+    import subprocess
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "_build", "liboracle_engine.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        vp = C.c_void_p
+        fp = C.POINTER(C.c_float)
+        up = C.POINTER(C.c_uint)
+        L.ref_create.restype = vp
+        L.ref_create.argtypes = [C.c_int, up, fp, fp, fp, fp, C.c_uint]
+        L.ref_set_coeffs_f64.argtypes = [vp] + [C.POINTER(C.c_double)] * 4
+        L.ref_add_excitation.argtypes = [vp, fp, fp, C.c_uint, C.c_int,
+                                         C.c_uint, up, C.POINTER(C.c_ushort), fp, up,
+                                         C.c_uint, up, C.POINTER(C.c_ushort), fp, up]
+        L.ref_add_upml.argtypes = [vp, up, up, fp, fp, fp, fp, fp, fp]
+        L.ref_add_mur.argtypes = [vp, C.c_int, C.c_uint, C.c_uint, fp, fp, C.c_uint]
+        L.ref_init.argtypes = [vp]
+        L.ref_iterate.argtypes = [vp, C.c_uint]
+        L.ref_numts.argtypes = [vp]
+        L.ref_numts.restype = C.c_uint
+        L.ref_get_fields.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.ref_volt_ptr.argtypes = [vp]
+        L.ref_volt_ptr.restype = vp
+        L.ref_curr_ptr.argtypes = [vp]
+        L.ref_curr_ptr.restype = vp
+        L.ref_num_unique.argtypes = [vp]
+        L.ref_num_unique.restype = C.c_uint
+        L.ref_get_opindex.argtypes = [vp, up]
+        L.ref_energy.argtypes = [vp]
+        L.ref_energy.restype = C.c_double
+        L.ref_destroy.argtypes = [vp]
+        _LIB = L
+    return _LIB
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+class RefEngine:
+    """One restated reference engine instance built from a RefOperator."""
+
+    def __init__(self, op: RefOperator, kind="basic", nthreads=1, coeffs64=None):
+        L = lib()
+        self.op = op
+        self.kind = kind
+        self.N = op.N
+        N = np.array(op.N, np.uint32)
+        self._keep = []
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (op.vv, op.vi, op.ii, op.iv)]
+        self.h = L.ref_create(KIND[kind], _p(N, C.c_uint), *[_p(a, C.c_float) for a in arrs], int(nthreads))
+        if kind == "basic_f64" and coeffs64 is not None:
+            c64 = [np.ascontiguousarray(a, np.float64) for a in coeffs64]
+            L.ref_set_coeffs_f64(self.h, *[_p(a, C.c_double) for a in c64])
+        # extensions in the reference's creation order: excitation, Mur, UPML
+        ve, ce = op.vexc, op.cexc
+        period_steps = int(op.period / op.dT) if op.period > 0 else 0
+        sv = np.ascontiguousarray(op.sig_volt, np.float32)
+        sc = np.ascontiguousarray(op.sig_curr, np.float32)
+        vidx = np.ascontiguousarray(ve.index, np.uint32)
+        cidx = np.ascontiguousarray(ce.index, np.uint32)
+        vd = np.ascontiguousarray(ve.dir, np.uint16)
+        cd = np.ascontiguousarray(ce.dir, np.uint16)
+        va = np.ascontiguousarray(ve.amp, np.float32)
+        ca = np.ascontiguousarray(ce.amp, np.float32)
+        vdl = np.ascontiguousarray(ve.delay, np.uint32)
+        cdl = np.ascontiguousarray(ce.delay, np.uint32)
+        self._keep += [sv, sc, vidx, cidx, vd, cd, va, ca, vdl, cdl]
+        L.ref_add_excitation(self.h, _p(sv, C.c_float), _p(sc, C.c_float), int(op.exc_length), period_steps,
+                             len(vd), _p(vidx, C.c_uint), _p(vd, C.c_ushort), _p(va, C.c_float), _p(vdl, C.c_uint),
+                             len(cd), _p(cidx, C.c_uint), _p(cd, C.c_ushort), _p(ca, C.c_float), _p(cdl, C.c_uint))
+        for f in op.mur:
+            cP = np.ascontiguousarray(f.coeff_nyP, np.float32)
+            cPP = np.ascontiguousarray(f.coeff_nyPP, np.float32)
+            self._keep += [cP, cPP]
+            L.ref_add_mur(self.h, f.ny, f.line, f.line_shift, _p(cP, C.c_float), _p(cPP, C.c_float), int(f.start_ts))
+        for s in op.upml:
+            st = np.array(s.start, np.uint32)
+            nm = np.array(s.num, np.uint32)
+            arrs = [np.ascontiguousarray(getattr(s, k), np.float32) for k in ("vv", "vvfo", "vvfn", "ii", "iifo", "iifn")]
+            self._keep += [st, nm] + arrs
+            L.ref_add_upml(self.h, _p(st, C.c_uint), _p(nm, C.c_uint), *[_p(a, C.c_float) for a in arrs])
+        L.ref_init(self.h)
+
+    @property
+    def numTS(self):
+        return lib().ref_numts(self.h)
+
+    def iterate(self, n):
+        lib().ref_iterate(self.h, int(n))
+
+    def fields(self):
+        """(volt, curr) as float64 arrays [3][Nx][Ny][Nz]."""
+        shape = (3,) + tuple(self.N)
+        v = np.empty(shape, np.float64)
+        c = np.empty(shape, np.float64)
+        lib().ref_get_fields(self.h, _p(v, C.c_double), _p(c, C.c_double))
+        return v, c
+
+    def raw_views(self):
+        """Zero-copy views on the basic engines' storage (float32 or float64)."""
+        L = lib()
+        dt = np.float64 if self.kind == "basic_f64" else np.float32
+        shape = (3,) + tuple(self.N)
+        n = int(np.prod(shape))
+        pv = L.ref_volt_ptr(self.h)
+        pc = L.ref_curr_ptr(self.h)
+        ct = C.c_double if dt == np.float64 else C.c_float
+        v = np.ctypeslib.as_array(C.cast(pv, C.POINTER(ct)), shape=(n,)).reshape(shape)
+        c = np.ctypeslib.as_array(C.cast(pc, C.POINTER(ct)), shape=(n,)).reshape(shape)
+        return v, c
+
+    def energy(self):
+        return lib().ref_energy(self.h)
+
+    def num_unique(self):
+        return lib().ref_num_unique(self.h)
+
+    def opindex(self):
+        nV = int(math.ceil(self.N[2] / 4.0))
+        out = np.zeros((self.N[0], self.N[1], nV), np.uint32)
+        lib().ref_get_opindex(self.h, _p(out, C.c_uint))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().ref_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ----------------------------------------------------------------------------
+# probes
+# ----------------------------------------------------------------------------
+class RefProbe:
+    """ProcessIntegral-derived probe (voltage=0, current=1, E-point=2, H-point=3)."""
+
+    def __init__(self, op: RefOperator, ptype, start, stop, weight=1.0, freqs=(), norm_dir=-1,
+                 oversampling=4, name=""):
+        self.op = op
+        self.ptype = ptype
+        self.weight = float(weight)
+        self.name = name
+        m = op.mesh
+        self.dual_time = ptype in (1, 3)
+        self.dual_mesh = ptype in (1, 3)
+        nyq = op.nyquist
+        self.interval = max(1, nyq // oversampling)       # SetProcessInterval(Nyquist/OverSampling)
+        self.fd_interval = 0
+        self.freqs = []
+        for f in freqs:                                     # Processing::AddFrequency
+            nts = int(math.floor(1 / f / 2 / op.dT))
+            if nts == 0:
+                continue
+            if self.fd_interval == 0:
+                self.fd_interval = nyq
+            if self.fd_interval > nts:
+                self.fd_interval = nts
+            self.freqs.append(float(f))
+        self.enabled = True
+        if ptype == 0:
+            # ProcessVoltage::DefineStartStopCoord: SnapToMesh on the primary mesh
+            self.start, self.start_in = m.snap(start, False)
+            self.stop, self.stop_in = m.snap(stop, False)
+        else:
+            dim, s, e = m.snap_box(start, stop, self.dual_mesh)
+            if dim < 0:
+                self.enabled = False
+                return
+            _, self.start_in = m.snap([min(a, b) for a, b in zip(start, stop)], self.dual_mesh)
+            _, self.stop_in = m.snap([max(a, b) for a, b in zip(start, stop)], self.dual_mesh)
+            self.start, self.stop = list(s), list(e)
+            self.dim = dim
+            if ptype == 1:
+                self._current_setup(start, stop, norm_dir, dim)
+        self.td = []            # (time, values*weight)
+        nint = 3 if ptype in (2, 3) else 1
+        self.fd = np.zeros((nint, len(self.freqs)), np.complex128)
+
+    def _current_setup(self, dstart, dstop, norm_dir, dim):
+        """ProcessCurrent::DefineStartStopCoord (Common/processcurrent.cpp:42-94)."""
+        m = self.op.mesh
+        self.norm_dir = norm_dir
+        if norm_dir < 0 or norm_dir > 2:
+            if dim != 2:
+                self.enabled = False
+                return
+            for n in range(3):
+                if self.stop[n] == self.start[n]:
+                    self.norm_dir = n
+        else:
+            dim = 0
+            for n in range(3):
+                if n == norm_dir:
+                    continue
+                if dstart[n] == dstop[n]:
+                    if m.dual_line[n][self.start[n]] > dstart[n] and self.start[n] > 0:
+                        self.start[n] -= 1
+                    if m.dual_line[n][self.start[n]] < dstart[n] and self.stop[n] < m.N[n] - 1:
+                        self.stop[n] += 1
+                if self.stop[n] > self.start[n]:
+                    dim += 1
+        if self.start[self.norm_dir] != self.stop[self.norm_dir]:
+            self.enabled = False
+
+    # -- integrals on a field snapshot (volt, curr) ---------------------------
+    def values(self, volt, curr):
+        p = self.ptype
+        if p == 0:
+            s, e = self.start, self.stop
+            if sum(1 for n in range(3) if s[n] != e[n]) != 1:
+                return [0.0]
+            r = 0.0
+            for n in range(3):
+                if s[n] < e[n]:
+                    pos = list(s)
+                    while pos[n] < e[n]:
+                        r += float(volt[n, pos[0], pos[1], pos[2]])
+                        pos[n] += 1
+                elif s[n] > e[n]:
+                    pos = list(e)
+                    while pos[n] < s[n]:
+                        r -= float(volt[n, pos[0], pos[1], pos[2]])
+                        pos[n] += 1
+            return [r]
+        if p == 1:
+            return [float(self._current(curr))]
+        m = self.op.mesh
+        out = []
+        pos = self.start
+        for n in range(3):
+            if p == 2:
+                d = m.edge_len(n, False)[pos[n]]
+                v = float(volt[n, pos[0], pos[1], pos[2]])
+            else:
+                d = m.edge_len(n, True)[pos[n]]
+                v = float(curr[n, pos[0], pos[1], pos[2]])
+            out.append(v / d if d else 0.0)
+        return out
+
+    def _current(self, curr):
+        s, e = self.start, self.stop
+        si, ei = self.start_in, self.stop_in
+        acc = np.float32(0)
+
+        def I(n, x, y, z):
+            return np.float32(curr[n, x, y, z])
+        nd = self.norm_dir
+        if nd == 0:
+            if ei[0] and si[2]:
+                for i in range(s[1] + 1, e[1] + 1):
+                    acc = np.float32(acc + I(1, e[0], i, s[2]))
+            if ei[0] and ei[1]:
+                for i in range(s[2] + 1, e[2] + 1):
+                    acc = np.float32(acc + I(2, e[0], e[1], i))
+            if si[0] and ei[2]:
+                for i in range(s[1] + 1, e[1] + 1):
+                    acc = np.float32(acc - I(1, s[0], i, e[2]))
+            if si[0] and si[1]:
+                for i in range(s[2] + 1, e[2] + 1):
+                    acc = np.float32(acc - I(2, s[0], s[1], i))
+        elif nd == 1:
+            if si[0] and si[1]:
+                for i in range(s[2] + 1, e[2] + 1):
+                    acc = np.float32(acc + I(2, s[0], s[1], i))
+            if ei[1] and ei[2]:
+                for i in range(s[0] + 1, e[0] + 1):
+                    acc = np.float32(acc + I(0, i, e[1], e[2]))
+            if ei[0] and ei[1]:
+                for i in range(s[2] + 1, e[2] + 1):
+                    acc = np.float32(acc - I(2, e[0], e[1], i))
+            if si[1] and si[2]:
+                for i in range(s[0] + 1, e[0] + 1):
+                    acc = np.float32(acc - I(0, i, s[1], s[2]))
+        elif nd == 2:
+            if si[1] and si[2]:
+                for i in range(s[0] + 1, e[0] + 1):
+                    acc = np.float32(acc + I(0, i, s[1], s[2]))
+            if ei[0] and si[2]:
+                for i in range(s[1] + 1, e[1] + 1):
+                    acc = np.float32(acc + I(1, e[0], i, s[2]))
+            if ei[1] and ei[2]:
+                for i in range(s[0] + 1, e[0] + 1):
+                    acc = np.float32(acc - I(0, i, e[1], e[2]))
+            if si[0] and ei[2]:
+                for i in range(s[1] + 1, e[1] + 1):
+                    acc = np.float32(acc - I(1, s[0], i, e[2]))
+        return acc
+
+    def check(self, ts):
+        if not self.enabled:
+            return False
+        if ts % self.interval == 0:
+            return True
+        if self.fd_interval and ts % self.fd_interval == 0:
+            return True
+        return False
+
+    def process(self, ts, volt, curr):
+        """ProcessIntegral::Process (Common/processintegral.cpp:134-171)."""
+        if not self.check(ts):
+            return
+        vals = self.values(volt, curr)
+        t = (ts + (0.5 if self.dual_time else 0.0)) * self.op.dT
+        if ts % self.interval == 0:
+            self.td.append((t, [v * self.weight for v in vals]))
+        if self.fd_interval and ts % self.fd_interval == 0:
+            dT = self.op.dT
+            for k, f in enumerate(self.freqs):
+                th = ((-2.0 * math.pi) * f) * t
+                e = cmath.exp(complex(-0.0, th))
+                for i, v in enumerate(vals):
+                    a = v * self.weight
+                    re = ((a * e.real) * 2.0) * dT * float(self.fd_interval)
+                    im = ((a * e.imag) * 2.0) * dT * float(self.fd_interval)
+                    self.fd[i, k] += complex(re, im)
+
+    def td_array(self):
+        if not self.td:
+            return np.zeros((0, 2))
+        return np.array([[t] + v for t, v in self.td])
+
+
+class RefFieldDumpFD:
+    """ProcessFieldsFD of an E (0) or H (1) field box with node interpolation
+    (Engine_Interface_FDTD NODE_INTERPOLATE, engine_interface_fdtd.cpp:63-124,
+    :126-204), accumulated in complex<float>."""
+
+    def __init__(self, op: RefOperator, dump_type, start, stop, freqs, interp=1):
+        self.op = op
+        self.dump_type = dump_type
+        self.interp = interp
+        m = op.mesh
+        self.dual_mesh = dump_type == 1 and interp == 0
+        self.dual_time = dump_type == 1
+        dim, s, e = m.snap_box(start, stop, self.dual_mesh)
+        self.pos = [np.arange(s[n], e[n] + 1) for n in range(3)]
+        nyq = op.nyquist
+        self.fd_interval = 0
+        self.freqs = []
+        for f in freqs:
+            nts = int(math.floor(1 / f / 2 / op.dT))
+            if nts == 0:
+                continue
+            if self.fd_interval == 0:
+                self.fd_interval = nyq
+            if self.fd_interval > nts:
+                self.fd_interval = nts
+            self.freqs.append(float(f))
+        shape = (len(self.freqs), 3) + tuple(len(p) for p in self.pos)
+        self.fd_re = np.zeros(shape, np.float32)
+        self.fd_im = np.zeros(shape, np.float32)
+
+    def field(self, volt, curr):
+        """CalcField for an E dump with node interpolation, as float32 [3][nx][ny][nz]."""
+        m = self.op.mesh
+        N = self.op.N
+        P = self.pos
+        out = np.zeros((3,) + tuple(len(p) for p in P), np.float32)
+        if self.dump_type != 0 or self.interp != 1:
+            raise NotImplementedError("oracle field dumps: E-field node interpolation only")
+        for n in range(3):
+            EL = m.edge_len(n, False)
+            idx = np.ix_(*P)
+            pn = P[n]
+            shape = [1, 1, 1]
+            shape[n] = len(pn)
+            delta = EL[pn]
+            raw = volt[n][idx] / EL[pn].reshape(shape)
+            # lower neighbour (pn-1) raw field, clipped index
+            lo = P.copy()
+            lo[n] = np.clip(pn - 1, 0, None)
+            raw_lo = volt[n][np.ix_(*lo)] / EL[np.clip(pn - 1, 0, None)].reshape(shape)
+            deltaDown = EL[np.clip(pn - 1, 0, None)]
+            with np.errstate(divide="ignore", invalid="ignore"):
+                rel = delta / (delta + deltaDown)
+            rel = rel.reshape(shape)
+            val = raw * (1.0 - rel) + raw_lo * rel
+            top = (pn == N[n] - 1).reshape(shape)
+            bot = (pn == 0).reshape(shape)
+            val = np.where(bot, raw, val)
+            val = np.where(top, raw_lo, val)
+            out[n] = val.astype(np.float32)
+        return out
+
+    def process(self, ts, volt, curr):
+        if not self.fd_interval or ts % self.fd_interval != 0:
+            return
+        F = self.field(volt, curr)
+        T = (ts + (0.5 if self.dual_time else 0.0)) * self.op.dT
+        dT = self.op.dT
+        for k, f in enumerate(self.freqs):
+            z = np.complex64(complex(-0.0, ((-2.0 * math.pi) * f) * T))
+            e = np.exp(z)                                   # exp in complex<float>
+            e = e * np.float32(2)
+            e = e * np.float32(dT * self.fd_interval)
+            er, ei = np.float32(e.real), np.float32(e.imag)
+            self.fd_re[k] = self.fd_re[k] + F * er
+            self.fd_im[k] = self.fd_im[k] + F * ei
+
+
+def run_reference(op: RefOperator, nrts, probes=(), dumps=(), kind="basic", nthreads=1, eng=None):
+    """RunFDTD (openems.cpp:1388-1494) with the end criterion disabled:
+    Process() at numTS=0 and after every timestep up to NrTS."""
+    if eng is None:
+        eng = RefEngine(op, kind, nthreads)
+    basic = kind in ("basic", "basic_f64")
+
+    def snap():
+        if basic:
+            return eng.raw_views()
+        return eng.fields()
+
+    def needs(ts):
+        return any(p.check(ts) for p in probes) or any(d.fd_interval and ts % d.fd_interval == 0 for d in dumps)
+
+    ts = eng.numTS
+    if needs(ts):
+        v, c = snap()
+        for p in probes:
+            p.process(ts, v, c)
+        for d in dumps:
+            d.process(ts, v, c)
+    while eng.numTS < nrts:
+        # advance to the next sample step (or the end), as RunFDTD does
+        nxt = nrts
+        for t in range(eng.numTS + 1, nrts + 1):
+            if needs(t):
+                nxt = t
+                break
+        eng.iterate(nxt - eng.numTS)
+        ts = eng.numTS
+        if needs(ts):
+            v, c = snap()
+            for p in probes:
+                p.process(ts, v, c)
+            for d in dumps:
+                d.process(ts, v, c)
+    return eng

@@ -1,0 +1,166 @@
+"""Parity of the HIP engine (through the C ABI) with the oracle's restated
+reference engine on the same operator and inputs.  The bar is bit-identical
+fields and probe samples; probe time series additionally within 1e-5
+relative L2 (the tolerance stated for the engine) where a reduced-order
+accumulation is involved."""
+import numpy as np
+import pytest
+
+from openems_amd import scenes
+from openems_amd.engine import EngineHIP
+from openems_amd.operator import Operator
+from openems_amd.simulation import Simulation
+from oracle.operator_ref import RefOperator
+from oracle.oracle import RefEngine, RefFieldDumpFD, RefProbe, run_reference
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    n = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (n if n > 0 else 1.0)
+
+
+def _ref_from_scene(scene):
+    return RefOperator(scene).build()
+
+
+def _fields_equal(eng, ref_eng):
+    v = eng.get_volt()
+    c = eng.get_curr()
+    rv, rc = ref_eng.raw_views()
+    return v, c, rv, rc
+
+
+@pytest.mark.parametrize("make", [
+    lambda: scenes.pec_cavity(32, 150),
+    lambda: scenes.upml_bricks(48, 150, nbricks=8, dump=False),
+    lambda: scenes.mur_layers(40, 36, 32, 200),
+])
+def test_fields_bit_identical(gpu, make):
+    scene = make()
+    nrts = scene["nrts"]
+    print("case", scene["name"], "steps", nrts, flush=True)
+    op = Operator(scene)
+    eng = EngineHIP(op)
+    eng.IterateTS(nrts)
+    ref = RefEngine(_ref_from_scene(scene), "basic")
+    ref.iterate(nrts)
+    v, c, rv, rc = _fields_equal(eng, ref)
+    assert eng.numTS == nrts
+    assert np.max(np.abs(rv)) > 0
+    assert np.array_equal(v, rv), f"voltage mismatches: {np.count_nonzero(v != rv)}"
+    assert np.array_equal(c, rc), f"current mismatches: {np.count_nonzero(c != rc)}"
+    e_gpu = eng.energy()
+    assert e_gpu == pytest.approx(ref.energy(), rel=1e-12)
+    eng.close()
+
+
+def test_pec_cavity_64_probes_2000_steps(gpu):
+    """The 64^3 PEC cavity workload: E-point and voltage-line probes vs the oracle."""
+    scene = scenes.pec_cavity(64, 2000)
+    print("case pec_cavity_64 2000 steps", flush=True)
+    sim = Simulation(scene)
+    sim.run()
+    res = sim.results()["probes"]
+    ref = _ref_from_scene(scene)
+    rp = [RefProbe(ref, p["type"], p["start"], p["stop"], p["weight"], p["freqs"], p["norm_dir"],
+                   scene["oversampling"], p["name"]) for p in scene["probes"]]
+    run_reference(ref, 2000, probes=rp)
+    for p in rp:
+        got = res[p.name]["td"]
+        want = p.td_array()
+        assert got.shape == want.shape
+        np.testing.assert_array_equal(got[:, 0], want[:, 0])
+        assert rel_l2(got[:, 1:], want[:, 1:]) <= 1e-5
+        assert np.array_equal(got[:, 1:], want[:, 1:])
+    sim.close()
+
+
+def test_port_probes_and_fd_dump(gpu):
+    """Lumped-port V/I probes with DFT and the E-field DFT plane (UPML scene)."""
+    scene = scenes.upml_bricks(48, 400, nbricks=8)
+    scene["probes"][0]["freqs"] = [5e9, 10e9]
+    scene["probes"][1]["freqs"] = [5e9, 10e9]
+    print("case upml_bricks_48 400 steps, port probes + fd dump", flush=True)
+    sim = Simulation(scene)
+    sim.run()
+    res = sim.results()
+    ref = _ref_from_scene(scene)
+    rp = [RefProbe(ref, p["type"], p["start"], p["stop"], p["weight"], p["freqs"], p["norm_dir"],
+                   scene["oversampling"], p["name"]) for p in scene["probes"]]
+    d = scene["dumps"][0]
+    rd = RefFieldDumpFD(ref, 0, d["start"], d["stop"], d["freqs"])
+    run_reference(ref, 400, probes=rp, dumps=[rd])
+    for p in rp:
+        got = res["probes"][p.name]
+        want = p.td_array()
+        assert np.array_equal(got["td"], want)
+        assert np.abs(want[:, 1]).max() > 0
+        np.testing.assert_allclose(got["fd"], p.fd, rtol=1e-12, atol=1e-30)
+    fd = res["dumps"][d["name"]]["fd"]
+    want = rd.fd_re + 1j * rd.fd_im
+    assert fd.shape == want.shape
+    assert rel_l2(fd, want) <= 1e-5
+    sim.close()
+
+
+def test_linearity_full_size_upml(gpu):
+    """Size-independent check at the full 256^3 UPML workload: scaling the
+    excitation by 2 scales every field exactly by 2 (binary scaling commutes
+    with rounding), and fields stay finite."""
+    s1 = scenes.upml_bricks(256, 60, dump=False)
+    s2 = scenes.upml_bricks(256, 60, dump=False)
+    s2["excitations"][0]["val"] = (0.0, 0.0, -2.0)
+    print("case upml_bricks_256 linearity 60 steps", flush=True)
+    e = []
+    for s in (s1, s2):
+        op = Operator(s)
+        eng = EngineHIP(op)
+        eng.IterateTS(60)
+        e.append((eng.get_volt(), eng.get_curr()))
+        eng.close()
+        op.close()
+    (v1, c1), (v2, c2) = e
+    assert np.all(np.isfinite(v1)) and np.max(np.abs(v1)) > 0
+    np.testing.assert_array_equal(v2, 2 * v1)
+    np.testing.assert_array_equal(c2, 2 * c1)
+
+
+def test_full_size_upml_against_oracle(gpu):
+    """256^3 UPML bricks workload, 40 steps, bit-identical to the oracle engine."""
+    scene = scenes.upml_bricks(256, 40, dump=False)
+    print("case upml_bricks_256 oracle parity 40 steps", flush=True)
+    op = Operator(scene)
+    eng = EngineHIP(op)
+    eng.IterateTS(40)
+    v = eng.get_volt()
+    c = eng.get_curr()
+    eng.close()
+
+    class _Op:   # the operator is pinned separately (test_operator_parity); reuse its arrays
+        pass
+    ro = _Op()
+    ro.N = op.N
+    ro.vv, ro.vi, ro.ii, ro.iv = op.coeff("vv"), op.coeff("vi"), op.coeff("ii"), op.coeff("iv")
+    ve, ce = op.excitation(False), op.excitation(True)
+    from oracle.operator_ref import ExcList, UPMLSlab
+    ro.vexc = ExcList(ve["index"], ve["dir"], ve["amp"], ve["delay"])
+    ro.cexc = ExcList(ce["index"], ce["dir"], ce["amp"], ce["delay"])
+    ro.sig_volt, ro.sig_curr = op.signal()
+    ro.exc_length = op.signal_length
+    ro.period, ro.dT = op.period, op.dT
+    ro.mur = []
+    ro.upml = []
+    for s in op.upml_slabs():
+        u = UPMLSlab(s["start"], s["num"])
+        for k in ("vv", "vvfo", "vvfn", "ii", "iifo", "iifn"):
+            setattr(u, k, s[k])
+        ro.upml.append(u)
+    ref = RefEngine(ro, "basic")
+    ref.iterate(40)
+    rv, rc = ref.raw_views()
+    assert np.array_equal(v, rv), f"voltage mismatches {np.count_nonzero(v != rv)}"
+    assert np.array_equal(c, rc), f"current mismatches {np.count_nonzero(c != rc)}"

@@ -1,0 +1,204 @@
+"""Pins the oracle (the CPU restatement of the reference) against what the
+mathematics fixes and against the reference's own cross-engine invariant
+(TESTSUITE/enginetests/cavity.m: basic, sse, sse-compressed and
+multithreaded engines agree on the same setup).  CPU only."""
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate
+
+from oracle.operator_ref import C0, EPS0, MUE0, Z0, RefOperator, nyquist_num
+from oracle.oracle import RefEngine, RefProbe, run_reference
+from openems_amd import scenes
+
+
+def _uniform_scene(n=16, d=1.0, bc=(0,) * 6, **kw):
+    l = np.arange(n) * d
+    s = {"lines": [l, l.copy(), l.copy()], "delta_unit": 1e-3, "bc": list(bc),
+         "signal": {"type": "gauss", "f0": 0.0, "fc": 10e9}, "nrts": 100,
+         "excitations": [{"type": 0, "val": (0, 0, 1), "start": (n // 2 * d,) * 2 + (n // 2 * d,),
+                          "stop": (n // 2 * d, n // 2 * d, (n // 2 + 1) * d)}]}
+    s.update(kw)
+    return s
+
+
+@pytest.mark.parametrize("d", [1.0, 0.5, 2.0])
+def test_timestep_uniform_vacuum_closed_form(d):
+    op = RefOperator(_uniform_scene(12, d)).build()
+    delta = d * 1e-3
+    exact = delta * math.sqrt(EPS0 * MUE0 / 3.0)
+    # the reference forms the CFL sum from float32 EC values: relative error ~1e-8
+    assert abs(op.dT - exact) / exact < 1e-7
+
+
+def test_timestep_var1_uniform_closed_form():
+    op = RefOperator(_uniform_scene(12, 1.0, ts_method=1)).build()
+    exact = 1e-3 * math.sqrt(EPS0 * MUE0 / 4.0)      # 2/sqrt(16/(LC)) with L=mu0*d, C=eps0*d
+    assert abs(op.dT - exact) / exact < 1e-6
+
+
+def test_vacuum_coefficients_closed_form():
+    op = RefOperator(_uniform_scene(12)).build()
+    d = 1e-3
+    inner = (slice(None), slice(2, 9), slice(2, 9), slice(2, 9))
+    assert np.all(op.vv[inner] == 1.0) and np.all(op.ii[inner] == 1.0)
+    np.testing.assert_allclose(op.vi[inner], op.dT / (EPS0 * d), rtol=2e-7)
+    np.testing.assert_allclose(op.iv[inner], op.dT / (MUE0 * d), rtol=2e-7)
+
+
+def test_pec_faces_and_last_current_lines_zeroed():
+    op = RefOperator(_uniform_scene(10)).build()
+    N = op.N
+    for n in range(3):
+        for c in range(3):
+            lo = [slice(None)] * 3
+            hi = [slice(None)] * 3
+            lo[n] = 0
+            hi[n] = N[n] - 1
+            if c != n:
+                assert not op.vv[(c,) + tuple(lo)].any() and not op.vi[(c,) + tuple(lo)].any()
+            assert not op.vv[(c,) + tuple(hi)].any()
+            assert not op.ii[(c,) + tuple(hi)].any() and not op.iv[(c,) + tuple(hi)].any()
+
+
+def test_gaussian_signal_properties():
+    op = RefOperator(_uniform_scene(12, nrts=1000)).build()
+    fc = 10e9
+    v = op.sig_volt
+    assert op.exc_length == math.ceil(18.0 / (2 * math.pi * fc) / op.dT)
+    # the pulse is cut to the number of timesteps when that is shorter (excitation.cpp:154-159)
+    assert RefOperator(_uniform_scene(12, nrts=100)).build().exc_length == 100
+    # f0 = 0: s(0) = exp(-9); peak at t0 = 9/(2 pi fc); symmetric about t0
+    assert abs(v[0] - math.exp(-9)) <= 2e-7 * math.exp(-9) * 8
+    t0 = 9.0 / (2 * math.pi * fc)
+    assert abs(int(np.argmax(v)) - t0 / op.dT) <= 1.0
+    ts = np.arange(len(v)) * op.dT
+    mirror = np.interp(2 * t0 - ts, ts, v)
+    ok = (2 * t0 - ts) >= 0
+    np.testing.assert_allclose(v[ok], mirror[ok], atol=2e-3)
+    assert op.nyquist == nyquist_num(fc, op.dT)
+    # current signal is the same pulse half a step later
+    np.testing.assert_allclose(op.sig_curr[:-1], np.interp(ts[:-1] + 0.5 * op.dT, ts, v), atol=5e-3)
+
+
+def test_sinusoid_timestep_snapped_to_period():
+    f0 = 15e9
+    op = RefOperator(_uniform_scene(12, signal={"type": "sinus", "f0": f0})).build()
+    steps = (1 / f0) / op.dT
+    assert abs(steps - round(steps)) < 1e-9
+    assert op.exc_length == round(2.0 / f0 / op.dT)
+    assert op.sig_volt[0] == 0
+    n = np.arange(1, op.exc_length)
+    np.testing.assert_allclose(op.sig_volt[1:], np.sin(2 * np.pi * f0 * n * op.dT), atol=1e-6)
+
+
+def test_upml_grading_integrates_to_design_reflection():
+    """The default grading kappa(D) of operator_ext_upml.cpp:30 is designed so that
+    the round-trip attenuation exp(-2 Z0 * integral(kappa)) equals 1e-6."""
+    W, size = 8e-3, 8
+    dl = W / size
+    val, _ = integrate.quad(lambda D: RefOperator.grading(D, dl, W, Z0, size), 0.0, W)
+    assert abs(2 * Z0 * val - math.log(1e6)) < 1e-9 * math.log(1e6)
+    # grading grows geometrically by 2.5 per cell
+    assert abs(RefOperator.grading(2 * dl, dl, W, Z0, size) / RefOperator.grading(dl, dl, W, Z0, size) - 2.5) < 1e-12
+
+
+def test_mur_coefficient_vacuum_closed_form():
+    s = _uniform_scene(12, bc=[2] * 6)
+    op = RefOperator(s).build()
+    d = 1e-3
+    c0t = C0 * op.dT
+    want = np.float32((c0t - d) / (c0t + d))
+    for f in op.mur:
+        assert np.all(f.coeff_nyP == want) and np.all(f.coeff_nyPP == want)
+
+
+def test_quarter_cell_averaging_at_a_dielectric_interface():
+    """Half-space eps_r=4 for z >= z_k: tangential edges on the interface see the
+    area average (1+4)/2, normal edges crossing it see 4 (AverageMatQuarterCell)."""
+    n, k = 12, 6
+    s = _uniform_scene(n, materials=[{"eps": 4.0, "start": (-1, -1, k), "stop": (n, n, n + 1)}])
+    op = RefOperator(s)
+    op.calc_ec()
+    e_x = op.effmat[0][0][4, 4, k] / EPS0
+    e_z = op.effmat[2][0][4, 4, k] / EPS0
+    e_z_below = op.effmat[2][0][4, 4, k - 1] / EPS0
+    assert abs(e_x - 2.5) < 1e-12
+    assert abs(e_z - 4.0) < 1e-12
+    assert abs(e_z_below - 1.0) < 1e-12
+
+
+def _cavity_like_scene():
+    """TESTSUITE/enginetests/cavity.m-like: 27x11x33 lines, BC
+    MUR, PML_8, PMC, PEC, PEC, PEC, an eps_r=3.66 brick and a soft source."""
+    lx = np.arange(27) * 1.0
+    ly = np.arange(11) * 1.0
+    lz = np.arange(33) * 1.0
+    return {"lines": [lx, ly, lz], "delta_unit": 1e-3, "bc": [2, 3, 1, 0, 0, 0], "pml_size": [8] * 6,
+            "materials": [{"eps": 3.66, "start": (3, 2, 5), "stop": (10, 8, 20), "priority": 1}],
+            "excitations": [{"type": 0, "val": (0, 1, 1), "start": (6, 5, 12), "stop": (6, 6, 13)}],
+            "signal": {"type": "gauss", "f0": 10e9, "fc": 8e9}, "nrts": 300}
+
+
+@pytest.mark.parametrize("kind,threads", [("sse", 1), ("sse_compressed", 1), ("multithreaded", 2),
+                                          ("multithreaded", 3)])
+def test_cross_engine_identity(oracle_lib, kind, threads):
+    op = RefOperator(_cavity_like_scene()).build()
+    ref = RefEngine(op, "basic")
+    ref.iterate(300)
+    v0, c0 = ref.fields()
+    e = RefEngine(op, kind, threads)
+    e.iterate(300)
+    v1, c1 = e.fields()
+    assert np.max(np.abs(v0)) > 0
+    # bit-identical apart from flush-to-zero of subnormals in the SSE engines
+    tiny = 1e-37
+    assert np.all((v0 == v1) | ((np.abs(v0) < tiny) & (np.abs(v1) < tiny)))
+    assert np.all((c0 == c1) | ((np.abs(c0) < tiny) & (np.abs(c1) < tiny)))
+
+
+def test_compressed_index_first_seen_order(oracle_lib):
+    op = RefOperator(_cavity_like_scene()).build()
+    e = RefEngine(op, "sse_compressed")
+    idx = e.opindex().ravel()
+    nu = e.num_unique()
+    assert idx[0] == 0 and idx.max() == nu - 1
+    seen = -1
+    for v in idx:                       # a new index is always the next unused one
+        assert v <= seen + 1
+        seen = max(seen, int(v))
+
+
+def test_pec_cavity_resonance(oracle_lib):
+    """Lowest TM110 resonance of a 20x16x12 mm PEC box, c/2*sqrt(1/a^2+1/b^2),
+    within 1 % (numerical dispersion) from an E_z probe spectrum."""
+    a, b, c = 20, 16, 12
+    s = {"lines": [np.arange(a + 1) * 1.0, np.arange(b + 1) * 1.0, np.arange(c + 1) * 1.0],
+         "delta_unit": 1e-3, "bc": [0] * 6, "signal": {"type": "gauss", "f0": 12e9, "fc": 8e9},
+         "nrts": 6000,
+         "excitations": [{"type": 0, "val": (0, 0, 1), "start": (3, 4, 5), "stop": (3, 4, 6)}]}
+    op = RefOperator(s).build()
+    pr = RefProbe(op, 2, (13, 9, 6), (13, 9, 6), oversampling=4)
+    run_reference(op, 6000, probes=[pr])
+    td = pr.td_array()
+    sig = td[:, 3] - td[:, 3].mean()
+    dt = td[1, 0] - td[0, 0]
+    spec = np.abs(np.fft.rfft(sig * np.hanning(len(sig)), 1 << 17))
+    f = np.fft.rfftfreq(1 << 17, dt)
+    band = (f > 8e9) & (f < 16e9)
+    fpk = f[band][np.argmax(spec[band])]
+    f110 = C0 / 2 * math.sqrt((1 / (a * 1e-3)) ** 2 + (1 / (b * 1e-3)) ** 2)
+    assert abs(fpk - f110) / f110 < 0.01
+
+
+def test_voltage_probe_is_line_sum(oracle_lib):
+    s = scenes.pec_cavity(24, 60)
+    op = RefOperator(s).build()
+    pr = RefProbe(op, 0, (15, 15, 0), (15, 15, 23))
+    eng = run_reference(op, 60, probes=[pr])
+    v, _ = eng.raw_views()
+    assert pr.td[-1][1][0] == pytest.approx(float(np.sum(v[2, 15, 15, :23].astype(np.float64))), rel=1e-12)
+    pr2 = RefProbe(op, 0, (15, 15, 23), (15, 15, 0))          # reversed direction flips the sign
+    pr2.process(60, v, None)
+    assert pr2.td[-1][1][0] == -pr.td[-1][1][0]
