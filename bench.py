@@ -1,0 +1,170 @@
+"""Benchmark: EC-FDTD cell-updates/s on one MI355X for the 256^3 UPML workload.
+
+A step is one full timestep of the hot path (voltage + current half steps with
+the fused UPML, lumped-port source, port probes and the 256x256 E-field DFT
+plane at 3 frequencies) on synthetic input: the seeded 40-brick scene of
+openems_amd.scenes.upml_bricks.  Fields, coefficients and extension tables are
+resident in HBM before timing starts.
+
+Prints one JSON line.  value = prod(numLines) * steps / wall time (MCells/s),
+the reference's own speed definition (openems.cpp:1410,1484-1485).
+
+roofline: the dominant (voltage or current update) kernel, achieved =
+algorithmic bytes per launch / average launch duration measured with HIP
+events on the engine stream over the timed steps; algorithmic bytes per cell
+update of one half step = 15 floats (read field, 3 curl inputs read once,
+2x3 coefficients, write field) + 15 floats more in UPML positions (3 flux
+read + 3 flux write + 9 UPML coefficients).
+
+cpu_baseline: the oracle's restated multithreaded SSE-compressed engine
+(the reference's default "fastest" engine) on the same operator, on a bounded
+number of timesteps on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "MCells/s & % HBM roofline (256³ UPML); batched speedup; adjoint/forward time"
+
+
+def pml_positions(op):
+    return sum(s["num"][0] * s["num"][1] * s["num"][2] for s in op.upml_slabs())
+
+
+def traffic_from_profile(kernel):
+    """HBM bytes per launch measured by rocprofv3 PMC runs (committed summary), or None."""
+    path = os.path.join(ROOT, "profiles", "upml_bricks_256_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d["kernels"][kernel]["hbm_bytes_per_launch"]
+    except Exception:
+        return None
+
+
+def cpu_baseline(op, nthreads, budget_s=15.0):
+    """Time the oracle's multithreaded SSE-compressed engine on this operator."""
+    import numpy as np
+    from oracle.operator_ref import ExcList, UPMLSlab
+    from oracle.oracle import RefEngine
+
+    class _Op:
+        pass
+    ro = _Op()
+    ro.N = op.N
+    ro.vv, ro.vi, ro.ii, ro.iv = op.coeff("vv"), op.coeff("vi"), op.coeff("ii"), op.coeff("iv")
+    ve, ce = op.excitation(False), op.excitation(True)
+    ro.vexc = ExcList(ve["index"], ve["dir"], ve["amp"], ve["delay"])
+    ro.cexc = ExcList(ce["index"], ce["dir"], ce["amp"], ce["delay"])
+    ro.sig_volt, ro.sig_curr = op.signal()
+    ro.exc_length = op.signal_length
+    ro.period, ro.dT = op.period, op.dT
+    ro.mur = []
+    ro.upml = []
+    for s in op.upml_slabs():
+        u = UPMLSlab(s["start"], s["num"])
+        for k in ("vv", "vvfo", "vvfn", "ii", "iifo", "iifn"):
+            setattr(u, k, s[k])
+        ro.upml.append(u)
+    eng = RefEngine(ro, "multithreaded", nthreads)
+    eng.iterate(2)
+    t0 = time.perf_counter()
+    eng.iterate(4)
+    per = (time.perf_counter() - t0) / 4
+    steps = max(4, int(budget_s / max(per, 1e-6)))
+    t0 = time.perf_counter()
+    eng.iterate(steps)
+    dt = time.perf_counter() - t0
+    eng.close()
+    cells = float(np.prod(op.N))
+    return {"value": round(cells * steps / dt / 1e6, 2), "unit": "MCells/s", "cores": int(nthreads),
+            "kind": "port",
+            "sample": f"restated multithreaded sse-compressed engine (x-slab threads + barriers, "
+                      f"FTZ/DAZ, UPML hooks), {steps} timesteps of the same 256^3 UPML operator, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the engine runs on a single MI355X in this build)", file=sys.stderr)
+        sys.exit(2)
+
+    from openems_amd import scenes
+    from openems_amd.simulation import Simulation
+
+    n = args.size
+    scene = scenes.upml_bricks(n, 10000)
+    t0 = time.perf_counter()
+    sim = Simulation(scene)
+    setup_s = time.perf_counter() - t0
+    eng = sim.eng
+    op = sim.op
+    cells = float(op.num_cells)
+    npml = pml_positions(op)
+
+    eng.IterateTS(args.warmup)
+    eng.sync()
+    eng.set_profiling(True)
+    t0 = time.perf_counter()
+    eng.IterateTS(args.steps)
+    eng.sync()
+    wall = time.perf_counter() - t0
+    dev_ms, volt_ms, curr_ms = eng.last_timing()
+    eng.set_profiling(False)
+
+    value = cells * args.steps / wall / 1e6
+    bytes_per_launch = 60.0 * (cells + npml)
+    kname, kms = ("k_volt", volt_ms) if volt_ms >= curr_ms else ("k_curr", curr_ms)
+    achieved = bytes_per_launch / (kms * 1e-3) / 1e9
+    traffic = traffic_from_profile(kname)
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "MCells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"upml_bricks_{n}: {n}^3 lines, 0.5 mm core graded x1.2 over 10 cells, "
+                               "40 seeded dielectric bricks, 8-cell UPML on all faces, 50 Ohm lumped port "
+                               "with Gaussian fc=20 GHz, port V/I probes, E-field DFT plane at 5/10/15 GHz",
+                   "cells": int(cells), "pml_positions": int(npml), "layout": "plain fp32 coefficients + fused UPML",
+                   "bytes_per_cell_update": round(120.0 * (cells + npml) / cells, 2)},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(kms, 5),
+                     "volt_ms": round(volt_ms, 5), "curr_ms": round(curr_ms, 5)},
+    }
+    print(f"# setup {setup_s:.1f}s, device time {dev_ms:.1f} ms for {args.steps} steps", file=sys.stderr)
+    if not args.no_cpu_baseline:
+        try:
+            nthreads = min(16, os.cpu_count() or 1)
+            out["cpu_baseline"] = cpu_baseline(op, nthreads, args.cpu_seconds)
+        except Exception as e:  # the baseline is reported, not required
+            out["cpu_baseline"] = {"value": None, "error": str(e)}
+    sim.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1206,7 +1206,7 @@ static int launch_samples(ems_engine* h)
 
 // one timestep in the reference's hook order (Engine::IterateTS, FDTD/engine.cpp:267-286,
 // priorities: UPML > default (lumped RLC, Lorentz, Mur) > excitation)
-static void launch_step(ems_engine* h, bool time_kernels, double* vms, double* cms)
+static void launch_step(ems_engine* h, hipEvent_t* ev)
 {
     const GridDesc& g = h->g;
     const unsigned nblk = g.qtiles * g.nzc;
@@ -1219,12 +1219,12 @@ static void launch_step(ems_engine* h, bool time_kernels, double* vms, double* c
             hipLaunchKernelGGL(k_lor_pre, dim3((h->lor[k].count + T - 1) / T), dim3(T), 0, h->stream, h->V, h->cs, h->lor[k]);
     for (const auto& r : h->rlc)
         hipLaunchKernelGGL(k_rlc_pre, dim3((r.count + T - 1) / T), dim3(T), 0, h->stream, r, h->numts_dev);
-    if (time_kernels) hipEventRecord(h->kev[0], h->stream);
+    if (ev) (void)hipEventRecord(ev[0], h->stream);
     if (h->pmlV.n)
         hipLaunchKernelGGL(k_volt<true>, dim3(nblk), dim3(kBlock), 0, h->stream, g, h->V, h->I, h->vv, h->vi, h->pmlV);
     else
         hipLaunchKernelGGL(k_volt<false>, dim3(nblk), dim3(kBlock), 0, h->stream, g, h->V, h->I, h->vv, h->vi, h->pmlV);
-    if (time_kernels) hipEventRecord(h->kev[1], h->stream);
+    if (ev) (void)hipEventRecord(ev[1], h->stream);
     // -- post hooks: Mur (reverse creation order, all read-only on V)
     for (int k = (int)h->mur.size() - 1; k >= 0; --k) {
         const auto& m = h->mur[k];
@@ -1249,14 +1249,14 @@ static void launch_step(ems_engine* h, bool time_kernels, double* vms, double* c
     for (size_t k = 1; k < h->lor.size(); k += 2)
         if (h->lor_i[k / 2])
             hipLaunchKernelGGL(k_lor_pre, dim3((h->lor[k].count + T - 1) / T), dim3(T), 0, h->stream, h->I, h->cs, h->lor[k]);
-    if (time_kernels) hipEventRecord(h->kev[2], h->stream);
+    if (ev) (void)hipEventRecord(ev[2], h->stream);
     if (h->pmlI.n)
         hipLaunchKernelGGL(k_curr<true>, dim3(nblk), dim3(kBlock), 0, h->stream, g, h->I, h->V, h->ii, h->iv, h->pmlI,
                            h->numts_dev);
     else
         hipLaunchKernelGGL(k_curr<false>, dim3(nblk), dim3(kBlock), 0, h->stream, g, h->I, h->V, h->ii, h->iv, h->pmlI,
                            h->numts_dev);
-    if (time_kernels) hipEventRecord(h->kev[3], h->stream);
+    if (ev) (void)hipEventRecord(ev[3], h->stream);
     for (size_t k = 1; k < h->lor.size(); k += 2)
         if (h->lor_i[k / 2])
             hipLaunchKernelGGL(k_lor_apply, dim3((h->lor[k].count + T - 1) / T), dim3(T), 0, h->stream, h->I, h->cs, h->lor[k]);
@@ -1265,20 +1265,17 @@ static void launch_step(ems_engine* h, bool time_kernels, double* vms, double* c
         hipLaunchKernelGGL(k_excite, dim3((e.K + T - 1) / T), dim3(T), 0, h->stream, h->I, h->cs, e.K, e.lin, e.dir,
                            e.amp, e.delay, h->sig[1], h->sigLen, h->period, h->numts_dev, -1);
     }
-    (void)vms;
-    (void)cms;
 }
 
 int ems_set_profiling(ems_engine* h, int enable)
 {
     if (!h) return fail(EMS_E_ARG, "null handle");
     h->profile = enable != 0;
-    if (h->profile && h->kev.empty()) {
-        h->kev.resize(4);
-        for (auto& e : h->kev) HIPCHK(hipEventCreate(&e));
-    }
     return 0;
 }
+
+// steps of one ems_iterate call whose kernels are bracketed by events
+constexpr unsigned kProfSteps = 4096;
 
 int ems_iterate(ems_engine* h, unsigned nsteps)
 {
@@ -1287,39 +1284,41 @@ int ems_iterate(ems_engine* h, unsigned nsteps)
     HIPCHK(hipSetDevice(h->device));
     int rc = sync_probe_tables(h);
     if (rc) return rc;
+    const unsigned nprof = h->profile ? std::min(nsteps, kProfSteps) : 0;
+    while (h->kev.size() < 4 * (size_t)nprof) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        h->kev.push_back(e);
+    }
     if (h->numTS == 0 && nsteps > 0) {
         // Process() before the first step (openems.cpp:1424-1425)
         launch_samples(h);
     }
     HIPCHK(hipEventRecord(h->ev0, h->stream));
-    double vsum = 0, csum = 0;
-    unsigned timed = 0;
     for (unsigned s = 0; s < nsteps; ++s) {
-        const bool tk = h->profile;
-        launch_step(h, tk, nullptr, nullptr);
+        launch_step(h, s < nprof ? &h->kev[4 * (size_t)s] : nullptr);
         ++h->numTS;
         launch_samples(h);
-        if (tk) {
-            HIPCHK(hipEventSynchronize(h->kev[3]));
-            float a = 0, b = 0;
-            hipEventElapsedTime(&a, h->kev[0], h->kev[1]);
-            hipEventElapsedTime(&b, h->kev[2], h->kev[3]);
-            vsum += a;
-            csum += b;
-            ++timed;
-        }
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(h->ev1, h->stream));
-    if (!h->async || h->profile) {
+    if (!h->async || nprof) {
         HIPCHK(hipEventSynchronize(h->ev1));
         float ms = 0;
-        hipEventElapsedTime(&ms, h->ev0, h->ev1);
+        HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
         h->last_ms = ms;
     }
-    if (timed) {
-        h->last_volt_ms = vsum / timed;
-        h->last_curr_ms = csum / timed;
+    if (nprof) {
+        double vsum = 0, csum = 0;
+        for (unsigned s = 0; s < nprof; ++s) {
+            float a = 0, b = 0;
+            HIPCHK(hipEventElapsedTime(&a, h->kev[4 * (size_t)s], h->kev[4 * (size_t)s + 1]));
+            HIPCHK(hipEventElapsedTime(&b, h->kev[4 * (size_t)s + 2], h->kev[4 * (size_t)s + 3]));
+            vsum += a;
+            csum += b;
+        }
+        h->last_volt_ms = vsum / nprof;
+        h->last_curr_ms = csum / nprof;
     }
     return 0;
 }

@@ -32,11 +32,16 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <functional>
 #include <limits>
+#include <memory>
+#include <sys/mman.h>
 #include <new>
 #include <string>
 #include <thread>
@@ -117,6 +122,39 @@ void parallel_for(unsigned n, int nthreads, const std::function<void(unsigned, u
     for (auto& t : th) t.join();
 }
 
+// uninitialised array: the big operator arrays are first touched by the
+// threads that fill them instead of a serial zero fill
+template <typename T>
+struct PodVec {
+    struct Free {
+        void operator()(T* q) const { std::free(q); }
+    };
+    std::unique_ptr<T[], Free> p;
+    size_t n = 0;
+    void alloc(size_t count)
+    {
+        // 2 MiB aligned and backed by transparent huge pages where the kernel allows it
+        const size_t align = size_t(2) << 20;
+        const size_t bytes = (std::max<size_t>(count, 1) * sizeof(T) + align - 1) / align * align;
+        void* q = nullptr;
+        if (posix_memalign(&q, align, bytes) != 0) throw std::bad_alloc();
+        madvise(q, bytes, MADV_HUGEPAGE);
+        p.reset(static_cast<T*>(q));
+        n = count;
+    }
+    void clear()
+    {
+        p.reset();
+        n = 0;
+    }
+    T& operator[](size_t i) { return p[i]; }
+    const T& operator[](size_t i) const { return p[i]; }
+    T* data() { return p.get(); }
+    const T* data() const { return p.get(); }
+    size_t size() const { return n; }
+    bool empty() const { return n == 0; }
+};
+
 // ranked box set with per-axis membership masks
 template <typename B>
 struct Ranked {
@@ -185,8 +223,8 @@ struct emsop {
     double period = 0;
     unsigned sigLen = 0;
     std::vector<float> sigV, sigC;
-    std::vector<float> coeff[4];  // vv vi ii iv
-    std::vector<float> ec[4];     // C G L R
+    PodVec<float> coeff[4];  // vv vi ii iv
+    PodVec<float> ec[4];     // C G L R
     // material sampling tables, one set per field component
     Ranked<MatBox> matrk;
     struct Samp {
@@ -438,7 +476,7 @@ void emsop::effmat_at(int ny, const unsigned* p, double* E) const
 int emsop::calc_ec()
 {
     const size_t tot = size();
-    for (int k = 0; k < 4; ++k) ec[k].assign(3 * tot, 0.0f);
+    for (int k = 0; k < 4; ++k) ec[k].alloc(3 * tot);
     prepare_sampling();
     std::vector<int> bad(N[0], 0);
     for (int ny = 0; ny < 3; ++ny) {
@@ -459,12 +497,18 @@ int emsop::calc_ec()
                         if (delta) {
                             ec[0][q] = (float)(E[0] * area / delta);
                             ec[1][q] = (float)(E[1] * area / delta);
+                        } else {
+                            ec[0][q] = 0.0f;
+                            ec[1][q] = 0.0f;
                         }
                         delta = EL(ny, p[ny], true);
                         area = EL(nP, p[nP], false) * EL(nPP, p[nPP], false);
                         if (delta) {
                             ec[2][q] = (float)(E[2] * area / delta);
                             ec[3][q] = (float)(E[3] * area / delta);
+                        } else {
+                            ec[2][q] = 0.0f;
+                            ec[3][q] = 0.0f;
                         }
                     }
         });
@@ -479,8 +523,8 @@ int emsop::calc_ec()
 // ---------------------------------------------------------------------------
 double emsop::timestep_var3() const
 {
-    const std::vector<float>& C = ec[0];
-    const std::vector<float>& L = ec[2];
+    const PodVec<float>& C = ec[0];
+    const PodVec<float>& L = ec[2];
     std::vector<double> best(N[2], 1e200);
     parallel_for(N[2], nthreads, [&](unsigned z0, unsigned z1) {
         for (int n = 0; n < 3; ++n) {
@@ -542,8 +586,8 @@ double emsop::timestep_var3() const
 
 double emsop::timestep_var1() const
 {
-    const std::vector<float>& C = ec[0];
-    const std::vector<float>& L = ec[2];
+    const PodVec<float>& C = ec[0];
+    const PodVec<float>& L = ec[2];
     double dt = 1e200;
     unsigned p[3];
     for (int n = 0; n < 3; ++n) {
@@ -1102,6 +1146,14 @@ int emsop::build_signal(unsigned nrts)
 // ---------------------------------------------------------------------------
 int emsop::build(unsigned nrts)
 {
+    const bool verbose = std::getenv("EMSOP_VERBOSE") != nullptr;
+    auto t0 = std::chrono::steady_clock::now();
+    auto lap = [&](const char* what) {
+        if (!verbose) return;
+        auto t1 = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "emsop: %-18s %8.3f s\n", what, std::chrono::duration<double>(t1 - t0).count());
+        t0 = t1;
+    };
     for (int n = 0; n < 3; ++n)
         if (N[n] < 3) return fail(EMSOP_E_ARG, "at least 3 mesh lines per direction are required");
     if (sigType < 0) return fail(EMSOP_E_STATE, "excitation signal not defined");
@@ -1109,7 +1161,9 @@ int emsop::build(unsigned nrts)
     try {
         int rc = calc_ec();
         if (rc) return rc;
+        lap("calc_ec");
         double dt_auto = (tsMethod == 3) ? timestep_var3() : timestep_var1();
+        lap("timestep");
         if (forcedDt > 0)
             dT = forcedDt;
         else
@@ -1122,7 +1176,7 @@ int emsop::build(unsigned nrts)
             dT = period / TS;
         }
         const size_t tot = 3 * size();
-        for (int k = 0; k < 4; ++k) coeff[k].assign(tot, 0.0f);
+        for (int k = 0; k < 4; ++k) coeff[k].alloc(tot);  // every entry is written below
         parallel_for(3 * N[0], nthreads, [&](unsigned a, unsigned b) {
             for (unsigned nx = a; nx < b; ++nx) {
                 int n = nx / N[0];
@@ -1131,16 +1185,21 @@ int emsop::build(unsigned nrts)
                     for (unsigned z = 0; z < N[2]; ++z) ec_to_coeff(n, idx(n, x, y, z));
             }
         });
+        lap("coefficients");
         apply_bc();
         calc_pec();
         lumped_rc();
         emsop_apply_magnetic_bc(this);
+        lap("bc/pec/lumped");
         rc = build_signal(nrts);
         if (rc) return rc;
         // extensions in the reference's creation order (openems.cpp:1185-1243)
         build_excitation();
+        lap("excitation");
         build_mur();
+        lap("mur");
         build_upml();
+        lap("upml");
     } catch (const std::bad_alloc&) {
         return fail(EMSOP_E_NOMEM, "out of host memory while building the operator");
     }
@@ -1462,9 +1521,7 @@ int emsop_snap_box(const emsop* h, const double start[3], const double stop[3], 
 void emsop_release_build_arrays(emsop* h)
 {
     if (!h) return;
-    for (int k = 0; k < 4; ++k) {
-        std::vector<float>().swap(h->ec[k]);
-    }
+    for (int k = 0; k < 4; ++k) h->ec[k].clear();
 }
 
 }  // extern "C"

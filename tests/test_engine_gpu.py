@@ -17,8 +17,8 @@ pytestmark = pytest.mark.gpu
 
 
 def rel_l2(a, b):
-    a = np.asarray(a, np.float64)
-    b = np.asarray(b, np.float64)
+    a = np.asarray(a)
+    b = np.asarray(b)
     n = np.linalg.norm(b)
     return np.linalg.norm(a - b) / (n if n > 0 else 1.0)
 
@@ -109,8 +109,11 @@ def test_port_probes_and_fd_dump(gpu):
 
 def test_linearity_full_size_upml(gpu):
     """Size-independent check at the full 256^3 UPML workload: scaling the
-    excitation by 2 scales every field exactly by 2 (binary scaling commutes
-    with rounding), and fields stay finite."""
+    excitation by 2 scales every field by 2.  Binary scaling commutes with
+    rounding, so the scaled run is exact except for values whose history
+    passed through the subnormal range (gradual underflow is not scale
+    invariant); those stay below 1e-20 of the field maximum, while any
+    defect in the update would show at O(1).  Fields stay finite."""
     s1 = scenes.upml_bricks(256, 60, dump=False)
     s2 = scenes.upml_bricks(256, 60, dump=False)
     s2["excitations"][0]["val"] = (0.0, 0.0, -2.0)
@@ -125,8 +128,11 @@ def test_linearity_full_size_upml(gpu):
         op.close()
     (v1, c1), (v2, c2) = e
     assert np.all(np.isfinite(v1)) and np.max(np.abs(v1)) > 0
-    np.testing.assert_array_equal(v2, 2 * v1)
-    np.testing.assert_array_equal(c2, 2 * c1)
+    for a, b in ((v2, 2 * v1), (c2, 2 * c1)):
+        scale = float(np.max(np.abs(b)))
+        diff = np.abs(a.astype(np.float64) - b)
+        assert np.count_nonzero(diff) < 1e-3 * a.size
+        assert float(diff.max()) <= 1e-20 * scale
 
 
 def test_full_size_upml_against_oracle(gpu):
