@@ -1,0 +1,100 @@
+"""The product's host operator builder (C++, include/ems_operator.h) against the
+oracle's numpy restatement: every float32 coefficient, EC value, timestep,
+excitation list, UPML slab and Mur coefficient must be bit-identical.  CPU only."""
+import numpy as np
+import pytest
+
+from openems_amd import scenes
+from openems_amd.operator import Operator
+from oracle.operator_ref import RefOperator
+
+
+def _compare(scene):
+    op = Operator(scene, nthreads=4)
+    ref = RefOperator(scene).build()
+    assert op.N == ref.N
+    assert op.dT == ref.dT
+    assert op.nyquist == ref.nyquist
+    for name, arr in zip("CGLR", ref.EC):
+        assert np.array_equal(op.ec(name), arr), f"EC {name}"
+    for name in ("vv", "vi", "ii", "iv"):
+        got, want = op.coeff(name), getattr(ref, name)
+        assert np.array_equal(got, want), f"{name}: {np.count_nonzero(got != want)} mismatches"
+    sv, sc = op.signal()
+    assert np.array_equal(sv, ref.sig_volt) and np.array_equal(sc, ref.sig_curr)
+    for curr, lst in ((False, ref.vexc), (True, ref.cexc)):
+        e = op.excitation(curr)
+        assert np.array_equal(e["index"], lst.index)
+        assert np.array_equal(e["dir"], lst.dir)
+        assert np.array_equal(e["amp"], lst.amp)
+        assert np.array_equal(e["delay"], lst.delay)
+    slabs = op.upml_slabs()
+    assert len(slabs) == len(ref.upml)
+    for a, b in zip(slabs, ref.upml):
+        assert tuple(a["start"]) == tuple(b.start) and tuple(a["num"]) == tuple(b.num)
+        for k in ("vv", "vvfo", "vvfn", "ii", "iifo", "iifn"):
+            assert np.array_equal(a[k], getattr(b, k)), f"UPML {k}"
+    faces = op.mur_faces()
+    assert len(faces) == len(ref.mur)
+    for a, b in zip(faces, ref.mur):
+        assert (a["ny"], a["line"], a["line_shift"], a["start_ts"]) == (b.ny, b.line, b.line_shift, b.start_ts)
+        assert np.array_equal(a["coeff_nyP"], b.coeff_nyP) and np.array_equal(a["coeff_nyPP"], b.coeff_nyPP)
+    return op, ref
+
+
+def test_pec_cavity(native):
+    _compare(scenes.pec_cavity(24, 100))
+
+
+def test_upml_bricks_lumped_port(native):
+    op, ref = _compare(scenes.upml_bricks(40, 300, nbricks=8))
+    assert len(op.upml_slabs()) == 6
+    assert op.excitation()["dir"].size == 16          # 2x2 edge columns x 4 cells of the port box
+
+
+def test_mur_layers(native):
+    op, _ = _compare(scenes.mur_layers(40, 36, 32, 100))
+    assert len(op.mur_faces()) == 6
+
+
+def test_mixed_boundaries_and_materials(native):
+    """TESTSUITE/enginetests/cavity.m boundary mix (MUR, PML_8, PMC, PEC, PEC, PEC),
+    an anisotropic lossy brick, a magnetic-loss brick, a metal plate and a hard source."""
+    l = [np.arange(27) * 1.0, np.arange(11) * 1.5, np.arange(33) * 0.75]
+    s = {"lines": l, "delta_unit": 1e-3, "bc": [2, 3, 1, 0, 0, 0], "pml_size": [8] * 6,
+         "materials": [{"eps": (3.66, 2.0, 4.0), "kappa": (0.01, 0.0, 0.2), "start": (3, 1.5, 3.75),
+                        "stop": (10, 12, 15), "priority": 2},
+                       {"eps": 2.0, "mue": 1.5, "sigma": 30.0, "start": (12, 3, 6), "stop": (16, 9, 12), "priority": 1}],
+         "metals": [{"start": (4, 4.5, 18), "stop": (8, 9, 18), "priority": 5}],
+         "excitations": [{"type": 1, "val": (0, 1, 1), "start": (6, 7.5, 9), "stop": (6, 9, 9.75)},
+                         {"type": 2, "val": (1, 0, 0), "delay": 3e-12, "start": (14, 3.7, 14.6), "stop": (15, 5.3, 15.4)}],
+         "signal": {"type": "gauss", "f0": 10e9, "fc": 8e9}, "nrts": 400}
+    op, ref = _compare(s)
+    assert op.excitation(True)["dir"].size > 0
+
+
+@pytest.mark.parametrize("extra", [{"ts_method": 1}, {"cell_constant_material": True},
+                                   {"ts_factor": 0.8}, {"signal": {"type": "sinus", "f0": 12e9}},
+                                   {"signal": {"type": "step"}}])
+def test_operator_options(native, extra):
+    s = scenes.upml_bricks(32, 200, nbricks=5)
+    s.update(extra)
+    _compare(s)
+
+
+def test_graded_full_size_256(native):
+    """The full 256^3 UPML workload's operator (bit-identical)."""
+    _compare(scenes.upml_bricks(256, 10000, dump=False))
+
+
+def test_snapping_matches_oracle(native):
+    s = scenes.upml_bricks(40, 100, nbricks=3)
+    op = Operator(s)
+    ref = RefOperator(s)
+    rng = np.random.default_rng(7)
+    lo = [l[0] - 1 for l in s["lines"]]
+    hi = [l[-1] + 1 for l in s["lines"]]
+    for _ in range(200):
+        c = [rng.uniform(lo[n], hi[n]) for n in range(3)]
+        for dual in (False, True):
+            assert op.snap_to_mesh(c, dual) == ref.mesh.snap(c, dual)
