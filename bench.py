@@ -37,15 +37,22 @@ def pml_positions(op):
     return sum(s["num"][0] * s["num"][1] * s["num"][2] for s in op.upml_slabs())
 
 
-def traffic_from_profile(kernel):
-    """HBM bytes per launch measured by rocprofv3 PMC runs (committed summary), or None."""
-    path = os.path.join(ROOT, "profiles", "upml_bricks_256_traffic.json")
-    try:
-        with open(path) as f:
-            d = json.load(f)
-        return d["kernels"][kernel]["hbm_bytes_per_launch"]
-    except Exception:
-        return None
+def traffic_from_profile(kernel, layout, size):
+    """HBM bytes per launch of `kernel` measured by rocprofv3 FETCH_SIZE / WRITE_SIZE
+    runs of this workload and coefficient layout (committed summary), or None."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", f"upml_bricks_{size}_traffic*.json"))):
+        try:
+            with open(path) as f:
+                d = json.load(f)
+        except (OSError, ValueError):
+            continue
+        if d.get("layout") != layout:
+            continue
+        for name, k in d.get("kernels", {}).items():
+            if name.split("<")[0] == kernel:
+                return k["hbm_bytes_per_launch"]
+    return None
 
 
 def cpu_baseline(op, nthreads, budget_s=15.0):
@@ -89,6 +96,20 @@ def cpu_baseline(op, nthreads, budget_s=15.0):
                       f"FTZ/DAZ, UPML hooks), {steps} timesteps of the same 256^3 UPML operator, {dt:.1f} s"}
 
 
+def copy_calibration(reps=20, nbytes=1 << 30):
+    """A plain device copy of known size (torch's copy kernel, nbytes read + nbytes
+    written per launch): the reference point that scales rocprofv3's FETCH_SIZE and
+    WRITE_SIZE counters to bytes."""
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes,
+                                           "launches": reps}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,7 +118,12 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (for scaling HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the engine runs on a single MI355X in this build)", file=sys.stderr)
@@ -130,7 +156,8 @@ def main():
     bytes_per_launch = 60.0 * (cells + npml)
     kname, kms = ("k_volt", volt_ms) if volt_ms >= curr_ms else ("k_curr", curr_ms)
     achieved = bytes_per_launch / (kms * 1e-3) / 1e9
-    traffic = traffic_from_profile(kname)
+    layout = "plain fp32 coefficients + fused UPML"
+    traffic = traffic_from_profile(kname, layout, n)
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -147,7 +174,7 @@ def main():
         "config": {"workload": f"upml_bricks_{n}: {n}^3 lines, 0.5 mm core graded x1.2 over 10 cells, "
                                "40 seeded dielectric bricks, 8-cell UPML on all faces, 50 Ohm lumped port "
                                "with Gaussian fc=20 GHz, port V/I probes, E-field DFT plane at 5/10/15 GHz",
-                   "cells": int(cells), "pml_positions": int(npml), "layout": "plain fp32 coefficients + fused UPML",
+                   "cells": int(cells), "pml_positions": int(npml), "layout": layout,
                    "bytes_per_cell_update": round(120.0 * (cells + npml) / cells, 2)},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

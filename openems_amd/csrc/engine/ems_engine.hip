@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <string>
@@ -54,19 +55,20 @@ struct GridDesc {
     unsigned qtiles;    // ceil(plane / kBlock)
 };
 
-struct SlabDev {
-    int x0, x1, y0, y1, z0, z1;  // inclusive ranges
+// One UPML slab (Operator_Ext_UPML range) in the device table.  All slabs of
+// one half step share one buffer; a slab's block holds twelve arrays
+// [c0, cfo, cfn, flux][component][local index] with the local index x-fastest.
+struct SlabTab {
+    int x0, x1, y0, y1, z0, z1;    // inclusive mesh ranges
     unsigned nx, ny;
-    unsigned long long cs;       // component stride inside the slab
-    float* c0;                   // vv / ii        [3][cs]
-    float* cfo;                  // vvfo / iifo
-    float* cfn;                  // vvfn / iifn
-    float* flux;                 // volt_flux / curr_flux
+    unsigned long long off;        // element offset of the slab block
+    unsigned long long cs;         // nx*ny*nz
 };
 
 struct PmlArgs {
-    int n;
-    SlabDev s[kMaxSlabs];
+    int n;                         // number of slabs (0: no UPML)
+    const SlabTab* tab;            // [n]
+    float* buf;                    // slab blocks
 };
 
 // Workgroups are dispatched round-robin over the XCDs by linear block id;
@@ -79,6 +81,99 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb)
     return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
 }
 
+// UPML membership of one (x, y) column: at most two slabs of the
+// Create_UPML layout share a column (lower / upper slab of one direction).
+struct Column {
+    long long base0, base1;        // slab block offset + local index of (x, y, slab z0)
+    int z00, z10, z01, z11;        // z ranges of the two slabs (empty: z0 > z1)
+    unsigned long long plane0, plane1, cs0, cs1;
+};
+
+__device__ __forceinline__ void column_slabs(const PmlArgs& pml, unsigned x, unsigned y, Column& c)
+{
+    c.base0 = c.base1 = 0;
+    c.z00 = c.z01 = 1;
+    c.z10 = c.z11 = 0;
+    c.plane0 = c.plane1 = c.cs0 = c.cs1 = 0;
+    bool first = true;
+    for (int s = 0; s < pml.n; ++s) {
+        const SlabTab S = pml.tab[s];
+        if ((int)x >= S.x0 && (int)x <= S.x1 && (int)y >= S.y0 && (int)y <= S.y1) {
+            const long long b = (long long)S.off + (long long)(x - S.x0) + (long long)S.nx * (long long)(y - S.y0);
+            const unsigned long long pl = (unsigned long long)S.nx * S.ny;
+            if (first) {
+                c.base0 = b; c.z00 = S.z0; c.z10 = S.z1; c.plane0 = pl; c.cs0 = S.cs;
+                first = false;
+            } else {
+                c.base1 = b; c.z01 = S.z0; c.z11 = S.z1; c.plane1 = pl; c.cs1 = S.cs;
+            }
+        }
+    }
+}
+
+// slab-local index of plane z, or false when the cell is outside the UPML
+__device__ __forceinline__ bool column_at(const Column& c, int z, unsigned long long& li, unsigned long long& scs)
+{
+    const bool a = z >= c.z00 && z <= c.z10;
+    const bool b = z >= c.z01 && z <= c.z11;
+    const long long base = a ? c.base0 : c.base1;
+    const unsigned long long pl = a ? c.plane0 : c.plane1;
+    const int zs = a ? c.z00 : c.z01;
+    scs = a ? c.cs0 : c.cs1;
+    li = (unsigned long long)(base + (long long)pl * (z - zs));
+    return a || b;
+}
+
+// Fused half-step update of the three components of one cell.
+//   plain: X = field;  f = X*cc + cv*curl;  field <- f
+//   UPML : X = flux;   f = X*cc + cv*curl;  field <- (c0*field - cfo*flux) + cfn*f;  flux <- f
+// The loads shared by both forms are issued once for every lane; the UPML
+// lanes add four exec-masked loads per component.
+__device__ __forceinline__ void cell_update(float* __restrict__ F, const float* __restrict__ cc,
+                                            const float* __restrict__ cv, unsigned long long cs,
+                                            unsigned long long p, const float* curl, bool pml,
+                                            float* __restrict__ pbuf, unsigned long long li,
+                                            unsigned long long scs)
+{
+    float x0[3], a[3], b[3];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+        x0[n] = F[n * cs + p];
+        a[n] = cc[n * cs + p];
+        b[n] = cv[n * cs + p];
+    }
+    float k0[3], ko[3], kn[3], fl[3];
+    if (pml) {
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            k0[n] = pbuf[li + n * scs];
+            ko[n] = pbuf[li + (3 + n) * scs];
+            kn[n] = pbuf[li + (6 + n) * scs];
+            fl[n] = pbuf[li + (9 + n) * scs];
+        }
+    }
+    float out[3], fnew[3];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+        const float xin = pml ? fl[n] : x0[n];
+        float f = xin * a[n];
+        f = f + b[n] * curl[n];
+        fnew[n] = f;
+        if (pml) {
+            const float t = k0[n] * x0[n] - ko[n] * fl[n];
+            out[n] = t + kn[n] * f;
+        } else {
+            out[n] = f;
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < 3; ++n) F[n * cs + p] = out[n];
+    if (pml) {
+#pragma unroll
+        for (int n = 0; n < 3; ++n) pbuf[li + (9 + n) * scs] = fnew[n];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // voltage half step (Engine::UpdateVoltages, FDTD/engine.cpp:110-168) with the
 // UPML pre/post hooks of Engine_Ext_UPML (engine_ext_upml.cpp:52-124) fused in:
@@ -86,6 +181,8 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb)
 //   main: V <- V*vv + vi*curl    (computed on the flux)
 //   post: F <- V;  V <- t + vvfn*F
 // The lower neighbour at index 0 is the cell itself (shift = bool(pos)).
+// Each thread owns one (x, y) column and sweeps g.zdepth planes, carrying the
+// z-1 currents in registers.
 // ---------------------------------------------------------------------------
 template <bool PML>
 __global__ __launch_bounds__(kBlock) void k_volt(const GridDesc g, float* __restrict__ V, const float* __restrict__ I,
@@ -110,19 +207,8 @@ __global__ __launch_bounds__(kBlock) void k_volt(const GridDesc g, float* __rest
     const unsigned long long pz = z0 ? p - g.plane : p;
     float Ix_zm = Ix[pz], Iy_zm = Iy[pz];
 
-    // candidate UPML slabs of this (x, y) column
-    int cand0 = -1, cand1 = -1;
-    if (PML) {
-        for (int s = 0; s < pml.n; ++s) {
-            const SlabDev& S = pml.s[s];
-            if ((int)x >= S.x0 && (int)x <= S.x1 && (int)y >= S.y0 && (int)y <= S.y1) {
-                if (cand0 < 0)
-                    cand0 = s;
-                else if (cand1 < 0)
-                    cand1 = s;
-            }
-        }
-    }
+    Column col;
+    if (PML) column_slabs(pml, x, y, col);
 
     for (unsigned z = z0; z < z1; ++z, p += g.plane) {
         const float ix = Ix[p], iy = Iy[p], iz = Iz[p];
@@ -134,40 +220,10 @@ __global__ __launch_bounds__(kBlock) void k_volt(const GridDesc g, float* __rest
         curl[2] = iy - iy_xm - ix + ix_ym;
         Ix_zm = ix;
         Iy_zm = iy;
-
-        int slab = -1;
-        if (PML) {
-            if (cand0 >= 0 && (int)z >= pml.s[cand0].z0 && (int)z <= pml.s[cand0].z1)
-                slab = cand0;
-            else if (cand1 >= 0 && (int)z >= pml.s[cand1].z0 && (int)z <= pml.s[cand1].z1)
-                slab = cand1;
-        }
-        if (!PML || slab < 0) {
-#pragma unroll
-            for (int n = 0; n < 3; ++n) {
-                const unsigned long long i = n * cs + p;
-                float v = V[i];
-                v = v * vv[i];
-                v = v + vi[i] * curl[n];
-                V[i] = v;
-            }
-        } else {
-            const SlabDev& S = pml.s[slab];
-            const unsigned long long l = (unsigned long long)(x - S.x0) +
-                                         (unsigned long long)S.nx * ((y - S.y0) + (unsigned long long)S.ny * (z - S.z0));
-#pragma unroll
-            for (int n = 0; n < 3; ++n) {
-                const unsigned long long i = n * cs + p;
-                const unsigned long long li = n * S.cs + l;
-                const float v0 = V[i];
-                const float f0 = S.flux[li];
-                const float tpre = S.c0[li] * v0 - S.cfo[li] * f0;
-                float f = f0 * vv[i];
-                f = f + vi[i] * curl[n];
-                V[i] = tpre + S.cfn[li] * f;
-                S.flux[li] = f;
-            }
-        }
+        bool in_pml = false;
+        unsigned long long li = 0, scs = 0;
+        if (PML) in_pml = column_at(col, (int)z, li, scs);
+        cell_update(V, vv, vi, cs, p, curl, in_pml, pml.buf, li, scs);
     }
 }
 
@@ -203,18 +259,8 @@ __global__ __launch_bounds__(kBlock) void k_curr(const GridDesc g, float* __rest
     // values at z carried from the previous iteration's z+1 loads
     float vx = Vx[p], vy = Vy[p];
 
-    int cand0 = -1, cand1 = -1;
-    if (PML) {
-        for (int s = 0; s < pml.n; ++s) {
-            const SlabDev& S = pml.s[s];
-            if ((int)x >= S.x0 && (int)x <= S.x1 && (int)y >= S.y0 && (int)y <= S.y1) {
-                if (cand0 < 0)
-                    cand0 = s;
-                else if (cand1 < 0)
-                    cand1 = s;
-            }
-        }
-    }
+    Column col;
+    if (PML) column_slabs(pml, x, y, col);
 
     for (unsigned z = z0; z < z1; ++z, p += g.plane) {
         const unsigned long long pzp = p + g.plane;
@@ -228,40 +274,10 @@ __global__ __launch_bounds__(kBlock) void k_curr(const GridDesc g, float* __rest
         curl[2] = vy - vy_xp - vx + vx_yp;
         vx = vx_zp;
         vy = vy_zp;
-
-        int slab = -1;
-        if (PML) {
-            if (cand0 >= 0 && (int)z >= pml.s[cand0].z0 && (int)z <= pml.s[cand0].z1)
-                slab = cand0;
-            else if (cand1 >= 0 && (int)z >= pml.s[cand1].z0 && (int)z <= pml.s[cand1].z1)
-                slab = cand1;
-        }
-        if (!PML || slab < 0) {
-#pragma unroll
-            for (int n = 0; n < 3; ++n) {
-                const unsigned long long i = n * cs + p;
-                float c = I[i];
-                c = c * ii[i];
-                c = c + iv[i] * curl[n];
-                I[i] = c;
-            }
-        } else {
-            const SlabDev& S = pml.s[slab];
-            const unsigned long long l = (unsigned long long)(x - S.x0) +
-                                         (unsigned long long)S.nx * ((y - S.y0) + (unsigned long long)S.ny * (z - S.z0));
-#pragma unroll
-            for (int n = 0; n < 3; ++n) {
-                const unsigned long long i = n * cs + p;
-                const unsigned long long li = n * S.cs + l;
-                const float c0 = I[i];
-                const float f0 = S.flux[li];
-                const float tpre = S.c0[li] * c0 - S.cfo[li] * f0;
-                float f = f0 * ii[i];
-                f = f + iv[i] * curl[n];
-                I[i] = tpre + S.cfn[li] * f;
-                S.flux[li] = f;
-            }
-        }
+        bool in_pml = false;
+        unsigned long long li = 0, scs = 0;
+        if (PML) in_pml = column_at(col, (int)z, li, scs);
+        cell_update(I, ii, iv, cs, p, curl, in_pml, pml.buf, li, scs);
     }
 }
 
@@ -643,7 +659,12 @@ struct ems_engine {
     int period = 0;
     bool has_exc = false;
 
-    // UPML
+    // UPML: host staging per slab until init, then one device buffer per half step
+    struct SlabHost {
+        SlabTab tab;
+        std::vector<float> v[3], i[3];   // vv, vvfo, vvfn / ii, iifo, iifn, [3][cs] x-fastest
+    };
+    std::vector<SlabHost> slabs;
     PmlArgs pmlV{}, pmlI{};
     // Mur faces in creation order
     std::vector<MurDev> mur;
@@ -759,11 +780,13 @@ int upload(ems_engine* h, T** d, const T* src, size_t n)
 
 unsigned choose_zdepth(const unsigned* N)
 {
-    // enough blocks to fill 256 CUs several times over; at least 4 planes per sweep
+    // z-sweep depth: 8 planes per thread measured best on MI355X for 256^3
+    // (4: +1 %, 16: +3 %, 32: +8 % step time); shallower sweeps when the grid
+    // would otherwise give fewer than 2048 workgroups for 256 CUs
     const unsigned plane = N[0] * N[1];
     const unsigned qt = (plane + kBlock - 1) / kBlock;
-    unsigned d = 32;
-    while (d > 4 && (unsigned long long)qt * ((N[2] + d - 1) / d) < 256ull * 16) d /= 2;
+    unsigned d = 8;
+    while (d > 1 && (unsigned long long)qt * ((N[2] + d - 1) / d) < 2048ull) d /= 2;
     return d;
 }
 
@@ -809,6 +832,10 @@ int ems_create(ems_engine** out, const unsigned numLines[3], const float* vv, co
     g.plane = (unsigned)plane;
     g.cs = h->cs;
     g.zdepth = choose_zdepth(h->N);
+    if (const char* zd = std::getenv("EMS_ZDEPTH")) {
+        const int v = std::atoi(zd);
+        if (v >= 1 && v <= 1024) g.zdepth = (unsigned)v;
+    }
     g.nzc = (g.Nz + g.zdepth - 1) / g.zdepth;
     g.qtiles = (g.plane + kBlock - 1) / kBlock;
     int rc = 0;
@@ -878,59 +905,84 @@ int ems_add_upml(ems_engine* h, const unsigned start[3], const unsigned num[3], 
 {
     if (!h || !start || !num || !vv || !vvfo || !vvfn || !ii || !iifo || !iifn) return fail(EMS_E_ARG, "null argument");
     if (h->inited) return fail(EMS_E_STATE, "engine already initialised");
-    if (h->pmlV.n >= (int)kMaxSlabs) return fail(EMS_E_ARG, "at most 6 UPML slabs");
+    if (h->slabs.size() >= kMaxSlabs) return fail(EMS_E_ARG, "at most 6 UPML slabs");
     for (int n = 0; n < 3; ++n)
         if (num[n] == 0 || start[n] + num[n] > h->N[n]) return fail(EMS_E_ARG, "UPML slab outside the mesh");
-    for (int s = 0; s < h->pmlV.n; ++s) {
-        const SlabDev& o = h->pmlV.s[s];
-        bool overlap = (int)start[0] <= o.x1 && (int)(start[0] + num[0] - 1) >= o.x0 && (int)start[1] <= o.y1 &&
-                       (int)(start[1] + num[1] - 1) >= o.y0 && (int)start[2] <= o.z1 && (int)(start[2] + num[2] - 1) >= o.z0;
+    SlabTab t{};
+    t.x0 = start[0];
+    t.y0 = start[1];
+    t.z0 = start[2];
+    t.x1 = start[0] + num[0] - 1;
+    t.y1 = start[1] + num[1] - 1;
+    t.z1 = start[2] + num[2] - 1;
+    t.nx = num[0];
+    t.ny = num[1];
+    t.cs = (unsigned long long)num[0] * num[1] * num[2];
+    for (const auto& o : h->slabs) {
+        const SlabTab& u = o.tab;
+        const bool overlap = t.x0 <= u.x1 && t.x1 >= u.x0 && t.y0 <= u.y1 && t.y1 >= u.y0 && t.z0 <= u.z1 && t.z1 >= u.z0;
         if (overlap) return fail(EMS_E_ARG, "overlapping UPML slabs are not supported");
     }
-    HIPCHK(hipSetDevice(h->device));
-    const unsigned long long scs = (unsigned long long)num[0] * num[1] * num[2];
+    // at most two slabs may share an (x, y) column (the kernels track two):
+    // reject a third slab whose xy footprint meets the overlap of two others
+    for (size_t a = 0; a < h->slabs.size(); ++a)
+        for (size_t b = a + 1; b < h->slabs.size(); ++b) {
+            const SlabTab& A = h->slabs[a].tab;
+            const SlabTab& B = h->slabs[b].tab;
+            const int xl = std::max({A.x0, B.x0, t.x0}), xh = std::min({A.x1, B.x1, t.x1});
+            const int yl = std::max({A.y0, B.y0, t.y0}), yh = std::min({A.y1, B.y1, t.y1});
+            if (xl <= xh && yl <= yh) return fail(EMS_E_ARG, "more than two UPML slabs share a mesh column");
+        }
     // repack [3][nx][ny][nz] (z fastest) -> [3][nz][ny][nx] (x fastest)
-    auto repack = [&](const float* src) {
-        std::vector<float> d(3 * scs);
+    auto repack = [&](const float* src, std::vector<float>& d) {
+        d.resize(3 * t.cs);
         for (int n = 0; n < 3; ++n)
             for (unsigned x = 0; x < num[0]; ++x)
                 for (unsigned y = 0; y < num[1]; ++y)
                     for (unsigned z = 0; z < num[2]; ++z)
-                        d[n * scs + x + (unsigned long long)num[0] * (y + (unsigned long long)num[1] * z)] =
+                        d[n * t.cs + x + (unsigned long long)num[0] * (y + (unsigned long long)num[1] * z)] =
                             src[(((size_t)n * num[0] + x) * num[1] + y) * num[2] + z];
-        return d;
     };
-    SlabDev sv{}, si{};
-    for (SlabDev* s : {&sv, &si}) {
-        s->x0 = start[0];
-        s->y0 = start[1];
-        s->z0 = start[2];
-        s->x1 = start[0] + num[0] - 1;
-        s->y1 = start[1] + num[1] - 1;
-        s->z1 = start[2] + num[2] - 1;
-        s->nx = num[0];
-        s->ny = num[1];
-        s->cs = scs;
+    ems_engine::SlabHost sh;
+    sh.tab = t;
+    repack(vv, sh.v[0]);
+    repack(vvfo, sh.v[1]);
+    repack(vvfn, sh.v[2]);
+    repack(ii, sh.i[0]);
+    repack(iifo, sh.i[1]);
+    repack(iifn, sh.i[2]);
+    h->slabs.push_back(std::move(sh));
+    return 0;
+}
+
+static int upload_upml(ems_engine* h)
+{
+    // one buffer per half step: per slab [c0, cfo, cfn, flux][3][cs]
+    if (h->slabs.empty()) return 0;
+    unsigned long long total = 0;
+    std::vector<SlabTab> tab;
+    for (auto& s : h->slabs) {
+        s.tab.off = total;
+        total += 12 * s.tab.cs;
+        tab.push_back(s.tab);
     }
     int rc;
-    std::vector<float> t;
-    t = repack(vv);
-    if ((rc = upload(h, &sv.c0, t.data(), t.size()))) return rc;
-    t = repack(vvfo);
-    if ((rc = upload(h, &sv.cfo, t.data(), t.size()))) return rc;
-    t = repack(vvfn);
-    if ((rc = upload(h, &sv.cfn, t.data(), t.size()))) return rc;
-    t = repack(ii);
-    if ((rc = upload(h, &si.c0, t.data(), t.size()))) return rc;
-    t = repack(iifo);
-    if ((rc = upload(h, &si.cfo, t.data(), t.size()))) return rc;
-    t = repack(iifn);
-    if ((rc = upload(h, &si.cfn, t.data(), t.size()))) return rc;
-    if ((rc = h->alloc(&sv.flux, 3 * scs)) || (rc = h->alloc(&si.flux, 3 * scs))) return rc;
-    HIPCHK(hipMemset(sv.flux, 0, 3 * scs * sizeof(float)));
-    HIPCHK(hipMemset(si.flux, 0, 3 * scs * sizeof(float)));
-    h->pmlV.s[h->pmlV.n++] = sv;
-    h->pmlI.s[h->pmlI.n++] = si;
+    for (int side = 0; side < 2; ++side) {
+        std::vector<float> stage(total, 0.0f);
+        for (const auto& s : h->slabs) {
+            const auto* src = side ? s.i : s.v;
+            for (int a = 0; a < 3; ++a)
+                std::copy(src[a].begin(), src[a].end(), stage.begin() + s.tab.off + (unsigned long long)a * 3 * s.tab.cs);
+        }
+        PmlArgs& P = side ? h->pmlI : h->pmlV;
+        float* buf;
+        SlabTab* dt;
+        if ((rc = upload(h, &buf, stage.data(), stage.size())) || (rc = upload(h, &dt, tab.data(), tab.size()))) return rc;
+        P.n = (int)tab.size();
+        P.buf = buf;
+        P.tab = dt;
+    }
+    h->slabs.clear();
     return 0;
 }
 
@@ -1057,6 +1109,10 @@ int ems_add_lorentz(ems_engine* h, unsigned count, const unsigned* pos, const fl
 int ems_init(ems_engine* h)
 {
     if (!h) return fail(EMS_E_ARG, "null handle");
+    if (h->inited) return 0;
+    HIPCHK(hipSetDevice(h->device));
+    int rc = upload_upml(h);
+    if (rc) return rc;
     h->inited = true;
     return 0;
 }
@@ -1280,9 +1336,10 @@ constexpr unsigned kProfSteps = 4096;
 int ems_iterate(ems_engine* h, unsigned nsteps)
 {
     if (!h) return fail(EMS_E_ARG, "null handle");
-    if (!h->inited) ems_init(h);
+    int rc;
+    if (!h->inited && (rc = ems_init(h))) return rc;
     HIPCHK(hipSetDevice(h->device));
-    int rc = sync_probe_tables(h);
+    rc = sync_probe_tables(h);
     if (rc) return rc;
     const unsigned nprof = h->profile ? std::min(nsteps, kProfSteps) : 0;
     while (h->kev.size() < 4 * (size_t)nprof) {
