@@ -11,10 +11,16 @@ the reference's own speed definition (openems.cpp:1410,1484-1485).
 
 roofline: the dominant (voltage or current update) kernel, achieved =
 algorithmic bytes per launch / average launch duration measured with HIP
-events on the engine stream over the timed steps; algorithmic bytes per cell
-update of one half step = 15 floats (read field, 3 curl inputs read once,
-2x3 coefficients, write field) + 15 floats more in UPML positions (3 flux
-read + 3 flux write + 9 UPML coefficients).
+events on the engine stream over the timed steps.  Algorithmic bytes per cell
+update of one half step, by coefficient layout:
+  plain      : 15 floats = 60 B (read field, 3 curl inputs read once,
+               2x3 coefficients, write field);
+  compressed : 9 floats + one uint16 table index = 38 B (coefficients come
+               from a table of unique tuples that stays in cache);
+plus, per UPML position, 15 floats = 60 B (plain UPML: 3 flux read + 3 flux
+write + 9 coefficients) or 6 floats + one uint32 index = 28 B (compressed
+UPML).  `plain_ceiling_frac` compares the cell-update rate with the HBM
+ceiling of the plain layout (8 TB/s / 143.57 B per cell-step).
 
 cpu_baseline: the oracle's restated multithreaded SSE-compressed engine
 (the reference's default "fastest" engine) on the same operator, on a bounded
@@ -31,6 +37,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "MCells/s & % HBM roofline (256³ UPML); batched speedup; adjoint/forward time"
+
+
+LAYOUT_BYTES = {"plain": 60.0, "compressed": 38.0}        # per cell, per half step
+PML_BYTES = {"plain": 60.0, "compressed": 28.0}           # per UPML position, per half step
+PLAIN_CELL_STEP_BYTES = 120.0
 
 
 def pml_positions(op):
@@ -116,6 +127,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--layout", default="auto", choices=["auto", "plain", "compressed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -135,7 +147,7 @@ def main():
     n = args.size
     scene = scenes.upml_bricks(n, 10000)
     t0 = time.perf_counter()
-    sim = Simulation(scene)
+    sim = Simulation(scene, layout=args.layout)
     setup_s = time.perf_counter() - t0
     eng = sim.eng
     op = sim.op
@@ -153,11 +165,16 @@ def main():
     eng.set_profiling(False)
 
     value = cells * args.steps / wall / 1e6
-    bytes_per_launch = 60.0 * (cells + npml)
+    lay, n_unique, play, n_unique_pml = eng.layout()
+    play = play or "plain"
+    bytes_per_launch = LAYOUT_BYTES[lay] * cells + PML_BYTES[play] * npml
+    plain_step_bytes = PLAIN_CELL_STEP_BYTES * (cells + npml)
     kname, kms = ("k_volt", volt_ms) if volt_ms >= curr_ms else ("k_curr", curr_ms)
     achieved = bytes_per_launch / (kms * 1e-3) / 1e9
-    layout = "plain fp32 coefficients + fused UPML"
+    layout = f"{lay} coefficients ({n_unique} unique) + fused {play} UPML ({n_unique_pml} unique)" \
+        if lay == "compressed" or play == "compressed" else "plain fp32 coefficients + fused UPML"
     traffic = traffic_from_profile(kname, layout, n)
+    plain_ceiling = HBM_PEAK_GBS * 1e9 / (plain_step_bytes / cells) / 1e6     # MCells/s
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -175,12 +192,15 @@ def main():
                                "40 seeded dielectric bricks, 8-cell UPML on all faces, 50 Ohm lumped port "
                                "with Gaussian fc=20 GHz, port V/I probes, E-field DFT plane at 5/10/15 GHz",
                    "cells": int(cells), "pml_positions": int(npml), "layout": layout,
-                   "bytes_per_cell_update": round(120.0 * (cells + npml) / cells, 2)},
+                   "bytes_per_cell_update": round(2 * bytes_per_launch / cells, 2),
+                   "plain_bytes_per_cell_update": round(plain_step_bytes / cells, 2)},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
                      "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(kms, 5),
-                     "volt_ms": round(volt_ms, 5), "curr_ms": round(curr_ms, 5)},
+                     "volt_ms": round(volt_ms, 5), "curr_ms": round(curr_ms, 5),
+                     "plain_ceiling_mcells": round(plain_ceiling, 1),
+                     "plain_ceiling_frac": round(value / plain_ceiling, 4)},
     }
     print(f"# setup {setup_s:.1f}s, device time {dev_ms:.1f} ms for {args.steps} steps", file=sys.stderr)
     if not args.no_cpu_baseline:

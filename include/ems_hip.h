@@ -9,6 +9,17 @@
  *
  * Reference interfaces replaced (paths relative to the openEMS tree):
  *   ems_create            Operator::CreateEngine / Engine::Init           FDTD/operator.cpp:68-72, FDTD/engine.cpp:51-59
+ *   ems_create_layout     same, with the device coefficient layout chosen explicitly
+ *   ems_create_sse_compressed
+ *                         Operator_SSE_Compressed::CreateEngine on the compressed operator
+ *                                                    FDTD/operator_sse_compressed.cpp:114-175, FDTD/operator_sse_compressed.h:49-77
+ *   ems_get_layout        Operator_SSE_Compressed::ShowStat (compression state, unique count)
+ *                                                    FDTD/operator_sse_compressed.cpp:105-112
+ *   ems_compress_operator Operator_SSE_Compressed::CompressOperator, per cell (host only)
+ *                                                    FDTD/operator_sse_compressed.cpp:114-175
+ *   ems_expand_sse_compressed
+ *                         Operator_SSE_Compressed::GetVV/GetVI/GetII/GetIV over the whole mesh (host only)
+ *                                                    FDTD/operator_sse_compressed.h:49-52
  *   ems_set_excitation    Operator_Ext_Excitation -> Engine_Ext_Excitation FDTD/extensions/engine_ext_excitation.cpp:24-99
  *   ems_add_upml          Operator_Ext_UPML -> Engine_Ext_UPML            FDTD/extensions/engine_ext_upml.cpp:24-234
  *   ems_add_mur           Operator_Ext_Mur_ABC -> Engine_Ext_Mur_ABC      FDTD/extensions/engine_ext_mur_abc.cpp:28-178
@@ -52,10 +63,58 @@ const char* ems_last_error(void);
 /* number of visible HIP devices (0 when no GPU is present) */
 int ems_device_count(void);
 
+/* Device coefficient layouts.
+ *   PLAIN      : vv, vi, ii, iv as float32 arrays (24 B per cell per half step).
+ *   COMPRESSED : one uint16 per cell into a table of unique coefficient tuples
+ *                (the per-cell form of the reference's compressed operator;
+ *                2 B per cell per half step); UPML coefficients likewise by a
+ *                per-position table index.  Possible when the operator has at
+ *                most 65536 unique tuples; results are bit-identical to PLAIN.
+ *   AUTO       : COMPRESSED when possible and the table is reused on average
+ *                by at least 16 cells, otherwise PLAIN (UPML decided alike). */
+#define EMS_LAYOUT_AUTO 0
+#define EMS_LAYOUT_PLAIN 1
+#define EMS_LAYOUT_COMPRESSED 2
+
 /* numLines[3]; vv, vi, ii, iv: host float32 [3][Nx][Ny][Nz] (reference N-I-J-K).
- * device: HIP device ordinal. */
+ * device: HIP device ordinal.  Uses EMS_LAYOUT_AUTO. */
 int ems_create(ems_engine** out, const unsigned numLines[3], const float* vv, const float* vi,
                const float* ii, const float* iv, int device);
+
+/* As ems_create with an explicit layout; EMS_LAYOUT_COMPRESSED fails with
+ * EMS_E_ARG when the operator has more than 65536 unique tuples. */
+int ems_create_layout(ems_engine** out, const unsigned numLines[3], const float* vv, const float* vi,
+                      const float* ii, const float* iv, int device, int layout);
+
+/* From the reference's SSE-compressed operator: op_index is m_Op_index
+ * ([Nx][Ny][Nz] unsigned, only z < nV = ceil(Nz/4) used), n_unique its table
+ * size and vv4[n], vi4[n], ii4[n], iv4[n] (n = 0..2) the f4_*_Compressed[n]
+ * tables (n_unique x 4 floats, lane l of entry u holds z = l*nV + v). */
+int ems_create_sse_compressed(ems_engine** out, const unsigned numLines[3], const unsigned* op_index,
+                              unsigned n_unique, const float* const vv4[3], const float* const vi4[3],
+                              const float* const ii4[3], const float* const iv4[3], int device, int layout);
+
+/* Host only (no device needed): the per-cell compression ems_create applies.
+ * idx receives Nx*Ny*Nz uint16 in x-fastest order (x + Nx*(y + Ny*z));
+ * tabV / tabI receive 8 floats per unique tuple, {vv_x, vi_x, vv_y, vi_y,
+ * vv_z, vi_z, 0, 0} and {ii_x, iv_x, ii_y, iv_y, ii_z, iv_z, 0, 0}, numbered
+ * in first-seen order over cells x -> y -> z (x outermost), tuples equal by
+ * float value.  Returns the number of unique tuples, or EMS_E_ARG when there
+ * are more than max_unique (at most 65536). */
+int ems_compress_operator(const unsigned numLines[3], const float* vv, const float* vi, const float* ii,
+                          const float* iv, unsigned short* idx, float* tabV, float* tabI, unsigned max_unique);
+
+/* Host only: the N-I-J-K operator arrays [3][Nx][Ny][Nz] of the reference's
+ * SSE-compressed operator (arguments as in ems_create_sse_compressed). */
+int ems_expand_sse_compressed(const unsigned numLines[3], const unsigned* op_index, unsigned n_unique,
+                              const float* const vv4[3], const float* const vi4[3], const float* const ii4[3],
+                              const float* const iv4[3], float* vv, float* vi, float* ii, float* iv);
+
+/* Active layouts and table sizes (known after ems_init for the UPML part):
+ * layout, pml_layout: EMS_LAYOUT_PLAIN or EMS_LAYOUT_COMPRESSED (pml_layout 0
+ * without UPML); n_unique / n_unique_pml: table entries (0 when plain). */
+int ems_get_layout(const ems_engine* h, int* layout, unsigned* n_unique, int* pml_layout,
+                   unsigned* n_unique_pml);
 
 /* Excitation signal (length samples, volt at n*dT, curr at (n+1/2)*dT),
  * period in timesteps (0 when aperiodic) and the voltage / current cell lists:

@@ -68,6 +68,13 @@ ENGINE_API = [
     ("ems_last_error", C.c_char_p, []),
     ("ems_device_count", C.c_int, []),
     ("ems_create", C.c_int, [C.POINTER(vp), up, fp, fp, fp, fp, C.c_int]),
+    ("ems_create_layout", C.c_int, [C.POINTER(vp), up, fp, fp, fp, fp, C.c_int, C.c_int]),
+    ("ems_create_sse_compressed", C.c_int, [C.POINTER(vp), up, up, C.c_uint, C.POINTER(fp), C.POINTER(fp),
+                                            C.POINTER(fp), C.POINTER(fp), C.c_int, C.c_int]),
+    ("ems_compress_operator", C.c_int, [up, fp, fp, fp, fp, usp, fp, fp, C.c_uint]),
+    ("ems_expand_sse_compressed", C.c_int, [up, up, C.c_uint, C.POINTER(fp), C.POINTER(fp), C.POINTER(fp),
+                                            C.POINTER(fp), fp, fp, fp, fp]),
+    ("ems_get_layout", C.c_int, [vp, ip, up, ip, up]),
     ("ems_set_excitation", C.c_int, [vp, fp, fp, C.c_uint, C.c_int, C.c_uint, up, usp, fp, up,
                                      C.c_uint, up, usp, fp, up]),
     ("ems_add_upml", C.c_int, [vp, up, up, fp, fp, fp, fp, fp, fp]),

@@ -18,6 +18,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libopenems_hip.so")
 SOURCES = [
     os.path.join(HERE, "csrc", "host", "ems_operator.cpp"),
+    os.path.join(HERE, "csrc", "engine", "ems_compress.cpp"),
     os.path.join(HERE, "csrc", "engine", "ems_engine.hip"),
 ]
 ARCH = os.environ.get("OPENEMS_AMD_ARCH", "gfx950")

@@ -22,6 +22,7 @@
 #include <thread>
 #include <vector>
 
+#include "ems_compress.h"
 #include "ems_hip.h"
 
 #pragma clang fp contract(off)
@@ -55,9 +56,22 @@ struct GridDesc {
     unsigned qtiles;    // ceil(plane / kBlock)
 };
 
+// Coefficients of one half step.  Plain layout: SoA arrays cc (vv or ii) and
+// cv (vi or iv).  Compressed layout: one uint16 per cell (shared by both half
+// steps) into a table of two float4 per entry, {c_x, v_x, c_y, v_y} and
+// {c_z, v_z, 0, 0}, one table per half step.
+struct CoefArgs {
+    const float* cc;
+    const float* cv;
+    const unsigned short* idx;
+    const float4* tab;
+};
+
 // One UPML slab (Operator_Ext_UPML range) in the device table.  All slabs of
-// one half step share one buffer; a slab's block holds twelve arrays
-// [c0, cfo, cfn, flux][component][local index] with the local index x-fastest.
+// one half step share one buffer.  Plain UPML layout: a slab's block holds
+// twelve arrays [c0, cfo, cfn, flux][component][local index]; compressed UPML
+// layout: four arrays [flux][component][local] + [table index][local], the
+// index stored as the bit pattern of a uint32.  The local index is x-fastest.
 struct SlabTab {
     int x0, x1, y0, y1, z0, z1;    // inclusive mesh ranges
     unsigned nx, ny;
@@ -69,7 +83,11 @@ struct PmlArgs {
     int n;                         // number of slabs (0: no UPML)
     const SlabTab* tab;            // [n]
     float* buf;                    // slab blocks
+    const float4* ptab;            // compressed UPML: 3 float4 per entry
+                                   // {c0_x, c0_y, c0_z, cfo_x}, {cfo_y, cfo_z, cfn_x, cfn_y}, {cfn_z, 0, 0, 0}
 };
+
+enum PmlMode { kNoPml = 0, kPmlPlain = 1, kPmlCompressed = 2 };
 
 // Workgroups are dispatched round-robin over the XCDs by linear block id;
 // remap so that each XCD owns one contiguous range of tiles (neighbouring
@@ -128,22 +146,31 @@ __device__ __forceinline__ bool column_at(const Column& c, int z, unsigned long 
 //   plain: X = field;  f = X*cc + cv*curl;  field <- f
 //   UPML : X = flux;   f = X*cc + cv*curl;  field <- (c0*field - cfo*flux) + cfn*f;  flux <- f
 // The loads shared by both forms are issued once for every lane; the UPML
-// lanes add four exec-masked loads per component.
-__device__ __forceinline__ void cell_update(float* __restrict__ F, const float* __restrict__ cc,
-                                            const float* __restrict__ cv, unsigned long long cs,
-                                            unsigned long long p, const float* curl, bool pml,
-                                            float* __restrict__ pbuf, unsigned long long li,
-                                            unsigned long long scs)
+// lanes add exec-masked loads (coefficients or a table index, and the flux).
+template <int PM, bool CC>
+__device__ __forceinline__ void cell_update(float* __restrict__ F, const CoefArgs& k, unsigned long long cs,
+                                            unsigned long long p, const float* curl, bool pml, const PmlArgs& P,
+                                            unsigned long long li, unsigned long long scs)
 {
     float x0[3], a[3], b[3];
 #pragma unroll
-    for (int n = 0; n < 3; ++n) {
-        x0[n] = F[n * cs + p];
-        a[n] = cc[n * cs + p];
-        b[n] = cv[n * cs + p];
+    for (int n = 0; n < 3; ++n) x0[n] = F[n * cs + p];
+    if (CC) {
+        const unsigned u = k.idx[p];
+        const float4 t0 = k.tab[2 * u], t1 = k.tab[2 * u + 1];
+        a[0] = t0.x; b[0] = t0.y;
+        a[1] = t0.z; b[1] = t0.w;
+        a[2] = t1.x; b[2] = t1.y;
+    } else {
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            a[n] = k.cc[n * cs + p];
+            b[n] = k.cv[n * cs + p];
+        }
     }
+    float* __restrict__ pbuf = P.buf;
     float k0[3], ko[3], kn[3], fl[3];
-    if (pml) {
+    if (PM == kPmlPlain && pml) {
 #pragma unroll
         for (int n = 0; n < 3; ++n) {
             k0[n] = pbuf[li + n * scs];
@@ -152,14 +179,23 @@ __device__ __forceinline__ void cell_update(float* __restrict__ F, const float* 
             fl[n] = pbuf[li + (9 + n) * scs];
         }
     }
+    if (PM == kPmlCompressed && pml) {
+#pragma unroll
+        for (int n = 0; n < 3; ++n) fl[n] = pbuf[li + n * scs];
+        const unsigned u = __float_as_uint(pbuf[li + 3 * scs]);
+        const float4 t0 = P.ptab[3 * u], t1 = P.ptab[3 * u + 1], t2 = P.ptab[3 * u + 2];
+        k0[0] = t0.x; k0[1] = t0.y; k0[2] = t0.z;
+        ko[0] = t0.w; ko[1] = t1.x; ko[2] = t1.y;
+        kn[0] = t1.z; kn[1] = t1.w; kn[2] = t2.x;
+    }
     float out[3], fnew[3];
 #pragma unroll
     for (int n = 0; n < 3; ++n) {
-        const float xin = pml ? fl[n] : x0[n];
+        const float xin = (PM != kNoPml && pml) ? fl[n] : x0[n];
         float f = xin * a[n];
         f = f + b[n] * curl[n];
         fnew[n] = f;
-        if (pml) {
+        if (PM != kNoPml && pml) {
             const float t = k0[n] * x0[n] - ko[n] * fl[n];
             out[n] = t + kn[n] * f;
         } else {
@@ -168,9 +204,10 @@ __device__ __forceinline__ void cell_update(float* __restrict__ F, const float* 
     }
 #pragma unroll
     for (int n = 0; n < 3; ++n) F[n * cs + p] = out[n];
-    if (pml) {
+    if (PM != kNoPml && pml) {
+        const unsigned long long fo = PM == kPmlPlain ? 9 : 0;
 #pragma unroll
-        for (int n = 0; n < 3; ++n) pbuf[li + (9 + n) * scs] = fnew[n];
+        for (int n = 0; n < 3; ++n) pbuf[li + (fo + n) * scs] = fnew[n];
     }
 }
 
@@ -184,10 +221,9 @@ __device__ __forceinline__ void cell_update(float* __restrict__ F, const float* 
 // Each thread owns one (x, y) column and sweeps g.zdepth planes, carrying the
 // z-1 currents in registers.
 // ---------------------------------------------------------------------------
-template <bool PML>
+template <int PM, bool CC>
 __global__ __launch_bounds__(kBlock) void k_volt(const GridDesc g, float* __restrict__ V, const float* __restrict__ I,
-                                                 const float* __restrict__ vv, const float* __restrict__ vi,
-                                                 const PmlArgs pml)
+                                                 const CoefArgs k, const PmlArgs pml)
 {
     const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
     const unsigned qt = t % g.qtiles, zc = t / g.qtiles;
@@ -208,7 +244,7 @@ __global__ __launch_bounds__(kBlock) void k_volt(const GridDesc g, float* __rest
     float Ix_zm = Ix[pz], Iy_zm = Iy[pz];
 
     Column col;
-    if (PML) column_slabs(pml, x, y, col);
+    if (PM != kNoPml) column_slabs(pml, x, y, col);
 
     for (unsigned z = z0; z < z1; ++z, p += g.plane) {
         const float ix = Ix[p], iy = Iy[p], iz = Iz[p];
@@ -222,8 +258,8 @@ __global__ __launch_bounds__(kBlock) void k_volt(const GridDesc g, float* __rest
         Iy_zm = iy;
         bool in_pml = false;
         unsigned long long li = 0, scs = 0;
-        if (PML) in_pml = column_at(col, (int)z, li, scs);
-        cell_update(V, vv, vi, cs, p, curl, in_pml, pml.buf, li, scs);
+        if (PM != kNoPml) in_pml = column_at(col, (int)z, li, scs);
+        cell_update<PM, CC>(V, k, cs, p, curl, in_pml, pml, li, scs);
     }
 }
 
@@ -234,10 +270,9 @@ __global__ __launch_bounds__(kBlock) void k_volt(const GridDesc g, float* __rest
 // timestep counter (++numTS of Engine::IterateTS): no kernel of this launch
 // reads it, and every later kernel sees the new value.
 // ---------------------------------------------------------------------------
-template <bool PML>
+template <int PM, bool CC>
 __global__ __launch_bounds__(kBlock) void k_curr(const GridDesc g, float* __restrict__ I, const float* __restrict__ V,
-                                                 const float* __restrict__ ii, const float* __restrict__ iv,
-                                                 const PmlArgs pml, unsigned* __restrict__ numts)
+                                                 const CoefArgs k, const PmlArgs pml, unsigned* __restrict__ numts)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(numts, 1u);
     const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
@@ -260,7 +295,7 @@ __global__ __launch_bounds__(kBlock) void k_curr(const GridDesc g, float* __rest
     float vx = Vx[p], vy = Vy[p];
 
     Column col;
-    if (PML) column_slabs(pml, x, y, col);
+    if (PM != kNoPml) column_slabs(pml, x, y, col);
 
     for (unsigned z = z0; z < z1; ++z, p += g.plane) {
         const unsigned long long pzp = p + g.plane;
@@ -276,8 +311,8 @@ __global__ __launch_bounds__(kBlock) void k_curr(const GridDesc g, float* __rest
         vy = vy_zp;
         bool in_pml = false;
         unsigned long long li = 0, scs = 0;
-        if (PML) in_pml = column_at(col, (int)z, li, scs);
-        cell_update(I, ii, iv, cs, p, curl, in_pml, pml.buf, li, scs);
+        if (PM != kNoPml) in_pml = column_at(col, (int)z, li, scs);
+        cell_update<PM, CC>(I, k, cs, p, curl, in_pml, pml, li, scs);
     }
 }
 
@@ -639,7 +674,14 @@ struct ems_engine {
     unsigned N[3] = {0, 0, 0};
     unsigned long long cs = 0;
     float *V = nullptr, *I = nullptr;
+    // coefficient layout: plain arrays or per-cell index + tables
+    int layout_req = EMS_LAYOUT_AUTO;
+    bool cmp = false;
     float *vv = nullptr, *vi = nullptr, *ii = nullptr, *iv = nullptr;
+    unsigned short* cidx = nullptr;
+    float* ctab[2] = {nullptr, nullptr};
+    unsigned n_unique = 0;
+    CoefArgs kV{}, kI{};
     unsigned* numts_dev = nullptr;
     unsigned numTS = 0;
     bool inited = false;
@@ -666,6 +708,8 @@ struct ems_engine {
     };
     std::vector<SlabHost> slabs;
     PmlArgs pmlV{}, pmlI{};
+    int pml_mode = kNoPml;
+    unsigned n_unique_pml = 0;
     // Mur faces in creation order
     std::vector<MurDev> mur;
     // lumped RLC and dispersive lists
@@ -806,11 +850,12 @@ int ems_device_count(void)
     return n;
 }
 
-int ems_create(ems_engine** out, const unsigned numLines[3], const float* vv, const float* vi, const float* ii,
-               const float* iv, int device)
+int ems_create_layout(ems_engine** out, const unsigned numLines[3], const float* vv, const float* vi,
+                      const float* ii, const float* iv, int device, int layout)
 {
     if (!out || !numLines || !vv || !vi || !ii || !iv) return fail(EMS_E_ARG, "null argument");
     *out = nullptr;
+    if (layout < EMS_LAYOUT_AUTO || layout > EMS_LAYOUT_COMPRESSED) return fail(EMS_E_ARG, "unknown layout");
     for (int n = 0; n < 3; ++n)
         if (numLines[n] < 3) return fail(EMS_E_ARG, "at least 3 mesh lines per direction are required");
     const unsigned long long plane = (unsigned long long)numLines[0] * numLines[1];
@@ -820,11 +865,23 @@ int ems_create(ems_engine** out, const unsigned numLines[3], const float* vv, co
     if (ndev <= 0) return fail(EMS_E_NODEV, "no HIP device available");
     if (device < 0 || device >= ndev) return fail(EMS_E_ARG, "invalid device ordinal");
     HIPCHK(hipSetDevice(device));
+    const unsigned long long tot = plane * numLines[2];
+    const unsigned long long cs = (tot + 63) / 64 * 64;
+    // per-cell compression (host): first-seen order x -> y -> z
+    std::vector<uint16_t> cidx;
+    std::vector<float> tab[2];
+    size_t U = 0;
+    if (layout != EMS_LAYOUT_PLAIN) {
+        U = ems::compress_cells(numLines, vv, vi, ii, iv, cs, cidx, tab[0], tab[1], ems::host_threads());
+        if (U == 0 && layout == EMS_LAYOUT_COMPRESSED)
+            return fail(EMS_E_ARG, "the operator has more than 65536 unique coefficient tuples");
+        if (layout == EMS_LAYOUT_AUTO && U * 16 > tot) U = 0;
+    }
     ems_engine* h = new ems_engine();
     h->device = device;
+    h->layout_req = layout;
     for (int n = 0; n < 3; ++n) h->N[n] = numLines[n];
-    const unsigned long long tot = plane * numLines[2];
-    h->cs = (tot + 63) / 64 * 64;
+    h->cs = cs;
     GridDesc& g = h->g;
     g.Nx = numLines[0];
     g.Ny = numLines[1];
@@ -844,20 +901,98 @@ int ems_create(ems_engine** out, const unsigned numLines[3], const float* vv, co
         return r;
     };
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(EMS_E_HIP, "stream creation failed"));
-    if ((rc = h->alloc(&h->V, 3 * h->cs)) || (rc = h->alloc(&h->I, 3 * h->cs)) || (rc = h->alloc(&h->vv, 3 * h->cs)) ||
-        (rc = h->alloc(&h->vi, 3 * h->cs)) || (rc = h->alloc(&h->ii, 3 * h->cs)) || (rc = h->alloc(&h->iv, 3 * h->cs)) ||
-        (rc = h->alloc(&h->numts_dev, 4)))
+    if ((rc = h->alloc(&h->V, 3 * h->cs)) || (rc = h->alloc(&h->I, 3 * h->cs)) || (rc = h->alloc(&h->numts_dev, 4)))
         return bail(rc);
     if (hipMemset(h->V, 0, 3 * h->cs * sizeof(float)) != hipSuccess ||
         hipMemset(h->I, 0, 3 * h->cs * sizeof(float)) != hipSuccess ||
         hipMemset(h->numts_dev, 0, 4 * sizeof(unsigned)) != hipSuccess)
         return bail(fail(EMS_E_HIP, "hipMemset failed"));
-    if ((rc = upload_nijk(h, h->vv, vv)) || (rc = upload_nijk(h, h->vi, vi)) || (rc = upload_nijk(h, h->ii, ii)) ||
-        (rc = upload_nijk(h, h->iv, iv)))
-        return bail(rc);
+    if (U > 0) {
+        h->cmp = true;
+        h->n_unique = (unsigned)U;
+        if ((rc = upload(h, &h->cidx, (const unsigned short*)cidx.data(), cidx.size())) ||
+            (rc = upload(h, &h->ctab[0], tab[0].data(), tab[0].size())) ||
+            (rc = upload(h, &h->ctab[1], tab[1].data(), tab[1].size())))
+            return bail(rc);
+    } else {
+        if ((rc = h->alloc(&h->vv, 3 * h->cs)) || (rc = h->alloc(&h->vi, 3 * h->cs)) ||
+            (rc = h->alloc(&h->ii, 3 * h->cs)) || (rc = h->alloc(&h->iv, 3 * h->cs)))
+            return bail(rc);
+        if ((rc = upload_nijk(h, h->vv, vv)) || (rc = upload_nijk(h, h->vi, vi)) || (rc = upload_nijk(h, h->ii, ii)) ||
+            (rc = upload_nijk(h, h->iv, iv)))
+            return bail(rc);
+    }
+    h->kV = CoefArgs{h->vv, h->vi, h->cidx, (const float4*)h->ctab[0]};
+    h->kI = CoefArgs{h->ii, h->iv, h->cidx, (const float4*)h->ctab[1]};
     hipEventCreate(&h->ev0);
     hipEventCreate(&h->ev1);
     *out = h;
+    return 0;
+}
+
+int ems_create(ems_engine** out, const unsigned numLines[3], const float* vv, const float* vi, const float* ii,
+               const float* iv, int device)
+{
+    return ems_create_layout(out, numLines, vv, vi, ii, iv, device, EMS_LAYOUT_AUTO);
+}
+
+int ems_create_sse_compressed(ems_engine** out, const unsigned numLines[3], const unsigned* op_index,
+                              unsigned n_unique, const float* const vv4[3], const float* const vi4[3],
+                              const float* const ii4[3], const float* const iv4[3], int device, int layout)
+{
+    if (!out || !numLines) return fail(EMS_E_ARG, "null argument");
+    *out = nullptr;
+    const size_t M = (size_t)numLines[0] * numLines[1] * numLines[2];
+    std::vector<float> a(3 * M), b(3 * M), c(3 * M), d(3 * M);
+    const int rc = ems_expand_sse_compressed(numLines, op_index, n_unique, vv4, vi4, ii4, iv4, a.data(), b.data(),
+                                             c.data(), d.data());
+    if (rc) return rc;
+    return ems_create_layout(out, numLines, a.data(), b.data(), c.data(), d.data(), device, layout);
+}
+
+int ems_compress_operator(const unsigned numLines[3], const float* vv, const float* vi, const float* ii,
+                          const float* iv, unsigned short* idx, float* tabV, float* tabI, unsigned max_unique)
+{
+    if (!numLines || !vv || !vi || !ii || !iv || !idx || !tabV || !tabI) return fail(EMS_E_ARG, "null argument");
+    const unsigned long long tot = (unsigned long long)numLines[0] * numLines[1] * numLines[2];
+    if (tot == 0) return fail(EMS_E_ARG, "empty mesh");
+    std::vector<uint16_t> ci;
+    std::vector<float> tv, ti;
+    const size_t U = ems::compress_cells(numLines, vv, vi, ii, iv, tot, ci, tv, ti, ems::host_threads());
+    if (U == 0 || U > max_unique) return fail(EMS_E_ARG, "more unique coefficient tuples than the table holds");
+    std::memcpy(idx, ci.data(), tot * sizeof(uint16_t));
+    std::memcpy(tabV, tv.data(), tv.size() * sizeof(float));
+    std::memcpy(tabI, ti.data(), ti.size() * sizeof(float));
+    return (int)U;
+}
+
+int ems_expand_sse_compressed(const unsigned numLines[3], const unsigned* op_index, unsigned n_unique,
+                              const float* const vv4[3], const float* const vi4[3], const float* const ii4[3],
+                              const float* const iv4[3], float* vv, float* vi, float* ii, float* iv)
+{
+    if (!numLines || !op_index || !vv4 || !vi4 || !ii4 || !iv4 || !vv || !vi || !ii || !iv)
+        return fail(EMS_E_ARG, "null argument");
+    for (int n = 0; n < 3; ++n)
+        if (!vv4[n] || !vi4[n] || !ii4[n] || !iv4[n]) return fail(EMS_E_ARG, "null compressed table");
+    if (n_unique == 0) return fail(EMS_E_ARG, "empty compressed table");
+    const size_t Nx = numLines[0], Ny = numLines[1], Nz = numLines[2], nV = (Nz + 3) / 4;
+    for (size_t xy = 0; xy < Nx * Ny; ++xy)
+        for (size_t v = 0; v < nV; ++v)
+            if (op_index[xy * Nz + v] >= n_unique) return fail(EMS_E_ARG, "compressed index outside the table");
+    ems::expand_sse_compressed(numLines, op_index, n_unique, vv4, vi4, ii4, iv4, vv, vi, ii, iv, ems::host_threads());
+    return 0;
+}
+
+int ems_get_layout(const ems_engine* h, int* layout, unsigned* n_unique, int* pml_layout, unsigned* n_unique_pml)
+{
+    if (!h) return fail(EMS_E_ARG, "null handle");
+    if (layout) *layout = h->cmp ? EMS_LAYOUT_COMPRESSED : EMS_LAYOUT_PLAIN;
+    if (n_unique) *n_unique = h->n_unique;
+    if (pml_layout)
+        *pml_layout = h->pml_mode == kPmlCompressed ? EMS_LAYOUT_COMPRESSED
+                      : h->pml_mode == kPmlPlain    ? EMS_LAYOUT_PLAIN
+                                                    : 0;
+    if (n_unique_pml) *n_unique_pml = h->n_unique_pml;
     return 0;
 }
 
@@ -957,22 +1092,60 @@ int ems_add_upml(ems_engine* h, const unsigned start[3], const unsigned num[3], 
 
 static int upload_upml(ems_engine* h)
 {
-    // one buffer per half step: per slab [c0, cfo, cfn, flux][3][cs]
+    // one buffer per half step.  Plain: per slab [c0, cfo, cfn, flux][3][cs].
+    // Compressed: per slab [flux][3][cs] + [index][cs], the index into a table
+    // of the unique 18-float tuples (V: c0, cfo, cfn; I: likewise) shared by
+    // both half steps, numbered in first-seen order over slabs in creation
+    // order, local index x-fastest.
     if (h->slabs.empty()) return 0;
+    std::vector<unsigned long long> first;
+    unsigned long long npos = 0;
+    for (const auto& s : h->slabs) {
+        first.push_back(npos);
+        npos += s.tab.cs;
+    }
+    std::vector<uint32_t> pidx;
+    std::vector<float> uniq;
+    size_t U = 0;
+    if (h->layout_req != EMS_LAYOUT_PLAIN) {
+        auto get = [&](size_t i, float* t) {
+            size_t s = 0;
+            while (s + 1 < first.size() && i >= first[s + 1]) ++s;
+            const auto& S = h->slabs[s];
+            const unsigned long long j = i - first[s], cs = S.tab.cs;
+            for (int a = 0; a < 3; ++a)
+                for (int n = 0; n < 3; ++n) {
+                    t[3 * a + n] = S.v[a][n * cs + j];
+                    t[9 + 3 * a + n] = S.i[a][n * cs + j];
+                }
+        };
+        U = ems::dedup_tuples<18>(npos, get, pidx, uniq, 65536, ems::host_threads());
+        if (h->layout_req == EMS_LAYOUT_AUTO && U * 16 > npos) U = 0;
+    }
+    const bool cmp = U > 0;
+    const unsigned per = cmp ? 4 : 12;
     unsigned long long total = 0;
     std::vector<SlabTab> tab;
     for (auto& s : h->slabs) {
         s.tab.off = total;
-        total += 12 * s.tab.cs;
+        total += per * s.tab.cs;
         tab.push_back(s.tab);
     }
     int rc;
     for (int side = 0; side < 2; ++side) {
         std::vector<float> stage(total, 0.0f);
-        for (const auto& s : h->slabs) {
-            const auto* src = side ? s.i : s.v;
-            for (int a = 0; a < 3; ++a)
-                std::copy(src[a].begin(), src[a].end(), stage.begin() + s.tab.off + (unsigned long long)a * 3 * s.tab.cs);
+        for (size_t k = 0; k < h->slabs.size(); ++k) {
+            const auto& s = h->slabs[k];
+            if (cmp) {
+                for (unsigned long long j = 0; j < s.tab.cs; ++j) {
+                    const uint32_t u = pidx[first[k] + j];
+                    std::memcpy(&stage[s.tab.off + 3 * s.tab.cs + j], &u, 4);
+                }
+            } else {
+                const auto* src = side ? s.i : s.v;
+                for (int a = 0; a < 3; ++a)
+                    std::copy(src[a].begin(), src[a].end(), stage.begin() + s.tab.off + (unsigned long long)a * 3 * s.tab.cs);
+            }
         }
         PmlArgs& P = side ? h->pmlI : h->pmlV;
         float* buf;
@@ -981,7 +1154,18 @@ static int upload_upml(ems_engine* h)
         P.n = (int)tab.size();
         P.buf = buf;
         P.tab = dt;
+        P.ptab = nullptr;
+        if (cmp) {
+            std::vector<float> pt(U * 12, 0.0f);
+            for (size_t u = 0; u < U; ++u)
+                for (int q = 0; q < 9; ++q) pt[u * 12 + q] = uniq[u * 18 + 9 * side + q];
+            float* dpt;
+            if ((rc = upload(h, &dpt, pt.data(), pt.size()))) return rc;
+            P.ptab = (const float4*)dpt;
+        }
     }
+    h->pml_mode = cmp ? kPmlCompressed : kPmlPlain;
+    h->n_unique_pml = cmp ? (unsigned)U : 0;
     h->slabs.clear();
     return 0;
 }
@@ -1260,6 +1444,42 @@ static int launch_samples(ems_engine* h)
     return 0;
 }
 
+}  // extern "C"
+
+namespace {
+
+template <int PM, bool CC>
+void launch_volt_t(ems_engine* h, unsigned nblk)
+{
+    hipLaunchKernelGGL((k_volt<PM, CC>), dim3(nblk), dim3(kBlock), 0, h->stream, h->g, h->V, h->I, h->kV, h->pmlV);
+}
+
+template <int PM, bool CC>
+void launch_curr_t(ems_engine* h, unsigned nblk)
+{
+    hipLaunchKernelGGL((k_curr<PM, CC>), dim3(nblk), dim3(kBlock), 0, h->stream, h->g, h->I, h->V, h->kI, h->pmlI,
+                       h->numts_dev);
+}
+
+template <bool CC>
+void launch_update(ems_engine* h, bool curr, unsigned nblk)
+{
+    const int pm = h->pmlV.n ? h->pml_mode : kNoPml;
+    if (pm == kPmlCompressed) curr ? launch_curr_t<kPmlCompressed, CC>(h, nblk) : launch_volt_t<kPmlCompressed, CC>(h, nblk);
+    else if (pm == kPmlPlain) curr ? launch_curr_t<kPmlPlain, CC>(h, nblk) : launch_volt_t<kPmlPlain, CC>(h, nblk);
+    else curr ? launch_curr_t<kNoPml, CC>(h, nblk) : launch_volt_t<kNoPml, CC>(h, nblk);
+}
+
+void dispatch_update(ems_engine* h, bool curr, unsigned nblk)
+{
+    if (h->cmp) launch_update<true>(h, curr, nblk);
+    else launch_update<false>(h, curr, nblk);
+}
+
+}  // namespace
+
+extern "C" {
+
 // one timestep in the reference's hook order (Engine::IterateTS, FDTD/engine.cpp:267-286,
 // priorities: UPML > default (lumped RLC, Lorentz, Mur) > excitation)
 static void launch_step(ems_engine* h, hipEvent_t* ev)
@@ -1276,10 +1496,7 @@ static void launch_step(ems_engine* h, hipEvent_t* ev)
     for (const auto& r : h->rlc)
         hipLaunchKernelGGL(k_rlc_pre, dim3((r.count + T - 1) / T), dim3(T), 0, h->stream, r, h->numts_dev);
     if (ev) (void)hipEventRecord(ev[0], h->stream);
-    if (h->pmlV.n)
-        hipLaunchKernelGGL(k_volt<true>, dim3(nblk), dim3(kBlock), 0, h->stream, g, h->V, h->I, h->vv, h->vi, h->pmlV);
-    else
-        hipLaunchKernelGGL(k_volt<false>, dim3(nblk), dim3(kBlock), 0, h->stream, g, h->V, h->I, h->vv, h->vi, h->pmlV);
+    dispatch_update(h, false, nblk);
     if (ev) (void)hipEventRecord(ev[1], h->stream);
     // -- post hooks: Mur (reverse creation order, all read-only on V)
     for (int k = (int)h->mur.size() - 1; k >= 0; --k) {
@@ -1306,12 +1523,7 @@ static void launch_step(ems_engine* h, hipEvent_t* ev)
         if (h->lor_i[k / 2])
             hipLaunchKernelGGL(k_lor_pre, dim3((h->lor[k].count + T - 1) / T), dim3(T), 0, h->stream, h->I, h->cs, h->lor[k]);
     if (ev) (void)hipEventRecord(ev[2], h->stream);
-    if (h->pmlI.n)
-        hipLaunchKernelGGL(k_curr<true>, dim3(nblk), dim3(kBlock), 0, h->stream, g, h->I, h->V, h->ii, h->iv, h->pmlI,
-                           h->numts_dev);
-    else
-        hipLaunchKernelGGL(k_curr<false>, dim3(nblk), dim3(kBlock), 0, h->stream, g, h->I, h->V, h->ii, h->iv, h->pmlI,
-                           h->numts_dev);
+    dispatch_update(h, true, nblk);
     if (ev) (void)hipEventRecord(ev[3], h->stream);
     for (size_t k = 1; k < h->lor.size(); k += 2)
         if (h->lor_i[k / 2])

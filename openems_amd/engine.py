@@ -14,9 +14,53 @@ import numpy as np
 
 from ._native import ProbeDesc, check, dp, fp, lib, ptr
 
+# device coefficient layouts (include/ems_hip.h)
+LAYOUT_AUTO, LAYOUT_PLAIN, LAYOUT_COMPRESSED = 0, 1, 2
+LAYOUTS = {"auto": LAYOUT_AUTO, "plain": LAYOUT_PLAIN, "compressed": LAYOUT_COMPRESSED}
+
+
+def compress_operator(N, vv, vi, ii, iv, max_unique=65536):
+    """Per-cell compression of an N-I-J-K operator as the engine applies it
+    (host only): returns (idx [Nz][Ny][Nx] uint16, tabV [U][8], tabI [U][8])."""
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (vv, vi, ii, iv)]
+    tot = int(np.prod(N))
+    idx = np.empty(tot, np.uint16)
+    tv = np.empty((max_unique, 8), np.float32)
+    ti = np.empty((max_unique, 8), np.float32)
+    U = check(lib().ems_compress_operator((C.c_uint * 3)(*N), *[ptr(a, C.c_float) for a in arrs],
+                                          ptr(idx, C.c_ushort), ptr(tv, C.c_float), ptr(ti, C.c_float),
+                                          int(max_unique)))
+    return idx.reshape(N[2], N[1], N[0]), tv[:U].copy(), ti[:U].copy()
+
+
+def _table_ptrs(tables):
+    """[3] arrays of n_unique x 4 floats -> (float*)[3]"""
+    arrs = [np.ascontiguousarray(t, np.float32) for t in tables]
+    return arrs, (fp * 3)(*[ptr(a, C.c_float) for a in arrs])
+
+
+def expand_sse_compressed(N, op_index, vv4, vi4, ii4, iv4):
+    """N-I-J-K arrays from the reference's SSE-compressed operator (host only)."""
+    opi = np.ascontiguousarray(op_index, np.uint32)
+    U = len(vv4[0])
+    keep, out = [], []
+    ptrs = []
+    for t in (vv4, vi4, ii4, iv4):
+        a, p = _table_ptrs(t)
+        keep.append(a)
+        ptrs.append(p)
+    for _ in range(4):
+        out.append(np.empty((3,) + tuple(N), np.float32))
+    check(lib().ems_expand_sse_compressed((C.c_uint * 3)(*N), ptr(opi, C.c_uint), U, *ptrs,
+                                          *[ptr(o, C.c_float) for o in out]))
+    return out
+
 
 class EngineHIP:
-    def __init__(self, op, device=0):
+    def __init__(self, op, device=0, layout="auto", sse_compressed=None):
+        """op: Operator; layout: "auto" | "plain" | "compressed";
+        sse_compressed: optional (op_index, vv4, vi4, ii4, iv4), the reference's
+        SSE-compressed form of the same operator, used instead of op's arrays."""
         L = lib()
         if L.ems_device_count() <= 0:
             raise RuntimeError("EngineHIP needs a HIP device; none is visible")
@@ -24,8 +68,20 @@ class EngineHIP:
         self.N = op.N
         h = C.c_void_p()
         n = (C.c_uint * 3)(*op.N)
-        check(L.ems_create(C.byref(h), n, op.coeff_ptr("vv"), op.coeff_ptr("vi"), op.coeff_ptr("ii"),
-                           op.coeff_ptr("iv"), int(device)))
+        lay = LAYOUTS[layout] if isinstance(layout, str) else int(layout)
+        if sse_compressed is not None:
+            opi, *tabs = sse_compressed
+            opi = np.ascontiguousarray(opi, np.uint32)
+            keep, ptrs = [opi], []
+            for t in tabs:
+                a, p = _table_ptrs(t)
+                keep.append(a)
+                ptrs.append(p)
+            check(L.ems_create_sse_compressed(C.byref(h), n, ptr(opi, C.c_uint), len(tabs[0][0]), *ptrs,
+                                              int(device), lay))
+        else:
+            check(L.ems_create_layout(C.byref(h), n, op.coeff_ptr("vv"), op.coeff_ptr("vi"), op.coeff_ptr("ii"),
+                                      op.coeff_ptr("iv"), int(device), lay))
         self.h = h
         self._keep = []
         try:
@@ -103,6 +159,13 @@ class EngineHIP:
 
     def set_profiling(self, flag):
         check(lib().ems_set_profiling(self.h, int(flag)))
+
+    def layout(self):
+        """(layout, n_unique, pml_layout, n_unique_pml) as names and table sizes"""
+        a, b, c, d = C.c_int(), C.c_uint(), C.c_int(), C.c_uint()
+        check(lib().ems_get_layout(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
+        names = {0: None, 1: "plain", 2: "compressed"}
+        return names[a.value], b.value, names[c.value], d.value
 
     def last_timing(self):
         t, v, c = C.c_double(), C.c_double(), C.c_double()

@@ -1058,6 +1058,18 @@ void ref_get_opindex(void* hh, unsigned* out)
             for (unsigned v = 0; v < e->nV; ++v) out[q++] = e->opidx[((size_t)x * e->N[1] + y) * e->N[2] + v];
 }
 
+// compressed tables f4_{vv,vi,ii,iv}_Compressed[n] (which = 0..3), U x 4 floats
+void ref_get_ctab(void* hh, int which, int n, float* out)
+{
+    Handle* h = (Handle*)hh;
+    SSEEngine* e = h->se;
+    if (!e || !e->compressed || which < 0 || which > 3 || n < 0 || n > 2) return;
+    const std::vector<f4vector>* t[4] = {e->cvv, e->cvi, e->cii, e->civ};
+    const auto& v = t[which][n];
+    for (size_t u = 0; u < v.size(); ++u)
+        for (int l = 0; l < 4; ++l) out[4 * u + l] = v[u].f[l];
+}
+
 // CalcFastEnergy (engine_interface_fdtd.cpp:302-347): sum over [0,N-1)^3 in double
 double ref_energy(void* hh)
 {

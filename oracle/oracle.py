@@ -69,6 +69,7 @@ def lib():
         L.ref_num_unique.argtypes = [vp]
         L.ref_num_unique.restype = C.c_uint
         L.ref_get_opindex.argtypes = [vp, up]
+        L.ref_get_ctab.argtypes = [vp, C.c_int, C.c_int, fp]
         L.ref_energy.argtypes = [vp]
         L.ref_energy.restype = C.c_double
         L.ref_destroy.argtypes = [vp]
@@ -164,6 +165,26 @@ class RefEngine:
         out = np.zeros((self.N[0], self.N[1], nV), np.uint32)
         lib().ref_get_opindex(self.h, _p(out, C.c_uint))
         return out
+
+    def op_index_full(self):
+        """m_Op_index in its allocated layout [Nx][Ny][Nz] (entries z >= nV unused, zero)"""
+        o = self.opindex()
+        full = np.zeros(tuple(self.N), np.uint32)
+        full[:, :, :o.shape[2]] = o
+        return full
+
+    def ctables(self):
+        """(vv4, vi4, ii4, iv4): each [3] arrays of U x 4 floats (f4_*_Compressed[n])"""
+        U = self.num_unique()
+        out = []
+        for which in range(4):
+            per = []
+            for n in range(3):
+                a = np.zeros((U, 4), np.float32)
+                lib().ref_get_ctab(self.h, which, n, _p(a, C.c_float))
+                per.append(a)
+            out.append(per)
+        return tuple(out)
 
     def close(self):
         if self.h:

@@ -34,17 +34,22 @@ def _fields_equal(eng, ref_eng):
     return v, c, rv, rc
 
 
+@pytest.mark.parametrize("layout", ["plain", "compressed"])
 @pytest.mark.parametrize("make", [
     lambda: scenes.pec_cavity(32, 150),
     lambda: scenes.upml_bricks(48, 150, nbricks=8, dump=False),
     lambda: scenes.mur_layers(40, 36, 32, 200),
 ])
-def test_fields_bit_identical(gpu, make):
+def test_fields_bit_identical(gpu, make, layout):
     scene = make()
     nrts = scene["nrts"]
-    print("case", scene["name"], "steps", nrts, flush=True)
+    print("case", scene["name"], "steps", nrts, "layout", layout, flush=True)
     op = Operator(scene)
-    eng = EngineHIP(op)
+    eng = EngineHIP(op, layout=layout)
+    lay, nu, play, nup = eng.layout()
+    assert lay == layout and (nu > 0) == (layout == "compressed")
+    if op.upml_slabs():
+        assert play == layout and (nup > 0) == (layout == "compressed")
     eng.IterateTS(nrts)
     ref = RefEngine(_ref_from_scene(scene), "basic")
     ref.iterate(nrts)
@@ -136,15 +141,18 @@ def test_linearity_full_size_upml(gpu):
 
 
 def test_full_size_upml_against_oracle(gpu):
-    """256^3 UPML bricks workload, 40 steps, bit-identical to the oracle engine."""
+    """256^3 UPML bricks workload, 40 steps, both coefficient layouts
+    bit-identical to the oracle engine."""
     scene = scenes.upml_bricks(256, 40, dump=False)
-    print("case upml_bricks_256 oracle parity 40 steps", flush=True)
     op = Operator(scene)
-    eng = EngineHIP(op)
-    eng.IterateTS(40)
-    v = eng.get_volt()
-    c = eng.get_curr()
-    eng.close()
+    got = {}
+    for layout in ("compressed", "plain"):
+        print("case upml_bricks_256 oracle parity 40 steps, layout", layout, flush=True)
+        eng = EngineHIP(op, layout=layout)
+        assert eng.layout()[0] == layout and eng.layout()[2] == layout
+        eng.IterateTS(40)
+        got[layout] = (eng.get_volt(), eng.get_curr())
+        eng.close()
 
     class _Op:   # the operator is pinned separately (test_operator_parity); reuse its arrays
         pass
@@ -168,5 +176,27 @@ def test_full_size_upml_against_oracle(gpu):
     ref = RefEngine(ro, "basic")
     ref.iterate(40)
     rv, rc = ref.raw_views()
-    assert np.array_equal(v, rv), f"voltage mismatches {np.count_nonzero(v != rv)}"
-    assert np.array_equal(c, rc), f"current mismatches {np.count_nonzero(c != rc)}"
+    for layout, (v, c) in got.items():
+        assert np.array_equal(v, rv), f"{layout}: voltage mismatches {np.count_nonzero(v != rv)}"
+        assert np.array_equal(c, rc), f"{layout}: current mismatches {np.count_nonzero(c != rc)}"
+
+
+def test_sse_compressed_operator_import(gpu):
+    """An engine created from the reference's SSE-compressed operator form
+    (m_Op_index + f4 tables, here the oracle's restatement of it) runs
+    bit-identically to the oracle's sse-compressed engine."""
+    scene = scenes.upml_bricks(40, 120, nbricks=8, dump=False)
+    print("case upml_bricks_40 sse-compressed import, 120 steps", flush=True)
+    op = Operator(scene)
+    ref = RefEngine(_ref_from_scene(scene), "sse_compressed")
+    form = (ref.op_index_full(),) + ref.ctables()
+    eng = EngineHIP(op, sse_compressed=form)
+    assert eng.layout()[0] == "compressed"
+    eng.IterateTS(120)
+    ref.iterate(120)
+    v, c = eng.get_volt(), eng.get_curr()
+    rv, rc = ref.fields()
+    assert np.max(np.abs(rv)) > 0
+    assert np.array_equal(v, rv) and np.array_equal(c, rc)
+    eng.close()
+    ref.close()
