@@ -19,7 +19,10 @@ update of one half step, by coefficient layout:
                from a table of unique tuples that stays in cache);
 plus, per UPML position, 15 floats = 60 B (plain UPML: 3 flux read + 3 flux
 write + 9 coefficients) or 6 floats + one uint32 index = 28 B (compressed
-UPML).  `plain_ceiling_frac` compares the cell-update rate with the HBM
+UPML).  The fused single-sweep step (k_step) moves a whole timestep per
+launch: per cell 12 floats (read and write V and I) + 12 coefficient floats
+(plain) or one uint16 index (compressed) = 96 B or 50 B; per UPML position
+the two half steps' UPML bytes (120 B plain, 52 B compressed).  `plain_ceiling_frac` compares the cell-update rate with the HBM
 ceiling of the plain layout (8 TB/s / 143.57 B per cell-step).
 
 cpu_baseline: the oracle's restated multithreaded SSE-compressed engine
@@ -41,6 +44,8 @@ METRIC = "MCells/s & % HBM roofline (256³ UPML); batched speedup; adjoint/forwa
 
 LAYOUT_BYTES = {"plain": 60.0, "compressed": 38.0}        # per cell, per half step
 PML_BYTES = {"plain": 60.0, "compressed": 28.0}           # per UPML position, per half step
+FUSED_BYTES = {"plain": 96.0, "compressed": 50.0}         # per cell, per fused step
+FUSED_PML_BYTES = {"plain": 120.0, "compressed": 52.0}    # per UPML position, per fused step
 PLAIN_CELL_STEP_BYTES = 120.0
 
 
@@ -128,6 +133,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--layout", default="auto", choices=["auto", "plain", "compressed"])
+    ap.add_argument("--step", default="auto", choices=["auto", "fused", "halves"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -147,7 +153,8 @@ def main():
     n = args.size
     scene = scenes.upml_bricks(n, 10000)
     t0 = time.perf_counter()
-    sim = Simulation(scene, layout=args.layout)
+    sim = Simulation(scene, layout=args.layout,
+                     fused={"auto": None, "fused": True, "halves": False}[args.step])
     setup_s = time.perf_counter() - t0
     eng = sim.eng
     op = sim.op
@@ -167,12 +174,22 @@ def main():
     value = cells * args.steps / wall / 1e6
     lay, n_unique, play, n_unique_pml = eng.layout()
     play = play or "plain"
-    bytes_per_launch = LAYOUT_BYTES[lay] * cells + PML_BYTES[play] * npml
+    fused, zdepth = eng.fused()
+    if fused:
+        bytes_per_launch = FUSED_BYTES[lay] * cells + FUSED_PML_BYTES[play] * npml
+        step_bytes = bytes_per_launch
+    else:
+        bytes_per_launch = LAYOUT_BYTES[lay] * cells + PML_BYTES[play] * npml
+        step_bytes = 2 * bytes_per_launch
     plain_step_bytes = PLAIN_CELL_STEP_BYTES * (cells + npml)
-    kname, kms = ("k_volt", volt_ms) if volt_ms >= curr_ms else ("k_curr", curr_ms)
+    if fused:
+        kname, kms = "k_step", volt_ms
+    else:
+        kname, kms = ("k_volt", volt_ms) if volt_ms >= curr_ms else ("k_curr", curr_ms)
     achieved = bytes_per_launch / (kms * 1e-3) / 1e9
     layout = f"{lay} coefficients ({n_unique} unique) + fused {play} UPML ({n_unique_pml} unique)" \
         if lay == "compressed" or play == "compressed" else "plain fp32 coefficients + fused UPML"
+    layout += f", {'single-sweep fused step' if fused else 'two half steps'} (z depth {zdepth})"
     traffic = traffic_from_profile(kname, layout, n)
     plain_ceiling = HBM_PEAK_GBS * 1e9 / (plain_step_bytes / cells) / 1e6     # MCells/s
     out = {
@@ -192,7 +209,7 @@ def main():
                                "40 seeded dielectric bricks, 8-cell UPML on all faces, 50 Ohm lumped port "
                                "with Gaussian fc=20 GHz, port V/I probes, E-field DFT plane at 5/10/15 GHz",
                    "cells": int(cells), "pml_positions": int(npml), "layout": layout,
-                   "bytes_per_cell_update": round(2 * bytes_per_launch / cells, 2),
+                   "bytes_per_cell_update": round(step_bytes / cells, 2),
                    "plain_bytes_per_cell_update": round(plain_step_bytes / cells, 2)},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -27,6 +27,8 @@
  *                                                    FDTD/extensions/engine_ext_lorentzmaterial.cpp:79-173
  *   ems_add_lumped_rlc    Engine_Ext_LumpedRLC                            FDTD/extensions/engine_ext_lumpedRLC.cpp:83-147
  *   ems_iterate           Engine::IterateTS                              FDTD/engine.cpp:267-286
+ *   ems_set_fused         Engine::IterateTS step form (single sweep or two half steps)
+ *   ems_get_fused                                                        FDTD/engine.cpp:267-286
  *   ems_get_numts         Engine::GetNumberOfTimesteps                   FDTD/engine.h:74
  *   ems_get_volt/curr     Engine::GetVolt / GetCurr (bulk)                FDTD/engine.h:77-93
  *   ems_set_volt/curr     Engine::SetVolt / SetCurr (bulk)                FDTD/engine.h:95-101
@@ -109,6 +111,20 @@ int ems_compress_operator(const unsigned numLines[3], const float* vv, const flo
 int ems_expand_sse_compressed(const unsigned numLines[3], const unsigned* op_index, unsigned n_unique,
                               const float* const vv4[3], const float* const vi4[3], const float* const ii4[3],
                               const float* const iv4[3], float* vv, float* vi, float* ii, float* iv);
+
+/* Step form, before ems_init.  The fused form advances a whole timestep in
+ * one z-sweep over double-buffered fields (voltage update, voltage sources,
+ * current update, current sources, UPML in both halves), bit-identical to the
+ * two-half-step form with about 40 % fewer algorithmic bytes.  It needs every active
+ * extension to have a fused form: Mur, Lorentz / Drude and lumped RLC keep the
+ * two-half-step form, as do current sources on the last line of a direction
+ * and more than 64 sources in one 64 x 16 column tile.
+ * mode: -1 automatic (default: two half steps, the faster form on MI355X
+ * so far), 0 two half steps, 1 fused (ems_init then fails with EMS_E_ARG
+ * when the engine has no fused form). */
+int ems_set_fused(ems_engine* h, int mode);
+/* fused: 1 when the fused form is active; zdepth: z planes per workgroup */
+int ems_get_fused(const ems_engine* h, int* fused, unsigned* zdepth);
 
 /* Active layouts and table sizes (known after ems_init for the UPML part):
  * layout, pml_layout: EMS_LAYOUT_PLAIN or EMS_LAYOUT_COMPRESSED (pml_layout 0

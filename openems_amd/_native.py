@@ -75,6 +75,8 @@ ENGINE_API = [
     ("ems_expand_sse_compressed", C.c_int, [up, up, C.c_uint, C.POINTER(fp), C.POINTER(fp), C.POINTER(fp),
                                             C.POINTER(fp), fp, fp, fp, fp]),
     ("ems_get_layout", C.c_int, [vp, ip, up, ip, up]),
+    ("ems_set_fused", C.c_int, [vp, C.c_int]),
+    ("ems_get_fused", C.c_int, [vp, ip, up]),
     ("ems_set_excitation", C.c_int, [vp, fp, fp, C.c_uint, C.c_int, C.c_uint, up, usp, fp, up,
                                      C.c_uint, up, usp, fp, up]),
     ("ems_add_upml", C.c_int, [vp, up, up, fp, fp, fp, fp, fp, fp]),

@@ -19,6 +19,7 @@
 #include <cstring>
 #include <functional>
 #include <string>
+#include <unordered_map>
 #include <thread>
 #include <vector>
 
@@ -67,24 +68,30 @@ struct CoefArgs {
     const float4* tab;
 };
 
-// One UPML slab (Operator_Ext_UPML range) in the device table.  All slabs of
-// one half step share one buffer.  Plain UPML layout: a slab's block holds
-// twelve arrays [c0, cfo, cfn, flux][component][local index]; compressed UPML
-// layout: four arrays [flux][component][local] + [table index][local], the
-// index stored as the bit pattern of a uint32.  The local index is x-fastest.
+// One UPML slab (Operator_Ext_UPML range).  The positions of all slabs are
+// numbered consecutively (slab after slab in creation order, x fastest inside
+// a slab); every UPML array is [component or coefficient][position].
 struct SlabTab {
     int x0, x1, y0, y1, z0, z1;    // inclusive mesh ranges
     unsigned nx, ny;
-    unsigned long long off;        // element offset of the slab block
-    unsigned long long cs;         // nx*ny*nz
+    unsigned long long off;        // number of the slab's first position
 };
 
+// UPML of one half step.  Plain UPML: coef[9][npos] = {c0, cfo, cfn} x
+// {x, y, z}.  Compressed UPML: pidx[npos] into a table of three float4 per
+// entry, {c0_x, c0_y, c0_z, cfo_x}, {cfo_y, cfo_z, cfn_x, cfn_y},
+// {cfn_z, 0, 0, 0}.  Flux [3][npos] is read from flux_in and written to
+// flux_out (the same buffer when the half step updates in place).  The slab
+// table travels in the kernel arguments.
 struct PmlArgs {
     int n;                         // number of slabs (0: no UPML)
-    const SlabTab* tab;            // [n]
-    float* buf;                    // slab blocks
-    const float4* ptab;            // compressed UPML: 3 float4 per entry
-                                   // {c0_x, c0_y, c0_z, cfo_x}, {cfo_y, cfo_z, cfn_x, cfn_y}, {cfn_z, 0, 0, 0}
+    unsigned long long npos;
+    SlabTab tab[kMaxSlabs];
+    const float* coef;
+    const unsigned* pidx;
+    const float4* ptab;
+    const float* flux_in;
+    float* flux_out;
 };
 
 enum PmlMode { kNoPml = 0, kPmlPlain = 1, kPmlCompressed = 2 };
@@ -99,64 +106,37 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb)
     return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
 }
 
-// UPML membership of one (x, y) column: at most two slabs of the
-// Create_UPML layout share a column (lower / upper slab of one direction).
-struct Column {
-    long long base0, base1;        // slab block offset + local index of (x, y, slab z0)
-    int z00, z10, z01, z11;        // z ranges of the two slabs (empty: z0 > z1)
-    unsigned long long plane0, plane1, cs0, cs1;
-};
-
-__device__ __forceinline__ void column_slabs(const PmlArgs& pml, unsigned x, unsigned y, Column& c)
+// UPML position of cell (x, y, z), or false outside every slab (slabs do not
+// overlap; positions fit 32 bits, checked on the host).  The slab is selected
+// first and its position computed once.
+__device__ __forceinline__ bool pml_at(const PmlArgs& P, int x, int y, int z, unsigned long long& li)
 {
-    c.base0 = c.base1 = 0;
-    c.z00 = c.z01 = 1;
-    c.z10 = c.z11 = 0;
-    c.plane0 = c.plane1 = c.cs0 = c.cs1 = 0;
-    bool first = true;
-    for (int s = 0; s < pml.n; ++s) {
-        const SlabTab S = pml.tab[s];
-        if ((int)x >= S.x0 && (int)x <= S.x1 && (int)y >= S.y0 && (int)y <= S.y1) {
-            const long long b = (long long)S.off + (long long)(x - S.x0) + (long long)S.nx * (long long)(y - S.y0);
-            const unsigned long long pl = (unsigned long long)S.nx * S.ny;
-            if (first) {
-                c.base0 = b; c.z00 = S.z0; c.z10 = S.z1; c.plane0 = pl; c.cs0 = S.cs;
-                first = false;
-            } else {
-                c.base1 = b; c.z01 = S.z0; c.z11 = S.z1; c.plane1 = pl; c.cs1 = S.cs;
-            }
+    int sel = -1;
+#pragma unroll
+    for (int s = 0; s < (int)kMaxSlabs; ++s) {
+        const SlabTab& S = P.tab[s];
+        if (s < P.n && x >= S.x0 && x <= S.x1 && y >= S.y0 && y <= S.y1 && z >= S.z0 && z <= S.z1) sel = s;
+    }
+    int bx = 0, by = 0, bz = 0;
+    unsigned nx = 0, ny = 0, off = 0;
+#pragma unroll
+    for (int s = 0; s < (int)kMaxSlabs; ++s) {
+        if (sel == s) {
+            const SlabTab& S = P.tab[s];
+            bx = S.x0; by = S.y0; bz = S.z0;
+            nx = S.nx; ny = S.ny; off = (unsigned)S.off;
         }
     }
+    li = off + (unsigned)(x - bx) + nx * ((unsigned)(y - by) + ny * (unsigned)(z - bz));
+    return sel >= 0;
 }
 
-// slab-local index of plane z, or false when the cell is outside the UPML
-__device__ __forceinline__ bool column_at(const Column& c, int z, unsigned long long& li, unsigned long long& scs)
+// main coefficients of cell p
+template <bool CC>
+__device__ __forceinline__ void load_coef(const CoefArgs& k, unsigned long long cs, unsigned long long p, unsigned u,
+                                          float a[3], float b[3])
 {
-    const bool a = z >= c.z00 && z <= c.z10;
-    const bool b = z >= c.z01 && z <= c.z11;
-    const long long base = a ? c.base0 : c.base1;
-    const unsigned long long pl = a ? c.plane0 : c.plane1;
-    const int zs = a ? c.z00 : c.z01;
-    scs = a ? c.cs0 : c.cs1;
-    li = (unsigned long long)(base + (long long)pl * (z - zs));
-    return a || b;
-}
-
-// Fused half-step update of the three components of one cell.
-//   plain: X = field;  f = X*cc + cv*curl;  field <- f
-//   UPML : X = flux;   f = X*cc + cv*curl;  field <- (c0*field - cfo*flux) + cfn*f;  flux <- f
-// The loads shared by both forms are issued once for every lane; the UPML
-// lanes add exec-masked loads (coefficients or a table index, and the flux).
-template <int PM, bool CC>
-__device__ __forceinline__ void cell_update(float* __restrict__ F, const CoefArgs& k, unsigned long long cs,
-                                            unsigned long long p, const float* curl, bool pml, const PmlArgs& P,
-                                            unsigned long long li, unsigned long long scs)
-{
-    float x0[3], a[3], b[3];
-#pragma unroll
-    for (int n = 0; n < 3; ++n) x0[n] = F[n * cs + p];
     if (CC) {
-        const unsigned u = k.idx[p];
         const float4 t0 = k.tab[2 * u], t1 = k.tab[2 * u + 1];
         a[0] = t0.x; b[0] = t0.y;
         a[1] = t0.z; b[1] = t0.w;
@@ -168,27 +148,39 @@ __device__ __forceinline__ void cell_update(float* __restrict__ F, const CoefArg
             b[n] = k.cv[n * cs + p];
         }
     }
-    float* __restrict__ pbuf = P.buf;
+}
+
+// Update of the three components of one cell from its old values x0.
+//   plain: f = x0*cc + cv*curl;  out = f
+//   UPML : f = F*cc + cv*curl;   out = (c0*x0 - cfo*F) + cfn*f;  F <- f
+// (F: the UPML flux).  The UPML lanes add exec-masked loads of the flux and
+// of their coefficients or table index.
+template <int PM>
+__device__ __forceinline__ void cell_math(const float x0[3], const float a[3], const float b[3], const float curl[3],
+                                          bool pml, const PmlArgs& P, unsigned long long li, bool store_flux,
+                                          float out[3])
+{
     float k0[3], ko[3], kn[3], fl[3];
+    if (PM != kNoPml && pml) {
+#pragma unroll
+        for (int n = 0; n < 3; ++n) fl[n] = P.flux_in[n * P.npos + li];
+    }
     if (PM == kPmlPlain && pml) {
 #pragma unroll
         for (int n = 0; n < 3; ++n) {
-            k0[n] = pbuf[li + n * scs];
-            ko[n] = pbuf[li + (3 + n) * scs];
-            kn[n] = pbuf[li + (6 + n) * scs];
-            fl[n] = pbuf[li + (9 + n) * scs];
+            k0[n] = P.coef[n * P.npos + li];
+            ko[n] = P.coef[(3 + n) * P.npos + li];
+            kn[n] = P.coef[(6 + n) * P.npos + li];
         }
     }
     if (PM == kPmlCompressed && pml) {
-#pragma unroll
-        for (int n = 0; n < 3; ++n) fl[n] = pbuf[li + n * scs];
-        const unsigned u = __float_as_uint(pbuf[li + 3 * scs]);
+        const unsigned u = P.pidx[li];
         const float4 t0 = P.ptab[3 * u], t1 = P.ptab[3 * u + 1], t2 = P.ptab[3 * u + 2];
         k0[0] = t0.x; k0[1] = t0.y; k0[2] = t0.z;
         ko[0] = t0.w; ko[1] = t1.x; ko[2] = t1.y;
         kn[0] = t1.z; kn[1] = t1.w; kn[2] = t2.x;
     }
-    float out[3], fnew[3];
+    float fnew[3];
 #pragma unroll
     for (int n = 0; n < 3; ++n) {
         const float xin = (PM != kNoPml && pml) ? fl[n] : x0[n];
@@ -202,12 +194,40 @@ __device__ __forceinline__ void cell_update(float* __restrict__ F, const CoefArg
             out[n] = f;
         }
     }
+    if (PM != kNoPml && pml && store_flux) {
 #pragma unroll
-    for (int n = 0; n < 3; ++n) F[n * cs + p] = out[n];
-    if (PM != kNoPml && pml) {
-        const unsigned long long fo = PM == kPmlPlain ? 9 : 0;
+        for (int n = 0; n < 3; ++n) P.flux_out[n * P.npos + li] = fnew[n];
+    }
+}
+
+// UPML hooks alone, for a UPML position the main update skips (the last
+// current line of a direction): Engine_Ext_UPML's pre and post hooks run over
+// the whole slab (engine_ext_upml.cpp:135-234), so there
+//   out = (c0*x0 - cfo*F) + cfn*F,  F unchanged.
+template <int PM>
+__device__ __forceinline__ void pml_hooks_only(const float x0[3], const PmlArgs& P, unsigned long long li, float out[3])
+{
+    float k0[3], ko[3], kn[3], fl[3];
 #pragma unroll
-        for (int n = 0; n < 3; ++n) pbuf[li + (fo + n) * scs] = fnew[n];
+    for (int n = 0; n < 3; ++n) fl[n] = P.flux_in[n * P.npos + li];
+    if (PM == kPmlPlain) {
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            k0[n] = P.coef[n * P.npos + li];
+            ko[n] = P.coef[(3 + n) * P.npos + li];
+            kn[n] = P.coef[(6 + n) * P.npos + li];
+        }
+    } else {
+        const unsigned u = P.pidx[li];
+        const float4 t0 = P.ptab[3 * u], t1 = P.ptab[3 * u + 1], t2 = P.ptab[3 * u + 2];
+        k0[0] = t0.x; k0[1] = t0.y; k0[2] = t0.z;
+        ko[0] = t0.w; ko[1] = t1.x; ko[2] = t1.y;
+        kn[0] = t1.z; kn[1] = t1.w; kn[2] = t2.x;
+    }
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+        const float t = k0[n] * x0[n] - ko[n] * fl[n];
+        out[n] = t + kn[n] * fl[n];
     }
 }
 
@@ -219,10 +239,10 @@ __device__ __forceinline__ void cell_update(float* __restrict__ F, const CoefArg
 //   post: F <- V;  V <- t + vvfn*F
 // The lower neighbour at index 0 is the cell itself (shift = bool(pos)).
 // Each thread owns one (x, y) column and sweeps g.zdepth planes, carrying the
-// z-1 currents in registers.
+// z-1 currents in registers.  In place.
 // ---------------------------------------------------------------------------
 template <int PM, bool CC>
-__global__ __launch_bounds__(kBlock) void k_volt(const GridDesc g, float* __restrict__ V, const float* __restrict__ I,
+__global__ __launch_bounds__(kBlock) void k_volt(const GridDesc g, float* V, const float* __restrict__ I,
                                                  const CoefArgs k, const PmlArgs pml)
 {
     const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
@@ -243,9 +263,6 @@ __global__ __launch_bounds__(kBlock) void k_volt(const GridDesc g, float* __rest
     const unsigned long long pz = z0 ? p - g.plane : p;
     float Ix_zm = Ix[pz], Iy_zm = Iy[pz];
 
-    Column col;
-    if (PM != kNoPml) column_slabs(pml, x, y, col);
-
     for (unsigned z = z0; z < z1; ++z, p += g.plane) {
         const float ix = Ix[p], iy = Iy[p], iz = Iz[p];
         const float iz_ym = Iz[p - ym], ix_ym = Ix[p - ym];
@@ -256,22 +273,27 @@ __global__ __launch_bounds__(kBlock) void k_volt(const GridDesc g, float* __rest
         curl[2] = iy - iy_xm - ix + ix_ym;
         Ix_zm = ix;
         Iy_zm = iy;
-        bool in_pml = false;
-        unsigned long long li = 0, scs = 0;
-        if (PM != kNoPml) in_pml = column_at(col, (int)z, li, scs);
-        cell_update<PM, CC>(V, k, cs, p, curl, in_pml, pml, li, scs);
+        float x0[3], a[3], b[3], out[3];
+#pragma unroll
+        for (int n = 0; n < 3; ++n) x0[n] = V[n * cs + p];
+        load_coef<CC>(k, cs, p, CC ? k.idx[p] : 0u, a, b);
+        unsigned long long li = 0;
+        const bool in_pml = PM != kNoPml && pml_at(pml, (int)x, (int)y, (int)z, li);
+        cell_math<PM>(x0, a, b, curl, in_pml, pml, li, true, out);
+#pragma unroll
+        for (int n = 0; n < 3; ++n) V[n * cs + p] = out[n];
     }
 }
 
 // ---------------------------------------------------------------------------
 // current half step (Engine::UpdateCurrents, FDTD/engine.cpp:170-222), same
 // UPML fusion with the current flux (engine_ext_upml.cpp:135-234).  Updates
-// x < Nx-1, y < Ny-1, z < Nz-1.  Thread (0,0) of tile 0 advances the device
-// timestep counter (++numTS of Engine::IterateTS): no kernel of this launch
-// reads it, and every later kernel sees the new value.
+// x < Nx-1, y < Ny-1, z < Nz-1, in place.  Thread (0,0) of tile 0 advances the
+// device timestep counter (++numTS of Engine::IterateTS): no kernel of this
+// launch reads it, and every later kernel sees the new value.
 // ---------------------------------------------------------------------------
 template <int PM, bool CC>
-__global__ __launch_bounds__(kBlock) void k_curr(const GridDesc g, float* __restrict__ I, const float* __restrict__ V,
+__global__ __launch_bounds__(kBlock) void k_curr(const GridDesc g, float* I, const float* __restrict__ V,
                                                  const CoefArgs k, const PmlArgs pml, unsigned* __restrict__ numts)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(numts, 1u);
@@ -280,9 +302,11 @@ __global__ __launch_bounds__(kBlock) void k_curr(const GridDesc g, float* __rest
     const unsigned q = qt * kBlock + threadIdx.x;
     if (q >= g.plane) return;
     const unsigned y = q / g.Nx, x = q - y * g.Nx;
-    if (x + 1 >= g.Nx || y + 1 >= g.Ny) return;
+    // columns / planes the main update skips only matter for UPML positions
+    const bool col_upd = x + 1 < g.Nx && y + 1 < g.Ny;
+    if (PM == kNoPml && !col_upd) return;
     const unsigned z0 = zc * g.zdepth;
-    const unsigned z1 = min(z0 + g.zdepth, g.Nz - 1);
+    const unsigned z1 = min(z0 + g.zdepth, PM == kNoPml ? g.Nz - 1 : g.Nz);
     if (z0 >= z1) return;
     const unsigned long long cs = g.cs;
     const unsigned Nx = g.Nx;
@@ -292,12 +316,25 @@ __global__ __launch_bounds__(kBlock) void k_curr(const GridDesc g, float* __rest
     const float* __restrict__ Vy = V + cs;
     const float* __restrict__ Vz = V + 2 * cs;
     // values at z carried from the previous iteration's z+1 loads
-    float vx = Vx[p], vy = Vy[p];
-
-    Column col;
-    if (PM != kNoPml) column_slabs(pml, x, y, col);
+    float vx = 0.0f, vy = 0.0f;
+    if (col_upd) {
+        vx = Vx[p];
+        vy = Vy[p];
+    }
 
     for (unsigned z = z0; z < z1; ++z, p += g.plane) {
+        if (PM != kNoPml && !(col_upd && z + 1 < g.Nz)) {
+            unsigned long long li = 0;
+            if (pml_at(pml, (int)x, (int)y, (int)z, li)) {
+                float x0[3], out[3];
+#pragma unroll
+                for (int n = 0; n < 3; ++n) x0[n] = I[n * cs + p];
+                pml_hooks_only<PM>(x0, pml, li, out);
+#pragma unroll
+                for (int n = 0; n < 3; ++n) I[n * cs + p] = out[n];
+            }
+            continue;
+        }
         const unsigned long long pzp = p + g.plane;
         const float vz = Vz[p];
         const float vx_zp = Vx[pzp], vy_zp = Vy[pzp];
@@ -309,34 +346,306 @@ __global__ __launch_bounds__(kBlock) void k_curr(const GridDesc g, float* __rest
         curl[2] = vy - vy_xp - vx + vx_yp;
         vx = vx_zp;
         vy = vy_zp;
-        bool in_pml = false;
-        unsigned long long li = 0, scs = 0;
-        if (PM != kNoPml) in_pml = column_at(col, (int)z, li, scs);
-        cell_update<PM, CC>(I, k, cs, p, curl, in_pml, pml, li, scs);
+        float x0[3], a[3], b[3], out[3];
+#pragma unroll
+        for (int n = 0; n < 3; ++n) x0[n] = I[n * cs + p];
+        load_coef<CC>(k, cs, p, CC ? k.idx[p] : 0u, a, b);
+        unsigned long long li = 0;
+        const bool in_pml = PM != kNoPml && pml_at(pml, (int)x, (int)y, (int)z, li);
+        cell_math<PM>(x0, a, b, curl, in_pml, pml, li, true, out);
+#pragma unroll
+        for (int n = 0; n < 3; ++n) I[n * cs + p] = out[n];
+    }
+}
+
+// Excitation entry as the fused step applies it: cell, component, amplitude
+// and delay of one Operator_Ext_Excitation list element.
+struct ExcEnt {
+    unsigned short x, y, z, dir;
+    float amp;
+    unsigned delay;
+};
+
+// sample of the excitation signal at timestep ts for a delay
+// (Engine_Ext_Excitation::Apply2Voltages, engine_ext_excitation.cpp:33-61)
+__device__ __forceinline__ int exc_sample(int ts, unsigned delay, int period, unsigned length)
+{
+    const int p = period > 0 ? period : ts + 1;
+    int e = ts - (int)delay;
+    e *= (e > 0);
+    e %= p;
+    e *= (e < (int)length);
+    return e;
+}
+
+struct ExcArgs {
+    const unsigned* beg;           // [ntiles + 1] entry ranges per xy tile, each sorted by z
+    const ExcEnt* ent;
+    const float* sig;
+    unsigned length;
+    int period;
+};
+
+// ---------------------------------------------------------------------------
+// One full timestep in one z-sweep (voltage half step, voltage excitation,
+// current half step, current excitation, with UPML fused in both halves):
+// the same per-element arithmetic as k_volt / k_curr + the excitation hooks,
+// so results are bit-identical to the two-half-step path.
+//
+// Fields and UPML fluxes are double-buffered: the step reads state n (V0, I0,
+// flux_in) and writes state n+1 (V1, I1, flux_out), so workgroups never see
+// each other's new values.  A workgroup owns a tile of kTX x kTY (x, y)
+// columns and a z chunk; each wave owns 4 rows of 64 columns.  Per plane z it
+// computes V^{n+1}(z) for its tile plus a halo (column x0+kTX, row y0+kTY) into
+// LDS (the halo values are recomputed, not stored), then I^{n+1}(z-1) from the
+// LDS planes z-1 and z.  The first plane of a chunk reads I^n(z0-1) from
+// memory; the plane z1 after the chunk is recomputed for I^{n+1}(z1-1).
+// Current-side sources must lie inside the updated range (x < Nx-1, ...).
+// The last workgroup to finish advances the device timestep counter.
+// ---------------------------------------------------------------------------
+constexpr unsigned kTX = 64, kTY = 16, kRowsPerWave = kTY / (kBlock / 64);
+
+struct StepArgs {
+    GridDesc g;
+    unsigned ntx, nty;             // xy tiles
+    const float* V0;
+    const float* I0;
+    float* V1;
+    float* I1;
+    CoefArgs kV, kI;
+    PmlArgs pV, pI;
+    ExcArgs eV, eI;
+    unsigned* numts;               // [0] timestep counter, [1] finished-workgroup ticket
+};
+
+template <int PM, bool CC>
+__device__ __forceinline__ void volt_cell(const StepArgs& a, unsigned x, unsigned y, unsigned z, unsigned long long p,
+                                          unsigned u, float ix, float iy, float iz, float ix_zm, float iy_zm,
+                                          bool store_flux, int ts, unsigned eb, unsigned ee, float v[3])
+{
+    const GridDesc& g = a.g;
+    const unsigned long long cs = g.cs;
+    const unsigned xm = x ? 1u : 0u, ym = y ? g.Nx : 0u;
+    const float* __restrict__ I0 = a.I0;
+    const float iz_ym = I0[2 * cs + p - ym], ix_ym = I0[p - ym];
+    const float iz_xm = I0[2 * cs + p - xm], iy_xm = I0[cs + p - xm];
+    float curl[3];
+    curl[0] = iz - iz_ym - iy + iy_zm;
+    curl[1] = ix - ix_zm - iz + iz_xm;
+    curl[2] = iy - iy_xm - ix + ix_ym;
+    float x0[3], ca[3], cb[3];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) x0[n] = a.V0[n * cs + p];
+    load_coef<CC>(a.kV, cs, p, u, ca, cb);
+    unsigned long long li = 0;
+    const bool in_pml = PM != kNoPml && pml_at(a.pV, (int)x, (int)y, (int)z, li);
+    cell_math<PM>(x0, ca, cb, curl, in_pml, a.pV, li, store_flux, v);
+    // voltage sources of this cell in list order
+    for (unsigned e = eb; e < ee; ++e) {
+        const ExcEnt E = a.eV.ent[e];
+        if (E.z == z && E.x == x && E.y == y) {
+            const int s = exc_sample(ts, E.delay, a.eV.period, a.eV.length);
+            v[E.dir] = v[E.dir] + E.amp * a.eV.sig[s];
+        }
+    }
+}
+
+// UPML hooks alone at a current cell the main update skips (fused step):
+// I^{n+1} from I^n, the flux unchanged (both flux buffers hold it)
+template <int PM>
+__device__ __forceinline__ void pml_hooks_cell(const StepArgs& a, unsigned x, unsigned y, unsigned z,
+                                               unsigned long long p, const float i0[3])
+{
+    unsigned long long li = 0;
+    if (!pml_at(a.pI, (int)x, (int)y, (int)z, li)) return;
+    float out[3];
+    pml_hooks_only<PM>(i0, a.pI, li, out);
+#pragma unroll
+    for (int n = 0; n < 3; ++n) a.I1[n * a.g.cs + p] = out[n];
+}
+
+template <int PM, bool CC>
+__global__ __launch_bounds__(kBlock) void k_step(const StepArgs a)
+{
+    __shared__ float sV[2][3][kTY + 1][kTX + 1];
+    const GridDesc& g = a.g;
+    const unsigned long long cs = g.cs;
+    const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
+    const unsigned tx = t % a.ntx, ty = (t / a.ntx) % a.nty, zc = t / (a.ntx * a.nty);
+    const unsigned tile = tx + a.ntx * ty;
+    const unsigned x0 = tx * kTX, y0 = ty * kTY;
+    const unsigned z0 = zc * g.zdepth;
+    const unsigned z1 = min(z0 + g.zdepth, g.Nz);
+    const unsigned zlast = min(z1, g.Nz - 1);
+    const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const unsigned x = x0 + lane;
+    const int ts = (int)a.numts[0];
+    const unsigned evb = a.eV.beg[tile], eve = a.eV.beg[tile + 1];
+    const unsigned eib = a.eI.beg[tile], eie = a.eI.beg[tile + 1];
+    // halo cells: column x0+kTX (rows 0..kTY) and row y0+kTY (columns 0..kTX-1)
+    const bool hcol = x0 + kTX <= g.Nx - 1, hrow = y0 + kTY <= g.Ny - 1;
+    const unsigned hi = threadIdx.x;
+    const bool h_on_col = hi <= kTY;
+    const unsigned hx = h_on_col ? x0 + kTX : x0 + (hi - (kTY + 1));
+    const unsigned hy = h_on_col ? y0 + hi : y0 + kTY;
+    const bool has_halo = h_on_col ? (hcol && hy < g.Ny) : (hi < kTY + 1 + kTX && hrow && hx < g.Nx);
+    const unsigned hr = hy - y0, hc = hx - x0;
+
+    // per row: I^n carried along z (x, y components at z-1) and the own I^n
+    // and coefficient index at the previous plane for the current update
+    float izm_x[kRowsPerWave], izm_y[kRowsPerWave];
+    float iown[kRowsPerWave][3], inew[kRowsPerWave][3];
+    unsigned uown[kRowsPerWave], unew[kRowsPerWave];
+    const unsigned long long pz0 = z0 ? (unsigned long long)(z0 - 1) * g.plane : 0ull;
+#pragma unroll
+    for (int r = 0; r < (int)kRowsPerWave; ++r) {
+        const unsigned y = y0 + wave * kRowsPerWave + r;
+        izm_x[r] = izm_y[r] = 0.0f;
+        uown[r] = unew[r] = 0;
+        iown[r][0] = iown[r][1] = iown[r][2] = 0.0f;
+        inew[r][0] = inew[r][1] = inew[r][2] = 0.0f;
+        if (x < g.Nx && y < g.Ny) {
+            const unsigned long long p = x + (unsigned long long)g.Nx * y + (z0 ? pz0 : (unsigned long long)z0 * g.plane);
+            izm_x[r] = a.I0[p];
+            izm_y[r] = a.I0[cs + p];
+        }
+    }
+    float h_izm_x = 0.0f, h_izm_y = 0.0f;
+    if (has_halo) {
+        const unsigned long long p = hx + (unsigned long long)g.Nx * hy + (z0 ? pz0 : 0ull);
+        h_izm_x = a.I0[p];
+        h_izm_y = a.I0[cs + p];
+    }
+
+    for (unsigned z = z0; z <= zlast; ++z) {
+        const unsigned s = z & 1u;
+        const unsigned long long zp = (unsigned long long)z * g.plane;
+        // -- V^{n+1} at plane z: own rows
+#pragma unroll
+        for (int r = 0; r < (int)kRowsPerWave; ++r) {
+            const unsigned row = wave * kRowsPerWave + r;
+            const unsigned y = y0 + row;
+            if (x < g.Nx && y < g.Ny) {
+                const unsigned long long p = x + (unsigned long long)g.Nx * y + zp;
+                const float ix = a.I0[p], iy = a.I0[cs + p], iz = a.I0[2 * cs + p];
+                const unsigned u = CC ? a.kV.idx[p] : 0u;
+                float v[3];
+                volt_cell<PM, CC>(a, x, y, z, p, u, ix, iy, iz, izm_x[r], izm_y[r], z < z1, ts, evb, eve, v);
+                izm_x[r] = ix;
+                izm_y[r] = iy;
+                sV[s][0][row][lane] = v[0];
+                sV[s][1][row][lane] = v[1];
+                sV[s][2][row][lane] = v[2];
+                if (z < z1) {
+#pragma unroll
+                    for (int n = 0; n < 3; ++n) a.V1[n * cs + p] = v[n];
+                }
+                // keep I^n and the coefficient index of this cell for I^{n+1}(z)
+                inew[r][0] = ix; inew[r][1] = iy; inew[r][2] = iz;
+                unew[r] = u;
+            }
+        }
+        // -- halo V^{n+1} at plane z (LDS only)
+        if (has_halo) {
+            const unsigned long long p = hx + (unsigned long long)g.Nx * hy + zp;
+            const float ix = a.I0[p], iy = a.I0[cs + p], iz = a.I0[2 * cs + p];
+            const unsigned u = CC ? a.kV.idx[p] : 0u;
+            float v[3];
+            volt_cell<PM, CC>(a, hx, hy, z, p, u, ix, iy, iz, h_izm_x, h_izm_y, false, ts, evb, eve, v);
+            h_izm_x = ix;
+            h_izm_y = iy;
+            sV[s][0][hr][hc] = v[0];
+            sV[s][1][hr][hc] = v[1];
+            sV[s][2][hr][hc] = v[2];
+        }
+        __syncthreads();
+        // -- I^{n+1} at plane z-1 from the LDS planes z-1 (slot s^1) and z (slot s)
+        if (z > z0) {
+            const unsigned zc1 = z - 1;
+            const unsigned long long zq = (unsigned long long)zc1 * g.plane;
+#pragma unroll
+            for (int r = 0; r < (int)kRowsPerWave; ++r) {
+                const unsigned row = wave * kRowsPerWave + r;
+                const unsigned y = y0 + row;
+                if (PM != kNoPml && x < g.Nx && y < g.Ny && !(x + 1 < g.Nx && y + 1 < g.Ny)) {
+                    pml_hooks_cell<PM>(a, x, y, zc1, x + (unsigned long long)g.Nx * y + zq, iown[r]);
+                } else if (x + 1 < g.Nx && y + 1 < g.Ny) {
+                    const unsigned long long p = x + (unsigned long long)g.Nx * y + zq;
+                    const unsigned o = s ^ 1u;
+                    const float vx = sV[o][0][row][lane], vy = sV[o][1][row][lane], vz = sV[o][2][row][lane];
+                    const float vx_zp = sV[s][0][row][lane], vy_zp = sV[s][1][row][lane];
+                    const float vz_yp = sV[o][2][row + 1][lane], vx_yp = sV[o][0][row + 1][lane];
+                    const float vz_xp = sV[o][2][row][lane + 1], vy_xp = sV[o][1][row][lane + 1];
+                    float curl[3];
+                    curl[0] = vz - vz_yp - vy + vy_zp;
+                    curl[1] = vx - vx_zp - vz + vz_xp;
+                    curl[2] = vy - vy_xp - vx + vx_yp;
+                    float ca[3], cb[3], out[3];
+                    load_coef<CC>(a.kI, cs, p, uown[r], ca, cb);
+                    unsigned long long li = 0;
+                    const bool in_pml = PM != kNoPml && pml_at(a.pI, (int)x, (int)y, (int)zc1, li);
+                    cell_math<PM>(iown[r], ca, cb, curl, in_pml, a.pI, li, true, out);
+                    for (unsigned e = eib; e < eie; ++e) {
+                        const ExcEnt E = a.eI.ent[e];
+                        if (E.z == zc1 && E.x == x && E.y == y) {
+                            const int sm = exc_sample(ts, E.delay, a.eI.period, a.eI.length);
+                            out[E.dir] = out[E.dir] + E.amp * a.eI.sig[sm];
+                        }
+                    }
+#pragma unroll
+                    for (int n = 0; n < 3; ++n) a.I1[n * cs + p] = out[n];
+                }
+            }
+        }
+        // I^n and the index of plane z serve the current update at the next plane
+#pragma unroll
+        for (int r = 0; r < (int)kRowsPerWave; ++r) {
+            iown[r][0] = inew[r][0]; iown[r][1] = inew[r][1]; iown[r][2] = inew[r][2];
+            uown[r] = unew[r];
+        }
+        __syncthreads();
+    }
+    // the last current plane is skipped by the main update: UPML hooks only
+    if (PM != kNoPml && z1 == g.Nz) {
+        const unsigned long long zq = (unsigned long long)(g.Nz - 1) * g.plane;
+#pragma unroll
+        for (int r = 0; r < (int)kRowsPerWave; ++r) {
+            const unsigned y = y0 + wave * kRowsPerWave + r;
+            if (x < g.Nx && y < g.Ny) pml_hooks_cell<PM>(a, x, y, g.Nz - 1, x + (unsigned long long)g.Nx * y + zq, iown[r]);
+        }
+    }
+    // the last workgroup to finish advances the timestep counter; every
+    // workgroup has read it before it takes a ticket
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned done = atomicAdd(&a.numts[1], 1u);
+        if (done == gridDim.x - 1) {
+            atomicExch(&a.numts[1], 0u);
+            atomicAdd(&a.numts[0], 1u);
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
 // soft / hard source application (Engine_Ext_Excitation::Apply2Voltages /
-// Apply2Current, engine_ext_excitation.cpp:33-99).  numTS is read from the
-// device counter; the current-side source runs after the counter advanced,
-// hence ts_bias = -1 there.
+// Apply2Current, engine_ext_excitation.cpp:33-99).  One thread per (component,
+// cell) group adds that cell's entries in list order, as the reference's
+// sequential loop does.  numTS is read from the device counter; the
+// current-side source runs after the counter advanced, hence ts_bias = -1.
 // ---------------------------------------------------------------------------
-__global__ void k_excite(float* __restrict__ F, unsigned long long cs, unsigned K, const unsigned* __restrict__ lin,
-                         const unsigned short* __restrict__ dir, const float* __restrict__ amp,
+__global__ void k_excite(float* __restrict__ F, unsigned G, const unsigned* __restrict__ gbeg,
+                         const unsigned long long* __restrict__ glin, const float* __restrict__ amp,
                          const unsigned* __restrict__ delay, const float* __restrict__ sig, unsigned length,
                          int period, const unsigned* __restrict__ numts, int ts_bias)
 {
-    const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= K) return;
+    const unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= G) return;
     const int ts = (int)*numts + ts_bias;
-    const int p = period > 0 ? period : ts + 1;
-    int e = ts - (int)delay[k];
-    e *= (e > 0);
-    e %= p;
-    e *= (e < (int)length);
-    const unsigned long long i = (unsigned long long)dir[k] * cs + lin[k];
-    F[i] = F[i] + amp[k] * sig[e];
+    const unsigned long long i = glin[g];
+    float v = F[i];
+    for (unsigned k = gbeg[g]; k < gbeg[g + 1]; ++k) v = v + amp[k] * sig[exc_sample(ts, delay[k], period, length)];
+    F[i] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -673,7 +982,8 @@ struct ems_engine {
     GridDesc g{};
     unsigned N[3] = {0, 0, 0};
     unsigned long long cs = 0;
-    float *V = nullptr, *I = nullptr;
+    float *V = nullptr, *I = nullptr;          // current state
+    float *V_alt = nullptr, *I_alt = nullptr;  // second buffers of the fused step (state n+1)
     // coefficient layout: plain arrays or per-cell index + tables
     int layout_req = EMS_LAYOUT_AUTO;
     bool cmp = false;
@@ -688,26 +998,34 @@ struct ems_engine {
     bool async = false;
     std::vector<void*> allocs;
 
-    // excitation
+    // excitation: entries grouped per (component, cell), each group in list order
     struct Exc {
-        unsigned K = 0;
-        unsigned* lin = nullptr;
-        unsigned short* dir = nullptr;
-        float* amp = nullptr;
-        unsigned* delay = nullptr;
+        unsigned K = 0, G = 0;
+        unsigned* gbeg = nullptr;         // [G + 1]
+        unsigned long long* glin = nullptr;   // [G] component * cs + cell
+        float* amp = nullptr;             // [K] in group order
+        unsigned* delay = nullptr;        // [K]
+        std::vector<ExcEnt> host;         // list order, for the fused step's tile lists
     } exc[2];
     float* sig[2] = {nullptr, nullptr};
     unsigned sigLen = 0;
     int period = 0;
     bool has_exc = false;
 
-    // UPML: host staging per slab until init, then one device buffer per half step
+    // UPML: host staging per slab until init
     struct SlabHost {
         SlabTab tab;
-        std::vector<float> v[3], i[3];   // vv, vvfo, vvfn / ii, iifo, iifn, [3][cs] x-fastest
+        unsigned long long cnt = 0;       // positions of the slab
+        std::vector<float> v[3], i[3];   // vv, vvfo, vvfn / ii, iifo, iifn, [3][cnt] x-fastest
     };
     std::vector<SlabHost> slabs;
     PmlArgs pmlV{}, pmlI{};
+    float* flux_alt[2] = {nullptr, nullptr};   // second flux buffers of the fused step (V, I)
+    // fused single-sweep step
+    int fused_req = -1;                   // -1 auto (two half steps), 0 off, 1 required
+    bool fused = false;
+    unsigned ntx = 0, nty = 0, fzdepth = 0, fnzc = 0;
+    ExcArgs eV{}, eI{};
     int pml_mode = kNoPml;
     unsigned n_unique_pml = 0;
     // Mur faces in creation order
@@ -1017,18 +1335,44 @@ int ems_set_excitation(ems_engine* h, const float* sig_volt, const float* sig_cu
     const unsigned* del[2] = {vdelay, cdelay};
     for (int c = 0; c < 2; ++c) {
         const unsigned K = Ks[c];
-        h->exc[c].K = K;
+        auto& E = h->exc[c];
+        E.K = K;
         if (!K) continue;
         if (!idx[c] || !dir[c] || !amp[c] || !del[c]) return fail(EMS_E_ARG, "null excitation list");
-        std::vector<unsigned> lin(K);
+        // group the entries by (component, cell) in order of first appearance
+        std::unordered_map<unsigned long long, unsigned> gid;
+        std::vector<unsigned long long> glin;
+        std::vector<unsigned> grp(K);
+        E.host.resize(K);
         for (unsigned k = 0; k < K; ++k) {
             const unsigned x = idx[c][k], y = idx[c][K + k], z = idx[c][2 * K + k];
             if (x >= h->N[0] || y >= h->N[1] || z >= h->N[2] || dir[c][k] > 2)
                 return fail(EMS_E_ARG, "excitation cell outside the mesh");
-            lin[k] = (unsigned)h->lin(x, y, z);
+            const unsigned long long l = dir[c][k] * h->cs + h->lin(x, y, z);
+            auto it = gid.find(l);
+            if (it == gid.end()) {
+                it = gid.emplace(l, (unsigned)glin.size()).first;
+                glin.push_back(l);
+            }
+            grp[k] = it->second;
+            E.host[k] = ExcEnt{(unsigned short)std::min(x, 65535u), (unsigned short)std::min(y, 65535u),
+                               (unsigned short)std::min(z, 65535u), dir[c][k], amp[c][k], del[c][k]};
         }
-        if ((rc = upload(h, &h->exc[c].lin, lin.data(), K)) || (rc = upload(h, &h->exc[c].dir, dir[c], K)) ||
-            (rc = upload(h, &h->exc[c].amp, amp[c], K)) || (rc = upload(h, &h->exc[c].delay, del[c], K)))
+        const unsigned G = (unsigned)glin.size();
+        std::vector<unsigned> gbeg(G + 1, 0), order(K);
+        for (unsigned k = 0; k < K; ++k) ++gbeg[grp[k] + 1];
+        for (unsigned g = 0; g < G; ++g) gbeg[g + 1] += gbeg[g];
+        std::vector<unsigned> fill(gbeg.begin(), gbeg.end() - 1);
+        for (unsigned k = 0; k < K; ++k) order[fill[grp[k]]++] = k;   // stable: list order inside a group
+        std::vector<float> ga(K);
+        std::vector<unsigned> gd(K);
+        for (unsigned k = 0; k < K; ++k) {
+            ga[k] = amp[c][order[k]];
+            gd[k] = del[c][order[k]];
+        }
+        E.G = G;
+        if ((rc = upload(h, &E.gbeg, gbeg.data(), G + 1)) || (rc = upload(h, &E.glin, glin.data(), G)) ||
+            (rc = upload(h, &E.amp, ga.data(), K)) || (rc = upload(h, &E.delay, gd.data(), K)))
             return rc;
     }
     h->has_exc = true;
@@ -1052,34 +1396,24 @@ int ems_add_upml(ems_engine* h, const unsigned start[3], const unsigned num[3], 
     t.z1 = start[2] + num[2] - 1;
     t.nx = num[0];
     t.ny = num[1];
-    t.cs = (unsigned long long)num[0] * num[1] * num[2];
     for (const auto& o : h->slabs) {
         const SlabTab& u = o.tab;
         const bool overlap = t.x0 <= u.x1 && t.x1 >= u.x0 && t.y0 <= u.y1 && t.y1 >= u.y0 && t.z0 <= u.z1 && t.z1 >= u.z0;
         if (overlap) return fail(EMS_E_ARG, "overlapping UPML slabs are not supported");
     }
-    // at most two slabs may share an (x, y) column (the kernels track two):
-    // reject a third slab whose xy footprint meets the overlap of two others
-    for (size_t a = 0; a < h->slabs.size(); ++a)
-        for (size_t b = a + 1; b < h->slabs.size(); ++b) {
-            const SlabTab& A = h->slabs[a].tab;
-            const SlabTab& B = h->slabs[b].tab;
-            const int xl = std::max({A.x0, B.x0, t.x0}), xh = std::min({A.x1, B.x1, t.x1});
-            const int yl = std::max({A.y0, B.y0, t.y0}), yh = std::min({A.y1, B.y1, t.y1});
-            if (xl <= xh && yl <= yh) return fail(EMS_E_ARG, "more than two UPML slabs share a mesh column");
-        }
+    ems_engine::SlabHost sh;
+    sh.tab = t;
+    sh.cnt = (unsigned long long)num[0] * num[1] * num[2];
     // repack [3][nx][ny][nz] (z fastest) -> [3][nz][ny][nx] (x fastest)
     auto repack = [&](const float* src, std::vector<float>& d) {
-        d.resize(3 * t.cs);
+        d.resize(3 * sh.cnt);
         for (int n = 0; n < 3; ++n)
             for (unsigned x = 0; x < num[0]; ++x)
                 for (unsigned y = 0; y < num[1]; ++y)
                     for (unsigned z = 0; z < num[2]; ++z)
-                        d[n * t.cs + x + (unsigned long long)num[0] * (y + (unsigned long long)num[1] * z)] =
+                        d[n * sh.cnt + x + (unsigned long long)num[0] * (y + (unsigned long long)num[1] * z)] =
                             src[(((size_t)n * num[0] + x) * num[1] + y) * num[2] + z];
     };
-    ems_engine::SlabHost sh;
-    sh.tab = t;
     repack(vv, sh.v[0]);
     repack(vvfo, sh.v[1]);
     repack(vvfn, sh.v[2]);
@@ -1092,18 +1426,19 @@ int ems_add_upml(ems_engine* h, const unsigned start[3], const unsigned num[3], 
 
 static int upload_upml(ems_engine* h)
 {
-    // one buffer per half step.  Plain: per slab [c0, cfo, cfn, flux][3][cs].
-    // Compressed: per slab [flux][3][cs] + [index][cs], the index into a table
-    // of the unique 18-float tuples (V: c0, cfo, cfn; I: likewise) shared by
-    // both half steps, numbered in first-seen order over slabs in creation
-    // order, local index x-fastest.
+    // All slabs' positions are numbered consecutively; every array is
+    // [k][npos].  Plain: coef[9][npos] per half step.  Compressed: one index
+    // array into a table of the unique 18-float tuples (V: c0, cfo, cfn; I:
+    // likewise), numbered in first-seen order over that position order.
     if (h->slabs.empty()) return 0;
-    std::vector<unsigned long long> first;
     unsigned long long npos = 0;
-    for (const auto& s : h->slabs) {
+    std::vector<unsigned long long> first;
+    for (auto& s : h->slabs) {
         first.push_back(npos);
-        npos += s.tab.cs;
+        s.tab.off = npos;
+        npos += s.cnt;
     }
+    if (npos >= 0xFFFFFFFFull) return fail(EMS_E_ARG, "too many UPML positions for 32-bit numbering");
     std::vector<uint32_t> pidx;
     std::vector<float> uniq;
     size_t U = 0;
@@ -1112,49 +1447,26 @@ static int upload_upml(ems_engine* h)
             size_t s = 0;
             while (s + 1 < first.size() && i >= first[s + 1]) ++s;
             const auto& S = h->slabs[s];
-            const unsigned long long j = i - first[s], cs = S.tab.cs;
+            const unsigned long long j = i - first[s], cnt = S.cnt;
             for (int a = 0; a < 3; ++a)
                 for (int n = 0; n < 3; ++n) {
-                    t[3 * a + n] = S.v[a][n * cs + j];
-                    t[9 + 3 * a + n] = S.i[a][n * cs + j];
+                    t[3 * a + n] = S.v[a][n * cnt + j];
+                    t[9 + 3 * a + n] = S.i[a][n * cnt + j];
                 }
         };
         U = ems::dedup_tuples<18>(npos, get, pidx, uniq, 65536, ems::host_threads());
         if (h->layout_req == EMS_LAYOUT_AUTO && U * 16 > npos) U = 0;
     }
     const bool cmp = U > 0;
-    const unsigned per = cmp ? 4 : 12;
-    unsigned long long total = 0;
-    std::vector<SlabTab> tab;
-    for (auto& s : h->slabs) {
-        s.tab.off = total;
-        total += per * s.tab.cs;
-        tab.push_back(s.tab);
-    }
     int rc;
+    unsigned* dpidx = nullptr;
+    if (cmp && (rc = upload(h, &dpidx, pidx.data(), pidx.size()))) return rc;
     for (int side = 0; side < 2; ++side) {
-        std::vector<float> stage(total, 0.0f);
-        for (size_t k = 0; k < h->slabs.size(); ++k) {
-            const auto& s = h->slabs[k];
-            if (cmp) {
-                for (unsigned long long j = 0; j < s.tab.cs; ++j) {
-                    const uint32_t u = pidx[first[k] + j];
-                    std::memcpy(&stage[s.tab.off + 3 * s.tab.cs + j], &u, 4);
-                }
-            } else {
-                const auto* src = side ? s.i : s.v;
-                for (int a = 0; a < 3; ++a)
-                    std::copy(src[a].begin(), src[a].end(), stage.begin() + s.tab.off + (unsigned long long)a * 3 * s.tab.cs);
-            }
-        }
         PmlArgs& P = side ? h->pmlI : h->pmlV;
-        float* buf;
-        SlabTab* dt;
-        if ((rc = upload(h, &buf, stage.data(), stage.size())) || (rc = upload(h, &dt, tab.data(), tab.size()))) return rc;
-        P.n = (int)tab.size();
-        P.buf = buf;
-        P.tab = dt;
-        P.ptab = nullptr;
+        P = PmlArgs{};
+        P.n = (int)h->slabs.size();
+        P.npos = npos;
+        for (size_t k = 0; k < h->slabs.size(); ++k) P.tab[k] = h->slabs[k].tab;
         if (cmp) {
             std::vector<float> pt(U * 12, 0.0f);
             for (size_t u = 0; u < U; ++u)
@@ -1162,7 +1474,26 @@ static int upload_upml(ems_engine* h)
             float* dpt;
             if ((rc = upload(h, &dpt, pt.data(), pt.size()))) return rc;
             P.ptab = (const float4*)dpt;
+            P.pidx = dpidx;
+        } else {
+            std::vector<float> stage(9 * npos);
+            for (size_t k = 0; k < h->slabs.size(); ++k) {
+                const auto& S = h->slabs[k];
+                const auto* src = side ? S.i : S.v;
+                for (int a = 0; a < 3; ++a)
+                    for (int n = 0; n < 3; ++n)
+                        std::copy(src[a].begin() + n * S.cnt, src[a].begin() + (n + 1) * S.cnt,
+                                  stage.begin() + (3 * a + n) * npos + first[k]);
+            }
+            float* dc;
+            if ((rc = upload(h, &dc, stage.data(), stage.size()))) return rc;
+            P.coef = dc;
         }
+        float* fl;
+        if ((rc = h->alloc(&fl, 3 * npos))) return rc;
+        HIPCHK(hipMemset(fl, 0, 3 * npos * sizeof(float)));
+        P.flux_in = fl;
+        P.flux_out = fl;
     }
     h->pml_mode = cmp ? kPmlCompressed : kPmlPlain;
     h->n_unique_pml = cmp ? (unsigned)U : 0;
@@ -1290,6 +1621,113 @@ int ems_add_lorentz(ems_engine* h, unsigned count, const unsigned* pos, const fl
     return 0;
 }
 
+// z-sweep depth of the fused step: 16 planes per workgroup, shallower when
+// the grid would otherwise give fewer than 1024 workgroups
+static unsigned choose_fused_zdepth(unsigned ntiles, unsigned Nz)
+{
+    unsigned d = 16;
+    while (d > 4 && (unsigned long long)ntiles * ((Nz + d - 1) / d) < 1024ull) d /= 2;
+    if (const char* zd = std::getenv("EMS_FZDEPTH")) {
+        const int v = std::atoi(zd);
+        if (v >= 1 && v <= 1024) d = (unsigned)v;
+    }
+    return d;
+}
+
+// Decide and prepare the fused single-sweep step: every active extension
+// must have a fused form (UPML, excitation); Mur, Lorentz and lumped RLC keep
+// the two-half-step path.
+static int setup_fused(ems_engine* h)
+{
+    h->fused = false;
+    // automatic mode keeps the two half steps: on MI355X the per-plane
+    // barriers of the fused sweep cost more than its saved traffic (256^3 UPML,
+    // compressed: 0.50 ms per step fused, 0.38 ms in two half steps)
+    if (h->fused_req <= 0) return 0;
+    std::string why;
+    const unsigned Nx = h->N[0], Ny = h->N[1], Nz = h->N[2];
+    if (!h->mur.empty()) why = "Mur ABC has no fused form";
+    else if (!h->lor.empty()) why = "Lorentz / Drude materials have no fused form";
+    else if (!h->rlc.empty()) why = "lumped RLC elements have no fused form";
+    else if (Nx >= 65535 || Ny >= 65535 || Nz >= 65535) why = "mesh too large for the fused step";
+    for (const auto& e : h->exc[1].host)
+        if (why.empty() && (e.x + 1u >= Nx || e.y + 1u >= Ny || e.z + 1u >= Nz))
+            why = "a current source lies outside the updated range";
+    const unsigned ntx = (Nx + kTX - 1) / kTX, nty = (Ny + kTY - 1) / kTY, ntiles = ntx * nty;
+    // per-tile source lists in list order: voltage sources of the tile and
+    // its halo (column x0+kTX, row y0+kTY), current sources of the tile
+    std::vector<std::vector<ExcEnt>> lists[2];
+    size_t most = 0;
+    for (int c = 0; c < 2 && why.empty(); ++c) {
+        lists[c].assign(ntiles, {});
+        for (const auto& e : h->exc[c].host) {
+            const unsigned txs[2] = {e.x / kTX, (c == 0 && e.x % kTX == 0 && e.x > 0) ? e.x / kTX - 1 : ~0u};
+            const unsigned tys[2] = {e.y / kTY, (c == 0 && e.y % kTY == 0 && e.y > 0) ? e.y / kTY - 1 : ~0u};
+            for (unsigned tx : txs)
+                for (unsigned ty : tys)
+                    if (tx < ntx && ty < nty) lists[c][tx + ntx * ty].push_back(e);
+        }
+        for (const auto& l : lists[c]) most = std::max(most, l.size());
+    }
+    if (why.empty() && most > 64) why = "more than 64 sources in one tile";
+    if (!why.empty()) {
+        if (h->fused_req == 1) return fail(EMS_E_ARG, "fused step unavailable: " + why);
+        return 0;
+    }
+    int rc;
+    for (int c = 0; c < 2; ++c) {
+        std::vector<unsigned> beg(ntiles + 1, 0);
+        std::vector<ExcEnt> ent;
+        for (unsigned t = 0; t < ntiles; ++t) {
+            ent.insert(ent.end(), lists[c][t].begin(), lists[c][t].end());
+            beg[t + 1] = (unsigned)ent.size();
+        }
+        ExcArgs& E = c ? h->eI : h->eV;
+        unsigned* db;
+        ExcEnt* de;
+        if ((rc = upload(h, &db, beg.data(), beg.size())) || (rc = upload(h, &de, ent.data(), ent.size()))) return rc;
+        E.beg = db;
+        E.ent = de;
+        E.sig = h->sig[c];
+        E.length = h->sigLen;
+        E.period = h->period;
+    }
+    // second buffers for state n+1; cells a step does not write (the last
+    // current line of each direction) must hold the same values in both
+    if ((rc = h->alloc(&h->V_alt, 3 * h->cs)) || (rc = h->alloc(&h->I_alt, 3 * h->cs))) return rc;
+    HIPCHK(hipMemcpy(h->V_alt, h->V, 3 * h->cs * sizeof(float), hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(h->I_alt, h->I, 3 * h->cs * sizeof(float), hipMemcpyDeviceToDevice));
+    for (int side = 0; side < 2; ++side) {
+        PmlArgs& P = side ? h->pmlI : h->pmlV;
+        if (!P.n) continue;
+        if ((rc = h->alloc(&h->flux_alt[side], 3 * P.npos))) return rc;
+        HIPCHK(hipMemcpy(h->flux_alt[side], P.flux_in, 3 * P.npos * sizeof(float), hipMemcpyDeviceToDevice));
+    }
+    h->ntx = ntx;
+    h->nty = nty;
+    h->fzdepth = choose_fused_zdepth(ntiles, Nz);
+    h->fnzc = (Nz + h->fzdepth - 1) / h->fzdepth;
+    h->fused = true;
+    return 0;
+}
+
+int ems_set_fused(ems_engine* h, int mode)
+{
+    if (!h) return fail(EMS_E_ARG, "null handle");
+    if (h->inited) return fail(EMS_E_STATE, "engine already initialised");
+    if (mode < -1 || mode > 1) return fail(EMS_E_ARG, "fused mode is -1 (auto), 0 (off) or 1 (on)");
+    h->fused_req = mode;
+    return 0;
+}
+
+int ems_get_fused(const ems_engine* h, int* fused, unsigned* zdepth)
+{
+    if (!h) return fail(EMS_E_ARG, "null handle");
+    if (fused) *fused = h->fused ? 1 : 0;
+    if (zdepth) *zdepth = h->fused ? h->fzdepth : h->g.zdepth;
+    return 0;
+}
+
 int ems_init(ems_engine* h)
 {
     if (!h) return fail(EMS_E_ARG, "null handle");
@@ -1297,6 +1735,7 @@ int ems_init(ems_engine* h)
     HIPCHK(hipSetDevice(h->device));
     int rc = upload_upml(h);
     if (rc) return rc;
+    if ((rc = setup_fused(h))) return rc;
     h->inited = true;
     return 0;
 }
@@ -1476,6 +1915,64 @@ void dispatch_update(ems_engine* h, bool curr, unsigned nblk)
     else launch_update<false>(h, curr, nblk);
 }
 
+template <int PM, bool CC>
+void launch_step_t(ems_engine* h, const StepArgs& a, unsigned nblk)
+{
+    hipLaunchKernelGGL((k_step<PM, CC>), dim3(nblk), dim3(kBlock), 0, h->stream, a);
+}
+
+// one fused step from state n (h->V, h->I, current fluxes) into the second
+// buffers, which then become the current state
+void fused_step(ems_engine* h, hipEvent_t* ev)
+{
+    StepArgs a{};
+    a.g = h->g;
+    a.g.zdepth = h->fzdepth;
+    a.g.nzc = h->fnzc;
+    a.ntx = h->ntx;
+    a.nty = h->nty;
+    a.V0 = h->V;
+    a.I0 = h->I;
+    a.V1 = h->V_alt;
+    a.I1 = h->I_alt;
+    a.kV = h->kV;
+    a.kI = h->kI;
+    a.pV = h->pmlV;
+    a.pI = h->pmlI;
+    if (a.pV.n) a.pV.flux_out = h->flux_alt[0];
+    if (a.pI.n) a.pI.flux_out = h->flux_alt[1];
+    a.eV = h->eV;
+    a.eI = h->eI;
+    a.numts = h->numts_dev;
+    const unsigned nblk = h->ntx * h->nty * h->fnzc;
+    const int pm = h->pmlV.n ? h->pml_mode : kNoPml;
+    if (ev) (void)hipEventRecord(ev[0], h->stream);
+    if (h->cmp) {
+        if (pm == kPmlCompressed) launch_step_t<kPmlCompressed, true>(h, a, nblk);
+        else if (pm == kPmlPlain) launch_step_t<kPmlPlain, true>(h, a, nblk);
+        else launch_step_t<kNoPml, true>(h, a, nblk);
+    } else {
+        if (pm == kPmlCompressed) launch_step_t<kPmlCompressed, false>(h, a, nblk);
+        else if (pm == kPmlPlain) launch_step_t<kPmlPlain, false>(h, a, nblk);
+        else launch_step_t<kNoPml, false>(h, a, nblk);
+    }
+    if (ev) {
+        (void)hipEventRecord(ev[1], h->stream);
+        (void)hipEventRecord(ev[2], h->stream);
+        (void)hipEventRecord(ev[3], h->stream);
+    }
+    std::swap(h->V, h->V_alt);
+    std::swap(h->I, h->I_alt);
+    for (int side = 0; side < 2; ++side) {
+        PmlArgs& P = side ? h->pmlI : h->pmlV;
+        if (!P.n) continue;
+        float* cur = const_cast<float*>(P.flux_in);
+        P.flux_in = h->flux_alt[side];
+        P.flux_out = h->flux_alt[side];
+        h->flux_alt[side] = cur;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1484,6 +1981,10 @@ extern "C" {
 // priorities: UPML > default (lumped RLC, Lorentz, Mur) > excitation)
 static void launch_step(ems_engine* h, hipEvent_t* ev)
 {
+    if (h->fused) {
+        fused_step(h, ev);
+        return;
+    }
     const GridDesc& g = h->g;
     const unsigned nblk = g.qtiles * g.nzc;
     const unsigned T = 256;
@@ -1513,10 +2014,10 @@ static void launch_step(ems_engine* h, hipEvent_t* ev)
         const auto& m = h->mur[k];
         hipLaunchKernelGGL(k_mur_apply, dim3((m.na * m.nb + T - 1) / T), dim3(T), 0, h->stream, h->V, m, h->numts_dev);
     }
-    if (h->exc[0].K) {
+    if (h->exc[0].G) {
         const auto& e = h->exc[0];
-        hipLaunchKernelGGL(k_excite, dim3((e.K + T - 1) / T), dim3(T), 0, h->stream, h->V, h->cs, e.K, e.lin, e.dir,
-                           e.amp, e.delay, h->sig[0], h->sigLen, h->period, h->numts_dev, 0);
+        hipLaunchKernelGGL(k_excite, dim3((e.G + T - 1) / T), dim3(T), 0, h->stream, h->V, e.G, e.gbeg, e.glin, e.amp,
+                           e.delay, h->sig[0], h->sigLen, h->period, h->numts_dev, 0);
     }
     // -- current half step: Lorentz pre, fused UPML + update, Lorentz apply, excitation
     for (size_t k = 1; k < h->lor.size(); k += 2)
@@ -1528,10 +2029,10 @@ static void launch_step(ems_engine* h, hipEvent_t* ev)
     for (size_t k = 1; k < h->lor.size(); k += 2)
         if (h->lor_i[k / 2])
             hipLaunchKernelGGL(k_lor_apply, dim3((h->lor[k].count + T - 1) / T), dim3(T), 0, h->stream, h->I, h->cs, h->lor[k]);
-    if (h->exc[1].K) {
+    if (h->exc[1].G) {
         const auto& e = h->exc[1];
-        hipLaunchKernelGGL(k_excite, dim3((e.K + T - 1) / T), dim3(T), 0, h->stream, h->I, h->cs, e.K, e.lin, e.dir,
-                           e.amp, e.delay, h->sig[1], h->sigLen, h->period, h->numts_dev, -1);
+        hipLaunchKernelGGL(k_excite, dim3((e.G + T - 1) / T), dim3(T), 0, h->stream, h->I, e.G, e.gbeg, e.glin, e.amp,
+                           e.delay, h->sig[1], h->sigLen, h->period, h->numts_dev, -1);
     }
 }
 
@@ -1620,18 +2121,28 @@ static int get_field(ems_engine* h, const float* d, float* out)
     return 0;
 }
 
-static int set_field(ems_engine* h, float* d, const float* in)
+static int set_field(ems_engine* h, float* d, float* alt, const float* in)
 {
     if (!h || !in) return fail(EMS_E_ARG, "null argument");
     HIPCHK(hipSetDevice(h->device));
     HIPCHK(hipStreamSynchronize(h->stream));
-    return upload_nijk(h, d, in);
+    int rc = upload_nijk(h, d, in);
+    if (rc || !alt) return rc;
+    // the fused step's second buffer must agree on the cells a step leaves alone
+    HIPCHK(hipMemcpy(alt, d, 3 * h->cs * sizeof(float), hipMemcpyDeviceToDevice));
+    return 0;
 }
 
 int ems_get_volt(ems_engine* h, float* out) { return h ? get_field(h, h->V, out) : fail(EMS_E_ARG, "null handle"); }
 int ems_get_curr(ems_engine* h, float* out) { return h ? get_field(h, h->I, out) : fail(EMS_E_ARG, "null handle"); }
-int ems_set_volt(ems_engine* h, const float* in) { return h ? set_field(h, h->V, in) : fail(EMS_E_ARG, "null handle"); }
-int ems_set_curr(ems_engine* h, const float* in) { return h ? set_field(h, h->I, in) : fail(EMS_E_ARG, "null handle"); }
+int ems_set_volt(ems_engine* h, const float* in)
+{
+    return h ? set_field(h, h->V, h->V_alt, in) : fail(EMS_E_ARG, "null handle");
+}
+int ems_set_curr(ems_engine* h, const float* in)
+{
+    return h ? set_field(h, h->I, h->I_alt, in) : fail(EMS_E_ARG, "null handle");
+}
 
 int ems_energy(ems_engine* h, double* out)
 {

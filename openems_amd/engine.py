@@ -57,10 +57,12 @@ def expand_sse_compressed(N, op_index, vv4, vi4, ii4, iv4):
 
 
 class EngineHIP:
-    def __init__(self, op, device=0, layout="auto", sse_compressed=None):
+    def __init__(self, op, device=0, layout="auto", sse_compressed=None, fused=None):
         """op: Operator; layout: "auto" | "plain" | "compressed";
         sse_compressed: optional (op_index, vv4, vi4, ii4, iv4), the reference's
-        SSE-compressed form of the same operator, used instead of op's arrays."""
+        SSE-compressed form of the same operator, used instead of op's arrays;
+        fused: None (automatic), False (two half steps) or True (single-sweep
+        step required)."""
         L = lib()
         if L.ems_device_count() <= 0:
             raise RuntimeError("EngineHIP needs a HIP device; none is visible")
@@ -84,6 +86,8 @@ class EngineHIP:
                                       op.coeff_ptr("iv"), int(device), lay))
         self.h = h
         self._keep = []
+        if fused is not None:
+            check(L.ems_set_fused(h, 1 if fused else 0))
         try:
             self._extensions()
         except Exception:
@@ -166,6 +170,12 @@ class EngineHIP:
         check(lib().ems_get_layout(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
         names = {0: None, 1: "plain", 2: "compressed"}
         return names[a.value], b.value, names[c.value], d.value
+
+    def fused(self):
+        """(True if the single-sweep step is active, z planes per workgroup)"""
+        f, d = C.c_int(), C.c_uint()
+        check(lib().ems_get_fused(self.h, C.byref(f), C.byref(d)))
+        return bool(f.value), d.value
 
     def last_timing(self):
         t, v, c = C.c_double(), C.c_double(), C.c_double()

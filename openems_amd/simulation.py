@@ -15,13 +15,13 @@ from .processing import FieldDumpFD, Probe
 
 
 class Simulation:
-    def __init__(self, scene, device=0, nthreads=0, layout="auto"):
+    def __init__(self, scene, device=0, nthreads=0, layout="auto", fused=None):
         self.scene = scene
         self.nrts = int(scene.get("nrts", 1000))
         t0 = time.perf_counter()
         self.op = Operator(scene, nthreads=nthreads)
         self.setup_s = time.perf_counter() - t0
-        self.eng = EngineHIP(self.op, device=device, layout=layout)
+        self.eng = EngineHIP(self.op, device=device, layout=layout, fused=fused)
         os_ = int(scene.get("oversampling", 4))
         self.probes = [Probe(self.op, p, self.nrts, os_) for p in scene.get("probes", [])]
         self.dumps = [FieldDumpFD(self.op, d, self.nrts) for d in scene.get("dumps", [])]

@@ -34,22 +34,27 @@ def _fields_equal(eng, ref_eng):
     return v, c, rv, rc
 
 
+@pytest.mark.parametrize("step", ["fused", "halves"])
 @pytest.mark.parametrize("layout", ["plain", "compressed"])
 @pytest.mark.parametrize("make", [
     lambda: scenes.pec_cavity(32, 150),
     lambda: scenes.upml_bricks(48, 150, nbricks=8, dump=False),
     lambda: scenes.mur_layers(40, 36, 32, 200),
 ])
-def test_fields_bit_identical(gpu, make, layout):
+def test_fields_bit_identical(gpu, make, layout, step):
+    """Fields after 150-200 steps, both coefficient layouts, both step forms."""
     scene = make()
     nrts = scene["nrts"]
-    print("case", scene["name"], "steps", nrts, "layout", layout, flush=True)
+    print("case", scene["name"], "steps", nrts, "layout", layout, "step", step, flush=True)
     op = Operator(scene)
-    eng = EngineHIP(op, layout=layout)
+    # Mur has no fused form: the automatic mode keeps two half steps there
+    want_fused = step == "fused" and not op.mur_faces()
+    eng = EngineHIP(op, layout=layout, fused=True if want_fused else (None if step == "fused" else False))
     lay, nu, play, nup = eng.layout()
     assert lay == layout and (nu > 0) == (layout == "compressed")
     if op.upml_slabs():
         assert play == layout and (nup > 0) == (layout == "compressed")
+    assert eng.fused()[0] == want_fused
     eng.IterateTS(nrts)
     ref = RefEngine(_ref_from_scene(scene), "basic")
     ref.iterate(nrts)
@@ -190,7 +195,7 @@ def test_sse_compressed_operator_import(gpu):
     op = Operator(scene)
     ref = RefEngine(_ref_from_scene(scene), "sse_compressed")
     form = (ref.op_index_full(),) + ref.ctables()
-    eng = EngineHIP(op, sse_compressed=form)
+    eng = EngineHIP(op, sse_compressed=form, layout="compressed")
     assert eng.layout()[0] == "compressed"
     eng.IterateTS(120)
     ref.iterate(120)
@@ -200,3 +205,66 @@ def test_sse_compressed_operator_import(gpu):
     assert np.array_equal(v, rv) and np.array_equal(c, rc)
     eng.close()
     ref.close()
+
+
+@pytest.mark.parametrize("layout", ["plain", "compressed"])
+def test_fused_step_sources_on_tile_edges(gpu, layout):
+    """128^3 PEC cavity: the source edge (64, 64, 64) lies on a column-tile
+    corner of the fused step (tiles of 64 x 16), so its value is recomputed
+    in three neighbouring tiles' halos; fields stay bit-identical."""
+    scene = scenes.pec_cavity(128, 60)
+    print("case pec_cavity_128 tile-edge source, 60 steps, layout", layout, flush=True)
+    op = Operator(scene)
+    eng = EngineHIP(op, layout=layout, fused=True)
+    assert eng.fused()[0]
+    eng.IterateTS(60)
+    ref = RefEngine(_ref_from_scene(scene), "basic")
+    ref.iterate(60)
+    v, c, rv, rc = _fields_equal(eng, ref)
+    assert np.max(np.abs(rv)) > 0
+    assert np.array_equal(v, rv) and np.array_equal(c, rc)
+    eng.close()
+
+
+@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("make", [
+    lambda: scenes.pec_cavity(40, 40),
+    lambda: scenes.upml_bricks(40, 40, nbricks=6, dump=False),
+])
+def test_arbitrary_initial_fields(gpu, make, fused):
+    """Random initial fields everywhere, including the last current line of
+    each direction that the main update skips: there the values stay put
+    (PEC cavity) or follow the UPML hooks alone, which run over the whole
+    slab (UPML scene); both step forms match the oracle bit for bit."""
+    scene = make()
+    print("case", scene["name"], "random initial fields, fused", fused, flush=True)
+    op = Operator(scene)
+    eng = EngineHIP(op, fused=fused)
+    assert eng.fused()[0] == fused
+    ref = RefEngine(_ref_from_scene(scene), "basic")
+    rv, rc = ref.raw_views()
+    rng = np.random.default_rng(11)
+    v0 = rng.standard_normal(rv.shape).astype(np.float32)
+    c0 = rng.standard_normal(rc.shape).astype(np.float32)
+    rv[...] = v0
+    rc[...] = c0
+    eng.set_volt(v0)
+    eng.set_curr(c0)
+    eng.IterateTS(40)
+    ref.iterate(40)
+    v, c = eng.get_volt(), eng.get_curr()
+    if not op.upml_slabs():
+        assert np.array_equal(c[:, -1, :, :], c0[:, -1, :, :])
+    assert np.array_equal(v, rv), f"voltage mismatches {np.count_nonzero(v != rv)}"
+    assert np.array_equal(c, rc), f"current mismatches {np.count_nonzero(c != rc)}"
+    eng.close()
+
+
+def test_fused_step_refused_where_unavailable(gpu):
+    """Mur ABC has no fused form: requiring the fused step fails loudly."""
+    scene = scenes.mur_layers(24, 20, 16, 10)
+    print("case mur_layers fused refused", flush=True)
+    op = Operator(scene)
+    from openems_amd._native import NativeError
+    with pytest.raises(NativeError, match="fused"):
+        EngineHIP(op, fused=True)
