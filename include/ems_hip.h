@@ -38,6 +38,9 @@
  *   ems_add_fd_dump       ProcessFieldsFD::Process                       Common/processfields_fd.cpp:72-107
  *   ems_energy            Engine_Interface_FDTD::CalcFastEnergy          FDTD/engine_interface_fdtd.cpp:302-347
  *   ems_destroy           Engine::~Engine / Engine::Reset                FDTD/engine.cpp:45-108
+ *   ems_batch_*           no counterpart: many independent Engine::IterateTS loops
+ *                         (openems.cpp:1388-1494, one per instance) advanced by one
+ *                         launch per half step
  *
  * Conventions: every int-returning function returns EMS_OK (0) or a negative
  * EMS_E_* code and never exits the process; ems_last_error() gives the
@@ -227,6 +230,33 @@ int ems_last_timing(ems_engine* h, double* total_ms, double* volt_ms, double* cu
 int ems_set_profiling(ems_engine* h, int enable);
 
 void ems_destroy(ems_engine* h);
+
+/* ---- batched engine --------------------------------------------------------
+ * Many independent engines (own mesh, operator, UPML, sources, probes) advance
+ * together: each half step is one persistent launch over a work table of
+ * (instance, tile) items, plus one launch each for the sources and the probes
+ * of all instances.  Every instance stays bit-identical to running its engine
+ * alone.  The engines must be initialised, share a device, a coefficient
+ * layout and a UPML layout, run two half steps (ems_set_fused(h, 0)), stand at
+ * the same timestep, and carry no Mur, Lorentz, lumped RLC or field dump.  The
+ * batch does not own them: destroy it before them.  While batched, advance the
+ * engines only through ems_batch_iterate; ems_probe_results / ems_get_volt /
+ * ems_get_curr on each engine read its results. */
+typedef struct ems_batch ems_batch;
+int ems_batch_create(ems_batch** out, ems_engine* const* engines, unsigned count);
+/* tile_x: 0 = 256 consecutive plane positions per workgroup, else tile_x x
+ * (256/tile_x) columns; zdepth[count]: z planes per workgroup and instance
+ * (NULL: each instance's default); grid: workgroups per launch (0: one per
+ * work item, else a persistent grid that strides over the items). */
+int ems_batch_set_config(ems_batch* b, unsigned tile_x, const unsigned* zdepth, unsigned grid);
+int ems_batch_get_config(const ems_batch* b, unsigned* tile_x, unsigned* zdepth, unsigned* grid, unsigned* nitems);
+/* Time every candidate configuration for `steps` steps on the batch itself,
+ * restore the advanced state, and keep the fastest (best_ms: its time per step). */
+int ems_batch_autotune(ems_batch* b, unsigned steps, double* best_ms);
+int ems_batch_iterate(ems_batch* b, unsigned nsteps);
+/* device time of the last ems_batch_iterate call */
+int ems_batch_last_ms(const ems_batch* b, double* ms);
+void ems_batch_destroy(ems_batch* b);
 
 #ifdef __cplusplus
 }

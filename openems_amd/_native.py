@@ -100,6 +100,13 @@ ENGINE_API = [
     ("ems_last_timing", C.c_int, [vp, dp, dp, dp]),
     ("ems_set_profiling", C.c_int, [vp, C.c_int]),
     ("ems_destroy", None, [vp]),
+    ("ems_batch_create", C.c_int, [C.POINTER(vp), C.POINTER(vp), C.c_uint]),
+    ("ems_batch_set_config", C.c_int, [vp, C.c_uint, up, C.c_uint]),
+    ("ems_batch_get_config", C.c_int, [vp, up, up, up, up]),
+    ("ems_batch_autotune", C.c_int, [vp, C.c_uint, dp]),
+    ("ems_batch_iterate", C.c_int, [vp, C.c_uint]),
+    ("ems_batch_last_ms", C.c_int, [vp, dp]),
+    ("ems_batch_destroy", None, [vp]),
 ]
 
 _LIB = None

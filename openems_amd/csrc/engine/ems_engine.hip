@@ -54,8 +54,41 @@ struct GridDesc {
     unsigned long long cs;  // component stride
     unsigned zdepth;    // z-sweep depth per block
     unsigned nzc;       // number of z chunks
-    unsigned qtiles;    // ceil(plane / kBlock)
+    unsigned qtiles;    // (x, y) tiles per z chunk
+    unsigned tx;        // tile shape: 0 = kBlock consecutive plane positions, else tx x (kBlock/tx) columns
+    unsigned ntx;       // tiles along x (2D tiles)
 };
+
+// tiles of one z chunk for a tile shape
+inline void set_tiles(GridDesc& g, unsigned tx)
+{
+    g.tx = tx;
+    if (tx == 0) {
+        g.ntx = 0;
+        g.qtiles = (g.plane + kBlock - 1) / kBlock;
+    } else {
+        const unsigned ty = kBlock / tx;
+        g.ntx = (g.Nx + tx - 1) / tx;
+        g.qtiles = g.ntx * ((g.Ny + ty - 1) / ty);
+    }
+    g.nzc = (g.Nz + g.zdepth - 1) / g.zdepth;
+}
+
+// (x, y) column of this thread in tile qt, false outside the mesh
+__device__ __forceinline__ bool tile_column(const GridDesc& g, unsigned qt, unsigned& x, unsigned& y)
+{
+    if (g.tx == 0) {
+        const unsigned q = qt * kBlock + threadIdx.x;
+        y = q / g.Nx;
+        x = q - y * g.Nx;
+        return q < g.plane;
+    }
+    const unsigned ty = kBlock / g.tx;
+    const unsigned bx = qt % g.ntx, by = qt / g.ntx;
+    x = bx * g.tx + threadIdx.x % g.tx;
+    y = by * ty + threadIdx.x / g.tx;
+    return x < g.Nx && y < g.Ny;
+}
 
 // Coefficients of one half step.  Plain layout: SoA arrays cc (vv or ii) and
 // cv (vi or iv).  Compressed layout: one uint16 per cell (shared by both half
@@ -242,19 +275,17 @@ __device__ __forceinline__ void pml_hooks_only(const float x0[3], const PmlArgs&
 // z-1 currents in registers.  In place.
 // ---------------------------------------------------------------------------
 template <int PM, bool CC>
-__global__ __launch_bounds__(kBlock) void k_volt(const GridDesc g, float* V, const float* __restrict__ I,
-                                                 const CoefArgs k, const PmlArgs pml)
+__device__ __forceinline__ void volt_tile(const GridDesc& g, unsigned t, float* V, const float* __restrict__ I,
+                                          const CoefArgs& k, const PmlArgs& pml)
 {
-    const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
     const unsigned qt = t % g.qtiles, zc = t / g.qtiles;
-    const unsigned q = qt * kBlock + threadIdx.x;
-    if (q >= g.plane) return;
-    const unsigned y = q / g.Nx, x = q - y * g.Nx;
+    unsigned x, y;
+    if (!tile_column(g, qt, x, y)) return;
     const unsigned z0 = zc * g.zdepth;
     const unsigned z1 = min(z0 + g.zdepth, g.Nz);
     const unsigned long long cs = g.cs;
     const unsigned xm = x ? 1u : 0u, ym = y ? g.Nx : 0u;
-    unsigned long long p = (unsigned long long)q + (unsigned long long)g.plane * z0;
+    unsigned long long p = x + (unsigned long long)g.Nx * y + (unsigned long long)g.plane * z0;
 
     const float* __restrict__ Ix = I;
     const float* __restrict__ Iy = I + cs;
@@ -285,6 +316,13 @@ __global__ __launch_bounds__(kBlock) void k_volt(const GridDesc g, float* V, con
     }
 }
 
+template <int PM, bool CC>
+__global__ __launch_bounds__(kBlock) void k_volt(const GridDesc g, float* V, const float* __restrict__ I,
+                                                 const CoefArgs k, const PmlArgs pml)
+{
+    volt_tile<PM, CC>(g, xcd_tile(blockIdx.x, gridDim.x), V, I, k, pml);
+}
+
 // ---------------------------------------------------------------------------
 // current half step (Engine::UpdateCurrents, FDTD/engine.cpp:170-222), same
 // UPML fusion with the current flux (engine_ext_upml.cpp:135-234).  Updates
@@ -293,15 +331,12 @@ __global__ __launch_bounds__(kBlock) void k_volt(const GridDesc g, float* V, con
 // launch reads it, and every later kernel sees the new value.
 // ---------------------------------------------------------------------------
 template <int PM, bool CC>
-__global__ __launch_bounds__(kBlock) void k_curr(const GridDesc g, float* I, const float* __restrict__ V,
-                                                 const CoefArgs k, const PmlArgs pml, unsigned* __restrict__ numts)
+__device__ __forceinline__ void curr_tile(const GridDesc& g, unsigned t, float* I, const float* __restrict__ V,
+                                          const CoefArgs& k, const PmlArgs& pml)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(numts, 1u);
-    const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
     const unsigned qt = t % g.qtiles, zc = t / g.qtiles;
-    const unsigned q = qt * kBlock + threadIdx.x;
-    if (q >= g.plane) return;
-    const unsigned y = q / g.Nx, x = q - y * g.Nx;
+    unsigned x, y;
+    if (!tile_column(g, qt, x, y)) return;
     // columns / planes the main update skips only matter for UPML positions
     const bool col_upd = x + 1 < g.Nx && y + 1 < g.Ny;
     if (PM == kNoPml && !col_upd) return;
@@ -310,7 +345,7 @@ __global__ __launch_bounds__(kBlock) void k_curr(const GridDesc g, float* I, con
     if (z0 >= z1) return;
     const unsigned long long cs = g.cs;
     const unsigned Nx = g.Nx;
-    unsigned long long p = (unsigned long long)q + (unsigned long long)g.plane * z0;
+    unsigned long long p = x + (unsigned long long)g.Nx * y + (unsigned long long)g.plane * z0;
 
     const float* __restrict__ Vx = V;
     const float* __restrict__ Vy = V + cs;
@@ -356,6 +391,14 @@ __global__ __launch_bounds__(kBlock) void k_curr(const GridDesc g, float* I, con
 #pragma unroll
         for (int n = 0; n < 3; ++n) I[n * cs + p] = out[n];
     }
+}
+
+template <int PM, bool CC>
+__global__ __launch_bounds__(kBlock) void k_curr(const GridDesc g, float* I, const float* __restrict__ V,
+                                                 const CoefArgs k, const PmlArgs pml, unsigned* __restrict__ numts)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(numts, 1u);
+    curr_tile<PM, CC>(g, xcd_tile(blockIdx.x, gridDim.x), I, V, k, pml);
 }
 
 // Excitation entry as the fused step applies it: cell, component, amplitude
@@ -794,16 +837,13 @@ struct ProbeDev {
 
 constexpr unsigned kProbeChunk = 2048;
 
-__global__ __launch_bounds__(kBlock) void k_probe(const float* __restrict__ V, const float* __restrict__ I,
-                                                  const ProbeDev* __restrict__ probes,
-                                                  const unsigned long long* __restrict__ elin,
-                                                  const signed char* __restrict__ esign,
-                                                  const double* __restrict__ ediv,
-                                                  const unsigned* __restrict__ numts)
+// one probe sample by one workgroup (values gathered in parallel, summed by
+// one lane in the reference's term order)
+__device__ void probe_block(const ProbeDev& P, const float* __restrict__ V, const float* __restrict__ I,
+                            const unsigned long long* __restrict__ elin, const signed char* __restrict__ esign,
+                            const double* __restrict__ ediv, unsigned ts)
 {
     __shared__ float vals[kProbeChunk];
-    const ProbeDev P = probes[blockIdx.x];
-    const unsigned ts = *numts;
     if (ts < P.start_ts || ts > P.stop_ts) return;
     const bool due = (P.interval && ts % P.interval == 0) || (P.fd_interval && ts % P.fd_interval == 0);
     if (!due) return;
@@ -856,6 +896,90 @@ __global__ __launch_bounds__(kBlock) void k_probe(const float* __restrict__ V, c
             *P.count = slot + 1;
         }
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_probe(const float* __restrict__ V, const float* __restrict__ I,
+                                                  const ProbeDev* __restrict__ probes,
+                                                  const unsigned long long* __restrict__ elin,
+                                                  const signed char* __restrict__ esign,
+                                                  const double* __restrict__ ediv,
+                                                  const unsigned* __restrict__ numts)
+{
+    probe_block(probes[blockIdx.x], V, I, elin, esign, ediv, *numts);
+}
+
+// ---------------------------------------------------------------------------
+// Batched engine: many independent instances (own mesh, operator, UPML,
+// sources, probes) advance together.  Each half step is one persistent
+// launch: a workgroup walks the work table (instance, tile) with a stride of
+// the grid size; the per-tile arithmetic is that of k_volt / k_curr, so every
+// instance is bit-identical to its own engine.
+// ---------------------------------------------------------------------------
+struct BatchInst {
+    GridDesc g;
+    float* V;
+    float* I;
+    CoefArgs kV, kI;
+    PmlArgs pV, pI;
+    unsigned* numts;
+    const float* sig[2];
+    unsigned sigLen;
+    int period;
+};
+
+template <int PM, bool CC, bool CURR>
+__global__ __launch_bounds__(kBlock) void k_batch_half(const BatchInst* __restrict__ inst,
+                                                       const uint2* __restrict__ items, unsigned nitems)
+{
+    for (unsigned w = xcd_tile(blockIdx.x, gridDim.x); w < nitems; w += gridDim.x) {
+        const uint2 it = items[w];   // (instance, tile)
+        const BatchInst& B = inst[it.x];
+        if (CURR) {
+            if (it.y == 0 && threadIdx.x == 0) atomicAdd(B.numts, 1u);
+            curr_tile<PM, CC>(B.g, it.y, B.I, B.V, B.kI, B.pI);
+        } else {
+            volt_tile<PM, CC>(B.g, it.y, B.V, B.I, B.kV, B.pV);
+        }
+    }
+}
+
+// sources of all instances: one thread per (instance, component, cell) group
+struct BatchExcGroup {
+    unsigned inst;
+    unsigned beg, end;             // entries in amp / delay
+    unsigned long long lin;        // component * cs + cell
+};
+
+__global__ void k_batch_excite(const BatchInst* __restrict__ inst, const BatchExcGroup* __restrict__ grp, unsigned G,
+                               const float* __restrict__ amp, const unsigned* __restrict__ delay, int curr, int ts_bias)
+{
+    const unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= G) return;
+    const BatchExcGroup E = grp[g];
+    const BatchInst& B = inst[E.inst];
+    float* F = curr ? B.I : B.V;
+    const float* sig = B.sig[curr];
+    const int ts = (int)*B.numts + ts_bias;
+    float v = F[E.lin];
+    for (unsigned k = E.beg; k < E.end; ++k) v = v + amp[k] * sig[exc_sample(ts, delay[k], B.period, B.sigLen)];
+    F[E.lin] = v;
+}
+
+// probes of all instances, one workgroup each; a probe not due returns at once
+struct BatchProbe {
+    ProbeDev P;
+    unsigned inst;
+    const unsigned long long* elin;
+    const signed char* esign;
+    const double* ediv;
+};
+
+__global__ __launch_bounds__(kBlock) void k_batch_probe(const BatchInst* __restrict__ inst,
+                                                        const BatchProbe* __restrict__ probes)
+{
+    const BatchProbe& Q = probes[blockIdx.x];
+    const BatchInst& B = inst[Q.inst];
+    probe_block(Q.P, B.V, B.I, Q.elin, Q.esign, Q.ediv, *B.numts);
 }
 
 // ---------------------------------------------------------------------------
@@ -1006,6 +1130,10 @@ struct ems_engine {
         float* amp = nullptr;             // [K] in group order
         unsigned* delay = nullptr;        // [K]
         std::vector<ExcEnt> host;         // list order, for the fused step's tile lists
+        std::vector<unsigned> h_gbeg;      // host copies of the groups (batched engine)
+        std::vector<unsigned long long> h_glin;
+        std::vector<float> h_amp;
+        std::vector<unsigned> h_delay;
     } exc[2];
     float* sig[2] = {nullptr, nullptr};
     unsigned sigLen = 0;
@@ -1211,8 +1339,7 @@ int ems_create_layout(ems_engine** out, const unsigned numLines[3], const float*
         const int v = std::atoi(zd);
         if (v >= 1 && v <= 1024) g.zdepth = (unsigned)v;
     }
-    g.nzc = (g.Nz + g.zdepth - 1) / g.zdepth;
-    g.qtiles = (g.plane + kBlock - 1) / kBlock;
+    set_tiles(g, 0);
     int rc = 0;
     auto bail = [&](int r) {
         ems_destroy(h);
@@ -1371,6 +1498,10 @@ int ems_set_excitation(ems_engine* h, const float* sig_volt, const float* sig_cu
             gd[k] = del[c][order[k]];
         }
         E.G = G;
+        E.h_gbeg = gbeg;
+        E.h_glin = glin;
+        E.h_amp = ga;
+        E.h_delay = gd;
         if ((rc = upload(h, &E.gbeg, gbeg.data(), G + 1)) || (rc = upload(h, &E.glin, glin.data(), G)) ||
             (rc = upload(h, &E.amp, ga.data(), K)) || (rc = upload(h, &E.delay, gd.data(), K)))
             return rc;
@@ -1703,6 +1834,7 @@ static int setup_fused(ems_engine* h)
         if ((rc = h->alloc(&h->flux_alt[side], 3 * P.npos))) return rc;
         HIPCHK(hipMemcpy(h->flux_alt[side], P.flux_in, 3 * P.npos * sizeof(float), hipMemcpyDeviceToDevice));
     }
+    HIPCHK(hipDeviceSynchronize());
     h->ntx = ntx;
     h->nty = nty;
     h->fzdepth = choose_fused_zdepth(ntiles, Nz);
@@ -2130,6 +2262,7 @@ static int set_field(ems_engine* h, float* d, float* alt, const float* in)
     if (rc || !alt) return rc;
     // the fused step's second buffer must agree on the cells a step leaves alone
     HIPCHK(hipMemcpy(alt, d, 3 * h->cs * sizeof(float), hipMemcpyDeviceToDevice));
+    HIPCHK(hipDeviceSynchronize());
     return 0;
 }
 
@@ -2187,6 +2320,418 @@ void ems_destroy(ems_engine* h)
     for (auto& e : h->kev) hipEventDestroy(e);
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
+}
+
+
+// ---------------------------------------------------------------------------
+// batched engine
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct ems_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<ems_engine*> eng;
+    int pm = kNoPml;
+    bool cmp = false;
+    std::vector<BatchInst> h_inst;
+    BatchInst* d_inst = nullptr;
+    std::vector<uint2> h_items;
+    uint2* d_items = nullptr;
+    size_t items_cap = 0;
+    unsigned grid = 0;                       // workgroups of one half-step launch
+    unsigned tx = 0;
+    std::vector<unsigned> zdepth;
+    BatchExcGroup* d_grp[2] = {nullptr, nullptr};
+    float* d_amp[2] = {nullptr, nullptr};
+    unsigned* d_delay[2] = {nullptr, nullptr};
+    unsigned G[2] = {0, 0};
+    BatchProbe* d_probes = nullptr;
+    unsigned nprobes = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0;
+    int resident = 0;                        // workgroups resident per CU (occupancy)
+    int cus = 0;
+    std::vector<void*> allocs;
+
+    template <typename T>
+    int alloc(T** p, size_t n)
+    {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+        if (e != hipSuccess) return fail(EMS_E_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        allocs.push_back(q);
+        *p = (T*)q;
+        return 0;
+    }
+    template <typename T>
+    int upload(T** d, const T* src, size_t n)
+    {
+        int rc = alloc(d, n);
+        if (rc) return rc;
+        if (n) HIPCHK(hipMemcpy(*d, src, n * sizeof(T), hipMemcpyHostToDevice));
+        return 0;
+    }
+};
+
+namespace {
+
+template <int PM, bool CC, bool CURR>
+void batch_launch_t(ems_batch* b)
+{
+    hipLaunchKernelGGL((k_batch_half<PM, CC, CURR>), dim3(b->grid), dim3(kBlock), 0, b->stream, b->d_inst, b->d_items,
+                       (unsigned)b->h_items.size());
+}
+
+template <bool CURR>
+void batch_launch(ems_batch* b)
+{
+    if (b->cmp) {
+        if (b->pm == kPmlCompressed) batch_launch_t<kPmlCompressed, true, CURR>(b);
+        else if (b->pm == kPmlPlain) batch_launch_t<kPmlPlain, true, CURR>(b);
+        else batch_launch_t<kNoPml, true, CURR>(b);
+    } else {
+        if (b->pm == kPmlCompressed) batch_launch_t<kPmlCompressed, false, CURR>(b);
+        else if (b->pm == kPmlPlain) batch_launch_t<kPmlPlain, false, CURR>(b);
+        else batch_launch_t<kNoPml, false, CURR>(b);
+    }
+}
+
+// work table and instance descriptors for the current tile shape / depths
+int batch_build_items(ems_batch* b, unsigned tx, const std::vector<unsigned>& zd, unsigned grid)
+{
+    const size_t n = b->eng.size();
+    std::vector<uint2> items;
+    for (size_t i = 0; i < n; ++i) {
+        BatchInst& B = b->h_inst[i];
+        B.g.zdepth = zd[i];
+        set_tiles(B.g, tx);
+        const unsigned long long cnt = (unsigned long long)B.g.qtiles * B.g.nzc;
+        if (cnt > 0xFFFFFFFFull) return fail(EMS_E_ARG, "instance too large for the batch work table");
+        for (unsigned t = 0; t < (unsigned)cnt; ++t) items.push_back(make_uint2((unsigned)i, t));
+    }
+    if (items.size() > 0xFFFFFFFFull) return fail(EMS_E_ARG, "batch work table too large");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpy(b->d_inst, b->h_inst.data(), n * sizeof(BatchInst), hipMemcpyHostToDevice));
+    if (items.size() > b->items_cap || !b->d_items) {
+        int rc = b->alloc(&b->d_items, items.size());
+        if (rc) return rc;
+        b->items_cap = items.size();
+    }
+    b->h_items = items;
+    HIPCHK(hipMemcpy(b->d_items, items.data(), items.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    b->tx = tx;
+    b->zdepth = zd;
+    const unsigned full = (unsigned)items.size();
+    b->grid = grid == 0 ? full : std::min(grid, full);
+    return 0;
+}
+
+void batch_step(ems_batch* b)
+{
+    const unsigned T = 256;
+    batch_launch<false>(b);
+    if (b->G[0])
+        hipLaunchKernelGGL(k_batch_excite, dim3((b->G[0] + T - 1) / T), dim3(T), 0, b->stream, b->d_inst, b->d_grp[0],
+                           b->G[0], b->d_amp[0], b->d_delay[0], 0, 0);
+    batch_launch<true>(b);
+    if (b->G[1])
+        hipLaunchKernelGGL(k_batch_excite, dim3((b->G[1] + T - 1) / T), dim3(T), 0, b->stream, b->d_inst, b->d_grp[1],
+                           b->G[1], b->d_amp[1], b->d_delay[1], 1, -1);
+    if (b->nprobes)
+        hipLaunchKernelGGL(k_batch_probe, dim3(b->nprobes), dim3(kBlock), 0, b->stream, b->d_inst, b->d_probes);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ems_batch_create(ems_batch** out, ems_engine* const* engines, unsigned count)
+{
+    if (!out || !engines || count == 0) return fail(EMS_E_ARG, "null argument or empty batch");
+    *out = nullptr;
+    const ems_engine* e0 = engines[0];
+    if (!e0) return fail(EMS_E_ARG, "null engine");
+    for (unsigned i = 0; i < count; ++i) {
+        const ems_engine* e = engines[i];
+        if (!e) return fail(EMS_E_ARG, "null engine");
+        if (!e->inited) return fail(EMS_E_STATE, "batch engines must be initialised (ems_init)");
+        if (e->device != e0->device) return fail(EMS_E_ARG, "batch engines must share one device");
+        if (e->cmp != e0->cmp) return fail(EMS_E_ARG, "batch engines must share one coefficient layout");
+        const int pm = e->pmlV.n ? e->pml_mode : kNoPml, pm0 = e0->pmlV.n ? e0->pml_mode : kNoPml;
+        if (pm != pm0) return fail(EMS_E_ARG, "batch engines must share one UPML layout (all with or all without)");
+        if (!e->mur.empty() || !e->lor.empty() || !e->rlc.empty() || !e->dumps.empty())
+            return fail(EMS_E_ARG, "batched instances support UPML, sources and probes only");
+        if (e->fused) return fail(EMS_E_ARG, "batched instances run two half steps (ems_set_fused(h, 0))");
+        if (e->numTS != e0->numTS) return fail(EMS_E_STATE, "batch engines must be at the same timestep");
+        for (unsigned j = 0; j < i; ++j)
+            if (engines[j] == e) return fail(EMS_E_ARG, "an engine appears twice in the batch");
+    }
+    HIPCHK(hipSetDevice(e0->device));
+    ems_batch* b = new ems_batch();
+    auto bail = [&](int r) {
+        ems_batch_destroy(b);
+        return r;
+    };
+    b->device = e0->device;
+    b->eng.assign(engines, engines + count);
+    b->cmp = e0->cmp;
+    b->pm = e0->pmlV.n ? e0->pml_mode : kNoPml;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(EMS_E_HIP, "stream creation failed"));
+    hipEventCreate(&b->ev0);
+    hipEventCreate(&b->ev1);
+    int rc;
+    std::vector<BatchExcGroup> grp[2];
+    std::vector<float> amp[2];
+    std::vector<unsigned> del[2];
+    std::vector<BatchProbe> probes;
+    for (unsigned i = 0; i < count; ++i) {
+        ems_engine* e = engines[i];
+        HIPCHK(hipStreamSynchronize(e->stream));
+        if ((rc = sync_probe_tables(e))) return bail(rc);
+        BatchInst B{};
+        B.g = e->g;
+        B.V = e->V;
+        B.I = e->I;
+        B.kV = e->kV;
+        B.kI = e->kI;
+        B.pV = e->pmlV;
+        B.pI = e->pmlI;
+        B.numts = e->numts_dev;
+        B.sig[0] = e->sig[0];
+        B.sig[1] = e->sig[1];
+        B.sigLen = e->sigLen;
+        B.period = e->period;
+        b->h_inst.push_back(B);
+        b->zdepth.push_back(e->g.zdepth);
+        for (int c = 0; c < 2; ++c) {
+            const auto& E = e->exc[c];
+            for (unsigned g = 0; g < E.G; ++g) {
+                BatchExcGroup q{};
+                q.inst = i;
+                q.lin = E.h_glin[g];
+                q.beg = (unsigned)amp[c].size() + E.h_gbeg[g];
+                q.end = (unsigned)amp[c].size() + E.h_gbeg[g + 1];
+                grp[c].push_back(q);
+            }
+            amp[c].insert(amp[c].end(), E.h_amp.begin(), E.h_amp.end());
+            del[c].insert(del[c].end(), E.h_delay.begin(), E.h_delay.end());
+        }
+        for (const auto& P : e->probes) {
+            BatchProbe q{};
+            q.P = P;
+            q.inst = i;
+            q.elin = e->d_elin;
+            q.esign = e->d_esign;
+            q.ediv = e->d_ediv;
+            probes.push_back(q);
+        }
+    }
+    if ((rc = b->alloc(&b->d_inst, count))) return bail(rc);
+    for (int c = 0; c < 2; ++c) {
+        b->G[c] = (unsigned)grp[c].size();
+        if (!b->G[c]) continue;
+        if ((rc = b->upload(&b->d_grp[c], grp[c].data(), grp[c].size())) ||
+            (rc = b->upload(&b->d_amp[c], amp[c].data(), amp[c].size())) ||
+            (rc = b->upload(&b->d_delay[c], del[c].data(), del[c].size())))
+            return bail(rc);
+    }
+    b->nprobes = (unsigned)probes.size();
+    if (b->nprobes && (rc = b->upload(&b->d_probes, probes.data(), probes.size()))) return bail(rc);
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, b->device));
+    b->cus = prop.multiProcessorCount;
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_batch_half<kPmlCompressed, true, false>, kBlock, 0) !=
+            hipSuccess ||
+        occ <= 0)
+        occ = 4;
+    b->resident = occ;
+    // default: each instance's own depth, linear tiles, one workgroup per item
+    if ((rc = batch_build_items(b, 0, b->zdepth, 0))) return bail(rc);
+    *out = b;
+    return 0;
+}
+
+int ems_batch_set_config(ems_batch* b, unsigned tile_x, const unsigned* zdepth, unsigned grid)
+{
+    if (!b) return fail(EMS_E_ARG, "null batch");
+    if (tile_x != 0 && (tile_x > kBlock || kBlock % tile_x || tile_x < 8))
+        return fail(EMS_E_ARG, "tile_x must be 0 or a divisor of 256 from 8 to 256");
+    std::vector<unsigned> zd(b->eng.size());
+    for (size_t i = 0; i < zd.size(); ++i) {
+        zd[i] = zdepth ? zdepth[i] : choose_zdepth(b->eng[i]->N);
+        if (zd[i] < 1 || zd[i] > 4096) return fail(EMS_E_ARG, "z depth out of range");
+    }
+    return batch_build_items(b, tile_x, zd, grid);
+}
+
+int ems_batch_get_config(const ems_batch* b, unsigned* tile_x, unsigned* zdepth, unsigned* grid, unsigned* nitems)
+{
+    if (!b) return fail(EMS_E_ARG, "null batch");
+    if (tile_x) *tile_x = b->tx;
+    if (zdepth)
+        for (size_t i = 0; i < b->zdepth.size(); ++i) zdepth[i] = b->zdepth[i];
+    if (grid) *grid = b->grid;
+    if (nitems) *nitems = (unsigned)b->h_items.size();
+    return 0;
+}
+
+int ems_batch_iterate(ems_batch* b, unsigned nsteps)
+{
+    if (!b) return fail(EMS_E_ARG, "null batch");
+    HIPCHK(hipSetDevice(b->device));
+    for (auto* e : b->eng) HIPCHK(hipStreamSynchronize(e->stream));
+    if (b->eng[0]->numTS == 0 && nsteps > 0 && b->nprobes)
+        // Process() before the first step (openems.cpp:1424-1425)
+        hipLaunchKernelGGL(k_batch_probe, dim3(b->nprobes), dim3(kBlock), 0, b->stream, b->d_inst, b->d_probes);
+    HIPCHK(hipEventRecord(b->ev0, b->stream));
+    for (unsigned s = 0; s < nsteps; ++s) batch_step(b);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(b->ev1, b->stream));
+    HIPCHK(hipEventSynchronize(b->ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->last_ms = ms;
+    for (auto* e : b->eng) e->numTS += nsteps;
+    return 0;
+}
+
+// Autotuner: time every candidate tile shape x z-depth policy x grid size on
+// the batch itself for `steps` steps, restore the state the candidates
+// advanced (fields, UPML fluxes, counters, probe sample counts), and keep the
+// fastest.  Policies: each instance's own depth (choose_zdepth), or 4, 8, 16
+// for all.  Grids: one workgroup per work item, or the resident workgroups.
+int ems_batch_autotune(ems_batch* b, unsigned steps, double* best_ms)
+{
+    if (!b) return fail(EMS_E_ARG, "null batch");
+    if (steps == 0) steps = 4;
+    HIPCHK(hipSetDevice(b->device));
+    for (auto* e : b->eng) HIPCHK(hipStreamSynchronize(e->stream));
+    // snapshot
+    struct Saved {
+        float *V, *I, *fV, *fI;
+        unsigned numts[2];
+        std::vector<unsigned> counts;
+    };
+    std::vector<Saved> sv(b->eng.size());
+    std::vector<void*> tmp;
+    auto cleanup = [&]() {
+        for (void* p : tmp) hipFree(p);
+    };
+    auto dalloc = [&](float** p, size_t n) -> bool {
+        void* q = nullptr;
+        if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(float)) != hipSuccess) return false;
+        tmp.push_back(q);
+        *p = (float*)q;
+        return true;
+    };
+    for (size_t i = 0; i < b->eng.size(); ++i) {
+        ems_engine* e = b->eng[i];
+        Saved& S = sv[i];
+        S.fV = S.fI = nullptr;
+        if (!dalloc(&S.V, 3 * e->cs) || !dalloc(&S.I, 3 * e->cs)) {
+            cleanup();
+            return fail(EMS_E_NOMEM, "no device memory for the autotuner's snapshot");
+        }
+        hipMemcpy(S.V, e->V, 3 * e->cs * sizeof(float), hipMemcpyDeviceToDevice);
+        hipMemcpy(S.I, e->I, 3 * e->cs * sizeof(float), hipMemcpyDeviceToDevice);
+        if (e->pmlV.n) {
+            if (!dalloc(&S.fV, 3 * e->pmlV.npos) || !dalloc(&S.fI, 3 * e->pmlI.npos)) {
+                cleanup();
+                return fail(EMS_E_NOMEM, "no device memory for the autotuner's snapshot");
+            }
+            hipMemcpy(S.fV, e->pmlV.flux_in, 3 * e->pmlV.npos * sizeof(float), hipMemcpyDeviceToDevice);
+            hipMemcpy(S.fI, e->pmlI.flux_in, 3 * e->pmlI.npos * sizeof(float), hipMemcpyDeviceToDevice);
+        }
+        hipMemcpy(S.numts, e->numts_dev, 2 * sizeof(unsigned), hipMemcpyDeviceToHost);
+        for (const auto& P : e->probes) {
+            unsigned c = 0;
+            hipMemcpy(&c, P.count, sizeof(unsigned), hipMemcpyDeviceToHost);
+            S.counts.push_back(c);
+        }
+    }
+    HIPCHK(hipDeviceSynchronize());
+    auto restore = [&]() {
+        for (size_t i = 0; i < b->eng.size(); ++i) {
+            ems_engine* e = b->eng[i];
+            Saved& S = sv[i];
+            hipMemcpy(e->V, S.V, 3 * e->cs * sizeof(float), hipMemcpyDeviceToDevice);
+            hipMemcpy(e->I, S.I, 3 * e->cs * sizeof(float), hipMemcpyDeviceToDevice);
+            if (e->pmlV.n) {
+                hipMemcpy(const_cast<float*>(e->pmlV.flux_in), S.fV, 3 * e->pmlV.npos * sizeof(float),
+                          hipMemcpyDeviceToDevice);
+                hipMemcpy(const_cast<float*>(e->pmlI.flux_in), S.fI, 3 * e->pmlI.npos * sizeof(float),
+                          hipMemcpyDeviceToDevice);
+            }
+            hipMemcpy(e->numts_dev, S.numts, 2 * sizeof(unsigned), hipMemcpyHostToDevice);
+            for (size_t k = 0; k < e->probes.size(); ++k)
+                hipMemcpy(e->probes[k].count, &S.counts[k], sizeof(unsigned), hipMemcpyHostToDevice);
+        }
+        // device-to-device copies are not ordered with the batch's stream
+        hipDeviceSynchronize();
+    };
+    const unsigned txs[3] = {0, 64, 32};
+    const unsigned depths[4] = {0, 4, 8, 16};   // 0: each instance's own choice
+    const unsigned grids[2] = {0, (unsigned)(b->resident * b->cus)};
+    double best = 1e30;
+    unsigned btx = 0, bdep = 0, bgrid = 0;
+    const unsigned ts0 = b->eng[0]->numTS;
+    for (unsigned tx : txs)
+        for (unsigned dep : depths)
+            for (unsigned grid : grids) {
+                std::vector<unsigned> zd(b->eng.size());
+                for (size_t i = 0; i < zd.size(); ++i) zd[i] = dep ? dep : choose_zdepth(b->eng[i]->N);
+                int rc = batch_build_items(b, tx, zd, grid);
+                if (rc) {
+                    restore();
+                    cleanup();
+                    return rc;
+                }
+                batch_step(b);   // warm-up
+                hipEventRecord(b->ev0, b->stream);
+                for (unsigned s = 1; s < steps; ++s) batch_step(b);
+                hipEventRecord(b->ev1, b->stream);
+                hipEventSynchronize(b->ev1);
+                float ms = 0;
+                hipEventElapsedTime(&ms, b->ev0, b->ev1);
+                const double per = steps > 1 ? ms / (steps - 1) : ms;
+                if (per < best) {
+                    best = per;
+                    btx = tx;
+                    bdep = dep;
+                    bgrid = grid;
+                }
+                restore();
+            }
+    (void)ts0;
+    std::vector<unsigned> zd(b->eng.size());
+    for (size_t i = 0; i < zd.size(); ++i) zd[i] = bdep ? bdep : choose_zdepth(b->eng[i]->N);
+    int rc = batch_build_items(b, btx, zd, bgrid);
+    HIPCHK(hipStreamSynchronize(b->stream));
+    cleanup();
+    if (best_ms) *best_ms = best;
+    return rc;
+}
+
+int ems_batch_last_ms(const ems_batch* b, double* ms)
+{
+    if (!b || !ms) return fail(EMS_E_ARG, "null argument");
+    *ms = b->last_ms;
+    return 0;
+}
+
+void ems_batch_destroy(ems_batch* b)
+{
+    if (!b) return;
+    hipSetDevice(b->device);
+    if (b->stream) hipStreamSynchronize(b->stream);
+    for (void* p : b->allocs) hipFree(p);
+    if (b->ev0) hipEventDestroy(b->ev0);
+    if (b->ev1) hipEventDestroy(b->ev1);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
 }
 
 }  // extern "C"
