@@ -159,8 +159,10 @@ def run_batched(count, steps, tune_steps=6):
         digests.append(_field_digest(sim.eng))
         tds.append({k: v["td"] for k, v in sim.results()["probes"].items()})
         sim.close()
+    print(f"# batched leg: sequential {seq_s:.2f} s for {count} instances", file=sys.stderr, flush=True)
     b = BatchEngine(scs)
     tuned_ms = b.autotune(steps=tune_steps)
+    print(f"# batched leg: autotuned {b.config()['tile_x']=} {tuned_ms:.3f} ms/step", file=sys.stderr, flush=True)
     cfg = b.config()
     t0 = time.perf_counter()
     b.iterate(steps)
