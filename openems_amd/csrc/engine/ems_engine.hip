@@ -839,7 +839,7 @@ constexpr unsigned kProbeChunk = 2048;
 
 // one probe sample by one workgroup (values gathered in parallel, summed by
 // one lane in the reference's term order)
-__device__ void probe_block(const ProbeDev& P, const float* __restrict__ V, const float* __restrict__ I,
+__device__ void probe_block(const ProbeDev P, const float* __restrict__ V, const float* __restrict__ I,
                             const unsigned long long* __restrict__ elin, const signed char* __restrict__ esign,
                             const double* __restrict__ ediv, unsigned ts)
 {
@@ -870,7 +870,10 @@ __device__ void probe_block(const ProbeDev& P, const float* __restrict__ V, cons
                         acc_f = acc_f - vals[e];
                 } else if (base + e < 3) {
                     const double d = ediv[j];
-                    out[base + e] = d != 0.0 ? (double)vals[e] / d : 0.0;
+                    const double r = d != 0.0 ? (double)vals[e] / d : 0.0;
+                    if (base + e == 0) out[0] = r;
+                    if (base + e == 1) out[1] = r;
+                    if (base + e == 2) out[2] = r;
                 }
             }
         }
@@ -879,20 +882,15 @@ __device__ void probe_block(const ProbeDev& P, const float* __restrict__ V, cons
     if (threadIdx.x == 0) {
         const unsigned slot = *P.count;
         if (slot < P.capacity) {
+            // the three results as plain selects, stored unconditionally
+            const bool integral = P.type == EMS_PROBE_VOLTAGE || P.type == EMS_PROBE_CURRENT;
+            const double r0 = P.type == EMS_PROBE_VOLTAGE ? acc_d : P.type == EMS_PROBE_CURRENT ? (double)acc_f : out[0];
+            const double r1 = integral ? 0.0 : out[1];
+            const double r2 = integral ? 0.0 : out[2];
             P.ts_out[slot] = ts;
-            if (P.type == EMS_PROBE_VOLTAGE) {
-                P.val_out[3 * slot] = acc_d;
-                P.val_out[3 * slot + 1] = 0.0;
-                P.val_out[3 * slot + 2] = 0.0;
-            } else if (P.type == EMS_PROBE_CURRENT) {
-                P.val_out[3 * slot] = (double)acc_f;
-                P.val_out[3 * slot + 1] = 0.0;
-                P.val_out[3 * slot + 2] = 0.0;
-            } else {
-                P.val_out[3 * slot] = out[0];
-                P.val_out[3 * slot + 1] = out[1];
-                P.val_out[3 * slot + 2] = out[2];
-            }
+            P.val_out[3 * slot] = r0;
+            P.val_out[3 * slot + 1] = r1;
+            P.val_out[3 * slot + 2] = r2;
             *P.count = slot + 1;
         }
     }
