@@ -38,6 +38,7 @@
  *   ems_add_fd_dump       ProcessFieldsFD::Process                       Common/processfields_fd.cpp:72-107
  *   ems_energy            Engine_Interface_FDTD::CalcFastEnergy          FDTD/engine_interface_fdtd.cpp:302-347
  *   ems_destroy           Engine::~Engine / Engine::Reset                FDTD/engine.cpp:45-108
+ *   ems_adj_*             no counterpart: discrete adjoint of Engine::IterateTS for gradients
  *   ems_batch_*           no counterpart: many independent Engine::IterateTS loops
  *                         (openems.cpp:1388-1494, one per instance) advanced by one
  *                         launch per half step
@@ -257,6 +258,27 @@ int ems_batch_iterate(ems_batch* b, unsigned nsteps);
 /* device time of the last ems_batch_iterate call */
 int ems_batch_last_ms(const ems_batch* b, double* ms);
 void ems_batch_destroy(ems_batch* b);
+
+/* ---- adjoint gradients ------------------------------------------------------
+ * No counterpart in the reference: the discrete adjoint of the engine's
+ * timestep (two half steps with UPML and sources) for the objective
+ *   J = sum_p |F_p|^2,  F_p = sum_{n=1..N} (V[comp_p](pos_p) / divisor_p)(n) exp(-2 pi i f n dT) dT
+ * (V after step n, N steps from the engine's state at ems_adj_forward).
+ * ems_adj_forward runs the N steps, saving the state every checkpoint_every
+ * steps; ems_adj_backward replays each segment from its checkpoint and runs
+ * the transposed half steps backwards in time, accumulating dJ/dvi and dJ/dvv
+ * (double, [3][box]) on the edges of the inclusive sensitivity box
+ * box_lo..box_hi, which must not overlap the UPML.  The engine runs two half
+ * steps and carries no Mur, Lorentz or lumped RLC; its probes and dumps are
+ * not sampled during these runs.  Afterwards the engine holds the state at
+ * the end of the forward run. */
+typedef struct ems_adjoint ems_adjoint;
+int ems_adj_create(ems_adjoint** out, ems_engine* h, unsigned nprobe, const unsigned* comp, const unsigned* pos,
+                   const double* divisor, double freq, double dT, const unsigned box_lo[3], const unsigned box_hi[3],
+                   unsigned checkpoint_every);
+int ems_adj_forward(ems_adjoint* a, unsigned nsteps, double* J);
+int ems_adj_backward(ems_adjoint* a, double* dJ_dvi, double* dJ_dvv);
+void ems_adj_destroy(ems_adjoint* a);
 
 #ifdef __cplusplus
 }

@@ -54,7 +54,23 @@ int emsop_set_background(emsop* h, double eps_r, double mue_r, double kappa, dou
 /* material box: per-direction eps_r, mue_r, kappa (S/m), sigma (Ohm/m) */
 int emsop_add_material(emsop* h, const double eps[3], const double mue[3], const double kappa[3],
                        const double sigma[3], int priority, const double start[3], const double stop[3]);
+/* material grid: ncells[0] x ncells[1] x ncells[2] mesh cells from cell
+ * cell0 (cell i spans lines i..i+1) with eps_r and kappa (S/m, may be NULL)
+ * per cell, x fastest, isotropic; mue_r, sigma uniform.  Equivalent to
+ * one-cell material boxes of this priority added in x-fastest order (a point
+ * on a face shared by grid cells takes the lowest cell index); replaces them
+ * for large design regions.  The grid must be set first. */
+int emsop_add_material_grid(emsop* h, const unsigned cell0[3], const unsigned ncells[3], const double* eps_r,
+                            const double* kappa, double mue_r, double sigma, int priority);
 int emsop_add_metal(emsop* h, int priority, const double start[3], const double stop[3]);
+/* After emsop_build: for the edge (component n, pos), the four cells of its
+ * quarter-cell average (cells[3*q + axis], 0xFFFFFFFF for a cell before the
+ * first line; q = up-right, up-left, down-right, down-left in the
+ * (n+1, n+2) plane) and dC/d eps_r of each, where
+ * C = eps0 * sum_q eps_q A_q / sum_q A_q * EdgeArea / EdgeLength
+ * (Operator::AverageMatQuarterCell + Calc_ECPos, FDTD/operator.cpp:1189-1254,1447-1510);
+ * dG/d kappa = dC/d eps_r / eps0.  Quarter-cell averaging only. */
+int emsop_eps_weights(const emsop* h, int n, const unsigned pos[3], unsigned cells[12], double dC[4]);
 /* lumped element; parallel=1 for the R||C form folded into the operator */
 int emsop_add_lumped(emsop* h, int dir, double R, double C, double L, int caps, int parallel,
                      int priority, const double start[3], const double stop[3]);

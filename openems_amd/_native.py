@@ -31,6 +31,8 @@ OPERATOR_API = [
     ("emsop_set_grid", C.c_int, [vp, dp, C.c_uint, dp, C.c_uint, dp, C.c_uint, C.c_double]),
     ("emsop_set_background", C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_double]),
     ("emsop_add_material", C.c_int, [vp, dp, dp, dp, dp, C.c_int, dp, dp]),
+    ("emsop_add_material_grid", C.c_int, [vp, up, up, dp, dp, C.c_double, C.c_double, C.c_int]),
+    ("emsop_eps_weights", C.c_int, [vp, C.c_int, up, up, dp]),
     ("emsop_add_metal", C.c_int, [vp, C.c_int, dp, dp]),
     ("emsop_add_lumped", C.c_int, [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, dp, dp]),
     ("emsop_add_excitation", C.c_int, [vp, C.c_int, dp, C.c_double, C.c_int, dp, dp]),
@@ -107,6 +109,10 @@ ENGINE_API = [
     ("ems_batch_iterate", C.c_int, [vp, C.c_uint]),
     ("ems_batch_last_ms", C.c_int, [vp, dp]),
     ("ems_batch_destroy", None, [vp]),
+    ("ems_adj_create", C.c_int, [C.POINTER(vp), vp, C.c_uint, up, up, dp, C.c_double, C.c_double, up, up, C.c_uint]),
+    ("ems_adj_forward", C.c_int, [vp, C.c_uint, dp]),
+    ("ems_adj_backward", C.c_int, [vp, dp, dp]),
+    ("ems_adj_destroy", None, [vp]),
 ]
 
 _LIB = None

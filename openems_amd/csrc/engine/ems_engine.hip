@@ -1085,6 +1085,261 @@ __global__ __launch_bounds__(kBlock) void k_energy(const float* __restrict__ V, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of the timestep (discrete adjoint of the two half steps with UPML),
+// for gradients of a DFT-of-probe objective with respect to the coefficients.
+// Forward half step of one cell (n = x, y, z), curl c, UPML flux F:
+//   plain: X' = cc*X + cv*c
+//   UPML : f = cc*F + cv*c;  X' = c0*X - cfo*F + cfn*f;  F' = f
+// The transposed half step maps the adjoint (lX', lF') of the new state to
+// the adjoint of the old state and the adjoint of the curl:
+//   plain: lX = cc*lX';  lc = cv*lX'
+//   UPML : lf = lF' + cfn*lX';  lX = c0*lX';  lF = -cfo*lX' + cc*lf;  lc = cv*lf
+// and lc is scattered back through the transposed curl, evaluated here in
+// gather form (terms whose forward index clamps to the cell itself vanish,
+// as the forward curl cancels there).
+// ---------------------------------------------------------------------------
+struct AdjBox {
+    unsigned x0, y0, z0, nx, ny, nz;   // inclusive-exclusive box of edges
+    unsigned long long n;              // nx*ny*nz
+};
+
+__device__ __forceinline__ bool in_adj_box(const AdjBox& b, unsigned x, unsigned y, unsigned z, unsigned long long& k)
+{
+    if (x < b.x0 || y < b.y0 || z < b.z0 || x >= b.x0 + b.nx || y >= b.y0 + b.ny || z >= b.z0 + b.nz) return false;
+    k = (x - b.x0) + (unsigned long long)b.nx * ((y - b.y0) + (unsigned long long)b.ny * (z - b.z0));
+    return true;
+}
+
+// UPML coefficients c0, cfo, cfn of position li
+template <int PM>
+__device__ __forceinline__ void pml_coefs(const PmlArgs& P, unsigned long long li, float k0[3], float ko[3], float kn[3])
+{
+    if (PM == kPmlPlain) {
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            k0[n] = P.coef[n * P.npos + li];
+            ko[n] = P.coef[(3 + n) * P.npos + li];
+            kn[n] = P.coef[(6 + n) * P.npos + li];
+        }
+    } else {
+        const unsigned u = P.pidx[li];
+        const float4 t0 = P.ptab[3 * u], t1 = P.ptab[3 * u + 1], t2 = P.ptab[3 * u + 2];
+        k0[0] = t0.x; k0[1] = t0.y; k0[2] = t0.z;
+        ko[0] = t0.w; ko[1] = t1.x; ko[2] = t1.y;
+        kn[0] = t1.z; kn[1] = t1.w; kn[2] = t2.x;
+    }
+}
+
+// transposed current half step, one thread per (x, y) column sweeping z.
+// lI' = lI + curl_I^T(B) (B: curl adjoint left by the transposed voltage
+// half step of the later timestep); writes lI, lFI and A = curl_V adjoint.
+template <int PM, bool CC>
+__global__ __launch_bounds__(kBlock) void k_adj_curr(const GridDesc g, float* __restrict__ lI,
+                                                     const float* __restrict__ B, float* __restrict__ A,
+                                                     const CoefArgs k, const PmlArgs pml, float* __restrict__ lFI)
+{
+    const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
+    const unsigned qt = t % g.qtiles, zc = t / g.qtiles;
+    unsigned x, y;
+    if (!tile_column(g, qt, x, y)) return;
+    const unsigned z0 = zc * g.zdepth, z1 = min(z0 + g.zdepth, g.Nz);
+    const unsigned long long cs = g.cs;
+    const unsigned Nx = g.Nx;
+    const bool col_upd = x + 1 < g.Nx && y + 1 < g.Ny;
+    for (unsigned z = z0; z < z1; ++z) {
+        const unsigned long long p = x + (unsigned long long)Nx * y + (unsigned long long)g.plane * z;
+        // lI' = lI + curl_I^T(B); curl_I of the voltage update uses lower
+        // neighbours, clamped to the cell itself at index 0
+        float l[3];
+        {
+            const float* B0 = B;
+            const float* B1 = B + cs;
+            const float* B2 = B + 2 * cs;
+            float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+            if (z > 0) { s0 += B1[p]; s1 -= B0[p]; }
+            if (z + 1 < g.Nz) { s0 -= B1[p + g.plane]; s1 += B0[p + g.plane]; }
+            if (y > 0) { s0 -= B2[p]; s2 += B0[p]; }
+            if (y + 1 < g.Ny) { s0 += B2[p + Nx]; s2 -= B0[p + Nx]; }
+            if (x > 0) { s1 += B2[p]; s2 -= B1[p]; }
+            if (x + 1 < g.Nx) { s1 -= B2[p + 1]; s2 += B1[p + 1]; }
+            l[0] = lI[p] + s0;
+            l[1] = lI[cs + p] + s1;
+            l[2] = lI[2 * cs + p] + s2;
+        }
+        const bool upd = col_upd && z + 1 < g.Nz;
+        float lo[3], a[3] = {0.0f, 0.0f, 0.0f};
+        unsigned long long li = 0;
+        const bool in_pml = PM != kNoPml && pml_at(pml, (int)x, (int)y, (int)z, li);
+        if (upd) {
+            float ca[3], cb[3];
+            load_coef<CC>(k, cs, p, CC ? k.idx[p] : 0u, ca, cb);
+            if (in_pml) {
+                float k0[3], ko[3], kn[3];
+                pml_coefs<PM>(pml, li, k0, ko, kn);
+#pragma unroll
+                for (int n = 0; n < 3; ++n) {
+                    const float lf = lFI[n * pml.npos + li] + kn[n] * l[n];
+                    lo[n] = k0[n] * l[n];
+                    lFI[n * pml.npos + li] = ca[n] * lf - ko[n] * l[n];
+                    a[n] = cb[n] * lf;
+                }
+            } else {
+#pragma unroll
+                for (int n = 0; n < 3; ++n) {
+                    lo[n] = ca[n] * l[n];
+                    a[n] = cb[n] * l[n];
+                }
+            }
+        } else if (in_pml) {
+            // UPML hooks alone: X' = c0*X + (cfn - cfo)*F, F' = F
+            float k0[3], ko[3], kn[3];
+            pml_coefs<PM>(pml, li, k0, ko, kn);
+#pragma unroll
+            for (int n = 0; n < 3; ++n) {
+                lo[n] = k0[n] * l[n];
+                lFI[n * pml.npos + li] = lFI[n * pml.npos + li] + (kn[n] - ko[n]) * l[n];
+            }
+        } else {
+#pragma unroll
+            for (int n = 0; n < 3; ++n) lo[n] = l[n];
+        }
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            lI[n * cs + p] = lo[n];
+            A[n * cs + p] = a[n];
+        }
+    }
+}
+
+// transposed voltage half step: lV' = lV + curl_V^T(A); writes lV, lFV,
+// B = curl_I adjoint, and accumulates dJ/dvi, dJ/dvv on the box edges from
+// the recorded forward curl and voltage of the step.
+template <int PM, bool CC>
+__global__ __launch_bounds__(kBlock) void k_adj_volt(const GridDesc g, float* __restrict__ lV,
+                                                     const float* __restrict__ A, float* __restrict__ B,
+                                                     const CoefArgs k, const PmlArgs pml, float* __restrict__ lFV,
+                                                     const AdjBox box, const float* __restrict__ hc,
+                                                     const float* __restrict__ hv, double* __restrict__ gvi,
+                                                     double* __restrict__ gvv)
+{
+    const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
+    const unsigned qt = t % g.qtiles, zc = t / g.qtiles;
+    unsigned x, y;
+    if (!tile_column(g, qt, x, y)) return;
+    const unsigned z0 = zc * g.zdepth, z1 = min(z0 + g.zdepth, g.Nz);
+    const unsigned long long cs = g.cs;
+    const unsigned Nx = g.Nx;
+    for (unsigned z = z0; z < z1; ++z) {
+        const unsigned long long p = x + (unsigned long long)Nx * y + (unsigned long long)g.plane * z;
+        // lV' = lV + curl_V^T(A); curl_V of the current update uses upper
+        // neighbours, A is zero where the current update does not run
+        float l[3];
+        {
+            const float* A0 = A;
+            const float* A1 = A + cs;
+            const float* A2 = A + 2 * cs;
+            float s0 = A1[p] - A2[p];
+            float s1 = A2[p] - A0[p];
+            float s2 = A0[p] - A1[p];
+            if (z > 0) { s0 -= A1[p - g.plane]; s1 += A0[p - g.plane]; }
+            if (y > 0) { s0 += A2[p - Nx]; s2 -= A0[p - Nx]; }
+            if (x > 0) { s1 -= A2[p - 1]; s2 += A1[p - 1]; }
+            l[0] = lV[p] + s0;
+            l[1] = lV[cs + p] + s1;
+            l[2] = lV[2 * cs + p] + s2;
+        }
+        float ca[3], cb[3], lo[3], b[3];
+        load_coef<CC>(k, cs, p, CC ? k.idx[p] : 0u, ca, cb);
+        unsigned long long li = 0;
+        const bool in_pml = PM != kNoPml && pml_at(pml, (int)x, (int)y, (int)z, li);
+        if (in_pml) {
+            float k0[3], ko[3], kn[3];
+            pml_coefs<PM>(pml, li, k0, ko, kn);
+#pragma unroll
+            for (int n = 0; n < 3; ++n) {
+                const float lf = lFV[n * pml.npos + li] + kn[n] * l[n];
+                lo[n] = k0[n] * l[n];
+                lFV[n * pml.npos + li] = ca[n] * lf - ko[n] * l[n];
+                b[n] = cb[n] * lf;
+            }
+        } else {
+#pragma unroll
+            for (int n = 0; n < 3; ++n) {
+                lo[n] = ca[n] * l[n];
+                b[n] = cb[n] * l[n];
+            }
+            unsigned long long kb;
+            if (in_adj_box(box, x, y, z, kb)) {
+#pragma unroll
+                for (int n = 0; n < 3; ++n) {
+                    gvi[n * box.n + kb] += (double)l[n] * (double)hc[n * box.n + kb];
+                    gvv[n * box.n + kb] += (double)l[n] * (double)hv[n * box.n + kb];
+                }
+            }
+        }
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            lV[n * cs + p] = lo[n];
+            B[n * cs + p] = b[n];
+        }
+    }
+}
+
+// forward curl of the currents and the voltages on the box edges before a
+// step (the inputs of the coming voltage update)
+__global__ void k_adj_record(const GridDesc g, const float* __restrict__ V, const float* __restrict__ I,
+                             const AdjBox box, float* __restrict__ hc, float* __restrict__ hv)
+{
+    const unsigned long long kk = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x;
+    if (kk >= box.n) return;
+    const unsigned x = box.x0 + (unsigned)(kk % box.nx);
+    const unsigned y = box.y0 + (unsigned)((kk / box.nx) % box.ny);
+    const unsigned z = box.z0 + (unsigned)(kk / ((unsigned long long)box.nx * box.ny));
+    const unsigned long long cs = g.cs;
+    const unsigned long long p = x + (unsigned long long)g.Nx * y + (unsigned long long)g.plane * z;
+    const unsigned long long xm = x ? 1 : 0, ym = y ? g.Nx : 0, zm = z ? g.plane : 0;
+    const float* Ix = I;
+    const float* Iy = I + cs;
+    const float* Iz = I + 2 * cs;
+    hc[kk] = Iz[p] - Iz[p - ym] - Iy[p] + Iy[p - zm];
+    hc[box.n + kk] = Ix[p] - Ix[p - zm] - Iz[p] + Iz[p - xm];
+    hc[2 * box.n + kk] = Iy[p] - Iy[p - xm] - Ix[p] + Ix[p - ym];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) hv[n * box.n + kk] = V[n * cs + p];
+}
+
+// objective probes: F_p += (V[lin_p] / div_p) * w, w = exp(-2 pi i f t) * dT
+// at t = ts*dT (voltages after step ts)
+struct AdjProbe {
+    unsigned long long lin;
+    double div;
+};
+
+__global__ void k_adj_dft(const float* __restrict__ V, const AdjProbe* __restrict__ pr, unsigned np, double f,
+                          double dT, const unsigned* __restrict__ numts, double* __restrict__ F)
+{
+    const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const double t = (double)*numts * dT;
+    const double ph = -2.0 * 3.141592653589793238462643383279 * f * t;
+    const double v = (double)V[pr[p].lin] / pr[p].div;
+    F[2 * p] += v * cos(ph) * dT;
+    F[2 * p + 1] += v * sin(ph) * dT;
+}
+
+// adjoint source of J = sum |F_p|^2 at step ts: dJ/dV = 2 Re(conj(F) w) / div
+__global__ void k_adj_src(float* __restrict__ lV, const AdjProbe* __restrict__ pr, unsigned np, double f, double dT,
+                          unsigned ts, const double* __restrict__ F)
+{
+    const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const double t = (double)ts * dT;
+    const double ph = -2.0 * 3.141592653589793238462643383279 * f * t;
+    const double g = 2.0 * (F[2 * p] * cos(ph) * dT + F[2 * p + 1] * sin(ph) * dT) / pr[p].div;
+    lV[pr[p].lin] = (float)((double)lV[pr[p].lin] + g);
+}
+
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
@@ -2730,6 +2985,282 @@ void ems_batch_destroy(ems_batch* b)
     if (b->ev1) hipEventDestroy(b->ev1);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
+}
+
+
+// ---------------------------------------------------------------------------
+// adjoint gradients
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+
+// one saved state: fields, UPML fluxes, timestep
+struct AdjCkpt {
+    float *V = nullptr, *I = nullptr, *FV = nullptr, *FI = nullptr;
+    unsigned ts = 0;
+};
+
+}  // namespace
+
+struct ems_adjoint {
+    ems_engine* h = nullptr;
+    AdjProbe* d_probes = nullptr;
+    unsigned np = 0;
+    double freq = 0, dT = 0;
+    AdjBox box{};
+    unsigned K = 256;
+    unsigned n0 = 0, nsteps = 0;
+    double* d_F = nullptr;
+    std::vector<double> F;
+    std::vector<AdjCkpt> ck;   // state at n0 + s*K, s = 0..S-1, then the final state
+    float *lV = nullptr, *lI = nullptr, *A = nullptr, *B = nullptr, *lFV = nullptr, *lFI = nullptr;
+    float *hc = nullptr, *hv = nullptr;   // [K][3][box.n]
+    double *gvi = nullptr, *gvv = nullptr;
+    bool forward_done = false;
+    std::vector<void*> allocs;
+    template <typename T>
+    int alloc(T** p, size_t n)
+    {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+        if (e != hipSuccess) return fail(EMS_E_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        allocs.push_back(q);
+        *p = (T*)q;
+        return 0;
+    }
+};
+
+namespace {
+
+int adj_save(ems_adjoint* a, AdjCkpt& c)
+{
+    ems_engine* h = a->h;
+    const size_t fb = 3 * h->cs * sizeof(float);
+    int rc;
+    if (!c.V) {
+        if ((rc = a->alloc(&c.V, 3 * h->cs)) || (rc = a->alloc(&c.I, 3 * h->cs))) return rc;
+        if (h->pmlV.n && ((rc = a->alloc(&c.FV, 3 * h->pmlV.npos)) || (rc = a->alloc(&c.FI, 3 * h->pmlI.npos))))
+            return rc;
+    }
+    HIPCHK(hipMemcpyAsync(c.V, h->V, fb, hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(c.I, h->I, fb, hipMemcpyDeviceToDevice, h->stream));
+    if (h->pmlV.n) {
+        HIPCHK(hipMemcpyAsync(c.FV, h->pmlV.flux_in, 3 * h->pmlV.npos * sizeof(float), hipMemcpyDeviceToDevice,
+                              h->stream));
+        HIPCHK(hipMemcpyAsync(c.FI, h->pmlI.flux_in, 3 * h->pmlI.npos * sizeof(float), hipMemcpyDeviceToDevice,
+                              h->stream));
+    }
+    c.ts = h->numTS;
+    return 0;
+}
+
+int adj_restore(ems_adjoint* a, const AdjCkpt& c)
+{
+    ems_engine* h = a->h;
+    const size_t fb = 3 * h->cs * sizeof(float);
+    HIPCHK(hipMemcpyAsync(h->V, c.V, fb, hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->I, c.I, fb, hipMemcpyDeviceToDevice, h->stream));
+    if (h->pmlV.n) {
+        HIPCHK(hipMemcpyAsync(const_cast<float*>(h->pmlV.flux_in), c.FV, 3 * h->pmlV.npos * sizeof(float),
+                              hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(const_cast<float*>(h->pmlI.flux_in), c.FI, 3 * h->pmlI.npos * sizeof(float),
+                              hipMemcpyDeviceToDevice, h->stream));
+    }
+    const unsigned ts[2] = {c.ts, 0u};
+    HIPCHK(hipMemcpyAsync(h->numts_dev, ts, sizeof(ts), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->numTS = c.ts;
+    return 0;
+}
+
+template <int PM, bool CC>
+void adj_kernels_t(ems_adjoint* a, const float* hc, const float* hv)
+{
+    ems_engine* h = a->h;
+    const unsigned nblk = h->g.qtiles * h->g.nzc;
+    PmlArgs pI = h->pmlI, pV = h->pmlV;
+    hipLaunchKernelGGL((k_adj_curr<PM, CC>), dim3(nblk), dim3(kBlock), 0, h->stream, h->g, a->lI, a->B, a->A, h->kI,
+                       pI, a->lFI);
+    hipLaunchKernelGGL((k_adj_volt<PM, CC>), dim3(nblk), dim3(kBlock), 0, h->stream, h->g, a->lV, a->A, a->B, h->kV,
+                       pV, a->lFV, a->box, hc, hv, a->gvi, a->gvv);
+}
+
+void adj_kernels(ems_adjoint* a, const float* hc, const float* hv)
+{
+    ems_engine* h = a->h;
+    const int pm = h->pmlV.n ? h->pml_mode : kNoPml;
+    if (h->cmp) {
+        if (pm == kPmlCompressed) adj_kernels_t<kPmlCompressed, true>(a, hc, hv);
+        else if (pm == kPmlPlain) adj_kernels_t<kPmlPlain, true>(a, hc, hv);
+        else adj_kernels_t<kNoPml, true>(a, hc, hv);
+    } else {
+        if (pm == kPmlCompressed) adj_kernels_t<kPmlCompressed, false>(a, hc, hv);
+        else if (pm == kPmlPlain) adj_kernels_t<kPmlPlain, false>(a, hc, hv);
+        else adj_kernels_t<kNoPml, false>(a, hc, hv);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ems_adj_create(ems_adjoint** out, ems_engine* h, unsigned nprobe, const unsigned* comp, const unsigned* pos,
+                   const double* divisor, double freq, double dT, const unsigned box_lo[3], const unsigned box_hi[3],
+                   unsigned checkpoint_every)
+{
+    if (!out || !h || !comp || !pos || !divisor || !box_lo || !box_hi) return fail(EMS_E_ARG, "null argument");
+    *out = nullptr;
+    if (!h->inited) return fail(EMS_E_STATE, "initialise the engine (ems_init) first");
+    if (h->fused) return fail(EMS_E_ARG, "the adjoint runs the two-half-step form (ems_set_fused(h, 0))");
+    if (!h->mur.empty() || !h->lor.empty() || !h->rlc.empty())
+        return fail(EMS_E_ARG, "the adjoint supports UPML and sources only (no Mur, Lorentz, lumped RLC)");
+    if (nprobe == 0 || !(dT > 0)) return fail(EMS_E_ARG, "no objective probes or bad timestep");
+    if (checkpoint_every == 0) return fail(EMS_E_ARG, "checkpoint interval must be positive");
+    AdjBox box{};
+    box.x0 = box_lo[0];
+    box.y0 = box_lo[1];
+    box.z0 = box_lo[2];
+    for (int n = 0; n < 3; ++n)
+        if (box_hi[n] < box_lo[n] || box_hi[n] >= h->N[n]) return fail(EMS_E_ARG, "sensitivity box outside the mesh");
+    box.nx = box_hi[0] - box_lo[0] + 1;
+    box.ny = box_hi[1] - box_lo[1] + 1;
+    box.nz = box_hi[2] - box_lo[2] + 1;
+    box.n = (unsigned long long)box.nx * box.ny * box.nz;
+    // the box must stay clear of the UPML (its coefficients are not the ones differentiated)
+    for (int s = 0; s < h->pmlV.n; ++s) {
+        const SlabTab& S = h->pmlV.tab[s];
+        if ((int)box_lo[0] <= S.x1 && (int)box_hi[0] >= S.x0 && (int)box_lo[1] <= S.y1 && (int)box_hi[1] >= S.y0 &&
+            (int)box_lo[2] <= S.z1 && (int)box_hi[2] >= S.z0)
+            return fail(EMS_E_ARG, "the sensitivity box overlaps the UPML");
+    }
+    std::vector<AdjProbe> pr(nprobe);
+    for (unsigned p = 0; p < nprobe; ++p) {
+        const unsigned x = pos[p], y = pos[nprobe + p], z = pos[2 * nprobe + p];
+        if (x >= h->N[0] || y >= h->N[1] || z >= h->N[2] || comp[p] > 2 || !(divisor[p] != 0.0))
+            return fail(EMS_E_ARG, "objective probe outside the mesh or zero divisor");
+        pr[p].lin = comp[p] * h->cs + h->lin(x, y, z);
+        pr[p].div = divisor[p];
+    }
+    HIPCHK(hipSetDevice(h->device));
+    ems_adjoint* a = new ems_adjoint();
+    auto bail = [&](int r) {
+        ems_adj_destroy(a);
+        return r;
+    };
+    a->h = h;
+    a->np = nprobe;
+    a->freq = freq;
+    a->dT = dT;
+    a->box = box;
+    a->K = checkpoint_every;
+    int rc;
+    const size_t c3 = 3 * h->cs;
+    const size_t f3V = h->pmlV.n ? 3 * h->pmlV.npos : 1, f3I = h->pmlI.n ? 3 * h->pmlI.npos : 1;
+    if ((rc = a->alloc(&a->d_probes, nprobe)) || (rc = a->alloc(&a->d_F, 2 * nprobe)) || (rc = a->alloc(&a->lV, c3)) ||
+        (rc = a->alloc(&a->lI, c3)) || (rc = a->alloc(&a->A, c3)) || (rc = a->alloc(&a->B, c3)) ||
+        (rc = a->alloc(&a->lFV, f3V)) || (rc = a->alloc(&a->lFI, f3I)) ||
+        (rc = a->alloc(&a->hc, (size_t)a->K * 3 * box.n)) || (rc = a->alloc(&a->hv, (size_t)a->K * 3 * box.n)) ||
+        (rc = a->alloc(&a->gvi, 3 * box.n)) || (rc = a->alloc(&a->gvv, 3 * box.n)))
+        return bail(rc);
+    if (hipMemcpy(a->d_probes, pr.data(), nprobe * sizeof(AdjProbe), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(EMS_E_HIP, "upload of the objective probes failed"));
+    *out = a;
+    return 0;
+}
+
+int ems_adj_forward(ems_adjoint* a, unsigned nsteps, double* J)
+{
+    if (!a || nsteps == 0) return fail(EMS_E_ARG, "null adjoint or no steps");
+    ems_engine* h = a->h;
+    HIPCHK(hipSetDevice(h->device));
+    int rc;
+    if ((rc = sync_probe_tables(h))) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemsetAsync(a->d_F, 0, 2 * a->np * sizeof(double), h->stream));
+    a->n0 = h->numTS;
+    a->nsteps = nsteps;
+    const unsigned S = (nsteps + a->K - 1) / a->K;
+    if (a->ck.size() < (size_t)S + 1) a->ck.resize(S + 1);
+    const unsigned T = 256;
+    for (unsigned s = 0; s < S; ++s) {
+        if ((rc = adj_save(a, a->ck[s]))) return rc;
+        const unsigned len = std::min(a->K, nsteps - s * a->K);
+        for (unsigned j = 0; j < len; ++j) {
+            launch_step(h, nullptr);
+            ++h->numTS;
+            hipLaunchKernelGGL(k_adj_dft, dim3((a->np + T - 1) / T), dim3(T), 0, h->stream, h->V, a->d_probes, a->np,
+                               a->freq, a->dT, h->numts_dev, a->d_F);
+        }
+    }
+    if ((rc = adj_save(a, a->ck[S]))) return rc;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    a->F.assign(2 * a->np, 0.0);
+    HIPCHK(hipMemcpy(a->F.data(), a->d_F, a->F.size() * sizeof(double), hipMemcpyDeviceToHost));
+    double j = 0;
+    for (unsigned p = 0; p < a->np; ++p) j += a->F[2 * p] * a->F[2 * p] + a->F[2 * p + 1] * a->F[2 * p + 1];
+    if (J) *J = j;
+    a->forward_done = true;
+    return 0;
+}
+
+int ems_adj_backward(ems_adjoint* a, double* dJ_dvi, double* dJ_dvv)
+{
+    if (!a) return fail(EMS_E_ARG, "null adjoint");
+    if (!a->forward_done) return fail(EMS_E_STATE, "run ems_adj_forward first");
+    ems_engine* h = a->h;
+    HIPCHK(hipSetDevice(h->device));
+    const size_t c3 = 3 * h->cs * sizeof(float);
+    HIPCHK(hipMemsetAsync(a->lV, 0, c3, h->stream));
+    HIPCHK(hipMemsetAsync(a->lI, 0, c3, h->stream));
+    HIPCHK(hipMemsetAsync(a->A, 0, c3, h->stream));
+    HIPCHK(hipMemsetAsync(a->B, 0, c3, h->stream));
+    if (h->pmlV.n) {
+        HIPCHK(hipMemsetAsync(a->lFV, 0, 3 * h->pmlV.npos * sizeof(float), h->stream));
+        HIPCHK(hipMemsetAsync(a->lFI, 0, 3 * h->pmlI.npos * sizeof(float), h->stream));
+    }
+    HIPCHK(hipMemsetAsync(a->gvi, 0, 3 * a->box.n * sizeof(double), h->stream));
+    HIPCHK(hipMemsetAsync(a->gvv, 0, 3 * a->box.n * sizeof(double), h->stream));
+    const unsigned S = (a->nsteps + a->K - 1) / a->K;
+    const unsigned T = 256;
+    const unsigned rblk = (unsigned)((a->box.n + T - 1) / T);
+    int rc;
+    for (int s = (int)S - 1; s >= 0; --s) {
+        if ((rc = adj_restore(a, a->ck[s]))) return rc;
+        const unsigned len = std::min(a->K, a->nsteps - s * a->K);
+        // replay the segment, recording the inputs of each voltage update
+        for (unsigned j = 0; j < len; ++j) {
+            hipLaunchKernelGGL(k_adj_record, dim3(rblk), dim3(T), 0, h->stream, h->g, h->V, h->I, a->box,
+                               a->hc + (size_t)j * 3 * a->box.n, a->hv + (size_t)j * 3 * a->box.n);
+            launch_step(h, nullptr);
+            ++h->numTS;
+        }
+        // reverse steps of the segment
+        for (int j = (int)len - 1; j >= 0; --j) {
+            const unsigned ts = a->n0 + (unsigned)s * a->K + (unsigned)j + 1;
+            hipLaunchKernelGGL(k_adj_src, dim3((a->np + T - 1) / T), dim3(T), 0, h->stream, a->lV, a->d_probes,
+                               a->np, a->freq, a->dT, ts, a->d_F);
+            adj_kernels(a, a->hc + (size_t)j * 3 * a->box.n, a->hv + (size_t)j * 3 * a->box.n);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    // leave the engine at the end of the forward run
+    if ((rc = adj_restore(a, a->ck[S]))) return rc;
+    if (dJ_dvi) HIPCHK(hipMemcpy(dJ_dvi, a->gvi, 3 * a->box.n * sizeof(double), hipMemcpyDeviceToHost));
+    if (dJ_dvv) HIPCHK(hipMemcpy(dJ_dvv, a->gvv, 3 * a->box.n * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+void ems_adj_destroy(ems_adjoint* a)
+{
+    if (!a) return;
+    if (a->h) {
+        hipSetDevice(a->h->device);
+        if (a->h->stream) hipStreamSynchronize(a->h->stream);
+    }
+    for (void* p : a->allocs) hipFree(p);
+    delete a;
 }
 
 }  // extern "C"

@@ -72,6 +72,16 @@ struct Box {
 struct MatBox : Box {
     double eps[3], mue[3], kappa[3], sigma[3];
     bool metal;
+    int grid = -1;   // >= 0: per-cell eps_r / kappa from emsop::grids[grid]
+};
+
+// Material grid: a box of whole mesh cells with its own eps_r and kappa per
+// cell (isotropic), equivalent to one-cell boxes of one priority added in
+// x-fastest order: a point on a face shared by grid cells takes the cell of
+// the lowest index along each axis.
+struct MatGrid {
+    unsigned i0[3], n[3];
+    std::vector<double> eps, kappa;   // [n2][n1][n0] (x fastest)
 };
 
 struct LumpedBox : Box {
@@ -203,6 +213,7 @@ struct emsop {
     double gd = 1;
     double bg[4] = {1, 1, 0, 0};
     std::vector<MatBox> mats;
+    std::vector<MatGrid> grids;
     std::vector<LumpedBox> lumped;
     std::vector<ExcBox> excs;
     int bc[6] = {0, 0, 0, 0, 0, 0};
@@ -229,6 +240,8 @@ struct emsop {
     Ranked<MatBox> matrk;
     struct Samp {
         std::vector<uint64_t> eps_n, P_up, P_dn, PP_up, PP_dn, mu_dn, mu_up, P_half, PP_half, dual_n, dual_P, dual_PP;
+        // sample coordinates (for material grids)
+        std::vector<double> c_eps_n, cPu, cPd, cPh, cPPu, cPPd, cPPh, c_mu_dn, c_mu_up;
     } samp[3];
     ExcList exc[2];
     std::vector<Slab> upml;
@@ -310,7 +323,20 @@ struct emsop {
 
     int build(unsigned nrts);
     void prepare_sampling();
-    double matval(int r, int kind, int ny) const;
+    double matval(int r, int kind, int ny, const double* pt) const;
+    int cell_of(int n, double c) const;
+    double grid_eps(const MatBox& b, const double* pt) const
+    {
+        const MatGrid& G = grids[b.grid];
+        size_t q = 0, stride = 1;
+        for (int a = 0; a < 3; ++a) {
+            int k = cell_of(a, pt[a]);
+            k = std::min(std::max(k, (int)G.i0[a]), (int)(G.i0[a] + G.n[a] - 1));
+            q += (size_t)(k - (int)G.i0[a]) * stride;
+            stride *= G.n[a];
+        }
+        return G.eps[q];
+    }
     void effmat_at(int ny, const unsigned* p, double* E) const;
     int calc_ec();
     double timestep_var3() const;
@@ -352,6 +378,15 @@ void emsop::prepare_sampling()
             cPPd[i] = lines[nPP][i] - rawD(nPP, (int)i - 1) * 0.25;
             cPPh[i] = lines[nPP][i] + rawD(nPP, i) * 0.5;
         }
+        S.c_eps_n = c_eps_n;
+        S.cPu = cPu;
+        S.cPd = cPd;
+        S.cPh = cPh;
+        S.cPPu = cPPu;
+        S.cPPd = cPPd;
+        S.cPPh = cPPh;
+        S.c_mu_dn = c_mu_dn;
+        S.c_mu_up = c_mu_up;
         S.eps_n = matrk.masks(n, c_eps_n);
         S.P_up = matrk.masks(nP, cPu);
         S.P_dn = matrk.masks(nP, cPd);
@@ -367,11 +402,32 @@ void emsop::prepare_sampling()
     }
 }
 
-double emsop::matval(int r, int kind, int ny) const
+// lowest cell index k with lines[k] <= c <= lines[k+1]
+int emsop::cell_of(int n, double c) const
+{
+    const auto& l = lines[n];
+    const size_t j = std::lower_bound(l.begin(), l.end(), c) - l.begin();
+    if (j == 0) return 0;
+    if (j >= l.size()) return (int)l.size() - 2;
+    return (int)j - 1;
+}
+
+double emsop::matval(int r, int kind, int ny, const double* pt) const
 {
     // kind 0 eps, 1 kappa, 2 mue, 3 sigma (GetMaterial, operator.cpp:1281-1334)
     if (r < 0) return kind == 0 ? bg[0] : kind == 1 ? bg[2] : kind == 2 ? bg[1] : bg[3];
     const MatBox* b = matrk.box[r];
+    if (b->grid >= 0 && kind < 2) {
+        const MatGrid& G = grids[b->grid];
+        size_t q = 0, stride = 1;
+        for (int a = 0; a < 3; ++a) {
+            int k = cell_of(a, pt[a]);
+            k = std::min(std::max(k, (int)G.i0[a]), (int)(G.i0[a] + G.n[a] - 1));
+            q += (size_t)(k - (int)G.i0[a]) * stride;
+            stride *= G.n[a];
+        }
+        return kind == 0 ? G.eps[q] : G.kappa[q];
+    }
     switch (kind) {
     case 0: return b->eps[ny];
     case 1: return b->kappa[ny];
@@ -398,18 +454,24 @@ void emsop::effmat_at(int ny, const unsigned* p, double* E) const
         double A1 = Wp(nP, iP - 1) * Wp(nPP, iPP);
         double A2 = Wp(nP, iP) * Wp(nPP, iPP - 1);
         double A3 = Wp(nP, iP - 1) * Wp(nPP, iPP - 1);
+        double pt[4][3];
+        for (int q = 0; q < 4; ++q) pt[q][n] = S.c_eps_n[in];
+        pt[0][nP] = S.cPu[iP];  pt[0][nPP] = S.cPPu[iPP];
+        pt[1][nP] = S.cPd[iP];  pt[1][nPP] = S.cPPu[iPP];
+        pt[2][nP] = S.cPu[iP];  pt[2][nPP] = S.cPPd[iPP];
+        pt[3][nP] = S.cPd[iP];  pt[3][nPP] = S.cPPd[iPP];
         double area = 0;
-        E0 = matval(r0, 0, ny) * A0;
-        E1 = matval(r0, 1, ny) * A0;
+        E0 = matval(r0, 0, ny, pt[0]) * A0;
+        E1 = matval(r0, 1, ny, pt[0]) * A0;
         area += A0;
-        E0 += matval(r1, 0, ny) * A1;
-        E1 += matval(r1, 1, ny) * A1;
+        E0 += matval(r1, 0, ny, pt[1]) * A1;
+        E1 += matval(r1, 1, ny, pt[1]) * A1;
         area += A1;
-        E0 += matval(r2, 0, ny) * A2;
-        E1 += matval(r2, 1, ny) * A2;
+        E0 += matval(r2, 0, ny, pt[2]) * A2;
+        E1 += matval(r2, 1, ny, pt[2]) * A2;
         area += A2;
-        E0 += matval(r3, 0, ny) * A3;
-        E1 += matval(r3, 1, ny) * A3;
+        E0 += matval(r3, 0, ny, pt[3]) * A3;
+        E1 += matval(r3, 1, ny, pt[3]) * A3;
         area += A3;
         E0 *= kEPS0 / area;
         E1 /= area;
@@ -417,14 +479,19 @@ void emsop::effmat_at(int ny, const unsigned* p, double* E) const
         const uint64_t* mPPh = &S.PP_half[(size_t)iPP * W];
         int rd = matrk.winner(&S.mu_dn[(size_t)in * W], mPh, mPPh);
         int ru = matrk.winner(&S.mu_up[(size_t)in * W], mPh, mPPh);
+        double pd[3], pu[3];
+        pd[n] = S.c_mu_dn[in];
+        pu[n] = S.c_mu_up[in];
+        pd[nP] = pu[nP] = S.cPh[iP];
+        pd[nPP] = pu[nPP] = S.cPPh[iPP];
         double dny = Wp(n, in - 1);
-        E2 = dny / matval(rd, 2, ny);
-        double sg = matval(rd, 3, ny);
+        E2 = dny / matval(rd, 2, ny, pd);
+        double sg = matval(rd, 3, ny, pd);
         E3 = sg ? dny / sg : 0;
         double length = dny;
         dny = Wp(n, in);
-        E2 += dny / matval(ru, 2, ny);
-        sg = matval(ru, 3, ny);
+        E2 += dny / matval(ru, 2, ny, pu);
+        sg = matval(ru, 3, ny, pu);
         if (sg)
             E3 += dny / sg;
         else
@@ -444,8 +511,12 @@ void emsop::effmat_at(int ny, const unsigned* p, double* E) const
             if (lp[nP] < 0 || lp[nPP] < 0) continue;
             int r = matrk.winner(&S.dual_n[(size_t)in * W], &S.dual_P[(size_t)lp[nP] * W], &S.dual_PP[(size_t)lp[nPP] * W]);
             double A = Wp(nP, lp[nP]) * Wp(nPP, lp[nPP]);
-            E0 += matval(r, 0, ny) * A;
-            E1 += matval(r, 1, ny) * A;
+            double pt[3];
+            pt[n] = dualL[n][in];
+            pt[nP] = dualL[nP][lp[nP]];
+            pt[nPP] = dualL[nPP][lp[nPP]];
+            E0 += matval(r, 0, ny, pt) * A;
+            E1 += matval(r, 1, ny, pt) * A;
             area += A;
         }
         E0 *= kEPS0 / area;
@@ -456,8 +527,12 @@ void emsop::effmat_at(int ny, const unsigned* p, double* E) const
             if (ln < 0) continue;
             int r = matrk.winner(&S.dual_n[(size_t)ln * W], &S.dual_P[(size_t)iP * W], &S.dual_PP[(size_t)iPP * W]);
             double dny = Wp(n, ln);
-            E2 += dny / matval(r, 2, ny);
-            double sg = matval(r, 3, ny);
+            double pt[3];
+            pt[n] = dualL[n][ln];
+            pt[nP] = dualL[nP][iP];
+            pt[nPP] = dualL[nPP][iPP];
+            E2 += dny / matval(r, 2, ny, pt);
+            double sg = matval(r, 3, ny, pt);
             if (sg)
                 E3 += dny / sg;
             else
@@ -901,13 +976,18 @@ void emsop::build_mur()
                 size_t k = (size_t)i * nb + j;
                 if (r >= 0) {
                     const MatBox* b = rk.box[r];
-                    double eps = b->eps[nyP], mue = b->mue[nyP];
+                    double pt[3];
+                    pt[ny] = cn[0];
+                    pt[nyP] = lines[nyP][i];
+                    pt[nyPP] = lines[nyPP][j];
+                    const double geps = b->grid >= 0 ? grid_eps(*b, pt) : 0.0;
+                    double eps = b->grid >= 0 ? geps : b->eps[nyP], mue = b->mue[nyP];
                     if (vph[f] > 0.0)
                         c0t = vph[f] * dT;
                     else
                         c0t = kC0 * dT / std::sqrt(eps * mue);
                     F.cP[k] = (float)((c0t - delta) / (c0t + delta));
-                    eps = b->eps[nyPP];
+                    eps = b->grid >= 0 ? geps : b->eps[nyPP];
                     mue = b->mue[nyPP];
                     if (vph[f] > 0.0)
                         c0t = vph[f] * dT;
@@ -1276,6 +1356,75 @@ int emsop_add_material(emsop* h, const double eps[3], const double mue[3], const
     b.order = next_order(h);
     b.metal = false;
     h->mats.push_back(b);
+    return 0;
+}
+
+int emsop_add_material_grid(emsop* h, const unsigned cell0[3], const unsigned ncells[3], const double* eps_r,
+                            const double* kappa, double mue_r, double sigma, int priority)
+{
+    if (!h || !cell0 || !ncells || !eps_r) return fail(EMSOP_E_ARG, "null argument");
+    for (int n = 0; n < 3; ++n) {
+        if (h->lines[n].size() < 2) return fail(EMSOP_E_STATE, "set the grid before adding a material grid");
+        if (ncells[n] == 0 || (size_t)cell0[n] + ncells[n] >= h->lines[n].size())
+            return fail(EMSOP_E_ARG, "material grid outside the mesh");
+    }
+    if (!(mue_r > 0)) return fail(EMSOP_E_ARG, "mue_r must be > 0");
+    MatGrid G;
+    size_t cnt = 1;
+    for (int n = 0; n < 3; ++n) {
+        G.i0[n] = cell0[n];
+        G.n[n] = ncells[n];
+        cnt *= ncells[n];
+    }
+    G.eps.assign(eps_r, eps_r + cnt);
+    G.kappa.assign(cnt, 0.0);
+    if (kappa) G.kappa.assign(kappa, kappa + cnt);
+    for (size_t q = 0; q < cnt; ++q)
+        if (!(G.eps[q] > 0)) return fail(EMSOP_E_ARG, "eps_r must be > 0");
+    MatBox b;
+    for (int n = 0; n < 3; ++n) {
+        b.lo[n] = h->lines[n][cell0[n]];
+        b.hi[n] = h->lines[n][cell0[n] + ncells[n]];
+        b.eps[n] = 1.0;
+        b.mue[n] = mue_r;
+        b.kappa[n] = 0.0;
+        b.sigma[n] = sigma;
+    }
+    b.prio = priority;
+    b.order = next_order(h);
+    b.metal = false;
+    b.grid = (int)h->grids.size();
+    h->grids.push_back(std::move(G));
+    h->mats.push_back(b);
+    return 0;
+}
+
+int emsop_eps_weights(const emsop* h, int n, const unsigned pos[3], unsigned cells[12], double dC[4])
+{
+    if (!h || !pos || !cells || !dC) return fail(EMSOP_E_ARG, "null argument");
+    if (n < 0 || n > 2) return fail(EMSOP_E_ARG, "bad component");
+    if (h->cellConst) return fail(EMSOP_E_STATE, "weights are defined for quarter-cell averaging");
+    for (int a = 0; a < 3; ++a)
+        if (pos[a] >= h->N[a]) return fail(EMSOP_E_ARG, "edge outside the mesh");
+    const int nP = (n + 1) % 3, nPP = (n + 2) % 3;
+    const int iP = pos[nP], iPP = pos[nPP];
+    const int sP[4] = {0, -1, 0, -1}, sPP[4] = {0, 0, -1, -1};
+    double A[4], area = 0;
+    for (int q = 0; q < 4; ++q) {
+        A[q] = h->Wp(nP, iP + sP[q]) * h->Wp(nPP, iPP + sPP[q]);
+        area += A[q];
+    }
+    const double delta = h->EL(n, pos[n], false);
+    const double earea = h->EL(nP, pos[nP], true) * h->EL(nPP, pos[nPP], true);
+    for (int q = 0; q < 4; ++q) {
+        int c[3];
+        c[n] = (int)pos[n];
+        c[nP] = iP + sP[q];
+        c[nPP] = iPP + sPP[q];
+        for (int a = 0; a < 3; ++a) cells[3 * q + a] = c[a] < 0 ? 0xFFFFFFFFu : (unsigned)c[a];
+        // C = eps0 * sum_q eps_q A_q / area * earea / delta (AverageMatQuarterCell + Calc_ECPos)
+        dC[q] = (delta && area > 0) ? kEPS0 * A[q] / area * earea / delta : 0.0;
+    }
     return 0;
 }
 
