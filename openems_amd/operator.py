@@ -68,6 +68,20 @@ class Operator:
         check_op(L.emsop_set_background(self.h, float(bg.get("eps", 1.0)), float(bg.get("mue", 1.0)),
                                         float(bg.get("kappa", 0.0)), float(bg.get("sigma", 0.0))))
         for m in s.get("materials", []):
+            if "grid" in m:
+                # material grid: per-cell eps_r / kappa over whole mesh cells
+                g = m["grid"]
+                c0 = np.asarray(g["cell0"], np.uint32)
+                nc = np.asarray(g["ncells"], np.uint32)
+                eps = np.ascontiguousarray(np.asarray(g["eps"], np.float64).reshape(-1))
+                kap = g.get("kappa")
+                kap = None if kap is None else np.ascontiguousarray(np.asarray(kap, np.float64).reshape(-1))
+                self._keep_grid = getattr(self, "_keep_grid", []) + [c0, nc, eps, kap]
+                check_op(L.emsop_add_material_grid(self.h, ptr(c0, C.c_uint), ptr(nc, C.c_uint), ptr(eps, C.c_double),
+                                                   None if kap is None else ptr(kap, C.c_double),
+                                                   float(m.get("mue", 1.0)), float(m.get("sigma", 0.0)),
+                                                   int(m.get("priority", 0))))
+                continue
             arrs = [_v3(m.get(k, d)) for k, d in (("eps", 1.0), ("mue", 1.0), ("kappa", 0.0), ("sigma", 0.0))]
             st, sp = _v3(m["start"]), _v3(m["stop"])
             check_op(L.emsop_add_material(self.h, *[ptr(a, C.c_double) for a in arrs], int(m.get("priority", 0)),

@@ -196,6 +196,43 @@ def batched_sweep(count=64, nrts=4000, seed=20260204, lo=48, hi=160):
     return scenes
 
 
+def adjoint_design(nx=128, ny=128, nz=96, d=0.25, pml=8, design=(48, 48, 8), nrts=8000, seed=20260205,
+                   fc=40e9, freq=30e9, eps_range=(1.0, 12.0)):
+    """Inverse-design workload: uniform d-mm mesh, UPML, soft E_z Gaussian
+    sheet source near the low-x side, a centred design region (material grid,
+    eps_r ~ U[eps_range], kappa = 0) and a 4 x 4 set of E_z objective probes
+    near the high-x side.  Returns (scene, probes, freq) for
+    openems_amd.adjoint.DesignProblem; probes are (component, (x, y, z))."""
+    rng = np.random.default_rng(seed)
+    lines = [np.arange(n) * d for n in (nx, ny, nz)]
+    nd = [int(v) for v in design]
+    c0 = [(n - 1 - k) // 2 for n, k in zip((nx, ny, nz), nd)]
+    eps = rng.uniform(eps_range[0], eps_range[1], size=(nd[2], nd[1], nd[0]))
+    xs = pml + max(2, nx // 16)
+    ys0, ys1 = ny // 2 - max(2, ny // 8), ny // 2 + max(2, ny // 8)
+    zs0, zs1 = nz // 2 - max(2, nz // 12), nz // 2 + max(2, nz // 12)
+    xp = nx - 1 - pml - max(2, nx // 16)
+    py = [ny // 2 + k * max(1, ny // 32) for k in (-3, -1, 1, 3)]
+    pz = [nz // 2 + k * max(1, nz // 48) for k in (-3, -1, 1, 3)]
+    probes = [(2, (xp, y, z)) for z in pz for y in py]
+    scene = {
+        "name": f"adjoint_design_{nx}x{ny}x{nz}",
+        "lines": lines,
+        "delta_unit": 1e-3,
+        "bc": [3] * 6,
+        "pml_size": [pml] * 6,
+        "materials": [{"grid": {"cell0": c0, "ncells": nd, "eps": eps, "kappa": np.zeros_like(eps)},
+                       "mue": 1.0, "sigma": 0.0, "priority": 10}],
+        "excitations": [{"type": 0, "val": (0.0, 0.0, 1.0), "start": (lines[0][xs], lines[1][ys0], lines[2][zs0]),
+                         "stop": (lines[0][xs], lines[1][ys1], lines[2][zs1]), "priority": 0}],
+        "signal": {"type": "gauss", "f0": 0.0, "fc": fc},
+        "nrts": nrts,
+        "oversampling": 4,
+        "probes": [],
+    }
+    return scene, probes, freq
+
+
 WORKLOADS = {
     "pec_cavity_64": lambda: pec_cavity(64, 2000),
     "upml_bricks_256": lambda: upml_bricks(256, 10000),

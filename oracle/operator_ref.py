@@ -331,11 +331,32 @@ class RefOperator:
 
         mats, metals = [], []
         for k, m in enumerate(scene.get("materials", [])):
+            if "grid" in m:
+                # a material grid is, in CSXCAD terms, one box per cell, all of
+                # the grid's priority, added in x-fastest order (earlier wins ties)
+                g = m["grid"]
+                c0, nc = [int(v) for v in g["cell0"]], [int(v) for v in g["ncells"]]
+                eps = np.asarray(g["eps"], np.float64).reshape(-1)
+                kap = np.zeros_like(eps) if g.get("kappa") is None else np.asarray(g["kappa"], np.float64).reshape(-1)
+                L = [np.asarray(l, np.float64) for l in scene["lines"]]
+                ncell = nc[0] * nc[1] * nc[2]
+                j = 0
+                for iz in range(nc[2]):
+                    for iy in range(nc[1]):
+                        for ix in range(nc[0]):
+                            c = (c0[0] + ix, c0[1] + iy, c0[2] + iz)
+                            lo = [L[a][c[a]] for a in range(3)]
+                            hi = [L[a][c[a] + 1] for a in range(3)]
+                            payload = {"eps": _vec3(eps[j]), "mue": _vec3(m.get("mue", 1.0)),
+                                       "kappa": _vec3(kap[j]), "sigma": _vec3(m.get("sigma", 0.0)), "metal": False}
+                            mats.append((m.get("priority", 0), k + j / (ncell + 1.0), lo, hi, payload))
+                            j += 1
+                continue
             payload = {"eps": _vec3(m.get("eps", 1.0)), "mue": _vec3(m.get("mue", 1.0)),
                        "kappa": _vec3(m.get("kappa", 0.0)), "sigma": _vec3(m.get("sigma", 0.0)),
                        "metal": False}
             mats.append((m.get("priority", 0), k, m["start"], m["stop"], payload))
-        off = len(mats)
+        off = len(scene.get("materials", []))
         for k, m in enumerate(scene.get("metals", [])):
             metals.append((m.get("priority", 0), off + k, m["start"], m["stop"], {"metal": True}))
         self.mat_painter = BoxPainter(mats)

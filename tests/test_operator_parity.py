@@ -98,3 +98,43 @@ def test_snapping_matches_oracle(native):
         c = [rng.uniform(lo[n], hi[n]) for n in range(3)]
         for dual in (False, True):
             assert op.snap_to_mesh(c, dual) == ref.mesh.snap(c, dual)
+
+
+def test_material_grid_matches_one_cell_boxes(native):
+    """A material grid (per-cell eps_r / kappa) builds the same operator as the
+    oracle's expansion into one-cell boxes of equal priority, and its
+    quarter-cell weights reproduce C and dC/d eps_r."""
+    s, _, _ = scenes.adjoint_design(30, 28, 26, pml=6, design=(6, 5, 3), nrts=100)
+    g = s["materials"][0]["grid"]
+    g["kappa"] = np.random.default_rng(3).uniform(0, 0.05, size=g["eps"].shape)
+    op, ref = _compare(s)
+    import ctypes as C
+    from openems_amd._native import lib
+    cells = (C.c_uint * 12)()
+    w = (C.c_double * 4)()
+    c0, nc = g["cell0"], g["ncells"]
+    eps = g["eps"]
+    Cc = op.ec("C")
+    for n, pos in [(0, (c0[0] + 2, c0[1] + 2, c0[2] + 1)), (1, (c0[0] + 1, c0[1] + 3, c0[2] + 2)),
+                   (2, (c0[0] + 3, c0[1] + 1, c0[2] + 1))]:
+        assert lib().emsop_eps_weights(op.h, n, (C.c_uint * 3)(*pos), cells, w) == 0
+        total = 0.0
+        for q in range(4):
+            c = [cells[3 * q + a] for a in range(3)]
+            r = [c[a] - c0[a] for a in range(3)]
+            assert all(0 <= r[a] < nc[a] for a in range(3))
+            total += w[q] * eps[r[2], r[1], r[0]]
+        assert total == pytest.approx(float(Cc[n][pos]), rel=1e-6)
+    # derivative: raising one cell's eps_r by 1 changes C by its weight
+    n, pos = 0, (c0[0] + 2, c0[1] + 2, c0[2] + 1)
+    lib().emsop_eps_weights(op.h, n, (C.c_uint * 3)(*pos), cells, w)
+    c = [cells[a] for a in range(3)]
+    s2 = scenes.adjoint_design(30, 28, 26, pml=6, design=(6, 5, 3), nrts=100)[0]
+    s2["materials"][0]["grid"]["eps"] = eps.copy()
+    s2["materials"][0]["grid"]["eps"][c[2] - c0[2], c[1] - c0[1], c[0] - c0[0]] += 1.0
+    s2["materials"][0]["grid"]["kappa"] = g["kappa"]
+    op2 = Operator(s2)
+    dCn = float(op2.ec("C")[n][pos]) - float(Cc[n][pos])
+    assert dCn == pytest.approx(w[0], rel=1e-5)
+    op2.close()
+    op.close()
