@@ -160,6 +160,16 @@ def run_batched(count, steps, tune_steps=6):
         tds.append({k: v["td"] for k, v in sim.results()["probes"].items()})
         sim.close()
     print(f"# batched leg: sequential {seq_s:.2f} s for {count} instances", file=sys.stderr, flush=True)
+    # the same instances one after another on the plain-coefficient engine
+    seq_plain_s = 0.0
+    for sc in scs:
+        sim = Simulation(sc, layout="plain")
+        sim.eng.sync()
+        t0 = time.perf_counter()
+        sim.run(steps)
+        sim.eng.sync()
+        seq_plain_s += time.perf_counter() - t0
+        sim.close()
     b = BatchEngine(scs)
     tuned_ms = b.autotune(steps=tune_steps)
     print(f"# batched leg: autotuned {b.config()['tile_x']=} {tuned_ms:.3f} ms/step", file=sys.stderr, flush=True)
@@ -180,6 +190,8 @@ def run_batched(count, steps, tune_steps=6):
            "instances": count, "cells": cells, "steps": steps,
            "sequential_s": round(seq_s, 3), "batched_s": round(bat_s, 3),
            "speedup": round(seq_s / bat_s, 3),
+           "sequential_plain_s": round(seq_plain_s, 3),
+           "speedup_vs_plain_engine": round(seq_plain_s / bat_s, 3),
            "batched_mcells": round(cells * steps / bat_s / 1e6, 1),
            "sequential_mcells": round(cells * steps / seq_s / 1e6, 1),
            "bit_identical": bool(same), "layout": b.layout,

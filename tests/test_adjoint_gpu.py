@@ -109,6 +109,6 @@ def test_torch_autograd_function(gpu):
     J = objective(e, k, prob)
     J.backward()
     J2, ge, gk = prob.value_and_grad(eps, np.zeros_like(eps))
-    assert float(J) == J2
+    assert J.item() == J2
     assert np.array_equal(e.grad.numpy(), ge) and np.array_equal(k.grad.numpy(), gk)
     prob.close()
