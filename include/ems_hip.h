@@ -251,8 +251,16 @@ int ems_batch_create(ems_batch** out, ems_engine* const* engines, unsigned count
  * work item, else a persistent grid that strides over the items). */
 int ems_batch_set_config(ems_batch* b, unsigned tile_x, const unsigned* zdepth, unsigned grid);
 int ems_batch_get_config(const ems_batch* b, unsigned* tile_x, unsigned* zdepth, unsigned* grid, unsigned* nitems);
-/* Time every candidate configuration for `steps` steps on the batch itself,
- * restore the advanced state, and keep the fastest (best_ms: its time per step). */
+/* Instances are stepped in groups: every group advances all steps of an
+ * ems_batch_iterate call before the next group starts, so a group whose
+ * device working set fits the last-level cache stays resident across steps.
+ * group_bytes: working-set budget of a group, packed first-fit by decreasing
+ * size (0: all instances in one group, the default). */
+int ems_batch_set_grouping(ems_batch* b, unsigned long long group_bytes);
+int ems_batch_get_grouping(const ems_batch* b, unsigned long long* group_bytes, unsigned* ngroups);
+/* Time every candidate configuration (tile shape, z depths, grid, group
+ * budget) for `steps` steps of every group on the batch itself, restore the
+ * advanced state, and keep the fastest (best_ms: its time per step). */
 int ems_batch_autotune(ems_batch* b, unsigned steps, double* best_ms);
 int ems_batch_iterate(ems_batch* b, unsigned nsteps);
 /* device time of the last ems_batch_iterate call */

@@ -106,6 +106,8 @@ ENGINE_API = [
     ("ems_batch_set_config", C.c_int, [vp, C.c_uint, up, C.c_uint]),
     ("ems_batch_get_config", C.c_int, [vp, up, up, up, up]),
     ("ems_batch_autotune", C.c_int, [vp, C.c_uint, dp]),
+    ("ems_batch_set_grouping", C.c_int, [vp, C.c_ulonglong]),
+    ("ems_batch_get_grouping", C.c_int, [vp, C.POINTER(C.c_ulonglong), up]),
     ("ems_batch_iterate", C.c_int, [vp, C.c_uint]),
     ("ems_batch_last_ms", C.c_int, [vp, dp]),
     ("ems_batch_destroy", None, [vp]),

@@ -63,12 +63,19 @@ class BatchEngine:
             zd_p = None
         check(lib().ems_batch_set_config(self.b, int(tile_x), zd_p, int(grid)))
 
+    def set_grouping(self, group_bytes):
+        """Working-set budget of a group of instances stepped together (0: one group)."""
+        check(lib().ems_batch_set_grouping(self.b, int(group_bytes)))
+
     def config(self):
         tx, grid, nitems = C.c_uint(), C.c_uint(), C.c_uint()
         zd = np.zeros(self.count, np.uint32)
         check(lib().ems_batch_get_config(self.b, C.byref(tx), zd.ctypes.data_as(up), C.byref(grid),
                                          C.byref(nitems)))
-        return {"tile_x": tx.value, "zdepth": zd.tolist(), "grid": grid.value, "work_items": nitems.value}
+        gb, ng = C.c_ulonglong(), C.c_uint()
+        check(lib().ems_batch_get_grouping(self.b, C.byref(gb), C.byref(ng)))
+        return {"tile_x": tx.value, "zdepth": zd.tolist(), "grid": grid.value, "work_items": nitems.value,
+                "group_bytes": gb.value, "groups": ng.value}
 
     def iterate(self, nsteps):
         check(lib().ems_batch_iterate(self.b, int(nsteps)))

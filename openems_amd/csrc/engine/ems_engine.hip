@@ -2589,21 +2589,34 @@ struct ems_batch {
     bool cmp = false;
     std::vector<BatchInst> h_inst;
     BatchInst* d_inst = nullptr;
-    std::vector<uint2> h_items;
-    uint2* d_items = nullptr;
-    size_t items_cap = 0;
-    unsigned grid = 0;                       // workgroups of one half-step launch
-    unsigned tx = 0;
+    // per-instance host lists, assembled into per-group device tables
+    std::vector<std::vector<BatchExcGroup>> igrp[2];   // beg/end local to the instance's amp list
+    std::vector<std::vector<float>> iamp[2];
+    std::vector<std::vector<unsigned>> idel[2];
+    std::vector<std::vector<BatchProbe>> iprobes;
+    std::vector<size_t> ibytes;                         // device working set of each instance
+    // groups of instances stepped together (all steps of a group run before
+    // the next group starts, so a group that fits the last-level cache stays
+    // resident); every group has its own work table and source / probe lists
+    struct Group {
+        std::vector<unsigned> inst;
+        uint2* d_items = nullptr;
+        unsigned nitems = 0, grid = 0;
+        BatchExcGroup* d_grp[2] = {nullptr, nullptr};
+        float* d_amp[2] = {nullptr, nullptr};
+        unsigned* d_delay[2] = {nullptr, nullptr};
+        unsigned G[2] = {0, 0};
+        BatchProbe* d_probes = nullptr;
+        unsigned nprobes = 0;
+    };
+    std::vector<Group> groups;
+    std::vector<void*> gallocs;                         // group tables of the current configuration
+    unsigned tx = 0, grid_req = 0;
+    size_t group_bytes = 0;
     std::vector<unsigned> zdepth;
-    BatchExcGroup* d_grp[2] = {nullptr, nullptr};
-    float* d_amp[2] = {nullptr, nullptr};
-    unsigned* d_delay[2] = {nullptr, nullptr};
-    unsigned G[2] = {0, 0};
-    BatchProbe* d_probes = nullptr;
-    unsigned nprobes = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0;
-    int resident = 0;                        // workgroups resident per CU (occupancy)
+    int resident = 0;                                    // workgroups resident per CU (occupancy)
     int cus = 0;
     std::vector<void*> allocs;
 
@@ -2618,11 +2631,14 @@ struct ems_batch {
         return 0;
     }
     template <typename T>
-    int upload(T** d, const T* src, size_t n)
+    int gupload(T** d, const T* src, size_t n)
     {
-        int rc = alloc(d, n);
-        if (rc) return rc;
-        if (n) HIPCHK(hipMemcpy(*d, src, n * sizeof(T), hipMemcpyHostToDevice));
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+        if (e != hipSuccess) return fail(EMS_E_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        gallocs.push_back(q);
+        *d = (T*)q;
+        if (n) HIPCHK(hipMemcpy(q, src, n * sizeof(T), hipMemcpyHostToDevice));
         return 0;
     }
 };
@@ -2630,69 +2646,131 @@ struct ems_batch {
 namespace {
 
 template <int PM, bool CC, bool CURR>
-void batch_launch_t(ems_batch* b)
+void batch_launch_t(ems_batch* b, const ems_batch::Group& g)
 {
-    hipLaunchKernelGGL((k_batch_half<PM, CC, CURR>), dim3(b->grid), dim3(kBlock), 0, b->stream, b->d_inst, b->d_items,
-                       (unsigned)b->h_items.size());
+    hipLaunchKernelGGL((k_batch_half<PM, CC, CURR>), dim3(g.grid), dim3(kBlock), 0, b->stream, b->d_inst, g.d_items,
+                       g.nitems);
 }
 
 template <bool CURR>
-void batch_launch(ems_batch* b)
+void batch_launch(ems_batch* b, const ems_batch::Group& g)
 {
     if (b->cmp) {
-        if (b->pm == kPmlCompressed) batch_launch_t<kPmlCompressed, true, CURR>(b);
-        else if (b->pm == kPmlPlain) batch_launch_t<kPmlPlain, true, CURR>(b);
-        else batch_launch_t<kNoPml, true, CURR>(b);
+        if (b->pm == kPmlCompressed) batch_launch_t<kPmlCompressed, true, CURR>(b, g);
+        else if (b->pm == kPmlPlain) batch_launch_t<kPmlPlain, true, CURR>(b, g);
+        else batch_launch_t<kNoPml, true, CURR>(b, g);
     } else {
-        if (b->pm == kPmlCompressed) batch_launch_t<kPmlCompressed, false, CURR>(b);
-        else if (b->pm == kPmlPlain) batch_launch_t<kPmlPlain, false, CURR>(b);
-        else batch_launch_t<kNoPml, false, CURR>(b);
+        if (b->pm == kPmlCompressed) batch_launch_t<kPmlCompressed, false, CURR>(b, g);
+        else if (b->pm == kPmlPlain) batch_launch_t<kPmlPlain, false, CURR>(b, g);
+        else batch_launch_t<kNoPml, false, CURR>(b, g);
     }
 }
 
-// work table and instance descriptors for the current tile shape / depths
-int batch_build_items(ems_batch* b, unsigned tx, const std::vector<unsigned>& zd, unsigned grid)
+// Configuration: tile shape, z depth per instance, grid size, and the
+// working-set budget of a group (0: all instances in one group).  Groups are
+// packed first-fit by decreasing working set.
+int batch_build(ems_batch* b, unsigned tx, const std::vector<unsigned>& zd, unsigned grid, size_t group_bytes)
 {
     const size_t n = b->eng.size();
-    std::vector<uint2> items;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (void* p : b->gallocs) hipFree(p);
+    b->gallocs.clear();
+    b->groups.clear();
     for (size_t i = 0; i < n; ++i) {
         BatchInst& B = b->h_inst[i];
         B.g.zdepth = zd[i];
         set_tiles(B.g, tx);
-        const unsigned long long cnt = (unsigned long long)B.g.qtiles * B.g.nzc;
-        if (cnt > 0xFFFFFFFFull) return fail(EMS_E_ARG, "instance too large for the batch work table");
-        for (unsigned t = 0; t < (unsigned)cnt; ++t) items.push_back(make_uint2((unsigned)i, t));
+        if ((unsigned long long)B.g.qtiles * B.g.nzc > 0xFFFFFFFFull)
+            return fail(EMS_E_ARG, "instance too large for the batch work table");
     }
-    if (items.size() > 0xFFFFFFFFull) return fail(EMS_E_ARG, "batch work table too large");
-    HIPCHK(hipStreamSynchronize(b->stream));
     HIPCHK(hipMemcpy(b->d_inst, b->h_inst.data(), n * sizeof(BatchInst), hipMemcpyHostToDevice));
-    if (items.size() > b->items_cap || !b->d_items) {
-        int rc = b->alloc(&b->d_items, items.size());
-        if (rc) return rc;
-        b->items_cap = items.size();
+    // partition
+    std::vector<std::vector<unsigned>> parts;
+    if (group_bytes == 0) {
+        parts.emplace_back();
+        for (unsigned i = 0; i < n; ++i) parts[0].push_back(i);
+    } else {
+        std::vector<unsigned> order(n);
+        for (unsigned i = 0; i < n; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](unsigned a, unsigned c) { return b->ibytes[a] > b->ibytes[c]; });
+        std::vector<size_t> load;
+        for (unsigned i : order) {
+            size_t k = 0;
+            while (k < parts.size() && load[k] + b->ibytes[i] > group_bytes) ++k;
+            if (k == parts.size()) {
+                parts.emplace_back();
+                load.push_back(0);
+            }
+            parts[k].push_back(i);
+            load[k] += b->ibytes[i];
+        }
+        for (auto& p : parts) std::sort(p.begin(), p.end());
     }
-    b->h_items = items;
-    HIPCHK(hipMemcpy(b->d_items, items.data(), items.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    for (const auto& p : parts) {
+        ems_batch::Group g;
+        g.inst = p;
+        std::vector<uint2> items;
+        for (unsigned i : p) {
+            const BatchInst& B = b->h_inst[i];
+            const unsigned cnt = B.g.qtiles * B.g.nzc;
+            for (unsigned t = 0; t < cnt; ++t) items.push_back(make_uint2(i, t));
+        }
+        int rc;
+        if ((rc = b->gupload(&g.d_items, items.data(), items.size()))) return rc;
+        g.nitems = (unsigned)items.size();
+        g.grid = grid == 0 ? g.nitems : std::min(grid, g.nitems);
+        for (int c = 0; c < 2; ++c) {
+            std::vector<BatchExcGroup> grp;
+            std::vector<float> amp;
+            std::vector<unsigned> del;
+            for (unsigned i : p) {
+                for (BatchExcGroup q : b->igrp[c][i]) {
+                    q.beg += (unsigned)amp.size();
+                    q.end += (unsigned)amp.size();
+                    grp.push_back(q);
+                }
+                amp.insert(amp.end(), b->iamp[c][i].begin(), b->iamp[c][i].end());
+                del.insert(del.end(), b->idel[c][i].begin(), b->idel[c][i].end());
+            }
+            g.G[c] = (unsigned)grp.size();
+            if (g.G[c] && ((rc = b->gupload(&g.d_grp[c], grp.data(), grp.size())) ||
+                           (rc = b->gupload(&g.d_amp[c], amp.data(), amp.size())) ||
+                           (rc = b->gupload(&g.d_delay[c], del.data(), del.size()))))
+                return rc;
+        }
+        std::vector<BatchProbe> pr;
+        for (unsigned i : p) pr.insert(pr.end(), b->iprobes[i].begin(), b->iprobes[i].end());
+        g.nprobes = (unsigned)pr.size();
+        if (g.nprobes && (rc = b->gupload(&g.d_probes, pr.data(), pr.size()))) return rc;
+        b->groups.push_back(g);
+    }
     b->tx = tx;
     b->zdepth = zd;
-    const unsigned full = (unsigned)items.size();
-    b->grid = grid == 0 ? full : std::min(grid, full);
+    b->grid_req = grid;
+    b->group_bytes = group_bytes;
     return 0;
 }
 
-void batch_step(ems_batch* b)
+void batch_step(ems_batch* b, const ems_batch::Group& g)
 {
     const unsigned T = 256;
-    batch_launch<false>(b);
-    if (b->G[0])
-        hipLaunchKernelGGL(k_batch_excite, dim3((b->G[0] + T - 1) / T), dim3(T), 0, b->stream, b->d_inst, b->d_grp[0],
-                           b->G[0], b->d_amp[0], b->d_delay[0], 0, 0);
-    batch_launch<true>(b);
-    if (b->G[1])
-        hipLaunchKernelGGL(k_batch_excite, dim3((b->G[1] + T - 1) / T), dim3(T), 0, b->stream, b->d_inst, b->d_grp[1],
-                           b->G[1], b->d_amp[1], b->d_delay[1], 1, -1);
-    if (b->nprobes)
-        hipLaunchKernelGGL(k_batch_probe, dim3(b->nprobes), dim3(kBlock), 0, b->stream, b->d_inst, b->d_probes);
+    batch_launch<false>(b, g);
+    if (g.G[0])
+        hipLaunchKernelGGL(k_batch_excite, dim3((g.G[0] + T - 1) / T), dim3(T), 0, b->stream, b->d_inst, g.d_grp[0],
+                           g.G[0], g.d_amp[0], g.d_delay[0], 0, 0);
+    batch_launch<true>(b, g);
+    if (g.G[1])
+        hipLaunchKernelGGL(k_batch_excite, dim3((g.G[1] + T - 1) / T), dim3(T), 0, b->stream, b->d_inst, g.d_grp[1],
+                           g.G[1], g.d_amp[1], g.d_delay[1], 1, -1);
+    if (g.nprobes)
+        hipLaunchKernelGGL(k_batch_probe, dim3(g.nprobes), dim3(kBlock), 0, b->stream, b->d_inst, g.d_probes);
+}
+
+// every group advances nsteps, group after group
+void batch_run(ems_batch* b, unsigned nsteps)
+{
+    for (const auto& g : b->groups)
+        for (unsigned s = 0; s < nsteps; ++s) batch_step(b, g);
 }
 
 }  // namespace
@@ -2735,10 +2813,12 @@ int ems_batch_create(ems_batch** out, ems_engine* const* engines, unsigned count
     hipEventCreate(&b->ev0);
     hipEventCreate(&b->ev1);
     int rc;
-    std::vector<BatchExcGroup> grp[2];
-    std::vector<float> amp[2];
-    std::vector<unsigned> del[2];
-    std::vector<BatchProbe> probes;
+    for (int c = 0; c < 2; ++c) {
+        b->igrp[c].resize(count);
+        b->iamp[c].resize(count);
+        b->idel[c].resize(count);
+    }
+    b->iprobes.resize(count);
     for (unsigned i = 0; i < count; ++i) {
         ems_engine* e = engines[i];
         HIPCHK(hipStreamSynchronize(e->stream));
@@ -2758,18 +2838,28 @@ int ems_batch_create(ems_batch** out, ems_engine* const* engines, unsigned count
         B.period = e->period;
         b->h_inst.push_back(B);
         b->zdepth.push_back(e->g.zdepth);
+        // device working set: fields, coefficients, UPML
+        size_t bytes = 2 * 3 * e->cs * sizeof(float);
+        bytes += e->cmp ? e->cs * sizeof(unsigned short) + 2 * 32 * (size_t)e->n_unique : 4 * 3 * e->cs * sizeof(float);
+        if (e->pmlV.n) {
+            const size_t np = e->pmlV.npos;
+            bytes += 2 * 3 * np * sizeof(float);
+            bytes += e->pml_mode == kPmlCompressed ? np * sizeof(unsigned) + 2 * 48 * (size_t)e->n_unique_pml
+                                                   : 2 * 9 * np * sizeof(float);
+        }
+        b->ibytes.push_back(bytes);
         for (int c = 0; c < 2; ++c) {
             const auto& E = e->exc[c];
             for (unsigned g = 0; g < E.G; ++g) {
                 BatchExcGroup q{};
                 q.inst = i;
                 q.lin = E.h_glin[g];
-                q.beg = (unsigned)amp[c].size() + E.h_gbeg[g];
-                q.end = (unsigned)amp[c].size() + E.h_gbeg[g + 1];
-                grp[c].push_back(q);
+                q.beg = E.h_gbeg[g];
+                q.end = E.h_gbeg[g + 1];
+                b->igrp[c][i].push_back(q);
             }
-            amp[c].insert(amp[c].end(), E.h_amp.begin(), E.h_amp.end());
-            del[c].insert(del[c].end(), E.h_delay.begin(), E.h_delay.end());
+            b->iamp[c][i] = E.h_amp;
+            b->idel[c][i] = E.h_delay;
         }
         for (const auto& P : e->probes) {
             BatchProbe q{};
@@ -2778,20 +2868,10 @@ int ems_batch_create(ems_batch** out, ems_engine* const* engines, unsigned count
             q.elin = e->d_elin;
             q.esign = e->d_esign;
             q.ediv = e->d_ediv;
-            probes.push_back(q);
+            b->iprobes[i].push_back(q);
         }
     }
     if ((rc = b->alloc(&b->d_inst, count))) return bail(rc);
-    for (int c = 0; c < 2; ++c) {
-        b->G[c] = (unsigned)grp[c].size();
-        if (!b->G[c]) continue;
-        if ((rc = b->upload(&b->d_grp[c], grp[c].data(), grp[c].size())) ||
-            (rc = b->upload(&b->d_amp[c], amp[c].data(), amp[c].size())) ||
-            (rc = b->upload(&b->d_delay[c], del[c].data(), del[c].size())))
-            return bail(rc);
-    }
-    b->nprobes = (unsigned)probes.size();
-    if (b->nprobes && (rc = b->upload(&b->d_probes, probes.data(), probes.size()))) return bail(rc);
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, b->device));
     b->cus = prop.multiProcessorCount;
@@ -2801,8 +2881,8 @@ int ems_batch_create(ems_batch** out, ems_engine* const* engines, unsigned count
         occ <= 0)
         occ = 4;
     b->resident = occ;
-    // default: each instance's own depth, linear tiles, one workgroup per item
-    if ((rc = batch_build_items(b, 0, b->zdepth, 0))) return bail(rc);
+    // default: each instance's own depth, linear tiles, one workgroup per item, one group
+    if ((rc = batch_build(b, 0, b->zdepth, 0, 0))) return bail(rc);
     *out = b;
     return 0;
 }
@@ -2817,7 +2897,13 @@ int ems_batch_set_config(ems_batch* b, unsigned tile_x, const unsigned* zdepth, 
         zd[i] = zdepth ? zdepth[i] : choose_zdepth(b->eng[i]->N);
         if (zd[i] < 1 || zd[i] > 4096) return fail(EMS_E_ARG, "z depth out of range");
     }
-    return batch_build_items(b, tile_x, zd, grid);
+    return batch_build(b, tile_x, zd, grid, b->group_bytes);
+}
+
+int ems_batch_set_grouping(ems_batch* b, unsigned long long group_bytes)
+{
+    if (!b) return fail(EMS_E_ARG, "null batch");
+    return batch_build(b, b->tx, b->zdepth, b->grid_req, (size_t)group_bytes);
 }
 
 int ems_batch_get_config(const ems_batch* b, unsigned* tile_x, unsigned* zdepth, unsigned* grid, unsigned* nitems)
@@ -2826,8 +2912,20 @@ int ems_batch_get_config(const ems_batch* b, unsigned* tile_x, unsigned* zdepth,
     if (tile_x) *tile_x = b->tx;
     if (zdepth)
         for (size_t i = 0; i < b->zdepth.size(); ++i) zdepth[i] = b->zdepth[i];
-    if (grid) *grid = b->grid;
-    if (nitems) *nitems = (unsigned)b->h_items.size();
+    if (grid) *grid = b->grid_req;
+    if (nitems) {
+        unsigned t = 0;
+        for (const auto& g : b->groups) t += g.nitems;
+        *nitems = t;
+    }
+    return 0;
+}
+
+int ems_batch_get_grouping(const ems_batch* b, unsigned long long* group_bytes, unsigned* ngroups)
+{
+    if (!b) return fail(EMS_E_ARG, "null batch");
+    if (group_bytes) *group_bytes = b->group_bytes;
+    if (ngroups) *ngroups = (unsigned)b->groups.size();
     return 0;
 }
 
@@ -2836,11 +2934,13 @@ int ems_batch_iterate(ems_batch* b, unsigned nsteps)
     if (!b) return fail(EMS_E_ARG, "null batch");
     HIPCHK(hipSetDevice(b->device));
     for (auto* e : b->eng) HIPCHK(hipStreamSynchronize(e->stream));
-    if (b->eng[0]->numTS == 0 && nsteps > 0 && b->nprobes)
+    if (b->eng[0]->numTS == 0 && nsteps > 0)
         // Process() before the first step (openems.cpp:1424-1425)
-        hipLaunchKernelGGL(k_batch_probe, dim3(b->nprobes), dim3(kBlock), 0, b->stream, b->d_inst, b->d_probes);
+        for (const auto& g : b->groups)
+            if (g.nprobes)
+                hipLaunchKernelGGL(k_batch_probe, dim3(g.nprobes), dim3(kBlock), 0, b->stream, b->d_inst, g.d_probes);
     HIPCHK(hipEventRecord(b->ev0, b->stream));
-    for (unsigned s = 0; s < nsteps; ++s) batch_step(b);
+    batch_run(b, nsteps);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(b->ev1, b->stream));
     HIPCHK(hipEventSynchronize(b->ev1));
@@ -2851,18 +2951,19 @@ int ems_batch_iterate(ems_batch* b, unsigned nsteps)
     return 0;
 }
 
-// Autotuner: time every candidate tile shape x z-depth policy x grid size on
-// the batch itself for `steps` steps, restore the state the candidates
-// advanced (fields, UPML fluxes, counters, probe sample counts), and keep the
-// fastest.  Policies: each instance's own depth (choose_zdepth), or 4, 8, 16
-// for all.  Grids: one workgroup per work item, or the resident workgroups.
+// Autotuner: time every candidate tile shape x z-depth policy x grid size x
+// group budget on the batch itself for `steps` steps of every group, restore
+// the state the candidates advanced (fields, UPML fluxes, counters, probe
+// sample counts), and keep the fastest.  Depth policies: each instance's own
+// (choose_zdepth) or 8 for all; grids: one workgroup per work item or the
+// resident workgroups; group budgets: one group, 64, 128 or 192 MiB (the
+// last-level cache holds 256 MiB).
 int ems_batch_autotune(ems_batch* b, unsigned steps, double* best_ms)
 {
     if (!b) return fail(EMS_E_ARG, "null batch");
-    if (steps == 0) steps = 4;
+    if (steps < 2) steps = 2;
     HIPCHK(hipSetDevice(b->device));
     for (auto* e : b->eng) HIPCHK(hipStreamSynchronize(e->stream));
-    // snapshot
     struct Saved {
         float *V, *I, *fV, *fI;
         unsigned numts[2];
@@ -2925,43 +3026,50 @@ int ems_batch_autotune(ems_batch* b, unsigned steps, double* best_ms)
         // device-to-device copies are not ordered with the batch's stream
         hipDeviceSynchronize();
     };
-    const unsigned txs[3] = {0, 64, 32};
-    const unsigned depths[4] = {0, 4, 8, 16};   // 0: each instance's own choice
+    const unsigned txs[2] = {0, 64};
+    const unsigned depths[2] = {0, 8};   // 0: each instance's own choice
     const unsigned grids[2] = {0, (unsigned)(b->resident * b->cus)};
+    const size_t mib = size_t(1) << 20;
+    const size_t budgets[4] = {0, 64 * mib, 128 * mib, 192 * mib};
     double best = 1e30;
     unsigned btx = 0, bdep = 0, bgrid = 0;
-    const unsigned ts0 = b->eng[0]->numTS;
-    for (unsigned tx : txs)
-        for (unsigned dep : depths)
-            for (unsigned grid : grids) {
-                std::vector<unsigned> zd(b->eng.size());
-                for (size_t i = 0; i < zd.size(); ++i) zd[i] = dep ? dep : choose_zdepth(b->eng[i]->N);
-                int rc = batch_build_items(b, tx, zd, grid);
-                if (rc) {
+    size_t bbud = 0;
+    for (size_t bud : budgets)
+        for (unsigned tx : txs)
+            for (unsigned dep : depths)
+                for (unsigned grid : grids) {
+                    std::vector<unsigned> zd(b->eng.size());
+                    for (size_t i = 0; i < zd.size(); ++i) zd[i] = dep ? dep : choose_zdepth(b->eng[i]->N);
+                    int rc = batch_build(b, tx, zd, grid, bud);
+                    if (rc) {
+                        restore();
+                        cleanup();
+                        return rc;
+                    }
+                    // per group: one warm-up step, then steps-1 timed steps
+                    double tot = 0;
+                    for (const auto& g : b->groups) {
+                        batch_step(b, g);
+                        hipEventRecord(b->ev0, b->stream);
+                        for (unsigned s = 1; s < steps; ++s) batch_step(b, g);
+                        hipEventRecord(b->ev1, b->stream);
+                        hipEventSynchronize(b->ev1);
+                        float ms = 0;
+                        hipEventElapsedTime(&ms, b->ev0, b->ev1);
+                        tot += ms / (steps - 1);
+                    }
+                    if (tot < best) {
+                        best = tot;
+                        btx = tx;
+                        bdep = dep;
+                        bgrid = grid;
+                        bbud = bud;
+                    }
                     restore();
-                    cleanup();
-                    return rc;
                 }
-                batch_step(b);   // warm-up
-                hipEventRecord(b->ev0, b->stream);
-                for (unsigned s = 1; s < steps; ++s) batch_step(b);
-                hipEventRecord(b->ev1, b->stream);
-                hipEventSynchronize(b->ev1);
-                float ms = 0;
-                hipEventElapsedTime(&ms, b->ev0, b->ev1);
-                const double per = steps > 1 ? ms / (steps - 1) : ms;
-                if (per < best) {
-                    best = per;
-                    btx = tx;
-                    bdep = dep;
-                    bgrid = grid;
-                }
-                restore();
-            }
-    (void)ts0;
     std::vector<unsigned> zd(b->eng.size());
     for (size_t i = 0; i < zd.size(); ++i) zd[i] = bdep ? bdep : choose_zdepth(b->eng[i]->N);
-    int rc = batch_build_items(b, btx, zd, bgrid);
+    int rc = batch_build(b, btx, zd, bgrid, bbud);
     HIPCHK(hipStreamSynchronize(b->stream));
     cleanup();
     if (best_ms) *best_ms = best;
@@ -2980,6 +3088,7 @@ void ems_batch_destroy(ems_batch* b)
     if (!b) return;
     hipSetDevice(b->device);
     if (b->stream) hipStreamSynchronize(b->stream);
+    for (void* p : b->gallocs) hipFree(p);
     for (void* p : b->allocs) hipFree(p);
     if (b->ev0) hipEventDestroy(b->ev0);
     if (b->ev1) hipEventDestroy(b->ev1);

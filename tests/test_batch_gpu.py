@@ -56,7 +56,8 @@ def test_batch_instances_match_oracle(gpu):
     b.close()
 
 
-@pytest.mark.parametrize("config", ["autotune", "tiles32_depth3_grid7", "tiles64_own_depths_full_grid"])
+@pytest.mark.parametrize("config", ["autotune", "tiles32_depth3_grid7", "tiles64_own_depths_full_grid",
+                                    "groups_of_one_or_two"])
 def test_batch_equals_engines_alone(gpu, config):
     scs = _small_batch(4, 90, seed=77)
     print("case batch of 4 vs engines alone, 90 steps,", config, flush=True)
@@ -67,6 +68,9 @@ def test_batch_equals_engines_alone(gpu, config):
         assert ms > 0
     elif config == "tiles32_depth3_grid7":
         b.set_config(tile_x=32, zdepth=[3] * b.count, grid=7)
+    elif config == "groups_of_one_or_two":
+        b.set_grouping(3 << 20)          # a few MiB: several groups, stepped one after another
+        assert b.config()["groups"] > 1
     else:
         b.set_config(tile_x=64, zdepth=None, grid=0)
     print("config", b.config(), flush=True)
