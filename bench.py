@@ -264,6 +264,44 @@ def run_adjoint(nsteps=8000, K=256, seed=5):
             "grad_norm_eps": float(np.linalg.norm(ge))}
 
 
+def run_memory_scale(steps=200, warmup=20):
+    """Memory-scale workload: 1024x1024x768 lines (805 M cells), 12 seeded
+    dielectric layers, Mur on every face, sinusoidal 15 GHz E_x sheet source;
+    default engine (compressed layout).  Throughput over `steps` steps after
+    `warmup`, with the update kernels' average times (HIP events)."""
+    from openems_amd import scenes
+    from openems_amd.engine import EngineHIP
+    from openems_amd.operator import Operator
+    scene = scenes.mur_layers(nrts=steps + warmup)
+    t0 = time.perf_counter()
+    op = Operator(scene)
+    build_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    eng = EngineHIP(op)
+    create_s = time.perf_counter() - t0
+    cells = float(op.num_cells)
+    lay, nu, _, _ = eng.layout()
+    eng.IterateTS(warmup)
+    eng.sync()
+    eng.set_profiling(True)
+    t0 = time.perf_counter()
+    eng.IterateTS(steps)
+    eng.sync()
+    wall = time.perf_counter() - t0
+    _, vms, cms = eng.last_timing()
+    eng.close()
+    op.close()
+    kms = max(vms, cms)
+    per_launch = LAYOUT_BYTES[lay] * cells
+    return {"workload": "mur_layers: 1024x1024x768 lines, 12 seeded layers, Mur x6, sinusoidal E_x sheet, "
+                        f"{steps} steps", "cells": int(cells), "layout": f"{lay} ({nu} unique)",
+            "value": round(cells * steps / wall / 1e6, 1), "unit": "MCells/s", "ms_per_step": round(wall * 1e3 / steps, 3),
+            "operator_build_s": round(build_s, 1), "engine_create_s": round(create_s, 1),
+            "update_kernel_ms": round(kms, 4), "bytes_per_launch": int(per_launch),
+            "achieved_GBs": round(per_launch / (kms * 1e-3) / 1e9, 1),
+            "frac": round(per_launch / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -278,6 +316,8 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=4000)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--memory-scale", action="store_true",
+                    help="also run the 1024x1024x768 Mur workload (about 80 GB of host memory for its build)")
     ap.add_argument("--adjoint-steps", type=int, default=8000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (for scaling HBM byte counters)")
@@ -382,6 +422,12 @@ def main():
         except Exception as e:  # reported, the headline line still prints
             out["adjoint"] = {"error": str(e)}
         print(f"# adjoint: {out['adjoint']}", file=sys.stderr)
+    if args.memory_scale:
+        try:
+            out["memory_scale"] = run_memory_scale()
+        except Exception as e:  # reported, the headline line still prints
+            out["memory_scale"] = {"error": str(e)}
+        print(f"# memory scale: {out['memory_scale']}", file=sys.stderr)
     print(json.dumps(out), flush=True)
 
 
