@@ -37,6 +37,8 @@
  *                                                    Common/processfieldprobe.cpp:77-92
  *   ems_add_fd_dump       ProcessFieldsFD::Process                       Common/processfields_fd.cpp:72-107
  *   ems_energy            Engine_Interface_FDTD::CalcFastEnergy          FDTD/engine_interface_fdtd.cpp:302-347
+ *   ems_set_steady_state  Operator_Ext_SteadyState -> Engine_Ext_SteadyState
+ *   ems_steady_state                                 FDTD/extensions/engine_ext_steadystate.cpp:50-107
  *   ems_destroy           Engine::~Engine / Engine::Reset                FDTD/engine.cpp:45-108
  *   ems_adj_*             no counterpart: discrete adjoint of Engine::IterateTS for gradients
  *   ems_batch_*           no counterpart: many independent Engine::IterateTS loops
@@ -115,6 +117,21 @@ int ems_compress_operator(const unsigned numLines[3], const float* vv, const flo
 int ems_expand_sse_compressed(const unsigned numLines[3], const unsigned* op_index, unsigned n_unique,
                               const float* const vv4[3], const float* const vi4[3], const float* const ii4[3],
                               const float* const iv4[3], float* vv, float* vi, float* ii, float* iv);
+
+/* Steady-state detection (Operator_Ext_SteadyState / Engine_Ext_SteadyState,
+ * FDTD/extensions/engine_ext_steadystate.cpp:50-107, set up by
+ * openems.cpp:1205-1234 for periodic excitations), before ems_init.
+ * period_steps: m_TS_period = round(period / dT); probes dir[n], pos[3][n]
+ * (E probes on voltage edges).  Every step records the probe voltages after
+ * the voltage update (before the other apply hooks); every period from 2p on
+ * it sets the last max diff: the largest relative power change between the
+ * last two periods over the probes holding at least 1 % of the largest
+ * power, and the relative energy change.  history_capacity: evaluations kept. */
+int ems_set_steady_state(ems_engine* h, unsigned period_steps, unsigned nprobe, const int* dir, const unsigned* pos,
+                         unsigned history_capacity);
+/* Engine_Ext_SteadyState::GetLastDiff and the evaluation history (up to cap). */
+int ems_steady_state(ems_engine* h, double* last_max_diff, unsigned* evaluations, unsigned* ts, double* diff,
+                     unsigned cap);
 
 /* Step form, before ems_init.  The fused form advances a whole timestep in
  * one z-sweep over double-buffered fields (voltage update, voltage sources,

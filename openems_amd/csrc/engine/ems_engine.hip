@@ -1340,6 +1340,86 @@ __global__ void k_adj_src(float* __restrict__ lV, const AdjProbe* __restrict__ p
     lV[pr[p].lin] = (float)((double)lV[pr[p].lin] + g);
 }
 
+// ---------------------------------------------------------------------------
+// steady-state detection (Engine_Ext_SteadyState::Apply2Voltages,
+// FDTD/extensions/engine_ext_steadystate.cpp:50-107).  Priority +2e6: it runs
+// before every other apply hook, on the voltages of the completed voltage
+// update.  Every step records the probe voltages at TS % 2p; at TS % p == 0,
+// TS >= 2p it compares the last two periods and the energy change.
+// ---------------------------------------------------------------------------
+struct SteadyDev {
+    unsigned n, p, cap;
+    const unsigned long long* lin;   // [n] component * cs + cell
+    float* rec;                      // [n][2p]
+    double* state;                   // [0] last max diff, [1] last total energy, [2] evaluations
+    unsigned* hist_ts;               // [cap]
+    double* hist_diff;               // [cap]
+};
+
+__global__ void k_ss_record(const float* __restrict__ V, const SteadyDev s, const unsigned* __restrict__ numts)
+{
+    const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= s.n) return;
+    const unsigned rel = *numts % (2 * s.p);
+    s.rec[(size_t)k * 2 * s.p + rel] = V[s.lin[k]];
+}
+
+// one thread: the reference's evaluation, energy from k_energy's partials
+__global__ void k_ss_eval(const SteadyDev s, const double* __restrict__ part, unsigned nb,
+                          const unsigned* __restrict__ numts)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const unsigned TS = *numts, p = s.p;
+    const unsigned rel_pos = TS % (2 * p);
+    double e = 0, m = 0;
+    for (unsigned b = 0; b < nb; ++b) {
+        e += part[2 * b];
+        m += part[2 * b + 1];
+    }
+    const double curr_total_energy = 8.85418781762e-12 * e + 1.256637062e-6 * m;
+    bool no_valid = true;
+    double last_max_diff = 0;
+    const double last_total_energy = s.state[1];
+    if (last_total_energy > 0) {
+        last_max_diff = fabs(curr_total_energy - last_total_energy) / last_total_energy;
+        no_valid = false;
+    }
+    unsigned old_pos = 0, new_pos = p;
+    if (rel_pos <= p) {
+        new_pos = 0;
+        old_pos = p;
+    }
+    double max_pow = 0;
+    for (unsigned n = 0; n < s.n; ++n) {
+        const float* buf = s.rec + (size_t)n * 2 * p;
+        double cp = 0;
+        for (unsigned nt = 0; nt < p; ++nt) cp += (double)buf[nt + new_pos] * (double)buf[nt + new_pos];
+        max_pow = fmax(max_pow, cp);
+    }
+    for (unsigned n = 0; n < s.n; ++n) {
+        const float* buf = s.rec + (size_t)n * 2 * p;
+        double cp = 0, dp = 0;
+        for (unsigned nt = 0; nt < p; ++nt) {
+            const double a = buf[nt + new_pos], b = buf[nt + old_pos];
+            cp += a * a;
+            dp += (b - a) * (b - a);
+        }
+        if (cp > max_pow * 1e-2) {
+            last_max_diff = fmax(last_max_diff, dp / cp);
+            no_valid = false;
+        }
+    }
+    if (no_valid || last_max_diff > 1) last_max_diff = 1;
+    s.state[0] = last_max_diff;
+    s.state[1] = curr_total_energy;
+    const unsigned k = (unsigned)s.state[2];
+    if (k < s.cap) {
+        s.hist_ts[k] = TS;
+        s.hist_diff[k] = last_max_diff;
+    }
+    s.state[2] = (double)(k + 1);
+}
+
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
@@ -1433,6 +1513,11 @@ struct ems_engine {
     std::vector<unsigned> dump_npts;
 
     double* energy_part = nullptr;
+
+    // steady-state detection
+    bool has_ss = false;
+    SteadyDev ss{};
+    double* ss_part = nullptr;
 
     // timing
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -2033,6 +2118,7 @@ static int setup_fused(ems_engine* h)
     if (!h->mur.empty()) why = "Mur ABC has no fused form";
     else if (!h->lor.empty()) why = "Lorentz / Drude materials have no fused form";
     else if (!h->rlc.empty()) why = "lumped RLC elements have no fused form";
+    else if (h->has_ss) why = "steady-state detection has no fused form";
     else if (Nx >= 65535 || Ny >= 65535 || Nz >= 65535) why = "mesh too large for the fused step";
     for (const auto& e : h->exc[1].host)
         if (why.empty() && (e.x + 1u >= Nx || e.y + 1u >= Ny || e.z + 1u >= Nz))
@@ -2110,6 +2196,58 @@ int ems_get_fused(const ems_engine* h, int* fused, unsigned* zdepth)
     if (!h) return fail(EMS_E_ARG, "null handle");
     if (fused) *fused = h->fused ? 1 : 0;
     if (zdepth) *zdepth = h->fused ? h->fzdepth : h->g.zdepth;
+    return 0;
+}
+
+int ems_set_steady_state(ems_engine* h, unsigned period_steps, unsigned nprobe, const int* dir, const unsigned* pos,
+                         unsigned history_capacity)
+{
+    if (!h || !dir || !pos) return fail(EMS_E_ARG, "null argument");
+    if (h->inited) return fail(EMS_E_STATE, "engine already initialised");
+    if (h->has_ss) return fail(EMS_E_STATE, "steady-state detection already set");
+    if (period_steps == 0 || nprobe == 0) return fail(EMS_E_ARG, "empty period or probe list");
+    std::vector<unsigned long long> lin(nprobe);
+    for (unsigned k = 0; k < nprobe; ++k) {
+        const unsigned x = pos[k], y = pos[nprobe + k], z = pos[2 * nprobe + k];
+        if (dir[k] < 0 || dir[k] > 2 || x >= h->N[0] || y >= h->N[1] || z >= h->N[2])
+            return fail(EMS_E_ARG, "steady-state probe outside the mesh");
+        lin[k] = (unsigned long long)dir[k] * h->cs + h->lin(x, y, z);
+    }
+    HIPCHK(hipSetDevice(h->device));
+    SteadyDev S{};
+    S.n = nprobe;
+    S.p = period_steps;
+    S.cap = std::max(1u, history_capacity);
+    unsigned long long* dl;
+    int rc;
+    if ((rc = upload(h, &dl, lin.data(), nprobe)) || (rc = h->alloc(&S.rec, (size_t)nprobe * 2 * period_steps)) ||
+        (rc = h->alloc(&S.state, 3)) || (rc = h->alloc(&S.hist_ts, S.cap)) || (rc = h->alloc(&S.hist_diff, S.cap)) ||
+        (rc = h->alloc(&h->ss_part, 2 * 1024)))
+        return rc;
+    S.lin = dl;
+    HIPCHK(hipMemset(S.rec, 0, (size_t)nprobe * 2 * period_steps * sizeof(float)));
+    const double st[3] = {1.0, 0.0, 0.0};
+    HIPCHK(hipMemcpy(S.state, st, sizeof(st), hipMemcpyHostToDevice));
+    h->ss = S;
+    h->has_ss = true;
+    return 0;
+}
+
+int ems_steady_state(ems_engine* h, double* last_max_diff, unsigned* evaluations, unsigned* ts, double* diff,
+                     unsigned cap)
+{
+    if (!h) return fail(EMS_E_ARG, "null handle");
+    if (!h->has_ss) return fail(EMS_E_STATE, "no steady-state detection");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    double st[3];
+    HIPCHK(hipMemcpy(st, h->ss.state, sizeof(st), hipMemcpyDeviceToHost));
+    if (last_max_diff) *last_max_diff = st[0];
+    const unsigned n = (unsigned)st[2];
+    if (evaluations) *evaluations = n;
+    const unsigned m = std::min(std::min(n, h->ss.cap), cap);
+    if (ts && m) HIPCHK(hipMemcpy(ts, h->ss.hist_ts, m * sizeof(unsigned), hipMemcpyDeviceToHost));
+    if (diff && m) HIPCHK(hipMemcpy(diff, h->ss.hist_diff, m * sizeof(double), hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -2389,7 +2527,18 @@ static void launch_step(ems_engine* h, hipEvent_t* ev)
         const auto& m = h->mur[k];
         hipLaunchKernelGGL(k_mur_post, dim3((m.na * m.nb + T - 1) / T), dim3(T), 0, h->stream, h->V, m, h->numts_dev);
     }
-    // -- apply hooks: lumped RLC, Lorentz, Mur (reverse creation order), excitation
+    // -- apply hooks: steady-state record (priority +2e6), lumped RLC, Lorentz,
+    //    Mur (reverse creation order), excitation
+    if (h->has_ss) {
+        const SteadyDev& S = h->ss;
+        hipLaunchKernelGGL(k_ss_record, dim3((S.n + T - 1) / T), dim3(T), 0, h->stream, h->V, S, h->numts_dev);
+        if (h->numTS % S.p == 0 && h->numTS >= 2 * S.p) {
+            const unsigned nb = 1024;
+            hipLaunchKernelGGL(k_energy, dim3(nb), dim3(kBlock), 0, h->stream, h->V, h->I, h->cs, h->N[0], h->N[1],
+                               h->N[2], h->ss_part);
+            hipLaunchKernelGGL(k_ss_eval, dim3(1), dim3(64), 0, h->stream, S, h->ss_part, nb, h->numts_dev);
+        }
+    }
     for (const auto& r : h->rlc)
         hipLaunchKernelGGL(k_rlc_apply, dim3((r.count + T - 1) / T), dim3(T), 0, h->stream, h->V, r, h->numts_dev);
     for (size_t k = 0; k < h->lor.size(); k += 2)
@@ -2791,7 +2940,7 @@ int ems_batch_create(ems_batch** out, ems_engine* const* engines, unsigned count
         if (e->cmp != e0->cmp) return fail(EMS_E_ARG, "batch engines must share one coefficient layout");
         const int pm = e->pmlV.n ? e->pml_mode : kNoPml, pm0 = e0->pmlV.n ? e0->pml_mode : kNoPml;
         if (pm != pm0) return fail(EMS_E_ARG, "batch engines must share one UPML layout (all with or all without)");
-        if (!e->mur.empty() || !e->lor.empty() || !e->rlc.empty() || !e->dumps.empty())
+        if (!e->mur.empty() || !e->lor.empty() || !e->rlc.empty() || !e->dumps.empty() || e->has_ss)
             return fail(EMS_E_ARG, "batched instances support UPML, sources and probes only");
         if (e->fused) return fail(EMS_E_ARG, "batched instances run two half steps (ems_set_fused(h, 0))");
         if (e->numTS != e0->numTS) return fail(EMS_E_STATE, "batch engines must be at the same timestep");

@@ -57,10 +57,12 @@ def expand_sse_compressed(N, op_index, vv4, vi4, ii4, iv4):
 
 
 class EngineHIP:
-    def __init__(self, op, device=0, layout="auto", sse_compressed=None, fused=None):
+    def __init__(self, op, device=0, layout="auto", sse_compressed=None, fused=None, steady_state=None):
         """op: Operator; layout: "auto" | "plain" | "compressed";
         sse_compressed: optional (op_index, vv4, vi4, ii4, iv4), the reference's
         SSE-compressed form of the same operator, used instead of op's arrays;
+        steady_state: optional (period_steps, dirs, positions) of a
+        steady-state detection (Engine_Ext_SteadyState);
         fused: None (automatic), False (two half steps) or True (single-sweep
         step required)."""
         L = lib()
@@ -88,6 +90,8 @@ class EngineHIP:
         self._keep = []
         if fused is not None:
             check(L.ems_set_fused(h, 1 if fused else 0))
+        if steady_state is not None:
+            self.set_steady_state(*steady_state)
         try:
             self._extensions()
         except Exception:
@@ -170,6 +174,23 @@ class EngineHIP:
         check(lib().ems_get_layout(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
         names = {0: None, 1: "plain", 2: "compressed"}
         return names[a.value], b.value, names[c.value], d.value
+
+    def set_steady_state(self, period_steps, dirs, positions, history=4096):
+        """Engine_Ext_SteadyState: period in timesteps, E probes (dir, (x, y, z))."""
+        d = np.ascontiguousarray(dirs, np.int32)
+        pos = np.ascontiguousarray(np.asarray(positions, np.uint32).T)
+        self._keep += [d, pos]
+        check(lib().ems_set_steady_state(self.h, int(period_steps), len(d), ptr(d, C.c_int), ptr(pos, C.c_uint),
+                                         int(history)))
+
+    def steady_state(self, cap=4096):
+        """(GetLastDiff, [(timestep, diff) of every evaluation])"""
+        last, n = C.c_double(), C.c_uint()
+        ts = np.zeros(cap, np.uint32)
+        df = np.zeros(cap)
+        check(lib().ems_steady_state(self.h, C.byref(last), C.byref(n), ptr(ts, C.c_uint), ptr(df, C.c_double), cap))
+        m = min(n.value, cap)
+        return last.value, list(zip(ts[:m].tolist(), df[:m].tolist()))
 
     def fused(self):
         """(True if the single-sweep step is active, z planes per workgroup)"""

@@ -2,12 +2,18 @@
 
 ``Simulation(scene).run()`` builds the operator (Operator), creates the
 engine with its extensions (EngineHIP), registers the probes and dumps, and
-advances NrTS steps with the end criterion disabled, sampling at numTS = 0
-and after every step that a probe is due (openems.cpp:1388-1494).
+advances NrTS steps, sampling at numTS = 0 and after every step that a probe
+is due (openems.cpp:1388-1494).  For a periodic excitation it sets up the
+steady-state detection of openems.cpp:1205-1234; with
+scene["end_criteria"] > 0 the run stops once its last difference falls to the
+criterion (checked every period, where the reference checks every few
+seconds of wall time).
 """
 from __future__ import annotations
 
 import time
+
+import numpy as np
 
 from .engine import EngineHIP
 from .operator import Operator
@@ -21,7 +27,12 @@ class Simulation:
         t0 = time.perf_counter()
         self.op = Operator(scene, nthreads=nthreads)
         self.setup_s = time.perf_counter() - t0
-        self.eng = EngineHIP(self.op, device=device, layout=layout, fused=fused)
+        self.ss_period = 0
+        ss = None
+        if self.op.period > 0 and scene.get("steady_state", True):
+            ss = steady_state_probes(self.op)
+            self.ss_period = ss[0]
+        self.eng = EngineHIP(self.op, device=device, layout=layout, fused=fused, steady_state=ss)
         os_ = int(scene.get("oversampling", 4))
         self.probes = [Probe(self.op, p, self.nrts, os_) for p in scene.get("probes", [])]
         self.dumps = [FieldDumpFD(self.op, d, self.nrts) for d in scene.get("dumps", [])]
@@ -32,11 +43,16 @@ class Simulation:
 
     def run(self, nsteps=None, chunk=0):
         n = self.nrts if nsteps is None else int(nsteps)
+        end = float(self.scene.get("end_criteria", 0.0))
+        if end > 0 and self.ss_period:
+            chunk = self.ss_period if chunk <= 0 else min(chunk, self.ss_period)
         done = 0
         while done < n:
             k = n - done if chunk <= 0 else min(chunk, n - done)
             self.eng.IterateTS(k)
             done += k
+            if end > 0 and self.ss_period and self.eng.steady_state(1)[0] <= end:
+                break
         return self
 
     def results(self):
@@ -51,3 +67,29 @@ class Simulation:
     def close(self):
         self.eng.close()
         self.op.close()
+
+
+def steady_state_probes(op):
+    """Steady-state detection of openems.cpp:1205-1234: period
+    m_TS_period = round(T / dT) (Operator_Ext_SteadyState::BuildExtension,
+    operator_ext_steadystate.cpp:50-56) and E probes in all three directions
+    at the mesh centre, then, for each axis n, at the centre with pos[n]
+    multiplied by 1/4 and by 3/4 -- integer divisions, so pos[n] = 0 both
+    times (21 probes)."""
+    period = int(round(op.period / op.dT))
+    dirs, pos = [], []
+    centre = [op.N[k] // 2 for k in range(3)]
+    for d in range(3):
+        dirs.append(d)
+        pos.append(list(centre))
+    for n in range(3):
+        p = list(centre)
+        p[n] *= 1 // 4
+        for d in range(3):
+            dirs.append(d)
+            pos.append(list(p))
+        p[n] *= 3 // 4
+        for d in range(3):
+            dirs.append(d)
+            pos.append(list(p))
+    return period, dirs, pos

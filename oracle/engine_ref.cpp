@@ -380,6 +380,66 @@ struct LorentzExt : ExtT<T, Eng> {
     void applyI(Eng& e, unsigned tid) override { if (!tid) apply<true>(e); }
 };
 
+// --- steady-state detection (Engine_Ext_SteadyState::Apply2Voltages,
+// FDTD/extensions/engine_ext_steadystate.cpp:50-107; priority +2e6, so its
+// Apply2Voltages runs before every other apply hook).  The energy is
+// Engine_Interface_FDTD::CalcFastEnergy (engine_interface_fdtd.cpp:302-347);
+// abs() of the relative energy change is taken as fabs.
+double fast_energy(const EngineBase& e);
+
+template <typename T, typename Eng>
+struct SteadyExt : ExtT<T, Eng> {
+    unsigned p = 0;                                // m_TS_period
+    std::vector<int> dir;
+    std::vector<unsigned> pos[3];
+    std::vector<std::vector<double>> rec;          // m_E_records, [probe][2p]
+    double last_max_diff = 1, last_total_energy = 0;
+    std::vector<unsigned> hist_ts;                 // timestep of every evaluation
+    std::vector<double> hist_diff;                 // m_last_max_diff after it
+    void applyV(Eng& e, unsigned tid) override
+    {
+        if (tid) return;
+        const unsigned TS = e.numTS;
+        const unsigned rel_pos = TS % (2 * p);
+        for (size_t n = 0; n < rec.size(); ++n) rec[n][rel_pos] = e.getV(dir[n], pos[0][n], pos[1][n], pos[2][n]);
+        if ((TS % p == 0) && (TS >= 2 * p)) {
+            bool no_valid = true;
+            last_max_diff = 0;
+            const double curr_total_energy = fast_energy(e);
+            if (last_total_energy > 0) {
+                last_max_diff = std::fabs(curr_total_energy - last_total_energy) / last_total_energy;
+                no_valid = false;
+            }
+            last_total_energy = curr_total_energy;
+            unsigned old_pos = 0, new_pos = p;
+            if (rel_pos <= p) {
+                new_pos = 0;
+                old_pos = p;
+            }
+            std::vector<double> curr_pow(rec.size()), diff_pow(rec.size());
+            double max_pow = 0;
+            for (size_t n = 0; n < rec.size(); ++n) {
+                const double* buf = rec[n].data();
+                curr_pow[n] = 0;
+                diff_pow[n] = 0;
+                for (unsigned nt = 0; nt < p; ++nt) {
+                    curr_pow[n] += buf[nt + new_pos] * buf[nt + new_pos];
+                    diff_pow[n] += (buf[nt + old_pos] - buf[nt + new_pos]) * (buf[nt + old_pos] - buf[nt + new_pos]);
+                }
+                max_pow = std::max(max_pow, curr_pow[n]);
+            }
+            for (size_t n = 0; n < rec.size(); ++n)
+                if (curr_pow[n] > max_pow * 1e-2) {
+                    last_max_diff = std::max(last_max_diff, diff_pow[n] / curr_pow[n]);
+                    no_valid = false;
+                }
+            if (no_valid || last_max_diff > 1) last_max_diff = 1;
+            hist_ts.push_back(TS);
+            hist_diff.push_back(last_max_diff);
+        }
+    }
+};
+
 // ---------------------------------------------------------------------------
 // extension list ordering (Engine::SortExtensionByPriority, FDTD/engine.cpp:87-98)
 // ---------------------------------------------------------------------------
@@ -778,6 +838,7 @@ struct Handle {
     SSEEngine* se = nullptr;
     unsigned nthreads = 1;
     bool inited = false;
+    void* ss = nullptr;   // steady-state extension (owned by the engine's extension list)
 };
 
 template <typename Eng, typename T>
@@ -1070,11 +1131,49 @@ void ref_get_ctab(void* hh, int which, int n, float* out)
         for (int l = 0; l < 4; ++l) out[4 * u + l] = v[u].f[l];
 }
 
-// CalcFastEnergy (engine_interface_fdtd.cpp:302-347): sum over [0,N-1)^3 in double
-double ref_energy(void* hh)
+// steady-state detection with period p (timesteps) and probes dir[n], pos[3][n]
+void ref_add_steady_state(void* hh, unsigned p, unsigned n, const int* dir, const unsigned* pos)
 {
     Handle* h = (Handle*)hh;
-    EngineBase* e = h->eng;
+    auto build = [&](auto* S) {
+        S->prio = 2e6;
+        S->p = p;
+        S->dir.assign(dir, dir + n);
+        for (int d = 0; d < 3; ++d) S->pos[d].assign(pos + (size_t)d * n, pos + (size_t)(d + 1) * n);
+        S->rec.assign(n, std::vector<double>(2 * (size_t)p, 0.0));
+        h->ss = S;
+    };
+    if (h->bf) { auto* S = new SteadyExt<float, BasicEngine<float>>(); build(S); h->bf->exts.push_back(S); }
+    else if (h->bd) { auto* S = new SteadyExt<double, BasicEngine<double>>(); build(S); h->bd->exts.push_back(S); }
+    else if (h->se) { auto* S = new SteadyExt<float, SSEEngine>(); build(S); h->se->exts.push_back(S); }
+}
+
+// evaluations so far: timesteps and m_last_max_diff values (up to cap)
+unsigned ref_steady_history(void* hh, unsigned* ts, double* diff, unsigned cap)
+{
+    Handle* h = (Handle*)hh;
+    if (!h->ss) return 0;
+    const std::vector<unsigned>* hts = nullptr;
+    const std::vector<double>* hd = nullptr;
+    if (h->bf) { auto* S = (SteadyExt<float, BasicEngine<float>>*)h->ss; hts = &S->hist_ts; hd = &S->hist_diff; }
+    else if (h->bd) { auto* S = (SteadyExt<double, BasicEngine<double>>*)h->ss; hts = &S->hist_ts; hd = &S->hist_diff; }
+    else { auto* S = (SteadyExt<float, SSEEngine>*)h->ss; hts = &S->hist_ts; hd = &S->hist_diff; }
+    const unsigned n = (unsigned)std::min<size_t>(cap, hts->size());
+    for (unsigned k = 0; k < n; ++k) {
+        ts[k] = (*hts)[k];
+        diff[k] = (*hd)[k];
+    }
+    return (unsigned)hts->size();
+}
+
+// CalcFastEnergy (engine_interface_fdtd.cpp:302-347): sum over [0,N-1)^3 in double
+double ref_energy(void* hh) { return fast_energy(*((Handle*)hh)->eng); }
+
+}  // extern "C"
+
+double fast_energy(const EngineBase& eb)
+{
+    const EngineBase* e = &eb;
     double E = 0, H = 0;
     for (unsigned x = 0; x + 1 < e->N[0]; ++x)
         for (unsigned y = 0; y + 1 < e->N[1]; ++y)
@@ -1086,6 +1185,8 @@ double ref_energy(void* hh)
                 }
     return 8.85418781762e-12 * E + 1.256637062e-6 * H;
 }
+
+extern "C" {
 
 void ref_destroy(void* hh)
 {

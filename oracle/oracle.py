@@ -70,6 +70,9 @@ def lib():
         L.ref_num_unique.restype = C.c_uint
         L.ref_get_opindex.argtypes = [vp, up]
         L.ref_get_ctab.argtypes = [vp, C.c_int, C.c_int, fp]
+        L.ref_add_steady_state.argtypes = [vp, C.c_uint, C.c_uint, C.POINTER(C.c_int), up]
+        L.ref_steady_history.argtypes = [vp, up, C.POINTER(C.c_double), C.c_uint]
+        L.ref_steady_history.restype = C.c_uint
         L.ref_energy.argtypes = [vp]
         L.ref_energy.restype = C.c_double
         L.ref_destroy.argtypes = [vp]
@@ -84,7 +87,9 @@ def _p(a, ct):
 class RefEngine:
     """One restated reference engine instance built from a RefOperator."""
 
-    def __init__(self, op: RefOperator, kind="basic", nthreads=1, coeffs64=None):
+    def __init__(self, op: RefOperator, kind="basic", nthreads=1, coeffs64=None, steady_state=None):
+        """steady_state: optional (period_steps, dirs, positions) for the
+        restated Engine_Ext_SteadyState."""
         L = lib()
         self.op = op
         self.kind = kind
@@ -124,6 +129,12 @@ class RefEngine:
             arrs = [np.ascontiguousarray(getattr(s, k), np.float32) for k in ("vv", "vvfo", "vvfn", "ii", "iifo", "iifn")]
             self._keep += [st, nm] + arrs
             L.ref_add_upml(self.h, _p(st, C.c_uint), _p(nm, C.c_uint), *[_p(a, C.c_float) for a in arrs])
+        if steady_state is not None:
+            per, dirs, pos = steady_state
+            d = np.ascontiguousarray(dirs, np.int32)
+            ps = np.ascontiguousarray(np.asarray(pos, np.uint32).T)
+            self._keep += [d, ps]
+            L.ref_add_steady_state(self.h, int(per), len(d), _p(d, C.c_int), _p(ps, C.c_uint))
         L.ref_init(self.h)
 
     @property
@@ -165,6 +176,14 @@ class RefEngine:
         out = np.zeros((self.N[0], self.N[1], nV), np.uint32)
         lib().ref_get_opindex(self.h, _p(out, C.c_uint))
         return out
+
+    def steady_history(self, cap=4096):
+        """[(timestep, last max diff)] of every steady-state evaluation"""
+        ts = np.zeros(cap, np.uint32)
+        df = np.zeros(cap)
+        n = lib().ref_steady_history(self.h, _p(ts, C.c_uint), _p(df, C.c_double), cap)
+        m = min(n, cap)
+        return list(zip(ts[:m].tolist(), df[:m].tolist()))
 
     def op_index_full(self):
         """m_Op_index in its allocated layout [Nx][Ny][Nz] (entries z >= nV unused, zero)"""

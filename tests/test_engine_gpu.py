@@ -268,3 +268,29 @@ def test_fused_step_refused_where_unavailable(gpu):
     from openems_amd._native import NativeError
     with pytest.raises(NativeError, match="fused"):
         EngineHIP(op, fused=True)
+
+
+def test_steady_state_detection_matches_oracle(gpu):
+    """Sinusoidal Mur scene: the restated Engine_Ext_SteadyState (21 E probes
+    of openems.cpp:1205-1234, period m_TS_period) gives the same evaluation
+    steps and last-difference values as the oracle's extension; the probe
+    power term is summed in the same order, the energy term agrees to
+    rounding of the energy sum's order."""
+    from openems_amd.simulation import steady_state_probes
+    scene = scenes.mur_layers(40, 36, 32, 600)
+    print("case mur_layers_40 steady state 600 steps", flush=True)
+    sim = Simulation(scene)
+    assert sim.ss_period > 0
+    sim.run()
+    last, hist = sim.eng.steady_state()
+    ss = steady_state_probes(sim.op)
+    assert len(ss[1]) == 21
+    ref = RefEngine(_ref_from_scene(scene), "basic", steady_state=ss)
+    ref.iterate(600)
+    rh = ref.steady_history()
+    assert len(hist) == len(rh) and len(rh) >= 3
+    for (t1, d1), (t2, d2) in zip(hist, rh):
+        assert t1 == t2
+        assert d1 == pytest.approx(d2, rel=1e-9, abs=1e-300)
+    assert last == hist[-1][1] and last < 1
+    sim.close()

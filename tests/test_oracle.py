@@ -202,3 +202,46 @@ def test_voltage_probe_is_line_sum(oracle_lib):
     pr2 = RefProbe(op, 0, (15, 15, 23), (15, 15, 0))          # reversed direction flips the sign
     pr2.process(60, v, None)
     assert pr2.td[-1][1][0] == -pr.td[-1][1][0]
+
+
+def test_steady_state_probe_set_and_convergence():
+    """openems.cpp:1205-1234: three centre probes, then per axis the centre
+    with pos[n] * (1/4) and * (3/4) in integer arithmetic (so pos[n] = 0,
+    twice): 21 probes.  With no excitation the probes never hold power and
+    the detection reports 1; with a sinusoid in an absorbing box the last
+    difference falls well below 1 after the transient."""
+    import numpy as np
+    from openems_amd.operator import Operator
+    from openems_amd.simulation import steady_state_probes
+    from openems_amd import scenes
+    from oracle.operator_ref import RefOperator
+    from oracle.oracle import RefEngine
+    s = scenes.mur_layers(20, 18, 16, 2000)
+    op = Operator(s)
+    per, dirs, pos = steady_state_probes(op)
+    assert per == int(round(op.period / op.dT)) and per > 0
+    assert len(dirs) == 21 and dirs[:3] == [0, 1, 2]
+    c = [n // 2 for n in op.N]
+    assert pos[:3] == [c, c, c]
+    for n in range(3):
+        q = list(c)
+        q[n] = 0
+        assert pos[3 + 6 * n: 9 + 6 * n] == [q] * 6
+    # no excitation: never valid -> 1
+    s0 = dict(s)
+    s0["excitations"] = [dict(s["excitations"][0], val=(0.0, 0.0, 0.0))]
+    ref0 = RefEngine(RefOperator(s0).build(), "basic", steady_state=(per, dirs, pos))
+    ref0.iterate(6 * per)
+    h0 = ref0.steady_history()
+    assert [t for t, _ in h0] == [k * per for k in range(2, 6)]
+    assert all(d == 1.0 for _, d in h0)
+    ref0.close()
+    ref = RefEngine(RefOperator(s).build(), "basic", steady_state=(per, dirs, pos))
+    ref.iterate(2000)
+    h = ref.steady_history()
+    assert len(h) >= 10
+    ds = [d for _, d in h]
+    assert ds[-1] < 0.1 and ds[-1] < 0.5 * max(ds)
+    assert all(b <= a for a, b in zip(ds[len(ds) // 2:], ds[len(ds) // 2 + 1:]))   # settling
+    ref.close()
+    op.close()
