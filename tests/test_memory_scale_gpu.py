@@ -2,8 +2,8 @@
 805 M cells, 12 seeded dielectric layers, Mur on every face, sinusoidal E_x
 sheet source, 16 E-point probes and a z-line voltage probe): the compressed
 layout (uint16 index, 1.6 GB) and the plain layout (38.6 GB of coefficients)
-run bit-identically -- same energy sums and probe samples -- and the fields
-stay finite and non-zero.  Parity with the oracle on the same scene at
+run bit-identically -- same energy sums and probe samples -- and the
+energy is finite and non-zero.  Parity with the oracle on the same scene at
 40 x 36 x 32 lines is covered by test_engine_gpu.py."""
 import numpy as np
 import pytest
@@ -45,6 +45,7 @@ def test_memory_scale_layouts_agree(gpu):
     op.close()
     assert np.isfinite(ec) and ec > 0
     assert ec == ep
+    # the probes sit 284 cells from the source plane: after 40 steps they
+    # still read zero, identically in both layouts
     for a, b in zip(tc, tp):
         assert a.shape == b.shape and np.array_equal(a, b)
-    assert any(np.abs(t[:, 1:]).max() > 0 for t in tc)
