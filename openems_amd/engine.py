@@ -14,6 +14,8 @@ import numpy as np
 
 from ._native import ProbeDesc, check, dp, fp, lib, ptr
 
+RLC_KEYS = ("i2v", "ilv", "vvd", "vv2", "vj1", "vj2", "ib0", "b1", "b2")
+
 # device coefficient layouts (include/ems_hip.h)
 LAYOUT_AUTO, LAYOUT_PLAIN, LAYOUT_COMPRESSED = 0, 1, 2
 LAYOUTS = {"auto": LAYOUT_AUTO, "plain": LAYOUT_PLAIN, "compressed": LAYOUT_COMPRESSED}
@@ -57,12 +59,19 @@ def expand_sse_compressed(N, op_index, vv4, vi4, ii4, iv4):
 
 
 class EngineHIP:
-    def __init__(self, op, device=0, layout="auto", sse_compressed=None, fused=None, steady_state=None):
+    def __init__(self, op, device=0, layout="auto", sse_compressed=None, fused=None, steady_state=None,
+                 lorentz=None, rlc=None):
         """op: Operator; layout: "auto" | "plain" | "compressed";
         sse_compressed: optional (op_index, vv4, vi4, ii4, iv4), the reference's
         SSE-compressed form of the same operator, used instead of op's arrays;
         steady_state: optional (period_steps, dirs, positions) of a
         steady-state detection (Engine_Ext_SteadyState);
+        lorentz: optional list of dispersion orders (Engine_Ext_LorentzMaterial
+        tables), each a dict pos [3][K], v_int, v_ext, v_lor (or None), i_int,
+        i_ext, i_lor [3][K], volt, curr (bool);
+        rlc: optional lumped RLC table (Engine_Ext_LumpedRLC), a dict pos
+        [3][K], dir [K] and the nine coefficient arrays i2v, ilv, vvd, vv2,
+        vj1, vj2, ib0, b1, b2 [K];
         fused: None (automatic), False (two half steps) or True (single-sweep
         step required)."""
         L = lib()
@@ -92,6 +101,8 @@ class EngineHIP:
             check(L.ems_set_fused(h, 1 if fused else 0))
         if steady_state is not None:
             self.set_steady_state(*steady_state)
+        self._lorentz = lorentz or []
+        self._rlc = rlc
         try:
             self._extensions()
         except Exception:
@@ -120,6 +131,23 @@ class EngineHIP:
             st = (C.c_uint * 3)(*s["start"])
             nm = (C.c_uint * 3)(*s["num"])
             check(L.ems_add_upml(self.h, st, nm, *s["_ptrs"]))
+        for o in self._lorentz:
+            pos = np.ascontiguousarray(o["pos"], np.uint32)
+            arrs = {k: (None if o.get(k) is None else np.ascontiguousarray(o[k], np.float32))
+                    for k in ("v_int", "v_ext", "v_lor", "i_int", "i_ext", "i_lor")}
+            self._keep += [pos] + [a for a in arrs.values() if a is not None]
+            fpn = lambda a: None if a is None else ptr(a, C.c_float)
+            check(L.ems_add_lorentz(self.h, pos.shape[1], ptr(pos, C.c_uint), fpn(arrs["v_int"]), fpn(arrs["v_ext"]),
+                                    fpn(arrs["v_lor"]), fpn(arrs["i_int"]), fpn(arrs["i_ext"]), fpn(arrs["i_lor"]),
+                                    int(bool(o.get("volt", True))), int(bool(o.get("curr", True)))))
+        if self._rlc is not None:
+            r = self._rlc
+            pos = np.ascontiguousarray(r["pos"], np.uint32)
+            d = np.ascontiguousarray(r["dir"], np.int32)
+            cs = [np.ascontiguousarray(r[k], np.float32) for k in RLC_KEYS]
+            self._keep += [pos, d] + cs
+            cp = (fp * 9)(*[ptr(a, C.c_float) for a in cs])
+            check(L.ems_add_lumped_rlc(self.h, pos.shape[1], ptr(pos, C.c_uint), ptr(d, C.c_int), cp))
         check(L.ems_init(self.h))
 
     # -- Engine API ------------------------------------------------------------

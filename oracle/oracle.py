@@ -71,6 +71,10 @@ def lib():
         L.ref_get_opindex.argtypes = [vp, up]
         L.ref_get_ctab.argtypes = [vp, C.c_int, C.c_int, fp]
         L.ref_add_steady_state.argtypes = [vp, C.c_uint, C.c_uint, C.POINTER(C.c_int), up]
+        pfp = C.POINTER(fp)
+        L.ref_add_lorentz.argtypes = [vp, C.c_uint, up, C.POINTER(up), pfp, pfp, pfp, pfp, pfp, pfp,
+                                      C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.ref_add_rlc.argtypes = [vp, C.c_uint, up, C.POINTER(C.c_int), pfp]
         L.ref_steady_history.argtypes = [vp, up, C.POINTER(C.c_double), C.c_uint]
         L.ref_steady_history.restype = C.c_uint
         L.ref_energy.argtypes = [vp]
@@ -87,9 +91,12 @@ def _p(a, ct):
 class RefEngine:
     """One restated reference engine instance built from a RefOperator."""
 
-    def __init__(self, op: RefOperator, kind="basic", nthreads=1, coeffs64=None, steady_state=None):
+    def __init__(self, op: RefOperator, kind="basic", nthreads=1, coeffs64=None, steady_state=None,
+                 lorentz=None, rlc=None):
         """steady_state: optional (period_steps, dirs, positions) for the
-        restated Engine_Ext_SteadyState."""
+        restated Engine_Ext_SteadyState; lorentz / rlc: ADE tables in the
+        form openems_amd.engine.EngineHIP takes them (created after Mur and
+        UPML, as the reference's setup does)."""
         L = lib()
         self.op = op
         self.kind = kind
@@ -129,6 +136,38 @@ class RefEngine:
             arrs = [np.ascontiguousarray(getattr(s, k), np.float32) for k in ("vv", "vvfo", "vvfn", "ii", "iifo", "iifn")]
             self._keep += [st, nm] + arrs
             L.ref_add_upml(self.h, _p(st, C.c_uint), _p(nm, C.c_uint), *[_p(a, C.c_float) for a in arrs])
+        if lorentz:
+            order = len(lorentz)
+            cnt = np.array([np.asarray(o["pos"]).shape[1] for o in lorentz], np.uint32)
+            keep = []
+
+            def arr(a, dt):
+                a = np.ascontiguousarray(a, dt)
+                keep.append(a)
+                return a
+            pos = [arr(o["pos"], np.uint32) for o in lorentz]
+            tabs = {}
+            for k in ("v_int", "v_ext", "v_lor", "i_int", "i_ext", "i_lor"):
+                tabs[k] = [arr(o[k] if o.get(k) is not None else np.zeros((3, c), np.float32), np.float32)
+                           for o, c in zip(lorentz, cnt)]
+            has_l = arr([int(o.get("v_lor") is not None) for o in lorentz], np.int32)
+            has_v = arr([int(bool(o.get("volt", True))) for o in lorentz], np.int32)
+            has_c = arr([int(bool(o.get("curr", True))) for o in lorentz], np.int32)
+            self._keep += keep + [cnt]
+            pp = (C.POINTER(C.c_uint) * order)(*[_p(a, C.c_uint) for a in pos])
+            fpp = {k: (C.POINTER(C.c_float) * order)(*[_p(a, C.c_float) for a in v]) for k, v in tabs.items()}
+            self._keep += [pp] + list(fpp.values())
+            L.ref_add_lorentz(self.h, order, _p(cnt, C.c_uint), pp, fpp["v_int"], fpp["v_ext"], fpp["v_lor"],
+                              fpp["i_int"], fpp["i_ext"], fpp["i_lor"], _p(has_l, C.c_int), _p(has_v, C.c_int),
+                              _p(has_c, C.c_int))
+        if rlc is not None:
+            keys = ("i2v", "ilv", "vvd", "vv2", "vj1", "vj2", "ib0", "b1", "b2")
+            pos = np.ascontiguousarray(rlc["pos"], np.uint32)
+            d = np.ascontiguousarray(rlc["dir"], np.int32)
+            cs = [np.ascontiguousarray(rlc[k], np.float32) for k in keys]
+            cp = (C.POINTER(C.c_float) * 9)(*[_p(a, C.c_float) for a in cs])
+            self._keep += [pos, d, cp] + cs
+            L.ref_add_rlc(self.h, pos.shape[1], _p(pos, C.c_uint), _p(d, C.c_int), cp)
         if steady_state is not None:
             per, dirs, pos = steady_state
             d = np.ascontiguousarray(dirs, np.int32)

@@ -294,3 +294,53 @@ def test_steady_state_detection_matches_oracle(gpu):
         assert d1 == pytest.approx(d2, rel=1e-9, abs=1e-300)
     assert last == hist[-1][1] and last < 1
     sim.close()
+
+
+def _ade_tables(op, seed=21):
+    """Synthetic Lorentz / Drude orders and lumped RLC elements (stable,
+    small coefficients) on random interior cells."""
+    rng = np.random.default_rng(seed)
+    N = op.N
+
+    def cells(k):
+        return np.stack([rng.integers(3, N[a] - 3, size=k) for a in range(3)]).astype(np.uint32)
+
+    def tab(k, lo, hi):
+        return rng.uniform(lo, hi, size=(3, k)).astype(np.float32)
+    k0, k1 = 40, 25
+    lorentz = [
+        {"pos": cells(k0), "v_int": tab(k0, 0.9, 0.99), "v_ext": tab(k0, 1e-3, 1e-2), "v_lor": tab(k0, 1e-3, 1e-2),
+         "i_int": tab(k0, 0.9, 0.99), "i_ext": tab(k0, 1e-3, 1e-2), "i_lor": tab(k0, 1e-3, 1e-2),
+         "volt": True, "curr": True},
+        {"pos": cells(k1), "v_int": tab(k1, 0.8, 0.95), "v_ext": tab(k1, 1e-3, 5e-3), "v_lor": None,
+         "i_int": tab(k1, 0.8, 0.95), "i_ext": tab(k1, 1e-3, 5e-3), "i_lor": None, "volt": True, "curr": False},
+    ]
+    kr = 6
+    rlc = {"pos": cells(kr), "dir": rng.integers(0, 3, size=kr).astype(np.int32)}
+    for name, lo, hi in (("i2v", 0.0, 0.1), ("ilv", 0.0, 0.1), ("vvd", 0.5, 0.9), ("vv2", -0.05, 0.05),
+                         ("vj1", -0.05, 0.05), ("vj2", -0.05, 0.05), ("ib0", 0.0, 0.1), ("b1", -0.1, 0.1),
+                         ("b2", -0.1, 0.1)):
+        rlc[name] = rng.uniform(lo, hi, size=kr).astype(np.float32)
+    return lorentz, rlc
+
+
+@pytest.mark.parametrize("layout", ["plain", "compressed"])
+def test_lorentz_and_lumped_rlc_kernels(gpu, layout):
+    """Engine_Ext_LorentzMaterial (a Lorentz and a Drude order, the latter on
+    voltages only) and Engine_Ext_LumpedRLC with synthetic tables, created
+    after UPML as in the reference's setup: fields bit-identical to the
+    oracle after 150 steps."""
+    scene = scenes.upml_bricks(40, 150, nbricks=6, dump=False)
+    print("case upml_bricks_40 + Lorentz/Drude + lumped RLC, layout", layout, flush=True)
+    op = Operator(scene)
+    lorentz, rlc = _ade_tables(op)
+    eng = EngineHIP(op, layout=layout, lorentz=lorentz, rlc=rlc)
+    assert not eng.fused()[0]
+    eng.IterateTS(150)
+    ref = RefEngine(_ref_from_scene(scene), "basic", lorentz=lorentz, rlc=rlc)
+    ref.iterate(150)
+    v, c, rv, rc = _fields_equal(eng, ref)
+    assert np.max(np.abs(rv)) > 0
+    assert np.array_equal(v, rv), f"voltage mismatches {np.count_nonzero(v != rv)}"
+    assert np.array_equal(c, rc), f"current mismatches {np.count_nonzero(c != rc)}"
+    eng.close()
