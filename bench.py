@@ -19,7 +19,7 @@ update of one half step, by coefficient layout:
                from a table of unique tuples that stays in cache);
 plus, per UPML position, 15 floats = 60 B (plain UPML: 3 flux read + 3 flux
 write + 9 coefficients) or 6 floats + one uint32 index = 28 B (compressed
-UPML).  The fused single-sweep step (k_step) moves a whole timestep per
+UPML).  The single-sweep steps (k_step, k_step2) move a whole timestep per
 launch: per cell 12 floats (read and write V and I) + 12 coefficient floats
 (plain) or one uint16 index (compressed) = 96 B or 50 B; per UPML position
 the two half steps' UPML bytes (120 B plain, 52 B compressed).  `plain_ceiling_frac` compares the cell-update rate with the HBM
@@ -309,7 +309,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--layout", default="auto", choices=["auto", "plain", "compressed"])
-    ap.add_argument("--step", default="auto", choices=["auto", "fused", "halves"])
+    ap.add_argument("--step", default="auto", choices=["auto", "halves", "lds", "columns"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--batched-count", type=int, default=64)
@@ -337,7 +337,7 @@ def main():
     scene = scenes.upml_bricks(n, 10000)
     t0 = time.perf_counter()
     sim = Simulation(scene, layout=args.layout,
-                     fused={"auto": None, "fused": True, "halves": False}[args.step])
+                     fused=None if args.step == "auto" else args.step)
     setup_s = time.perf_counter() - t0
     eng = sim.eng
     op = sim.op
@@ -366,13 +366,14 @@ def main():
         step_bytes = 2 * bytes_per_launch
     plain_step_bytes = PLAIN_CELL_STEP_BYTES * (cells + npml)
     if fused:
-        kname, kms = "k_step", volt_ms
+        kname, kms = ("k_step", "k_step2")[fused - 1], volt_ms
     else:
         kname, kms = ("k_volt", volt_ms) if volt_ms >= curr_ms else ("k_curr", curr_ms)
     achieved = bytes_per_launch / (kms * 1e-3) / 1e9
     layout = f"{lay} coefficients ({n_unique} unique) + fused {play} UPML ({n_unique_pml} unique)" \
         if lay == "compressed" or play == "compressed" else "plain fp32 coefficients + fused UPML"
-    layout += f", {'single-sweep fused step' if fused else 'two half steps'} (z depth {zdepth})"
+    form = ("two half steps", "single sweep, LDS planes (k_step)", "single sweep, independent columns (k_step2)")[fused]
+    layout += f", {form} (z depth {zdepth})"
     traffic = traffic_from_profile(kname, layout, n)
     plain_ceiling = HBM_PEAK_GBS * 1e9 / (plain_step_bytes / cells) / 1e6     # MCells/s
     out = {

@@ -27,6 +27,7 @@
  *                                                    FDTD/extensions/engine_ext_lorentzmaterial.cpp:79-173
  *   ems_add_lumped_rlc    Engine_Ext_LumpedRLC                            FDTD/extensions/engine_ext_lumpedRLC.cpp:83-147
  *   ems_iterate           Engine::IterateTS                              FDTD/engine.cpp:267-286
+ *   ems_set_graph         Engine::IterateTS loop replayed as HIP graphs per timestep chunk
  *   ems_set_fused         Engine::IterateTS step form (single sweep or two half steps)
  *   ems_get_fused                                                        FDTD/engine.cpp:267-286
  *   ems_get_numts         Engine::GetNumberOfTimesteps                   FDTD/engine.h:74
@@ -133,18 +134,22 @@ int ems_set_steady_state(ems_engine* h, unsigned period_steps, unsigned nprobe, 
 int ems_steady_state(ems_engine* h, double* last_max_diff, unsigned* evaluations, unsigned* ts, double* diff,
                      unsigned cap);
 
-/* Step form, before ems_init.  The fused form advances a whole timestep in
- * one z-sweep over double-buffered fields (voltage update, voltage sources,
+/* Step form, before ems_init.  A single-sweep form advances a whole timestep
+ * in one launch over double-buffered fields (voltage update, voltage sources,
  * current update, current sources, UPML in both halves), bit-identical to the
- * two-half-step form with about 40 % fewer algorithmic bytes.  It needs every active
- * extension to have a fused form: Mur, Lorentz / Drude and lumped RLC keep the
- * two-half-step form, as do current sources on the last line of a direction
- * and more than 64 sources in one 64 x 16 column tile.
- * mode: -1 automatic (default: two half steps, the faster form on MI355X
- * so far), 0 two half steps, 1 fused (ems_init then fails with EMS_E_ARG
- * when the engine has no fused form). */
+ * two-half-step form with about 40 % fewer algorithmic bytes.  Form 1 stages
+ * V^{n+1} planes of 64 x 16 column tiles in LDS; form 2 sweeps independent
+ * columns, recomputing the neighbour voltages each current update needs.
+ * Both need every active extension to have a single-sweep form: Mur,
+ * Lorentz / Drude, lumped RLC and steady-state detection keep two half steps,
+ * as do current sources on the last line of a direction and more than 64
+ * sources in one tile (form 1) or column (form 2).
+ * mode: -1 automatic (two half steps, the faster form on UPML meshes),
+ * 0 two half steps, 1 LDS sweep, 2 column sweep (ems_init then fails with
+ * EMS_E_ARG when the engine has no single-sweep form). */
 int ems_set_fused(ems_engine* h, int mode);
-/* fused: 1 when the fused form is active; zdepth: z planes per workgroup */
+/* fused: active form (0 two half steps, 1 LDS sweep, 2 column sweep);
+ * zdepth: z planes per workgroup */
 int ems_get_fused(const ems_engine* h, int* fused, unsigned* zdepth);
 
 /* Active layouts and table sizes (known after ems_init for the UPML part):
@@ -194,6 +199,21 @@ int ems_init(ems_engine* h);
 int ems_iterate(ems_engine* h, unsigned nsteps);
 unsigned ems_get_numts(const ems_engine* h);
 int ems_set_async(ems_engine* h, int async);
+
+/* Timestep chunks as HIP graphs (the IterateTS loop, FDTD/engine.cpp:267-286,
+ * with its ProcessingArray sampling).  ems_iterate replays every full chunk of
+ * chunk_steps steps from a graph captured from the direct launches the first
+ * time that chunk's launch pattern occurs (probe, dump and steady-state
+ * decisions of its steps, buffer parity of the single-sweep forms).  The
+ * chunk is rounded up to a multiple of the pattern's period (the sampling
+ * intervals, the steady-state period) when that period is at most 1024
+ * steps, so chunks repeat; at most 16 patterns are kept, further ones launch
+ * directly.  Remaining steps and profiled steps launch directly.  Default 32
+ * steps; 0 launches every step directly.  Results are identical either way.
+ * Adding probes or dumps drops the captured graphs. */
+int ems_set_graph(ems_engine* h, unsigned chunk_steps);
+/* chunk length, graphs captured and chunks replayed so far */
+int ems_get_graph(const ems_engine* h, unsigned* chunk_steps, unsigned* captures, unsigned* replays);
 int ems_sync(ems_engine* h);
 
 /* whole-field transfers in the reference N-I-J-K layout, [3][Nx][Ny][Nz] */

@@ -91,6 +91,8 @@ ENGINE_API = [
     ("ems_iterate", C.c_int, [vp, C.c_uint]),
     ("ems_get_numts", C.c_uint, [vp]),
     ("ems_set_async", C.c_int, [vp, C.c_int]),
+    ("ems_set_graph", C.c_int, [vp, C.c_uint]),
+    ("ems_get_graph", C.c_int, [vp, up, up, up]),
     ("ems_sync", C.c_int, [vp]),
     ("ems_get_volt", C.c_int, [vp, fp]),
     ("ems_get_curr", C.c_int, [vp, fp]),

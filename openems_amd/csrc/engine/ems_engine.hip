@@ -21,6 +21,7 @@
 #include <string>
 #include <unordered_map>
 #include <thread>
+#include <numeric>
 #include <vector>
 
 #include "ems_compress.h"
@@ -47,6 +48,7 @@ int fail(int code, const std::string& msg)
 constexpr unsigned kBlock = 256;      // threads per block of the stencil kernels (4 wave64)
 constexpr unsigned kXCD = 8;          // accelerator complex dies on MI355X
 constexpr unsigned kMaxSlabs = 6;     // UPML slabs (Operator_Ext_UPML::Create_UPML creates at most 6)
+constexpr size_t kMaxGraphs = 16;     // distinct timestep-chunk graphs kept per engine
 
 struct GridDesc {
     unsigned Nx, Ny, Nz;
@@ -458,7 +460,7 @@ struct StepArgs {
     CoefArgs kV, kI;
     PmlArgs pV, pI;
     ExcArgs eV, eI;
-    unsigned* numts;               // [0] timestep counter, [1] finished-workgroup ticket
+    const unsigned* numts;         // [0] timestep counter (advanced by k_tick after the step)
 };
 
 template <int PM, bool CC>
@@ -657,17 +659,339 @@ __global__ __launch_bounds__(kBlock) void k_step(const StepArgs a)
             if (x < g.Nx && y < g.Ny) pml_hooks_cell<PM>(a, x, y, g.Nz - 1, x + (unsigned long long)g.Nx * y + zq, iown[r]);
         }
     }
-    // the last workgroup to finish advances the timestep counter; every
-    // workgroup has read it before it takes a ticket
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned done = atomicAdd(&a.numts[1], 1u);
-        if (done == gridDim.x - 1) {
-            atomicExch(&a.numts[1], 0u);
-            atomicAdd(&a.numts[0], 1u);
+}
+
+// ---------------------------------------------------------------------------
+// Fused single-sweep step without LDS or barriers (k_step2).  The same step as
+// k_step (state n in V0, I0 and the flux inputs; state n+1 into V1, I1 and the
+// flux outputs), on the two-half-step tiles: a workgroup owns kBlock
+// consecutive (x, y) columns and a z chunk, and each thread sweeps its own
+// column.  The current update at (x, y, z-1) needs V^{n+1} of its column at
+// planes z-1 and z and of the columns (x+1, y) (components z, y) and
+// (x, y+1) (components z, x) at plane z-1.  The thread recomputes those four
+// neighbour components itself, with the same arithmetic as their owners
+// (sources included, flux not stored), and carries plane z-1 in registers,
+// so no value crosses threads.  Each plane first issues every load (clamped
+// to valid addresses where a value is not needed, so the body has no
+// branches before its arithmetic), then computes, then stores.  Sources come
+// from per-column lists: the entries of column q are beg[q] .. beg[q+1], in
+// list order.
+// ---------------------------------------------------------------------------
+struct Step2Args {
+    GridDesc g;
+    CoefArgs kV, kI;
+    PmlArgs pV, pI;                // one slab table and index array for both halves
+    ExcArgs eV, eI;                // beg: [plane + 1] per-column ranges
+    const unsigned* numts;         // [0] timestep counter (advanced by k_tick after the step)
+};
+
+// slabs whose (x, y) range holds column (x, y), one bit per slab
+__device__ __forceinline__ unsigned pml_colmask(const PmlArgs& P, int x, int y)
+{
+    unsigned m = 0;
+#pragma unroll
+    for (int s = 0; s < (int)kMaxSlabs; ++s) {
+        const SlabTab& S = P.tab[s];
+        if (s < P.n && x >= S.x0 && x <= S.x1 && y >= S.y0 && y <= S.y1) m |= 1u << s;
+    }
+    return m;
+}
+
+// UPML position of (x, y, z) for a column whose slab mask is m; 0 (a valid
+// position) when the cell lies in no slab
+__device__ __forceinline__ bool pml_at_col(const PmlArgs& P, unsigned m, int x, int y, int z, unsigned& li)
+{
+    int sel = -1;
+#pragma unroll
+    for (int s = 0; s < (int)kMaxSlabs; ++s) {
+        const SlabTab& S = P.tab[s];
+        if ((m >> s & 1u) && z >= S.z0 && z <= S.z1) sel = s;
+    }
+    int bx = 0, by = 0, bz = 0;
+    unsigned nx = 0, ny = 0, off = 0;
+#pragma unroll
+    for (int s = 0; s < (int)kMaxSlabs; ++s) {
+        if (sel == s) {
+            const SlabTab& S = P.tab[s];
+            bx = S.x0; by = S.y0; bz = S.z0;
+            nx = S.nx; ny = S.ny; off = (unsigned)S.off;
         }
     }
+    const unsigned l = off + (unsigned)(x - bx) + nx * ((unsigned)(y - by) + ny * (unsigned)(z - bz));
+    li = sel >= 0 ? l : 0u;
+    return sel >= 0;
+}
+
+// main coefficients (c, v) of component n at cell p: plain arrays or table
+// entry u, whose float2 n is {c_n, v_n}
+template <bool CC>
+__device__ __forceinline__ float2 coef_n(const CoefArgs& k, unsigned long long cs, unsigned long long p, unsigned u, int n)
+{
+    if (CC) return reinterpret_cast<const float2*>(k.tab)[4 * u + n];
+    return make_float2(k.cc[n * cs + p], k.cv[n * cs + p]);
+}
+
+// UPML coefficients (c0, cfo, cfn) of component n at position li: plain
+// arrays or the 12 floats of table entry w
+template <int PM>
+__device__ __forceinline__ void pml_coef_n(const PmlArgs& P, unsigned li, unsigned w, int n, float& k0, float& ko,
+                                           float& kn)
+{
+    if (PM == kPmlPlain) {
+        k0 = P.coef[n * P.npos + li];
+        ko = P.coef[(3 + n) * P.npos + li];
+        kn = P.coef[(6 + n) * P.npos + li];
+    } else {
+        const float* t = reinterpret_cast<const float*>(P.ptab) + 12 * (unsigned long long)w;
+        k0 = t[n];
+        ko = t[3 + n];
+        kn = t[6 + n];
+    }
+}
+
+// one component of the update (cell_math for a single component): returns
+// the new value, f receives the new flux
+template <int PM>
+__device__ __forceinline__ float comp_math(float x0, float2 c, float curl, bool pml, float fl, float k0, float ko, float kn,
+                                           float& f)
+{
+    const float xin = (PM != kNoPml && pml) ? fl : x0;
+    float g = xin * c.x;
+    g = g + c.y * curl;
+    f = g;
+    if (PM == kNoPml) return g;
+    const float t = k0 * x0 - ko * fl;
+    const float o = t + kn * g;
+    return pml ? o : g;
+}
+
+// sources of one column at plane z and component n, added in list order
+// (Engine_Ext_Excitation::Apply2Voltages / Apply2Current)
+__device__ __forceinline__ float col_sources(const ExcArgs& e, unsigned b, unsigned end, unsigned z, unsigned n, int ts,
+                                             float v)
+{
+    for (unsigned k = b; k < end; ++k) {
+        const ExcEnt E = e.ent[k];
+        if (E.z == z && E.dir == n) v = v + E.amp * e.sig[exc_sample(ts, E.delay, e.period, e.length)];
+    }
+    return v;
+}
+
+template <int PM, bool CC>
+__global__ __launch_bounds__(kBlock) void k_step2(const Step2Args a, const float* __restrict__ V0,
+                                                  const float* __restrict__ I0, float* __restrict__ V1,
+                                                  float* __restrict__ I1, const float* __restrict__ fVin,
+                                                  float* __restrict__ fVout, const float* __restrict__ fIin,
+                                                  float* __restrict__ fIout)
+{
+    const GridDesc& g = a.g;
+    const unsigned long long cs = g.cs;
+    const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
+    const unsigned qt = t % g.qtiles, zc = t / g.qtiles;
+    unsigned x, y;
+    const bool valid = tile_column(g, qt, x, y);
+    const int ts = (int)a.numts[0];
+    if (valid) {
+        const unsigned Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+        const unsigned long long plane = g.plane;
+        const unsigned long long npV = a.pV.npos, npI = a.pI.npos;
+        const unsigned z0 = zc * g.zdepth;
+        const unsigned z1 = min(z0 + g.zdepth, Nz);
+        const unsigned zlast = min(z1, Nz - 1);
+        const unsigned q = x + Nx * y;
+        const bool upd = x + 1 < Nx && y + 1 < Ny;        // the current update runs in this column
+        const unsigned long long xm = x ? 1ull : 0ull, ym = y ? (unsigned long long)Nx : 0ull;
+        // neighbour columns (the column itself where no current update runs)
+        const unsigned long long dr = upd ? 1ull : 0ull, du = upd ? (unsigned long long)Nx : 0ull;
+        // source ranges of the own column and of the two neighbour columns
+        const unsigned evb = a.eV.beg[q], eve = a.eV.beg[q + 1];
+        const unsigned eib = a.eI.beg[q], eie = a.eI.beg[q + 1];
+        const unsigned erb = a.eV.beg[q + dr], ere = upd ? a.eV.beg[q + 2] : erb;
+        const unsigned eub = a.eV.beg[q + du], eue = upd ? a.eV.beg[q + Nx + 1] : eub;
+        unsigned m_own = 0, m_r = 0, m_u = 0;
+        if (PM != kNoPml) {
+            m_own = pml_colmask(a.pV, (int)x, (int)y);
+            if (upd) {
+                m_r = pml_colmask(a.pV, (int)x + 1, (int)y);
+                m_u = pml_colmask(a.pV, (int)x, (int)y + 1);
+            }
+        }
+        // I^n at z-1 (the cell itself at z = 0): own x, y; (x+1, y) x; (x, y+1) y
+        const unsigned long long pzm = z0 ? (unsigned long long)(z0 - 1) * plane + q : (unsigned long long)q;
+        float ix_zm = I0[pzm], iy_zm = I0[cs + pzm];
+        float ixr_zm = I0[pzm + dr], iyu_zm = I0[cs + pzm + du];
+        // plane z-1 carried for the current update: own V^{n+1} and I^n, the
+        // neighbour components, the coefficient index, UPML position and index
+        float vm0 = 0.0f, vm1 = 0.0f, vm2 = 0.0f, vrz = 0.0f, vry = 0.0f, vuz = 0.0f, vux = 0.0f;
+        float im0 = 0.0f, im1 = 0.0f, im2 = 0.0f;
+        unsigned um = 0, lim = 0, wm = 0;
+        bool pmlm = false;
+        unsigned long long pm = q;
+        for (unsigned z = z0; z <= zlast; ++z) {
+            const unsigned long long p = (unsigned long long)z * plane + q;
+            const unsigned long long pr = p + dr, pu = p + du;
+            // ---- loads
+            const float ix = I0[p], iy = I0[cs + p], iz = I0[2 * cs + p];
+            const float iz_ym = I0[2 * cs + p - ym], ix_ym = I0[p - ym];
+            const float iz_xm = I0[2 * cs + p - xm], iy_xm = I0[cs + p - xm];
+            const float ixr = I0[pr], iyr = I0[cs + pr], izr = I0[2 * cs + pr], ixr_ym = I0[pr - ym];
+            const float ixu = I0[pu], iyu = I0[cs + pu], izu = I0[2 * cs + pu], iyu_xm = I0[cs + pu - xm];
+            const float v0x = V0[p], v0y = V0[cs + p], v0z = V0[2 * cs + p];
+            const float r0z = V0[2 * cs + pr], r0y = V0[cs + pr];
+            const float u0z = V0[2 * cs + pu], u0x = V0[pu];
+            const unsigned u = CC ? a.kV.idx[p] : 0u;
+            const unsigned ur = CC ? a.kV.idx[pr] : 0u;
+            const unsigned uu = CC ? a.kV.idx[pu] : 0u;
+            const float2 cV0 = coef_n<CC>(a.kV, cs, p, u, 0), cV1 = coef_n<CC>(a.kV, cs, p, u, 1),
+                         cV2 = coef_n<CC>(a.kV, cs, p, u, 2);
+            const float2 cR2 = coef_n<CC>(a.kV, cs, pr, ur, 2), cR1 = coef_n<CC>(a.kV, cs, pr, ur, 1);
+            const float2 cU2 = coef_n<CC>(a.kV, cs, pu, uu, 2), cU0 = coef_n<CC>(a.kV, cs, pu, uu, 0);
+            const float2 cI0 = coef_n<CC>(a.kI, cs, pm, um, 0), cI1 = coef_n<CC>(a.kI, cs, pm, um, 1),
+                         cI2 = coef_n<CC>(a.kI, cs, pm, um, 2);
+            unsigned li = 0, lr = 0, lu = 0, w = 0;
+            bool pml = false, pmr = false, pmu = false;
+            float fl[3] = {0.0f, 0.0f, 0.0f}, k0[3] = {0.0f, 0.0f, 0.0f}, ko[3] = {0.0f, 0.0f, 0.0f},
+                  kn[3] = {0.0f, 0.0f, 0.0f};
+            float flr2 = 0.0f, flr1 = 0.0f, flu2 = 0.0f, flu0 = 0.0f;
+            float kr[2][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}}, ku[2][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+            float flI[3] = {0.0f, 0.0f, 0.0f}, kI0[3] = {0.0f, 0.0f, 0.0f}, kIo[3] = {0.0f, 0.0f, 0.0f},
+                  kIn[3] = {0.0f, 0.0f, 0.0f};
+            if (PM != kNoPml) {
+                pml = pml_at_col(a.pV, m_own, (int)x, (int)y, (int)z, li);
+                pmr = pml_at_col(a.pV, m_r, (int)x + 1, (int)y, (int)z, lr);
+                pmu = pml_at_col(a.pV, m_u, (int)x, (int)y + 1, (int)z, lu);
+                w = PM == kPmlCompressed ? a.pV.pidx[li] : 0u;
+                const unsigned wr = PM == kPmlCompressed ? a.pV.pidx[lr] : 0u;
+                const unsigned wu = PM == kPmlCompressed ? a.pV.pidx[lu] : 0u;
+#pragma unroll
+                for (int n = 0; n < 3; ++n) {
+                    fl[n] = fVin[n * npV + li];
+                    pml_coef_n<PM>(a.pV, li, w, n, k0[n], ko[n], kn[n]);
+                    flI[n] = fIin[n * npI + lim];
+                    pml_coef_n<PM>(a.pI, lim, wm, n, kI0[n], kIo[n], kIn[n]);
+                }
+                flr2 = fVin[2 * npV + lr];
+                flr1 = fVin[npV + lr];
+                pml_coef_n<PM>(a.pV, lr, wr, 2, kr[0][0], kr[0][1], kr[0][2]);
+                pml_coef_n<PM>(a.pV, lr, wr, 1, kr[1][0], kr[1][1], kr[1][2]);
+                flu2 = fVin[2 * npV + lu];
+                flu0 = fVin[lu];
+                pml_coef_n<PM>(a.pV, lu, wu, 2, ku[0][0], ku[0][1], ku[0][2]);
+                pml_coef_n<PM>(a.pV, lu, wu, 0, ku[1][0], ku[1][1], ku[1][2]);
+            }
+            // ---- own V^{n+1}(x, y, z)
+            float v[3], fv[3];
+            {
+                const float c0 = iz - iz_ym - iy + iy_zm;
+                const float c1 = ix - ix_zm - iz + iz_xm;
+                const float c2 = iy - iy_xm - ix + ix_ym;
+                v[0] = comp_math<PM>(v0x, cV0, c0, pml, fl[0], k0[0], ko[0], kn[0], fv[0]);
+                v[1] = comp_math<PM>(v0y, cV1, c1, pml, fl[1], k0[1], ko[1], kn[1], fv[1]);
+                v[2] = comp_math<PM>(v0z, cV2, c2, pml, fl[2], k0[2], ko[2], kn[2], fv[2]);
+            }
+            // ---- neighbour components: (x+1, y) z and y, (x, y+1) z and x
+            float nrz, nry, nuz, nux, dummy;
+            {
+                const float c2r = iyr - iy - ixr + ixr_ym;
+                const float c1r = ixr - ixr_zm - izr + iz;
+                nrz = comp_math<PM>(r0z, cR2, c2r, pmr, flr2, kr[0][0], kr[0][1], kr[0][2], dummy);
+                nry = comp_math<PM>(r0y, cR1, c1r, pmr, flr1, kr[1][0], kr[1][1], kr[1][2], dummy);
+                const float c2u = iyu - iyu_xm - ixu + ix;
+                const float c0u = izu - iz - iyu + iyu_zm;
+                nuz = comp_math<PM>(u0z, cU2, c2u, pmu, flu2, ku[0][0], ku[0][1], ku[0][2], dummy);
+                nux = comp_math<PM>(u0x, cU0, c0u, pmu, flu0, ku[1][0], ku[1][1], ku[1][2], dummy);
+            }
+            // ---- voltage sources
+            if (evb != eve) {
+#pragma unroll
+                for (int n = 0; n < 3; ++n) v[n] = col_sources(a.eV, evb, eve, z, n, ts, v[n]);
+            }
+            if (erb != ere) {
+                nrz = col_sources(a.eV, erb, ere, z, 2, ts, nrz);
+                nry = col_sources(a.eV, erb, ere, z, 1, ts, nry);
+            }
+            if (eub != eue) {
+                nuz = col_sources(a.eV, eub, eue, z, 2, ts, nuz);
+                nux = col_sources(a.eV, eub, eue, z, 0, ts, nux);
+            }
+            // ---- I^{n+1}(x, y, z-1): main update, or the UPML hooks alone in
+            // a column the main update skips
+            float io[3], fi[3];
+            {
+                const float c0 = vm2 - vuz - vm1 + v[1];
+                const float c1 = vm0 - v[0] - vm2 + vrz;
+                const float c2 = vm1 - vry - vm0 + vux;
+                const float im[3] = {im0, im1, im2};
+                io[0] = comp_math<PM>(im[0], cI0, c0, pmlm, flI[0], kI0[0], kIo[0], kIn[0], fi[0]);
+                io[1] = comp_math<PM>(im[1], cI1, c1, pmlm, flI[1], kI0[1], kIo[1], kIn[1], fi[1]);
+                io[2] = comp_math<PM>(im[2], cI2, c2, pmlm, flI[2], kI0[2], kIo[2], kIn[2], fi[2]);
+                if (PM != kNoPml && !upd) {
+#pragma unroll
+                    for (int n = 0; n < 3; ++n) {
+                        const float tt = kI0[n] * im[n] - kIo[n] * flI[n];
+                        io[n] = tt + kIn[n] * flI[n];
+                    }
+                }
+            }
+            const bool wI = z > z0 && (upd || (PM != kNoPml && pmlm));
+            if (z > z0 && upd && eib != eie) {
+#pragma unroll
+                for (int n = 0; n < 3; ++n) io[n] = col_sources(a.eI, eib, eie, z - 1, n, ts, io[n]);
+            }
+            // ---- stores
+            if (z < z1) {
+                V1[p] = v[0];
+                V1[cs + p] = v[1];
+                V1[2 * cs + p] = v[2];
+                if (PM != kNoPml && pml) {
+                    fVout[li] = fv[0];
+                    fVout[npV + li] = fv[1];
+                    fVout[2 * npV + li] = fv[2];
+                }
+            }
+            if (wI) {
+                I1[pm] = io[0];
+                I1[cs + pm] = io[1];
+                I1[2 * cs + pm] = io[2];
+                if (PM != kNoPml && pmlm && upd) {
+                    fIout[lim] = fi[0];
+                    fIout[npI + lim] = fi[1];
+                    fIout[2 * npI + lim] = fi[2];
+                }
+            }
+            // ---- carry plane z
+            vm0 = v[0]; vm1 = v[1]; vm2 = v[2];
+            vrz = nrz; vry = nry; vuz = nuz; vux = nux;
+            im0 = ix; im1 = iy; im2 = iz;
+            ix_zm = ix; iy_zm = iy;
+            ixr_zm = ixr; iyu_zm = iyu;
+            um = u; lim = li; wm = w; pmlm = pml;
+            pm = p;
+        }
+        // the last current plane is skipped by the main update: UPML hooks only
+        if (PM != kNoPml && z1 == Nz && pmlm) {
+            const float im[3] = {im0, im1, im2};
+#pragma unroll
+            for (int n = 0; n < 3; ++n) {
+                float c0, co, cn;
+                pml_coef_n<PM>(a.pI, lim, wm, n, c0, co, cn);
+                const float fl = fIin[n * npI + lim];
+                const float tt = c0 * im[n] - co * fl;
+                I1[n * cs + pm] = tt + cn * fl;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// ++numTS after a single-sweep step (Engine::IterateTS, FDTD/engine.cpp:283).
+// Every workgroup of the step reads the counter, so the step itself cannot
+// advance it; a one-thread launch after it does, with no fence or atomic
+// ordering inside the step.
+// ---------------------------------------------------------------------------
+__global__ void k_tick(unsigned* __restrict__ numts)
+{
+    if (threadIdx.x == 0) numts[0] = numts[0] + 1u;
 }
 
 // ---------------------------------------------------------------------------
@@ -1441,6 +1765,7 @@ struct ems_engine {
     unsigned long long cs = 0;
     float *V = nullptr, *I = nullptr;          // current state
     float *V_alt = nullptr, *I_alt = nullptr;  // second buffers of the fused step (state n+1)
+    float* V_base = nullptr;                   // V at set-up (buffer parity of the fused step)
     // coefficient layout: plain arrays or per-cell index + tables
     int layout_req = EMS_LAYOUT_AUTO;
     bool cmp = false;
@@ -1483,8 +1808,8 @@ struct ems_engine {
     PmlArgs pmlV{}, pmlI{};
     float* flux_alt[2] = {nullptr, nullptr};   // second flux buffers of the fused step (V, I)
     // fused single-sweep step
-    int fused_req = -1;                   // -1 auto (two half steps), 0 off, 1 required
-    bool fused = false;
+    int fused_req = -1;                   // -1 auto, 0 two half steps, 1 LDS sweep, 2 column sweep
+    int fused = 0;                        // active form: 0 two half steps, 1 k_step, 2 k_step2
     unsigned ntx = 0, nty = 0, fzdepth = 0, fnzc = 0;
     ExcArgs eV{}, eI{};
     int pml_mode = kNoPml;
@@ -1518,6 +1843,19 @@ struct ems_engine {
     bool has_ss = false;
     SteadyDev ss{};
     double* ss_part = nullptr;
+
+    // timestep chunks replayed as HIP graphs: captured per launch pattern
+    // (the probe / dump / steady-state decisions of the chunk's steps and the
+    // buffer parity of the single-sweep forms), instantiated once, replayed
+    struct GraphEnt {
+        std::vector<unsigned char> key;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    unsigned graph_chunk = 32;            // requested steps per graph (0: direct launches)
+    unsigned graph_len = 0;               // steps per graph of the captured set
+    std::vector<GraphEnt> graphs;
+    unsigned graph_replays = 0, graph_captures = 0;
 
     // timing
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -2105,14 +2443,16 @@ static unsigned choose_fused_zdepth(unsigned ntiles, unsigned Nz)
 
 // Decide and prepare the fused single-sweep step: every active extension
 // must have a fused form (UPML, excitation); Mur, Lorentz and lumped RLC keep
-// the two-half-step path.
+// the two-half-step path.  Form 1 (k_step) stages V^{n+1} planes in LDS per
+// 64 x 16 tile; form 2 (k_step2) sweeps single columns and recomputes the
+// neighbour components it needs.
 static int setup_fused(ems_engine* h)
 {
-    h->fused = false;
-    // automatic mode keeps the two half steps: on MI355X the per-plane
-    // barriers of the fused sweep cost more than its saved traffic (256^3 UPML,
-    // compressed: 0.50 ms per step fused, 0.38 ms in two half steps)
-    if (h->fused_req <= 0) return 0;
+    h->fused = 0;
+    // automatic mode keeps the two half steps (256^3 UPML, compressed: 0.50 ms
+    // per step in the LDS sweep, 0.38 ms in two half steps)
+    const int form = h->fused_req;
+    if (form <= 0) return 0;
     std::string why;
     const unsigned Nx = h->N[0], Ny = h->N[1], Nz = h->N[2];
     if (!h->mur.empty()) why = "Mur ABC has no fused form";
@@ -2123,14 +2463,20 @@ static int setup_fused(ems_engine* h)
     for (const auto& e : h->exc[1].host)
         if (why.empty() && (e.x + 1u >= Nx || e.y + 1u >= Ny || e.z + 1u >= Nz))
             why = "a current source lies outside the updated range";
-    const unsigned ntx = (Nx + kTX - 1) / kTX, nty = (Ny + kTY - 1) / kTY, ntiles = ntx * nty;
-    // per-tile source lists in list order: voltage sources of the tile and
-    // its halo (column x0+kTX, row y0+kTY), current sources of the tile
+    // source lists in list order: per tile (form 1: voltage sources of the
+    // tile and its halo, column x0+kTX and row y0+kTY; current sources of the
+    // tile) or per (x, y) column (form 2)
+    const unsigned ntx = (Nx + kTX - 1) / kTX, nty = (Ny + kTY - 1) / kTY;
+    const unsigned nlists = form == 1 ? ntx * nty : h->g.plane;
     std::vector<std::vector<ExcEnt>> lists[2];
     size_t most = 0;
     for (int c = 0; c < 2 && why.empty(); ++c) {
-        lists[c].assign(ntiles, {});
+        lists[c].assign(nlists, {});
         for (const auto& e : h->exc[c].host) {
+            if (form == 2) {
+                lists[c][e.x + Nx * (unsigned)e.y].push_back(e);
+                continue;
+            }
             const unsigned txs[2] = {e.x / kTX, (c == 0 && e.x % kTX == 0 && e.x > 0) ? e.x / kTX - 1 : ~0u};
             const unsigned tys[2] = {e.y / kTY, (c == 0 && e.y % kTY == 0 && e.y > 0) ? e.y / kTY - 1 : ~0u};
             for (unsigned tx : txs)
@@ -2139,16 +2485,13 @@ static int setup_fused(ems_engine* h)
         }
         for (const auto& l : lists[c]) most = std::max(most, l.size());
     }
-    if (why.empty() && most > 64) why = "more than 64 sources in one tile";
-    if (!why.empty()) {
-        if (h->fused_req == 1) return fail(EMS_E_ARG, "fused step unavailable: " + why);
-        return 0;
-    }
+    if (why.empty() && most > 64) why = form == 1 ? "more than 64 sources in one tile" : "more than 64 sources in one column";
+    if (!why.empty()) return fail(EMS_E_ARG, "fused step unavailable: " + why);
     int rc;
     for (int c = 0; c < 2; ++c) {
-        std::vector<unsigned> beg(ntiles + 1, 0);
+        std::vector<unsigned> beg(nlists + 1, 0);
         std::vector<ExcEnt> ent;
-        for (unsigned t = 0; t < ntiles; ++t) {
+        for (unsigned t = 0; t < nlists; ++t) {
             ent.insert(ent.end(), lists[c][t].begin(), lists[c][t].end());
             beg[t + 1] = (unsigned)ent.size();
         }
@@ -2174,11 +2517,18 @@ static int setup_fused(ems_engine* h)
         HIPCHK(hipMemcpy(h->flux_alt[side], P.flux_in, 3 * P.npos * sizeof(float), hipMemcpyDeviceToDevice));
     }
     HIPCHK(hipDeviceSynchronize());
-    h->ntx = ntx;
-    h->nty = nty;
-    h->fzdepth = choose_fused_zdepth(ntiles, Nz);
-    h->fnzc = (Nz + h->fzdepth - 1) / h->fzdepth;
-    h->fused = true;
+    h->V_base = h->V;
+    if (form == 1) {
+        h->ntx = ntx;
+        h->nty = nty;
+        h->fzdepth = choose_fused_zdepth(ntx * nty, Nz);
+        h->fnzc = (Nz + h->fzdepth - 1) / h->fzdepth;
+    } else {
+        // the two-half-step tiles (kBlock consecutive columns) and z depth
+        h->fzdepth = h->g.zdepth;
+        h->fnzc = h->g.nzc;
+    }
+    h->fused = form;
     return 0;
 }
 
@@ -2186,7 +2536,8 @@ int ems_set_fused(ems_engine* h, int mode)
 {
     if (!h) return fail(EMS_E_ARG, "null handle");
     if (h->inited) return fail(EMS_E_STATE, "engine already initialised");
-    if (mode < -1 || mode > 1) return fail(EMS_E_ARG, "fused mode is -1 (auto), 0 (off) or 1 (on)");
+    if (mode < -1 || mode > 2)
+        return fail(EMS_E_ARG, "step form is -1 (auto), 0 (two half steps), 1 (LDS sweep) or 2 (column sweep)");
     h->fused_req = mode;
     return 0;
 }
@@ -2194,7 +2545,7 @@ int ems_set_fused(ems_engine* h, int mode)
 int ems_get_fused(const ems_engine* h, int* fused, unsigned* zdepth)
 {
     if (!h) return fail(EMS_E_ARG, "null handle");
-    if (fused) *fused = h->fused ? 1 : 0;
+    if (fused) *fused = h->fused;
     if (zdepth) *zdepth = h->fused ? h->fzdepth : h->g.zdepth;
     return 0;
 }
@@ -2263,6 +2614,8 @@ int ems_init(ems_engine* h)
     return 0;
 }
 
+static void drop_graphs(ems_engine* h);
+
 int ems_add_probe(ems_engine* h, const ems_probe_desc* d, int* id)
 {
     if (!h || !d) return fail(EMS_E_ARG, "null argument");
@@ -2321,6 +2674,7 @@ int ems_add_fd_dump(ems_engine* h, int field, const unsigned start[3], const uns
     if (!h || !start || !stop || !factors || !edge_len) return fail(EMS_E_ARG, "null argument");
     if (field != 0) return fail(EMS_E_ARG, "only E-field frequency-domain dumps are supported");
     if (fd_interval == 0 || nfreq == 0 || nsamples == 0) return fail(EMS_E_ARG, "empty frequency-domain dump");
+    drop_graphs(h);     // the dump changes the launch pattern of every chunk
     DumpDev d{};
     for (int n = 0; n < 3; ++n) {
         if (start[n] > stop[n] || stop[n] >= h->N[n]) return fail(EMS_E_ARG, "dump box outside the mesh");
@@ -2363,6 +2717,8 @@ int ems_fd_dump_results(ems_engine* h, int id, float* out)
 static int sync_probe_tables(ems_engine* h)
 {
     if (!h->probes_dirty) return 0;
+    // captured chunks hold the old probe tables
+    drop_graphs(h);
     int rc;
     if ((rc = upload(h, &h->d_probes, h->probes.data(), h->probes.size())) ||
         (rc = upload(h, &h->d_elin, h->h_elin.data(), h->h_elin.size())) ||
@@ -2444,9 +2800,75 @@ void launch_step_t(ems_engine* h, const StepArgs& a, unsigned nblk)
     hipLaunchKernelGGL((k_step<PM, CC>), dim3(nblk), dim3(kBlock), 0, h->stream, a);
 }
 
+void lds_step(ems_engine* h);
+
+}  // namespace
+
+// state n+1 written by a single-sweep step becomes the current state
+static void swap_step_buffers(ems_engine* h)
+{
+    std::swap(h->V, h->V_alt);
+    std::swap(h->I, h->I_alt);
+    for (int side = 0; side < 2; ++side) {
+        PmlArgs& P = side ? h->pmlI : h->pmlV;
+        if (!P.n) continue;
+        float* cur = const_cast<float*>(P.flux_in);
+        P.flux_in = h->flux_alt[side];
+        P.flux_out = h->flux_alt[side];
+        h->flux_alt[side] = cur;
+    }
+}
+
+namespace {
+
+template <int PM, bool CC>
+void launch_step2_t(ems_engine* h, const Step2Args& a, unsigned nblk)
+{
+    hipLaunchKernelGGL((k_step2<PM, CC>), dim3(nblk), dim3(kBlock), 0, h->stream, a, h->V, h->I, h->V_alt, h->I_alt,
+                       h->pmlV.flux_in, h->flux_alt[0], h->pmlI.flux_in, h->flux_alt[1]);
+}
+
+void column_step(ems_engine* h)
+{
+    Step2Args a{};
+    a.g = h->g;
+    a.kV = h->kV;
+    a.kI = h->kI;
+    a.pV = h->pmlV;
+    a.pI = h->pmlI;
+    a.eV = h->eV;
+    a.eI = h->eI;
+    a.numts = h->numts_dev;
+    const unsigned nblk = h->g.qtiles * h->g.nzc;
+    const int pm = h->pmlV.n ? h->pml_mode : kNoPml;
+    if (h->cmp) {
+        if (pm == kPmlCompressed) launch_step2_t<kPmlCompressed, true>(h, a, nblk);
+        else if (pm == kPmlPlain) launch_step2_t<kPmlPlain, true>(h, a, nblk);
+        else launch_step2_t<kNoPml, true>(h, a, nblk);
+    } else {
+        if (pm == kPmlCompressed) launch_step2_t<kPmlCompressed, false>(h, a, nblk);
+        else if (pm == kPmlPlain) launch_step2_t<kPmlPlain, false>(h, a, nblk);
+        else launch_step2_t<kNoPml, false>(h, a, nblk);
+    }
+}
+
 // one fused step from state n (h->V, h->I, current fluxes) into the second
 // buffers, which then become the current state
 void fused_step(ems_engine* h, hipEvent_t* ev)
+{
+    if (ev) (void)hipEventRecord(ev[0], h->stream);
+    if (h->fused == 2) column_step(h);
+    else lds_step(h);
+    if (ev) {
+        (void)hipEventRecord(ev[1], h->stream);
+        (void)hipEventRecord(ev[2], h->stream);
+    }
+    hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, h->stream, h->numts_dev);
+    if (ev) (void)hipEventRecord(ev[3], h->stream);
+    swap_step_buffers(h);
+}
+
+void lds_step(ems_engine* h)
 {
     StepArgs a{};
     a.g = h->g;
@@ -2469,7 +2891,6 @@ void fused_step(ems_engine* h, hipEvent_t* ev)
     a.numts = h->numts_dev;
     const unsigned nblk = h->ntx * h->nty * h->fnzc;
     const int pm = h->pmlV.n ? h->pml_mode : kNoPml;
-    if (ev) (void)hipEventRecord(ev[0], h->stream);
     if (h->cmp) {
         if (pm == kPmlCompressed) launch_step_t<kPmlCompressed, true>(h, a, nblk);
         else if (pm == kPmlPlain) launch_step_t<kPmlPlain, true>(h, a, nblk);
@@ -2478,21 +2899,6 @@ void fused_step(ems_engine* h, hipEvent_t* ev)
         if (pm == kPmlCompressed) launch_step_t<kPmlCompressed, false>(h, a, nblk);
         else if (pm == kPmlPlain) launch_step_t<kPmlPlain, false>(h, a, nblk);
         else launch_step_t<kNoPml, false>(h, a, nblk);
-    }
-    if (ev) {
-        (void)hipEventRecord(ev[1], h->stream);
-        (void)hipEventRecord(ev[2], h->stream);
-        (void)hipEventRecord(ev[3], h->stream);
-    }
-    std::swap(h->V, h->V_alt);
-    std::swap(h->I, h->I_alt);
-    for (int side = 0; side < 2; ++side) {
-        PmlArgs& P = side ? h->pmlI : h->pmlV;
-        if (!P.n) continue;
-        float* cur = const_cast<float*>(P.flux_in);
-        P.flux_in = h->flux_alt[side];
-        P.flux_out = h->flux_alt[side];
-        h->flux_alt[side] = cur;
     }
 }
 
@@ -2580,6 +2986,118 @@ int ems_set_profiling(ems_engine* h, int enable)
 // steps of one ems_iterate call whose kernels are bracketed by events
 constexpr unsigned kProfSteps = 4096;
 
+// Steps per captured chunk: the requested length rounded up to a multiple of
+// the period of the launch pattern (the sampling intervals of probes and
+// dumps, the steady-state period, 2 for the buffer parity of a single-sweep
+// form) when that period is at most 1024 steps, so that chunks repeat.
+static unsigned graph_steps(const ems_engine* h)
+{
+    unsigned long long P = h->fused ? 2 : 1;
+    auto add = [&P](unsigned v) {
+        if (v) P = P / std::gcd(P, (unsigned long long)v) * v;
+    };
+    for (const auto& p : h->probes) {
+        add(p.interval);
+        add(p.fd_interval);
+    }
+    for (const auto& d : h->dumps) add(d.fd_interval);
+    if (h->has_ss) add(h->ss.p);
+    const unsigned c = h->graph_chunk;
+    if (P > 1024) return c;
+    return (unsigned)((c + P - 1) / P * P);
+}
+
+// launch pattern of the next `steps` steps from the host timestep on
+static std::vector<unsigned char> chunk_key(const ems_engine* h, unsigned steps)
+{
+    std::vector<unsigned char> key;
+    key.push_back((unsigned char)(h->fused && h->V != h->V_base ? 1 : 0));
+    for (unsigned k = 0; k < steps; ++k) {
+        const unsigned ts = h->numTS + k;          // numTS before step k of the chunk
+        unsigned char b = 0;
+        if (h->has_ss && ts % h->ss.p == 0 && ts >= 2 * h->ss.p) b |= 1;
+        if (!h->probes.empty() && probe_due(h, ts + 1)) b |= 2;
+        key.push_back(b);
+        for (size_t d = 0; d < h->dumps.size(); ++d) {
+            const DumpDev& D = h->dumps[d];
+            key.push_back((unsigned char)((ts + 1) % D.fd_interval == 0 && (ts + 1) / D.fd_interval < D.nsamples));
+        }
+    }
+    return key;
+}
+
+// one chunk of graph_chunk steps: replay the graph of its launch pattern,
+// captured from the direct launches the first time the pattern occurs
+static int graph_chunk(ems_engine* h, unsigned steps)
+{
+    const std::vector<unsigned char> key = chunk_key(h, steps);
+    ems_engine::GraphEnt* G = nullptr;
+    for (auto& e : h->graphs)
+        if (e.key == key) G = &e;
+    if (G) {
+        HIPCHK(hipGraphLaunch(G->exec, h->stream));
+        // host state as the captured launches left it
+        for (unsigned k = 0; k < steps; ++k) {
+            if (h->fused) swap_step_buffers(h);
+            ++h->numTS;
+        }
+        ++h->graph_replays;
+        return 0;
+    }
+    if (h->graphs.size() >= kMaxGraphs) {
+        // too many distinct patterns: direct launches
+        for (unsigned k = 0; k < steps; ++k) {
+            launch_step(h, nullptr);
+            ++h->numTS;
+            launch_samples(h);
+        }
+        return 0;
+    }
+    ems_engine::GraphEnt e;
+    e.key = key;
+    HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    for (unsigned k = 0; k < steps; ++k) {
+        launch_step(h, nullptr);
+        ++h->numTS;
+        launch_samples(h);
+    }
+    const hipError_t ec = hipStreamEndCapture(h->stream, &e.graph);
+    if (ec != hipSuccess) return fail(EMS_E_HIP, std::string("graph capture failed: ") + hipGetErrorString(ec));
+    HIPCHK(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
+    HIPCHK(hipGraphLaunch(e.exec, h->stream));
+    h->graphs.push_back(std::move(e));
+    ++h->graph_captures;
+    return 0;
+}
+
+static void drop_graphs(ems_engine* h)
+{
+    if (!h->graphs.empty()) hipStreamSynchronize(h->stream);
+    for (auto& G : h->graphs) {
+        if (G.exec) hipGraphExecDestroy(G.exec);
+        if (G.graph) hipGraphDestroy(G.graph);
+    }
+    h->graphs.clear();
+}
+
+int ems_set_graph(ems_engine* h, unsigned chunk_steps)
+{
+    if (!h) return fail(EMS_E_ARG, "null handle");
+    if (chunk_steps > 4096) return fail(EMS_E_ARG, "graph chunk above 4096 steps");
+    drop_graphs(h);
+    h->graph_chunk = chunk_steps;
+    return 0;
+}
+
+int ems_get_graph(const ems_engine* h, unsigned* chunk_steps, unsigned* captures, unsigned* replays)
+{
+    if (!h) return fail(EMS_E_ARG, "null handle");
+    if (chunk_steps) *chunk_steps = h->graph_chunk;
+    if (captures) *captures = h->graph_captures;
+    if (replays) *replays = h->graph_replays;
+    return 0;
+}
+
 int ems_iterate(ems_engine* h, unsigned nsteps)
 {
     if (!h) return fail(EMS_E_ARG, "null handle");
@@ -2599,8 +3117,25 @@ int ems_iterate(ems_engine* h, unsigned nsteps)
         launch_samples(h);
     }
     HIPCHK(hipEventRecord(h->ev0, h->stream));
-    for (unsigned s = 0; s < nsteps; ++s) {
-        launch_step(h, s < nprof ? &h->kev[4 * (size_t)s] : nullptr);
+    unsigned s = 0;
+    for (; s < nprof; ++s) {
+        launch_step(h, &h->kev[4 * (size_t)s]);
+        ++h->numTS;
+        launch_samples(h);
+    }
+    if (h->graph_chunk) {
+        const unsigned steps = graph_steps(h);
+        if (steps != h->graph_len) {
+            drop_graphs(h);
+            h->graph_len = steps;
+        }
+        while (nsteps - s >= steps) {
+            if ((rc = graph_chunk(h, steps))) return rc;
+            s += steps;
+        }
+    }
+    for (; s < nsteps; ++s) {
+        launch_step(h, nullptr);
         ++h->numTS;
         launch_samples(h);
     }
@@ -2716,6 +3251,10 @@ void ems_destroy(ems_engine* h)
     if (!h) return;
     hipSetDevice(h->device);
     if (h->stream) hipStreamSynchronize(h->stream);
+    for (auto& G : h->graphs) {
+        if (G.exec) hipGraphExecDestroy(G.exec);
+        if (G.graph) hipGraphDestroy(G.graph);
+    }
     for (void* p : h->allocs) hipFree(p);
     if (h->ev0) hipEventDestroy(h->ev0);
     if (h->ev1) hipEventDestroy(h->ev1);

@@ -58,6 +58,9 @@ def expand_sse_compressed(N, op_index, vv4, vi4, ii4, iv4):
     return out
 
 
+STEP_FORMS = {"halves": 0, "lds": 1, "columns": 2}
+
+
 class EngineHIP:
     def __init__(self, op, device=0, layout="auto", sse_compressed=None, fused=None, steady_state=None,
                  lorentz=None, rlc=None):
@@ -72,8 +75,10 @@ class EngineHIP:
         rlc: optional lumped RLC table (Engine_Ext_LumpedRLC), a dict pos
         [3][K], dir [K] and the nine coefficient arrays i2v, ilv, vvd, vv2,
         vj1, vj2, ib0, b1, b2 [K];
-        fused: None (automatic), False (two half steps) or True (single-sweep
-        step required)."""
+        fused: step form, None (automatic), False or "halves" (two half
+        steps), True or "lds" (single sweep staging planes in LDS, k_step) or
+        "columns" (single sweep of independent columns, k_step2); a required
+        single-sweep form fails when an active extension has none."""
         L = lib()
         if L.ems_device_count() <= 0:
             raise RuntimeError("EngineHIP needs a HIP device; none is visible")
@@ -98,7 +103,7 @@ class EngineHIP:
         self.h = h
         self._keep = []
         if fused is not None:
-            check(L.ems_set_fused(h, 1 if fused else 0))
+            check(L.ems_set_fused(h, STEP_FORMS[fused] if isinstance(fused, str) else (1 if fused else 0)))
         if steady_state is not None:
             self.set_steady_state(*steady_state)
         self._lorentz = lorentz or []
@@ -196,6 +201,16 @@ class EngineHIP:
     def set_profiling(self, flag):
         check(lib().ems_set_profiling(self.h, int(flag)))
 
+    def set_graph(self, chunk_steps):
+        """Replay chunks of chunk_steps timesteps as captured HIP graphs (0: direct launches)."""
+        check(lib().ems_set_graph(self.h, int(chunk_steps)))
+
+    def graph(self):
+        """(chunk steps, graphs captured, chunks replayed)"""
+        a, b, c = C.c_uint(), C.c_uint(), C.c_uint()
+        check(lib().ems_get_graph(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
     def layout(self):
         """(layout, n_unique, pml_layout, n_unique_pml) as names and table sizes"""
         a, b, c, d = C.c_int(), C.c_uint(), C.c_int(), C.c_uint()
@@ -221,10 +236,11 @@ class EngineHIP:
         return last.value, list(zip(ts[:m].tolist(), df[:m].tolist()))
 
     def fused(self):
-        """(True if the single-sweep step is active, z planes per workgroup)"""
+        """(active step form: 0 two half steps, 1 LDS sweep, 2 column sweep;
+        z planes per workgroup)"""
         f, d = C.c_int(), C.c_uint()
         check(lib().ems_get_fused(self.h, C.byref(f), C.byref(d)))
-        return bool(f.value), d.value
+        return f.value, d.value
 
     def last_timing(self):
         t, v, c = C.c_double(), C.c_double(), C.c_double()

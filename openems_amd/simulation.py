@@ -21,7 +21,7 @@ from .processing import FieldDumpFD, Probe
 
 
 class Simulation:
-    def __init__(self, scene, device=0, nthreads=0, layout="auto", fused=None):
+    def __init__(self, scene, device=0, nthreads=0, layout="auto", fused=None, graph=None):
         self.scene = scene
         self.nrts = int(scene.get("nrts", 1000))
         t0 = time.perf_counter()
@@ -33,6 +33,8 @@ class Simulation:
             ss = steady_state_probes(self.op)
             self.ss_period = ss[0]
         self.eng = EngineHIP(self.op, device=device, layout=layout, fused=fused, steady_state=ss)
+        if graph is not None:
+            self.eng.set_graph(graph)
         os_ = int(scene.get("oversampling", 4))
         self.probes = [Probe(self.op, p, self.nrts, os_) for p in scene.get("probes", [])]
         self.dumps = [FieldDumpFD(self.op, d, self.nrts) for d in scene.get("dumps", [])]

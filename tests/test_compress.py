@@ -109,8 +109,8 @@ def test_sse_compressed_import_rejects_bad_index(native):
 
 def test_update_kernels_have_no_fused_or_packed_fp32(native):
     """Bit-exactness needs every product and sum rounded on its own: the
-    gfx950 code of the update kernels holds no f32 FMA / MAC and no packed
-    f32 instruction."""
+    gfx950 code of the update kernels (both half steps and both single-sweep
+    steps) holds no f32 FMA / MAC and no packed f32 instruction."""
     src = os.path.join(ROOT, "openems_amd", "csrc", "engine", "ems_engine.hip")
     from openems_amd import build as B
     with tempfile.TemporaryDirectory() as d:
@@ -127,8 +127,9 @@ def test_update_kernels_have_no_fused_or_packed_fp32(native):
             kernels.setdefault(cur, [])
         elif cur:
             kernels[cur].append(line)
-    upd = {k: v for k, v in kernels.items() if "k_volt" in k or "k_curr" in k}
-    assert len(upd) == 12                                 # 3 UPML modes x 2 layouts x 2 half steps
+    upd = {k: v for k, v in kernels.items() if re.search(r"k_(volt|curr|step|step2)I", k)}
+    # 3 UPML modes x 2 layouts x (2 half steps + 2 single-sweep forms)
+    assert len(upd) == 24
     bad = re.compile(r"\bv_(fma\w*_f32|fmac\w*_f32|mac_f32|mad_f32|pk_\w+_f32|fma_mix\w*|dot\w*)")
     for k, lines in upd.items():
         hits = [l.strip() for l in lines if bad.search(l)]

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from openems_amd import scenes
-from openems_amd.engine import EngineHIP
+from openems_amd.engine import STEP_FORMS, EngineHIP
 from openems_amd.operator import Operator
 from openems_amd.simulation import Simulation
 from oracle.operator_ref import RefOperator
@@ -34,7 +34,7 @@ def _fields_equal(eng, ref_eng):
     return v, c, rv, rc
 
 
-@pytest.mark.parametrize("step", ["fused", "halves"])
+@pytest.mark.parametrize("step", ["lds", "columns", "halves"])
 @pytest.mark.parametrize("layout", ["plain", "compressed"])
 @pytest.mark.parametrize("make", [
     lambda: scenes.pec_cavity(32, 150),
@@ -42,19 +42,19 @@ def _fields_equal(eng, ref_eng):
     lambda: scenes.mur_layers(40, 36, 32, 200),
 ])
 def test_fields_bit_identical(gpu, make, layout, step):
-    """Fields after 150-200 steps, both coefficient layouts, both step forms."""
+    """Fields after 150-200 steps, both coefficient layouts, every step form."""
     scene = make()
     nrts = scene["nrts"]
     print("case", scene["name"], "steps", nrts, "layout", layout, "step", step, flush=True)
     op = Operator(scene)
     # Mur has no fused form: the automatic mode keeps two half steps there
-    want_fused = step == "fused" and not op.mur_faces()
-    eng = EngineHIP(op, layout=layout, fused=True if want_fused else (None if step == "fused" else False))
+    want = 0 if op.mur_faces() else STEP_FORMS[step]
+    eng = EngineHIP(op, layout=layout, fused=step if want or step == "halves" else None)
     lay, nu, play, nup = eng.layout()
     assert lay == layout and (nu > 0) == (layout == "compressed")
     if op.upml_slabs():
         assert play == layout and (nup > 0) == (layout == "compressed")
-    assert eng.fused()[0] == want_fused
+    assert eng.fused()[0] == want
     eng.IterateTS(nrts)
     ref = RefEngine(_ref_from_scene(scene), "basic")
     ref.iterate(nrts)
@@ -207,16 +207,18 @@ def test_sse_compressed_operator_import(gpu):
     ref.close()
 
 
+@pytest.mark.parametrize("form", ["lds", "columns"])
 @pytest.mark.parametrize("layout", ["plain", "compressed"])
-def test_fused_step_sources_on_tile_edges(gpu, layout):
+def test_fused_step_sources_on_tile_edges(gpu, layout, form):
     """128^3 PEC cavity: the source edge (64, 64, 64) lies on a column-tile
-    corner of the fused step (tiles of 64 x 16), so its value is recomputed
-    in three neighbouring tiles' halos; fields stay bit-identical."""
+    corner of the LDS sweep (tiles of 64 x 16) and on a z-chunk start, so its
+    value is recomputed in neighbouring tiles' halos, by its neighbour
+    columns and by the chunk below; fields stay bit-identical."""
     scene = scenes.pec_cavity(128, 60)
-    print("case pec_cavity_128 tile-edge source, 60 steps, layout", layout, flush=True)
+    print("case pec_cavity_128 tile-edge source, 60 steps, layout", layout, "form", form, flush=True)
     op = Operator(scene)
-    eng = EngineHIP(op, layout=layout, fused=True)
-    assert eng.fused()[0]
+    eng = EngineHIP(op, layout=layout, fused=form)
+    assert eng.fused()[0] == STEP_FORMS[form]
     eng.IterateTS(60)
     ref = RefEngine(_ref_from_scene(scene), "basic")
     ref.iterate(60)
@@ -226,7 +228,7 @@ def test_fused_step_sources_on_tile_edges(gpu, layout):
     eng.close()
 
 
-@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("fused", ["lds", "columns", "halves"])
 @pytest.mark.parametrize("make", [
     lambda: scenes.pec_cavity(40, 40),
     lambda: scenes.upml_bricks(40, 40, nbricks=6, dump=False),
@@ -240,7 +242,7 @@ def test_arbitrary_initial_fields(gpu, make, fused):
     print("case", scene["name"], "random initial fields, fused", fused, flush=True)
     op = Operator(scene)
     eng = EngineHIP(op, fused=fused)
-    assert eng.fused()[0] == fused
+    assert eng.fused()[0] == STEP_FORMS[fused]
     ref = RefEngine(_ref_from_scene(scene), "basic")
     rv, rc = ref.raw_views()
     rng = np.random.default_rng(11)
@@ -260,14 +262,15 @@ def test_arbitrary_initial_fields(gpu, make, fused):
     eng.close()
 
 
-def test_fused_step_refused_where_unavailable(gpu):
-    """Mur ABC has no fused form: requiring the fused step fails loudly."""
+@pytest.mark.parametrize("form", ["lds", "columns"])
+def test_fused_step_refused_where_unavailable(gpu, form):
+    """Mur ABC has no fused form: requiring a single-sweep step fails loudly."""
     scene = scenes.mur_layers(24, 20, 16, 10)
-    print("case mur_layers fused refused", flush=True)
+    print("case mur_layers fused refused, form", form, flush=True)
     op = Operator(scene)
     from openems_amd._native import NativeError
     with pytest.raises(NativeError, match="fused"):
-        EngineHIP(op, fused=True)
+        EngineHIP(op, fused=form)
 
 
 def test_steady_state_detection_matches_oracle(gpu):
@@ -344,3 +347,56 @@ def test_lorentz_and_lumped_rlc_kernels(gpu, layout):
     assert np.array_equal(v, rv), f"voltage mismatches {np.count_nonzero(v != rv)}"
     assert np.array_equal(c, rc), f"current mismatches {np.count_nonzero(c != rc)}"
     eng.close()
+
+
+@pytest.mark.parametrize("chunk", [8, 7])
+@pytest.mark.parametrize("form", ["halves", "lds", "columns"])
+def test_graph_chunks_match_direct_launches(gpu, form, chunk):
+    """Timestep chunks replayed as HIP graphs give the same fields, probe
+    series and frequency-domain dump as direct launches, over several
+    ems_iterate calls with reads in between (every replay of every captured
+    chunk is checked through the state it leaves)."""
+    # port probes every 6 steps and a dump every 25: the chunk is rounded up
+    # to the 150-step period of that pattern
+    scene = scenes.upml_bricks(48, 1200, nbricks=8, dump=True)
+    print("case upml_bricks_48 graphs, form", form, "chunk", chunk, flush=True)
+    ref = Simulation(scene, fused=form, graph=0)
+    sim = Simulation(scene, fused=form, graph=chunk)
+    assert sim.eng.graph()[0] == chunk
+    for k in (150, 77, 523, 450):
+        ref.eng.IterateTS(k)
+        sim.eng.IterateTS(k)
+        assert np.array_equal(sim.eng.get_volt(), ref.eng.get_volt())
+        assert np.array_equal(sim.eng.get_curr(), ref.eng.get_curr())
+    _, captures, replays = sim.eng.graph()
+    assert captures >= 1 and replays > captures
+    a, b = sim.results(), ref.results()
+    assert a["numTS"] == b["numTS"] == 1200
+    for name, pr in b["probes"].items():
+        assert np.array_equal(a["probes"][name]["td"], pr["td"])
+        assert np.array_equal(a["probes"][name]["fd"], pr["fd"])
+    for name, d in b["dumps"].items():
+        assert np.array_equal(a["dumps"][name]["fd"], d["fd"])
+    sim.close()
+    ref.close()
+
+
+def test_graph_chunks_with_mur_and_steady_state(gpu):
+    """Mur faces and the steady-state record / evaluate launches inside
+    captured chunks: fields and the steady-state history equal direct
+    launches."""
+    # probes every 7 steps, steady-state period 59: chunks of 413 steps
+    scene = scenes.mur_layers(40, 36, 32, 1700)
+    print("case mur_layers graphs + steady state", flush=True)
+    ref = Simulation(scene, graph=0)
+    sim = Simulation(scene, graph=16)
+    assert ref.ss_period > 0
+    for k in (500, 1200):
+        ref.eng.IterateTS(k)
+        sim.eng.IterateTS(k)
+    assert np.array_equal(sim.eng.get_volt(), ref.eng.get_volt())
+    assert np.array_equal(sim.eng.get_curr(), ref.eng.get_curr())
+    assert sim.eng.steady_state(64) == ref.eng.steady_state(64)
+    assert sim.eng.graph()[2] > 0
+    sim.close()
+    ref.close()
