@@ -63,6 +63,13 @@ int emsop_add_material(emsop* h, const double eps[3], const double mue[3], const
 int emsop_add_material_grid(emsop* h, const unsigned cell0[3], const unsigned ncells[3], const double* eps_r,
                             const double* kappa, double mue_r, double sigma, int priority);
 int emsop_add_metal(emsop* h, int priority, const double start[3], const double stop[3]);
+/* Lorentz / Drude dispersion of the material box added last (CSPropLorentzMaterial,
+ * built by Operator_Ext_LorentzMaterial::BuildExtension,
+ * FDTD/extensions/operator_ext_lorentzmaterial.cpp:120-445): params[(o*6 + k)*3 + n]
+ * for order o < norder, direction n, k = eps plasma frequency (Hz), eps
+ * relaxation time (s), eps Lorentz pole frequency (Hz), mue plasma
+ * frequency, mue relaxation time, mue Lorentz pole frequency; 0 = unset. */
+int emsop_set_dispersion(emsop* h, unsigned norder, const double* params);
 /* After emsop_build: for the edge (component n, pos), the four cells of its
  * quarter-cell average (cells[3*q + axis], 0xFFFFFFFF for a cell before the
  * first line; q = up-right, up-left, down-right, down-left in the
@@ -114,6 +121,22 @@ int emsop_exc_get(const emsop* h, int curr, unsigned* idx, unsigned short* dir, 
 unsigned emsop_upml_count(const emsop* h);
 /* arrays: vv, vvfo, vvfn, ii, iifo, iifn, each [3][num0][num1][num2] */
 int emsop_upml_get(const emsop* h, unsigned k, unsigned start[3], unsigned num[3], const float* arrays[6]);
+
+/* Operator_Ext_LorentzMaterial tables (FDTD/extensions/operator_ext_lorentzmaterial.cpp:120-445):
+ * number of dispersion orders, and per order the cell count, the flags
+ * on = {volt ADE, curr ADE, volt Lorentz pole, curr Lorentz pole}, the cell
+ * positions pos[3][count] and tabs[3*k + n] = v_int, v_ext, i_int, i_ext,
+ * v_lor, i_lor of direction n ([count] float32, valid only where the
+ * matching flag is on).  Pointers stay valid until emsop_destroy. */
+unsigned emsop_lorentz_orders(const emsop* h);
+int emsop_lorentz_get(const emsop* h, unsigned order, unsigned* count, int on[4], const unsigned* pos[3],
+                      const float* tabs[18]);
+/* Operator_Ext_LumpedRLC tables (FDTD/extensions/operator_ext_lumpedRLC.cpp:112-548):
+ * element edges of parallel RLC (L > 0) and series RLC elements, pos[3][K],
+ * dir[K], coef[9][K] = i2v, ilv, vvd, vv2, vj1, vj2, ib0, b1, b2 (float32).
+ * Their edges' EC values and coefficients are rewritten in the operator. */
+unsigned emsop_rlc_count(const emsop* h);
+int emsop_rlc_get(const emsop* h, const unsigned* pos[3], const int** dir, const float* coef[9]);
 
 unsigned emsop_mur_count(const emsop* h);
 /* coefficients [N[nyP]][N[nyPP]] for the two tangential components */

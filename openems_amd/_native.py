@@ -57,6 +57,11 @@ OPERATOR_API = [
     ("emsop_exc_get", C.c_int, [vp, C.c_int, up, usp, fp, up]),
     ("emsop_upml_count", C.c_uint, [vp]),
     ("emsop_upml_get", C.c_int, [vp, C.c_uint, up, up, C.POINTER(fp)]),
+    ("emsop_set_dispersion", C.c_int, [vp, C.c_uint, dp]),
+    ("emsop_lorentz_orders", C.c_uint, [vp]),
+    ("emsop_lorentz_get", C.c_int, [vp, C.c_uint, up, ip, C.POINTER(up), C.POINTER(fp)]),
+    ("emsop_rlc_count", C.c_uint, [vp]),
+    ("emsop_rlc_get", C.c_int, [vp, C.POINTER(up), C.POINTER(ip), C.POINTER(fp)]),
     ("emsop_mur_count", C.c_uint, [vp]),
     ("emsop_mur_get", C.c_int, [vp, C.c_uint, ip, up, up, up, C.POINTER(fp), C.POINTER(fp)]),
     ("emsop_disc_line", C.c_double, [vp, C.c_int, C.c_uint, C.c_int]),

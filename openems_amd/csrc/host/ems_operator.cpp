@@ -23,6 +23,8 @@
 //                                                 FDTD/extensions/operator_ext_upml.cpp:69-445
 //   Operator_Ext_Mur_ABC::BuildExtension          FDTD/extensions/operator_ext_mur_abc.cpp:80-186
 //   Engine_Ext_Mur_ABC late start                 FDTD/extensions/engine_ext_mur_abc.cpp:28-60
+//   Operator_Ext_LorentzMaterial::BuildExtension  FDTD/extensions/operator_ext_lorentzmaterial.cpp:120-445
+//   Operator_Ext_LumpedRLC::BuildExtension        FDTD/extensions/operator_ext_lumpedRLC.cpp:112-548
 //
 // Material lookup: the CSXCAD rule "highest priority primitive containing the
 // point wins" is evaluated with per-axis bitmasks over boxes ranked by
@@ -73,6 +75,31 @@ struct MatBox : Box {
     double eps[3], mue[3], kappa[3], sigma[3];
     bool metal;
     int grid = -1;   // >= 0: per-cell eps_r / kappa from emsop::grids[grid]
+    int disp = -1;   // >= 0: Lorentz / Drude dispersion emsop::disps[disp]
+};
+
+// Lorentz / Drude dispersion of a material (CSPropLorentzMaterial): per
+// order o and direction n, p[(o * 6 + k) * 3 + n] with k = eps plasma
+// frequency (Hz), eps relaxation time (s), eps Lorentz pole frequency (Hz),
+// then the same three for mue.
+struct Dispersion {
+    unsigned norder;
+    std::vector<double> p;
+    double at(unsigned o, int k, int n) const { return o < norder ? p[((size_t)o * 6 + k) * 3 + n] : 0.0; }
+};
+
+// Operator_Ext_LorentzMaterial tables of one dispersion order
+struct LorOrder {
+    bool von = false, ion = false, vlon = false, ilon = false;
+    std::vector<unsigned> pos[3];
+    std::vector<float> t[6][3];     // v_int, v_ext, i_int, i_ext, v_lor, i_lor
+};
+
+// Operator_Ext_LumpedRLC tables
+struct RlcTab {
+    std::vector<unsigned> pos[3];
+    std::vector<int> dir;
+    std::vector<float> c[9];        // i2v, ilv, vvd, vv2, vj1, vj2, ib0, b1, b2
 };
 
 // Material grid: a box of whole mesh cells with its own eps_r and kappa per
@@ -246,6 +273,9 @@ struct emsop {
     ExcList exc[2];
     std::vector<Slab> upml;
     std::vector<MurFace> mur;
+    std::vector<Dispersion> disps;
+    std::vector<LorOrder> lor;
+    RlcTab rlc;
 
     size_t size() const { return (size_t)N[0] * N[1] * N[2]; }
     size_t idx(int n, unsigned x, unsigned y, unsigned z) const
@@ -348,6 +378,8 @@ struct emsop {
     int build_excitation();
     void build_mur();
     void build_upml();
+    void build_lorentz();
+    void build_rlc();
     int build_signal(unsigned nrts);
     void grading(int n, int ny, unsigned p, const int* bcs, const unsigned* size, double& kv, double& ki) const;
 };
@@ -1280,11 +1312,250 @@ int emsop::build(unsigned nrts)
         lap("mur");
         build_upml();
         lap("upml");
+        build_lorentz();
+        build_rlc();
+        lap("lorentz/rlc");
     } catch (const std::bad_alloc&) {
         return fail(EMSOP_E_NOMEM, "out of host memory while building the operator");
     }
     built = true;
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Lorentz / Drude materials (Operator_Ext_LorentzMaterial::BuildExtension,
+// FDTD/extensions/operator_ext_lorentzmaterial.cpp:120-445; Debye materials
+// are not restated).  Per order, the cells with an active component in x ->
+// y -> z order; the material at an edge is the MATERIAL|METAL winner at its
+// Yee point (primary edge centre for voltages, dual edge centre for
+// currents).  Coefficients in double, stored as float.
+// ---------------------------------------------------------------------------
+void emsop::build_lorentz()
+{
+    lor.clear();
+    unsigned order_max = 0;
+    for (const auto& m : mats)
+        if (m.disp >= 0) order_max = std::max(order_max, disps[m.disp].norder);
+    if (!order_max) return;
+    Ranked<MatBox> rk;
+    rk.init(mats, [](const MatBox&) { return true; });
+    const unsigned W = rk.W;
+    // masks of the voltage (primary edge centre) and current (dual edge
+    // centre) Yee points of each component
+    std::vector<uint64_t> mv[3][3], mi[3][3];
+    for (int n = 0; n < 3; ++n)
+        for (int a = 0; a < 3; ++a) {
+            mv[n][a] = rk.masks(a, a == n ? dualL[a] : lines[a]);
+            mi[n][a] = rk.masks(a, a == n ? lines[a] : dualL[a]);
+        }
+    lor.resize(order_max);
+    for (unsigned o = 0; o < order_max; ++o) {
+        LorOrder& T = lor[o];
+        std::vector<double> acc[6][3];
+        unsigned p[3];
+        for (p[0] = 0; p[0] < N[0]; ++p[0])
+            for (p[1] = 0; p[1] < N[1]; ++p[1])
+                for (p[2] = 0; p[2] < N[2]; ++p[2]) {
+                    bool on = false;
+                    double LD[3], RD[3], CL[3], CD[3], GD[3], LL[3];
+                    for (int n = 0; n < 3; ++n) {
+                        LD[n] = RD[n] = CL[n] = 0;
+                        if (p[n] >= N[n] - 1) continue;                       // GetYeeCoords (operator.cpp:189-193)
+                        const size_t q = idx(n, p[0], p[1], p[2]);
+                        if (coeff[1][q] == 0) continue;
+                        const int r = rk.winner(&mv[n][0][(size_t)p[0] * W], &mv[n][1][(size_t)p[1] * W],
+                                                &mv[n][2][(size_t)p[2] * W]);
+                        if (r < 0 || rk.box[r]->disp < 0) continue;
+                        const Dispersion& D = disps[rk.box[r]->disp];
+                        const double wp = D.at(o, 0, n) * 2 * kPI;
+                        const double C = ec[0][q];
+                        if (wp > 0 && C > 0) {
+                            on = true;
+                            T.von = true;
+                            LD[n] = 1 / (wp * wp * C);
+                        }
+                        const double tr = D.at(o, 1, n);
+                        if (tr > 0 && T.von) RD[n] = LD[n] / tr;
+                        const double wl = D.at(o, 2, n) * 2 * kPI;
+                        if (wl > 0 && LD[n] > 0) {
+                            T.vlon = true;
+                            CL[n] = 1 / (wl * wl * LD[n]);
+                        }
+                    }
+                    for (int n = 0; n < 3; ++n) {
+                        CD[n] = GD[n] = LL[n] = 0;
+                        const int nP = (n + 1) % 3, nPP = (n + 2) % 3;
+                        if (p[nP] >= N[nP] - 1 || p[nPP] >= N[nPP] - 1) continue;   // operator.cpp:194-200
+                        const size_t q = idx(n, p[0], p[1], p[2]);
+                        if (coeff[3][q] == 0) continue;
+                        const int r = rk.winner(&mi[n][0][(size_t)p[0] * W], &mi[n][1][(size_t)p[1] * W],
+                                                &mi[n][2][(size_t)p[2] * W]);
+                        if (r < 0 || rk.box[r]->disp < 0) continue;
+                        const Dispersion& D = disps[rk.box[r]->disp];
+                        const double wp = D.at(o, 3, n) * 2 * kPI;
+                        const double L = ec[2][q];
+                        if (wp > 0 && L > 0) {
+                            on = true;
+                            T.ion = true;
+                            CD[n] = 1 / (wp * wp * L);
+                        }
+                        const double tr = D.at(o, 4, n);
+                        if (tr > 0 && T.ion) GD[n] = CD[n] / tr;
+                        const double wl = D.at(o, 5, n) * 2 * kPI;
+                        if (wl > 0 && CD[n] > 0) {
+                            T.ilon = true;
+                            LL[n] = 1 / (wl * wl * CD[n]);
+                        }
+                    }
+                    if (!on) continue;
+                    for (int n = 0; n < 3; ++n) {
+                        T.pos[n].push_back(p[n]);
+                        const size_t q = idx(n, p[0], p[1], p[2]);
+                        const double VI = coeff[1][q], IV = coeff[3][q];
+                        if (LD[n] > 0) {
+                            acc[0][n].push_back((2.0 * LD[n] - dT * RD[n]) / (2.0 * LD[n] + dT * RD[n]));
+                            acc[1][n].push_back(dT / (LD[n] + dT * RD[n] / 2.0) * VI);
+                        } else {
+                            acc[0][n].push_back(1);
+                            acc[1][n].push_back(0);
+                        }
+                        if (CD[n] > 0) {
+                            acc[2][n].push_back((2.0 * CD[n] - dT * GD[n]) / (2.0 * CD[n] + dT * GD[n]));
+                            acc[3][n].push_back(dT / (CD[n] + dT * GD[n] / 2.0) * IV);
+                        } else {
+                            acc[2][n].push_back(1);
+                            acc[3][n].push_back(0);
+                        }
+                        acc[4][n].push_back(CL[n] > 0 ? dT / CL[n] / VI : 0.0);
+                        acc[5][n].push_back(LL[n] > 0 ? dT / LL[n] / IV : 0.0);
+                    }
+                }
+        for (int k = 0; k < 6; ++k)
+            for (int n = 0; n < 3; ++n) {
+                T.t[k][n].resize(acc[k][n].size());
+                for (size_t i = 0; i < acc[k][n].size(); ++i) T.t[k][n][i] = (float)acc[k][n][i];
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Lumped RLC elements (Operator_Ext_LumpedRLC::BuildExtension,
+// FDTD/extensions/operator_ext_lumpedRLC.cpp:112-548): parallel elements
+// with L > 0 and series elements, in scene order.  Rewrites EC_C / EC_G and
+// the coefficients of their edges and builds the ADE tables.
+// ---------------------------------------------------------------------------
+void emsop::build_rlc()
+{
+    constexpr double kZFact = 20.0;     // LUMPED_RLC_Z_FACT (operator_ext_lumpedRLC.h:29)
+    rlc = RlcTab{};
+    const double fMax = f0 + fc;
+    for (const auto& le : lumped) {
+        double C = le.C, R = le.R, L = le.L;
+        if (C < 0.0) C = 0.0;
+        if (R < 0.0) R = 0;
+        if (L < 0.0) L = 0;
+        if (std::isnan(C)) C = 0.0;
+        if (std::isnan(L)) L = 0.0;
+        if (std::isnan(R)) R = 0.0;
+        const bool parallel = le.parallel;
+        // IsLElumpedRLC (operator_ext_lumpedRLC.cpp:550-561) reads the raw inductance
+        if (!((parallel && le.L > 0.0) || !parallel)) continue;
+        const int dir = le.dir;
+        if (dir < 0 || dir > 2) continue;
+        const int d1 = (dir + 1) % 3, d2 = (dir + 2) % 3;
+        unsigned s[3], e[3];
+        const int dim = snap_box(le.lo, le.hi, false, s, e, nullptr, nullptr);
+        if (dim <= 0) continue;
+        if (s[dir] == e[dir]) continue;
+        const unsigned n0 = e[dir] - s[dir], n1 = e[d1] - s[d1] + 1, n2 = e[d2] - s[d2] + 1;
+        const unsigned Npar = n1 * n2;
+        const double dL = L * Npar / n0, dR = R * Npar / n0;
+        double dG = (1 / R) * n0 / Npar;
+        double dC = C * n0 / Npar;
+        double ib0 = 2.0 * dT * dC / (4.0 * dL * dC + 2.0 * dT * dR * dC + dT * dT);
+        double b1 = (dT * dT - 4.0 * dL * dC) / (dT * dC);
+        double b2 = (4.0 * dL * dC - 2.0 * dT * dR * dC + dT * dT) / (2.0 * dT * dC);
+        if (!parallel && C == 0) {
+            ib0 = dT / (2.0 * dL + dT * dR);
+            b1 = -4.0 * dL / dT;
+            b2 = (2.0 * dL - dT * dR) / dT;
+        }
+        if (parallel && R == 0.0) dG = 0.0;
+        unsigned p[3];
+        for (p[dir] = s[dir]; p[dir] < e[dir]; ++p[dir])
+            for (p[d1] = s[d1]; p[d1] <= e[d1]; ++p[d1])
+                for (p[d2] = s[d2]; p[d2] <= e[d2]; ++p[d2]) {
+                    const size_t q = idx(dir, p[0], p[1], p[2]);
+                    double v[9];
+                    if (parallel) {
+                        // a set capacitor replaces the edge's own; otherwise the
+                        // edge's capacitance is taken (and kept for the later edges)
+                        if (dC > 0)
+                            ec[0][q] = (float)dC;
+                        else
+                            dC = ec[0][q];
+                        v[0] = (dT / dC) / (1.0 + dT * dG / (2.0 * dC));
+                        if (R >= 0) ec[1][q] = (float)dG;
+                        v[1] = L > 0 ? dT / dL : 0.0;
+                        if (dC == 0) {
+                            double Cd = ec[0][q];
+                            const double Zmin = std::max(dR, 2 * kPI * fMax * dL);
+                            const double Zcd_min = 1.0 / (2.0 * kPI * fMax * Cd);
+                            if (Zcd_min < kZFact * Zmin) {
+                                Cd = 1.0 / (2 * kPI * fMax * Zmin * kZFact);
+                                ec[0][q] = (float)Cd;
+                            }
+                        }
+                        v[2] = 1.0;
+                        v[3] = v[4] = v[5] = v[6] = v[7] = v[8] = 0.0;
+                    } else {
+                        ec[1][q] = 0.0f;
+                        float Cd = ec[0][q];
+                        // pow(v, 2) of the reference, which GCC folds to v * v
+                        double Zmin;
+                        if (dC) {
+                            const double X = 2 * kPI * fMax * dL - 1.0 / (dC * 2 * kPI * fMax);
+                            Zmin = std::sqrt(dR * dR + X * X);
+                        } else {
+                            const double X = 2 * kPI * fMax * dL;
+                            Zmin = std::sqrt(dR * dR + X * X);
+                        }
+                        const double Zcd_min = 1.0 / (2.0 * kPI * fMax * Cd);
+                        if (Zcd_min < kZFact * Zmin) {
+                            Cd = (float)(1.0 / (2 * kPI * fMax * Zmin * kZFact));
+                            ec[0][q] = Cd;
+                        }
+                        v[0] = 0.0;
+                        v[1] = 0.0;
+                        v[3] = 0.5 * dT * ib0 / Cd;
+                        v[4] = 0.5 * dT * (b1 * ib0 - 1.0) / Cd;
+                        v[5] = 0.5 * dT * b2 * ib0 / Cd;
+                        v[2] = 1.0 / (1.0 + 0.5 * dT * ib0 / Cd);
+                        v[6] = ib0;
+                        v[7] = b1;
+                        v[8] = b2;
+                    }
+                    ec_to_coeff(dir, q);
+                    rlc.dir.push_back(dir);
+                    for (int k = 0; k < 9; ++k) rlc.c[k].push_back((float)v[k]);
+                    for (int a = 0; a < 3; ++a) rlc.pos[a].push_back(p[a]);
+                }
+        if (le.caps) {
+            for (p[d1] = s[d1]; p[d1] <= e[d1]; ++p[d1])
+                for (p[d2] = s[d2]; p[d2] <= e[d2]; ++p[d2])
+                    for (unsigned pl : {s[dir], e[dir]}) {
+                        p[dir] = pl;
+                        if (p[d1] < e[d1]) {
+                            coeff[0][idx(d1, p[0], p[1], p[2])] = 0;
+                            coeff[1][idx(d1, p[0], p[1], p[2])] = 0;
+                        }
+                        if (p[d2] < e[d2]) {
+                            coeff[0][idx(d2, p[0], p[1], p[2])] = 0;
+                            coeff[1][idx(d2, p[0], p[1], p[2])] = 0;
+                        }
+                    }
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1356,6 +1627,22 @@ int emsop_add_material(emsop* h, const double eps[3], const double mue[3], const
     b.order = next_order(h);
     b.metal = false;
     h->mats.push_back(b);
+    return 0;
+}
+
+int emsop_set_dispersion(emsop* h, unsigned norder, const double* params)
+{
+    if (!h || (norder && !params)) return fail(EMSOP_E_ARG, "null argument");
+    if (h->mats.empty() || h->mats.back().metal || h->mats.back().grid >= 0)
+        return fail(EMSOP_E_STATE, "dispersion needs a material box added just before");
+    if (norder == 0 || norder > 64) return fail(EMSOP_E_ARG, "dispersion order must be 1..64");
+    Dispersion d;
+    d.norder = norder;
+    d.p.assign(params, params + (size_t)norder * 18);
+    for (double v : d.p)
+        if (!(v >= 0)) return fail(EMSOP_E_ARG, "dispersion frequencies and times must be >= 0");
+    h->mats.back().disp = (int)h->disps.size();
+    h->disps.push_back(std::move(d));
     return 0;
 }
 
@@ -1610,6 +1897,38 @@ int emsop_upml_get(const emsop* h, unsigned k, unsigned start[3], unsigned num[3
     }
     if (arrays)
         for (int a = 0; a < 6; ++a) arrays[a] = s.a[a].data();
+    return 0;
+}
+
+unsigned emsop_lorentz_orders(const emsop* h) { return h && h->built ? (unsigned)h->lor.size() : 0; }
+
+int emsop_lorentz_get(const emsop* h, unsigned order, unsigned* count, int on[4], const unsigned* pos[3],
+                      const float* tabs[18])
+{
+    if (!h || !count || !on || !pos || !tabs) return fail(EMSOP_E_ARG, "null argument");
+    if (!h->built) return fail(EMSOP_E_STATE, "operator not built");
+    if (order >= h->lor.size()) return fail(EMSOP_E_ARG, "dispersion order out of range");
+    const LorOrder& T = h->lor[order];
+    *count = (unsigned)T.pos[0].size();
+    on[0] = T.von;
+    on[1] = T.ion;
+    on[2] = T.vlon;
+    on[3] = T.ilon;
+    for (int n = 0; n < 3; ++n) pos[n] = T.pos[n].data();
+    for (int k = 0; k < 6; ++k)
+        for (int n = 0; n < 3; ++n) tabs[3 * k + n] = T.t[k][n].data();
+    return 0;
+}
+
+unsigned emsop_rlc_count(const emsop* h) { return h && h->built ? (unsigned)h->rlc.dir.size() : 0; }
+
+int emsop_rlc_get(const emsop* h, const unsigned* pos[3], const int** dir, const float* coef[9])
+{
+    if (!h || !pos || !dir || !coef) return fail(EMSOP_E_ARG, "null argument");
+    if (!h->built) return fail(EMSOP_E_STATE, "operator not built");
+    for (int n = 0; n < 3; ++n) pos[n] = h->rlc.pos[n].data();
+    *dir = h->rlc.dir.data();
+    for (int k = 0; k < 9; ++k) coef[k] = h->rlc.c[k].data();
     return 0;
 }
 

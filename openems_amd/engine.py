@@ -69,12 +69,13 @@ class EngineHIP:
         SSE-compressed form of the same operator, used instead of op's arrays;
         steady_state: optional (period_steps, dirs, positions) of a
         steady-state detection (Engine_Ext_SteadyState);
-        lorentz: optional list of dispersion orders (Engine_Ext_LorentzMaterial
-        tables), each a dict pos [3][K], v_int, v_ext, v_lor (or None), i_int,
-        i_ext, i_lor [3][K], volt, curr (bool);
-        rlc: optional lumped RLC table (Engine_Ext_LumpedRLC), a dict pos
-        [3][K], dir [K] and the nine coefficient arrays i2v, ilv, vvd, vv2,
-        vj1, vj2, ib0, b1, b2 [K];
+        lorentz: dispersion orders (Engine_Ext_LorentzMaterial tables), each a
+        dict pos [3][K], v_int, v_ext, v_lor (or None), i_int, i_ext, i_lor
+        [3][K], volt, curr (bool); None takes the operator's own
+        (Operator.lorentz_tables);
+        rlc: lumped RLC table (Engine_Ext_LumpedRLC), a dict pos [3][K], dir
+        [K] and the nine coefficient arrays i2v, ilv, vvd, vv2, vj1, vj2, ib0,
+        b1, b2 [K]; None takes the operator's own (Operator.rlc_table);
         fused: step form, None (automatic), False or "halves" (two half
         steps), True or "lds" (single sweep staging planes in LDS, k_step) or
         "columns" (single sweep of independent columns, k_step2); a required
@@ -106,8 +107,13 @@ class EngineHIP:
             check(L.ems_set_fused(h, STEP_FORMS[fused] if isinstance(fused, str) else (1 if fused else 0)))
         if steady_state is not None:
             self.set_steady_state(*steady_state)
+        # the operator's own Lorentz / lumped RLC tables unless tables are given
+        if lorentz is None and hasattr(op, "lorentz_tables"):
+            lorentz = [o for o in op.lorentz_tables() if o["pos"].shape[1] > 0 and (o["volt"] or o["curr"])]
+        if rlc is None and hasattr(op, "rlc_table"):
+            rlc = op.rlc_table()
         self._lorentz = lorentz or []
-        self._rlc = rlc
+        self._rlc = rlc if rlc else None
         try:
             self._extensions()
         except Exception:

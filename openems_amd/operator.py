@@ -38,6 +38,22 @@ def parse_bc(bc):
     return types, sizes
 
 
+RLC_COEFS = ("i2v", "ilv", "vvd", "vv2", "vj1", "vj2", "ib0", "b1", "b2")
+DISPERSION_KEYS = ("eps_plasma", "eps_relax", "eps_pole", "mue_plasma", "mue_relax", "mue_pole")
+
+
+def dispersion_params(orders):
+    """[norder][6][3] float64 of a material's "lorentz" entry: a list with one
+    dict per dispersion order holding eps_plasma, eps_relax, eps_pole,
+    mue_plasma, mue_relax, mue_pole (plasma and pole frequencies in Hz,
+    relaxation times in s; scalars or per-direction triples, 0 = unset)."""
+    out = np.zeros((len(orders), 6, 3), np.float64)
+    for o, d in enumerate(orders):
+        for k, key in enumerate(DISPERSION_KEYS):
+            out[o, k] = _v3(d.get(key, 0.0))
+    return np.ascontiguousarray(out)
+
+
 class Operator:
     """Builds the operator of a scene dict (see openems_amd.scenes)."""
 
@@ -86,6 +102,9 @@ class Operator:
             st, sp = _v3(m["start"]), _v3(m["stop"])
             check_op(L.emsop_add_material(self.h, *[ptr(a, C.c_double) for a in arrs], int(m.get("priority", 0)),
                                           ptr(st, C.c_double), ptr(sp, C.c_double)))
+            if m.get("lorentz"):
+                prm = dispersion_params(m["lorentz"])
+                check_op(L.emsop_set_dispersion(self.h, prm.shape[0], ptr(prm, C.c_double)))
         for m in s.get("metals", []):
             st, sp = _v3(m["start"]), _v3(m["stop"])
             check_op(L.emsop_add_metal(self.h, int(m.get("priority", 0)), ptr(st, C.c_double), ptr(sp, C.c_double)))
@@ -111,6 +130,49 @@ class Operator:
                                       float(s.get("ts_factor", 1.0))))
         check_op(L.emsop_set_cell_constant_material(self.h, int(bool(s.get("cell_constant_material", False)))))
         check_op(L.emsop_set_threads(self.h, int(nthreads)))
+
+    # ---- dispersive materials and lumped RLC elements -----------------------------
+    def lorentz_tables(self):
+        """Operator_Ext_LorentzMaterial tables, one dict per dispersion order in
+        the form EngineHIP(lorentz=...) takes (empty without dispersive materials)."""
+        L = lib()
+        out = []
+        for o in range(L.emsop_lorentz_orders(self.h)):
+            cnt = C.c_uint()
+            on = (C.c_int * 4)()
+            pos = (C.POINTER(C.c_uint) * 3)()
+            tabs = (C.POINTER(C.c_float) * 18)()
+            check_op(L.emsop_lorentz_get(self.h, o, C.byref(cnt), on, pos, tabs))
+            K = cnt.value
+
+            def arr(k):
+                return np.array([np.ctypeslib.as_array(tabs[3 * k + n], (K,)) if K else np.zeros(0, np.float32)
+                                 for n in range(3)], np.float32)
+            d = {"pos": np.array([np.ctypeslib.as_array(pos[n], (K,)) if K else np.zeros(0, np.uint32)
+                                  for n in range(3)], np.uint32),
+                 "volt": bool(on[0]), "curr": bool(on[1])}
+            d["v_int"], d["v_ext"] = (arr(0), arr(1)) if on[0] else (np.ones((3, K), np.float32), np.zeros((3, K), np.float32))
+            d["i_int"], d["i_ext"] = (arr(2), arr(3)) if on[1] else (np.ones((3, K), np.float32), np.zeros((3, K), np.float32))
+            d["v_lor"] = arr(4) if on[2] else None
+            d["i_lor"] = arr(5) if on[3] else None
+            out.append(d)
+        return out
+
+    def rlc_table(self):
+        """Operator_Ext_LumpedRLC table in the form EngineHIP(rlc=...) takes, or None."""
+        L = lib()
+        K = L.emsop_rlc_count(self.h)
+        if K == 0:
+            return None
+        pos = (C.POINTER(C.c_uint) * 3)()
+        dr = C.POINTER(C.c_int)()
+        coef = (C.POINTER(C.c_float) * 9)()
+        check_op(L.emsop_rlc_get(self.h, pos, C.byref(dr), coef))
+        d = {"pos": np.array([np.ctypeslib.as_array(pos[n], (K,)) for n in range(3)], np.uint32),
+             "dir": np.ctypeslib.as_array(dr, (K,)).astype(np.int32)}
+        for k, name in enumerate(RLC_COEFS):
+            d[name] = np.ctypeslib.as_array(coef[k], (K,)).astype(np.float32)
+        return d
 
     # ---- scalars ------------------------------------------------------------
     @property

@@ -120,6 +120,59 @@ def upml_bricks(n=256, nrts=10000, nbricks=40, seed=20260202, pml=8, dump=True):
     return scene
 
 
+def dispersive_rlc(n=40, nrts=300, pml=6):
+    """Graded n^3-line mesh with UPML, a Drude box reaching into the UPML, a
+    two-order Lorentz box (eps and mue poles), a series RLC element, a
+    parallel RLC element on the edges' own capacitance and a parallel LC with
+    R = 0, driven by a soft E_z Gaussian (fc = 20 GHz)."""
+    lines = [graded_lines(n, 0.5, max(1, n // 16), 1.2) for _ in range(3)]
+    L = lines
+    c = n // 2
+
+    def box(i0, i1):
+        return tuple(L[a][i0[a]] for a in range(3)), tuple(L[a][i1[a]] for a in range(3))
+
+    s_drude, e_drude = box((2, c - 6, c - 6), (c - 8, c + 6, c + 6))
+    s_lor, e_lor = box((c + 3, c - 5, c - 4), (c + 9, c + 5, c + 4))
+    mats = [
+        {"eps": 1.5, "kappa": 0.01, "priority": 2, "start": s_drude, "stop": e_drude,
+         "lorentz": [{"eps_plasma": 18e9, "eps_relax": 2e-11}]},
+        {"eps": 2.0, "mue": 1.2, "priority": 3, "start": s_lor, "stop": e_lor,
+         "lorentz": [{"eps_plasma": 12e9, "eps_relax": 5e-11, "eps_pole": 9e9},
+                     {"eps_plasma": (6e9, 7e9, 8e9), "mue_plasma": 10e9, "mue_relax": 4e-11, "mue_pole": 8e9}]},
+        {"eps": 4.0, "priority": 1, "start": box((c - 4, c + 4, c - 6), (c + 2, c + 9, c + 3))[0],
+         "stop": box((c - 4, c + 4, c - 6), (c + 2, c + 9, c + 3))[1]},
+    ]
+    s_ser, e_ser = box((c - 2, c - 2, c - 3), (c - 1, c - 1, c + 1))
+    s_par, e_par = box((c + 1, c - 3, c - 2), (c + 2, c - 3, c + 2))
+    s_lc, e_lc = box((c - 5, c + 1, c - 1), (c - 5, c + 3, c - 1))
+    scene = {
+        "name": f"dispersive_rlc_{n}",
+        "lines": lines,
+        "delta_unit": 1e-3,
+        "bc": [3] * 6,
+        "pml_size": [pml] * 6,
+        "materials": mats,
+        "lumped": [
+            {"dir": 2, "R": 50.0, "C": 1e-11, "L": 1e-10, "caps": True, "type": "series", "priority": 0,
+             "start": s_ser, "stop": e_ser},
+            {"dir": 2, "R": 100.0, "C": 0.0, "L": 2e-9, "caps": False, "type": "parallel", "priority": 0,
+             "start": s_par, "stop": e_par},
+            {"dir": 1, "R": 0.0, "C": 2e-13, "L": 5e-10, "caps": False, "type": "parallel", "priority": 0,
+             "start": s_lc, "stop": e_lc},
+        ],
+        "excitations": [{"type": 0, "val": (0.0, 0.0, 1.0), "start": (L[0][c - 3], L[1][c + 2], L[2][c - 2]),
+                         "stop": (L[0][c - 3], L[1][c + 2], L[2][c + 2]), "priority": 0}],
+        "signal": {"type": "gauss", "f0": 0.0, "fc": 20e9},
+        "nrts": nrts,
+        "oversampling": 4,
+        "probes": [_probe(0, (L[0][c - 2], L[1][c - 2], L[2][c - 3]), (L[0][c - 2], L[1][c - 2], L[2][c + 1]),
+                          name="ser_ut")],
+        "dumps": [],
+    }
+    return scene
+
+
 def mur_layers(nx=1024, ny=1024, nz=768, nrts=4000, seed=20260203, f0=15e9):
     """Layered dielectric, Mur ABC on every face, sinusoidal E_x sheet source."""
     d = 0.25

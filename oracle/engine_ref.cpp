@@ -343,7 +343,7 @@ struct LorentzExt : ExtT<T, Eng> {
                     const auto& cint = CURR ? i_int[n][o] : v_int[n][o];
                     const auto& cext = CURR ? i_ext[n][o] : v_ext[n][o];
                     T f = CURR ? (T)e.GetCurr(n, x, y, z) : (T)e.GetVolt(n, x, y, z);
-                    if (has_lorentz[o]) {
+                    if (has_lorentz[o] & (CURR ? 2 : 1)) {
                         // engine_ext_lorentzmaterial.cpp:89-97: Lor first, then ADE
                         const auto& clor = CURR ? i_Lor[n][o] : v_Lor[n][o];
                         lor[k] += clor[k] * ade[k];
@@ -987,7 +987,8 @@ void ref_add_mur(void* hh, int ny, unsigned line, unsigned shift, const float* c
 
 // Lorentz/Drude ADE lists.  Per order o: count[o] cells, positions pos[3][count],
 // coefficient arrays [3][count] (v_int, v_ext, v_lor, i_int, i_ext, i_lor);
-// flags: has_lorentz, has_volt, has_curr.
+// flags: has_lorentz (bit 0: voltage Lorentz pole, bit 1: current Lorentz pole,
+// m_volt_Lor_ADE_On / m_curr_Lor_ADE_On), has_volt, has_curr.
 void ref_add_lorentz(void* hh, unsigned order, const unsigned* count, const unsigned* const* pos,
                      const float* const* v_int, const float* const* v_ext, const float* const* v_lor,
                      const float* const* i_int, const float* const* i_ext, const float* const* i_lor,
@@ -1015,10 +1016,8 @@ void ref_add_lorentz(void* hh, unsigned order, const unsigned* count, const unsi
                 L->v_ext[d][o].assign(v_ext[o] + (size_t)d * c, v_ext[o] + (size_t)(d + 1) * c);
                 L->i_int[d][o].assign(i_int[o] + (size_t)d * c, i_int[o] + (size_t)(d + 1) * c);
                 L->i_ext[d][o].assign(i_ext[o] + (size_t)d * c, i_ext[o] + (size_t)(d + 1) * c);
-                if (has_lorentz[o]) {
-                    L->v_Lor[d][o].assign(v_lor[o] + (size_t)d * c, v_lor[o] + (size_t)(d + 1) * c);
-                    L->i_Lor[d][o].assign(i_lor[o] + (size_t)d * c, i_lor[o] + (size_t)(d + 1) * c);
-                }
+                if (has_lorentz[o] & 1) L->v_Lor[d][o].assign(v_lor[o] + (size_t)d * c, v_lor[o] + (size_t)(d + 1) * c);
+                if (has_lorentz[o] & 2) L->i_Lor[d][o].assign(i_lor[o] + (size_t)d * c, i_lor[o] + (size_t)(d + 1) * c);
                 L->volt_ADE[d][o].assign(c, 0); L->curr_ADE[d][o].assign(c, 0);
                 L->volt_Lor[d][o].assign(c, 0); L->curr_Lor[d][o].assign(c, 0);
             }

@@ -27,6 +27,8 @@ FDTD/excitation.cpp:150-276 signals          :func:`excitation_signal`
 operator_ext_excitation.cpp:105-297          :meth:`RefOperator.build_excitation`
 operator_ext_upml.cpp:69-445                 :meth:`RefOperator.build_upml`
 operator_ext_mur_abc.cpp:80-186              :meth:`RefOperator.build_mur`
+operator_ext_lorentzmaterial.cpp:120-445     :meth:`RefOperator.build_lorentz`
+operator_ext_lumpedRLC.cpp:112-548           :meth:`RefOperator.build_rlc`
 ==========================================  ===========================================
 
 CSXCAD is not available; its point-in-box / priority semantics are restated
@@ -146,6 +148,20 @@ def _vec3(v):
     if np.ndim(v) == 0:
         return (float(v),) * 3
     return tuple(float(x) for x in v)
+
+
+_DISP_KEYS = ("eps_plasma", "eps_relax", "eps_pole", "mue_plasma", "mue_relax", "mue_pole")
+
+
+def _dispersion(orders):
+    """CSPropLorentzMaterial parameters as [order][k][n] (k in _DISP_KEYS), or None."""
+    if not orders:
+        return None
+    out = np.zeros((len(orders), 6, 3))
+    for o, d in enumerate(orders):
+        for k, key in enumerate(_DISP_KEYS):
+            out[o, k] = _vec3(d.get(key, 0.0))
+    return out
 
 
 class BoxPainter:
@@ -354,7 +370,7 @@ class RefOperator:
                 continue
             payload = {"eps": _vec3(m.get("eps", 1.0)), "mue": _vec3(m.get("mue", 1.0)),
                        "kappa": _vec3(m.get("kappa", 0.0)), "sigma": _vec3(m.get("sigma", 0.0)),
-                       "metal": False}
+                       "metal": False, "disp": _dispersion(m.get("lorentz"))}
             mats.append((m.get("priority", 0), k, m["start"], m["stop"], payload))
         off = len(scene.get("materials", []))
         for k, m in enumerate(scene.get("metals", [])):
@@ -759,6 +775,198 @@ class RefOperator:
                                 vv[(nyPP,) + tuple(pos)] = 0
                                 vi[(nyPP,) + tuple(pos)] = 0
 
+    # -- dispersive materials and lumped RLC --------------------------------------
+    def build_lorentz(self, C, L, vi, iv, dT):
+        """Operator_Ext_LorentzMaterial::BuildExtension
+        (operator_ext_lorentzmaterial.cpp:120-445), Lorentz / Drude materials
+        (no Debye): per dispersion order, the cells with an active component
+        in x -> y -> z order and their float32 tables; the material of an edge
+        is the MATERIAL|METAL winner at its Yee point (GetYeeCoords,
+        operator.cpp:182-202).  Returns a list of dicts (pos, v_int, v_ext,
+        i_int, i_ext, v_lor, i_lor, volt, curr, volt_lor, curr_lor)."""
+        boxes = self.pec_painter.boxes
+        order_max = max([b[4]["disp"].shape[0] for b in boxes if b[4].get("disp") is not None] + [0])
+        if order_max == 0:
+            return []
+        N = self.N
+        # per component and side: winner's dispersion parameters [k][x][y][z] for an order
+        win = {}
+        for n in range(3):
+            win[(n, False)] = self.pec_painter.paint_index(self.yee_coords(n, False))
+            win[(n, True)] = self.pec_painter.paint_index(self.yee_coords(n, True))
+
+        def param(n, dual, o, k):
+            w = win[(n, dual)]
+            out = np.zeros(w.shape)
+            for bi, b in enumerate(boxes):
+                d = b[4].get("disp")
+                if d is None or o >= d.shape[0]:
+                    continue
+                out[w == bi] = d[o, k, n]
+            return out
+
+        idx = [np.arange(N[a]) for a in range(3)]
+        X, Y, Z = np.meshgrid(*idx, indexing="ij")
+        P = (X, Y, Z)
+        out = []
+        for o in range(order_max):
+            LD, RD, CL, CD, GD, LL = ([np.zeros(N) for _ in range(3)] for _ in range(6))
+            on = np.zeros(N, bool)
+            flags = [False] * 4
+            for n in range(3):
+                nP, nPP = (n + 1) % 3, (n + 2) % 3
+                # voltages: primary edge centre
+                ok = (P[n] < N[n] - 1) & (vi[n] != 0)
+                wp = param(n, False, o, 0) * 2 * PI
+                Cn = C[n].astype(np.float64)
+                a = ok & (wp > 0) & (Cn > 0)
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    LD[n] = np.where(a, 1 / (wp * wp * Cn), 0.0)
+                    tr = param(n, False, o, 1)
+                    RD[n] = np.where(ok & (tr > 0) & (LD[n] > 0), LD[n] / tr, 0.0)
+                    wl = param(n, False, o, 2) * 2 * PI
+                    b = ok & (wl > 0) & (LD[n] > 0)
+                    CL[n] = np.where(b, 1 / (wl * wl * LD[n]), 0.0)
+                on |= a
+                flags[0] |= bool(a.any())
+                flags[2] |= bool(b.any())
+                # currents: dual edge centre
+                ok = (P[nP] < N[nP] - 1) & (P[nPP] < N[nPP] - 1) & (iv[n] != 0)
+                wp = param(n, True, o, 3) * 2 * PI
+                Ln = L[n].astype(np.float64)
+                a = ok & (wp > 0) & (Ln > 0)
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    CD[n] = np.where(a, 1 / (wp * wp * Ln), 0.0)
+                    tr = param(n, True, o, 4)
+                    GD[n] = np.where(ok & (tr > 0) & (CD[n] > 0), CD[n] / tr, 0.0)
+                    wl = param(n, True, o, 5) * 2 * PI
+                    b = ok & (wl > 0) & (CD[n] > 0)
+                    LL[n] = np.where(b, 1 / (wl * wl * CD[n]), 0.0)
+                on |= a
+                flags[1] |= bool(a.any())
+                flags[3] |= bool(b.any())
+            sel = np.nonzero(on)                       # x -> y -> z order
+            t = {k: np.zeros((3, len(sel[0])), np.float32) for k in
+                 ("v_int", "v_ext", "i_int", "i_ext", "v_lor", "i_lor")}
+            for n in range(3):
+                ld, rd, cl = LD[n][sel], RD[n][sel], CL[n][sel]
+                cd, gd, ll = CD[n][sel], GD[n][sel], LL[n][sel]
+                VI = vi[n][sel].astype(np.float64)
+                IV = iv[n][sel].astype(np.float64)
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    t["v_int"][n] = np.where(ld > 0, (2.0 * ld - dT * rd) / (2.0 * ld + dT * rd), 1.0)
+                    t["v_ext"][n] = np.where(ld > 0, dT / (ld + dT * rd / 2.0) * VI, 0.0)
+                    t["i_int"][n] = np.where(cd > 0, (2.0 * cd - dT * gd) / (2.0 * cd + dT * gd), 1.0)
+                    t["i_ext"][n] = np.where(cd > 0, dT / (cd + dT * gd / 2.0) * IV, 0.0)
+                    t["v_lor"][n] = np.where(cl > 0, dT / cl / VI, 0.0)
+                    t["i_lor"][n] = np.where(ll > 0, dT / ll / IV, 0.0)
+            out.append({"pos": np.array(sel, np.uint32), **t, "volt": flags[0], "curr": flags[1],
+                        "volt_lor": flags[2], "curr_lor": flags[3]})
+        return out
+
+    def build_rlc(self, C, G, L, R, vv, vi, ii, iv, dT):
+        """Operator_Ext_LumpedRLC::BuildExtension (operator_ext_lumpedRLC.cpp:112-548):
+        parallel elements with L > 0 and series elements in scene order; rewrites
+        the EC C / G values and coefficients of their edges.  Returns the table
+        (pos [3][K], dir [K], i2v, ilv, vvd, vv2, vj1, vj2, ib0, b1, b2 [K]) or None."""
+        Z_FACT = 20.0                         # LUMPED_RLC_Z_FACT (operator_ext_lumpedRLC.h:29)
+        sig = self.signal
+        fMax = float(sig.get("f0", 0.0)) + float(sig.get("fc", 0.0))
+        m = self.mesh
+        rows = []
+        for le in self.scene.get("lumped", []):
+            parallel = le.get("type", "parallel") == "parallel"
+            Lr = float(le.get("L", 0.0))
+            Cv, Rv, Lv = float(le.get("C", 0.0)), float(le.get("R", -1.0)), Lr
+            Cv = 0.0 if (Cv < 0.0 or math.isnan(Cv)) else Cv
+            Rv = 0.0 if (Rv < 0.0 or math.isnan(Rv)) else Rv
+            Lv = 0.0 if (Lv < 0.0 or math.isnan(Lv)) else Lv
+            if not ((parallel and Lr > 0.0) or not parallel):     # IsLElumpedRLC (:550-561)
+                continue
+            d = int(le["dir"])
+            d1, d2 = (d + 1) % 3, (d + 2) % 3
+            dim, s, e = m.snap_box(le["start"], le["stop"], False)
+            if dim <= 0 or s[d] == e[d]:
+                continue
+            n0, n1, n2 = e[d] - s[d], e[d1] - s[d1] + 1, e[d2] - s[d2] + 1
+            Npar = n1 * n2
+            dL = Lv * Npar / n0
+            dR = Rv * Npar / n0
+            with np.errstate(divide="ignore"):
+                dG = (1 / Rv if Rv != 0 else math.inf) * n0 / Npar
+            dC = Cv * n0 / Npar
+            with np.errstate(divide="ignore", invalid="ignore"):
+                den = 4.0 * dL * dC + 2.0 * dT * dR * dC + dT * dT
+                ib0 = 2.0 * dT * dC / den if den != 0 else math.inf
+                b1 = (dT * dT - 4.0 * dL * dC) / (dT * dC) if dC != 0 else math.nan
+                b2 = (4.0 * dL * dC - 2.0 * dT * dR * dC + dT * dT) / (2.0 * dT * dC) if dC != 0 else math.nan
+            if not parallel and Cv == 0:
+                ib0 = dT / (2.0 * dL + dT * dR)
+                b1 = -4.0 * dL / dT
+                b2 = (2.0 * dL - dT * dR) / dT
+            if parallel and Rv == 0.0:
+                dG = 0.0
+            for p in range(s[d], e[d]):
+                for q in range(s[d1], e[d1] + 1):
+                    for r in range(s[d2], e[d2] + 1):
+                        pos = [0, 0, 0]
+                        pos[d], pos[d1], pos[d2] = p, q, r
+                        ix = (d,) + tuple(pos)
+                        if parallel:
+                            if dC > 0:
+                                C[ix] = np.float32(dC)
+                            else:
+                                dC = float(C[ix])
+                            i2v = (dT / dC) / (1.0 + dT * dG / (2.0 * dC))
+                            if Rv >= 0:
+                                G[ix] = np.float32(dG)
+                            ilv = dT / dL if Lv > 0 else 0.0
+                            if dC == 0:
+                                Cd = float(C[ix])
+                                Zmin = max(dR, 2 * PI * fMax * dL)
+                                Zcd_min = 1.0 / (2.0 * PI * fMax * Cd)
+                                if Zcd_min < Z_FACT * Zmin:
+                                    C[ix] = np.float32(1.0 / (2 * PI * fMax * Zmin * Z_FACT))
+                            v = [i2v, ilv, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0]
+                        else:
+                            G[ix] = np.float32(0.0)
+                            Cd = np.float32(C[ix])
+                            # pow(v, 2) in the reference, folded by GCC to v * v
+                            X = 2 * PI * fMax * dL - 1.0 / (dC * 2 * PI * fMax) if dC else 2 * PI * fMax * dL
+                            Zmin = math.sqrt(dR * dR + X * X)
+                            Zcd_min = 1.0 / (2.0 * PI * fMax * float(Cd))
+                            if Zcd_min < Z_FACT * Zmin:
+                                Cd = np.float32(1.0 / (2 * PI * fMax * Zmin * Z_FACT))
+                                C[ix] = Cd
+                            Cd = float(Cd)
+                            v = [0.0, 0.0, 1.0 / (1.0 + 0.5 * dT * ib0 / Cd), 0.5 * dT * ib0 / Cd,
+                                 0.5 * dT * (b1 * ib0 - 1.0) / Cd, 0.5 * dT * b2 * ib0 / Cd, ib0, b1, b2]
+                        a, b = ec_to_coeff(np.array([C[ix]], np.float32), np.array([G[ix]], np.float32), dT)
+                        vv[ix], vi[ix] = a[0], b[0]
+                        a, b = ec_to_coeff(np.array([L[ix]], np.float32), np.array([R[ix]], np.float32), dT)
+                        ii[ix], iv[ix] = a[0], b[0]
+                        rows.append((pos, d, v))
+            if le.get("caps", True):
+                for q in range(s[d1], e[d1] + 1):
+                    for r in range(s[d2], e[d2] + 1):
+                        for p in (s[d], e[d]):
+                            pos = [0, 0, 0]
+                            pos[d], pos[d1], pos[d2] = p, q, r
+                            if q < e[d1]:
+                                vv[(d1,) + tuple(pos)] = 0
+                                vi[(d1,) + tuple(pos)] = 0
+                            if r < e[d2]:
+                                vv[(d2,) + tuple(pos)] = 0
+                                vi[(d2,) + tuple(pos)] = 0
+        if not rows:
+            return None
+        names = ("i2v", "ilv", "vvd", "vv2", "vj1", "vj2", "ib0", "b1", "b2")
+        out = {"pos": np.array([r[0] for r in rows], np.uint32).T.copy(),
+               "dir": np.array([r[1] for r in rows], np.int32)}
+        for k, nm in enumerate(names):
+            out[nm] = np.array([r[2][k] for r in rows], np.float64).astype(np.float32)
+        return out
+
     # -- extensions -----------------------------------------------------------
     def build_excitation(self, vv, vi, ii, iv, dT):
         """Operator_Ext_Excitation::BuildExtension (operator_ext_excitation.cpp:105-297),
@@ -1124,5 +1332,8 @@ class RefOperator:
         for f in self.mur:
             f.start_ts = 0 if f.maxdelay < 0 else f.maxdelay + length + 10
         self.upml = self.build_upml(vv, vi, ii, iv, dT)
+        # dispersive materials, then lumped RLC elements (openems.cpp:1236-1242)
+        self.lorentz = self.build_lorentz(C, L, vi, iv, dT)
+        self.rlc = self.build_rlc(C, G, L, R, vv, vi, ii, iv, dT)
         self.vv, self.vi, self.ii, self.iv = vv, vi, ii, iv
         return self

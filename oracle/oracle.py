@@ -150,7 +150,9 @@ class RefEngine:
             for k in ("v_int", "v_ext", "v_lor", "i_int", "i_ext", "i_lor"):
                 tabs[k] = [arr(o[k] if o.get(k) is not None else np.zeros((3, c), np.float32), np.float32)
                            for o, c in zip(lorentz, cnt)]
-            has_l = arr([int(o.get("v_lor") is not None) for o in lorentz], np.int32)
+            # Lorentz pole per side (m_volt_Lor_ADE_On / m_curr_Lor_ADE_On): bit 0 volt, bit 1 curr
+            has_l = arr([int(o.get("v_lor") is not None) | (int(o.get("i_lor") is not None) << 1)
+                         for o in lorentz], np.int32)
             has_v = arr([int(bool(o.get("volt", True))) for o in lorentz], np.int32)
             has_c = arr([int(bool(o.get("curr", True))) for o in lorentz], np.int32)
             self._keep += keep + [cnt]

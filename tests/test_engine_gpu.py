@@ -400,3 +400,41 @@ def test_graph_chunks_with_mur_and_steady_state(gpu):
     assert sim.eng.graph()[2] > 0
     sim.close()
     ref.close()
+
+
+def _oracle_ade(ref):
+    """The oracle operator's Lorentz orders and RLC table in the engine's form."""
+    lor = []
+    for o in ref.lorentz:
+        if o["pos"].shape[1] == 0 or not (o["volt"] or o["curr"]):
+            continue
+        d = dict(o)
+        d["v_lor"] = o["v_lor"] if o["volt_lor"] else None
+        d["i_lor"] = o["i_lor"] if o["curr_lor"] else None
+        lor.append(d)
+    return lor, ref.rlc
+
+
+@pytest.mark.parametrize("layout", ["plain", "compressed"])
+def test_dispersive_and_rlc_scene_from_the_operator(gpu, layout):
+    """Drude / Lorentz materials and series / parallel RLC elements built by
+    the product's operator (Operator_Ext_LorentzMaterial,
+    Operator_Ext_LumpedRLC) and run by the engine: fields bit-identical to
+    the oracle engine fed by the oracle operator's own tables."""
+    scene = scenes.dispersive_rlc(40, 300)
+    print("case dispersive_rlc_40, 300 steps, layout", layout, flush=True)
+    op = Operator(scene)
+    eng = EngineHIP(op, layout=layout)
+    assert not eng.fused()[0]
+    eng.IterateTS(300)
+    ref_op = _ref_from_scene(scene)
+    lor, rlc = _oracle_ade(ref_op)
+    assert len(lor) == 2 and rlc is not None
+    ref = RefEngine(ref_op, "basic", lorentz=lor, rlc=rlc)
+    ref.iterate(300)
+    v, c, rv, rc = _fields_equal(eng, ref)
+    assert np.max(np.abs(rv)) > 0
+    assert np.array_equal(v, rv), f"voltage mismatches {np.count_nonzero(v != rv)}"
+    assert np.array_equal(c, rc), f"current mismatches {np.count_nonzero(c != rc)}"
+    eng.close()
+    ref.close()

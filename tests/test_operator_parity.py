@@ -138,3 +138,29 @@ def test_material_grid_matches_one_cell_boxes(native):
     assert dCn == pytest.approx(w[0], rel=1e-5)
     op2.close()
     op.close()
+
+
+def test_lorentz_and_lumped_rlc_tables(native):
+    """Operator_Ext_LorentzMaterial and Operator_Ext_LumpedRLC: the product's
+    tables and the operator they rewrite equal the oracle's, bit for bit
+    (Drude box reaching into the UPML, two-order Lorentz box with a mue pole,
+    series RLC, parallel RLC on the edges' own capacitance, LC with R = 0)."""
+    scene = scenes.dispersive_rlc()
+    op, ref = _compare(scene)
+    lt = op.lorentz_tables()
+    assert len(lt) == len(ref.lorentz) == 2
+    for a, b in zip(lt, ref.lorentz):
+        assert np.array_equal(a["pos"], b["pos"])
+        assert (a["volt"], a["curr"], a["v_lor"] is not None, a["i_lor"] is not None) == \
+            (b["volt"], b["curr"], b["volt_lor"], b["curr_lor"])
+        for k in ("v_int", "v_ext", "i_int", "i_ext", "v_lor", "i_lor"):
+            if a[k] is not None:
+                assert np.array_equal(a[k], b[k]), k
+    assert lt[0]["pos"].shape[1] > 0 and lt[1]["curr"]
+    rt = op.rlc_table()
+    assert np.array_equal(rt["pos"], ref.rlc["pos"]) and np.array_equal(rt["dir"], ref.rlc["dir"])
+    for k in ("i2v", "ilv", "vvd", "vv2", "vj1", "vj2", "ib0", "b1", "b2"):
+        assert np.array_equal(rt[k], ref.rlc[k]), k
+    # series: 4 z edges x 2 x 2 columns; parallel: 4 z edges x 2 x 1; LC: 2 y edges
+    assert rt["dir"].size == 16 + 8 + 2
+    assert rt["dir"].tolist() == [2] * 24 + [1] * 2

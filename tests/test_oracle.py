@@ -245,3 +245,148 @@ def test_steady_state_probe_set_and_convergence():
     assert all(b <= a for a, b in zip(ds[len(ds) // 2:], ds[len(ds) // 2 + 1:]))   # settling
     ref.close()
     op.close()
+
+
+def _drude_scene(tau=2e-11, fp=15e9, pole=0.0, mue=False):
+    """Uniform 20^3 PEC box with one dispersive box."""
+    l = np.arange(20, dtype=np.float64)
+    d = {"eps_plasma": fp, "eps_relax": tau, "eps_pole": pole}
+    if mue:
+        d.update({"mue_plasma": fp, "mue_relax": tau, "mue_pole": pole})
+    return {"lines": [l, l.copy(), l.copy()], "delta_unit": 1e-3, "bc": [0] * 6,
+            "materials": [{"eps": 2.0, "mue": 1.5, "priority": 1, "start": (5, 5, 5), "stop": (12, 11, 13),
+                           "lorentz": [d]}],
+            "excitations": [{"type": 0, "val": (0, 0, 1.0), "start": (3, 3, 3), "stop": (3, 3, 4)}],
+            "signal": {"type": "gauss", "f0": 0.0, "fc": 10e9}, "nrts": 100}
+
+
+def test_drude_coefficients_closed_form():
+    """Drude order (plasma frequency wp, relaxation time tau) on an edge of
+    capacitance C: L_D = 1/(wp^2 C), R_D = L_D/tau, so
+    v_int = (2 - dT/tau)/(2 + dT/tau), independent of C, and
+    v_ext = dT wp^2 C / (1 + dT/(2 tau)) * vi; the Lorentz pole wl adds
+    v_lor = dT wl^2 / (wp^2 C vi).  Same on the current side with L, iv."""
+    tau, fp, pole = 2e-11, 15e9, 9e9
+    ref = RefOperator(_drude_scene(tau, fp, pole, mue=True)).build()
+    (o,) = ref.lorentz
+    dT = ref.dT
+    C, G, L, R = ref.EC
+    wp, wl = 2 * np.pi * fp, 2 * np.pi * pole
+    x, y, z = o["pos"].astype(np.int64)
+    assert o["volt"] and o["curr"] and o["volt_lor"] and o["curr_lor"]
+    for n in range(3):
+        c = C[n][x, y, z].astype(np.float64)
+        vi = ref.vi[n][x, y, z].astype(np.float64)
+        act = (o["v_ext"][n] != 0)
+        assert act.sum() > 50
+        assert np.allclose(o["v_int"][n][act], (2 - dT / tau) / (2 + dT / tau), rtol=2e-7, atol=0)
+        want = dT * wp ** 2 * c / (1 + dT / (2 * tau)) * vi
+        assert np.allclose(o["v_ext"][n][act], want[act], rtol=1e-6, atol=0)
+        assert np.allclose(o["v_lor"][n][act], (dT * wl ** 2 / (wp ** 2 * c * vi))[act], rtol=1e-6, atol=0)
+        assert np.all(o["v_int"][n][~act] == 1) and np.all(o["v_lor"][n][~act] == 0)
+        lc = L[n][x, y, z].astype(np.float64)
+        iv = ref.iv[n][x, y, z].astype(np.float64)
+        ac = (o["i_ext"][n] != 0)
+        assert ac.sum() > 50
+        assert np.allclose(o["i_int"][n][ac], (2 - dT / tau) / (2 + dT / tau), rtol=2e-7, atol=0)
+        assert np.allclose(o["i_ext"][n][ac], (dT * wp ** 2 * lc / (1 + dT / (2 * tau)) * iv)[ac], rtol=1e-6, atol=0)
+        assert np.allclose(o["i_lor"][n][ac], (dT * wl ** 2 / (wp ** 2 * lc * iv))[ac], rtol=1e-6, atol=0)
+
+
+def test_dispersive_cells_are_those_inside_the_box():
+    """Brute force over the mesh: a cell is listed exactly when one of its
+    voltage edge centres (primary) or current edge centres (dual) lies in the
+    closed dispersive box with a non-zero coefficient; Drude only (no pole)."""
+    sc = _drude_scene()
+    ref = RefOperator(sc).build()
+    (o,) = ref.lorentz
+    assert not o["volt_lor"] and not o["curr"]
+    lo, hi = np.array(sc["materials"][0]["start"]), np.array(sc["materials"][0]["stop"])
+    N = ref.N
+    want = []
+    for x in range(N[0]):
+        for y in range(N[1]):
+            for z in range(N[2]):
+                on = False
+                for n in range(3):
+                    p = [x, y, z]
+                    if p[n] >= N[n] - 1 or ref.vi[n][x, y, z] == 0:
+                        continue
+                    pt = np.array(p, np.float64)
+                    pt[n] += 0.5
+                    on |= bool(np.all(pt >= lo) & np.all(pt <= hi))
+                if on:
+                    want.append((x, y, z))
+    assert [tuple(int(v) for v in c) for c in o["pos"].T] == want
+
+
+def _rlc_scene(kind, n0, R=20.0, L=2e-9, Cap=1e-12):
+    l = np.arange(16, dtype=np.float64)
+    return {"lines": [l, l.copy(), l.copy()], "delta_unit": 1e-3, "bc": [0] * 6,
+            "lumped": [{"dir": 2, "R": R, "C": Cap, "L": L, "caps": False, "type": kind,
+                        "start": (7, 7, 6), "stop": (7, 7, 6 + n0)}],
+            "excitations": [{"type": 0, "val": (0, 0, 1.0), "start": (3, 3, 3), "stop": (3, 3, 4)}],
+            "signal": {"type": "gauss", "f0": 0.0, "fc": 10e9}, "nrts": 100}
+
+
+def test_series_rlc_spread_over_edges():
+    """A series RLC over n0 edges in series: each edge carries L/n0, R/n0 and
+    C*n0, so L*C and R*C per edge stay those of the element.  Hence ib0
+    doubles and b1, b2 halve from one edge to two, and for one edge
+    ib0 = 2 dT C / (4 L C + 2 dT R C + dT^2)."""
+    a = RefOperator(_rlc_scene("series", 1)).build().rlc
+    b = RefOperator(_rlc_scene("series", 2)).build().rlc
+    assert a["dir"].size == 1 and b["dir"].size == 2
+    assert np.allclose(b["ib0"], 2 * a["ib0"][0], rtol=1e-6)
+    assert np.allclose(b["b1"], a["b1"][0] / 2, rtol=1e-6) and np.allclose(b["b2"], a["b2"][0] / 2, rtol=1e-6)
+    ref = RefOperator(_rlc_scene("series", 1)).build()
+    dT, R, L, Cap = ref.dT, 20.0, 2e-9, 1e-12
+    assert a["ib0"][0] == pytest.approx(2 * dT * Cap / (4 * L * Cap + 2 * dT * R * Cap + dT * dT), rel=1e-6)
+    assert a["b1"][0] == pytest.approx((dT * dT - 4 * L * Cap) / (dT * Cap), rel=1e-6)
+    assert np.all(a["i2v"] == 0) and np.all(a["ilv"] == 0)
+    # the edge's own conductance is cleared and its coefficients recomputed
+    assert ref.EC[1][2][7, 7, 6] == 0
+
+
+def test_parallel_rlc_conductance_and_inductance():
+    """Parallel RLC over one edge: C and G = 1/R replace the edge's own, so
+    vi = dT/C / (1 + dT/(2 R C)) and i2v equals it; ilv = dT/L."""
+    ref = RefOperator(_rlc_scene("parallel", 1)).build()
+    t = ref.rlc
+    dT = ref.dT
+    assert t["dir"].size == 1
+    want = (dT / 1e-12) / (1 + dT / (2 * 20.0 * 1e-12))
+    assert t["i2v"][0] == pytest.approx(want, rel=1e-6)
+    assert ref.vi[2][7, 7, 6] == pytest.approx(want, rel=1e-6)
+    assert t["ilv"][0] == pytest.approx(dT / 2e-9, rel=1e-6)
+    assert t["vvd"][0] == 1 and t["ib0"][0] == 0
+    # a parallel element without L is the main operator's R||C, not an RLC entry
+    assert RefOperator(_rlc_scene("parallel", 1, L=0.0)).build().rlc is None
+
+
+def test_lorentz_pole_flags_are_per_side():
+    """m_volt_Lor_ADE_On and m_curr_Lor_ADE_On are separate
+    (engine_ext_lorentzmaterial.cpp:86-173): an order with a current-side
+    pole and no voltage-side pole applies the pole to the currents only.
+    A zero voltage pole table equals none; dropping the current pole
+    changes the fields."""
+    sc = _drude_scene(mue=True)
+    sc["materials"][0]["lorentz"][0].update({"eps_pole": 0.0, "mue_pole": 9e9})
+    ref = RefOperator(sc).build()
+    (o,) = ref.lorentz
+    assert o["curr_lor"] and not o["volt_lor"]
+
+    def run(v_lor, i_lor):
+        d = dict(o)
+        d["v_lor"], d["i_lor"] = v_lor, i_lor
+        e = RefEngine(ref, "basic", lorentz=[d])
+        e.iterate(60)
+        v, c = (np.array(a) for a in e.raw_views())
+        e.close()
+        return v, c
+
+    base = run(None, o["i_lor"])
+    same = run(np.zeros_like(o["v_lor"]), o["i_lor"])
+    nopole = run(None, None)
+    assert np.array_equal(base[0], same[0]) and np.array_equal(base[1], same[1])
+    assert not np.array_equal(base[1], nopole[1])
