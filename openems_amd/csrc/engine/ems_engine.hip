@@ -266,6 +266,26 @@ __device__ __forceinline__ void pml_hooks_only(const float x0[3], const PmlArgs&
     }
 }
 
+// Field accesses of the cell being updated.  It is read and written once per
+// launch; on meshes whose fields exceed the last-level cache a non-temporal
+// access keeps the caches for the neighbour rows of the other field, which
+// adjacent columns and z chunks read again (256^3 PEC: 0.127 -> 0.103 ms per
+// half step).  On smaller meshes the fields stay cached from one half step to
+// the next, and plain accesses are faster.
+template <bool NT>
+__device__ __forceinline__ float ld_own(const float* q)
+{
+    if (NT) return __builtin_nontemporal_load(q);
+    return *q;
+}
+
+template <bool NT>
+__device__ __forceinline__ void st_own(float* q, float v)
+{
+    if (NT) __builtin_nontemporal_store(v, q);
+    else *q = v;
+}
+
 // ---------------------------------------------------------------------------
 // voltage half step (Engine::UpdateVoltages, FDTD/engine.cpp:110-168) with the
 // UPML pre/post hooks of Engine_Ext_UPML (engine_ext_upml.cpp:52-124) fused in:
@@ -276,7 +296,7 @@ __device__ __forceinline__ void pml_hooks_only(const float x0[3], const PmlArgs&
 // Each thread owns one (x, y) column and sweeps g.zdepth planes, carrying the
 // z-1 currents in registers.  In place.
 // ---------------------------------------------------------------------------
-template <int PM, bool CC>
+template <int PM, bool CC, bool NT>
 __device__ __forceinline__ void volt_tile(const GridDesc& g, unsigned t, float* V, const float* __restrict__ I,
                                           const CoefArgs& k, const PmlArgs& pml)
 {
@@ -308,21 +328,21 @@ __device__ __forceinline__ void volt_tile(const GridDesc& g, unsigned t, float* 
         Iy_zm = iy;
         float x0[3], a[3], b[3], out[3];
 #pragma unroll
-        for (int n = 0; n < 3; ++n) x0[n] = V[n * cs + p];
+        for (int n = 0; n < 3; ++n) x0[n] = ld_own<NT>(&V[n * cs + p]);
         load_coef<CC>(k, cs, p, CC ? k.idx[p] : 0u, a, b);
         unsigned long long li = 0;
         const bool in_pml = PM != kNoPml && pml_at(pml, (int)x, (int)y, (int)z, li);
         cell_math<PM>(x0, a, b, curl, in_pml, pml, li, true, out);
 #pragma unroll
-        for (int n = 0; n < 3; ++n) V[n * cs + p] = out[n];
+        for (int n = 0; n < 3; ++n) st_own<NT>(&V[n * cs + p], out[n]);
     }
 }
 
-template <int PM, bool CC>
+template <int PM, bool CC, bool NT>
 __global__ __launch_bounds__(kBlock) void k_volt(const GridDesc g, float* V, const float* __restrict__ I,
                                                  const CoefArgs k, const PmlArgs pml)
 {
-    volt_tile<PM, CC>(g, xcd_tile(blockIdx.x, gridDim.x), V, I, k, pml);
+    volt_tile<PM, CC, NT>(g, xcd_tile(blockIdx.x, gridDim.x), V, I, k, pml);
 }
 
 // ---------------------------------------------------------------------------
@@ -332,7 +352,7 @@ __global__ __launch_bounds__(kBlock) void k_volt(const GridDesc g, float* V, con
 // device timestep counter (++numTS of Engine::IterateTS): no kernel of this
 // launch reads it, and every later kernel sees the new value.
 // ---------------------------------------------------------------------------
-template <int PM, bool CC>
+template <int PM, bool CC, bool NT>
 __device__ __forceinline__ void curr_tile(const GridDesc& g, unsigned t, float* I, const float* __restrict__ V,
                                           const CoefArgs& k, const PmlArgs& pml)
 {
@@ -365,10 +385,10 @@ __device__ __forceinline__ void curr_tile(const GridDesc& g, unsigned t, float* 
             if (pml_at(pml, (int)x, (int)y, (int)z, li)) {
                 float x0[3], out[3];
 #pragma unroll
-                for (int n = 0; n < 3; ++n) x0[n] = I[n * cs + p];
+                for (int n = 0; n < 3; ++n) x0[n] = ld_own<NT>(&I[n * cs + p]);
                 pml_hooks_only<PM>(x0, pml, li, out);
 #pragma unroll
-                for (int n = 0; n < 3; ++n) I[n * cs + p] = out[n];
+                for (int n = 0; n < 3; ++n) st_own<NT>(&I[n * cs + p], out[n]);
             }
             continue;
         }
@@ -385,22 +405,22 @@ __device__ __forceinline__ void curr_tile(const GridDesc& g, unsigned t, float* 
         vy = vy_zp;
         float x0[3], a[3], b[3], out[3];
 #pragma unroll
-        for (int n = 0; n < 3; ++n) x0[n] = I[n * cs + p];
+        for (int n = 0; n < 3; ++n) x0[n] = ld_own<NT>(&I[n * cs + p]);
         load_coef<CC>(k, cs, p, CC ? k.idx[p] : 0u, a, b);
         unsigned long long li = 0;
         const bool in_pml = PM != kNoPml && pml_at(pml, (int)x, (int)y, (int)z, li);
         cell_math<PM>(x0, a, b, curl, in_pml, pml, li, true, out);
 #pragma unroll
-        for (int n = 0; n < 3; ++n) I[n * cs + p] = out[n];
+        for (int n = 0; n < 3; ++n) st_own<NT>(&I[n * cs + p], out[n]);
     }
 }
 
-template <int PM, bool CC>
+template <int PM, bool CC, bool NT>
 __global__ __launch_bounds__(kBlock) void k_curr(const GridDesc g, float* I, const float* __restrict__ V,
                                                  const CoefArgs k, const PmlArgs pml, unsigned* __restrict__ numts)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(numts, 1u);
-    curr_tile<PM, CC>(g, xcd_tile(blockIdx.x, gridDim.x), I, V, k, pml);
+    curr_tile<PM, CC, NT>(g, xcd_tile(blockIdx.x, gridDim.x), I, V, k, pml);
 }
 
 // Excitation entry as the fused step applies it: cell, component, amplitude
@@ -1258,9 +1278,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_half(const BatchInst* __restri
         const BatchInst& B = inst[it.x];
         if (CURR) {
             if (it.y == 0 && threadIdx.x == 0) atomicAdd(B.numts, 1u);
-            curr_tile<PM, CC>(B.g, it.y, B.I, B.V, B.kI, B.pI);
+            curr_tile<PM, CC, false>(B.g, it.y, B.I, B.V, B.kI, B.pI);
         } else {
-            volt_tile<PM, CC>(B.g, it.y, B.V, B.I, B.kV, B.pV);
+            volt_tile<PM, CC, false>(B.g, it.y, B.V, B.I, B.kV, B.pV);
         }
     }
 }
@@ -1810,6 +1830,7 @@ struct ems_engine {
     // fused single-sweep step
     int fused_req = -1;                   // -1 auto, 0 two half steps, 1 LDS sweep, 2 column sweep
     int fused = 0;                        // active form: 0 two half steps, 1 k_step, 2 k_step2
+    bool nt_fields = false;               // non-temporal own-field accesses in the half steps
     unsigned ntx = 0, nty = 0, fzdepth = 0, fnzc = 0;
     ExcArgs eV{}, eI{};
     int pml_mode = kNoPml;
@@ -2016,6 +2037,9 @@ int ems_create_layout(ems_engine** out, const unsigned numLines[3], const float*
         if (v >= 1 && v <= 1024) g.zdepth = (unsigned)v;
     }
     set_tiles(g, 0);
+    // fields larger than the last-level cache (256 MB on MI355X) stream through it
+    h->nt_fields = 2ull * 3ull * h->cs * sizeof(float) > (192ull << 20);
+    if (const char* nt = std::getenv("EMS_NT_FIELDS")) h->nt_fields = std::atoi(nt) != 0;
     int rc = 0;
     auto bail = [&](int r) {
         ems_destroy(h);
@@ -2769,14 +2793,21 @@ namespace {
 template <int PM, bool CC>
 void launch_volt_t(ems_engine* h, unsigned nblk)
 {
-    hipLaunchKernelGGL((k_volt<PM, CC>), dim3(nblk), dim3(kBlock), 0, h->stream, h->g, h->V, h->I, h->kV, h->pmlV);
+    if (h->nt_fields)
+        hipLaunchKernelGGL((k_volt<PM, CC, true>), dim3(nblk), dim3(kBlock), 0, h->stream, h->g, h->V, h->I, h->kV, h->pmlV);
+    else
+        hipLaunchKernelGGL((k_volt<PM, CC, false>), dim3(nblk), dim3(kBlock), 0, h->stream, h->g, h->V, h->I, h->kV, h->pmlV);
 }
 
 template <int PM, bool CC>
 void launch_curr_t(ems_engine* h, unsigned nblk)
 {
-    hipLaunchKernelGGL((k_curr<PM, CC>), dim3(nblk), dim3(kBlock), 0, h->stream, h->g, h->I, h->V, h->kI, h->pmlI,
-                       h->numts_dev);
+    if (h->nt_fields)
+        hipLaunchKernelGGL((k_curr<PM, CC, true>), dim3(nblk), dim3(kBlock), 0, h->stream, h->g, h->I, h->V, h->kI,
+                           h->pmlI, h->numts_dev);
+    else
+        hipLaunchKernelGGL((k_curr<PM, CC, false>), dim3(nblk), dim3(kBlock), 0, h->stream, h->g, h->I, h->V, h->kI,
+                           h->pmlI, h->numts_dev);
 }
 
 template <bool CC>

@@ -128,8 +128,8 @@ def test_update_kernels_have_no_fused_or_packed_fp32(native):
         elif cur:
             kernels[cur].append(line)
     upd = {k: v for k, v in kernels.items() if re.search(r"k_(volt|curr|step|step2)I", k)}
-    # 3 UPML modes x 2 layouts x (2 half steps + 2 single-sweep forms)
-    assert len(upd) == 24
+    # 3 UPML modes x 2 layouts x (2 half steps x 2 field-access forms + 2 single-sweep forms)
+    assert len(upd) == 36
     bad = re.compile(r"\bv_(fma\w*_f32|fmac\w*_f32|mac_f32|mad_f32|pk_\w+_f32|fma_mix\w*|dot\w*)")
     for k, lines in upd.items():
         hits = [l.strip() for l in lines if bad.search(l)]
