@@ -189,11 +189,13 @@ __device__ __forceinline__ void load_coef(const CoefArgs& k, unsigned long long 
 //   plain: f = x0*cc + cv*curl;  out = f
 //   UPML : f = F*cc + cv*curl;   out = (c0*x0 - cfo*F) + cfn*f;  F <- f
 // (F: the UPML flux).  The UPML lanes add exec-masked loads of the flux and
-// of their coefficients or table index.
+// of their coefficients or table index.  main = false selects the UPML hooks
+// alone, out = (c0*x0 - cfo*F) + cfn*F, for a UPML cell the main update
+// skips (the caller then stores no flux).
 template <int PM>
 __device__ __forceinline__ void cell_math(const float x0[3], const float a[3], const float b[3], const float curl[3],
                                           bool pml, const PmlArgs& P, unsigned long long li, bool store_flux,
-                                          float out[3])
+                                          float out[3], bool main = true)
 {
     float k0[3], ko[3], kn[3], fl[3];
     if (PM != kNoPml && pml) {
@@ -223,8 +225,10 @@ __device__ __forceinline__ void cell_math(const float x0[3], const float a[3], c
         f = f + b[n] * curl[n];
         fnew[n] = f;
         if (PM != kNoPml && pml) {
+            // a UPML cell the main update skips takes the hooks alone:
+            // out = (c0*x0 - cfo*F) + cfn*F (pml_hooks_only)
             const float t = k0[n] * x0[n] - ko[n] * fl[n];
-            out[n] = t + kn[n] * f;
+            out[n] = t + kn[n] * (main ? f : fl[n]);
         } else {
             out[n] = f;
         }
@@ -366,37 +370,27 @@ __device__ __forceinline__ void curr_tile(const GridDesc& g, unsigned t, float* 
     const unsigned z1 = min(z0 + g.zdepth, PM == kNoPml ? g.Nz - 1 : g.Nz);
     if (z0 >= z1) return;
     const unsigned long long cs = g.cs;
-    const unsigned Nx = g.Nx;
+    // Cells the main update skips (last line of a direction) take the UPML
+    // hooks alone, on the same path as the main update: their neighbour
+    // loads read the cell itself (valid addresses, values unused), and the
+    // result is selected per lane, so a wave holding such a lane does not
+    // run a second, serialised branch.
+    const unsigned long long dx = col_upd ? 1ull : 0ull, dy = col_upd ? (unsigned long long)g.Nx : 0ull;
     unsigned long long p = x + (unsigned long long)g.Nx * y + (unsigned long long)g.plane * z0;
 
     const float* __restrict__ Vx = V;
     const float* __restrict__ Vy = V + cs;
     const float* __restrict__ Vz = V + 2 * cs;
     // values at z carried from the previous iteration's z+1 loads
-    float vx = 0.0f, vy = 0.0f;
-    if (col_upd) {
-        vx = Vx[p];
-        vy = Vy[p];
-    }
+    float vx = Vx[p], vy = Vy[p];
 
     for (unsigned z = z0; z < z1; ++z, p += g.plane) {
-        if (PM != kNoPml && !(col_upd && z + 1 < g.Nz)) {
-            unsigned long long li = 0;
-            if (pml_at(pml, (int)x, (int)y, (int)z, li)) {
-                float x0[3], out[3];
-#pragma unroll
-                for (int n = 0; n < 3; ++n) x0[n] = ld_own<NT>(&I[n * cs + p]);
-                pml_hooks_only<PM>(x0, pml, li, out);
-#pragma unroll
-                for (int n = 0; n < 3; ++n) st_own<NT>(&I[n * cs + p], out[n]);
-            }
-            continue;
-        }
-        const unsigned long long pzp = p + g.plane;
+        const bool main = col_upd && z + 1 < g.Nz;
+        const unsigned long long pzp = z + 1 < g.Nz ? p + g.plane : p;
         const float vz = Vz[p];
         const float vx_zp = Vx[pzp], vy_zp = Vy[pzp];
-        const float vz_yp = Vz[p + Nx], vx_yp = Vx[p + Nx];
-        const float vz_xp = Vz[p + 1], vy_xp = Vy[p + 1];
+        const float vz_yp = Vz[p + dy], vx_yp = Vx[p + dy];
+        const float vz_xp = Vz[p + dx], vy_xp = Vy[p + dx];
         float curl[3];
         curl[0] = vz - vz_yp - vy + vy_zp;
         curl[1] = vx - vx_zp - vz + vz_xp;
@@ -409,9 +403,11 @@ __device__ __forceinline__ void curr_tile(const GridDesc& g, unsigned t, float* 
         load_coef<CC>(k, cs, p, CC ? k.idx[p] : 0u, a, b);
         unsigned long long li = 0;
         const bool in_pml = PM != kNoPml && pml_at(pml, (int)x, (int)y, (int)z, li);
-        cell_math<PM>(x0, a, b, curl, in_pml, pml, li, true, out);
+        cell_math<PM>(x0, a, b, curl, in_pml, pml, li, main, out, main);
+        if (main || in_pml) {
 #pragma unroll
-        for (int n = 0; n < 3; ++n) st_own<NT>(&I[n * cs + p], out[n]);
+            for (int n = 0; n < 3; ++n) st_own<NT>(&I[n * cs + p], out[n]);
+        }
     }
 }
 
