@@ -70,6 +70,10 @@ int emsop_add_metal(emsop* h, int priority, const double start[3], const double 
  * relaxation time (s), eps Lorentz pole frequency (Hz), mue plasma
  * frequency, mue relaxation time, mue Lorentz pole frequency; 0 = unset. */
 int emsop_set_dispersion(emsop* h, unsigned norder, const double* params);
+/* Debye dispersion of the material box added last (CSPropDebyeMaterial, same
+ * builder): params[(o*2 + k)*3 + n], k = eps delta, relaxation time (s);
+ * a relaxation time below 2 dT leaves the order off for that edge. */
+int emsop_set_debye(emsop* h, unsigned norder, const double* params);
 /* After emsop_build: for the edge (component n, pos), the four cells of its
  * quarter-cell average (cells[3*q + axis], 0xFFFFFFFF for a cell before the
  * first line; q = up-right, up-left, down-right, down-left in the

@@ -58,6 +58,7 @@ OPERATOR_API = [
     ("emsop_upml_count", C.c_uint, [vp]),
     ("emsop_upml_get", C.c_int, [vp, C.c_uint, up, up, C.POINTER(fp)]),
     ("emsop_set_dispersion", C.c_int, [vp, C.c_uint, dp]),
+    ("emsop_set_debye", C.c_int, [vp, C.c_uint, dp]),
     ("emsop_lorentz_orders", C.c_uint, [vp]),
     ("emsop_lorentz_get", C.c_int, [vp, C.c_uint, up, ip, C.POINTER(up), C.POINTER(fp)]),
     ("emsop_rlc_count", C.c_uint, [vp]),

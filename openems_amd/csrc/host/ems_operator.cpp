@@ -84,6 +84,7 @@ struct MatBox : Box {
 // then the same three for mue.
 struct Dispersion {
     unsigned norder;
+    bool debye = false;   // CSPropDebyeMaterial: k = 0 eps delta, 1 eps relaxation time (s)
     std::vector<double> p;
     double at(unsigned o, int k, int n) const { return o < norder ? p[((size_t)o * 6 + k) * 3 + n] : 0.0; }
 };
@@ -1323,9 +1324,8 @@ int emsop::build(unsigned nrts)
 }
 
 // ---------------------------------------------------------------------------
-// Lorentz / Drude materials (Operator_Ext_LorentzMaterial::BuildExtension,
-// FDTD/extensions/operator_ext_lorentzmaterial.cpp:120-445; Debye materials
-// are not restated).  Per order, the cells with an active component in x ->
+// Lorentz / Drude and Debye materials (Operator_Ext_LorentzMaterial::
+// BuildExtension, FDTD/extensions/operator_ext_lorentzmaterial.cpp:120-445).  Per order, the cells with an active component in x ->
 // y -> z order; the material at an edge is the MATERIAL|METAL winner at its
 // Yee point (primary edge centre for voltages, dual edge centre for
 // currents).  Coefficients in double, stored as float.
@@ -1367,6 +1367,19 @@ void emsop::build_lorentz()
                                                 &mv[n][2][(size_t)p[2] * W]);
                         if (r < 0 || rk.box[r]->disp < 0) continue;
                         const Dispersion& D = disps[rk.box[r]->disp];
+                        if (D.debye) {
+                            // CSPropDebyeMaterial (operator_ext_lorentzmaterial.cpp:259-275)
+                            const double A = EL((n + 1) % 3, p[(n + 1) % 3], true) * EL((n + 2) % 3, p[(n + 2) % 3], true);
+                            CL[n] = 8.85418781762e-12 * D.at(o, 0, n) * A / EL(n, p[n], false);
+                            const double tr = D.at(o, 1, n);
+                            if (CL[n] > 0 && tr > 0 && tr > 2.0 * dT) {
+                                RD[n] = tr / CL[n];
+                                on = true;
+                                T.von = true;
+                                T.vlon = true;
+                            }
+                            continue;
+                        }
                         const double wp = D.at(o, 0, n) * 2 * kPI;
                         const double C = ec[0][q];
                         if (wp > 0 && C > 0) {
@@ -1392,6 +1405,7 @@ void emsop::build_lorentz()
                                                 &mi[n][2][(size_t)p[2] * W]);
                         if (r < 0 || rk.box[r]->disp < 0) continue;
                         const Dispersion& D = disps[rk.box[r]->disp];
+                        if (D.debye) continue;     // no current-side Debye term
                         const double wp = D.at(o, 3, n) * 2 * kPI;
                         const double L = ec[2][q];
                         if (wp > 0 && L > 0) {
@@ -1415,6 +1429,10 @@ void emsop::build_lorentz()
                         if (LD[n] > 0) {
                             acc[0][n].push_back((2.0 * LD[n] - dT * RD[n]) / (2.0 * LD[n] + dT * RD[n]));
                             acc[1][n].push_back(dT / (LD[n] + dT * RD[n] / 2.0) * VI);
+                        } else if (RD[n] > 0 && CL[n] > 0) {
+                            // Debye: R_D C_L = tau
+                            acc[0][n].push_back((2.0 * dT - RD[n] * CL[n]) / (CL[n] * RD[n]));
+                            acc[1][n].push_back(2.0 / RD[n] * VI);
                         } else {
                             acc[0][n].push_back(1);
                             acc[1][n].push_back(0);
@@ -1641,6 +1659,28 @@ int emsop_set_dispersion(emsop* h, unsigned norder, const double* params)
     d.p.assign(params, params + (size_t)norder * 18);
     for (double v : d.p)
         if (!(v >= 0)) return fail(EMSOP_E_ARG, "dispersion frequencies and times must be >= 0");
+    h->mats.back().disp = (int)h->disps.size();
+    h->disps.push_back(std::move(d));
+    return 0;
+}
+
+int emsop_set_debye(emsop* h, unsigned norder, const double* params)
+{
+    if (!h || !params) return fail(EMSOP_E_ARG, "null argument");
+    if (h->mats.empty() || h->mats.back().metal || h->mats.back().grid >= 0)
+        return fail(EMSOP_E_STATE, "dispersion needs a material box added just before");
+    if (norder == 0 || norder > 64) return fail(EMSOP_E_ARG, "dispersion order must be 1..64");
+    Dispersion d;
+    d.norder = norder;
+    d.debye = true;
+    d.p.assign((size_t)norder * 18, 0.0);
+    for (unsigned o = 0; o < norder; ++o)
+        for (int k = 0; k < 2; ++k)
+            for (int n = 0; n < 3; ++n) {
+                const double v = params[((size_t)o * 2 + k) * 3 + n];
+                if (!(v >= 0)) return fail(EMSOP_E_ARG, "Debye eps delta and relaxation time must be >= 0");
+                d.p[((size_t)o * 6 + k) * 3 + n] = v;
+            }
     h->mats.back().disp = (int)h->disps.size();
     h->disps.push_back(std::move(d));
     return 0;

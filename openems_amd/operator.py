@@ -54,6 +54,16 @@ def dispersion_params(orders):
     return np.ascontiguousarray(out)
 
 
+def debye_params(orders):
+    """[norder][2][3] float64 of a material's "debye" entry: one dict per
+    order with eps_delta and eps_relax (s), scalars or per-direction triples."""
+    out = np.zeros((len(orders), 2, 3), np.float64)
+    for o, d in enumerate(orders):
+        out[o, 0] = _v3(d.get("eps_delta", 0.0))
+        out[o, 1] = _v3(d.get("eps_relax", 0.0))
+    return np.ascontiguousarray(out)
+
+
 class Operator:
     """Builds the operator of a scene dict (see openems_amd.scenes)."""
 
@@ -105,6 +115,9 @@ class Operator:
             if m.get("lorentz"):
                 prm = dispersion_params(m["lorentz"])
                 check_op(L.emsop_set_dispersion(self.h, prm.shape[0], ptr(prm, C.c_double)))
+            elif m.get("debye"):
+                prm = debye_params(m["debye"])
+                check_op(L.emsop_set_debye(self.h, prm.shape[0], ptr(prm, C.c_double)))
         for m in s.get("metals", []):
             st, sp = _v3(m["start"]), _v3(m["stop"])
             check_op(L.emsop_add_metal(self.h, int(m.get("priority", 0)), ptr(st, C.c_double), ptr(sp, C.c_double)))

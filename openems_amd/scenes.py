@@ -122,7 +122,8 @@ def upml_bricks(n=256, nrts=10000, nbricks=40, seed=20260202, pml=8, dump=True):
 
 def dispersive_rlc(n=40, nrts=300, pml=6):
     """Graded n^3-line mesh with UPML, a Drude box reaching into the UPML, a
-    two-order Lorentz box (eps and mue poles), a series RLC element, a
+    two-order Lorentz box (eps and mue poles), a two-order Debye box (the
+    second order's relaxation time below 2 dT, so it stays off), a series RLC element, a
     parallel RLC element on the edges' own capacitance and a parallel LC with
     R = 0, driven by a soft E_z Gaussian (fc = 20 GHz)."""
     lines = [graded_lines(n, 0.5, max(1, n // 16), 1.2) for _ in range(3)]
@@ -142,6 +143,9 @@ def dispersive_rlc(n=40, nrts=300, pml=6):
                      {"eps_plasma": (6e9, 7e9, 8e9), "mue_plasma": 10e9, "mue_relax": 4e-11, "mue_pole": 8e9}]},
         {"eps": 4.0, "priority": 1, "start": box((c - 4, c + 4, c - 6), (c + 2, c + 9, c + 3))[0],
          "stop": box((c - 4, c + 4, c - 6), (c + 2, c + 9, c + 3))[1]},
+        {"eps": 2.5, "priority": 4, "start": box((c - 12, c - 12, c + 5), (c - 6, c - 7, c + 10))[0],
+         "stop": box((c - 12, c - 12, c + 5), (c - 6, c - 7, c + 10))[1],
+         "debye": [{"eps_delta": (1.0, 0.8, 0.6), "eps_relax": 2e-11}, {"eps_delta": 1.0, "eps_relax": 1e-12}]},
     ]
     s_ser, e_ser = box((c - 2, c - 2, c - 3), (c - 1, c - 1, c + 1))
     s_par, e_par = box((c + 1, c - 3, c - 2), (c + 2, c - 3, c + 2))

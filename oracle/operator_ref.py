@@ -164,6 +164,17 @@ def _dispersion(orders):
     return out
 
 
+def _debye(orders):
+    """CSPropDebyeMaterial parameters as [order][k][n] (k: eps delta, relaxation time), or None."""
+    if not orders:
+        return None
+    out = np.zeros((len(orders), 2, 3))
+    for o, d in enumerate(orders):
+        out[o, 0] = _vec3(d.get("eps_delta", 0.0))
+        out[o, 1] = _vec3(d.get("eps_relax", 0.0))
+    return out
+
+
 class BoxPainter:
     """Highest-priority-box lookup on a tensor-product grid of sample points.
 
@@ -370,7 +381,7 @@ class RefOperator:
                 continue
             payload = {"eps": _vec3(m.get("eps", 1.0)), "mue": _vec3(m.get("mue", 1.0)),
                        "kappa": _vec3(m.get("kappa", 0.0)), "sigma": _vec3(m.get("sigma", 0.0)),
-                       "metal": False, "disp": _dispersion(m.get("lorentz"))}
+                       "metal": False, "disp": _dispersion(m.get("lorentz")), "debye": _debye(m.get("debye"))}
             mats.append((m.get("priority", 0), k, m["start"], m["stop"], payload))
         off = len(scene.get("materials", []))
         for k, m in enumerate(scene.get("metals", [])):
@@ -778,14 +789,15 @@ class RefOperator:
     # -- dispersive materials and lumped RLC --------------------------------------
     def build_lorentz(self, C, L, vi, iv, dT):
         """Operator_Ext_LorentzMaterial::BuildExtension
-        (operator_ext_lorentzmaterial.cpp:120-445), Lorentz / Drude materials
-        (no Debye): per dispersion order, the cells with an active component
+        (operator_ext_lorentzmaterial.cpp:120-445), Lorentz / Drude and Debye
+        materials: per dispersion order, the cells with an active component
         in x -> y -> z order and their float32 tables; the material of an edge
         is the MATERIAL|METAL winner at its Yee point (GetYeeCoords,
-        operator.cpp:182-202).  Returns a list of dicts (pos, v_int, v_ext,
+        operator.cpp:182-202); Debye materials take the voltage side.  Returns a list of dicts (pos, v_int, v_ext,
         i_int, i_ext, v_lor, i_lor, volt, curr, volt_lor, curr_lor)."""
         boxes = self.pec_painter.boxes
-        order_max = max([b[4]["disp"].shape[0] for b in boxes if b[4].get("disp") is not None] + [0])
+        m = self.mesh
+        order_max = max([b[4][k].shape[0] for b in boxes for k in ("disp", "debye") if b[4].get(k) is not None] + [0])
         if order_max == 0:
             return []
         N = self.N
@@ -795,14 +807,22 @@ class RefOperator:
             win[(n, False)] = self.pec_painter.paint_index(self.yee_coords(n, False))
             win[(n, True)] = self.pec_painter.paint_index(self.yee_coords(n, True))
 
-        def param(n, dual, o, k):
+        def param(n, dual, o, k, key="disp"):
             w = win[(n, dual)]
             out = np.zeros(w.shape)
             for bi, b in enumerate(boxes):
-                d = b[4].get("disp")
+                d = b[4].get(key)
                 if d is None or o >= d.shape[0]:
                     continue
                 out[w == bi] = d[o, k, n]
+            return out
+
+        def is_debye(n):
+            w = win[(n, False)]
+            out = np.zeros(w.shape, bool)
+            for bi, b in enumerate(boxes):
+                if b[4].get("debye") is not None:
+                    out |= (w == bi)
             return out
 
         idx = [np.arange(N[a]) for a in range(3)]
@@ -827,6 +847,19 @@ class RefOperator:
                     wl = param(n, False, o, 2) * 2 * PI
                     b = ok & (wl > 0) & (LD[n] > 0)
                     CL[n] = np.where(b, 1 / (wl * wl * LD[n]), 0.0)
+                # Debye (operator_ext_lorentzmaterial.cpp:259-275): C_L from eps delta,
+                # R_D = tau / C_L when tau > 2 dT; C_L is kept even when tau is too small
+                db = ok & is_debye(n)
+                if db.any():
+                    EA = (m.edge_len(nP, True)[P[nP]] * m.edge_len(nPP, True)[P[nPP]])
+                    cl_d = 8.85418781762e-12 * param(n, False, o, 0, "debye") * EA / m.edge_len(n, False)[P[n]]
+                    tau = param(n, False, o, 1, "debye")
+                    CL[n] = np.where(db, cl_d, CL[n])
+                    c = db & (cl_d > 0) & (tau > 0) & (tau > 2.0 * dT)
+                    with np.errstate(divide="ignore", invalid="ignore"):
+                        RD[n] = np.where(c, tau / cl_d, RD[n])
+                    a = a | c
+                    b = b | c
                 on |= a
                 flags[0] |= bool(a.any())
                 flags[2] |= bool(b.any())
@@ -854,8 +887,11 @@ class RefOperator:
                 VI = vi[n][sel].astype(np.float64)
                 IV = iv[n][sel].astype(np.float64)
                 with np.errstate(divide="ignore", invalid="ignore"):
-                    t["v_int"][n] = np.where(ld > 0, (2.0 * ld - dT * rd) / (2.0 * ld + dT * rd), 1.0)
-                    t["v_ext"][n] = np.where(ld > 0, dT / (ld + dT * rd / 2.0) * VI, 0.0)
+                    deb = (ld <= 0) & (rd > 0) & (cl > 0)
+                    t["v_int"][n] = np.where(ld > 0, (2.0 * ld - dT * rd) / (2.0 * ld + dT * rd),
+                                             np.where(deb, (2.0 * dT - rd * cl) / (cl * rd), 1.0))
+                    t["v_ext"][n] = np.where(ld > 0, dT / (ld + dT * rd / 2.0) * VI,
+                                             np.where(deb, 2.0 / rd * VI, 0.0))
                     t["i_int"][n] = np.where(cd > 0, (2.0 * cd - dT * gd) / (2.0 * cd + dT * gd), 1.0)
                     t["i_ext"][n] = np.where(cd > 0, dT / (cd + dT * gd / 2.0) * IV, 0.0)
                     t["v_lor"][n] = np.where(cl > 0, dT / cl / VI, 0.0)

@@ -390,3 +390,32 @@ def test_lorentz_pole_flags_are_per_side():
     nopole = run(None, None)
     assert np.array_equal(base[0], same[0]) and np.array_equal(base[1], same[1])
     assert not np.array_equal(base[1], nopole[1])
+
+
+def test_debye_coefficients_closed_form():
+    """Debye order (delta eps, tau > 2 dT) on a uniform mesh of cell d:
+    C_L = eps0 delta_eps d^2 / d, R_D = tau / C_L, so v_int = (2 dT - tau)/tau,
+    v_ext = 2 C_L vi / tau and v_lor = dT/(C_L vi); a second order with
+    tau < 2 dT stays off.  Voltages only."""
+    l = np.arange(16, dtype=np.float64)
+    d = 1e-3
+    de, tau = 3.0, 2e-11
+    sc = {"lines": [l, l.copy(), l.copy()], "delta_unit": d, "bc": [0] * 6,
+          "materials": [{"eps": 2.0, "priority": 1, "start": (4, 4, 4), "stop": (10, 11, 9),
+                         "debye": [{"eps_delta": de, "eps_relax": tau}, {"eps_delta": 1.0, "eps_relax": 1e-13}]}],
+          "excitations": [{"type": 0, "val": (0, 0, 1.0), "start": (2, 2, 2), "stop": (2, 2, 3)}],
+          "signal": {"type": "gauss", "f0": 0.0, "fc": 10e9}, "nrts": 100}
+    ref = RefOperator(sc).build()
+    o0, o1 = ref.lorentz
+    dT = ref.dT
+    assert o0["volt"] and o0["volt_lor"] and not o0["curr"]
+    assert o1["pos"].shape[1] == 0
+    CL = EPS0 * de * d * d / d
+    x, y, z = o0["pos"].astype(np.int64)
+    for n in range(3):
+        vi = ref.vi[n][x, y, z].astype(np.float64)
+        act = o0["v_ext"][n] != 0
+        assert act.sum() > 100
+        assert np.allclose(o0["v_int"][n][act], (2 * dT - tau) / tau, rtol=1e-6, atol=0)
+        assert np.allclose(o0["v_ext"][n][act], (2 * CL * vi / tau)[act], rtol=1e-6, atol=0)
+        assert np.allclose(o0["v_lor"][n][act], (dT / (CL * vi))[act], rtol=1e-6, atol=0)
