@@ -55,7 +55,25 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    build_mirror_check(verbose)
     return LIB
+
+
+MIRROR_SRC = os.path.join(ROOT, "tests", "cpp", "engine_mirror.cpp")
+MIRROR_BIN = os.path.join(LIBDIR, "engine_mirror")
+
+
+def build_mirror_check(verbose=False):
+    """The C++ host-class check program (include/openems_hip.hpp), linked
+    against the library; run by the GPU tests."""
+    if not os.path.exists(MIRROR_SRC):
+        return None
+    cmd = ["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-I", os.path.join(ROOT, "include"), MIRROR_SRC,
+           "-o", MIRROR_BIN, "-L", LIBDIR, "-lopenems_hip", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, stderr=None if verbose else subprocess.PIPE)
+    return MIRROR_BIN
 
 
 if __name__ == "__main__":

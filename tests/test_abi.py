@@ -60,3 +60,20 @@ def test_engine_refuses_without_device_or_raises_cleanly(native):
     op = Operator(scenes.pec_cavity(12, 10))
     with pytest.raises(RuntimeError):
         EngineHIP(op)
+
+
+def test_cpp_host_classes_compile(native):
+    """include/openems_hip.hpp (Operator_HIP / Engine_HIP over the C ABI)
+    compiles on its own, and the check program the GPU tests run is built
+    against the library."""
+    import subprocess
+    import tempfile
+    from openems_amd import build as B
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "tu.cpp")
+        with open(src, "w") as f:
+            f.write('#include "openems_hip.hpp"\nint main() { return openems_hip::Operator_HIP::New() ? 0 : 1; }\n')
+        r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+                            src], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+    assert os.path.exists(B.MIRROR_BIN)

@@ -3,6 +3,8 @@ reference engine on the same operator and inputs.  The bar is bit-identical
 fields and probe samples; probe time series additionally within 1e-5
 relative L2 (the tolerance stated for the engine) where a reduced-order
 accumulation is involved."""
+import os
+
 import numpy as np
 import pytest
 
@@ -438,3 +440,65 @@ def test_dispersive_and_rlc_scene_from_the_operator(gpu, layout):
     assert np.array_equal(c, rc), f"current mismatches {np.count_nonzero(c != rc)}"
     eng.close()
     ref.close()
+
+
+def _mirror_scene():
+    """The scene tests/cpp/engine_mirror.cpp builds through the C++ host classes."""
+    l = np.arange(28, dtype=np.float64)
+    return {
+        "name": "cpp_mirror", "lines": [l, l.copy(), l.copy()], "delta_unit": 1e-3,
+        "bc": [3, 3, 0, 0, 0, 0], "pml_size": [6] * 6,
+        "materials": [
+            {"eps": 3.0, "kappa": 0.02, "priority": 1, "start": (8, 8, 8), "stop": (18, 16, 20)},
+            {"eps": 1.0, "priority": 3, "start": (19, 9, 9), "stop": (22, 14, 18),
+             "lorentz": [{"eps_plasma": 12e9, "eps_relax": 30e-12}]},
+        ],
+        "metals": [{"priority": 2, "start": (12, 20, 4), "stop": (14, 22, 24)}],
+        "lumped": [{"dir": 2, "R": 50.0, "C": 0.0, "L": 0.0, "caps": True, "type": "parallel", "priority": 0,
+                    "start": (14, 14, 10), "stop": (14, 14, 12)}],
+        "excitations": [{"type": 0, "val": (0.0, 0.0, 1.0), "start": (14, 14, 10), "stop": (14, 14, 12)}],
+        "signal": {"type": "gauss", "f0": 0.0, "fc": 15e9},
+        "nrts": 400,
+    }
+
+
+def test_cpp_host_classes_match_python(gpu):
+    """Operator_HIP / Engine_HIP (include/openems_hip.hpp, C++) and the Python
+    host give bit-identical timestep, coefficient, fields after per-element
+    Set* calls, step counts and energies on a scene with UPML, a dielectric,
+    a metal, a lumped port, a source and a Drude material."""
+    import json
+    import subprocess
+    from openems_amd import build as B
+    print("case C++ host classes (engine_mirror)", flush=True)
+    assert os.path.exists(B.MIRROR_BIN), "build the library first (it builds the check program)"
+    out = subprocess.run([B.MIRROR_BIN], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    got = json.loads(out.stdout)
+    fx = lambda s: float.fromhex(s)
+    op = Operator(_mirror_scene())
+    eng = EngineHIP(op)
+    eng.IterateTS(120)
+    assert fx(got["dT"]) == op.dT
+    assert got["numTS_a"] == eng.numTS == 120
+    v, c = eng.get_volt(), eng.get_curr()
+    assert np.float32(fx(got["vi_a"])) == op.coeff("vi")[2, 14, 14, 10]
+    assert np.float32(fx(got["v_a"])) == v[2, 14, 14, 11]
+    assert np.float32(fx(got["v_b"])) == v[0, 20, 11, 12]
+    assert np.float32(fx(got["i_a"])) == c[1, 10, 12, 14]
+    assert fx(got["energy_a"]) == eng.energy()
+    v[0, 5, 5, 5] = np.float32(0.25)
+    c[1, 7, 9, 11] = np.float32(-0.5)
+    eng.set_volt(v)
+    eng.set_curr(c)
+    eng.IterateTS(80)
+    v, c = eng.get_volt(), eng.get_curr()
+    assert got["numTS_b"] == eng.numTS == 200
+    assert abs(v[2, 14, 14, 11]) > 0
+    assert np.float32(fx(got["v_c"])) == v[2, 14, 14, 11]
+    assert np.float32(fx(got["v_d"])) == v[0, 5, 5, 5]
+    assert np.float32(fx(got["i_b"])) == c[1, 7, 9, 11]
+    assert np.float32(fx(got["v_e"])) == v[1, 21, 12, 13]
+    assert fx(got["energy_b"]) == eng.energy()
+    eng.close()
+    op.close()
