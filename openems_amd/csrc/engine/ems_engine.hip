@@ -1963,12 +1963,13 @@ int upload(ems_engine* h, T** d, const T* src, size_t n)
 
 unsigned choose_zdepth(const unsigned* N)
 {
-    // z-sweep depth: 8 planes per thread measured best on MI355X for 256^3
-    // (4: +1 %, 16: +3 %, 32: +8 % step time); shallower sweeps when the grid
-    // would otherwise give fewer than 2048 workgroups for 256 CUs
+    // z-sweep depth: 4 planes per thread measured best on MI355X (256^3 UPML:
+    // 3 and 6 within 1 %, 8: +1.4 %, 16: +5 %, 32: +14 % step time; 1024 x
+    // 1024 x 768 Mur: 8 +0.4 %); shallower sweeps when the grid would
+    // otherwise give fewer than 2048 workgroups for 256 CUs
     const unsigned plane = N[0] * N[1];
     const unsigned qt = (plane + kBlock - 1) / kBlock;
-    unsigned d = 8;
+    unsigned d = 4;
     while (d > 1 && (unsigned long long)qt * ((N[2] + d - 1) / d) < 2048ull) d /= 2;
     return d;
 }
