@@ -246,11 +246,17 @@ int ems_add_probe(ems_engine* h, const ems_probe_desc* d, int* id);
  * ts[count], values[count*3] (slot 0 used by voltage/current probes) */
 int ems_probe_results(ems_engine* h, int id, unsigned* count, unsigned* ts, double* values);
 
-/* ---- frequency-domain field dump (E field, node interpolation) ----
- * box start/stop mesh indices, nfreq complex<float> factors per sample are
- * supplied up front: factors[nsamples][nfreq][2] for samples taken at
- * ts = k*fd_interval (k = 0..nsamples-1). */
-int ems_add_fd_dump(ems_engine* h, int field, const unsigned start[3], const unsigned stop[3],
+/* ---- frequency-domain field dump (ProcessFieldsFD, dump types 10 and 11) ----
+ * field 0 = E (voltage / edge length), 1 = H (current / dual edge length);
+ * interp 0 = none, 1 = node, 2 = cell interpolation
+ * (Engine_Interface_FDTD::GetRawInterpolatedField / ...DualField,
+ * FDTD/engine_interface_fdtd.cpp:63-204).  Box start/stop are mesh indices
+ * (snapped by the caller, on the dual mesh where the reference uses it);
+ * edge_len[3] are the primary (E) or dual (H) edge lengths per axis.  The
+ * nfreq complex<float> factors per sample are supplied up front:
+ * factors[nsamples][nfreq][2] for samples taken at ts = k*fd_interval
+ * (k = 0..nsamples-1), with the half-step time shift of H already in them. */
+int ems_add_fd_dump(ems_engine* h, int field, int interp, const unsigned start[3], const unsigned stop[3],
                     unsigned fd_interval, unsigned nfreq, unsigned nsamples, const float* factors,
                     const double* const edge_len[3], int* id);
 /* out: [nfreq][3][nx][ny][nz][2] float32 (re, im), dump-box N-I-J-K order */

@@ -88,6 +88,17 @@ int emsop_add_lumped(emsop* h, int dir, double R, double C, double L, int caps, 
 /* excitation box: type 0 soft E, 1 hard E, 2 soft H, 3 hard H; val = per-direction amplitude */
 int emsop_add_excitation(emsop* h, int type, const double val[3], double delay, int priority,
                          const double start[3], const double stop[3]);
+/* Polyline wire of a metal property (CSPrimCurve): npts points pts[3*i + axis]
+ * in drawing units; every segment's mesh path (emsop_find_path) gets
+ * vv = vi = 0 (Operator::CalcPEC_Curves, FDTD/operator.cpp:2086-2118). */
+int emsop_add_metal_curve(emsop* h, unsigned npts, const double* pts);
+/* Polyline excitation (CSPrimCurve of an excitation property): type 0 soft or
+ * 1 hard voltage excitation on the path edges of every segment, amplitude
+ * val[n] times the edge length, entered after all box excitations in the
+ * order added (Operator_Ext_Excitation::BuildExtension,
+ * FDTD/extensions/operator_ext_excitation.cpp:235-287); types 2 and 3 add nothing. */
+int emsop_add_excitation_curve(emsop* h, int type, const double val[3], double delay, unsigned npts,
+                               const double* pts);
 /* bc: 0 PEC, 1 PMC, 2 Mur, 3 PML, -1 none (openEMS::m_BC_type) */
 int emsop_set_boundary(emsop* h, const int bc[6], const unsigned pml_size[6], const double mur_vph[6]);
 int emsop_set_signal(emsop* h, int type, double f0, double fc);
@@ -155,6 +166,12 @@ int emsop_snap_to_mesh(const emsop* h, const double coord[3], int dual, unsigned
 /* SnapBox2Mesh (method 0): returns dimension or -2 outside */
 int emsop_snap_box(const emsop* h, const double start[3], const double stop[3], int dual,
                    unsigned ustart[3], unsigned ustop[3], int start_in[3], int stop_in[3]);
+/* Operator::FindPath (FDTD/operator.cpp:390-496): the staircase of mesh edges
+ * between the nodes nearest start and stop; edge t is component dir[t] at node
+ * pos[3*t + axis].  *count is set to the number of edges (0: no path); fails
+ * with EMSOP_E_ARG if it exceeds max_edges.  pos and dir may be NULL. */
+int emsop_find_path(const emsop* h, const double start[3], const double stop[3], unsigned max_edges,
+                    unsigned* pos, int* dir, unsigned* count);
 
 /* release the EC and effective-material arrays once extensions are built */
 void emsop_release_build_arrays(emsop* h);

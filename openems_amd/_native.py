@@ -34,6 +34,8 @@ OPERATOR_API = [
     ("emsop_add_material_grid", C.c_int, [vp, up, up, dp, dp, C.c_double, C.c_double, C.c_int]),
     ("emsop_eps_weights", C.c_int, [vp, C.c_int, up, up, dp]),
     ("emsop_add_metal", C.c_int, [vp, C.c_int, dp, dp]),
+    ("emsop_add_metal_curve", C.c_int, [vp, C.c_uint, dp]),
+    ("emsop_add_excitation_curve", C.c_int, [vp, C.c_int, dp, C.c_double, C.c_uint, dp]),
     ("emsop_add_lumped", C.c_int, [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, dp, dp]),
     ("emsop_add_excitation", C.c_int, [vp, C.c_int, dp, C.c_double, C.c_int, dp, dp]),
     ("emsop_set_boundary", C.c_int, [vp, ip, up, dp]),
@@ -69,6 +71,7 @@ OPERATOR_API = [
     ("emsop_edge_length", C.c_double, [vp, C.c_int, C.c_uint, C.c_int]),
     ("emsop_snap_to_mesh", C.c_int, [vp, dp, C.c_int, up, ip]),
     ("emsop_snap_box", C.c_int, [vp, dp, dp, C.c_int, up, up, ip, ip]),
+    ("emsop_find_path", C.c_int, [vp, dp, dp, C.c_uint, up, ip, up]),
     ("emsop_release_build_arrays", None, [vp]),
 ]
 
@@ -106,7 +109,7 @@ ENGINE_API = [
     ("ems_set_curr", C.c_int, [vp, fp]),
     ("ems_add_probe", C.c_int, [vp, C.POINTER(ProbeDesc), ip]),
     ("ems_probe_results", C.c_int, [vp, C.c_int, up, up, dp]),
-    ("ems_add_fd_dump", C.c_int, [vp, C.c_int, up, up, C.c_uint, C.c_uint, C.c_uint, fp, C.POINTER(dp), ip]),
+    ("ems_add_fd_dump", C.c_int, [vp, C.c_int, C.c_int, up, up, C.c_uint, C.c_uint, C.c_uint, fp, C.POINTER(dp), ip]),
     ("ems_fd_dump_results", C.c_int, [vp, C.c_int, fp]),
     ("ems_energy", C.c_int, [vp, dp]),
     ("ems_last_timing", C.c_int, [vp, dp, dp, dp]),

@@ -1329,12 +1329,70 @@ struct DumpDev {
     unsigned s[3], n[3];          // box start and extent (mesh indices)
     unsigned N[3];
     unsigned fd_interval, nfreq, nsamples;
+    int field;                    // 0: E from the voltages, 1: H from the currents
+    int interp;                   // 0: none, 1: node, 2: cell interpolation
     const float* factors;         // [nsamples][nfreq][2]
-    const double* el[3];          // primary edge lengths per axis
+    const double* el[3];          // edge lengths per axis: primary (E) or dual (H)
     float* acc;                   // [nfreq][3][npts][2]
 };
 
-__global__ void k_fd_dump(const float* __restrict__ V, unsigned long long cs, unsigned Nx, unsigned plane, const DumpDev d,
+// GetRawField / GetRawDualField (engine_interface_fdtd.cpp:263-268,134-146):
+// the stored value over its edge length, zero for a zero length
+__device__ inline double dump_raw(const float* F, unsigned long long off, double delta)
+{
+    return delta ? (double)F[off] / delta : 0.0;
+}
+
+// Engine_Interface_FDTD::GetRawInterpolatedField (E, engine_interface_fdtd.cpp:63-124)
+// and GetRawInterpolatedDualField (H, :148-204), component n at node pos, in double
+__device__ inline double dump_value(const DumpDev& d, const float* F, unsigned long long cs, const unsigned* pos,
+                                    unsigned long long base, const unsigned long long* stride, int n)
+{
+    const int nP = (n + 1) % 3, nPP = (n + 2) % 3;
+    const float* C = F + (unsigned long long)n * cs;
+    const unsigned pn = pos[n];
+    const double* el = d.el[n];
+    if (d.interp == 0) return dump_raw(C, base, el[pn]);
+    if (d.field == 0) {
+        if (d.interp == 1) {
+            if (pn == d.N[n] - 1) return dump_raw(C, base - stride[n], el[pn - 1]);
+            const double delta = el[pn];
+            double out = dump_raw(C, base, delta);
+            if (delta == 0.0) return 0.0;
+            if (pn == 0) return out;
+            const double deltaDown = el[pn - 1];
+            const double deltaRel = delta / (delta + deltaDown);
+            return out * (1.0 - deltaRel) + dump_raw(C, base - stride[n], deltaDown) * deltaRel;
+        }
+        if (pos[0] == d.N[0] - 1 || pos[1] == d.N[1] - 1 || pos[2] == d.N[2] - 1) return 0.0;
+        const double delta = el[pn];
+        double out = dump_raw(C, base, delta);
+        out += dump_raw(C, base + stride[nP], delta);
+        out += dump_raw(C, base + stride[nP] + stride[nPP], delta);
+        out += dump_raw(C, base + stride[nPP], delta);
+        return out / 4;
+    }
+    if (d.interp == 1) {
+        if (pos[0] == d.N[0] - 1 || pos[1] == d.N[1] - 1 || pos[2] == d.N[2] - 1 || pos[nP] == 0 || pos[nPP] == 0)
+            return 0.0;
+        const double delta = el[pn];
+        double out = dump_raw(C, base, delta);
+        out += dump_raw(C, base - stride[nP], delta);
+        out += dump_raw(C, base - stride[nP] - stride[nPP], delta);
+        out += dump_raw(C, base - stride[nPP], delta);
+        return out / 4;
+    }
+    if (pn >= d.N[n] - 1) return 0.0;
+    const double delta = el[pn];
+    const double deltaUp = el[pn + 1];
+    const double deltaRel = delta / (delta + deltaUp);
+    return dump_raw(C, base, delta) * (1.0 - deltaRel) + dump_raw(C, base + stride[n], deltaUp) * deltaRel;
+}
+
+// ProcessFieldsFD::Process (processfields_fd.cpp:72-107): every fd_interval
+// steps, field (float) times exp(-2 pi i f T) * 2 dT fd_interval (complex<float>)
+// added to the running sum of each frequency.  One thread per dump point.
+__global__ void k_fd_dump(const float* __restrict__ F, unsigned long long cs, unsigned Nx, unsigned plane, const DumpDev d,
                           const unsigned* __restrict__ numts)
 {
     const unsigned ts = *numts;
@@ -1353,27 +1411,7 @@ __global__ void k_fd_dump(const float* __restrict__ V, unsigned long long cs, un
     const unsigned long long stride[3] = {1ull, (unsigned long long)Nx, (unsigned long long)plane};
     float E[3];
 #pragma unroll
-    for (int n = 0; n < 3; ++n) {
-        double out;
-        const unsigned pn = pos[n];
-        if (pn == d.N[n] - 1) {
-            const double delta = d.el[n][pn - 1];
-            const double raw = delta ? (double)V[n * cs + base - stride[n]] / delta : 0.0;
-            out = raw;
-        } else {
-            const double delta = d.el[n][pn];
-            out = delta ? (double)V[n * cs + base] / delta : 0.0;
-            if (delta == 0.0)
-                out = 0.0;
-            else if (pn != 0) {
-                const double deltaDown = d.el[n][pn - 1];
-                const double deltaRel = delta / (delta + deltaDown);
-                const double lo = deltaDown ? (double)V[n * cs + base - stride[n]] / deltaDown : 0.0;
-                out = out * (1.0 - deltaRel) + lo * deltaRel;
-            }
-        }
-        E[n] = (float)out;
-    }
+    for (int n = 0; n < 3; ++n) E[n] = (float)dump_value(d, F, cs, pos, base, stride, n);
     for (unsigned f = 0; f < d.nfreq; ++f) {
         const float er = d.factors[(ks * d.nfreq + f) * 2];
         const float ei = d.factors[(ks * d.nfreq + f) * 2 + 1];
@@ -2689,11 +2727,13 @@ int ems_probe_results(ems_engine* h, int id, unsigned* count, unsigned* ts, doub
     return 0;
 }
 
-int ems_add_fd_dump(ems_engine* h, int field, const unsigned start[3], const unsigned stop[3], unsigned fd_interval,
-                    unsigned nfreq, unsigned nsamples, const float* factors, const double* const edge_len[3], int* id)
+int ems_add_fd_dump(ems_engine* h, int field, int interp, const unsigned start[3], const unsigned stop[3],
+                    unsigned fd_interval, unsigned nfreq, unsigned nsamples, const float* factors,
+                    const double* const edge_len[3], int* id)
 {
     if (!h || !start || !stop || !factors || !edge_len) return fail(EMS_E_ARG, "null argument");
-    if (field != 0) return fail(EMS_E_ARG, "only E-field frequency-domain dumps are supported");
+    if (field != 0 && field != 1) return fail(EMS_E_ARG, "dump field must be 0 (E) or 1 (H)");
+    if (interp < 0 || interp > 2) return fail(EMS_E_ARG, "dump interpolation must be 0 (none), 1 (node) or 2 (cell)");
     if (fd_interval == 0 || nfreq == 0 || nsamples == 0) return fail(EMS_E_ARG, "empty frequency-domain dump");
     drop_graphs(h);     // the dump changes the launch pattern of every chunk
     DumpDev d{};
@@ -2706,6 +2746,8 @@ int ems_add_fd_dump(ems_engine* h, int field, const unsigned start[3], const uns
     d.fd_interval = fd_interval;
     d.nfreq = nfreq;
     d.nsamples = nsamples;
+    d.field = field;
+    d.interp = interp;
     HIPCHK(hipSetDevice(h->device));
     int rc;
     float* df;
@@ -2777,8 +2819,8 @@ static int launch_samples(ems_engine* h)
         const DumpDev& d = h->dumps[k];
         if (h->numTS % d.fd_interval || h->numTS / d.fd_interval >= d.nsamples) continue;
         const unsigned npts = h->dump_npts[k];
-        hipLaunchKernelGGL(k_fd_dump, dim3((npts + kBlock - 1) / kBlock), dim3(kBlock), 0, h->stream, h->V, h->cs,
-                           h->N[0], h->g.plane, d, h->numts_dev);
+        hipLaunchKernelGGL(k_fd_dump, dim3((npts + kBlock - 1) / kBlock), dim3(kBlock), 0, h->stream,
+                           d.field ? h->I : h->V, h->cs, h->N[0], h->g.plane, d, h->numts_dev);
     }
     return 0;
 }

@@ -135,6 +135,23 @@ struct MurFace {
     std::vector<float> cP, cPP;
 };
 
+// Polyline primitive (CSPrimCurve) treated as a wire: a metal curve zeroes
+// vv / vi on the mesh edges of its path, an excitation curve puts its edges on
+// the voltage excitation list (points in drawing units, in the order added)
+struct Curve {
+    std::vector<double> pts;   // [npts][3]
+    bool exc = false;
+    int type = 0;              // excitation type (0 soft, 1 hard)
+    double val[3] = {0, 0, 0};
+    double delay = 0;
+};
+
+// one mesh edge of a path: component dir at node pos
+struct PathEdge {
+    unsigned pos[3];
+    int dir;
+};
+
 struct ExcList {
     std::vector<unsigned> idx[3];
     std::vector<unsigned short> dir;
@@ -244,6 +261,7 @@ struct emsop {
     std::vector<MatGrid> grids;
     std::vector<LumpedBox> lumped;
     std::vector<ExcBox> excs;
+    std::vector<Curve> curves;
     int bc[6] = {0, 0, 0, 0, 0, 0};
     unsigned pml[6] = {8, 8, 8, 8, 8, 8};
     double vph[6] = {0, 0, 0, 0, 0, 0};
@@ -375,8 +393,11 @@ struct emsop {
     void ec_to_coeff(int n, size_t i);
     void apply_bc();
     void calc_pec();
+    bool find_path(const double* start, const double* stop, std::vector<PathEdge>& path) const;
+    void calc_pec_curves();
     int lumped_rc();
     int build_excitation();
+    void build_excitation_boxes();
     void build_mur();
     void build_upml();
     void build_lorentz();
@@ -790,6 +811,116 @@ void emsop::calc_pec()
     }
 }
 
+// Point_Line_Distance of CSXCAD (Cartesian mesh): foot = (P - s).(e - s) / |e - s|^2
+// is the position of the foot of the perpendicular along s -> e, dist the
+// distance of P from the line.  A zero-length line gives foot = NaN.
+static void point_line_distance(const double* P, const double* s, const double* e, double& foot, double& dist)
+{
+    const double d[3] = {e[0] - s[0], e[1] - s[1], e[2] - s[2]};
+    const double LL = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    foot = ((P[0] - s[0]) * d[0] + (P[1] - s[1]) * d[1] + (P[2] - s[2]) * d[2]) / LL;
+    const double q[3] = {P[0] - (s[0] + foot * d[0]), P[1] - (s[1] + foot * d[1]), P[2] - (s[2] + foot * d[2])};
+    dist = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+}
+
+// Operator::FindPath (operator.cpp:390-496) with SnapLine2Mesh
+// (operator.cpp:360-388): a staircase of mesh edges from the node nearest
+// start to the node nearest stop.  Each step moves to the neighbouring node
+// whose foot on the line lies further along and whose distance from the line
+// is smallest (the first candidate wins a tie, axes x, y, z, down before up);
+// then at most one edge per axis closes the gap to the stop node.  A path
+// that leaves the mesh is dropped (empty), as is a line wholly beyond one end
+// of an axis.
+bool emsop::find_path(const double* start, const double* stop, std::vector<PathEdge>& path) const
+{
+    path.clear();
+    unsigned us[3], ue[3];
+    for (int n = 0; n < 3; ++n) {
+        bool in;
+        us[n] = snap_line(n, start[n], in, false);
+        ue[n] = snap_line(n, stop[n], in, false);
+    }
+    for (int n = 0; n < 3; ++n) {
+        const double lo = lines[n][0], hi = lines[n][N[n] - 1];
+        if ((start[n] < lo && stop[n] < lo) || (start[n] > hi && stop[n] > hi)) return false;
+    }
+    unsigned cur[3] = {us[0], us[1], us[2]};
+    const double ms[3] = {lines[0][us[0]], lines[1][us[1]], lines[2][us[2]]};
+    const double me[3] = {lines[0][ue[0]], lines[1][ue[1]], lines[2][ue[2]]};
+    double foot = 0, dist = 0, startFoot, stopFoot;
+    point_line_distance(ms, start, stop, startFoot, dist);
+    point_line_distance(me, start, stop, stopFoot, dist);
+    double currFoot = startFoot, minFoot = startFoot, minDist = 0;
+    int minDir = 0;
+    bool up = false;
+    while (minFoot < stopFoot) {
+        minDist = 1e300;
+        for (int n = 0; n < 3; ++n) {
+            double P[3] = {lines[0][cur[0]], lines[1][cur[1]], lines[2][cur[2]]};
+            if (cur[n] >= 1) {
+                P[n] = lines[n][cur[n] - 1];
+                point_line_distance(P, start, stop, foot, dist);
+                if (foot > currFoot && dist < minDist) {
+                    minFoot = foot;
+                    minDist = dist;
+                    minDir = n;
+                    up = false;
+                }
+            }
+            if (cur[n] + 1 < N[n]) {
+                P[n] = lines[n][cur[n] + 1];
+                point_line_distance(P, start, stop, foot, dist);
+                if (foot > currFoot && dist < minDist) {
+                    minFoot = foot;
+                    minDist = dist;
+                    minDir = n;
+                    up = true;
+                }
+            }
+        }
+        PathEdge e{{cur[0], cur[1], cur[2]}, minDir};
+        if (up) {
+            cur[minDir] += 1;
+        } else {
+            cur[minDir] -= 1;   // unsigned: below line 0 wraps and fails the check below
+            e.pos[minDir] -= 1;
+        }
+        for (int n = 0; n < 3; ++n)
+            if (cur[n] >= N[n]) {
+                path.clear();
+                return false;
+            }
+        path.push_back(e);
+        currFoot = minFoot;
+    }
+    for (int n = 0; n < 3; ++n) {
+        if (cur[n] > ue[n]) {
+            --cur[n];
+            path.push_back(PathEdge{{cur[0], cur[1], cur[2]}, n});
+        } else if (cur[n] < ue[n]) {
+            path.push_back(PathEdge{{cur[0], cur[1], cur[2]}, n});
+        }
+    }
+    return !path.empty();
+}
+
+void emsop::calc_pec_curves()
+{
+    // CalcPEC_Curves (operator.cpp:2086-2118): every segment of a metal curve
+    std::vector<PathEdge> path;
+    for (const auto& c : curves) {
+        if (c.exc) continue;
+        for (size_t i = 1; i < c.pts.size() / 3; ++i) {
+            find_path(&c.pts[3 * (i - 1)], &c.pts[3 * i], path);
+            for (const auto& e : path) {
+                const size_t q = idx(e.dir, e.pos[0], e.pos[1], e.pos[2]);
+                coeff[0][q] = 0;
+                coeff[1][q] = 0;
+            }
+        }
+    }
+}
+
 int emsop::lumped_rc()
 {
     // Calc_LumpedElements (operator.cpp:1586-1763) for parallel-RC elements
@@ -895,7 +1026,37 @@ int emsop::build_excitation()
 {
     exc[0] = ExcList();
     exc[1] = ExcList();
-    if (excs.empty()) return 0;
+    if (!excs.empty()) build_excitation_boxes();
+    // curves after the boxes (operator_ext_excitation.cpp:235-287): voltage
+    // excitation on the path edges, type 0 (soft) or 1 (hard, zeroes vv / vi)
+    std::vector<PathEdge> path;
+    for (const auto& c : curves) {
+        if (!c.exc) continue;
+        for (size_t i = 1; i < c.pts.size() / 3; ++i) {
+            find_path(&c.pts[3 * (i - 1)], &c.pts[3 * i], path);
+            for (const auto& e : path) {
+                const int n = e.dir;
+                if (e.pos[n] >= N[n] - 1 || (c.type != 0 && c.type != 1)) continue;
+                const double amp = c.val[n] * EL(n, e.pos[n], false);
+                if (amp != 0) {
+                    for (int a = 0; a < 3; ++a) exc[0].idx[a].push_back(e.pos[a]);
+                    exc[0].dir.push_back((unsigned short)n);
+                    exc[0].amp.push_back((float)amp);
+                    exc[0].delay.push_back((unsigned)(c.delay / dT));
+                }
+                if (c.type == 1) {
+                    const size_t q = idx(n, e.pos[0], e.pos[1], e.pos[2]);
+                    coeff[0][q] = 0;
+                    coeff[1][q] = 0;
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+void emsop::build_excitation_boxes()
+{
     Ranked<ExcBox> rk;
     rk.init(excs, [](const ExcBox&) { return true; });
     const unsigned W = rk.W;
@@ -965,7 +1126,6 @@ int emsop::build_excitation()
             }
         }
     }
-    return 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1301,6 +1461,7 @@ int emsop::build(unsigned nrts)
         lap("coefficients");
         apply_bc();
         calc_pec();
+        calc_pec_curves();
         lumped_rc();
         emsop_apply_magnetic_bc(this);
         lap("bc/pec/lumped");
@@ -1813,6 +1974,31 @@ int emsop_add_excitation(emsop* h, int type, const double val[3], double delay, 
     return 0;
 }
 
+int emsop_add_metal_curve(emsop* h, unsigned npts, const double* pts)
+{
+    if (!h || !pts) return fail(EMSOP_E_ARG, "null argument");
+    if (npts < 2) return fail(EMSOP_E_ARG, "a curve needs at least two points");
+    Curve c;
+    c.pts.assign(pts, pts + 3 * (size_t)npts);
+    h->curves.push_back(c);
+    return 0;
+}
+
+int emsop_add_excitation_curve(emsop* h, int type, const double val[3], double delay, unsigned npts, const double* pts)
+{
+    if (!h || !val || !pts) return fail(EMSOP_E_ARG, "null argument");
+    if (type < 0 || type > 3) return fail(EMSOP_E_ARG, "excitation type must be 0..3");
+    if (npts < 2) return fail(EMSOP_E_ARG, "a curve needs at least two points");
+    Curve c;
+    c.pts.assign(pts, pts + 3 * (size_t)npts);
+    c.exc = true;
+    c.type = type;
+    for (int n = 0; n < 3; ++n) c.val[n] = val[n];
+    c.delay = delay;
+    h->curves.push_back(c);
+    return 0;
+}
+
 int emsop_set_boundary(emsop* h, const int bc[6], const unsigned pml_size[6], const double mur_vph[6])
 {
     if (!h || !bc) return fail(EMSOP_E_ARG, "null argument");
@@ -2024,6 +2210,22 @@ int emsop_snap_box(const emsop* h, const double start[3], const double stop[3], 
         if (stop_in) stop_in[n] = ei[n];
     }
     return dim;
+}
+
+int emsop_find_path(const emsop* h, const double start[3], const double stop[3], unsigned max_edges, unsigned* pos,
+                    int* dir, unsigned* count)
+{
+    if (!h || !start || !stop || !count || h->dualL[0].empty()) return fail(EMSOP_E_ARG, "bad argument or mesh not prepared");
+    std::vector<PathEdge> path;
+    h->find_path(start, stop, path);
+    *count = (unsigned)path.size();
+    if (path.size() > max_edges) return fail(EMSOP_E_ARG, "path longer than max_edges");
+    for (size_t t = 0; t < path.size(); ++t) {
+        if (pos)
+            for (int a = 0; a < 3; ++a) pos[3 * t + a] = path[t].pos[a];
+        if (dir) dir[t] = path[t].dir;
+    }
+    return 0;
 }
 
 void emsop_release_build_arrays(emsop* h)

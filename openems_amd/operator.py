@@ -119,6 +119,10 @@ class Operator:
                 prm = debye_params(m["debye"])
                 check_op(L.emsop_set_debye(self.h, prm.shape[0], ptr(prm, C.c_double)))
         for m in s.get("metals", []):
+            if "curve" in m:
+                pts = np.ascontiguousarray(np.asarray(m["curve"], np.float64).reshape(-1, 3))
+                check_op(L.emsop_add_metal_curve(self.h, pts.shape[0], ptr(pts, C.c_double)))
+                continue
             st, sp = _v3(m["start"]), _v3(m["stop"])
             check_op(L.emsop_add_metal(self.h, int(m.get("priority", 0)), ptr(st, C.c_double), ptr(sp, C.c_double)))
         for le in s.get("lumped", []):
@@ -128,6 +132,12 @@ class Operator:
                                         int(le.get("type", "parallel") == "parallel"), int(le.get("priority", 0)),
                                         ptr(st, C.c_double), ptr(sp, C.c_double)))
         for e in s.get("excitations", []):
+            if "curve" in e:
+                pts = np.ascontiguousarray(np.asarray(e["curve"], np.float64).reshape(-1, 3))
+                val = _v3(e.get("val", 0.0))
+                check_op(L.emsop_add_excitation_curve(self.h, int(e.get("type", 0)), ptr(val, C.c_double),
+                                                      float(e.get("delay", 0.0)), pts.shape[0], ptr(pts, C.c_double)))
+                continue
             st, sp, val = _v3(e["start"]), _v3(e["stop"]), _v3(e.get("val", 0.0))
             check_op(L.emsop_add_excitation(self.h, int(e.get("type", 0)), ptr(val, C.c_double),
                                             float(e.get("delay", 0.0)), int(e.get("priority", 0)),
@@ -312,6 +322,21 @@ class Operator:
         si, ei = (C.c_int * 3)(), (C.c_int * 3)()
         dim = lib().emsop_snap_box(self.h, ptr(a, C.c_double), ptr(b, C.c_double), int(dual), us, ue, si, ei)
         return dim, list(us), list(ue), [bool(v) for v in si], [bool(v) for v in ei]
+
+    def find_path(self, start, stop):
+        """Operator::FindPath (FDTD/operator.cpp:390-496): list of (dir, (x, y, z))
+        mesh edges from the node nearest start to the node nearest stop."""
+        a = np.asarray(start, np.float64)
+        b = np.asarray(stop, np.float64)
+        cnt = C.c_uint()
+        # a step moves along an axis only in the sign of the line's direction,
+        # so a path has at most sum(N) steps plus 3 closing edges
+        cap = int(sum(self.N)) + 3
+        pos = np.zeros((cap, 3), np.uint32)
+        dirs = np.zeros(cap, np.int32)
+        check_op(lib().emsop_find_path(self.h, ptr(a, C.c_double), ptr(b, C.c_double), cap, ptr(pos, C.c_uint),
+                                       ptr(dirs, C.c_int), C.byref(cnt)))
+        return [(int(dirs[t]), tuple(int(v) for v in pos[t])) for t in range(cnt.value)]
 
     def close(self):
         if getattr(self, "h", None):

@@ -218,21 +218,36 @@ class Probe:
 
 
 class FieldDumpFD:
-    """ProcessFieldsFD of the E field with node interpolation (dump type 10)."""
+    """ProcessFieldsFD (Common/processfields_fd.cpp:72-107) of the E field
+    (dump type 10, spec "field": "E") or the H field (type 11, "H"), with no
+    (spec "interp": 0), node (1, the default) or cell (2) interpolation.  As
+    set up by openEMS::SetupProcessing (openems.cpp:612-681): an H dump samples
+    at the dual time and, without interpolation, on the dual mesh; cell
+    interpolation always uses the dual mesh; the box snaps with SnapMethod 1
+    (processfields.cpp:40, operator.cpp:323-337)."""
 
     def __init__(self, op, spec, nrts):
         self.op = op
         self.spec = spec
         self.name = spec.get("name", "")
-        dim, st, sp, _, _ = op.snap_box(spec["start"], spec["stop"], False)
-        # ProcessFields snaps with SnapMethod 1 (processfields.cpp:40) and orders start <= stop
+        fld = spec.get("field", "E")
+        self.field = {"E": 0, "H": 1, 0: 0, 1: 1, 10: 0, 11: 1}[fld]
+        self.interp = int(spec.get("interp", 1))
+        if self.interp not in (0, 1, 2):
+            raise ValueError("dump interpolation must be 0 (none), 1 (node) or 2 (cell)")
+        self.dual_time = self.field == 1
+        dual = self.interp == 2 or (self.field == 1 and self.interp == 0)
+        self.dual_mesh = dual
+        dim, st, sp, _, _ = op.snap_box(spec["start"], spec["stop"], dual)
+        # a box outside the mesh disables the dump (Processing::DefineStartStopCoord, processing.cpp:180-188)
+        self.enabled = dim >= 0
         lo = [min(a, b) for a, b in zip(spec["start"], spec["stop"])]
         hi = [max(a, b) for a, b in zip(spec["start"], spec["stop"])]
         for n in range(3):
             if sp[n] > st[n]:
-                if op.disc_line(n, st[n]) > lo[n] and st[n] > 0:
+                if op.disc_line(n, st[n], dual) > lo[n] and st[n] > 0:
                     st[n] -= 1
-                if op.disc_line(n, sp[n]) < hi[n] and sp[n] < op.N[n] - 1:
+                if op.disc_line(n, sp[n], dual) < hi[n] and sp[n] < op.N[n] - 1:
                     sp[n] += 1
             if st[n] > sp[n]:
                 st[n], sp[n] = sp[n], st[n]
@@ -240,7 +255,7 @@ class FieldDumpFD:
         nyq = op.nyquist
         self.fd_interval = 0
         self.freqs = []
-        for f in spec.get("freqs", []):
+        for f in spec.get("freqs", []) if self.enabled else []:
             nts = nyquist_num(f, op.dT)
             if nts == 0:
                 continue
@@ -258,7 +273,8 @@ class FieldDumpFD:
         out = np.zeros((self.nsamples, len(self.freqs), 2), np.float32)
         scale = np.float32(dT * self.fd_interval)
         for k in range(self.nsamples):
-            T = (k * self.fd_interval) * dT
+            # GetTime(dualTime) = (numTS + 0.5 dualTime) dT (engine_interface_fdtd.h:52)
+            T = (k * self.fd_interval + (0.5 if self.dual_time else 0.0)) * dT
             for j, f in enumerate(self.freqs):
                 th = np.float32(((-2.0 * math.pi) * f) * T)
                 c = np.float32(_libm.cosf(th))
@@ -270,7 +286,8 @@ class FieldDumpFD:
     def register(self, eng):
         if not self.fd_interval:
             return
-        self.did, self.shape = eng.add_fd_dump(self.start, self.stop, self.fd_interval, self.factors())
+        self.did, self.shape = eng.add_fd_dump(self.start, self.stop, self.fd_interval, self.factors(),
+                                               field=self.field, interp=self.interp)
 
     def collect(self, eng):
         if self.did is None:

@@ -177,6 +177,45 @@ def dispersive_rlc(n=40, nrts=300, pml=6):
     return scene
 
 
+def wire_antenna(n=36, nrts=400, pml=6):
+    """Graded n^3-line mesh with UPML and wire (curve) primitives: a bent
+    dipole of two metal polylines (along z, then diagonal in x-z and in
+    x-y-z, off the mesh lines) fed by a soft voltage excitation curve across
+    its gap (Gaussian, fc = 20 GHz); a hard excitation curve on a second
+    diagonal wire; a dielectric box; the feed voltage as a probe."""
+    lines = [graded_lines(n, 0.5, max(1, n // 16), 1.2) for _ in range(3)]
+    L = lines
+    c = n // 2
+
+    def pt(i, j, k):
+        # a point given in fractional line indices (off the lines where not whole)
+        return tuple(float(np.interp(v, np.arange(len(L[a])), L[a])) for a, v in enumerate((i, j, k)))
+
+    arm_up = [pt(c, c, c + 1), pt(c, c, c + 6), pt(c + 4.3, c, c + 9.6), pt(c + 6.2, c + 2.7, c + 11.1)]
+    arm_dn = [pt(c, c, c - 1), pt(c, c, c - 6), pt(c - 3.6, c - 1.2, c - 9.4)]
+    return {
+        "name": f"wire_antenna_{n}",
+        "lines": lines,
+        "delta_unit": 1e-3,
+        "bc": [3] * 6,
+        "pml_size": [pml] * 6,
+        "materials": [{"eps": 3.0, "kappa": 0.005, "priority": 1,
+                       "start": pt(c - 8, c + 3, c - 5), "stop": pt(c - 2, c + 9, c + 4)}],
+        "metals": [{"curve": arm_up}, {"curve": arm_dn}],
+        "excitations": [
+            {"type": 0, "val": (0.0, 0.0, 1.0), "curve": [pt(c, c, c - 1), pt(c, c, c + 1)]},
+            {"type": 1, "val": (0.3, 0.2, 0.1), "delay": 2e-11,
+             "curve": [pt(c - 9.2, c - 7.5, c + 3.3), pt(c - 5.1, c - 4.4, c + 6.8)]},
+        ],
+        "signal": {"type": "gauss", "f0": 0.0, "fc": 20e9},
+        "nrts": nrts,
+        "oversampling": 4,
+        "probes": [_probe(0, pt(c, c, c - 1), pt(c, c, c + 1), name="feed_ut"),
+                   _probe(2, pt(c + 5, c - 4, c + 3), pt(c + 5, c - 4, c + 3), name="et")],
+        "dumps": [],
+    }
+
+
 def mur_layers(nx=1024, ny=1024, nz=768, nrts=4000, seed=20260203, f0=15e9):
     """Layered dielectric, Mur ABC on every face, sinusoidal E_x sheet source."""
     d = 0.25

@@ -384,12 +384,21 @@ class RefOperator:
                        "metal": False, "disp": _dispersion(m.get("lorentz")), "debye": _debye(m.get("debye"))}
             mats.append((m.get("priority", 0), k, m["start"], m["stop"], payload))
         off = len(scene.get("materials", []))
+        self.metal_curves = []
         for k, m in enumerate(scene.get("metals", [])):
+            if "curve" in m:
+                self.metal_curves.append(np.asarray(m["curve"], np.float64).reshape(-1, 3))
+                continue
             metals.append((m.get("priority", 0), off + k, m["start"], m["stop"], {"metal": True}))
         self.mat_painter = BoxPainter(mats)
         self.pec_painter = BoxPainter(mats + metals)
         excs = []
+        self.exc_curves = []
         for k, e in enumerate(scene.get("excitations", [])):
+            if "curve" in e:
+                self.exc_curves.append((np.asarray(e["curve"], np.float64).reshape(-1, 3), int(e.get("type", 0)),
+                                        _vec3(e.get("val", (0, 0, 0))), float(e.get("delay", 0.0))))
+                continue
             excs.append((e.get("priority", 0), k, e["start"], e["stop"],
                          {"type": int(e.get("type", 0)), "val": _vec3(e.get("val", (0, 0, 0))),
                           "delay": float(e.get("delay", 0.0)), "enabled": e.get("enabled", True)}))
@@ -701,10 +710,85 @@ class RefOperator:
             c[n] = m.lines[n]
         return c
 
+    def find_path(self, start, stop):
+        """Operator::FindPath (operator.cpp:390-496) after SnapLine2Mesh
+        (operator.cpp:360-388), with CSXCAD's Point_Line_Distance (foot of the
+        perpendicular as a fraction of start -> stop, distance from the line) in
+        plain floats: list of (dir, (x, y, z)) mesh edges."""
+        m = self.mesh
+        L, N = m.lines, self.N
+        start = [float(v) for v in start]
+        stop = [float(v) for v in stop]
+        us = [m.snap_line(n, start[n])[0] for n in range(3)]
+        ue = [m.snap_line(n, stop[n])[0] for n in range(3)]
+        for n in range(3):
+            lo, hi = float(L[n][0]), float(L[n][-1])
+            if (start[n] < lo and stop[n] < lo) or (start[n] > hi and stop[n] > hi):
+                return []
+        d = [stop[k] - start[k] for k in range(3)]
+        LL = d[0] * d[0] + d[1] * d[1] + d[2] * d[2]
+
+        def pld(P):
+            if LL == 0:
+                return math.nan, math.nan
+            f = ((P[0] - start[0]) * d[0] + (P[1] - start[1]) * d[1] + (P[2] - start[2]) * d[2]) / LL
+            q = [P[k] - (start[k] + f * d[k]) for k in range(3)]
+            return f, math.sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2])
+
+        cur = list(us)
+        startFoot = pld([float(L[k][us[k]]) for k in range(3)])[0]
+        stopFoot = pld([float(L[k][ue[k]]) for k in range(3)])[0]
+        currFoot = minFoot = startFoot
+        minDir, up = 0, False
+        path = []
+        while minFoot < stopFoot:
+            minDist = 1e300
+            for n in range(3):
+                P = [float(L[k][cur[k]]) for k in range(3)]
+                if cur[n] - 1 >= 0:
+                    P[n] = float(L[n][cur[n] - 1])
+                    f, dist = pld(P)
+                    if f > currFoot and dist < minDist:
+                        minFoot, minDist, minDir, up = f, dist, n, False
+                if cur[n] + 1 < N[n]:
+                    P[n] = float(L[n][cur[n] + 1])
+                    f, dist = pld(P)
+                    if f > currFoot and dist < minDist:
+                        minFoot, minDist, minDir, up = f, dist, n, True
+            pos = list(cur)
+            if up:
+                cur[minDir] += 1
+            else:
+                cur[minDir] -= 1
+                pos[minDir] -= 1
+            if any(cur[k] < 0 or cur[k] >= N[k] for k in range(3)):
+                return []  # the path left the mesh: dropped
+            path.append((minDir, tuple(pos)))
+            currFoot = minFoot
+        for n in range(3):
+            if cur[n] > ue[n]:
+                cur[n] -= 1
+                path.append((n, tuple(cur)))
+            elif cur[n] < ue[n]:
+                path.append((n, tuple(cur)))
+        return path
+
+    def curve_edges(self, pts):
+        """Path edges of every segment of a polyline, in order."""
+        out = []
+        for i in range(1, len(pts)):
+            out.extend(self.find_path(pts[i - 1], pts[i]))
+        return out
+
     def calc_pec(self, vv, vi):
         """CalcPEC_Range (operator.cpp:2045-2083): zero vv/vi where the winning
-        MATERIAL|METAL box at the edge centre is a metal."""
-        if not self.scene.get("metals"):
+        MATERIAL|METAL box at the edge centre is a metal; then CalcPEC_Curves
+        (operator.cpp:2086-2118): zero them on the path edges of metal curves."""
+        for pts in self.metal_curves:
+            for n, pos in self.curve_edges(pts):
+                vv[(n,) + pos] = 0
+                vi[(n,) + pos] = 0
+        if not any("curve" not in mt for mt in self.scene.get("metals", [])):
             return
         for n in range(3):
             co = self.yee_coords(n, False)
@@ -1010,9 +1094,39 @@ class RefOperator:
         N = self.N
         m = self.mesh
         volt, curr = [], []
-        if not self.exc_painter.boxes:
-            return (ExcList(np.zeros((3, 0), np.uint32), np.zeros(0, np.uint16),
-                            np.zeros(0, np.float32), np.zeros(0, np.uint32)),) * 2
+        if self.exc_painter.boxes:
+            self._excitation_boxes(vv, vi, ii, iv, dT, volt, curr)
+        # curves after the boxes (operator_ext_excitation.cpp:235-287)
+        ELp = [m.edge_len(n, False) for n in range(3)]
+        for pts, typ, val, delay in self.exc_curves:
+            for n, pos in self.curve_edges(pts):
+                if pos[n] >= N[n] - 1 or typ not in (0, 1):
+                    continue
+                amp = val[n] * ELp[n][pos[n]]
+                if amp != 0:
+                    volt.append((pos, n, f32(amp), int(delay / dT)))
+                if typ == 1:
+                    vv[(n,) + pos] = 0
+                    vi[(n,) + pos] = 0
+
+        def pack(lst):
+            K = len(lst)
+            idx = np.zeros((3, K), np.uint32)
+            d = np.zeros(K, np.uint16)
+            a = np.zeros(K, np.float32)
+            dl = np.zeros(K, np.uint32)
+            for i, (pos, n, amp, delay) in enumerate(lst):
+                idx[:, i] = pos
+                d[i] = n
+                a[i] = amp
+                dl[i] = delay
+            return ExcList(idx, d, a, dl)
+        return pack(volt), pack(curr)
+
+    def _excitation_boxes(self, vv, vi, ii, iv, dT, volt, curr):
+        """Box primitives: scan order z (outer), y, x (inner), then n."""
+        N = self.N
+        m = self.mesh
         win_v = [self.exc_painter.paint_index(self.yee_coords(n, False)) for n in range(3)]
         win_c = [self.exc_painter.paint_index(self.yee_coords(n, True)) for n in range(3)]
         ELp = [m.edge_len(n, False) for n in range(3)]
@@ -1055,20 +1169,6 @@ class RefOperator:
                 if e["type"] == 3:
                     ii[(n,) + pos] = 0
                     iv[(n,) + pos] = 0
-
-        def pack(lst):
-            K = len(lst)
-            idx = np.zeros((3, K), np.uint32)
-            d = np.zeros(K, np.uint16)
-            a = np.zeros(K, np.float32)
-            dl = np.zeros(K, np.uint32)
-            for i, (pos, n, amp, delay) in enumerate(lst):
-                idx[:, i] = pos
-                d[i] = n
-                a[i] = amp
-                dl[i] = delay
-            return ExcList(idx, d, a, dl)
-        return pack(volt), pack(curr)
 
     def upml_slabs(self):
         """Slab ranges of Operator_Ext_UPML::Create_UPML (operator_ext_upml.cpp:69-247)."""

@@ -451,18 +451,35 @@ class RefProbe:
 
 
 class RefFieldDumpFD:
-    """ProcessFieldsFD of an E (0) or H (1) field box with node interpolation
-    (Engine_Interface_FDTD NODE_INTERPOLATE, engine_interface_fdtd.cpp:63-124,
-    :126-204), accumulated in complex<float>."""
+    """ProcessFieldsFD (processfields_fd.cpp:72-107) of an E (0) or H (1)
+    field box with no (interp 0), node (1) or cell (2) interpolation
+    (Engine_Interface_FDTD::GetRawInterpolatedField, engine_interface_fdtd.cpp:63-124;
+    GetRawInterpolatedDualField, :148-204), accumulated in complex<float>.
+    Setup as openEMS::SetupProcessing (openems.cpp:612-681): an H dump takes
+    the dual time and, uninterpolated, the dual mesh; cell interpolation the
+    dual mesh; the box snaps with SnapMethod 1 (operator.cpp:323-337)."""
 
     def __init__(self, op: RefOperator, dump_type, start, stop, freqs, interp=1):
         self.op = op
         self.dump_type = dump_type
         self.interp = interp
         m = op.mesh
-        self.dual_mesh = dump_type == 1 and interp == 0
+        self.dual_mesh = interp == 2 or (dump_type == 1 and interp == 0)
         self.dual_time = dump_type == 1
         dim, s, e = m.snap_box(start, stop, self.dual_mesh)
+        self.enabled = dim >= 0      # a box outside the mesh disables the dump (processing.cpp:182-188)
+        if not self.enabled:
+            s, e = [0, 0, 0], [-1, -1, -1]
+            freqs = []
+        lo = [min(a, b) for a, b in zip(start, stop)]
+        hi = [max(a, b) for a, b in zip(start, stop)]
+        for n in range(3 if self.enabled else 0):
+            line = m.dual_line[n] if self.dual_mesh else m.lines[n]
+            if e[n] > s[n]:
+                if line[s[n]] > lo[n] and s[n] > 0:
+                    s[n] -= 1
+                if line[e[n]] < hi[n] and e[n] < op.N[n] - 1:
+                    e[n] += 1
         self.pos = [np.arange(s[n], e[n] + 1) for n in range(3)]
         nyq = op.nyquist
         self.fd_interval = 0
@@ -481,35 +498,83 @@ class RefFieldDumpFD:
         self.fd_im = np.zeros(shape, np.float32)
 
     def field(self, volt, curr):
-        """CalcField for an E dump with node interpolation, as float32 [3][nx][ny][nz]."""
+        """CalcField (processfields.cpp:283-330) as float32 [3][nx][ny][nz]:
+        a plain per-point loop in double, in the reference's operation order."""
         m = self.op.mesh
         N = self.op.N
+        H = self.dump_type == 1
+        F = curr if H else volt
         P = self.pos
         out = np.zeros((3,) + tuple(len(p) for p in P), np.float32)
-        if self.dump_type != 0 or self.interp != 1:
-            raise NotImplementedError("oracle field dumps: E-field node interpolation only")
         for n in range(3):
-            EL = m.edge_len(n, False)
-            idx = np.ix_(*P)
-            pn = P[n]
-            shape = [1, 1, 1]
-            shape[n] = len(pn)
-            delta = EL[pn]
-            raw = volt[n][idx] / EL[pn].reshape(shape)
-            # lower neighbour (pn-1) raw field, clipped index
-            lo = P.copy()
-            lo[n] = np.clip(pn - 1, 0, None)
-            raw_lo = volt[n][np.ix_(*lo)] / EL[np.clip(pn - 1, 0, None)].reshape(shape)
-            deltaDown = EL[np.clip(pn - 1, 0, None)]
-            with np.errstate(divide="ignore", invalid="ignore"):
-                rel = delta / (delta + deltaDown)
-            rel = rel.reshape(shape)
-            val = raw * (1.0 - rel) + raw_lo * rel
-            top = (pn == N[n] - 1).reshape(shape)
-            bot = (pn == 0).reshape(shape)
-            val = np.where(bot, raw, val)
-            val = np.where(top, raw_lo, val)
-            out[n] = val.astype(np.float32)
+            nP, nPP = (n + 1) % 3, (n + 2) % 3
+            EL = m.edge_len(n, H)
+            Fn = F[n]
+
+            def raw(q):
+                d = float(EL[q[n]])
+                return float(Fn[q]) / d if d else 0.0
+
+            for i, x in enumerate(P[0]):
+                for j, y in enumerate(P[1]):
+                    for k, z in enumerate(P[2]):
+                        p = [int(x), int(y), int(z)]
+                        last = any(p[a] == N[a] - 1 for a in range(3))
+                        if self.interp == 0:
+                            v = raw(tuple(p))
+                        elif not H and self.interp == 1:
+                            if p[n] == N[n] - 1:
+                                q = list(p)
+                                q[n] -= 1
+                                v = raw(tuple(q))
+                            else:
+                                delta = float(EL[p[n]])
+                                v = raw(tuple(p))
+                                if delta == 0:
+                                    v = 0.0
+                                elif p[n] != 0:
+                                    q = list(p)
+                                    q[n] -= 1
+                                    dd = float(EL[p[n] - 1])
+                                    rel = delta / (delta + dd)
+                                    v = v * (1.0 - rel) + raw(tuple(q)) * rel
+                        elif not H:  # cell
+                            if last:
+                                v = 0.0
+                            else:
+                                q = list(p)
+                                v = raw(tuple(q))
+                                q[nP] += 1
+                                v += raw(tuple(q))
+                                q[nPP] += 1
+                                v += raw(tuple(q))
+                                q[nP] -= 1
+                                v += raw(tuple(q))
+                                v /= 4
+                        elif self.interp == 1:  # H node
+                            if last or p[nP] == 0 or p[nPP] == 0:
+                                v = 0.0
+                            else:
+                                q = list(p)
+                                v = raw(tuple(q))
+                                q[nP] -= 1
+                                v += raw(tuple(q))
+                                q[nPP] -= 1
+                                v += raw(tuple(q))
+                                q[nP] += 1
+                                v += raw(tuple(q))
+                                v /= 4
+                        else:  # H cell
+                            if p[n] >= N[n] - 1:
+                                v = 0.0
+                            else:
+                                delta = float(EL[p[n]])
+                                up = float(EL[p[n] + 1])
+                                rel = delta / (delta + up)
+                                q = list(p)
+                                q[n] += 1
+                                v = raw(tuple(p)) * (1.0 - rel) + raw(tuple(q)) * rel
+                        out[n, i, j, k] = np.float32(v)
         return out
 
     def process(self, ts, volt, curr):

@@ -42,6 +42,7 @@ def _fields_equal(eng, ref_eng):
     lambda: scenes.pec_cavity(32, 150),
     lambda: scenes.upml_bricks(48, 150, nbricks=8, dump=False),
     lambda: scenes.mur_layers(40, 36, 32, 200),
+    lambda: scenes.wire_antenna(36, 200),
 ])
 def test_fields_bit_identical(gpu, make, layout, step):
     """Fields after 150-200 steps, both coefficient layouts, every step form."""
@@ -502,3 +503,36 @@ def test_cpp_host_classes_match_python(gpu):
     assert fx(got["energy_b"]) == eng.energy()
     eng.close()
     op.close()
+
+
+def test_fd_dumps_e_and_h_every_interpolation(gpu):
+    """ProcessFieldsFD of E (dump type 10) and H (type 11) with no, node and
+    cell interpolation, on an off-line plane and an off-line volume of the
+    wire-antenna scene: within 1e-5 relative L2 of the oracle's dumps."""
+    scene = scenes.wire_antenna(36, 300)
+    L = scene["lines"]
+    c = 18
+    plane = ((L[0][3] + 0.1, L[1][2] - 0.05, L[2][c] + 0.13), (L[0][33] - 0.2, L[1][34] + 0.1, L[2][c] + 0.13))
+    vol = ((L[0][c - 4] + 0.2, L[1][c - 3] - 0.1, L[2][c - 5] + 0.1), (L[0][c + 5] - 0.1, L[1][c + 4], L[2][c + 6] - 0.2))
+    dumps = []
+    for field in ("E", "H"):
+        for interp in (0, 1, 2):
+            for k, (a, b) in enumerate((plane, vol)):
+                dumps.append({"name": f"{field}{interp}_{k}", "start": a, "stop": b, "freqs": [8e9, 15e9],
+                              "field": field, "interp": interp})
+    scene["dumps"] = dumps
+    print("case wire_antenna_36 300 steps, 12 FD dumps (E/H x none/node/cell x plane/volume)", flush=True)
+    sim = Simulation(scene)
+    sim.run()
+    res = sim.results()
+    ref = _ref_from_scene(scene)
+    rds = [RefFieldDumpFD(ref, 0 if d["field"] == "E" else 1, d["start"], d["stop"], d["freqs"], interp=d["interp"])
+           for d in dumps]
+    run_reference(ref, 300, dumps=rds)
+    for d, rd in zip(dumps, rds):
+        got = res["dumps"][d["name"]]["fd"]
+        want = rd.fd_re + 1j * rd.fd_im
+        assert got.shape == want.shape, d["name"]
+        assert np.abs(want).max() > 0, d["name"]
+        assert rel_l2(got, want) <= 1e-5, (d["name"], rel_l2(got, want))
+    sim.close()

@@ -164,3 +164,66 @@ def test_lorentz_and_lumped_rlc_tables(native):
     # series: 4 z edges x 2 x 2 columns; parallel: 4 z edges x 2 x 1; LC: 2 y edges
     assert rt["dir"].size == 16 + 8 + 2
     assert rt["dir"].tolist() == [2] * 24 + [1] * 2
+
+
+def test_wire_paths_match_oracle(native):
+    """Operator::FindPath: the product's mesh-edge paths equal the oracle's
+    for random segments (inside, partly outside and wholly outside a graded
+    mesh), edge for edge."""
+    scene = scenes.wire_antenna(30, 10)
+    op = Operator(scene, nthreads=2)
+    ref = RefOperator(scene)
+    L = scene["lines"]
+    rng = np.random.default_rng(11)
+    lengths = []
+    for _ in range(400):
+        a = [rng.uniform(l[0] - 3, l[-1] + 3) for l in L]
+        b = [rng.uniform(l[0] - 3, l[-1] + 3) for l in L]
+        got = op.find_path(a, b)
+        assert got == ref.find_path(a, b), (a, b)
+        lengths.append(len(got))
+    assert max(lengths) > 20 and min(lengths) == 0
+
+
+def test_wire_curves_metal_and_excitation(native):
+    """Metal polylines (CalcPEC_Curves) and soft / hard excitation curves
+    (Operator_Ext_Excitation): coefficients and excitation lists bit-identical."""
+    scene = scenes.wire_antenna(36, 100)
+    op, ref = _compare(scene)
+    e = op.excitation(False)
+    assert e["dir"].size == 14
+    # the metal wire's edges carry no voltage update
+    arm = ref.curve_edges(ref.metal_curves[0])
+    assert len(arm) > 10
+    vv, vi = op.coeff("vv"), op.coeff("vi")
+    for n, pos in arm:
+        assert vv[(n,) + pos] == 0 and vi[(n,) + pos] == 0
+
+
+@pytest.mark.parametrize("field,interp", [("E", 0), ("E", 1), ("E", 2), ("H", 0), ("H", 1), ("H", 2)])
+def test_field_dump_boxes_match_oracle(native, field, interp):
+    """ProcessFields box snapping (SnapMethod 1, dual mesh for H without
+    interpolation and for cell interpolation): the product's dump box equals
+    the oracle's for random boxes on a graded mesh."""
+    from openems_amd.processing import FieldDumpFD
+    from oracle.oracle import RefFieldDumpFD
+    scene = scenes.wire_antenna(30, 50)
+    op = Operator(scene, nthreads=2)
+    ref = RefOperator(scene).build()
+    L = scene["lines"]
+    rng = np.random.default_rng(5)
+    for _ in range(40):
+        a = [rng.uniform(l[0] - 1, l[-1] + 1) for l in L]
+        b = [rng.uniform(l[0] - 1, l[-1] + 1) for l in L]
+        if rng.random() < 0.3:
+            b[2] = a[2]
+        spec = {"start": a, "stop": b, "freqs": [5e9], "field": field, "interp": interp}
+        d = FieldDumpFD(op, spec, 50)
+        r = RefFieldDumpFD(ref, 0 if field == "E" else 1, a, b, [5e9], interp=interp)
+        assert d.enabled == r.enabled
+        if not r.enabled:
+            assert d.fd_interval == 0
+            continue
+        assert list(d.start) == [int(p[0]) for p in r.pos]
+        assert list(d.stop) == [int(p[-1]) for p in r.pos]
+        assert d.dual_mesh == r.dual_mesh

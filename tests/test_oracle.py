@@ -419,3 +419,203 @@ def test_debye_coefficients_closed_form():
         assert np.allclose(o0["v_int"][n][act], (2 * dT - tau) / tau, rtol=1e-6, atol=0)
         assert np.allclose(o0["v_ext"][n][act], (2 * CL * vi / tau)[act], rtol=1e-6, atol=0)
         assert np.allclose(o0["v_lor"][n][act], (dT / (CL * vi))[act], rtol=1e-6, atol=0)
+
+
+def _wire_op(lines):
+    sc = {"lines": lines, "delta_unit": 1e-3, "signal": {"type": "gauss", "f0": 0.0, "fc": 10e9}, "nrts": 10}
+    return RefOperator(sc)
+
+
+def _walk(path, start_node):
+    """Follow a path edge by edge from start_node; returns the nodes visited
+    (None if an edge does not touch the current node)."""
+    cur = list(start_node)
+    nodes = [tuple(cur)]
+    for n, pos in path:
+        lo = list(pos)
+        hi = list(pos)
+        hi[n] += 1
+        if cur == lo:
+            cur = hi
+        elif cur == hi:
+            cur = lo
+        else:
+            return None
+        nodes.append(tuple(cur))
+    return nodes
+
+
+def test_wire_path_along_a_mesh_line():
+    """FindPath of a wire on a mesh line: exactly its edges, from the start
+    node upward, or downward (edge at the lower node) when reversed."""
+    l = np.arange(16.0)
+    op = _wire_op([l, l.copy(), l.copy()])
+    up = op.find_path((3, 5, 2), (3, 5, 9))
+    assert up == [(2, (3, 5, k)) for k in range(2, 9)]
+    down = op.find_path((3, 5, 9), (3, 5, 2))
+    assert down == [(2, (3, 5, k)) for k in range(8, 1, -1)]
+    assert op.find_path((1, 2, 4), (7, 2, 4)) == [(0, (k, 2, 4)) for k in range(1, 7)]
+    # points off the lines snap to the nearest node first
+    assert op.find_path((2.7, 5.2, 1.6), (3.3, 4.9, 9.4)) == up
+
+
+def test_wire_path_diagonal_staircase_ties_go_to_x():
+    """A 45-degree diagonal on a uniform mesh: the x and y candidates are
+    exactly equally far from the line, and the first candidate (x) wins, so
+    the staircase alternates x, y from the start node."""
+    l = np.arange(12.0)
+    op = _wire_op([l, l.copy(), l.copy()])
+    p = op.find_path((2, 3, 4), (6, 7, 4))
+    assert [n for n, _ in p] == [0, 1] * 4
+    assert _walk(p, (2, 3, 4))[-1] == (6, 7, 4)
+
+
+def test_wire_path_properties_on_a_graded_mesh():
+    """Random segments inside a graded mesh: the path is a connected walk of
+    mesh edges from the start node to the stop node; every step of the search
+    moves along the sign of the line's direction on its axis (only the closing
+    edges, at most one per axis at the end, may step back after the search
+    overshot the stop node); and each visited node lies closer to the line
+    than the longest cell diagonal."""
+    lines = [scenes.graded_lines(24, 0.5, 3, 1.25) for _ in range(3)]
+    op = _wire_op(lines)
+    rng = np.random.default_rng(3)
+    diag = math.sqrt(sum(float(np.max(np.diff(l))) ** 2 for l in lines))
+    for _ in range(200):
+        a = [rng.uniform(l[0], l[-1]) for l in lines]
+        b = [rng.uniform(l[0], l[-1]) for l in lines]
+        p = op.find_path(a, b)
+        us = [op.mesh.snap_line(n, a[n])[0] for n in range(3)]
+        ue = [op.mesh.snap_line(n, b[n])[0] for n in range(3)]
+        nodes = _walk(p, us)
+        assert nodes is not None
+        assert list(nodes[-1]) == ue
+        d = np.subtract(b, a)
+        back = [t for t, (u, v) in enumerate(zip(nodes[:-1], nodes[1:]))
+                if np.sign(np.subtract(v, u)[p[t][0]]) != np.sign(d[p[t][0]])]
+        assert all(t >= len(p) - 3 for t in back)
+        assert len(p) == sum(abs(ue[n] - us[n]) for n in range(3)) + 2 * len(back)
+        for u in nodes:
+            P = np.array([lines[k][u[k]] for k in range(3)])
+            f = np.dot(P - a, d) / np.dot(d, d)
+            assert np.linalg.norm(P - (a + f * d)) < diag
+
+
+def test_wire_path_outside_the_mesh_is_empty():
+    l = np.arange(10.0)
+    op = _wire_op([l, l.copy(), l.copy()])
+    assert op.find_path((-3, 2, 2), (-1, 5, 5)) == []       # wholly below x line 0
+    assert op.find_path((2, 2, 12), (5, 5, 11)) == []       # wholly above the last z line
+    assert op.find_path((2, 2, 2), (2, 2, 2)) == []         # zero length
+
+
+def _dump_op(lines):
+    sc = {"lines": lines, "delta_unit": 1e-3, "signal": {"type": "gauss", "f0": 0.0, "fc": 10e9}, "nrts": 10}
+    return RefOperator(sc).build()
+
+
+def test_field_dump_node_interpolation_is_linear_between_edge_centres():
+    """E with node interpolation on a graded mesh: a field whose value over
+    the edge length is linear in the edge-centre coordinate along its own
+    axis comes out as that linear function at the node (interior nodes), and
+    as the one neighbouring edge value at the first and last line."""
+    from oracle.oracle import RefFieldDumpFD
+    lines = [scenes.graded_lines(12, 1.0, 3, 1.3) for _ in range(3)]
+    op = _dump_op(lines)
+    m = op.mesh
+    N = op.N
+    volt = np.zeros((3,) + N, np.float32)
+    curr = np.zeros_like(volt)
+    f = lambda c: 0.25 + 0.125 * c          # exact in float32 for these coordinates
+    for n in range(3):
+        cen = 0.5 * (lines[n][:-1] + lines[n][1:])
+        raw = np.zeros(N[n])
+        raw[:-1] = f(cen)
+        raw[-1] = f(cen[-1])
+        EL = m.edge_len(n, False)
+        shape = [1, 1, 1]
+        shape[n] = N[n]
+        volt[n] = (raw * EL).reshape(shape).astype(np.float32)
+        volt[n] *= np.ones(N, np.float32)
+    d = RefFieldDumpFD(op, 0, [l[0] for l in lines], [l[-1] for l in lines], [1e9], interp=1)
+    out = d.field(volt, curr)
+    for n in range(3):
+        shape = [1, 1, 1]
+        shape[n] = N[n]
+        want = f(np.asarray(lines[n], float))
+        cen = 0.5 * (lines[n][:-1] + lines[n][1:])
+        want[0] = f(cen[0])
+        want[-1] = f(cen[-1])
+        got = np.moveaxis(out[n], n, 0)
+        np.testing.assert_allclose(got, want.reshape((-1, 1, 1)) * np.ones_like(got), rtol=2e-6)
+
+
+def test_field_dump_cell_and_h_averages_shift_by_half_a_cell():
+    """Uniform mesh, raw field = a + b * (index along n+1): E cell
+    interpolation averages towards +n+1 (a + b (i + 1/2)); H node
+    interpolation towards -n+1 (a + b (i - 1/2)) and is zero where an index
+    along n+1 or n+2 is 0 or any index is the last line; H cell
+    interpolation along n averages i and i + 1 of a field linear in the index
+    along n; no interpolation returns the raw value."""
+    from oracle.oracle import RefFieldDumpFD
+    l = np.arange(10.0)
+    op = _dump_op([l, l.copy(), l.copy()])
+    N = op.N
+    a, b = 0.5, 0.25
+    idx = np.arange(N[0], dtype=float)
+    full = [l[0], l[0], l[0]], [l[-1], l[-1], l[-1]]
+    for n in range(3):
+        nP = (n + 1) % 3
+        shape = [1, 1, 1]
+        shape[nP] = N[nP]
+        lin = (a + b * idx).reshape(shape) * np.ones(N)
+        volt = np.zeros((3,) + N, np.float32)
+        curr = np.zeros_like(volt)
+        volt[n] = (lin * 1e-3).astype(np.float32)     # edge length 1 mm
+        curr[n] = (lin * 1e-3).astype(np.float32)
+        e_cell = RefFieldDumpFD(op, 0, *full, [1e9], interp=2).field(volt, curr)[n]
+        h_node = RefFieldDumpFD(op, 1, *full, [1e9], interp=1).field(volt, curr)[n]
+        e_none = RefFieldDumpFD(op, 0, *full, [1e9], interp=0).field(volt, curr)[n]
+        np.testing.assert_allclose(e_none, lin.astype(np.float32), rtol=1e-6)
+        inner = tuple(slice(0, N[k] - 1) for k in range(3))
+        want = np.moveaxis(np.ones(N) * (a + b * (idx + 0.5)).reshape(shape), 0, 0)
+        np.testing.assert_allclose(e_cell[inner], want[inner], rtol=1e-6)
+        assert not np.any(e_cell[..., -1]) and not np.any(e_cell[-1]) and not np.any(e_cell[:, -1])
+        sl = [slice(0, N[k] - 1) for k in range(3)]
+        sl[nP] = slice(1, N[nP] - 1)
+        sl[(n + 2) % 3] = slice(1, N[(n + 2) % 3] - 1)
+        want = np.ones(N) * (a + b * (idx - 0.5)).reshape(shape)
+        np.testing.assert_allclose(h_node[tuple(sl)], want[tuple(sl)], rtol=1e-6)
+        zero = np.ones(N, bool)
+        zero[tuple(sl)] = False
+        assert not np.any(h_node[zero])
+        # H cell: linear along n itself
+        shape_n = [1, 1, 1]
+        shape_n[n] = N[n]
+        lin_n = (a + b * idx).reshape(shape_n) * np.ones(N)
+        curr[n] = (lin_n * 1e-3).astype(np.float32)
+        h_cell = RefFieldDumpFD(op, 1, *full, [1e9], interp=2).field(volt, curr)[n]
+        want = np.ones(N) * (a + b * (idx + 0.5)).reshape(shape_n)
+        keep = [slice(None)] * 3
+        keep[n] = slice(0, N[n] - 1)
+        np.testing.assert_allclose(h_cell[tuple(keep)], want[tuple(keep)], rtol=1e-6)
+        last = [slice(None)] * 3
+        last[n] = N[n] - 1
+        assert not np.any(h_cell[tuple(last)])
+
+
+def test_field_dump_box_snaps_outward_and_h_takes_the_dual_mesh():
+    """SnapMethod 1: a box edge between lines widens to the next line
+    outside; an H dump without interpolation snaps on the dual mesh and
+    samples at the dual time."""
+    from oracle.oracle import RefFieldDumpFD
+    l = np.arange(12.0)
+    op = _dump_op([l, l.copy(), l.copy()])
+    d = RefFieldDumpFD(op, 0, [2.3, 3.0, 4.0], [6.6, 8.0, 4.0], [1e9])
+    assert [int(p[0]) for p in d.pos] == [2, 3, 4] and [int(p[-1]) for p in d.pos] == [7, 8, 4]
+    h = RefFieldDumpFD(op, 1, [2.3, 3.0, 4.0], [6.6, 8.0, 4.0], [1e9], interp=0)
+    assert h.dual_mesh and h.dual_time
+    # dual line i sits at i + 0.5 and a coordinate snaps to the dual line
+    # below the first primary line at or above it: x 2.3 -> 2 (2.5), widened
+    # to 1 as 2.5 > 2.3; x 6.6 -> 6 (6.5), widened to 7; z 4.0 -> 3 both ends
+    assert [int(p[0]) for p in h.pos] == [1, 2, 3] and [int(p[-1]) for p in h.pos] == [7, 8, 3]
