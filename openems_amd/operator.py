@@ -315,13 +315,29 @@ class Operator:
         lib().emsop_snap_to_mesh(self.h, ptr(c, C.c_double), int(dual), out, ins)
         return list(out), [bool(v) for v in ins]
 
-    def snap_box(self, start, stop, dual=False):
+    def snap_box(self, start, stop, dual=False, method=0):
+        """SnapBox2Mesh (FDTD/operator.cpp:297-358): (dim, start index, stop
+        index, start inside, stop inside).  Method 0 snaps each corner to its
+        nearest line; method 1 then widens a non-flat axis to the next line
+        outside where the snapped line falls inside the box."""
         a = np.asarray(start, np.float64)
         b = np.asarray(stop, np.float64)
         us, ue = (C.c_uint * 3)(), (C.c_uint * 3)()
         si, ei = (C.c_int * 3)(), (C.c_int * 3)()
         dim = lib().emsop_snap_box(self.h, ptr(a, C.c_double), ptr(b, C.c_double), int(dual), us, ue, si, ei)
-        return dim, list(us), list(ue), [bool(v) for v in si], [bool(v) for v in ei]
+        us, ue = list(us), list(ue)
+        if method == 1 and dim >= 0:
+            dim = 0
+            for n in range(3):
+                lo, hi = min(a[n], b[n]), max(a[n], b[n])
+                if ue[n] > us[n]:
+                    if self.disc_line(n, us[n], dual) > lo and us[n] > 0:
+                        us[n] -= 1
+                    if self.disc_line(n, ue[n], dual) < hi and ue[n] < self.N[n] - 1:
+                        ue[n] += 1
+                if ue[n] > us[n]:
+                    dim += 1
+        return dim, us, ue, [bool(v) for v in si], [bool(v) for v in ei]
 
     def find_path(self, start, stop):
         """Operator::FindPath (FDTD/operator.cpp:390-496): list of (dir, (x, y, z))

@@ -30,6 +30,18 @@ _libm.sinf.restype = ctypes.c_float
 _libm.sinf.argtypes = [ctypes.c_float]
 
 PROBE_NAMES = {0: "voltage", 1: "current", 2: "E-field", 3: "H-field"}
+# GetProcessingName / GetIntegralName of ProcessVoltage, ProcessCurrent and
+# ProcessFieldProbe (Common/processvoltage.h:30, processvoltage.cpp:30-35,
+# processcurrent.h:29, processcurrent.cpp:35-40, processfieldprobe.cpp:32-64)
+PROCESSING_NAMES = {0: "voltage integration", 1: "current integration", 2: "electric field probe",
+                    3: "magnetic field probe"}
+INTEGRAL_NAMES = {0: ["voltage"], 1: ["current"], 2: ["Ex/(V/m)", "Ey/(V/m)", "Ez/(V/m)"],
+                  3: ["Hx/(A/m)", "Hy/(A/m)", "Hz/(A/m)"]}
+
+
+def _g(v, prec=6):
+    """A double as a C++ ostream prints it at the given precision (%g form)."""
+    return "%.*g" % (prec, v)
 
 
 def nyquist_num(f, dT):
@@ -82,7 +94,8 @@ class Probe:
             stop, _ = op.snap_to_mesh(s["stop"], False)
             self.start, self.stop = start, stop
             if sum(1 for n in range(3) if start[n] != stop[n]) != 1:
-                self.enabled = False
+                # not a line: the probe stays on and records zeros
+                # (CalcVoltageIntegral, FDTD/engine_interface_fdtd.cpp:208-212)
                 return
             for n in range(3):
                 if start[n] < stop[n]:
@@ -96,7 +109,8 @@ class Probe:
                         self._add(n, p, -1)
                         p[n] += 1
             return
-        dim, st, sp, sin, ein = op.snap_box(s["start"], s["stop"], self.dual)
+        # ProcessCurrent snaps with SnapMethod 1 (processcurrent.cpp:28), field probes with 0
+        dim, st, sp, sin, ein = op.snap_box(s["start"], s["stop"], self.dual, method=1 if self.ptype == 1 else 0)
         if dim < 0:
             self.enabled = False
             return
@@ -180,6 +194,52 @@ class Probe:
             self._add(n, p, sg)
         self.norm_dir = nd
 
+    # ---- probe files ----------------------------------------------------------------
+    def _coords_header(self):
+        op = self.op
+        gd = float(op.scene.get("delta_unit", 1.0))
+        out = []
+        for label, p in (("start", self.start), ("stop", self.stop)):
+            xyz = ",".join(_g(op.disc_line(n, p[n]) * gd) for n in range(3))
+            idx = ",".join(str(int(v)) for v in p)
+            out.append(f"% {label}-coordinates: ({xyz}) m -> [{idx}]\n")
+        return out
+
+    def write_files(self, td, fd, directory, version="openems_amd", when=None):
+        """Write the probe's time-domain file <name> and, with frequencies, its
+        frequency-domain file <name>_FD in the reference's text layout:
+        header of ProcessIntegral::InitProcess (Common/processintegral.cpp:41-72),
+        rows of ProcessIntegral::Process (:142-151; time and values at 12
+        significant digits) and ProcessIntegral::Dump_FD_Data (:92-132; 6
+        digits).  A disabled probe writes nothing.  Returns the paths written."""
+        import os
+        import time as _time
+        if not self.enabled:
+            return []
+        stamp = _time.ctime(when)
+        names = INTEGRAL_NAMES[self.ptype]
+        proc = PROCESSING_NAMES[self.ptype]
+        path = os.path.join(directory, self.name)
+        lines = [f"% time-domain {proc} by openEMS {version} @{stamp}\n"] + self._coords_header()
+        lines.append("% t/s" + "".join("\t" + n for n in names) + "\n")
+        for row in np.asarray(td):
+            lines.append(_g(float(row[0]), 12) + "".join("\t" + _g(float(v), 12) for v in row[1:]) + "\n")
+        with open(path, "w") as f:
+            f.writelines(lines)
+        paths = [path]
+        if self.freqs:
+            fd = np.asarray(fd)
+            lines = [f"% frequency-domain {proc} by openEMS {version} @{stamp}\n"] + self._coords_header()
+            lines.append("% f/Hz" + "".join("\t" + n + "\t" for n in names) + "\n")
+            lines.append("%" + "\treal\timag" * len(names) + "\n")
+            for j, f0 in enumerate(self.freqs):
+                lines.append(_g(f0) + "".join("\t" + _g(fd[i, j].real) + "\t" + _g(fd[i, j].imag)
+                                              for i in range(len(names))) + "\n")
+            with open(path + "_FD", "w") as f:
+                f.writelines(lines)
+            paths.append(path + "_FD")
+        return paths
+
     # ---- engine registration and results -----------------------------------------
     def register(self, eng):
         if not self.enabled or not self.comp:
@@ -238,19 +298,9 @@ class FieldDumpFD:
         self.dual_time = self.field == 1
         dual = self.interp == 2 or (self.field == 1 and self.interp == 0)
         self.dual_mesh = dual
-        dim, st, sp, _, _ = op.snap_box(spec["start"], spec["stop"], dual)
+        dim, st, sp, _, _ = op.snap_box(spec["start"], spec["stop"], dual, method=1)
         # a box outside the mesh disables the dump (Processing::DefineStartStopCoord, processing.cpp:180-188)
         self.enabled = dim >= 0
-        lo = [min(a, b) for a, b in zip(spec["start"], spec["stop"])]
-        hi = [max(a, b) for a, b in zip(spec["start"], spec["stop"])]
-        for n in range(3):
-            if sp[n] > st[n]:
-                if op.disc_line(n, st[n], dual) > lo[n] and st[n] > 0:
-                    st[n] -= 1
-                if op.disc_line(n, sp[n], dual) < hi[n] and sp[n] < op.N[n] - 1:
-                    sp[n] += 1
-            if st[n] > sp[n]:
-                st[n], sp[n] = sp[n], st[n]
         self.start, self.stop = st, sp
         nyq = op.nyquist
         self.fd_interval = 0

@@ -66,6 +66,16 @@ class Simulation:
             out["dumps"][d.name] = {"fd": d.collect(self.eng), "freqs": d.freqs}
         return out
 
+    def write_probes(self, directory):
+        """Probe text files in the reference's layout (one per probe, plus
+        <name>_FD where the probe has frequencies), as openEMS leaves them in
+        its simulation folder; returns the paths written."""
+        paths = []
+        for p in self.probes:
+            td, fd = p.collect(self.eng)
+            paths += p.write_files(td, fd, directory)
+        return paths
+
     def close(self):
         self.eng.close()
         self.op.close()

@@ -127,8 +127,10 @@ class Mesh:
             inside.append(ok)
         return idx, inside
 
-    def snap_box(self, start, stop, dual=False):
-        """SnapBox2Mesh with SnapMethod 0 (operator.cpp:303-327)."""
+    def snap_box(self, start, stop, dual=False, method=0):
+        """SnapBox2Mesh (operator.cpp:297-358), SnapMethod 0 (each corner to its
+        nearest line) or 1 (then a non-flat axis widens to the next line outside
+        where the snapped line lies inside the box)."""
         lstart = [min(a, b) for a, b in zip(start, stop)]
         lstop = [max(a, b) for a, b in zip(start, stop)]
         for n in range(3):
@@ -137,6 +139,14 @@ class Mesh:
                 return -2, None, None
         s, _ = self.snap(lstart, dual)
         e, _ = self.snap(lstop, dual)
+        if method == 1:
+            for n in range(3):
+                line = self.dual_line[n] if dual else self.lines[n]
+                if e[n] > s[n]:
+                    if line[s[n]] > lstart[n] and s[n] > 0:
+                        s[n] -= 1
+                    if line[e[n]] < lstop[n] and e[n] < self.N[n] - 1:
+                        e[n] += 1
         dim = sum(1 for n in range(3) if e[n] > s[n])
         return dim, s, e
 

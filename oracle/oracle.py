@@ -292,7 +292,8 @@ class RefProbe:
             self.start, self.start_in = m.snap(start, False)
             self.stop, self.stop_in = m.snap(stop, False)
         else:
-            dim, s, e = m.snap_box(start, stop, self.dual_mesh)
+            # ProcessCurrent snaps with SnapMethod 1 (processcurrent.cpp:28), field probes with 0
+            dim, s, e = m.snap_box(start, stop, self.dual_mesh, method=1 if ptype == 1 else 0)
             if dim < 0:
                 self.enabled = False
                 return
@@ -466,20 +467,11 @@ class RefFieldDumpFD:
         m = op.mesh
         self.dual_mesh = interp == 2 or (dump_type == 1 and interp == 0)
         self.dual_time = dump_type == 1
-        dim, s, e = m.snap_box(start, stop, self.dual_mesh)
+        dim, s, e = m.snap_box(start, stop, self.dual_mesh, method=1)
         self.enabled = dim >= 0      # a box outside the mesh disables the dump (processing.cpp:182-188)
         if not self.enabled:
             s, e = [0, 0, 0], [-1, -1, -1]
             freqs = []
-        lo = [min(a, b) for a, b in zip(start, stop)]
-        hi = [max(a, b) for a, b in zip(start, stop)]
-        for n in range(3 if self.enabled else 0):
-            line = m.dual_line[n] if self.dual_mesh else m.lines[n]
-            if e[n] > s[n]:
-                if line[s[n]] > lo[n] and s[n] > 0:
-                    s[n] -= 1
-                if line[e[n]] < hi[n] and e[n] < op.N[n] - 1:
-                    e[n] += 1
         self.pos = [np.arange(s[n], e[n] + 1) for n in range(3)]
         nyq = op.nyquist
         self.fd_interval = 0

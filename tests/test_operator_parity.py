@@ -227,3 +227,52 @@ def test_field_dump_boxes_match_oracle(native, field, interp):
         assert list(d.start) == [int(p[0]) for p in r.pos]
         assert list(d.stop) == [int(p[-1]) for p in r.pos]
         assert d.dual_mesh == r.dual_mesh
+
+
+@pytest.mark.parametrize("ptype", [0, 1, 2, 3])
+def test_probe_boxes_match_oracle(native, ptype):
+    """Probe snapping: voltage lines (SnapToMesh), current surfaces (dual
+    mesh, SnapMethod 1, normal-direction expansion) and field points: the
+    product's probe box and enable state equal the oracle's for random
+    off-line boxes."""
+    from openems_amd.processing import Probe
+    from oracle.oracle import RefProbe
+    scene = scenes.wire_antenna(30, 50)
+    op = Operator(scene, nthreads=2)
+    ref = RefOperator(scene).build()
+    L = scene["lines"]
+    rng = np.random.default_rng(17 + ptype)
+    seen = 0
+    for _ in range(60):
+        a = [rng.uniform(l[0], l[-1]) for l in L]
+        b = [rng.uniform(l[0], l[-1]) for l in L]
+        nd = -1
+        if ptype == 1:
+            nd = int(rng.integers(-1, 3))
+            k = nd if nd >= 0 else int(rng.integers(0, 3))
+            b[k] = a[k]
+        elif ptype == 0:
+            k = int(rng.integers(0, 3))
+            b = list(a)
+            b[k] = rng.uniform(L[k][0], L[k][-1])
+        spec = {"type": ptype, "start": a, "stop": b, "weight": 1.0, "freqs": [], "norm_dir": nd}
+        p = Probe(op, spec, 50)
+        r = RefProbe(ref, ptype, a, b, 1.0, [], nd)
+        assert p.enabled == r.enabled
+        if r.enabled:
+            seen += 1
+            assert list(p.start) == list(r.start) and list(p.stop) == list(r.stop)
+    assert seen > 10
+
+
+def test_current_probe_box_widens_with_snap_method_1(native):
+    """An off-line current box widens to the next dual line outside on each
+    non-flat axis (SnapMethod 1 of ProcessCurrent)."""
+    from oracle.oracle import RefProbe
+    l = np.arange(16.0)
+    sc = {"lines": [l, l.copy(), l.copy()], "delta_unit": 1e-3,
+          "signal": {"type": "gauss", "f0": 0.0, "fc": 10e9}, "nrts": 10}
+    ref = RefOperator(sc).build()
+    # dual lines at i + 0.5; x 3.7..9.2 snaps to 3 (3.5) .. 8 (8.5), both inside -> widened to 3 .. 9
+    r = RefProbe(ref, 1, (3.7, 4.6, 7.5), (9.2, 10.4, 7.5), 1.0, [], 2)
+    assert r.enabled and r.start[:2] == [3, 4] and r.stop[:2] == [9, 10]

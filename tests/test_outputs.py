@@ -1,0 +1,70 @@
+"""Output files in the reference's layout (CPU): probe text files as
+ProcessIntegral writes them (Common/processintegral.cpp:41-151), readable by
+the reference's own post-processing (np.loadtxt with '%' comments,
+python/openEMS/ports.py:27-46)."""
+import re
+
+import numpy as np
+
+from openems_amd import scenes
+from openems_amd.operator import Operator
+from openems_amd.processing import Probe
+
+
+def _probe(op, spec):
+    return Probe(op, spec, 200)
+
+
+def test_probe_text_files_layout_and_round_trip(native, tmp_path):
+    scene = scenes.wire_antenna(30, 200)
+    op = Operator(scene, nthreads=2)
+    L = scene["lines"]
+    spec = {"type": 0, "name": "port_ut", "start": (L[0][15], L[1][15], L[2][13]),
+            "stop": (L[0][15], L[1][15], L[2][17]), "weight": -1.0, "freqs": [5e9, 7.5e9], "norm_dir": -1}
+    p = _probe(op, spec)
+    rng = np.random.default_rng(2)
+    t = (np.arange(12) * p.interval) * op.dT
+    td = np.stack([t, rng.normal(size=12) * 1e-3], axis=1)
+    fd = rng.normal(size=(1, 2)) + 1j * rng.normal(size=(1, 2))
+    paths = p.write_files(td, fd, str(tmp_path), when=0)
+    assert [pp.split("/")[-1] for pp in paths] == ["port_ut", "port_ut_FD"]
+    lines = open(paths[0]).read().split("\n")
+    assert re.fullmatch(r"% time-domain voltage integration by openEMS \S+ @\w{3} \w{3} [ \d]\d \d\d:\d\d:\d\d \d{4}",
+                        lines[0])
+    gd = scene["delta_unit"]
+    m = re.fullmatch(r"% start-coordinates: \(([^)]*)\) m -> \[(\d+),(\d+),(\d+)\]", lines[1])
+    assert m and [int(v) for v in m.groups()[1:]] == [15, 15, 13]
+    assert np.allclose([float(v) for v in m.group(1).split(",")], [L[0][15] * gd, L[1][15] * gd, L[2][13] * gd],
+                       rtol=1e-5)
+    assert lines[2].startswith("% stop-coordinates: (") and lines[2].endswith("-> [15,15,17]")
+    assert lines[3] == "% t/s\tvoltage"
+    back = np.loadtxt(paths[0], comments="%")
+    np.testing.assert_allclose(back, td, rtol=1e-11)
+    fl = open(paths[1]).read().split("\n")
+    assert fl[0].startswith("% frequency-domain voltage integration by openEMS ")
+    assert fl[3] == "% f/Hz\tvoltage\t" and fl[4] == "%\treal\timag"
+    fb = np.loadtxt(paths[1], comments="%")
+    np.testing.assert_allclose(fb[:, 0], [5e9, 7.5e9])
+    np.testing.assert_allclose(fb[:, 1] + 1j * fb[:, 2], fd[0], rtol=1e-5)
+
+
+def test_field_probe_and_current_probe_headers(native, tmp_path):
+    scene = scenes.wire_antenna(30, 200)
+    op = Operator(scene, nthreads=2)
+    L = scene["lines"]
+    e = _probe(op, {"type": 2, "name": "ep", "start": (L[0][10],) * 1 + (L[1][11], L[2][12]),
+                    "stop": (L[0][10], L[1][11], L[2][12]), "freqs": []})
+    td = np.array([[0.0, 1.0, -2.0, 3.5e-7], [1e-12, 0.25, 0.5, 0.125]])
+    paths = e.write_files(td, None, str(tmp_path), when=0)
+    lines = open(paths[0]).read().split("\n")
+    assert lines[0].startswith("% time-domain electric field probe by openEMS ")
+    assert lines[3] == "% t/s\tEx/(V/m)\tEy/(V/m)\tEz/(V/m)"
+    assert lines[4] == "0\t1\t-2\t3.5e-07" and lines[5] == "1e-12\t0.25\t0.5\t0.125"
+    c = _probe(op, {"type": 1, "name": "it", "start": (L[0][12], L[1][12], L[2][15]),
+                    "stop": (L[0][18], L[1][18], L[2][15]), "freqs": [], "norm_dir": 2})
+    paths = c.write_files(np.zeros((1, 2)), None, str(tmp_path), when=0)
+    lines = open(paths[0]).read().split("\n")
+    assert lines[0].startswith("% time-domain current integration by openEMS ") and lines[3] == "% t/s\tcurrent"
+    # a probe outside the mesh is disabled and writes nothing
+    off = _probe(op, {"type": 2, "name": "off", "start": (-50, -50, -50), "stop": (-40, -40, -40), "freqs": []})
+    assert not off.enabled and off.write_files(np.zeros((0, 4)), None, str(tmp_path)) == []
