@@ -7,7 +7,9 @@ is due (openems.cpp:1388-1494).  For a periodic excitation it sets up the
 steady-state detection of openems.cpp:1205-1234; with
 scene["end_criteria"] > 0 the run stops once its last difference falls to the
 criterion (checked every period, where the reference checks every few
-seconds of wall time).
+seconds of wall time).  Without a periodic excitation the criterion is the
+reference's energy decay (field energy over its largest sample), checked at a
+fixed step interval.
 """
 from __future__ import annotations
 
@@ -46,6 +48,8 @@ class Simulation:
     def run(self, nsteps=None, chunk=0):
         n = self.nrts if nsteps is None else int(nsteps)
         end = float(self.scene.get("end_criteria", 0.0))
+        if end > 0 and not self.ss_period:
+            return self._run_energy(n, end)
         if end > 0 and self.ss_period:
             chunk = self.ss_period if chunk <= 0 else min(chunk, self.ss_period)
         done = 0
@@ -55,6 +59,36 @@ class Simulation:
             done += k
             if end > 0 and self.ss_period and self.eng.steady_state(1)[0] <= end:
                 break
+        return self
+
+    def _run_energy(self, n, end):
+        """The energy end criterion of openEMS::RunFDTD (openems.cpp:1424-1480):
+        the field energy (CalcFastEnergy) is sampled at the step of the
+        excitation's largest amplitude (Excitation::GetMaxExcitationTimestep,
+        FDTD/excitation.cpp:135-148) and then every scene["energy_check"]
+        steps (default 1000; the reference samples every 4 s of wall time);
+        the run ends once energy / its largest sample falls to end or below."""
+        check = max(1, int(self.scene.get("energy_check", 1000)))
+        sig_v, _ = self.op.signal()
+        amp = np.abs(np.asarray(sig_v, np.float32))
+        max_step = int(np.argmax(amp)) if amp.size and amp.max() > 0 else 0
+        start = self.eng.numTS
+        max_e, change = 0.0, 1.0
+        self.stopped_at = None
+        while self.eng.numTS - start < n and change > end:
+            ts = self.eng.numTS
+            nxt = min(start + n, (ts // check + 1) * check)
+            if ts < max_step:
+                nxt = min(nxt, max_step)
+            self.eng.IterateTS(nxt - ts)
+            ts = self.eng.numTS
+            if ts == max_step or ts % check == 0:
+                e = self.eng.energy()
+                max_e = max(max_e, e)
+                if ts % check == 0 and max_e:
+                    change = e / max_e
+        if change <= end:
+            self.stopped_at = self.eng.numTS
         return self
 
     def results(self):
