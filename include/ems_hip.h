@@ -37,6 +37,7 @@
  *                                                    Common/processvoltage.cpp:37-47, Common/processcurrent.cpp:96-171,
  *                                                    Common/processfieldprobe.cpp:77-92
  *   ems_add_fd_dump       ProcessFieldsFD::Process                       Common/processfields_fd.cpp:72-107
+ *   ems_add_td_dump       ProcessFieldsTD::Process                       Common/processfields_td.cpp:50-91
  *   ems_energy            Engine_Interface_FDTD::CalcFastEnergy          FDTD/engine_interface_fdtd.cpp:302-347
  *   ems_set_steady_state  Operator_Ext_SteadyState -> Engine_Ext_SteadyState
  *   ems_steady_state                                 FDTD/extensions/engine_ext_steadystate.cpp:50-107
@@ -261,6 +262,19 @@ int ems_add_fd_dump(ems_engine* h, int field, int interp, const unsigned start[3
                     const double* const edge_len[3], int* id);
 /* out: [nfreq][3][nx][ny][nz][2] float32 (re, im), dump-box N-I-J-K order */
 int ems_fd_dump_results(ems_engine* h, int id, float* out);
+
+/* ---- time-domain field dump (ProcessFieldsTD::Process, Common/processfields_td.cpp:50-91;
+ * dump types 0 and 1) ----
+ * The E (field 0) or H (field 1) field of the box start..stop, interpolated as
+ * for ems_add_fd_dump, stored after every step ts with ts % interval == 0 and
+ * start_ts <= ts <= stop_ts (Processing::CheckTimestep), at most capacity
+ * samples, all kept on the device until read. */
+int ems_add_td_dump(ems_engine* h, int field, int interp, const unsigned start[3], const unsigned stop[3],
+                    unsigned interval, unsigned start_ts, unsigned stop_ts, unsigned capacity,
+                    const double* const edge_len[3], int* id);
+/* *count = samples taken so far; ts[count] their steps; out[count][3][nx][ny][nz]
+ * float32 (dump-box N-I-J-K order); ts and out may be NULL */
+int ems_td_dump_results(ems_engine* h, int id, unsigned* count, unsigned* ts, float* out);
 
 /* eps0 * sum(V^2) + mu0 * sum(I^2) over [0,N-1)^3 */
 int ems_energy(ems_engine* h, double* out);

@@ -110,6 +110,9 @@ ENGINE_API = [
     ("ems_add_probe", C.c_int, [vp, C.POINTER(ProbeDesc), ip]),
     ("ems_probe_results", C.c_int, [vp, C.c_int, up, up, dp]),
     ("ems_add_fd_dump", C.c_int, [vp, C.c_int, C.c_int, up, up, C.c_uint, C.c_uint, C.c_uint, fp, C.POINTER(dp), ip]),
+    ("ems_add_td_dump", C.c_int, [vp, C.c_int, C.c_int, up, up, C.c_uint, C.c_uint, C.c_uint, C.c_uint,
+                                  C.POINTER(dp), ip]),
+    ("ems_td_dump_results", C.c_int, [vp, C.c_int, up, up, fp]),
     ("ems_fd_dump_results", C.c_int, [vp, C.c_int, fp]),
     ("ems_energy", C.c_int, [vp, dp]),
     ("ems_last_timing", C.c_int, [vp, dp, dp, dp]),

@@ -1325,16 +1325,28 @@ __global__ __launch_bounds__(kBlock) void k_batch_probe(const BatchInst* __restr
 // processfields_fd.cpp:72-107; Engine_Interface_FDTD::GetRawInterpolatedField
 // NODE_INTERPOLATE, engine_interface_fdtd.cpp:63-124), complex<float> sums.
 // ---------------------------------------------------------------------------
+// A field dump: frequency domain (td = 0; running DFT of the samples) or
+// time domain (td = 1; every sample stored).  Samples are taken after the
+// steps ts = first + k * fd_interval, k < nsamples.
 struct DumpDev {
     unsigned s[3], n[3];          // box start and extent (mesh indices)
     unsigned N[3];
     unsigned fd_interval, nfreq, nsamples;
+    unsigned first;               // step of the first sample (0 for frequency-domain dumps)
+    int td;
     int field;                    // 0: E from the voltages, 1: H from the currents
     int interp;                   // 0: none, 1: node, 2: cell interpolation
-    const float* factors;         // [nsamples][nfreq][2]
+    const float* factors;         // [nsamples][nfreq][2] (frequency domain)
     const double* el[3];          // edge lengths per axis: primary (E) or dual (H)
-    float* acc;                   // [nfreq][3][npts][2]
+    float* acc;                   // FD: [nfreq][3][npts][2]; TD: [nsamples][3][npts]
 };
+
+__host__ __device__ inline bool dump_sample(const DumpDev& d, unsigned ts, unsigned* k)
+{
+    if (ts < d.first || (ts - d.first) % d.fd_interval) return false;
+    *k = (ts - d.first) / d.fd_interval;
+    return *k < d.nsamples;
+}
 
 // GetRawField / GetRawDualField (engine_interface_fdtd.cpp:263-268,134-146):
 // the stored value over its edge length, zero for a zero length
@@ -1396,9 +1408,8 @@ __global__ void k_fd_dump(const float* __restrict__ F, unsigned long long cs, un
                           const unsigned* __restrict__ numts)
 {
     const unsigned ts = *numts;
-    if (ts % d.fd_interval) return;
-    const unsigned ks = ts / d.fd_interval;
-    if (ks >= d.nsamples) return;
+    unsigned ks;
+    if (!dump_sample(d, ts, &ks)) return;
     const unsigned npts = d.n[0] * d.n[1] * d.n[2];
     const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= npts) return;
@@ -1412,6 +1423,12 @@ __global__ void k_fd_dump(const float* __restrict__ F, unsigned long long cs, un
     float E[3];
 #pragma unroll
     for (int n = 0; n < 3; ++n) E[n] = (float)dump_value(d, F, cs, pos, base, stride, n);
+    if (d.td) {
+        // ProcessFieldsTD::Process (processfields_td.cpp:50-91): the sample itself
+#pragma unroll
+        for (int n = 0; n < 3; ++n) d.acc[((unsigned long long)ks * 3 + n) * npts + k] = E[n];
+        return;
+    }
     for (unsigned f = 0; f < d.nfreq; ++f) {
         const float er = d.factors[(ks * d.nfreq + f) * 2];
         const float ei = d.factors[(ks * d.nfreq + f) * 2 + 1];
@@ -1891,6 +1908,7 @@ struct ems_engine {
     // field dumps
     std::vector<DumpDev> dumps;
     std::vector<unsigned> dump_npts;
+    bool sampled0 = false;          // the samples due before the first step were taken
 
     double* energy_part = nullptr;
 
@@ -2727,37 +2745,51 @@ int ems_probe_results(ems_engine* h, int id, unsigned* count, unsigned* ts, doub
     return 0;
 }
 
-int ems_add_fd_dump(ems_engine* h, int field, int interp, const unsigned start[3], const unsigned stop[3],
-                    unsigned fd_interval, unsigned nfreq, unsigned nsamples, const float* factors,
-                    const double* const edge_len[3], int* id)
+}  // extern "C"
+
+// box, field, interpolation and edge lengths of a new dump
+static int dump_common(ems_engine* h, int field, int interp, const unsigned start[3], const unsigned stop[3],
+                       const double* const edge_len[3], DumpDev& d)
 {
-    if (!h || !start || !stop || !factors || !edge_len) return fail(EMS_E_ARG, "null argument");
+    if (!h || !start || !stop || !edge_len) return fail(EMS_E_ARG, "null argument");
     if (field != 0 && field != 1) return fail(EMS_E_ARG, "dump field must be 0 (E) or 1 (H)");
     if (interp < 0 || interp > 2) return fail(EMS_E_ARG, "dump interpolation must be 0 (none), 1 (node) or 2 (cell)");
-    if (fd_interval == 0 || nfreq == 0 || nsamples == 0) return fail(EMS_E_ARG, "empty frequency-domain dump");
-    drop_graphs(h);     // the dump changes the launch pattern of every chunk
-    DumpDev d{};
     for (int n = 0; n < 3; ++n) {
         if (start[n] > stop[n] || stop[n] >= h->N[n]) return fail(EMS_E_ARG, "dump box outside the mesh");
         d.s[n] = start[n];
         d.n[n] = stop[n] - start[n] + 1;
         d.N[n] = h->N[n];
     }
-    d.fd_interval = fd_interval;
-    d.nfreq = nfreq;
-    d.nsamples = nsamples;
     d.field = field;
     d.interp = interp;
     HIPCHK(hipSetDevice(h->device));
-    int rc;
-    float* df;
-    if ((rc = upload(h, &df, factors, (size_t)nsamples * nfreq * 2))) return rc;
-    d.factors = df;
     for (int n = 0; n < 3; ++n) {
         double* de;
+        int rc;
         if ((rc = upload(h, &de, edge_len[n], h->N[n]))) return rc;
         d.el[n] = de;
     }
+    return 0;
+}
+
+extern "C" {
+
+int ems_add_fd_dump(ems_engine* h, int field, int interp, const unsigned start[3], const unsigned stop[3],
+                    unsigned fd_interval, unsigned nfreq, unsigned nsamples, const float* factors,
+                    const double* const edge_len[3], int* id)
+{
+    if (!h || !factors) return fail(EMS_E_ARG, "null argument");
+    if (fd_interval == 0 || nfreq == 0 || nsamples == 0) return fail(EMS_E_ARG, "empty frequency-domain dump");
+    DumpDev d{};
+    int rc;
+    if ((rc = dump_common(h, field, interp, start, stop, edge_len, d))) return rc;
+    drop_graphs(h);     // the dump changes the launch pattern of every chunk
+    d.fd_interval = fd_interval;
+    d.nfreq = nfreq;
+    d.nsamples = nsamples;
+    float* df;
+    if ((rc = upload(h, &df, factors, (size_t)nsamples * nfreq * 2))) return rc;
+    d.factors = df;
     const size_t npts = (size_t)d.n[0] * d.n[1] * d.n[2];
     if ((rc = h->alloc(&d.acc, (size_t)nfreq * 3 * npts * 2))) return rc;
     HIPCHK(hipMemset(d.acc, 0, (size_t)nfreq * 3 * npts * 2 * sizeof(float)));
@@ -2770,10 +2802,57 @@ int ems_add_fd_dump(ems_engine* h, int field, int interp, const unsigned start[3
 int ems_fd_dump_results(ems_engine* h, int id, float* out)
 {
     if (!h || id < 0 || id >= (int)h->dumps.size() || !out) return fail(EMS_E_ARG, "bad dump id");
+    const DumpDev& d = h->dumps[id];
+    if (d.td) return fail(EMS_E_ARG, "dump id is a time-domain dump");
     HIPCHK(hipSetDevice(h->device));
     HIPCHK(hipStreamSynchronize(h->stream));
-    const DumpDev& d = h->dumps[id];
     HIPCHK(hipMemcpy(out, d.acc, (size_t)d.nfreq * 3 * h->dump_npts[id] * 2 * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int ems_add_td_dump(ems_engine* h, int field, int interp, const unsigned start[3], const unsigned stop[3],
+                    unsigned interval, unsigned start_ts, unsigned stop_ts, unsigned capacity,
+                    const double* const edge_len[3], int* id)
+{
+    if (!h) return fail(EMS_E_ARG, "null argument");
+    if (interval == 0 || capacity == 0) return fail(EMS_E_ARG, "empty time-domain dump");
+    if (stop_ts < start_ts) return fail(EMS_E_ARG, "dump window ends before it starts");
+    DumpDev d{};
+    int rc;
+    if ((rc = dump_common(h, field, interp, start, stop, edge_len, d))) return rc;
+    const unsigned long long first = ((unsigned long long)start_ts + interval - 1) / interval * interval;
+    if (first > stop_ts) return fail(EMS_E_ARG, "no sampling step inside the dump window");
+    const unsigned long long inwin = ((unsigned long long)stop_ts - first) / interval + 1;
+    drop_graphs(h);
+    d.td = 1;
+    d.fd_interval = interval;
+    d.first = (unsigned)first;
+    d.nsamples = (unsigned)std::min<unsigned long long>(capacity, inwin);
+    const size_t npts = (size_t)d.n[0] * d.n[1] * d.n[2];
+    if ((rc = h->alloc(&d.acc, (size_t)d.nsamples * 3 * npts))) return rc;
+    HIPCHK(hipMemset(d.acc, 0, (size_t)d.nsamples * 3 * npts * sizeof(float)));
+    h->dumps.push_back(d);
+    h->dump_npts.push_back((unsigned)npts);
+    if (id) *id = (int)h->dumps.size() - 1;
+    return 0;
+}
+
+int ems_td_dump_results(ems_engine* h, int id, unsigned* count, unsigned* ts, float* out)
+{
+    if (!h || id < 0 || id >= (int)h->dumps.size() || !count) return fail(EMS_E_ARG, "bad dump id");
+    const DumpDev& d = h->dumps[id];
+    if (!d.td) return fail(EMS_E_ARG, "dump id is a frequency-domain dump");
+    unsigned c = 0;
+    if (h->numTS >= d.first && (h->numTS > 0 || h->sampled0))
+        c = (unsigned)std::min<unsigned long long>(d.nsamples, (h->numTS - d.first) / d.fd_interval + 1);
+    *count = c;
+    if (ts)
+        for (unsigned k = 0; k < c; ++k) ts[k] = d.first + k * d.fd_interval;
+    if (out && c) {
+        HIPCHK(hipSetDevice(h->device));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        HIPCHK(hipMemcpy(out, d.acc, (size_t)c * 3 * h->dump_npts[id] * sizeof(float), hipMemcpyDeviceToHost));
+    }
     return 0;
 }
 
@@ -2803,8 +2882,9 @@ static bool probe_due(const ems_engine* h, unsigned ts)
 
 static bool dump_due(const ems_engine* h, unsigned ts)
 {
+    unsigned k;
     for (const auto& d : h->dumps)
-        if (ts % d.fd_interval == 0 && ts / d.fd_interval < d.nsamples) return true;
+        if (dump_sample(d, ts, &k)) return true;
     return false;
 }
 
@@ -2817,7 +2897,8 @@ static int launch_samples(ems_engine* h)
     }
     for (size_t k = 0; k < h->dumps.size(); ++k) {
         const DumpDev& d = h->dumps[k];
-        if (h->numTS % d.fd_interval || h->numTS / d.fd_interval >= d.nsamples) continue;
+        unsigned ks;
+        if (!dump_sample(d, h->numTS, &ks)) continue;
         const unsigned npts = h->dump_npts[k];
         hipLaunchKernelGGL(k_fd_dump, dim3((npts + kBlock - 1) / kBlock), dim3(kBlock), 0, h->stream,
                            d.field ? h->I : h->V, h->cs, h->N[0], h->g.plane, d, h->numts_dev);
@@ -3089,8 +3170,8 @@ static std::vector<unsigned char> chunk_key(const ems_engine* h, unsigned steps)
         if (!h->probes.empty() && probe_due(h, ts + 1)) b |= 2;
         key.push_back(b);
         for (size_t d = 0; d < h->dumps.size(); ++d) {
-            const DumpDev& D = h->dumps[d];
-            key.push_back((unsigned char)((ts + 1) % D.fd_interval == 0 && (ts + 1) / D.fd_interval < D.nsamples));
+            unsigned ks;
+            key.push_back((unsigned char)dump_sample(h->dumps[d], ts + 1, &ks));
         }
     }
     return key;
@@ -3185,6 +3266,7 @@ int ems_iterate(ems_engine* h, unsigned nsteps)
     if (h->numTS == 0 && nsteps > 0) {
         // Process() before the first step (openems.cpp:1424-1425)
         launch_samples(h);
+        h->sampled0 = true;
     }
     HIPCHK(hipEventRecord(h->ev0, h->stream));
     unsigned s = 0;

@@ -295,6 +295,33 @@ class EngineHIP:
         shape = (nf, 3) + tuple(int(stop[n]) - int(start[n]) + 1 for n in range(3))
         return did.value, shape
 
+    def add_td_dump(self, start, stop, interval, field=0, interp=1, start_ts=0, stop_ts=0xFFFFFFFF, capacity=1):
+        """Time-domain dump of the E (field 0) or H (field 1) field over the
+        mesh-index box start..stop after every step that is a multiple of
+        interval inside [start_ts, stop_ts]; keeps up to capacity samples."""
+        L = lib()
+        els = [np.ascontiguousarray(self.op.edge_lengths(n, field == 1), np.float64) for n in range(3)]
+        self._keep += els
+        elp = (dp * 3)(*[ptr(e, C.c_double) for e in els])
+        st = (C.c_uint * 3)(*start)
+        sp = (C.c_uint * 3)(*stop)
+        did = C.c_int()
+        check(L.ems_add_td_dump(self.h, int(field), int(interp), st, sp, int(interval), int(start_ts), int(stop_ts),
+                                int(capacity), elp, C.byref(did)))
+        return did.value, tuple(int(stop[n]) - int(start[n]) + 1 for n in range(3))
+
+    def td_dump_results(self, did, box):
+        """(steps [count], fields [count][3][nx][ny][nz] float32) sampled so far."""
+        L = lib()
+        cnt = C.c_uint()
+        check(L.ems_td_dump_results(self.h, did, C.byref(cnt), None, None))
+        n = cnt.value
+        ts = np.zeros(n, np.uint32)
+        out = np.zeros((n, 3) + tuple(box), np.float32)
+        if n:
+            check(L.ems_td_dump_results(self.h, did, C.byref(cnt), ptr(ts, C.c_uint), ptr(out, C.c_float)))
+        return ts, out
+
     def fd_dump_results(self, did, shape):
         out = np.zeros(shape + (2,), np.float32)
         check(lib().ems_fd_dump_results(self.h, did, ptr(out, C.c_float)))

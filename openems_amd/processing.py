@@ -344,3 +344,57 @@ class FieldDumpFD:
             return None
         raw = eng.fd_dump_results(self.did, self.shape)
         return raw[..., 0].astype(np.float32) + 1j * raw[..., 1].astype(np.float32)
+
+
+class FieldDumpTD:
+    """ProcessFieldsTD (Common/processfields_td.cpp:50-91) of the E field
+    (dump type 0, spec "field": "E") or the H field (type 1, "H") with no,
+    node or cell interpolation (spec "interp", default 1), set up as
+    FieldDumpFD: the field of the box at every step that
+    Processing::CheckTimestep accepts (processing.cpp:82-106) -- inside the
+    window spec "start_time" .. "stop_time" (seconds, SetProcessStartStopTime,
+    processing.cpp:211-226) and a multiple of ProcessInterval =
+    Nyquist / OverSampling (openems.cpp:631; 0 takes no samples).  The samples
+    stay on the device until collected."""
+
+    def __init__(self, op, spec, nrts, oversampling=4):
+        fd = FieldDumpFD(op, dict(spec, freqs=[]), nrts)
+        self.op = op
+        self.spec = spec
+        self.name = spec.get("name", "")
+        self.field, self.interp = fd.field, fd.interp
+        self.dual_time, self.dual_mesh = fd.dual_time, fd.dual_mesh
+        self.enabled = fd.enabled
+        self.start, self.stop = fd.start, fd.stop
+        self.interval = op.nyquist // int(oversampling)
+        dT = op.dT
+        self.start_ts, self.stop_ts = 0, 0xFFFFFFFF
+        t0, t1 = float(spec.get("start_time", 0.0)), float(spec.get("stop_time", 0.0))
+        if t0 > 0:
+            self.start_ts = int(math.floor(t0 / dT))
+        if t1 > 0:
+            self.stop_ts = int(math.ceil(t1 / dT))
+        if self.stop_ts <= self.start_ts:
+            self.start_ts, self.stop_ts = 0, 0xFFFFFFFF
+        self.steps = [ts for ts in range(nrts + 1) if self.due(ts)]
+        self.did = None
+
+    def due(self, ts):
+        return (self.enabled and self.interval > 0 and self.start_ts <= ts <= self.stop_ts
+                and ts % self.interval == 0)
+
+    def register(self, eng):
+        if not self.steps:
+            return
+        self.did, self.box = eng.add_td_dump(self.start, self.stop, self.interval, field=self.field,
+                                             interp=self.interp, start_ts=self.start_ts, stop_ts=self.stop_ts,
+                                             capacity=len(self.steps))
+
+    def collect(self, eng):
+        """(steps, times, fields [count][3][nx][ny][nz] float32); time at the
+        dual time for H, GetTime(dualTime) (engine_interface_fdtd.h:52)."""
+        if self.did is None:
+            return np.zeros(0, np.uint32), np.zeros(0), None
+        ts, out = eng.td_dump_results(self.did, self.box)
+        t = (ts.astype(np.float64) + (0.5 if self.dual_time else 0.0)) * self.op.dT
+        return ts, t, out

@@ -19,7 +19,7 @@ import numpy as np
 
 from .engine import EngineHIP
 from .operator import Operator
-from .processing import FieldDumpFD, Probe
+from .processing import FieldDumpFD, FieldDumpTD, Probe
 
 
 class Simulation:
@@ -39,7 +39,8 @@ class Simulation:
             self.eng.set_graph(graph)
         os_ = int(scene.get("oversampling", 4))
         self.probes = [Probe(self.op, p, self.nrts, os_) for p in scene.get("probes", [])]
-        self.dumps = [FieldDumpFD(self.op, d, self.nrts) for d in scene.get("dumps", [])]
+        self.dumps = [FieldDumpTD(self.op, d, self.nrts, os_) if d.get("domain") == "td"
+                      else FieldDumpFD(self.op, d, self.nrts) for d in scene.get("dumps", [])]
         for p in self.probes:
             p.register(self.eng)
         for d in self.dumps:
@@ -97,7 +98,11 @@ class Simulation:
             td, fd = p.collect(self.eng)
             out["probes"][p.name] = {"td": td, "fd": fd, "freqs": p.freqs}
         for d in self.dumps:
-            out["dumps"][d.name] = {"fd": d.collect(self.eng), "freqs": d.freqs}
+            if isinstance(d, FieldDumpTD):
+                ts, t, fields = d.collect(self.eng)
+                out["dumps"][d.name] = {"ts": ts, "time": t, "td": fields}
+            else:
+                out["dumps"][d.name] = {"fd": d.collect(self.eng), "freqs": d.freqs}
         return out
 
     def write_probes(self, directory):

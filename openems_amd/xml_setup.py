@@ -15,7 +15,8 @@ codes or PEC / PMC / MUR / PML_n, Mur phase velocities) and ``<Excitation>``
 The ``<ContinuousStructure>`` block is CSXCAD's format, read here for the
 subset the engine takes without CSXCAD: ``RectilinearGrid`` (DeltaUnit,
 X/Y/ZLines), ``BackgroundMaterial``, and the properties Material, Metal,
-Excitation, ProbeBox, DumpBox, LumpedElement, LorentzMaterial and
+Excitation, ProbeBox, DumpBox (time- and frequency-domain E / H dumps),
+LumpedElement, LorentzMaterial and
 DebyeMaterial with Box primitives (P1, P2, Priority) and, for Metal and
 Excitation, Curve primitives (Vertex points).  Properties are entered in file
 order, primitives in property order, as CSXCAD hands them to openEMS
@@ -258,13 +259,19 @@ def read_xml(source):
         elif prop.tag == "DumpBox":
             dtype = _int(prop, "DumpType", 0)
             mode = _int(prop, "DumpMode", 0)
-            if dtype not in (10, 11):
+            if dtype not in (0, 1, 10, 11):
                 skipped.append((name, f"dump type {dtype}"))
                 continue
             for nb, (prio, (p1, p2)) in enumerate(boxes):
-                dumps.append({"name": name if len(boxes) == 1 else f"{name}_{nb}", "start": list(p1),
-                              "stop": list(p2), "freqs": _fd_samples(prop), "field": "E" if dtype == 10 else "H",
-                              "interp": mode, "file_type": _int(prop, "FileType", 0)})
+                d = {"name": name if len(boxes) == 1 else f"{name}_{nb}", "start": list(p1), "stop": list(p2),
+                     "field": "E" if dtype in (0, 10) else "H", "interp": mode,
+                     "file_type": _int(prop, "FileType", 0)}
+                if dtype < 10:
+                    d.update(domain="td", start_time=float(prop.get("StartTime", 0.0)),
+                             stop_time=float(prop.get("StopTime", 0.0)))
+                else:
+                    d["freqs"] = _fd_samples(prop)
+                dumps.append(d)
         else:
             raise ValueError(f"property {prop.tag} {name!r} is not supported")
     scene.update(materials=mats, metals=metals, lumped=lumped, excitations=excs, probes=probes, dumps=dumps)
@@ -381,9 +388,13 @@ def write_xml(scene, path=None):
         if p.get("freqs"):
             ET.SubElement(el, "FD_Samples").text = ",".join(repr(float(f)) for f in p["freqs"])
     for k, d in enumerate(scene.get("dumps", [])):
-        fld = {"E": 10, "H": 11, 0: 10, 1: 11}[d.get("field", "E")]
+        fld = {"E": 0, "H": 1, 0: 0, 1: 1}[d.get("field", "E")] + (0 if d.get("domain") == "td" else 10)
         el = prop("DumpBox", d.get("name") or f"dump{k}", DumpType=str(fld), DumpMode=str(int(d.get("interp", 1))),
                   FileType=str(int(d.get("file_type", 0))))
+        if d.get("domain") == "td":
+            for key, attr in (("start_time", "StartTime"), ("stop_time", "StopTime")):
+                if d.get(key):
+                    el.set(attr, repr(float(d[key])))
         _box(ET.SubElement(el, "Primitives"), 0, d["start"], d["stop"])
         if d.get("freqs"):
             ET.SubElement(el, "FD_Samples").text = ",".join(repr(float(f)) for f in d["freqs"])

@@ -569,8 +569,11 @@ class RefFieldDumpFD:
                         out[n, i, j, k] = np.float32(v)
         return out
 
+    def due(self, ts):
+        return bool(self.fd_interval) and ts % self.fd_interval == 0
+
     def process(self, ts, volt, curr):
-        if not self.fd_interval or ts % self.fd_interval != 0:
+        if not self.due(ts):
             return
         F = self.field(volt, curr)
         T = (ts + (0.5 if self.dual_time else 0.0)) * self.op.dT
@@ -583,6 +586,38 @@ class RefFieldDumpFD:
             er, ei = np.float32(e.real), np.float32(e.imag)
             self.fd_re[k] = self.fd_re[k] + F * er
             self.fd_im[k] = self.fd_im[k] + F * ei
+
+
+class RefFieldDumpTD(RefFieldDumpFD):
+    """ProcessFieldsTD::Process (processfields_td.cpp:50-91): the interpolated
+    field of the box (as RefFieldDumpFD.field) kept at every step that
+    Processing::CheckTimestep accepts (processing.cpp:82-106): inside the
+    start/stop window and a multiple of ProcessInterval = Nyquist / OverSampling
+    (openems.cpp:631, integer division; 0 takes no samples).  The window comes
+    from SetProcessStartStopTime (processing.cpp:211-226)."""
+
+    def __init__(self, op: RefOperator, dump_type, start, stop, interp=1, oversampling=4,
+                 start_time=0.0, stop_time=0.0):
+        super().__init__(op, dump_type, start, stop, [], interp=interp)
+        self.interval = op.nyquist // int(oversampling)
+        self.start_ts, self.stop_ts = 0, 0xFFFFFFFF
+        if start_time > 0:
+            self.start_ts = int(math.floor(start_time / op.dT))
+        if stop_time > 0:
+            self.stop_ts = int(math.ceil(stop_time / op.dT))
+        if self.stop_ts <= self.start_ts:
+            self.start_ts, self.stop_ts = 0, 0xFFFFFFFF
+        self.samples = []       # (ts, time, float32 [3][nx][ny][nz])
+
+    def due(self, ts):
+        return (self.enabled and self.interval > 0 and self.start_ts <= ts <= self.stop_ts
+                and ts % self.interval == 0)
+
+    def process(self, ts, volt, curr):
+        if not self.due(ts):
+            return
+        T = (ts + (0.5 if self.dual_time else 0.0)) * self.op.dT
+        self.samples.append((ts, T, self.field(volt, curr)))
 
 
 def run_reference(op: RefOperator, nrts, probes=(), dumps=(), kind="basic", nthreads=1, eng=None):
@@ -598,7 +633,7 @@ def run_reference(op: RefOperator, nrts, probes=(), dumps=(), kind="basic", nthr
         return eng.fields()
 
     def needs(ts):
-        return any(p.check(ts) for p in probes) or any(d.fd_interval and ts % d.fd_interval == 0 for d in dumps)
+        return any(p.check(ts) for p in probes) or any(d.due(ts) for d in dumps)
 
     ts = eng.numTS
     if needs(ts):

@@ -536,3 +536,44 @@ def test_fd_dumps_e_and_h_every_interpolation(gpu):
         assert np.abs(want).max() > 0, d["name"]
         assert rel_l2(got, want) <= 1e-5, (d["name"], rel_l2(got, want))
     sim.close()
+
+
+def test_td_dumps_e_and_h_every_interpolation(gpu):
+    """ProcessFieldsTD of E (dump type 0) and H (type 1) with no, node and
+    cell interpolation, one of them inside a start/stop window: every sample
+    bit-identical to the oracle's, at the same steps."""
+    from oracle.oracle import RefFieldDumpTD
+    scene = scenes.wire_antenna(36, 240)
+    L = scene["lines"]
+    c = 18
+    plane = ((L[0][4] + 0.1, L[1][3] - 0.05, L[2][c] + 0.13), (L[0][32] - 0.2, L[1][33] + 0.1, L[2][c] + 0.13))
+    vol = ((L[0][c - 3] + 0.2, L[1][c - 3] - 0.1, L[2][c - 4] + 0.1), (L[0][c + 4] - 0.1, L[1][c + 3], L[2][c + 5] - 0.2))
+    dumps = []
+    for field in ("E", "H"):
+        for interp in (0, 1, 2):
+            for k, (a, b) in enumerate((plane, vol)):
+                d = {"name": f"td{field}{interp}_{k}", "start": a, "stop": b, "field": field, "interp": interp,
+                     "domain": "td"}
+                if field == "H" and interp == 1 and k == 0:
+                    d["start_time"], d["stop_time"] = 40e-12, 150e-12
+                dumps.append(d)
+    scene["dumps"] = dumps
+    print("case wire_antenna_36 240 steps, 12 TD dumps", flush=True)
+    sim = Simulation(scene)
+    sim.run()
+    res = sim.results()
+    ref = _ref_from_scene(scene)
+    rds = [RefFieldDumpTD(ref, 0 if d["field"] == "E" else 1, d["start"], d["stop"], interp=d["interp"],
+                          oversampling=scene["oversampling"], start_time=d.get("start_time", 0.0),
+                          stop_time=d.get("stop_time", 0.0)) for d in dumps]
+    run_reference(ref, 240, dumps=rds)
+    for d, rd in zip(dumps, rds):
+        got = res["dumps"][d["name"]]
+        assert list(got["ts"]) == [s[0] for s in rd.samples], d["name"]
+        assert len(rd.samples) > 3
+        np.testing.assert_array_equal(got["time"], [s[1] for s in rd.samples])
+        want = np.stack([s[2] for s in rd.samples])
+        assert got["td"].shape == want.shape, d["name"]
+        assert np.abs(want).max() > 0, d["name"]
+        assert np.array_equal(got["td"], want), (d["name"], np.count_nonzero(got["td"] != want))
+    sim.close()

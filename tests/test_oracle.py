@@ -619,3 +619,26 @@ def test_field_dump_box_snaps_outward_and_h_takes_the_dual_mesh():
     # below the first primary line at or above it: x 2.3 -> 2 (2.5), widened
     # to 1 as 2.5 > 2.3; x 6.6 -> 6 (6.5), widened to 7; z 4.0 -> 3 both ends
     assert [int(p[0]) for p in h.pos] == [1, 2, 3] and [int(p[-1]) for p in h.pos] == [7, 8, 3]
+
+
+def test_td_dump_schedule():
+    """ProcessFieldsTD sampling: multiples of Nyquist / OverSampling (integer
+    division) inside the start/stop window; floor / ceil of the window times;
+    an inverted window falls back to the whole run; an H dump's time is the
+    dual time."""
+    from oracle.oracle import RefFieldDumpTD
+    l = np.arange(10.0)
+    op = _dump_op([l, l.copy(), l.copy()])
+    iv = op.nyquist // 4
+    assert iv > 1
+    d = RefFieldDumpTD(op, 0, [1, 1, 1], [3, 3, 3], oversampling=4, start_time=2.5 * iv * op.dT,
+                       stop_time=6.2 * iv * op.dT)
+    due = [t for t in range(0, 10 * iv) if d.due(t)]
+    assert due == [3 * iv, 4 * iv, 5 * iv, 6 * iv]
+    inv = RefFieldDumpTD(op, 1, [1, 1, 1], [3, 3, 3], start_time=5e-9, stop_time=1e-9)
+    assert inv.start_ts == 0 and inv.stop_ts == 0xFFFFFFFF and inv.due(0)
+    none = RefFieldDumpTD(op, 0, [1, 1, 1], [3, 3, 3], oversampling=op.nyquist + 1)
+    assert none.interval == 0 and not any(none.due(t) for t in range(50))
+    v = np.zeros((3,) + op.N, np.float32)
+    inv.process(2 * inv.interval, v, v)
+    assert inv.samples[0][1] == (2 * inv.interval + 0.5) * op.dT
