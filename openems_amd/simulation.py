@@ -115,6 +115,29 @@ class Simulation:
             paths += p.write_files(td, fd, directory)
         return paths
 
+    def write_dumps(self, directory):
+        """Field dumps with FileType 1 as <name>.h5 in the reference's HDF5
+        layout (openems_amd.hdf5_out); returns the paths written.  FileType 0
+        (VTK) files are not written."""
+        import os
+        from . import hdf5_out
+        paths = []
+        for d in self.dumps:
+            if int(d.spec.get("file_type", 1)) != 1 or not getattr(d, "enabled", True):
+                continue
+            path = os.path.join(directory, d.name + ".h5")
+            if isinstance(d, FieldDumpTD):
+                ts, t, fields = d.collect(self.eng)
+                if fields is None:
+                    continue
+                paths.append(hdf5_out.write_td_dump(path, self.op, d, ts, t, fields))
+            else:
+                fd = d.collect(self.eng)
+                if fd is None:
+                    continue
+                paths.append(hdf5_out.write_fd_dump(path, self.op, d, fd, d.freqs))
+        return paths
+
     def close(self):
         self.eng.close()
         self.op.close()

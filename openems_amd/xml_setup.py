@@ -412,7 +412,8 @@ def write_xml(scene, path=None):
 def run_xml(path, out_dir=None, **sim_kwargs):
     """``openEMS <path>`` on the HIP engine: read the setup, build the
     operator, run up to NumberOfTimesteps (or the end criterion), write the
-    probe files into out_dir (default: the setup's folder) and return
+    probe files and the HDF5 field dumps (FileType 1) into out_dir (default:
+    the setup's folder) and return
     (Simulation results, list of files written)."""
     from .simulation import Simulation
     scene = read_xml(path)
@@ -421,7 +422,7 @@ def run_xml(path, out_dir=None, **sim_kwargs):
     try:
         sim.run()
         res = sim.results()
-        files = sim.write_probes(out_dir)
+        files = sim.write_probes(out_dir) + sim.write_dumps(out_dir)
     finally:
         sim.close()
     return res, files

@@ -68,3 +68,48 @@ def test_field_probe_and_current_probe_headers(native, tmp_path):
     # a probe outside the mesh is disabled and writes nothing
     off = _probe(op, {"type": 2, "name": "off", "start": (-50, -50, -50), "stop": (-40, -40, -40), "freqs": []})
     assert not off.enabled and off.write_files(np.zeros((0, 4)), None, str(tmp_path)) == []
+
+
+def test_hdf5_dump_files_in_the_reference_layout(native, tmp_path):
+    """Field dumps as HDF5_File_Writer lays them out: root version attribute,
+    /Mesh/x,y,z of the dump box (dual lines for an uninterpolated H dump) in
+    metres as float32, TD datasets named by the 8-digit step with a time
+    attribute, FD f<n>_real / f<n>_imag with frequency attributes, fields
+    stored [3][nz][ny][nx]."""
+    from openems_amd import hdf5_out
+    from openems_amd.processing import FieldDumpFD, FieldDumpTD
+    scene = scenes.wire_antenna(30, 200)
+    op = Operator(scene, nthreads=2)
+    L = scene["lines"]
+    spec = {"name": "Ht", "start": (L[0][5], L[1][6], L[2][7]), "stop": (L[0][9], L[1][12], L[2][7]),
+            "field": "H", "interp": 0, "domain": "td", "file_type": 1}
+    d = FieldDumpTD(op, spec, 200)
+    nx, ny, nz = (d.stop[n] - d.start[n] + 1 for n in range(3))
+    rng = np.random.default_rng(4)
+    ts = np.array([0, 7, 14], np.uint32)
+    t = (ts + 0.5) * op.dT
+    fields = rng.normal(size=(3, 3, nx, ny, nz)).astype(np.float32)
+    path = str(tmp_path / "Ht.h5")
+    hdf5_out.write_td_dump(path, op, d, ts, t, fields)
+    assert hdf5_out.read_attribute(path, "/", "openEMS_HDF5_version")[0] == 0.2
+    x, _ = hdf5_out.read_dataset(path, "/Mesh/x")
+    want = np.array([np.float32(op.disc_line(0, i, True)) * np.float32(1e-3) for i in range(d.start[0], d.stop[0] + 1)],
+                    np.float32)
+    assert d.dual_mesh and np.array_equal(x, want)
+    for k in range(3):
+        data, attrs = hdf5_out.read_dataset(path, "/FieldData/TD/%08d" % ts[k])
+        assert data.shape == (3, nz, ny, nx)
+        assert np.array_equal(data, np.transpose(fields[k], (0, 3, 2, 1)))
+        assert attrs["time"][0] == np.float32(t[k])
+    fspec = {"name": "Ef", "start": spec["start"], "stop": spec["stop"], "freqs": [5e9, 9e9], "file_type": 1}
+    fdump = FieldDumpFD(op, fspec, 200)
+    fd = (rng.normal(size=(2, 3, nx, ny, nz)) + 1j * rng.normal(size=(2, 3, nx, ny, nz))).astype(np.complex64)
+    path = str(tmp_path / "Ef.h5")
+    hdf5_out.write_fd_dump(path, op, fdump, fd, fdump.freqs)
+    assert np.array_equal(hdf5_out.read_attribute(path, "/FieldData/FD", "frequency", group=True), [5e9, 9e9])
+    for n in range(2):
+        re, attrs = hdf5_out.read_dataset(path, f"/FieldData/FD/f{n}_real")
+        im, _ = hdf5_out.read_dataset(path, f"/FieldData/FD/f{n}_imag")
+        assert attrs["frequency"][0] == np.float32(fdump.freqs[n])
+        assert np.array_equal(re + 1j * im, np.transpose(fd[n], (0, 3, 2, 1)))
+    assert hdf5_out.exists(path, "/Mesh/z") and not hdf5_out.exists(path, "/FieldData/TD")

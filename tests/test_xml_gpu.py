@@ -13,14 +13,26 @@ pytestmark = pytest.mark.gpu
 
 
 def test_xml_run_matches_the_scene_and_writes_probe_files(gpu, tmp_path):
+    from openems_amd import hdf5_out
     scene = scenes.wire_antenna(36, 400)
     scene["probes"][0]["freqs"] = [8e9, 12e9]
     scene["end_criteria"] = 1e-9            # far below what 400 steps reach
+    L = scene["lines"]
+    box = ((L[0][10], L[1][10], L[2][18]), (L[0][26], L[1][26], L[2][18]))
+    scene["dumps"] = [{"name": "Et", "start": box[0], "stop": box[1], "field": "E", "interp": 1, "domain": "td",
+                       "file_type": 1},
+                      {"name": "Hf", "start": box[0], "stop": box[1], "field": "H", "interp": 2, "freqs": [9e9],
+                       "file_type": 1}]
     path = tmp_path / "wire.xml"
     write_xml(scene, str(path))
     print("case wire_antenna_36 from XML, 400 steps", flush=True)
     res, files = run_xml(str(path), out_dir=str(tmp_path))
-    assert sorted(f.split("/")[-1] for f in files) == ["et", "feed_ut", "feed_ut_FD"]
+    assert sorted(f.split("/")[-1] for f in files) == ["Et.h5", "Hf.h5", "et", "feed_ut", "feed_ut_FD"]
+    td = res["dumps"]["Et"]
+    last, attrs = hdf5_out.read_dataset(str(tmp_path / "Et.h5"), "/FieldData/TD/%08d" % td["ts"][-1])
+    assert np.array_equal(last, np.transpose(td["td"][-1], (0, 3, 2, 1))) and np.abs(last).max() > 0
+    re, _ = hdf5_out.read_dataset(str(tmp_path / "Hf.h5"), "/FieldData/FD/f0_real")
+    assert np.array_equal(re, np.transpose(res["dumps"]["Hf"]["fd"][0].real, (0, 3, 2, 1)))
     sim = Simulation(read_xml(str(path)))
     sim.run()
     want = sim.results()
