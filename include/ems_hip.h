@@ -320,6 +320,10 @@ int ems_batch_get_grouping(const ems_batch* b, unsigned long long* group_bytes, 
  * advanced state, and keep the fastest (best_ms: its time per step). */
 int ems_batch_autotune(ems_batch* b, unsigned steps, double* best_ms);
 int ems_batch_iterate(ems_batch* b, unsigned nsteps);
+/* Each group's launches of `steps` steps are captured once as a HIP graph
+ * and replayed (0: direct launches, the default).  Results are the same. */
+int ems_batch_set_graph(ems_batch* b, unsigned steps);
+int ems_batch_get_graph(const ems_batch* b, unsigned* steps);
 /* device time of the last ems_batch_iterate call */
 int ems_batch_last_ms(const ems_batch* b, double* ms);
 void ems_batch_destroy(ems_batch* b);

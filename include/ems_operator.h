@@ -40,6 +40,7 @@ typedef struct emsop emsop;
 #define EMSOP_SIG_SINUS 1
 #define EMSOP_SIG_DIRAC 2
 #define EMSOP_SIG_STEP 3
+#define EMSOP_SIG_CUSTOM 10
 
 const char* emsop_last_error(void);
 
@@ -102,6 +103,14 @@ int emsop_add_excitation_curve(emsop* h, int type, const double val[3], double d
 /* bc: 0 PEC, 1 PMC, 2 Mur, 3 PML, -1 none (openEMS::m_BC_type) */
 int emsop_set_boundary(emsop* h, const int bc[6], const unsigned pml_size[6], const double mur_vph[6]);
 int emsop_set_signal(emsop* h, int type, double f0, double fc);
+/* custom excitation (Excitation::SetupCustomExcite / CalcCustomExcitation,
+ * FDTD/excitation.cpp:86-93,219-252): NrTS samples of the expression f(t) at
+ * t = n dT (voltages) and (n + 1/2) dT (currents), Nyquist number from f0.
+ * Syntax of the reference's function parser: numbers, t, pi, e, + - * / ^,
+ * comparisons, & | !, sin cos tan asin acos atan sinh cosh tanh exp log log10
+ * sqrt abs floor ceil int, atan2 pow min max, if(c, a, b).  A parse error
+ * fails with EMSOP_E_ARG and its position. */
+int emsop_set_custom_signal(emsop* h, const char* function, double f0);
 /* method: 1 or 3 (Rennings); forced_dt <= 0 means automatic; factor in (0,1] */
 int emsop_set_timestep(emsop* h, int method, double forced_dt, double factor);
 int emsop_set_cell_constant_material(emsop* h, int enable);

@@ -113,6 +113,13 @@ public:
     }
     void SetupGaussianPulse(double f0, double fc) { check_operator(emsop_set_signal(h_, EMSOP_SIG_GAUSS, f0, fc)); }
     void SetupSinusoidal(double f0) { check_operator(emsop_set_signal(h_, EMSOP_SIG_SINUS, f0, 0)); }
+    // Excitation::SetupDiracPulse / SetupStepExcite / SetupCustomExcite (FDTD/excitation.cpp:72-93)
+    void SetupDiracPulse(double f_max) { check_operator(emsop_set_signal(h_, EMSOP_SIG_DIRAC, f_max, 0)); }
+    void SetupStepExcite(double f_max) { check_operator(emsop_set_signal(h_, EMSOP_SIG_STEP, f_max, 0)); }
+    void SetupCustomExcite(const std::string& function, double f0)
+    {
+        check_operator(emsop_set_custom_signal(h_, function.c_str(), f0));
+    }
     void SetTimeStepMethod(int method) { ts_method_ = method; }
     void SetTimestepFactor(double factor) { ts_factor_ = factor; }
 

@@ -40,6 +40,7 @@ OPERATOR_API = [
     ("emsop_add_excitation", C.c_int, [vp, C.c_int, dp, C.c_double, C.c_int, dp, dp]),
     ("emsop_set_boundary", C.c_int, [vp, ip, up, dp]),
     ("emsop_set_signal", C.c_int, [vp, C.c_int, C.c_double, C.c_double]),
+    ("emsop_set_custom_signal", C.c_int, [vp, C.c_char_p, C.c_double]),
     ("emsop_set_timestep", C.c_int, [vp, C.c_int, C.c_double, C.c_double]),
     ("emsop_set_cell_constant_material", C.c_int, [vp, C.c_int]),
     ("emsop_set_threads", C.c_int, [vp, C.c_int]),
@@ -125,6 +126,8 @@ ENGINE_API = [
     ("ems_batch_set_grouping", C.c_int, [vp, C.c_ulonglong]),
     ("ems_batch_get_grouping", C.c_int, [vp, C.POINTER(C.c_ulonglong), up]),
     ("ems_batch_iterate", C.c_int, [vp, C.c_uint]),
+    ("ems_batch_set_graph", C.c_int, [vp, C.c_uint]),
+    ("ems_batch_get_graph", C.c_int, [vp, up]),
     ("ems_batch_last_ms", C.c_int, [vp, dp]),
     ("ems_batch_destroy", None, [vp]),
     ("ems_adj_create", C.c_int, [C.POINTER(vp), vp, C.c_uint, up, up, dp, C.c_double, C.c_double, up, up, C.c_uint]),

@@ -77,6 +77,16 @@ class BatchEngine:
         return {"tile_x": tx.value, "zdepth": zd.tolist(), "grid": grid.value, "work_items": nitems.value,
                 "group_bytes": gb.value, "groups": ng.value}
 
+    def set_graph(self, steps):
+        """Replay each group's launches of `steps` steps as a captured HIP
+        graph (0: direct launches)."""
+        check(lib().ems_batch_set_graph(self.b, int(steps)))
+
+    def graph(self):
+        v = C.c_uint()
+        check(lib().ems_batch_get_graph(self.b, C.byref(v)))
+        return v.value
+
     def iterate(self, nsteps):
         check(lib().ems_batch_iterate(self.b, int(nsteps)))
 

@@ -1321,9 +1321,10 @@ __global__ __launch_bounds__(kBlock) void k_batch_probe(const BatchInst* __restr
 }
 
 // ---------------------------------------------------------------------------
-// frequency-domain E dump with node interpolation (ProcessFieldsFD::Process,
-// processfields_fd.cpp:72-107; Engine_Interface_FDTD::GetRawInterpolatedField
-// NODE_INTERPOLATE, engine_interface_fdtd.cpp:63-124), complex<float> sums.
+// field dumps: ProcessFieldsFD::Process (processfields_fd.cpp:72-107, running
+// complex<float> sums) and ProcessFieldsTD::Process (processfields_td.cpp:50-91,
+// stored samples) of the interpolated E or H field
+// (Engine_Interface_FDTD, engine_interface_fdtd.cpp:63-204)
 // ---------------------------------------------------------------------------
 // A field dump: frequency domain (td = 0; running DFT of the samples) or
 // time domain (td = 1; every sample stored).  Samples are taken after the
@@ -3451,6 +3452,12 @@ struct ems_batch {
     };
     std::vector<Group> groups;
     std::vector<void*> gallocs;                         // group tables of the current configuration
+    // each group's launches of graph_steps steps, captured once as a HIP graph
+    // and replayed (0: direct launches, the default: replays measured no
+    // faster on the batch, whose launches are long enough to hide the
+    // launch overhead); dropped when the configuration changes
+    unsigned graph_steps = 0;
+    std::vector<hipGraphExec_t> gexec;
     unsigned tx = 0, grid_req = 0;
     size_t group_bytes = 0;
     std::vector<unsigned> zdepth;
@@ -3509,10 +3516,18 @@ void batch_launch(ems_batch* b, const ems_batch::Group& g)
 // Configuration: tile shape, z depth per instance, grid size, and the
 // working-set budget of a group (0: all instances in one group).  Groups are
 // packed first-fit by decreasing working set.
+void batch_drop_graphs(ems_batch* b)
+{
+    for (auto& x : b->gexec)
+        if (x) hipGraphExecDestroy(x);
+    b->gexec.clear();
+}
+
 int batch_build(ems_batch* b, unsigned tx, const std::vector<unsigned>& zd, unsigned grid, size_t group_bytes)
 {
     const size_t n = b->eng.size();
     HIPCHK(hipStreamSynchronize(b->stream));
+    batch_drop_graphs(b);
     for (void* p : b->gallocs) hipFree(p);
     b->gallocs.clear();
     b->groups.clear();
@@ -3607,10 +3622,33 @@ void batch_step(ems_batch* b, const ems_batch::Group& g)
 }
 
 // every group advances nsteps, group after group
-void batch_run(ems_batch* b, unsigned nsteps)
+// All kernel arguments of a step are the same at every step (the timestep
+// is read on the device), so a chunk of steps replays as a graph.
+int batch_run(ems_batch* b, unsigned nsteps)
 {
-    for (const auto& g : b->groups)
-        for (unsigned s = 0; s < nsteps; ++s) batch_step(b, g);
+    const unsigned S = b->graph_steps;
+    if (S && b->gexec.size() != b->groups.size()) b->gexec.assign(b->groups.size(), nullptr);
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+        const auto& g = b->groups[gi];
+        unsigned s = 0;
+        if (S && nsteps >= S) {
+            if (!b->gexec[gi]) {
+                hipGraph_t graph;
+                HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+                for (unsigned k = 0; k < S; ++k) batch_step(b, g);
+                HIPCHK(hipStreamEndCapture(b->stream, &graph));
+                hipError_t e = hipGraphInstantiate(&b->gexec[gi], graph, nullptr, nullptr, 0);
+                hipGraphDestroy(graph);
+                if (e != hipSuccess) {
+                    b->gexec[gi] = nullptr;
+                    return fail(EMS_E_HIP, std::string("graph instantiation failed: ") + hipGetErrorString(e));
+                }
+            }
+            for (; s + S <= nsteps; s += S) HIPCHK(hipGraphLaunch(b->gexec[gi], b->stream));
+        }
+        for (; s < nsteps; ++s) batch_step(b, g);
+    }
+    return 0;
 }
 
 }  // namespace
@@ -3780,7 +3818,8 @@ int ems_batch_iterate(ems_batch* b, unsigned nsteps)
             if (g.nprobes)
                 hipLaunchKernelGGL(k_batch_probe, dim3(g.nprobes), dim3(kBlock), 0, b->stream, b->d_inst, g.d_probes);
     HIPCHK(hipEventRecord(b->ev0, b->stream));
-    batch_run(b, nsteps);
+    int rc = batch_run(b, nsteps);
+    if (rc) return rc;
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(b->ev1, b->stream));
     HIPCHK(hipEventSynchronize(b->ev1));
@@ -3796,8 +3835,9 @@ int ems_batch_iterate(ems_batch* b, unsigned nsteps)
 // the state the candidates advanced (fields, UPML fluxes, counters, probe
 // sample counts), and keep the fastest.  Depth policies: each instance's own
 // (choose_zdepth) or 8 for all; grids: one workgroup per work item or the
-// resident workgroups; group budgets: one group, 64, 128 or 192 MiB (the
-// last-level cache holds 256 MiB).
+// resident workgroups; group budgets: one group, 64, 128, 160, 192 or 224 MiB
+// (the last-level cache holds 256 MiB; the budget also sets how many work
+// items a group's launches spread over the resident workgroups).
 int ems_batch_autotune(ems_batch* b, unsigned steps, double* best_ms)
 {
     if (!b) return fail(EMS_E_ARG, "null batch");
@@ -3870,7 +3910,7 @@ int ems_batch_autotune(ems_batch* b, unsigned steps, double* best_ms)
     const unsigned depths[2] = {0, 8};   // 0: each instance's own choice
     const unsigned grids[2] = {0, (unsigned)(b->resident * b->cus)};
     const size_t mib = size_t(1) << 20;
-    const size_t budgets[4] = {0, 64 * mib, 128 * mib, 192 * mib};
+    const size_t budgets[6] = {0, 64 * mib, 128 * mib, 160 * mib, 192 * mib, 224 * mib};
     double best = 1e30;
     unsigned btx = 0, bdep = 0, bgrid = 0;
     size_t bbud = 0;
@@ -3923,11 +3963,30 @@ int ems_batch_last_ms(const ems_batch* b, double* ms)
     return 0;
 }
 
+int ems_batch_set_graph(ems_batch* b, unsigned steps)
+{
+    if (!b) return fail(EMS_E_ARG, "null batch");
+    if (steps > 4096) return fail(EMS_E_ARG, "graph chunk longer than 4096 steps");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    batch_drop_graphs(b);
+    b->graph_steps = steps;
+    return 0;
+}
+
+int ems_batch_get_graph(const ems_batch* b, unsigned* steps)
+{
+    if (!b || !steps) return fail(EMS_E_ARG, "null argument");
+    *steps = b->graph_steps;
+    return 0;
+}
+
 void ems_batch_destroy(ems_batch* b)
 {
     if (!b) return;
     hipSetDevice(b->device);
     if (b->stream) hipStreamSynchronize(b->stream);
+    batch_drop_graphs(b);
     for (void* p : b->gallocs) hipFree(p);
     for (void* p : b->allocs) hipFree(p);
     if (b->ev0) hipEventDestroy(b->ev0);

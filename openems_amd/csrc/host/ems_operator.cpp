@@ -34,6 +34,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cctype>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -267,6 +268,7 @@ struct emsop {
     double vph[6] = {0, 0, 0, 0, 0, 0};
     int sigType = -1;
     double f0 = 0, fc = 0;
+    std::string sigFunc;     // custom signal: f(t) in the function-parser syntax
     int tsMethod = 3;
     double forcedDt = 0, tsFactor = 1;
     bool cellConst = false;
@@ -1354,6 +1356,248 @@ void emsop::build_upml()
 // ---------------------------------------------------------------------------
 // excitation signal (FDTD/excitation.cpp:150-276)
 // ---------------------------------------------------------------------------
+// Expression of t for custom excitation signals, in the syntax of the
+// function parser the reference uses (fparser; Excitation::CalcCustomExcitation,
+// FDTD/excitation.cpp:219-252): numbers, t, the constants pi and e, + - * / ^
+// (power, right-associative, binding tighter than a leading minus),
+// comparisons (< <= > >= = !=, giving 1 or 0), & and | (logical), !, the
+// functions sin cos tan asin acos atan sinh cosh tanh exp log log10 sqrt
+// abs floor ceil int (round to nearest), atan2 pow min max (two arguments)
+// and if(c, a, b).  Parsed once into a tree and evaluated in double, left to
+// right as written.
+namespace {
+struct Expr {
+    enum Op { NUM, VAR, NEG, NOT, ADD, SUB, MUL, DIV, POW, LT, LE, GT, GE, EQ, NE, AND, OR, FN1, FN2, IF };
+    Op op;
+    double v = 0;
+    int fn = 0;
+    std::vector<std::unique_ptr<Expr>> a;
+};
+
+struct ExprParser {
+    const std::string& s;
+    size_t i = 0;
+    std::string err;
+    explicit ExprParser(const std::string& str) : s(str) {}
+    void ws() { while (i < s.size() && std::isspace((unsigned char)s[i])) ++i; }
+    bool eat(const char* tok)
+    {
+        ws();
+        size_t n = std::strlen(tok);
+        if (s.compare(i, n, tok) == 0) {
+            i += n;
+            return true;
+        }
+        return false;
+    }
+    std::unique_ptr<Expr> mk(Expr::Op op, std::unique_ptr<Expr> x, std::unique_ptr<Expr> y = nullptr)
+    {
+        auto e = std::make_unique<Expr>();
+        e->op = op;
+        e->a.push_back(std::move(x));
+        if (y) e->a.push_back(std::move(y));
+        return e;
+    }
+    std::unique_ptr<Expr> fail_(const std::string& m)
+    {
+        if (err.empty()) err = m + " at position " + std::to_string(i);
+        return nullptr;
+    }
+    // or := and ('|' and)* ; and := cmp ('&' cmp)* ; cmp := sum (relop sum)?
+    std::unique_ptr<Expr> orx()
+    {
+        auto x = andx();
+        while (x && eat("|")) x = mk(Expr::OR, std::move(x), andx());
+        return x;
+    }
+    std::unique_ptr<Expr> andx()
+    {
+        auto x = cmp();
+        while (x && eat("&")) x = mk(Expr::AND, std::move(x), cmp());
+        return x;
+    }
+    std::unique_ptr<Expr> cmp()
+    {
+        auto x = sum();
+        if (!x) return x;
+        static const std::pair<const char*, Expr::Op> ops[] = {{"<=", Expr::LE}, {">=", Expr::GE}, {"!=", Expr::NE},
+                                                               {"<", Expr::LT},  {">", Expr::GT},  {"=", Expr::EQ}};
+        for (const auto& o : ops)
+            if (eat(o.first)) return mk(o.second, std::move(x), sum());
+        return x;
+    }
+    std::unique_ptr<Expr> sum()
+    {
+        auto x = prod();
+        while (x) {
+            if (eat("+")) x = mk(Expr::ADD, std::move(x), prod());
+            else if (eat("-")) x = mk(Expr::SUB, std::move(x), prod());
+            else break;
+        }
+        return x;
+    }
+    std::unique_ptr<Expr> prod()
+    {
+        auto x = unary();
+        while (x) {
+            if (eat("*")) x = mk(Expr::MUL, std::move(x), unary());
+            else if (eat("/")) x = mk(Expr::DIV, std::move(x), unary());
+            else break;
+        }
+        return x;
+    }
+    std::unique_ptr<Expr> unary()
+    {
+        if (eat("-")) {
+            auto x = unary();
+            return x ? mk(Expr::NEG, std::move(x)) : nullptr;
+        }
+        if (eat("!")) {
+            auto x = unary();
+            return x ? mk(Expr::NOT, std::move(x)) : nullptr;
+        }
+        return power();
+    }
+    std::unique_ptr<Expr> power()
+    {
+        auto x = atom();
+        if (x && eat("^")) {
+            auto y = unary();   // right-associative; the exponent may carry a sign
+            return y ? mk(Expr::POW, std::move(x), std::move(y)) : nullptr;
+        }
+        return x;
+    }
+    std::unique_ptr<Expr> atom()
+    {
+        ws();
+        if (i >= s.size()) return fail_("unexpected end");
+        if (eat("(")) {
+            auto x = orx();
+            if (!x) return x;
+            if (!eat(")")) return fail_("')' expected");
+            return x;
+        }
+        const char c = s[i];
+        if (std::isdigit((unsigned char)c) || c == '.') {
+            const char* b = s.c_str() + i;
+            char* end = nullptr;
+            double v = std::strtod(b, &end);
+            if (end == b) return fail_("bad number");
+            i += (size_t)(end - b);
+            auto e = std::make_unique<Expr>();
+            e->op = Expr::NUM;
+            e->v = v;
+            return e;
+        }
+        if (std::isalpha((unsigned char)c) || c == '_') {
+            size_t b = i;
+            while (i < s.size() && (std::isalnum((unsigned char)s[i]) || s[i] == '_')) ++i;
+            const std::string name = s.substr(b, i - b);
+            auto e = std::make_unique<Expr>();
+            if (name == "t") {
+                e->op = Expr::VAR;
+                return e;
+            }
+            if (name == "pi" || name == "e") {
+                e->op = Expr::NUM;
+                e->v = name == "pi" ? 3.14159265358979323846 : 2.71828182845904523536;
+                return e;
+            }
+            static const char* f1[] = {"sin",  "cos",  "tan", "asin", "acos",  "atan", "sinh", "cosh",
+                                       "tanh", "exp",  "log", "log10", "sqrt", "abs",  "floor", "ceil", "int"};
+            static const char* f2[] = {"atan2", "pow", "min", "max"};
+            int fn = -1, ar = 0;
+            for (int k = 0; k < (int)(sizeof(f1) / sizeof(*f1)); ++k)
+                if (name == f1[k]) fn = k, ar = 1;
+            for (int k = 0; k < (int)(sizeof(f2) / sizeof(*f2)); ++k)
+                if (name == f2[k]) fn = k, ar = 2;
+            if (name == "if") fn = 0, ar = 3;
+            if (fn < 0) return fail_("unknown name '" + name + "'");
+            if (!eat("(")) return fail_("'(' expected after " + name);
+            e->op = ar == 1 ? Expr::FN1 : (ar == 2 ? Expr::FN2 : Expr::IF);
+            e->fn = fn;
+            for (int k = 0; k < ar; ++k) {
+                if (k && !eat(",")) return fail_("',' expected");
+                auto x = orx();
+                if (!x) return x;
+                e->a.push_back(std::move(x));
+            }
+            if (!eat(")")) return fail_("')' expected");
+            return e;
+        }
+        return fail_(std::string("unexpected '") + c + "'");
+    }
+};
+
+double expr_eval(const Expr& e, double t)
+{
+    auto A = [&](int k) { return expr_eval(*e.a[k], t); };
+    switch (e.op) {
+    case Expr::NUM: return e.v;
+    case Expr::VAR: return t;
+    case Expr::NEG: return -A(0);
+    case Expr::NOT: return A(0) == 0 ? 1.0 : 0.0;
+    case Expr::ADD: return A(0) + A(1);
+    case Expr::SUB: return A(0) - A(1);
+    case Expr::MUL: return A(0) * A(1);
+    case Expr::DIV: return A(0) / A(1);
+    case Expr::POW: return std::pow(A(0), A(1));
+    case Expr::LT: return A(0) < A(1) ? 1.0 : 0.0;
+    case Expr::LE: return A(0) <= A(1) ? 1.0 : 0.0;
+    case Expr::GT: return A(0) > A(1) ? 1.0 : 0.0;
+    case Expr::GE: return A(0) >= A(1) ? 1.0 : 0.0;
+    case Expr::EQ: return A(0) == A(1) ? 1.0 : 0.0;
+    case Expr::NE: return A(0) != A(1) ? 1.0 : 0.0;
+    case Expr::AND: return (A(0) != 0 && A(1) != 0) ? 1.0 : 0.0;
+    case Expr::OR: return (A(0) != 0 || A(1) != 0) ? 1.0 : 0.0;
+    case Expr::IF: return A(0) != 0 ? A(1) : A(2);
+    case Expr::FN1: {
+        const double x = A(0);
+        switch (e.fn) {
+        case 0: return std::sin(x);
+        case 1: return std::cos(x);
+        case 2: return std::tan(x);
+        case 3: return std::asin(x);
+        case 4: return std::acos(x);
+        case 5: return std::atan(x);
+        case 6: return std::sinh(x);
+        case 7: return std::cosh(x);
+        case 8: return std::tanh(x);
+        case 9: return std::exp(x);
+        case 10: return std::log(x);
+        case 11: return std::log10(x);
+        case 12: return std::sqrt(x);
+        case 13: return std::fabs(x);
+        case 14: return std::floor(x);
+        case 15: return std::ceil(x);
+        default: return std::floor(x + 0.5);
+        }
+    }
+    case Expr::FN2: {
+        const double x = A(0), y = A(1);
+        switch (e.fn) {
+        case 0: return std::atan2(x, y);
+        case 1: return std::pow(x, y);
+        case 2: return x < y ? x : y;
+        default: return x > y ? x : y;
+        }
+    }
+    }
+    return 0;
+}
+
+// parse a custom signal; on failure returns null and sets err
+std::unique_ptr<Expr> parse_signal(const std::string& text, std::string& err)
+{
+    ExprParser P(text);
+    auto e = P.orx();
+    P.ws();
+    if (e && P.i != text.size()) e = P.fail_("unexpected text");
+    err = P.err;
+    return e;
+}
+}  // namespace
+
 static unsigned nyquist_num(double fmax, double dT)
 {
     // CalcNyquistNum (tools/useful.cpp:30-36)
@@ -1407,6 +1651,23 @@ int emsop::build_signal(unsigned nrts)
         sigC = {1.0f, 1.0f};
         nyquist = 1;
         break;
+    case EMSOP_SIG_CUSTOM: {
+        // CalcCustomExcitation (excitation.cpp:219-252): nrts samples of f(t)
+        std::string err;
+        auto ex = parse_signal(sigFunc, err);
+        if (!ex) return fail(EMSOP_E_ARG, "custom excitation: " + err);
+        sigLen = nrts;
+        sigV.resize(sigLen);
+        sigC.resize(sigLen);
+        for (unsigned n = 0; n < sigLen; ++n) {
+            double t = n * dT;
+            sigV[n] = (float)expr_eval(*ex, t);
+            t += 0.5 * dT;
+            sigC[n] = (float)expr_eval(*ex, t);
+        }
+        nyquist = nyquist_num(f0, dT);
+        break;
+    }
     default:
         return fail(EMSOP_E_STATE, "excitation signal not defined");
     }
@@ -2020,6 +2281,18 @@ int emsop_set_signal(emsop* h, int type, double f0, double fc)
     h->sigType = type;
     h->f0 = f0;
     h->fc = fc;
+    return 0;
+}
+
+int emsop_set_custom_signal(emsop* h, const char* function, double f0)
+{
+    if (!h || !function) return fail(EMSOP_E_ARG, "null argument");
+    std::string err;
+    if (!parse_signal(function, err)) return fail(EMSOP_E_ARG, "custom excitation: " + err);
+    h->sigType = EMSOP_SIG_CUSTOM;
+    h->sigFunc = function;
+    h->f0 = f0;
+    h->fc = 0;
     return 0;
 }
 

@@ -147,8 +147,11 @@ class Operator:
         vph = np.asarray(s.get("mur_vph", [0.0] * 6), np.float64)
         check_op(L.emsop_set_boundary(self.h, ptr(bc, C.c_int), ptr(pml, C.c_uint), ptr(vph, C.c_double)))
         sig = s["signal"]
-        check_op(L.emsop_set_signal(self.h, SIGNALS[sig["type"]], float(sig.get("f0", 0.0)),
-                                    float(sig.get("fc", 0.0))))
+        if sig["type"] == "custom":
+            check_op(L.emsop_set_custom_signal(self.h, str(sig["function"]).encode(), float(sig.get("f0", 0.0))))
+        else:
+            check_op(L.emsop_set_signal(self.h, SIGNALS[sig["type"]], float(sig.get("f0", 0.0)),
+                                        float(sig.get("fc", 0.0))))
         check_op(L.emsop_set_timestep(self.h, int(s.get("ts_method", 3)), float(s.get("forced_dt", 0.0)),
                                       float(s.get("ts_factor", 1.0))))
         check_op(L.emsop_set_cell_constant_material(self.h, int(bool(s.get("cell_constant_material", False)))))

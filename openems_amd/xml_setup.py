@@ -10,7 +10,8 @@ The ``<FDTD>`` block follows openEMS::Parse_XML_FDTDSetup
 (0 or absent means 1e-6), OverSampling (used when > 1), CellConstantMaterial,
 TimeStepMethod, TimeStep, TimeStepFactor, f_max, ``<BoundaryCond>`` (integer
 codes or PEC / PMC / MUR / PML_n, Mur phase velocities) and ``<Excitation>``
-(Type 0 Gaussian f0/fc, 1 sinusoid f0, 2 Dirac and 3 step with f_max).
+(Type 0 Gaussian f0/fc, 1 sinusoid f0, 2 Dirac and 3 step with f_max, 10
+custom Function of t with f0).
 
 The ``<ContinuousStructure>`` block is CSXCAD's format, read here for the
 subset the engine takes without CSXCAD: ``RectilinearGrid`` (DeltaUnit,
@@ -21,7 +22,7 @@ DebyeMaterial with Box primitives (P1, P2, Priority) and, for Metal and
 Excitation, Curve primitives (Vertex points).  Properties are entered in file
 order, primitives in property order, as CSXCAD hands them to openEMS
 (openems.cpp:483-694 for probes and dumps).  Anything else (other primitive
-types, custom excitation functions, cylindrical meshes) is refused with a
+types, cylindrical meshes) is refused with a
 ValueError naming it.
 """
 from __future__ import annotations
@@ -32,7 +33,7 @@ import xml.etree.ElementTree as ET
 import numpy as np
 
 _BOUNDS = ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax")
-_SIGNAL_TYPES = {0: "gauss", 1: "sinus", 2: "dirac", 3: "step"}
+_SIGNAL_TYPES = {0: "gauss", 1: "sinus", 2: "dirac", 3: "step", 10: "custom"}
 _SIGNAL_CODES = {v: k for k, v in _SIGNAL_TYPES.items()}
 
 
@@ -181,11 +182,13 @@ def read_xml(source):
         raise ValueError("no <Excitation> in the FDTD block")
     etype = _int(exc, "Type", -1)
     if etype not in _SIGNAL_TYPES:
-        raise ValueError(f"excitation type {etype} is not supported (custom functions need fparser)")
+        raise ValueError(f"excitation type {etype} is not supported")
     f_max = float(fdtd.get("f_max", 0.0))
     sig = {"type": _SIGNAL_TYPES[etype], "f0": float(exc.get("f0", 0.0)), "fc": float(exc.get("fc", 0.0))}
-    if etype in (2, 3):
+    if etype in (2, 3, 10):
         sig["f_max"] = f_max
+    if etype == 10:
+        sig["function"] = exc.get("Function", "")
     scene["signal"] = sig
     if fdtd.get("TimeStepMethod") is not None:
         scene["ts_method"] = _int(fdtd, "TimeStepMethod", 3)
@@ -308,7 +311,7 @@ def write_xml(scene, path=None):
     fd = ET.SubElement(root, "FDTD", NumberOfTimesteps=str(int(scene.get("nrts", 1000))),
                        endCriteria=repr(float(scene.get("end_criteria", 1e-6)) or 1e-6),
                        OverSampling=str(int(scene.get("oversampling", 4))))
-    if sig["type"] in ("dirac", "step"):
+    if sig["type"] in ("dirac", "step", "custom"):
         fd.set("f_max", repr(float(sig.get("f_max", 0.0))))
     if "ts_method" in scene:
         fd.set("TimeStepMethod", str(int(scene["ts_method"])))
@@ -318,8 +321,10 @@ def write_xml(scene, path=None):
         fd.set("TimeStepFactor", repr(float(scene["ts_factor"])))
     if scene.get("cell_constant_material"):
         fd.set("CellConstantMaterial", "1")
-    ET.SubElement(fd, "Excitation", Type=str(_SIGNAL_CODES[sig["type"]]), f0=repr(float(sig.get("f0", 0.0))),
-                  fc=repr(float(sig.get("fc", 0.0))))
+    ex = ET.SubElement(fd, "Excitation", Type=str(_SIGNAL_CODES[sig["type"]]), f0=repr(float(sig.get("f0", 0.0))),
+                       fc=repr(float(sig.get("fc", 0.0))))
+    if sig["type"] == "custom":
+        ex.set("Function", sig["function"])
     bc = ET.SubElement(fd, "BoundaryCond")
     pml = scene.get("pml_size", [8] * 6)
     for n, name in enumerate(_BOUNDS):

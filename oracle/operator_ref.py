@@ -242,7 +242,7 @@ def excitation_signal(sig, dT, nrts):
     """Returns (volt float32[L], curr float32[L], nyquist, period, length).
 
     Gaussian: excitation.cpp:150-176; sinusoid: :254-276; Dirac :178-198;
-    step :200-217.
+    step :200-217; custom :219-252.
     """
     kind = sig["type"]
     if kind == "gauss":
@@ -277,7 +277,64 @@ def excitation_signal(sig, dT, nrts):
         return (np.array([0, 1], np.float32), np.array([0, 1], np.float32), 1, 0.0, 2)
     if kind == "step":
         return (np.array([1, 1], np.float32), np.array([1, 1], np.float32), 1, 0.0, 2)
+    if kind == "custom":
+        # CalcCustomExcitation (excitation.cpp:219-252): nrts samples of f(t)
+        f = custom_function(sig["function"])
+        f0 = float(sig.get("f0", 0.0))
+        v = np.empty(nrts, dtype=np.float32)
+        c = np.empty(nrts, dtype=np.float32)
+        for n in range(nrts):
+            t = n * dT
+            v[n] = f(t)
+            t += 0.5 * dT
+            c[n] = f(t)
+        return v, c, nyquist_num(f0, dT), 0.0, nrts
     raise ValueError(kind)
+
+
+def custom_function(text):
+    """f(t) of a custom excitation written in the reference's function-parser
+    syntax (fparser: ^ power, = equality, & | logical, pi, e), evaluated through
+    Python's own expression parser: the text is rewritten into Python syntax
+    and its tree walked with IEEE double arithmetic and libm functions."""
+    import ast
+    import re
+    src = text.replace("^", "**")
+    src = re.sub(r"(?<![<>!=])=(?!=)", "==", src)
+    src = src.replace("&", " and ").replace("|", " or ")
+    src = re.sub(r"\bif\s*\(", "_if(", src)          # 'if' is a Python keyword
+    tree = ast.parse(src, mode="eval").body
+    fns = {"sin": math.sin, "cos": math.cos, "tan": math.tan, "asin": math.asin, "acos": math.acos,
+           "atan": math.atan, "sinh": math.sinh, "cosh": math.cosh, "tanh": math.tanh, "exp": math.exp,
+           "log": math.log, "log10": math.log10, "sqrt": math.sqrt, "abs": math.fabs, "floor": math.floor,
+           "ceil": math.ceil, "int": lambda x: math.floor(x + 0.5), "atan2": math.atan2, "pow": math.pow,
+           "min": lambda a, b: a if a < b else b, "max": lambda a, b: a if a > b else b}
+    consts = {"pi": 3.14159265358979323846, "e": 2.71828182845904523536}
+    binops = {ast.Add: lambda a, b: a + b, ast.Sub: lambda a, b: a - b, ast.Mult: lambda a, b: a * b,
+              ast.Div: lambda a, b: a / b, ast.Pow: math.pow}
+    cmps = {ast.Lt: lambda a, b: a < b, ast.LtE: lambda a, b: a <= b, ast.Gt: lambda a, b: a > b,
+            ast.GtE: lambda a, b: a >= b, ast.Eq: lambda a, b: a == b, ast.NotEq: lambda a, b: a != b}
+
+    def ev(node, t):
+        if isinstance(node, ast.Constant):
+            return float(node.value)
+        if isinstance(node, ast.Name):
+            return t if node.id == "t" else consts[node.id]
+        if isinstance(node, ast.BinOp):
+            return binops[type(node.op)](ev(node.left, t), ev(node.right, t))
+        if isinstance(node, ast.UnaryOp) and isinstance(node.op, ast.USub):
+            return -ev(node.operand, t)
+        if isinstance(node, ast.Compare) and len(node.ops) == 1:
+            return 1.0 if cmps[type(node.ops[0])](ev(node.left, t), ev(node.comparators[0], t)) else 0.0
+        if isinstance(node, ast.BoolOp):
+            vals = [ev(x, t) != 0 for x in node.values]
+            return 1.0 if (all(vals) if isinstance(node.op, ast.And) else any(vals)) else 0.0
+        if isinstance(node, ast.Call) and isinstance(node.func, ast.Name):
+            if node.func.id == "_if":
+                return ev(node.args[1], t) if ev(node.args[0], t) != 0 else ev(node.args[2], t)
+            return float(fns[node.func.id](*[ev(a, t) for a in node.args]))
+        raise ValueError(f"unsupported expression part {ast.dump(node)}")
+    return lambda t: ev(tree, float(t))
 
 
 def signal_period(sig):

@@ -57,7 +57,7 @@ def test_batch_instances_match_oracle(gpu):
 
 
 @pytest.mark.parametrize("config", ["autotune", "tiles32_depth3_grid7", "tiles64_own_depths_full_grid",
-                                    "groups_of_one_or_two"])
+                                    "groups_of_one_or_two", "graph_chunks_of_32", "graph_chunks_of_7"])
 def test_batch_equals_engines_alone(gpu, config):
     scs = _small_batch(4, 90, seed=77)
     print("case batch of 4 vs engines alone, 90 steps,", config, flush=True)
@@ -71,9 +71,15 @@ def test_batch_equals_engines_alone(gpu, config):
     elif config == "groups_of_one_or_two":
         b.set_grouping(3 << 20)          # a few MiB: several groups, stepped one after another
         assert b.config()["groups"] > 1
+    elif config == "graph_chunks_of_32":
+        assert b.graph() == 0            # direct launches by default
+        b.set_graph(32)
+    elif config == "graph_chunks_of_7":
+        b.set_grouping(3 << 20)
+        b.set_graph(7)                   # 40 = 5 x 7 + 5, 50 = 7 x 7 + 1 steps per call
     else:
         b.set_config(tile_x=64, zdepth=None, grid=0)
-    print("config", b.config(), flush=True)
+    print("config", b.config(), "graph", b.graph(), flush=True)
     b.iterate(40)
     b.iterate(50)
     _check_same(b, ref, 90)

@@ -577,3 +577,21 @@ def test_td_dumps_e_and_h_every_interpolation(gpu):
         assert np.abs(want).max() > 0, d["name"]
         assert np.array_equal(got["td"], want), (d["name"], np.count_nonzero(got["td"] != want))
     sim.close()
+
+
+def test_custom_excitation_runs_on_the_engine(gpu):
+    """A custom excitation signal (CalcCustomExcitation, NrTS samples of f(t))
+    drives the engine: fields bit-identical to the oracle's after 200 steps."""
+    scene = scenes.upml_bricks(32, 200, nbricks=4, dump=False)
+    scene["signal"] = {"type": "custom", "function": "sin(2*pi*8e9*t)*exp(-((t-1.2e-10)/4e-11)^2)", "f0": 8e9}
+    print("case upml_bricks_32 custom excitation, 200 steps", flush=True)
+    op = Operator(scene)
+    eng = EngineHIP(op)
+    eng.IterateTS(200)
+    ref = RefEngine(_ref_from_scene(scene), "basic")
+    ref.iterate(200)
+    v, c, rv, rc = _fields_equal(eng, ref)
+    assert np.max(np.abs(rv)) > 0
+    assert np.array_equal(v, rv) and np.array_equal(c, rc)
+    eng.close()
+    ref.close()

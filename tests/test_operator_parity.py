@@ -276,3 +276,27 @@ def test_current_probe_box_widens_with_snap_method_1(native):
     # dual lines at i + 0.5; x 3.7..9.2 snaps to 3 (3.5) .. 8 (8.5), both inside -> widened to 3 .. 9
     r = RefProbe(ref, 1, (3.7, 4.6, 7.5), (9.2, 10.4, 7.5), 1.0, [], 2)
     assert r.enabled and r.start[:2] == [3, 4] and r.stop[:2] == [9, 10]
+
+
+@pytest.mark.parametrize("function", [
+    "sin(2*pi*3e9*t)*exp(-((t-1e-9)/3e-10)^2)",
+    "if(t<2e-10, t/2e-10, 1) * cos(2*pi*5e9*t)",
+    "-t^2*1e18 + 3*t*1e9 - sqrt(abs(sin(t*1e10)))",
+    "max(0, min(1, t*4e9)) * (t>1e-10 & t<9e-10)",
+    "2^-1 + atan2(t*1e9, 1) - floor(t*7e9)/ceil(t*3e9+0.5) + int(t*2.5e9) + log10(1+t*1e9)",
+])
+def test_custom_excitation_signal(native, function):
+    """Custom excitation (CalcCustomExcitation): the product's parser and the
+    oracle's (Python's own expression parser) give the same float32 samples."""
+    sc = scenes.pec_cavity(16, 300)
+    sc["signal"] = {"type": "custom", "function": function, "f0": 5e9}
+    op, ref = _compare(sc)
+    assert op.signal()[0].size == 300
+
+
+def test_custom_excitation_parse_error(native):
+    from openems_amd._native import NativeError
+    sc = scenes.pec_cavity(16, 30)
+    sc["signal"] = {"type": "custom", "function": "sin(2*pi*t", "f0": 5e9}
+    with pytest.raises(NativeError, match="expected at position"):
+        Operator(sc)

@@ -642,3 +642,23 @@ def test_td_dump_schedule():
     v = np.zeros((3,) + op.N, np.float32)
     inv.process(2 * inv.interval, v, v)
     assert inv.samples[0][1] == (2 * inv.interval + 0.5) * op.dT
+
+
+def test_custom_signal_reduces_to_the_builtin_sinusoid():
+    """A custom excitation f(t) = sin(2*pi*f0*t) samples exactly the built-in
+    sinusoid (same operations in the same order) over the sinusoid's length
+    (but for the sinusoid's first current sample, set to 0),
+    and runs for NrTS samples with the Nyquist number of f0; the oracle's
+    parser keeps the reference parser's precedence (unary minus below ^,
+    right-associative ^, comparisons and & giving 1 or 0)."""
+    from oracle.operator_ref import custom_function, excitation_signal
+    f0 = 3e9
+    dT = 1.7e-12
+    vs, cs, nq, _, L = excitation_signal({"type": "sinus", "f0": f0}, dT, 5000)
+    vc, cc, nqc, per, Lc = excitation_signal({"type": "custom", "function": "sin(2*pi*3e9*t)", "f0": f0}, dT, 5000)
+    assert Lc == 5000 and per == 0.0 and nqc == nq
+    # the built-in sinusoid sets its first current sample to 0 (excitation.cpp:263-264)
+    assert np.array_equal(vc[:L], vs) and np.array_equal(cc[1:L], cs[1:]) and cs[0] == 0 and cc[0] != 0
+    f = custom_function("-t^2^0.5 + (t>1)*(t<3) + (t>=2 & t<=2) + if(t=4, 10, 0)")
+    assert f(4.0) == -(4.0 ** (2 ** 0.5)) + 0 + 0 + 10
+    assert f(2.0) == -(2.0 ** (2 ** 0.5)) + 1 + 1

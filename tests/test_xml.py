@@ -103,8 +103,11 @@ def test_structure_subset():
 def test_unsupported_parts_are_refused():
     with pytest.raises(ValueError, match="Cylinder|cylindrical"):
         read_xml(SETUP.replace('<FDTD NumberOfTimesteps="2e3"', '<FDTD CylinderCoords="1" NumberOfTimesteps="2e3"'))
-    with pytest.raises(ValueError, match="excitation type 10"):
-        read_xml(SETUP.replace('<Excitation Type="0" f0="2e9" fc="5e9"/>', '<Excitation Type="10" f0="2e9"/>'))
+    with pytest.raises(ValueError, match="excitation type 7"):
+        read_xml(SETUP.replace('<Excitation Type="0" f0="2e9" fc="5e9"/>', '<Excitation Type="7" f0="2e9"/>'))
+    custom = read_xml(SETUP.replace('<Excitation Type="0" f0="2e9" fc="5e9"/>',
+                                    '<Excitation Type="10" f0="2e9" Function="sin(2*pi*2e9*t)"/>'))
+    assert custom["signal"] == {"type": "custom", "f0": 2e9, "fc": 0.0, "f_max": 7e9, "function": "sin(2*pi*2e9*t)"}
     with pytest.raises(ValueError, match="Sphere"):
         read_xml(SETUP.replace('<Box Priority="4">', '<Sphere Priority="4"/><Box Priority="4">'))
 
