@@ -196,7 +196,7 @@ def run_batched(count, steps, tune_steps=6):
            "sequential_mcells": round(cells * steps / seq_s / 1e6, 1),
            "bit_identical": bool(same), "layout": b.layout,
            "autotuned": {"tile_x": cfg["tile_x"], "grid": cfg["grid"], "work_items": cfg["work_items"],
-                         "groups": cfg["groups"], "group_bytes": cfg["group_bytes"],
+                         "groups": cfg["groups"], "group_bytes": cfg["group_bytes"], "streams": b.streams(),
                          "zdepths": sorted(set(cfg["zdepth"])), "tuned_ms_per_step": round(tuned_ms, 4)},
            "sequential_engine": "default single engine (compressed coefficients, two half steps), one "
                                 "instance after another, setup excluded"}

@@ -324,6 +324,11 @@ int ems_batch_iterate(ems_batch* b, unsigned nsteps);
  * and replayed (0: direct launches, the default).  Results are the same. */
 int ems_batch_set_graph(ems_batch* b, unsigned steps);
 int ems_batch_get_graph(const ems_batch* b, unsigned* steps);
+/* Run the groups on n streams (1..4; group i on stream i % n), so that
+ * several groups advance at once; each group's working set should then be
+ * about the cache budget over n.  ems_batch_iterate returns when all are done. */
+int ems_batch_set_streams(ems_batch* b, unsigned n);
+int ems_batch_get_streams(const ems_batch* b, unsigned* n);
 /* device time of the last ems_batch_iterate call */
 int ems_batch_last_ms(const ems_batch* b, double* ms);
 void ems_batch_destroy(ems_batch* b);

@@ -128,6 +128,8 @@ ENGINE_API = [
     ("ems_batch_iterate", C.c_int, [vp, C.c_uint]),
     ("ems_batch_set_graph", C.c_int, [vp, C.c_uint]),
     ("ems_batch_get_graph", C.c_int, [vp, up]),
+    ("ems_batch_set_streams", C.c_int, [vp, C.c_uint]),
+    ("ems_batch_get_streams", C.c_int, [vp, up]),
     ("ems_batch_last_ms", C.c_int, [vp, dp]),
     ("ems_batch_destroy", None, [vp]),
     ("ems_adj_create", C.c_int, [C.POINTER(vp), vp, C.c_uint, up, up, dp, C.c_double, C.c_double, up, up, C.c_uint]),

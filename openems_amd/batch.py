@@ -87,6 +87,15 @@ class BatchEngine:
         check(lib().ems_batch_get_graph(self.b, C.byref(v)))
         return v.value
 
+    def set_streams(self, n):
+        """Run the groups on n streams (1..4) so that several advance at once."""
+        check(lib().ems_batch_set_streams(self.b, int(n)))
+
+    def streams(self):
+        v = C.c_uint()
+        check(lib().ems_batch_get_streams(self.b, C.byref(v)))
+        return v.value
+
     def iterate(self, nsteps):
         check(lib().ems_batch_iterate(self.b, int(nsteps)))
 

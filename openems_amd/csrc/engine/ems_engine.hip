@@ -3458,6 +3458,14 @@ struct ems_batch {
     // launch overhead); dropped when the configuration changes
     unsigned graph_steps = 0;
     std::vector<hipGraphExec_t> gexec;
+    // groups run on nstreams streams (group i on stream i % nstreams), so the
+    // short source / probe launches and the last waves of one group's half
+    // step overlap another group's work; stream 0 is `stream`
+    unsigned nstreams = 1;
+    std::vector<hipStream_t> xs;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> joins;
+    hipStream_t gstream(size_t gi) const { return gi % nstreams == 0 ? stream : xs[gi % nstreams - 1]; }
     unsigned tx = 0, grid_req = 0;
     size_t group_bytes = 0;
     std::vector<unsigned> zdepth;
@@ -3493,23 +3501,23 @@ struct ems_batch {
 namespace {
 
 template <int PM, bool CC, bool CURR>
-void batch_launch_t(ems_batch* b, const ems_batch::Group& g)
+void batch_launch_t(ems_batch* b, const ems_batch::Group& g, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_batch_half<PM, CC, CURR>), dim3(g.grid), dim3(kBlock), 0, b->stream, b->d_inst, g.d_items,
+    hipLaunchKernelGGL((k_batch_half<PM, CC, CURR>), dim3(g.grid), dim3(kBlock), 0, st, b->d_inst, g.d_items,
                        g.nitems);
 }
 
 template <bool CURR>
-void batch_launch(ems_batch* b, const ems_batch::Group& g)
+void batch_launch(ems_batch* b, const ems_batch::Group& g, hipStream_t st)
 {
     if (b->cmp) {
-        if (b->pm == kPmlCompressed) batch_launch_t<kPmlCompressed, true, CURR>(b, g);
-        else if (b->pm == kPmlPlain) batch_launch_t<kPmlPlain, true, CURR>(b, g);
-        else batch_launch_t<kNoPml, true, CURR>(b, g);
+        if (b->pm == kPmlCompressed) batch_launch_t<kPmlCompressed, true, CURR>(b, g, st);
+        else if (b->pm == kPmlPlain) batch_launch_t<kPmlPlain, true, CURR>(b, g, st);
+        else batch_launch_t<kNoPml, true, CURR>(b, g, st);
     } else {
-        if (b->pm == kPmlCompressed) batch_launch_t<kPmlCompressed, false, CURR>(b, g);
-        else if (b->pm == kPmlPlain) batch_launch_t<kPmlPlain, false, CURR>(b, g);
-        else batch_launch_t<kNoPml, false, CURR>(b, g);
+        if (b->pm == kPmlCompressed) batch_launch_t<kPmlCompressed, false, CURR>(b, g, st);
+        else if (b->pm == kPmlPlain) batch_launch_t<kPmlPlain, false, CURR>(b, g, st);
+        else batch_launch_t<kNoPml, false, CURR>(b, g, st);
     }
 }
 
@@ -3606,37 +3614,42 @@ int batch_build(ems_batch* b, unsigned tx, const std::vector<unsigned>& zd, unsi
     return 0;
 }
 
-void batch_step(ems_batch* b, const ems_batch::Group& g)
+void batch_step(ems_batch* b, const ems_batch::Group& g, hipStream_t st)
 {
     const unsigned T = 256;
-    batch_launch<false>(b, g);
+    batch_launch<false>(b, g, st);
     if (g.G[0])
-        hipLaunchKernelGGL(k_batch_excite, dim3((g.G[0] + T - 1) / T), dim3(T), 0, b->stream, b->d_inst, g.d_grp[0],
+        hipLaunchKernelGGL(k_batch_excite, dim3((g.G[0] + T - 1) / T), dim3(T), 0, st, b->d_inst, g.d_grp[0],
                            g.G[0], g.d_amp[0], g.d_delay[0], 0, 0);
-    batch_launch<true>(b, g);
+    batch_launch<true>(b, g, st);
     if (g.G[1])
-        hipLaunchKernelGGL(k_batch_excite, dim3((g.G[1] + T - 1) / T), dim3(T), 0, b->stream, b->d_inst, g.d_grp[1],
+        hipLaunchKernelGGL(k_batch_excite, dim3((g.G[1] + T - 1) / T), dim3(T), 0, st, b->d_inst, g.d_grp[1],
                            g.G[1], g.d_amp[1], g.d_delay[1], 1, -1);
     if (g.nprobes)
-        hipLaunchKernelGGL(k_batch_probe, dim3(g.nprobes), dim3(kBlock), 0, b->stream, b->d_inst, g.d_probes);
+        hipLaunchKernelGGL(k_batch_probe, dim3(g.nprobes), dim3(kBlock), 0, st, b->d_inst, g.d_probes);
 }
 
-// every group advances nsteps, group after group
 // All kernel arguments of a step are the same at every step (the timestep
 // is read on the device), so a chunk of steps replays as a graph.
 int batch_run(ems_batch* b, unsigned nsteps)
 {
     const unsigned S = b->graph_steps;
     if (S && b->gexec.size() != b->groups.size()) b->gexec.assign(b->groups.size(), nullptr);
+    if (b->nstreams > 1) {
+        // fork: the other streams start after what is queued on stream 0
+        HIPCHK(hipEventRecord(b->fork, b->stream));
+        for (auto st : b->xs) HIPCHK(hipStreamWaitEvent(st, b->fork, 0));
+    }
     for (size_t gi = 0; gi < b->groups.size(); ++gi) {
         const auto& g = b->groups[gi];
+        hipStream_t st = b->gstream(gi);
         unsigned s = 0;
         if (S && nsteps >= S) {
             if (!b->gexec[gi]) {
                 hipGraph_t graph;
-                HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-                for (unsigned k = 0; k < S; ++k) batch_step(b, g);
-                HIPCHK(hipStreamEndCapture(b->stream, &graph));
+                HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                for (unsigned k = 0; k < S; ++k) batch_step(b, g, st);
+                HIPCHK(hipStreamEndCapture(st, &graph));
                 hipError_t e = hipGraphInstantiate(&b->gexec[gi], graph, nullptr, nullptr, 0);
                 hipGraphDestroy(graph);
                 if (e != hipSuccess) {
@@ -3644,10 +3657,16 @@ int batch_run(ems_batch* b, unsigned nsteps)
                     return fail(EMS_E_HIP, std::string("graph instantiation failed: ") + hipGetErrorString(e));
                 }
             }
-            for (; s + S <= nsteps; s += S) HIPCHK(hipGraphLaunch(b->gexec[gi], b->stream));
+            for (; s + S <= nsteps; s += S) HIPCHK(hipGraphLaunch(b->gexec[gi], st));
         }
-        for (; s < nsteps; ++s) batch_step(b, g);
+        for (; s < nsteps; ++s) batch_step(b, g, st);
     }
+    if (b->nstreams > 1)
+        // join: stream 0 continues after every other stream's work
+        for (size_t k = 0; k < b->xs.size(); ++k) {
+            HIPCHK(hipEventRecord(b->joins[k], b->xs[k]));
+            HIPCHK(hipStreamWaitEvent(b->stream, b->joins[k], 0));
+        }
     return 0;
 }
 
@@ -3831,11 +3850,12 @@ int ems_batch_iterate(ems_batch* b, unsigned nsteps)
 }
 
 // Autotuner: time every candidate tile shape x z-depth policy x grid size x
-// group budget on the batch itself for `steps` steps of every group, restore
-// the state the candidates advanced (fields, UPML fluxes, counters, probe
-// sample counts), and keep the fastest.  Depth policies: each instance's own
-// (choose_zdepth) or 8 for all; grids: one workgroup per work item or the
-// resident workgroups; group budgets: one group, 64, 128, 160, 192 or 224 MiB
+// (stream count, group budget) on the batch itself for `steps` steps of all
+// groups, restore the state the candidates advanced (fields, UPML fluxes,
+// counters, probe sample counts), and keep the fastest.  Depth policies: each
+// instance's own (choose_zdepth), 4 or 8 for all; grids: one workgroup per
+// work item or the resident workgroups; plans: one stream with one group or
+// budgets of 128-224 MiB, two streams with 80-112 MiB, three with 56-64 MiB
 // (the last-level cache holds 256 MiB; the budget also sets how many work
 // items a group's launches spread over the resident workgroups).
 int ems_batch_autotune(ems_batch* b, unsigned steps, double* best_ms)
@@ -3907,49 +3927,63 @@ int ems_batch_autotune(ems_batch* b, unsigned steps, double* best_ms)
         hipDeviceSynchronize();
     };
     const unsigned txs[2] = {0, 64};
-    const unsigned depths[2] = {0, 8};   // 0: each instance's own choice
+    const unsigned depths[3] = {0, 4, 8};   // 0: each instance's own choice
     const unsigned grids[2] = {0, (unsigned)(b->resident * b->cus)};
     const size_t mib = size_t(1) << 20;
-    const size_t budgets[6] = {0, 64 * mib, 128 * mib, 160 * mib, 192 * mib, 224 * mib};
+    // (streams, group budget): one stream with budgets up to the cache size,
+    // several streams with a budget near the cache size over the stream count
+    const std::pair<unsigned, size_t> plans[] = {{1, 0},        {1, 128 * mib}, {1, 160 * mib}, {1, 192 * mib},
+                                                 {1, 224 * mib}, {2, 80 * mib},  {2, 96 * mib},  {2, 112 * mib},
+                                                 {3, 56 * mib},  {3, 64 * mib}};
+    const unsigned streams0 = b->nstreams;
     double best = 1e30;
-    unsigned btx = 0, bdep = 0, bgrid = 0;
+    unsigned btx = 0, bdep = 0, bgrid = 0, bns = streams0;
     size_t bbud = 0;
-    for (size_t bud : budgets)
+    for (const auto& plan : plans)
         for (unsigned tx : txs)
             for (unsigned dep : depths)
                 for (unsigned grid : grids) {
                     std::vector<unsigned> zd(b->eng.size());
                     for (size_t i = 0; i < zd.size(); ++i) zd[i] = dep ? dep : choose_zdepth(b->eng[i]->N);
-                    int rc = batch_build(b, tx, zd, grid, bud);
+                    int rc = batch_build(b, tx, zd, grid, plan.second);
+                    if (!rc) rc = ems_batch_set_streams(b, plan.first);
                     if (rc) {
                         restore();
                         cleanup();
                         return rc;
                     }
-                    // per group: one warm-up step, then steps-1 timed steps
-                    double tot = 0;
-                    for (const auto& g : b->groups) {
-                        batch_step(b, g);
-                        hipEventRecord(b->ev0, b->stream);
-                        for (unsigned s = 1; s < steps; ++s) batch_step(b, g);
-                        hipEventRecord(b->ev1, b->stream);
-                        hipEventSynchronize(b->ev1);
-                        float ms = 0;
-                        hipEventElapsedTime(&ms, b->ev0, b->ev1);
-                        tot += ms / (steps - 1);
+                    // one warm-up step of every group, then steps-1 timed steps of all groups
+                    if ((rc = batch_run(b, 1))) {
+                        restore();
+                        cleanup();
+                        return rc;
                     }
+                    hipEventRecord(b->ev0, b->stream);
+                    rc = batch_run(b, steps - 1);
+                    hipEventRecord(b->ev1, b->stream);
+                    hipEventSynchronize(b->ev1);
+                    if (rc) {
+                        restore();
+                        cleanup();
+                        return rc;
+                    }
+                    float ms = 0;
+                    hipEventElapsedTime(&ms, b->ev0, b->ev1);
+                    const double tot = ms / (steps - 1);
                     if (tot < best) {
                         best = tot;
                         btx = tx;
                         bdep = dep;
                         bgrid = grid;
-                        bbud = bud;
+                        bbud = plan.second;
+                        bns = plan.first;
                     }
                     restore();
                 }
     std::vector<unsigned> zd(b->eng.size());
     for (size_t i = 0; i < zd.size(); ++i) zd[i] = bdep ? bdep : choose_zdepth(b->eng[i]->N);
     int rc = batch_build(b, btx, zd, bgrid, bbud);
+    if (!rc) rc = ems_batch_set_streams(b, bns);
     HIPCHK(hipStreamSynchronize(b->stream));
     cleanup();
     if (best_ms) *best_ms = best;
@@ -3974,6 +4008,40 @@ int ems_batch_set_graph(ems_batch* b, unsigned steps)
     return 0;
 }
 
+int ems_batch_set_streams(ems_batch* b, unsigned n)
+{
+    if (!b) return fail(EMS_E_ARG, "null batch");
+    if (n < 1 || n > 4) return fail(EMS_E_ARG, "stream count must be 1..4");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (auto st : b->xs) HIPCHK(hipStreamSynchronize(st));
+    batch_drop_graphs(b);       // captured on the streams of the old assignment
+    if (!b->fork) HIPCHK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    while (b->xs.size() < n - 1) {
+        hipStream_t st;
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        b->xs.push_back(st);
+        hipEvent_t ev;
+        HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        b->joins.push_back(ev);
+    }
+    while (b->xs.size() > n - 1) {
+        hipStreamDestroy(b->xs.back());
+        hipEventDestroy(b->joins.back());
+        b->xs.pop_back();
+        b->joins.pop_back();
+    }
+    b->nstreams = n;
+    return 0;
+}
+
+int ems_batch_get_streams(const ems_batch* b, unsigned* n)
+{
+    if (!b || !n) return fail(EMS_E_ARG, "null argument");
+    *n = b->nstreams;
+    return 0;
+}
+
 int ems_batch_get_graph(const ems_batch* b, unsigned* steps)
 {
     if (!b || !steps) return fail(EMS_E_ARG, "null argument");
@@ -3986,7 +4054,11 @@ void ems_batch_destroy(ems_batch* b)
     if (!b) return;
     hipSetDevice(b->device);
     if (b->stream) hipStreamSynchronize(b->stream);
+    for (auto st : b->xs) hipStreamSynchronize(st);
     batch_drop_graphs(b);
+    for (auto st : b->xs) hipStreamDestroy(st);
+    for (auto ev : b->joins) hipEventDestroy(ev);
+    if (b->fork) hipEventDestroy(b->fork);
     for (void* p : b->gallocs) hipFree(p);
     for (void* p : b->allocs) hipFree(p);
     if (b->ev0) hipEventDestroy(b->ev0);

@@ -57,7 +57,8 @@ def test_batch_instances_match_oracle(gpu):
 
 
 @pytest.mark.parametrize("config", ["autotune", "tiles32_depth3_grid7", "tiles64_own_depths_full_grid",
-                                    "groups_of_one_or_two", "graph_chunks_of_32", "graph_chunks_of_7"])
+                                    "groups_of_one_or_two", "graph_chunks_of_32", "graph_chunks_of_7",
+                                    "three_streams", "two_streams_graphs"])
 def test_batch_equals_engines_alone(gpu, config):
     scs = _small_batch(4, 90, seed=77)
     print("case batch of 4 vs engines alone, 90 steps,", config, flush=True)
@@ -74,6 +75,14 @@ def test_batch_equals_engines_alone(gpu, config):
     elif config == "graph_chunks_of_32":
         assert b.graph() == 0            # direct launches by default
         b.set_graph(32)
+    elif config == "three_streams":
+        b.set_grouping(3 << 20)
+        b.set_streams(3)
+        assert b.streams() == 3 and b.config()["groups"] > 2
+    elif config == "two_streams_graphs":
+        b.set_grouping(3 << 20)
+        b.set_streams(2)
+        b.set_graph(16)
     elif config == "graph_chunks_of_7":
         b.set_grouping(3 << 20)
         b.set_graph(7)                   # 40 = 5 x 7 + 5, 50 = 7 x 7 + 1 steps per call
