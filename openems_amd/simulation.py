@@ -116,26 +116,35 @@ class Simulation:
         return paths
 
     def write_dumps(self, directory):
-        """Field dumps with FileType 1 as <name>.h5 in the reference's HDF5
-        layout (openems_amd.hdf5_out); returns the paths written.  FileType 0
-        (VTK) files are not written."""
+        """Field dumps as the reference writes them: FileType 1 as <name>.h5
+        (openems_amd.hdf5_out), FileType 0 as VTK RectilinearGrid files
+        (openems_amd.vtk_out); returns the paths written."""
         import os
-        from . import hdf5_out
+        from . import hdf5_out, vtk_out
         paths = []
         for d in self.dumps:
-            if int(d.spec.get("file_type", 1)) != 1 or not getattr(d, "enabled", True):
+            if not getattr(d, "enabled", True):
                 continue
-            path = os.path.join(directory, d.name + ".h5")
+            ftype = int(d.spec.get("file_type", 1))
+            if ftype not in (0, 1):
+                raise ValueError(f"dump {d.name!r}: file type {ftype} is not supported")
+            base = os.path.join(directory, d.name)
             if isinstance(d, FieldDumpTD):
                 ts, t, fields = d.collect(self.eng)
                 if fields is None:
                     continue
-                paths.append(hdf5_out.write_td_dump(path, self.op, d, ts, t, fields))
+                if ftype == 1:
+                    paths.append(hdf5_out.write_td_dump(base + ".h5", self.op, d, ts, t, fields))
+                else:
+                    paths += vtk_out.write_td_dump(base, self.op, d, ts, fields)
             else:
                 fd = d.collect(self.eng)
                 if fd is None:
                     continue
-                paths.append(hdf5_out.write_fd_dump(path, self.op, d, fd, d.freqs))
+                if ftype == 1:
+                    paths.append(hdf5_out.write_fd_dump(base + ".h5", self.op, d, fd, d.freqs))
+                else:
+                    paths += vtk_out.write_fd_dump(base, self.op, d, fd, d.freqs)
         return paths
 
     def close(self):

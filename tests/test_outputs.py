@@ -113,3 +113,40 @@ def test_hdf5_dump_files_in_the_reference_layout(native, tmp_path):
         assert attrs["frequency"][0] == np.float32(fdump.freqs[n])
         assert np.array_equal(re + 1j * im, np.transpose(fd[n], (0, 3, 2, 1)))
     assert hdf5_out.exists(path, "/Mesh/z") and not hdf5_out.exists(path, "/FieldData/TD")
+
+
+def test_vtk_dump_files(native, tmp_path):
+    """VTK dumps (FileType 0): RectilinearGrid files named as VTK_File_Writer
+    names them, the field array named after the dump type, points x fastest,
+    coordinates the dump lines in metres; frequency dumps as 21 phase
+    snapshots plus magnitude and phase."""
+    from openems_amd import vtk_out
+    from openems_amd.processing import FieldDumpFD, FieldDumpTD
+    scene = scenes.wire_antenna(30, 200)
+    op = Operator(scene, nthreads=2)
+    L = scene["lines"]
+    spec = {"name": "Et", "start": (L[0][5], L[1][6], L[2][7]), "stop": (L[0][9], L[1][12], L[2][8]),
+            "field": "E", "interp": 1, "domain": "td", "file_type": 0}
+    d = FieldDumpTD(op, spec, 200)
+    nx, ny, nz = (d.stop[n] - d.start[n] + 1 for n in range(3))
+    rng = np.random.default_rng(6)
+    fields = rng.normal(size=(2, 3, nx, ny, nz)).astype(np.float32)
+    paths = vtk_out.write_td_dump(str(tmp_path / "Et"), op, d, [0, 12], fields)
+    assert [p.split("/")[-1] for p in paths] == ["Et_0000000000.vtr", "Et_0000000012.vtr"]
+    coords, name, vec = vtk_out.read_vtr(paths[1])
+    assert name == "E-Field" and np.array_equal(vec, fields[1])
+    np.testing.assert_allclose(coords[0], [L[0][i] * 1e-3 for i in range(d.start[0], d.stop[0] + 1)], rtol=1e-15)
+    text = open(paths[1]).read()
+    assert 'type="RectilinearGrid"' in text and f'WholeExtent="0 {nx - 1} 0 {ny - 1} 0 {nz - 1}"' in text
+    fdump = FieldDumpFD(op, dict(spec, name="Hf", field="H", freqs=[8e9, 1.25e9 + 0.5], domain="fd"), 200)
+    nx, ny, nz = (fdump.stop[n] - fdump.start[n] + 1 for n in range(3))
+    fd = (rng.normal(size=(2, 3, nx, ny, nz)) + 1j * rng.normal(size=(2, 3, nx, ny, nz))).astype(np.complex64)
+    paths = vtk_out.write_fd_dump(str(tmp_path / "Hf"), op, fdump, fd, fdump.freqs)
+    names = [p.split("/")[-1] for p in paths]
+    assert len(names) == 2 * 23 and names[0] == "Hf_f=8000000000_p=000.vtr"
+    assert names[21] == "Hf_f=8000000000_abs.vtr" and names[22] == "Hf_f=8000000000_arg.vtr"
+    assert names[23].startswith("Hf_f=1250000000.500000_p=000")
+    _, name, p0 = vtk_out.read_vtr(paths[0])
+    assert name == "H-Field" and np.array_equal(p0, fd[0].real)
+    _, _, mag = vtk_out.read_vtr(paths[21])
+    np.testing.assert_allclose(mag, np.abs(fd[0]), rtol=1e-6)
