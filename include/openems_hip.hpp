@@ -40,6 +40,7 @@
 #include <algorithm>
 #include <stdexcept>
 #include <string>
+#include <array>
 #include <vector>
 
 #include "ems_hip.h"
@@ -105,6 +106,17 @@ public:
                           const double stop[3])
     {
         check_operator(emsop_add_excitation(h_, type, val, delay, priority, start, stop));
+    }
+    // wires (CSPrimCurve): points[i] in drawing units, as Operator::CalcPEC_Curves and
+    // Operator_Ext_Excitation take them (FDTD/operator.cpp:2086-2118, operator_ext_excitation.cpp:235-287)
+    void AddMetalCurve(const std::vector<std::array<double, 3>>& points)
+    {
+        check_operator(emsop_add_metal_curve(h_, (unsigned)points.size(), points.empty() ? nullptr : points[0].data()));
+    }
+    void AddExcitationCurve(int type, const double val[3], double delay, const std::vector<std::array<double, 3>>& points)
+    {
+        check_operator(emsop_add_excitation_curve(h_, type, val, delay, (unsigned)points.size(),
+                                                  points.empty() ? nullptr : points[0].data()));
     }
     // BC: 0 PEC, 1 PMC, 2 Mur, 3 PML, -1 none (openEMS::m_BC_type)
     void SetBoundaryCondition(const int BC[6], const unsigned pml_size[6], const double mur_vph[6])

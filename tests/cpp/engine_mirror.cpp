@@ -11,6 +11,7 @@
 // a Drude box (plasma 12 GHz, relaxation 30 ps) from (19, 9, 9) to (22, 14, 18);
 // Gaussian pulse f0 = 0, fc = 15 GHz.
 #include <cstdio>
+#include <array>
 #include <vector>
 
 #include "openems_hip.hpp"
@@ -44,6 +45,7 @@ int main()
     if (emsop_set_dispersion(op->handle(), 1, disp) < 0) return 2;
     const double m0[3] = {12, 20, 4}, m1[3] = {14, 22, 24};
     op->AddMetalBox(2, m0, m1);
+    op->AddMetalCurve({{{4.0, 4.0, 6.0}, {4.0, 4.0, 12.0}, {7.3, 5.6, 15.2}}});
     const double p0[3] = {14, 14, 10}, p1[3] = {14, 14, 12};
     op->AddLumpedElement(2, 50.0, 0.0, 0.0, true, true, 0, p0, p1);
     const double ez[3] = {0, 0, 1};

@@ -454,7 +454,8 @@ def _mirror_scene():
             {"eps": 1.0, "priority": 3, "start": (19, 9, 9), "stop": (22, 14, 18),
              "lorentz": [{"eps_plasma": 12e9, "eps_relax": 30e-12}]},
         ],
-        "metals": [{"priority": 2, "start": (12, 20, 4), "stop": (14, 22, 24)}],
+        "metals": [{"priority": 2, "start": (12, 20, 4), "stop": (14, 22, 24)},
+                   {"curve": [(4.0, 4.0, 6.0), (4.0, 4.0, 12.0), (7.3, 5.6, 15.2)]}],
         "lumped": [{"dir": 2, "R": 50.0, "C": 0.0, "L": 0.0, "caps": True, "type": "parallel", "priority": 0,
                     "start": (14, 14, 10), "stop": (14, 14, 12)}],
         "excitations": [{"type": 0, "val": (0.0, 0.0, 1.0), "start": (14, 14, 10), "stop": (14, 14, 12)}],
