@@ -259,6 +259,32 @@ def read_attribute(path, obj_path, name, dtype=np.float64, group=False):
     return out
 
 
+class _GInfo(C.Structure):
+    _fields_ = [("storage_type", C.c_int), ("nlinks", _hsize), ("max_corder", C.c_int64), ("mounted", C.c_bool)]
+
+
+def list_group(path, group):
+    """Names of the links in a group, in name order."""
+    L = _load()
+    for name, res, args in (("H5Gget_info", C.c_int, [_hid, C.POINTER(_GInfo)]),
+                            ("H5Lget_name_by_idx", C.c_ssize_t,
+                             [_hid, C.c_char_p, C.c_int, C.c_int, _hsize, C.c_char_p, C.c_size_t, _hid])):
+        getattr(L, name).restype = res
+        getattr(L, name).argtypes = args
+    f = _ok(L.H5Fopen(path.encode(), _H5F_ACC_RDONLY, _H5P_DEFAULT), "H5Fopen " + path)
+    g = _ok(L.H5Gopen2(f, group.encode(), _H5P_DEFAULT), "H5Gopen2 " + group)
+    info = _GInfo()
+    _ok(L.H5Gget_info(g, C.byref(info)), "H5Gget_info")
+    out = []
+    buf = C.create_string_buffer(256)
+    for k in range(info.nlinks):
+        n = _ok(L.H5Lget_name_by_idx(g, b".", 0, 0, k, buf, 256, _H5P_DEFAULT), "H5Lget_name_by_idx")
+        out.append(buf.value[:n].decode())
+    L.H5Gclose(g)
+    L.H5Fclose(f)
+    return out
+
+
 def exists(path, name):
     L = _load()
     f = _ok(L.H5Fopen(path.encode(), _H5F_ACC_RDONLY, _H5P_DEFAULT), "H5Fopen " + path)

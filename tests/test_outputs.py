@@ -113,6 +113,8 @@ def test_hdf5_dump_files_in_the_reference_layout(native, tmp_path):
         assert attrs["frequency"][0] == np.float32(fdump.freqs[n])
         assert np.array_equal(re + 1j * im, np.transpose(fd[n], (0, 3, 2, 1)))
     assert hdf5_out.exists(path, "/Mesh/z") and not hdf5_out.exists(path, "/FieldData/TD")
+    assert hdf5_out.list_group(path, "/FieldData/FD") == ["f0_imag", "f0_real", "f1_imag", "f1_real"]
+    assert hdf5_out.list_group(str(tmp_path / "Ht.h5"), "/FieldData/TD") == ["00000000", "00000007", "00000014"]
 
 
 def test_vtk_dump_files(native, tmp_path):

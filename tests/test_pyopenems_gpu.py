@@ -43,3 +43,65 @@ def test_lumped_port_script(gpu, tmp_path):
     back = np.loadtxt(str(tmp_path / "port_ut_1"), comments="%")
     np.testing.assert_allclose(back, td, rtol=1e-11, atol=1e-300)
     sim.close()
+
+
+def test_field_probes_equal_field_dumps(gpu, tmp_path):
+    """The reference's probe / dump consistency test (TESTSUITE/probes/
+    fieldprobes.m): an infinitesimal hard-source dipole with Mur on every
+    face; E and H time-domain HDF5 dumps (no interpolation) on the six faces
+    of a box and E / H point probes at the face centres.  At every sample the
+    dump at the probe point equals the probe (relative difference <= 1e-7,
+    times within 1e-13 s), and the fields are not trivially small."""
+    from openems_amd import hdf5_out
+    c0 = 299792458.0
+    unit = 1e-6
+    f_max = 1e9
+    dl = c0 / f_max / unit / 50
+    FDTD = openEMS(NrTS=1200, EndCriteria=1e-6, OverSampling=10)
+    FDTD.SetGaussExcite(0, f_max)
+    FDTD.SetBoundaryCond([2] * 6)
+    CSX = ContinuousStructure()
+    FDTD.SetCSX(CSX)
+    mesh = CSX.GetGrid()
+    mesh.SetDeltaUnit(unit)
+    for d in "xyz":
+        mesh.SetLines(d, np.arange(-40, 41) * (dl / 2))     # the dipole's line x = y = 0 exactly
+    CSX.AddExcitation("infDipole", 1, [0, 0, 1]).AddBox([0, 0, -dl / 2], [0, 0, dl / 2], priority=1)
+    s1 = np.array([-4.5] * 3) * dl / 2
+    s2 = np.array([4.5] * 3) * dl / 2
+    faces = {"xn": (s1, [s1[0], s2[1], s2[2]]), "xp": ([s2[0], s1[1], s1[2]], s2),
+             "yn": (s1, [s2[0], s1[1], s2[2]]), "yp": ([s1[0], s2[1], s1[2]], s2),
+             "zn": (s1, [s2[0], s2[1], s1[2]]), "zp": ([s1[0], s1[1], s2[2]], s2)}
+    for k, (a, b) in faces.items():
+        CSX.AddDump(f"Et_{k}", dump_type=0, dump_mode=0, file_type=1).AddBox(a, b)
+        CSX.AddDump(f"Ht_{k}", dump_type=1, dump_mode=0, file_type=1).AddBox(a, b)
+    coords = [[s1[0], 0, 0], [s2[0], 0, 0], [0, s1[1], 0], [0, s2[1], 0], [0, 0, s1[2]], [0, 0, s2[2]]]
+    for n, c in enumerate(coords):
+        CSX.AddProbe(f"et{n + 1}", 2).AddBox(c, c)
+        CSX.AddProbe(f"ht{n + 1}", 3).AddBox(c, c)
+    print("case fieldprobes: dipole, Mur, 12 TD dumps, 12 point probes, 1200 steps", flush=True)
+    assert FDTD.Run(str(tmp_path)) == 0
+    for n, k in enumerate(faces):
+        for fld, lim in (("E", 5e-3), ("H", 1e-7)):
+            path = str(tmp_path / f"{fld}t_{k}.h5")
+            lines = [hdf5_out.read_dataset(path, f"/Mesh/{a}")[0] for a in "xyz"]
+            idx = [int(np.argmin(np.abs(lines[a] - coords[n][a] * unit))) for a in range(3)]
+            probe = np.loadtxt(str(tmp_path / f"{fld.lower()}t{n + 1}"), comments="%")
+            names = sorted(nm for nm in _td_names(path))
+            assert len(names) == probe.shape[0]
+            vals = np.zeros((len(names), 3))
+            times = np.zeros(len(names))
+            for t, nm in enumerate(names):
+                data, attrs = hdf5_out.read_dataset(path, "/FieldData/TD/" + nm)
+                vals[t] = data[:, idx[2], idx[1], idx[0]]
+                times[t] = attrs["time"][0]
+            assert np.all(np.abs(times - probe[:, 0]) <= 1e-13)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                rel = np.abs((vals - probe[:, 1:4]) / vals)
+            assert not np.any(rel > 1e-7), (fld, k, np.nanmax(rel))
+            assert max(np.abs(vals[:, 0]).max(), np.abs(vals[:, 1]).max(), np.abs(vals[:, 2]).max()) >= lim
+
+
+def _td_names(path):
+    from openems_amd import hdf5_out
+    return hdf5_out.list_group(path, "/FieldData/TD")
