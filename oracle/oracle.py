@@ -498,6 +498,18 @@ class RefFieldDumpFD:
         F = curr if H else volt
         P = self.pos
         out = np.zeros((3,) + tuple(len(p) for p in P), np.float32)
+        if self.interp == 0:
+            # no interpolation: value / edge length per point, elementwise in double
+            idx = np.ix_(*P)
+            for n in range(3):
+                EL = m.edge_len(n, H)[P[n]]
+                shape = [1, 1, 1]
+                shape[n] = len(P[n])
+                d = np.broadcast_to(EL.reshape(shape), out.shape[1:])
+                v = np.asarray(F[n][idx], np.float64)
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    out[n] = np.where(d != 0, v / np.where(d != 0, d, 1.0), 0.0).astype(np.float32)
+            return out
         for n in range(3):
             nP, nPP = (n + 1) % 3, (n + 2) % 3
             EL = m.edge_len(n, H)

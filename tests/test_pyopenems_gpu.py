@@ -105,3 +105,50 @@ def test_field_probes_equal_field_dumps(gpu, tmp_path):
 def _td_names(path):
     from openems_amd import hdf5_out
     return hdf5_out.list_group(path, "/FieldData/TD")
+
+
+def test_engine_cross_check_cavity(gpu, tmp_path):
+    """The reference's engine cross-check (TESTSUITE/enginetests/cavity.m):
+    a 27 x 11 x 33 cavity with boundaries MUR, PML_8, PMC, PEC, PEC, PEC, an
+    eps_r = 3.66 brick, a soft [1 1 1] curve excitation, E and H point probes
+    and whole-mesh E / H time-domain HDF5 dumps without interpolation, 1000
+    steps.  Every dumped sample is bit-identical to the oracle engine's (the
+    reference requires identity between its engines)."""
+    from openems_amd import hdf5_out
+    from oracle.operator_ref import RefOperator
+    from oracle.oracle import RefFieldDumpTD, run_reference
+    a, b, d = 5e-2, 2e-2, 6e-2
+    FDTD = openEMS(NrTS=1000, EndCriteria=0)
+    FDTD.SetGaussExcite(4.5e9, 4.5e9)
+    FDTD.SetBoundaryCond(["MUR", "PML_8", "PMC", "PEC", "PEC", "PEC"])
+    CSX = ContinuousStructure()
+    FDTD.SetCSX(CSX)
+    mesh = CSX.GetGrid()
+    mesh.SetDeltaUnit(1)
+    x, y, z = np.linspace(0, a, 27), np.linspace(0, b, 11), np.linspace(0, d, 33)
+    for k, l in zip("xyz", (x, y, z)):
+        mesh.SetLines(k, l)
+    CSX.AddExcitation("excite1", 0, [1, 1, 1]).AddCurve([[x[17], x[18]], [y[6], y[7]], [z[21], z[22]]])
+    pt = [x[8], y[2], z[10]]
+    CSX.AddProbe("E_probe", 2).AddBox(pt, pt)
+    CSX.AddProbe("H_probe", 3).AddBox(pt, pt)
+    CSX.AddMaterial("RO4350B", epsilon=3.66).AddBox([x[2], y[2], z[2]], [x[4], y[3], z[5]], priority=100)
+    lo, hi = [x[0], y[0], z[0]], [x[-1], y[-1], z[-1]]
+    CSX.AddDump("Et", dump_type=0, dump_mode=0, file_type=1).AddBox(lo, hi)
+    CSX.AddDump("Ht", dump_type=1, dump_mode=0, file_type=1).AddBox(lo, hi)
+    print("case enginetests cavity 27x11x33, 1000 steps", flush=True)
+    scene = FDTD.scene()
+    assert FDTD.Run(str(tmp_path)) == 0
+    ref = RefOperator(scene).build()
+    assert ref.vexc.dir.size > 0
+    dumps = [RefFieldDumpTD(ref, k, lo, hi, interp=0) for k in (0, 1)]
+    run_reference(ref, 1000, dumps=dumps)
+    for name, rd in zip(("Et", "Ht"), dumps):
+        path = str(tmp_path / f"{name}.h5")
+        names = hdf5_out.list_group(path, "/FieldData/TD")
+        assert [int(nm) for nm in names] == [sm[0] for sm in rd.samples]
+        assert len(names) > 100
+        for nm, (ts, t, want) in zip(names, rd.samples):
+            got, attrs = hdf5_out.read_dataset(path, "/FieldData/TD/" + nm)
+            assert np.array_equal(got, np.transpose(want, (0, 3, 2, 1))), (name, nm)
+        assert np.abs(rd.samples[-1][2]).max() > 0
