@@ -152,3 +152,81 @@ def test_engine_cross_check_cavity(gpu, tmp_path):
             got, attrs = hdf5_out.read_dataset(path, "/FieldData/TD/" + nm)
             assert np.array_equal(got, np.transpose(want, (0, 3, 2, 1))), (name, nm)
         assert np.abs(rd.samples[-1][2]).max() > 0
+
+
+def _fft_time2freq(t, val):
+    """Single-sided spectrum of a sampled signal as the reference's test
+    helper computes it (matlab/FFT_time2freq.m): zero-padded FFT to the next
+    power of two, times the sample spacing, doubled, phase referred to t[0]."""
+    dt = t[1] - t[0]
+    nfft = 1 << int(np.ceil(np.log2(len(val))))
+    v = np.fft.fft(val, nfft) * dt
+    f = 1 / (2 * dt) * np.linspace(0, 1, nfft // 2 + 1)
+    v = 2 * v[:nfft // 2 + 1]
+    return f, v * np.exp(-1j * 2 * np.pi * f * t[0])
+
+
+def _check_frequency(f, val, f_upper, f_lower, rel_amplitude, kind):
+    """TESTSUITE/helperscripts/check_frequency.m: the largest value between
+    each pair of limits (nearest samples) is at least (inside) or at most
+    (outside) rel_amplitude times the overall largest value."""
+    m = val.max()
+    for lo, hi in zip(f_lower, f_upper):
+        i1, i2 = int(np.argmin(np.abs(f - lo))), int(np.argmin(np.abs(f - hi)))
+        w = val[i1:i2 + 1].max()
+        if kind == "inside" and w < m * rel_amplitude:
+            return False
+        if kind == "outside" and w > m * rel_amplitude:
+            return False
+    return True
+
+
+def test_cavity_resonances_against_closed_form(gpu, tmp_path):
+    """The reference's physics test TESTSUITE/combinedtests/cavity.m on the
+    engine: a 5 x 2 x 6 cm PEC cavity excited by a [1 1 1] curve source,
+    20000 steps; the spectra of three voltage probes peak at the analytic
+    TE101/102/201/202 resonances within +-0.13 % (at 60 % of the maximum) and
+    TM110/111 within -0.25 % / +0 % (27 %), and stay below 17 % of the maximum
+    between them (2 % guard bands)."""
+    c0 = 299792458.0
+    a, b, d = 5e-2, 2e-2, 6e-2
+    f_start, f_stop = 1e9, 10e9
+    FDTD = openEMS(NrTS=20000, EndCriteria=1e-6)
+    FDTD.SetGaussExcite((f_stop - f_start) / 2, (f_stop - f_start) / 2)
+    FDTD.SetBoundaryCond([0] * 6)
+    CSX = ContinuousStructure()
+    FDTD.SetCSX(CSX)
+    mesh = CSX.GetGrid()
+    mesh.SetDeltaUnit(1)
+    x, y, z = np.linspace(0, a, 26), np.linspace(0, b, 11), np.linspace(0, d, 32)
+    for k, l in zip("xyz", (x, y, z)):
+        mesh.SetLines(k, l)
+    CSX.AddExcitation("excite1", 0, [1, 1, 1]).AddCurve([[x[16], x[17]], [y[6], y[7]], [z[20], z[21]]])
+    CSX.AddProbe("ut1x", 0).AddBox([x[5], y[4], z[5]], [x[6], y[4], z[5]])
+    CSX.AddProbe("ut1y", 0).AddBox([x[5], y[4], z[5]], [x[5], y[5], z[5]])
+    CSX.AddProbe("ut1z", 0).AddBox([x[12], y[4], z[5]], [x[12], y[4], z[6]])
+    print("case combinedtests cavity, 20000 steps", flush=True)
+    assert FDTD.Run(str(tmp_path)) == 0
+    spectra = []
+    for name in ("ut1x", "ut1y", "ut1z"):
+        td = np.loadtxt(str(tmp_path / name), comments="%")
+        t, v = td[:, 0], td[:, 1]
+        i0 = int(np.argmin(np.abs(t - 7e-10)))
+        f, u = _fft_time2freq(t[i0:], v[i0:])
+        spectra.append(u)
+    i1, i2 = int(np.argmin(np.abs(f - f_start))), int(np.argmin(np.abs(f - f_stop)))
+    f = f[i1:i2 + 1]
+    uy, uz = np.abs(spectra[1][i1:i2 + 1]), np.abs(spectra[2][i1:i2 + 1])
+    k = lambda m, n, l: np.sqrt((m * np.pi / a) ** 2 + (n * np.pi / b) ** 2 + (l * np.pi / d) ** 2)
+    f_TE = np.array([c0 / (2 * np.pi) * k(*mnl) for mnl in ((1, 0, 1), (1, 0, 2), (2, 0, 1), (2, 0, 2))])
+    f_TM = np.array([c0 / (2 * np.pi) * k(*mnl) for mnl in ((1, 1, 0), (1, 1, 1))])
+
+    def outer(fs):
+        tmp = np.concatenate([[f_start], fs, [f_stop]])
+        return tmp[:-1] * (1 + 0.02), tmp[1:] * (1 - 0.02)
+    o1, o2 = outer(f_TE)
+    o1m, o2m = outer(f_TM)
+    assert _check_frequency(f, uy, f_TE * (1 + 1.3e-3), f_TE * (1 - 1.3e-3), 0.6, "inside")
+    assert _check_frequency(f, uz, f_TM * (1 + 0), f_TM * (1 - 2.5e-3), 0.27, "inside")
+    assert _check_frequency(f, uy, o2, o1, 0.17, "outside")
+    assert _check_frequency(f, uz, o2m, o1m, 0.17, "outside")
