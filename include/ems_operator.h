@@ -1,7 +1,7 @@
 /*
  * ems_operator.h — host-side EC-FDTD operator builder (C ABI).
  *
- * Builds, from a CSXCAD-free scene of axis-aligned boxes, exactly the arrays
+ * Builds, from a CSXCAD-free scene of axis-aligned boxes and wires, exactly the arrays
  * the reference Operator hands to its engines:
  *   - the float32 update coefficients vv, vi, ii, iv in the reference's
  *     N-I-J-K layout (component, x, y, z; z fastest)
@@ -16,7 +16,8 @@
  *     [Operator::Calc_LumpedElements, FDTD/operator.cpp:1586-1763]
  *
  * Replaces Operator::SetGeometryCSX + Operator::CalcECOperator
- * (FDTD/operator.cpp:822-837, 986-1097) for box-only scenes.
+ * (FDTD/operator.cpp:822-837, 986-1097) for scenes of boxes, material grids
+ * and wires (polyline curves).
  * All functions return 0 on success and a negative EMSOP_E_* code on error;
  * emsop_last_error() describes the last failure of the calling thread.
  */
