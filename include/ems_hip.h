@@ -255,11 +255,13 @@ int ems_probe_results(ems_engine* h, int id, unsigned* count, unsigned* ts, doub
  * (snapped by the caller, on the dual mesh where the reference uses it);
  * edge_len[3] are the primary (E) or dual (H) edge lengths per axis.  The
  * nfreq complex<float> factors per sample are supplied up front:
- * factors[nsamples][nfreq][2] for samples taken at ts = k*fd_interval
- * (k = 0..nsamples-1), with the half-step time shift of H already in them. */
+ * factors[nsamples][nfreq][2] for samples taken at ts = first_ts + k*fd_interval
+ * (k = 0..nsamples-1; first_ts a multiple of fd_interval, the first sampling
+ * step inside the dump's start/stop window), with the half-step time shift
+ * of H already in them. */
 int ems_add_fd_dump(ems_engine* h, int field, int interp, const unsigned start[3], const unsigned stop[3],
-                    unsigned fd_interval, unsigned nfreq, unsigned nsamples, const float* factors,
-                    const double* const edge_len[3], int* id);
+                    unsigned fd_interval, unsigned first_ts, unsigned nfreq, unsigned nsamples,
+                    const float* factors, const double* const edge_len[3], int* id);
 /* out: [nfreq][3][nx][ny][nz][2] float32 (re, im), dump-box N-I-J-K order */
 int ems_fd_dump_results(ems_engine* h, int id, float* out);
 

@@ -110,7 +110,8 @@ ENGINE_API = [
     ("ems_set_curr", C.c_int, [vp, fp]),
     ("ems_add_probe", C.c_int, [vp, C.POINTER(ProbeDesc), ip]),
     ("ems_probe_results", C.c_int, [vp, C.c_int, up, up, dp]),
-    ("ems_add_fd_dump", C.c_int, [vp, C.c_int, C.c_int, up, up, C.c_uint, C.c_uint, C.c_uint, fp, C.POINTER(dp), ip]),
+    ("ems_add_fd_dump", C.c_int, [vp, C.c_int, C.c_int, up, up, C.c_uint, C.c_uint, C.c_uint, C.c_uint, fp,
+                                  C.POINTER(dp), ip]),
     ("ems_add_td_dump", C.c_int, [vp, C.c_int, C.c_int, up, up, C.c_uint, C.c_uint, C.c_uint, C.c_uint,
                                   C.POINTER(dp), ip]),
     ("ems_td_dump_results", C.c_int, [vp, C.c_int, up, up, fp]),

@@ -2776,16 +2776,18 @@ static int dump_common(ems_engine* h, int field, int interp, const unsigned star
 extern "C" {
 
 int ems_add_fd_dump(ems_engine* h, int field, int interp, const unsigned start[3], const unsigned stop[3],
-                    unsigned fd_interval, unsigned nfreq, unsigned nsamples, const float* factors,
+                    unsigned fd_interval, unsigned first_ts, unsigned nfreq, unsigned nsamples, const float* factors,
                     const double* const edge_len[3], int* id)
 {
     if (!h || !factors) return fail(EMS_E_ARG, "null argument");
     if (fd_interval == 0 || nfreq == 0 || nsamples == 0) return fail(EMS_E_ARG, "empty frequency-domain dump");
+    if (first_ts % fd_interval) return fail(EMS_E_ARG, "first dump sample not on the sampling interval");
     DumpDev d{};
     int rc;
     if ((rc = dump_common(h, field, interp, start, stop, edge_len, d))) return rc;
     drop_graphs(h);     // the dump changes the launch pattern of every chunk
     d.fd_interval = fd_interval;
+    d.first = first_ts;
     d.nfreq = nfreq;
     d.nsamples = nsamples;
     float* df;

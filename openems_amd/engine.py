@@ -278,7 +278,7 @@ class EngineHIP:
             check(L.ems_probe_results(self.h, pid, C.byref(cnt), ptr(ts, C.c_uint), ptr(vals, C.c_double)))
         return ts, vals
 
-    def add_fd_dump(self, start, stop, fd_interval, factors, field=0, interp=1):
+    def add_fd_dump(self, start, stop, fd_interval, factors, field=0, interp=1, first_ts=0):
         """Frequency-domain dump of the E (field 0) or H (field 1) field over
         the mesh-index box start..stop, interpolation 0 none, 1 node, 2 cell."""
         L = lib()
@@ -290,7 +290,7 @@ class EngineHIP:
         st = (C.c_uint * 3)(*start)
         sp = (C.c_uint * 3)(*stop)
         did = C.c_int()
-        check(L.ems_add_fd_dump(self.h, int(field), int(interp), st, sp, int(fd_interval), nf, ns,
+        check(L.ems_add_fd_dump(self.h, int(field), int(interp), st, sp, int(fd_interval), int(first_ts), nf, ns,
                                 ptr(factors, C.c_float), elp, C.byref(did)))
         shape = (nf, 3) + tuple(int(stop[n]) - int(start[n]) + 1 for n in range(3))
         return did.value, shape

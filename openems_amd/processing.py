@@ -44,6 +44,21 @@ def _g(v, prec=6):
     return "%.*g" % (prec, v)
 
 
+def process_window(spec, dT):
+    """Processing::SetProcessStartStopTime (Common/processing.cpp:211-226):
+    steps [floor(start/dT), ceil(stop/dT)] from spec["start_time"] /
+    spec["stop_time"] (seconds, 0 = open); an empty window means the whole run."""
+    t0, t1 = float(spec.get("start_time", 0.0)), float(spec.get("stop_time", 0.0))
+    a, b = 0, 0xFFFFFFFF
+    if t0 > 0:
+        a = int(math.floor(t0 / dT))
+    if t1 > 0:
+        b = int(math.ceil(t1 / dT))
+    if b <= a:
+        a, b = 0, 0xFFFFFFFF
+    return a, b
+
+
 def nyquist_num(f, dT):
     """CalcNyquistNum (tools/useful.cpp:30-36)."""
     if f == 0:
@@ -74,7 +89,7 @@ class Probe:
             if self.fd_interval > nts:
                 self.fd_interval = nts
             self.freqs.append(float(f))
-        self.start_ts, self.stop_ts = 0, 0xFFFFFFFF
+        self.start_ts, self.stop_ts = process_window(spec, op.dT)
         self.enabled = True
         self._terms()
         self.steps = [ts for ts in range(nrts + 1) if self.due(ts)]
@@ -314,7 +329,14 @@ class FieldDumpFD:
             if self.fd_interval > nts:
                 self.fd_interval = nts
             self.freqs.append(float(f))
-        self.nsamples = nrts // self.fd_interval + 1 if self.fd_interval else 0
+        # samples at the multiples of fd_interval inside the window (Processing::CheckTimestep)
+        self.start_ts, self.stop_ts = process_window(spec, op.dT)
+        self.first, self.nsamples = 0, 0
+        if self.fd_interval:
+            iv = self.fd_interval
+            self.first = (self.start_ts + iv - 1) // iv * iv
+            last = min(nrts, self.stop_ts)
+            self.nsamples = (last - self.first) // iv + 1 if last >= self.first else 0
         self.did = None
 
     def factors(self):
@@ -324,7 +346,7 @@ class FieldDumpFD:
         scale = np.float32(dT * self.fd_interval)
         for k in range(self.nsamples):
             # GetTime(dualTime) = (numTS + 0.5 dualTime) dT (engine_interface_fdtd.h:52)
-            T = (k * self.fd_interval + (0.5 if self.dual_time else 0.0)) * dT
+            T = (self.first + k * self.fd_interval + (0.5 if self.dual_time else 0.0)) * dT
             for j, f in enumerate(self.freqs):
                 th = np.float32(((-2.0 * math.pi) * f) * T)
                 c = np.float32(_libm.cosf(th))
@@ -334,10 +356,10 @@ class FieldDumpFD:
         return out
 
     def register(self, eng):
-        if not self.fd_interval:
+        if not self.fd_interval or not self.nsamples:
             return
         self.did, self.shape = eng.add_fd_dump(self.start, self.stop, self.fd_interval, self.factors(),
-                                               field=self.field, interp=self.interp)
+                                               field=self.field, interp=self.interp, first_ts=self.first)
 
     def collect(self, eng):
         if self.did is None:
@@ -367,15 +389,7 @@ class FieldDumpTD:
         self.enabled = fd.enabled
         self.start, self.stop = fd.start, fd.stop
         self.interval = op.nyquist // int(oversampling)
-        dT = op.dT
-        self.start_ts, self.stop_ts = 0, 0xFFFFFFFF
-        t0, t1 = float(spec.get("start_time", 0.0)), float(spec.get("stop_time", 0.0))
-        if t0 > 0:
-            self.start_ts = int(math.floor(t0 / dT))
-        if t1 > 0:
-            self.stop_ts = int(math.ceil(t1 / dT))
-        if self.stop_ts <= self.start_ts:
-            self.start_ts, self.stop_ts = 0, 0xFFFFFFFF
+        self.start_ts, self.stop_ts = process_window(spec, op.dT)
         self.steps = [ts for ts in range(nrts + 1) if self.due(ts)]
         self.did = None
 

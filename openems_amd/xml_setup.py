@@ -258,7 +258,9 @@ def read_xml(source):
             for nb, (prio, (p1, p2)) in enumerate(boxes):
                 probes.append({"type": ptype, "name": name if len(boxes) == 1 else f"{name}_{nb}",
                                "start": list(p1), "stop": list(p2), "weight": float(prop.get("Weight", 1.0)),
-                               "freqs": _fd_samples(prop), "norm_dir": _int(prop, "NormDir", -1)})
+                               "freqs": _fd_samples(prop), "norm_dir": _int(prop, "NormDir", -1),
+                               "start_time": float(prop.get("StartTime", 0.0)),
+                               "stop_time": float(prop.get("StopTime", 0.0))})
         elif prop.tag == "DumpBox":
             dtype = _int(prop, "DumpType", 0)
             mode = _int(prop, "DumpMode", 0)
@@ -269,9 +271,9 @@ def read_xml(source):
                 d = {"name": name if len(boxes) == 1 else f"{name}_{nb}", "start": list(p1), "stop": list(p2),
                      "field": "E" if dtype in (0, 10) else "H", "interp": mode,
                      "file_type": _int(prop, "FileType", 0)}
+                d.update(start_time=float(prop.get("StartTime", 0.0)), stop_time=float(prop.get("StopTime", 0.0)))
                 if dtype < 10:
-                    d.update(domain="td", start_time=float(prop.get("StartTime", 0.0)),
-                             stop_time=float(prop.get("StopTime", 0.0)))
+                    d["domain"] = "td"
                 else:
                     d["freqs"] = _fd_samples(prop)
                 dumps.append(d)
@@ -389,6 +391,9 @@ def write_xml(scene, path=None):
     for k, p in enumerate(scene.get("probes", [])):
         el = prop("ProbeBox", p.get("name") or f"probe{k}", Type=str(int(p["type"])),
                   Weight=_fmt(p.get("weight", 1.0)), NormDir=str(int(p.get("norm_dir", -1))))
+        for key, attr in (("start_time", "StartTime"), ("stop_time", "StopTime")):
+            if p.get(key):
+                el.set(attr, repr(float(p[key])))
         _box(ET.SubElement(el, "Primitives"), 0, p["start"], p["stop"])
         if p.get("freqs"):
             ET.SubElement(el, "FD_Samples").text = ",".join(repr(float(f)) for f in p["freqs"])
@@ -396,10 +401,9 @@ def write_xml(scene, path=None):
         fld = {"E": 0, "H": 1, 0: 0, 1: 1}[d.get("field", "E")] + (0 if d.get("domain") == "td" else 10)
         el = prop("DumpBox", d.get("name") or f"dump{k}", DumpType=str(fld), DumpMode=str(int(d.get("interp", 1))),
                   FileType=str(int(d.get("file_type", 0))))
-        if d.get("domain") == "td":
-            for key, attr in (("start_time", "StartTime"), ("stop_time", "StopTime")):
-                if d.get(key):
-                    el.set(attr, repr(float(d[key])))
+        for key, attr in (("start_time", "StartTime"), ("stop_time", "StopTime")):
+            if d.get(key):
+                el.set(attr, repr(float(d[key])))
         _box(ET.SubElement(el, "Primitives"), 0, d["start"], d["stop"])
         if d.get("freqs"):
             ET.SubElement(el, "FD_Samples").text = ",".join(repr(float(f)) for f in d["freqs"])

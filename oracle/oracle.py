@@ -261,11 +261,23 @@ class RefEngine:
 # ----------------------------------------------------------------------------
 # probes
 # ----------------------------------------------------------------------------
+def _window(dT, start_time, stop_time):
+    """Processing::SetProcessStartStopTime (processing.cpp:211-226)."""
+    a, b = 0, 0xFFFFFFFF
+    if start_time > 0:
+        a = int(math.floor(start_time / dT))
+    if stop_time > 0:
+        b = int(math.ceil(stop_time / dT))
+    if b <= a:
+        a, b = 0, 0xFFFFFFFF
+    return a, b
+
+
 class RefProbe:
     """ProcessIntegral-derived probe (voltage=0, current=1, E-point=2, H-point=3)."""
 
     def __init__(self, op: RefOperator, ptype, start, stop, weight=1.0, freqs=(), norm_dir=-1,
-                 oversampling=4, name=""):
+                 oversampling=4, name="", start_time=0.0, stop_time=0.0):
         self.op = op
         self.ptype = ptype
         self.weight = float(weight)
@@ -287,6 +299,7 @@ class RefProbe:
                 self.fd_interval = nts
             self.freqs.append(float(f))
         self.enabled = True
+        self.start_ts, self.stop_ts = _window(op.dT, start_time, stop_time)
         if ptype == 0:
             # ProcessVoltage::DefineStartStopCoord: SnapToMesh on the primary mesh
             self.start, self.start_in = m.snap(start, False)
@@ -418,7 +431,8 @@ class RefProbe:
         return acc
 
     def check(self, ts):
-        if not self.enabled:
+        """Processing::CheckTimestep (processing.cpp:82-106)."""
+        if not self.enabled or ts < self.start_ts or ts > self.stop_ts:
             return False
         if ts % self.interval == 0:
             return True
@@ -460,8 +474,9 @@ class RefFieldDumpFD:
     the dual time and, uninterpolated, the dual mesh; cell interpolation the
     dual mesh; the box snaps with SnapMethod 1 (operator.cpp:323-337)."""
 
-    def __init__(self, op: RefOperator, dump_type, start, stop, freqs, interp=1):
+    def __init__(self, op: RefOperator, dump_type, start, stop, freqs, interp=1, start_time=0.0, stop_time=0.0):
         self.op = op
+        self.start_ts, self.stop_ts = _window(op.dT, start_time, stop_time)
         self.dump_type = dump_type
         self.interp = interp
         m = op.mesh
@@ -582,7 +597,7 @@ class RefFieldDumpFD:
         return out
 
     def due(self, ts):
-        return bool(self.fd_interval) and ts % self.fd_interval == 0
+        return bool(self.fd_interval) and self.start_ts <= ts <= self.stop_ts and ts % self.fd_interval == 0
 
     def process(self, ts, volt, curr):
         if not self.due(ts):
@@ -610,15 +625,8 @@ class RefFieldDumpTD(RefFieldDumpFD):
 
     def __init__(self, op: RefOperator, dump_type, start, stop, interp=1, oversampling=4,
                  start_time=0.0, stop_time=0.0):
-        super().__init__(op, dump_type, start, stop, [], interp=interp)
+        super().__init__(op, dump_type, start, stop, [], interp=interp, start_time=start_time, stop_time=stop_time)
         self.interval = op.nyquist // int(oversampling)
-        self.start_ts, self.stop_ts = 0, 0xFFFFFFFF
-        if start_time > 0:
-            self.start_ts = int(math.floor(start_time / op.dT))
-        if stop_time > 0:
-            self.stop_ts = int(math.ceil(stop_time / op.dT))
-        if self.stop_ts <= self.start_ts:
-            self.start_ts, self.stop_ts = 0, 0xFFFFFFFF
         self.samples = []       # (ts, time, float32 [3][nx][ny][nz])
 
     def due(self, ts):

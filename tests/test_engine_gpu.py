@@ -596,3 +596,36 @@ def test_custom_excitation_runs_on_the_engine(gpu):
     assert np.array_equal(v, rv) and np.array_equal(c, rc)
     eng.close()
     ref.close()
+
+
+def test_probe_and_dump_windows(gpu):
+    """Probes and a frequency-domain dump restricted to a start/stop window
+    (Processing::SetProcessStartStopTime): rows and running DFTs equal to the
+    oracle's windowed processing."""
+    scene = scenes.upml_bricks(40, 400, nbricks=4)
+    print("case upml_bricks_40 400 steps, windowed probes and dump", flush=True)
+    ref = _ref_from_scene(scene)
+    dT = ref.dT
+    for p in scene["probes"]:
+        p.update(start_time=57.3 * dT, stop_time=290.2 * dT, freqs=[5e9, 10e9])
+    d = scene["dumps"][0]
+    d.update(start_time=80.0 * dT, stop_time=333.0 * dT)
+    sim = Simulation(scene)
+    sim.run()
+    res = sim.results()
+    rp = [RefProbe(ref, p["type"], p["start"], p["stop"], p["weight"], p["freqs"], p["norm_dir"],
+                   scene["oversampling"], p["name"], start_time=p["start_time"], stop_time=p["stop_time"])
+          for p in scene["probes"]]
+    rd = RefFieldDumpFD(ref, 0, d["start"], d["stop"], d["freqs"], start_time=d["start_time"],
+                        stop_time=d["stop_time"])
+    run_reference(ref, 400, probes=rp, dumps=[rd])
+    for p in rp:
+        got = res["probes"][p.name]
+        want = p.td_array()
+        assert want.shape[0] > 5 and want[0, 0] >= 57 * dT and want[-1, 0] <= 291 * dT
+        assert np.array_equal(got["td"], want)
+        np.testing.assert_allclose(got["fd"], p.fd, rtol=1e-12, atol=1e-30)
+    fd = res["dumps"][d["name"]]["fd"]
+    want = rd.fd_re + 1j * rd.fd_im
+    assert np.abs(want).max() > 0 and rel_l2(fd, want) <= 1e-5
+    sim.close()

@@ -300,3 +300,31 @@ def test_custom_excitation_parse_error(native):
     sc["signal"] = {"type": "custom", "function": "sin(2*pi*t", "f0": 5e9}
     with pytest.raises(NativeError, match="expected at position"):
         Operator(sc)
+
+
+def test_processing_windows_match_oracle(native):
+    """Processing::SetProcessStartStopTime: probes sample only inside
+    [floor(start/dT), ceil(stop/dT)], a frequency-domain dump's first sample
+    is the first multiple of its interval in the window; an inverted window
+    means the whole run.  Product and oracle agree step for step."""
+    from openems_amd.processing import FieldDumpFD, Probe
+    from oracle.oracle import RefFieldDumpFD, RefProbe
+    scene = scenes.upml_bricks(40, 400, nbricks=4)
+    op = Operator(scene, nthreads=2)
+    ref = RefOperator(scene).build()
+    dT = op.dT
+    p0 = scene["probes"][0]
+    for t0, t1 in ((37.2 * dT, 211.5 * dT), (0.0, 100.1 * dT), (150 * dT, 0.0), (300 * dT, 100 * dT)):
+        spec = dict(p0, start_time=t0, stop_time=t1, freqs=[5e9])
+        p = Probe(op, spec, 400)
+        r = RefProbe(ref, p0["type"], p0["start"], p0["stop"], p0["weight"], [5e9], p0["norm_dir"],
+                     start_time=t0, stop_time=t1)
+        assert p.steps == [ts for ts in range(401) if r.check(ts)]
+        d = scene["dumps"][0]
+        fd = FieldDumpFD(op, dict(d, start_time=t0, stop_time=t1), 400)
+        rd = RefFieldDumpFD(ref, 0, d["start"], d["stop"], d["freqs"], start_time=t0, stop_time=t1)
+        due = [ts for ts in range(401) if rd.due(ts)]
+        assert [fd.first + k * fd.fd_interval for k in range(fd.nsamples)] == due
+    # closed form of the window: floor / ceil, inverted -> whole run
+    r = RefProbe(ref, 2, p0["start"], p0["start"], 1.0, [], -1, start_time=10.5 * dT, stop_time=20.2 * dT)
+    assert (r.start_ts, r.stop_ts) == (10, 21)

@@ -91,7 +91,7 @@ def test_structure_subset():
     assert (ut["name"], ut["type"], ut["weight"], ut["freqs"]) == ("port_ut1", 0, -1.0, [])
     assert (it["name"], it["type"], it["norm_dir"], it["freqs"]) == ("port_it1", 1, 2, [1e9, 2e9])
     assert sc["dumps"] == [{"name": "Ef", "start": [0.0, 0.0, 4.0], "stop": [20.0, 20.0, 4.0], "freqs": [3e9],
-                            "field": "E", "interp": 2, "file_type": 1},
+                            "field": "E", "interp": 2, "file_type": 1, "start_time": 0.0, "stop_time": 0.0},
                            {"name": "Et", "start": [0.0, 0.0, 4.0], "stop": [20.0, 20.0, 4.0], "field": "E",
                             "interp": 0, "file_type": 0, "domain": "td", "start_time": 0.0, "stop_time": 0.0}]
     assert sc["skipped"] == []
