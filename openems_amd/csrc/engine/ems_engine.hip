@@ -2883,14 +2883,6 @@ static bool probe_due(const ems_engine* h, unsigned ts)
     return false;
 }
 
-static bool dump_due(const ems_engine* h, unsigned ts)
-{
-    unsigned k;
-    for (const auto& d : h->dumps)
-        if (dump_sample(d, ts, &k)) return true;
-    return false;
-}
-
 static int launch_samples(ems_engine* h)
 {
     // ProcessingArray::Process after the step: probes and dumps due at numTS
