@@ -23,6 +23,8 @@
  *                                                                         FDTD/engine.h:54-101
  *   Engine_HIP::CalcFastEnergy          Engine_Interface_FDTD::CalcFastEnergy
  *                                                                         FDTD/engine_interface_fdtd.cpp:302-347
+ *   Engine_Interface_HIP probes         Engine_Interface_FDTD::CalcVoltageIntegral / GetRawField
+ *                                                                         FDTD/engine_interface_fdtd.cpp:134-268
  *
  * The engine runs every extension of the operator on the device (excitation,
  * UPML, Mur, Lorentz / Drude / Debye, lumped RLC), so no host extension hook
@@ -356,6 +358,84 @@ private:
 };
 
 inline Engine_HIP* Operator_HIP::CreateEngine(int device) const { return Engine_HIP::New(this, device); }
+
+// Probes reduced on the device (Engine_Interface_FDTD, FDTD/engine_interface_fdtd.cpp):
+// the voltage line integral of CalcVoltageIntegral (:206-232) and the point
+// fields of GetRawField / GetRawDualField (:134-146, :263-268), sampled after
+// every step that is a multiple of `interval` (and at step 0).  Register the
+// probes before the first IterateTS.
+class Engine_Interface_HIP {
+public:
+    Engine_Interface_HIP(Engine_HIP* eng, const Operator_HIP* op) : eng_(eng), op_(op) {}
+
+    // line from start to stop (mesh indices, one axis differing; otherwise the
+    // probe records zeros as the reference does); returns the probe id
+    int AddVoltageProbe(const unsigned start[3], const unsigned stop[3], unsigned interval, unsigned capacity)
+    {
+        std::vector<unsigned> comp, pos[3];
+        std::vector<int> sign;
+        const int axes = (start[0] != stop[0]) + (start[1] != stop[1]) + (start[2] != stop[2]);
+        for (int n = 0; n < 3 && axes == 1; ++n) {
+            if (start[n] == stop[n]) continue;
+            const bool up = start[n] < stop[n];
+            unsigned p[3] = {up ? start[0] : stop[0], up ? start[1] : stop[1], up ? start[2] : stop[2]};
+            for (; p[n] < (up ? stop[n] : start[n]); ++p[n]) {
+                comp.push_back(n);
+                for (int a = 0; a < 3; ++a) pos[a].push_back(p[a]);
+                sign.push_back(up ? 1 : -1);
+            }
+        }
+        std::vector<unsigned> P;
+        for (int a = 0; a < 3; ++a) P.insert(P.end(), pos[a].begin(), pos[a].end());
+        return add(EMS_PROBE_VOLTAGE, comp, P, sign, std::vector<double>(), interval, capacity);
+    }
+    // E (h_field = false) or H field at a mesh node: value / (dual) edge length
+    int AddFieldProbe(bool h_field, const unsigned node[3], unsigned interval, unsigned capacity)
+    {
+        std::vector<unsigned> comp = {0, 1, 2}, P;
+        for (int a = 0; a < 3; ++a)
+            for (int n = 0; n < 3; ++n) P.push_back(node[a]);
+        std::vector<double> div;
+        for (int n = 0; n < 3; ++n) div.push_back(op_->GetEdgeLength(n, node, h_field));
+        return add(h_field ? EMS_PROBE_HFIELD : EMS_PROBE_EFIELD, comp, P, std::vector<int>(3, 1), div, interval,
+                   capacity);
+    }
+    // samples so far: steps and values[count][3] (voltage probes use slot 0)
+    std::vector<double> GetSamples(int id, std::vector<unsigned>* steps = nullptr)
+    {
+        unsigned count = 0;
+        check_engine(ems_probe_results(eng_->handle(), id, &count, nullptr, nullptr));
+        std::vector<unsigned> ts(count);
+        std::vector<double> v(3 * (size_t)count);
+        if (count) check_engine(ems_probe_results(eng_->handle(), id, &count, ts.data(), v.data()));
+        if (steps) *steps = ts;
+        return v;
+    }
+
+private:
+    int add(int type, const std::vector<unsigned>& comp, const std::vector<unsigned>& pos, const std::vector<int>& sign,
+            const std::vector<double>& div, unsigned interval, unsigned capacity)
+    {
+        ems_probe_desc d{};
+        d.type = type;
+        d.nelem = (unsigned)comp.size();
+        d.comp = comp.data();
+        d.pos = pos.data();
+        d.sign = sign.data();
+        d.divisor = div.empty() ? nullptr : div.data();
+        d.interval = interval;
+        d.fd_interval = 0;
+        d.start_ts = 0;
+        d.stop_ts = 0xFFFFFFFFu;
+        d.capacity = capacity;
+        int id = -1;
+        check_engine(ems_add_probe(eng_->handle(), &d, &id));
+        return id;
+    }
+
+    Engine_HIP* eng_;
+    const Operator_HIP* op_;
+};
 
 }  // namespace openems_hip
 

@@ -61,6 +61,13 @@ int main()
     }
     try {
         Engine_HIP* eng = op->CreateEngine();
+        eng->Init();
+        Engine_Interface_HIP iface(eng, op);
+        const unsigned every = op->GetNyquistNum() / 4;
+        const unsigned ua[3] = {14, 14, 10}, ub[3] = {14, 14, 12}, en[3] = {20, 11, 12}, hn[3] = {10, 12, 14};
+        const int pu = iface.AddVoltageProbe(ua, ub, every, 512);
+        const int pe = iface.AddFieldProbe(false, en, every, 512);
+        const int ph = iface.AddFieldProbe(true, hn, every, 512);
         eng->IterateTS(120);
         std::printf("{");
         std::printf("\"dT\": \"%a\", ", op->GetTimestep());
@@ -79,6 +86,16 @@ int main()
         hexf("v_d", eng->GetVolt(0, 5, 5, 5));
         hexf("i_b", eng->GetCurr(1, q));
         hexf("v_e", eng->GetVolt(1, 21, 12, 13));
+        std::vector<unsigned> ts;
+        const std::vector<double> su = iface.GetSamples(pu, &ts), se = iface.GetSamples(pe), sh = iface.GetSamples(ph);
+        std::printf("\"probe_every\": %u, \"probe_count\": %u, ", every, (unsigned)ts.size());
+        std::printf("\"probe_u\": [");
+        for (size_t k = 0; k < ts.size(); ++k) std::printf("%s\"%a\"", k ? ", " : "", su[3 * k]);
+        std::printf("], \"probe_e\": [");
+        for (size_t k = 0; k < 3 * ts.size(); ++k) std::printf("%s\"%a\"", k ? ", " : "", se[k]);
+        std::printf("], \"probe_h\": [");
+        for (size_t k = 0; k < 3 * ts.size(); ++k) std::printf("%s\"%a\"", k ? ", " : "", sh[k]);
+        std::printf("], ");
         std::printf("\"energy_b\": \"%a\"}\n", eng->CalcFastEnergy());
         delete eng;
     } catch (const Error& e) {

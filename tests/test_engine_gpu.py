@@ -478,8 +478,15 @@ def test_cpp_host_classes_match_python(gpu):
     assert out.returncode == 0, out.stderr
     got = json.loads(out.stdout)
     fx = lambda s: float.fromhex(s)
+    from openems_amd.processing import Probe
     op = Operator(_mirror_scene())
     eng = EngineHIP(op)
+    specs = [{"type": 0, "name": "u", "start": (14, 14, 10), "stop": (14, 14, 12)},
+             {"type": 2, "name": "e", "start": (20, 11, 12), "stop": (20, 11, 12)},
+             {"type": 3, "name": "h", "start": (10.5, 12.5, 14.5), "stop": (10.5, 12.5, 14.5)}]
+    probes = [Probe(op, sp, 200) for sp in specs]
+    for p in probes:
+        p.register(eng)
     eng.IterateTS(120)
     assert fx(got["dT"]) == op.dT
     assert got["numTS_a"] == eng.numTS == 120
@@ -502,6 +509,14 @@ def test_cpp_host_classes_match_python(gpu):
     assert np.float32(fx(got["i_b"])) == c[1, 7, 9, 11]
     assert np.float32(fx(got["v_e"])) == v[1, 21, 12, 13]
     assert fx(got["energy_b"]) == eng.energy()
+    # Engine_Interface_HIP probes against the Python host's
+    assert got["probe_every"] == probes[0].interval
+    tds = [p.collect(eng)[0] for p in probes]
+    assert got["probe_count"] == tds[0].shape[0] > 10
+    assert [fx(x) for x in got["probe_u"]] == list(tds[0][:, 1])
+    assert [fx(x) for x in got["probe_e"]] == list(tds[1][:, 1:].ravel())
+    assert [fx(x) for x in got["probe_h"]] == list(tds[2][:, 1:].ravel())
+    assert np.abs(tds[0][:, 1]).max() > 0 and np.abs(tds[2][:, 1:]).max() > 0
     eng.close()
     op.close()
 
