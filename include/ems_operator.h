@@ -101,6 +101,28 @@ int emsop_add_metal_curve(emsop* h, unsigned npts, const double* pts);
  * FDTD/extensions/operator_ext_excitation.cpp:235-287); types 2 and 3 add nothing. */
 int emsop_add_excitation_curve(emsop* h, int type, const double val[3], double delay, unsigned npts,
                                const double* pts);
+/* Cylinder of a metal property (CSPrimCylinder, shell_width = 0) or
+ * cylindrical shell (CSPrimCylindricalShell: radius is the wall's mid radius,
+ * shell_width its thickness), axis start -> stop in drawing units.  A point is
+ * inside when the foot of its perpendicular on the axis lies within the axis
+ * segment and its distance d from the axis satisfies d <= radius (cylinder)
+ * or |d - radius| <= shell_width / 2 (shell).  It takes part in the
+ * MATERIAL|METAL lookup at every edge centre like a metal box
+ * (Operator::CalcPEC_Range, FDTD/operator.cpp:2045-2083). */
+int emsop_add_metal_cylinder(emsop* h, int priority, const double start[3], const double stop[3], double radius,
+                             double shell_width);
+/* Cylinder or cylindrical shell of an excitation property, as
+ * emsop_add_excitation with that shape (operator_ext_excitation.cpp:150-215). */
+int emsop_add_excitation_cylinder(emsop* h, int type, const double val[3], double delay, int priority,
+                                  const double start[3], const double stop[3], double radius, double shell_width);
+/* Weight functions of the excitation primitive added last (box, cylinder or
+ * curve; CSPropExcitation::SetWeightFunction): weight[n] is an expression in
+ * x, y, z (drawing units), rho = sqrt(x^2 + y^2), r = sqrt(x^2 + y^2 + z^2)
+ * and a = atan2(y, x), in the syntax of emsop_set_custom_signal, or NULL for
+ * weight 1.  The amplitude of an edge of direction n becomes
+ * (weight[n](Yee point) * val[n]) * edge length
+ * (operator_ext_excitation.cpp:176,212,267). */
+int emsop_set_excitation_weight(emsop* h, const char* const weight[3]);
 /* bc: 0 PEC, 1 PMC, 2 Mur, 3 PML, -1 none (openEMS::m_BC_type) */
 int emsop_set_boundary(emsop* h, const int bc[6], const unsigned pml_size[6], const double mur_vph[6]);
 int emsop_set_signal(emsop* h, int type, double f0, double fc);

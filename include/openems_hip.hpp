@@ -120,6 +120,24 @@ public:
         check_operator(emsop_add_excitation_curve(h_, type, val, delay, (unsigned)points.size(),
                                                   points.empty() ? nullptr : points[0].data()));
     }
+    // shell_width 0: solid cylinder (CSPrimCylinder); > 0: cylindrical shell of mid radius `radius`
+    void AddMetalCylinder(int priority, const double start[3], const double stop[3], double radius,
+                          double shell_width = 0)
+    {
+        check_operator(emsop_add_metal_cylinder(h_, priority, start, stop, radius, shell_width));
+    }
+    void AddExcitationCylinder(int type, const double val[3], double delay, int priority, const double start[3],
+                               const double stop[3], double radius, double shell_width = 0)
+    {
+        check_operator(emsop_add_excitation_cylinder(h_, type, val, delay, priority, start, stop, radius, shell_width));
+    }
+    // weight functions (x, y, z, rho, r, a) of the excitation primitive added last; an empty string means 1
+    void SetExcitationWeight(const std::string& wx, const std::string& wy, const std::string& wz)
+    {
+        const char* w[3] = {wx.empty() ? nullptr : wx.c_str(), wy.empty() ? nullptr : wy.c_str(),
+                            wz.empty() ? nullptr : wz.c_str()};
+        check_operator(emsop_set_excitation_weight(h_, w));
+    }
     // BC: 0 PEC, 1 PMC, 2 Mur, 3 PML, -1 none (openEMS::m_BC_type)
     void SetBoundaryCondition(const int BC[6], const unsigned pml_size[6], const double mur_vph[6])
     {

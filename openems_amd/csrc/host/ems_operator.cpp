@@ -66,10 +66,53 @@ int fail(int code, const std::string& msg)
     return code;
 }
 
+// Point_Line_Distance of CSXCAD (Cartesian mesh): foot = (P - s).(e - s) / |e - s|^2
+// is the position of the foot of the perpendicular along s -> e, dist the
+// distance of P from the line.  A zero-length line gives foot = NaN.
+static void point_line_distance(const double* P, const double* s, const double* e, double& foot, double& dist)
+{
+    const double d[3] = {e[0] - s[0], e[1] - s[1], e[2] - s[2]};
+    const double LL = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    foot = ((P[0] - s[0]) * d[0] + (P[1] - s[1]) * d[1] + (P[2] - s[2]) * d[2]) / LL;
+    const double q[3] = {P[0] - (s[0] + foot * d[0]), P[1] - (s[1] + foot * d[1]), P[2] - (s[2] + foot * d[2])};
+    dist = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+}
+
+// A primitive: an axis-aligned box (lo, hi), or a cylinder (CSPrimCylinder:
+// axis ax0 -> ax1, radius rad) or cylindrical shell (CSPrimCylindricalShell:
+// mid radius rad, wall width) whose bounding box is lo, hi.  A point is inside
+// a cylinder when the foot of its perpendicular on the axis lies within the
+// axis segment (0 <= foot <= 1) and its distance from the axis is at most the
+// radius; inside a shell when that distance is within width / 2 of rad.
 struct Box {
     double lo[3], hi[3];
     int prio;
     int order;
+    int shape = 0;            // 0 box, 1 cylinder, 2 cylindrical shell
+    double ax0[3] = {0, 0, 0}, ax1[3] = {0, 0, 0};
+    double rad = 0, width = 0;
+    bool contains(const double* P) const
+    {
+        if (shape == 0) return true;   // the per-axis masks decide for boxes
+        double foot, dist;
+        point_line_distance(P, ax0, ax1, foot, dist);
+        if (!(foot >= 0 && foot <= 1)) return false;
+        if (shape == 1) return dist <= rad;
+        return std::fabs(dist - rad) <= width / 2;
+    }
+    void set_cylinder(const double* a, const double* b, double r, double w)
+    {
+        shape = w > 0 ? 2 : 1;
+        rad = r;
+        width = w > 0 ? w : 0;
+        const double reach = r + width / 2;
+        for (int n = 0; n < 3; ++n) {
+            ax0[n] = a[n];
+            ax1[n] = b[n];
+            lo[n] = std::min(a[n], b[n]) - reach;
+            hi[n] = std::max(a[n], b[n]) + reach;
+        }
+    }
 };
 
 struct MatBox : Box {
@@ -119,10 +162,24 @@ struct LumpedBox : Box {
     bool caps, parallel;
 };
 
+// Excitation weight functions (CSPropExcitation::SetWeightFunction): one
+// expression per direction in x, y, z, rho, r, a (drawing units, a = the
+// azimuth); the amplitude of an edge is weight(Yee point) * value * length
+struct Expr;
+struct ExcWeight {
+    std::shared_ptr<const Expr> f[3];
+};
+double weight_eval(const ExcWeight& w, int n, const double* P);
+inline double weighted(const std::shared_ptr<const ExcWeight>& w, int n, double val, const double* P)
+{
+    return w ? weight_eval(*w, n, P) * val : val;
+}
+
 struct ExcBox : Box {
     int type;
     double val[3];
     double delay;
+    std::shared_ptr<const ExcWeight> weight;
 };
 
 struct Slab {
@@ -145,6 +202,7 @@ struct Curve {
     int type = 0;              // excitation type (0 soft, 1 hard)
     double val[3] = {0, 0, 0};
     double delay = 0;
+    std::shared_ptr<const ExcWeight> weight;
 };
 
 // one mesh edge of a path: component dir at node pos
@@ -248,6 +306,21 @@ struct Ranked {
         }
         return -1;
     }
+    // as winner, for sets that may hold cylinders: a set bit of a cylinder
+    // only means P is in its bounding box, so it wins only if it contains P
+    inline int winner_at(const uint64_t* a, const uint64_t* b, const uint64_t* c, const double* P) const
+    {
+        for (int w = (int)W - 1; w >= 0; --w) {
+            uint64_t x = a[w] & b[w] & c[w];
+            while (x) {
+                const int bit = 63 - __builtin_clzll(x);
+                const int r = w * 64 + bit;
+                if (box[r]->contains(P)) return r;
+                x &= ~((uint64_t)1 << bit);
+            }
+        }
+        return -1;
+    }
 };
 
 }  // namespace
@@ -263,6 +336,7 @@ struct emsop {
     std::vector<LumpedBox> lumped;
     std::vector<ExcBox> excs;
     std::vector<Curve> curves;
+    int last_exc = 0;        // 1: the excitation primitive added last is excs.back(), 2: curves.back()
     int bc[6] = {0, 0, 0, 0, 0, 0};
     unsigned pml[6] = {8, 8, 8, 8, 8, 8};
     double vph[6] = {0, 0, 0, 0, 0, 0};
@@ -800,11 +874,16 @@ void emsop::calc_pec()
     const unsigned W = rk.W;
     for (int n = 0; n < 3; ++n) {
         std::vector<uint64_t> m[3];
-        for (int a = 0; a < 3; ++a) m[a] = rk.masks(a, a == n ? dualL[a] : lines[a]);
+        const std::vector<double>* c[3];
+        for (int a = 0; a < 3; ++a) {
+            c[a] = a == n ? &dualL[a] : &lines[a];
+            m[a] = rk.masks(a, *c[a]);
+        }
         for (unsigned x = 0; x < N[0]; ++x)
             for (unsigned y = 0; y < N[1]; ++y)
                 for (unsigned z = 0; z < N[2]; ++z) {
-                    int r = rk.winner(&m[0][(size_t)x * W], &m[1][(size_t)y * W], &m[2][(size_t)z * W]);
+                    const double P[3] = {(*c[0])[x], (*c[1])[y], (*c[2])[z]};
+                    int r = rk.winner_at(&m[0][(size_t)x * W], &m[1][(size_t)y * W], &m[2][(size_t)z * W], P);
                     if (r >= 0 && rk.box[r]->metal) {
                         coeff[0][idx(n, x, y, z)] = 0;
                         coeff[1][idx(n, x, y, z)] = 0;
@@ -813,17 +892,6 @@ void emsop::calc_pec()
     }
 }
 
-// Point_Line_Distance of CSXCAD (Cartesian mesh): foot = (P - s).(e - s) / |e - s|^2
-// is the position of the foot of the perpendicular along s -> e, dist the
-// distance of P from the line.  A zero-length line gives foot = NaN.
-static void point_line_distance(const double* P, const double* s, const double* e, double& foot, double& dist)
-{
-    const double d[3] = {e[0] - s[0], e[1] - s[1], e[2] - s[2]};
-    const double LL = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-    foot = ((P[0] - s[0]) * d[0] + (P[1] - s[1]) * d[1] + (P[2] - s[2]) * d[2]) / LL;
-    const double q[3] = {P[0] - (s[0] + foot * d[0]), P[1] - (s[1] + foot * d[1]), P[2] - (s[2] + foot * d[2])};
-    dist = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
-}
 
 // Operator::FindPath (operator.cpp:390-496) with SnapLine2Mesh
 // (operator.cpp:360-388): a staircase of mesh edges from the node nearest
@@ -1039,7 +1107,9 @@ int emsop::build_excitation()
             for (const auto& e : path) {
                 const int n = e.dir;
                 if (e.pos[n] >= N[n] - 1 || (c.type != 0 && c.type != 1)) continue;
-                const double amp = c.val[n] * EL(n, e.pos[n], false);
+                double P[3];
+                for (int a = 0; a < 3; ++a) P[a] = (a == n ? dualL[a] : lines[a])[e.pos[a]];
+                const double amp = weighted(c.weight, n, c.val[n], P) * EL(n, e.pos[n], false);
                 if (amp != 0) {
                     for (int a = 0; a < 3; ++a) exc[0].idx[a].push_back(e.pos[a]);
                     exc[0].dir.push_back((unsigned short)n);
@@ -1087,12 +1157,17 @@ void emsop::build_excitation_boxes()
                 for (int n = 0; n < 3; ++n) {
                     if (p[n] >= N[n] - 1) continue;  // GetYeeCoords(...,false) == false
                     const std::vector<uint64_t>* m[3];
-                    for (int a = 0; a < 3; ++a) m[a] = (a == n) ? &mD[a] : &mP[a];
-                    int r = rk.winner(&(*m[0])[(size_t)p[0] * W], &(*m[1])[(size_t)p[1] * W], &(*m[2])[(size_t)p[2] * W]);
+                    double P[3];
+                    for (int a = 0; a < 3; ++a) {
+                        m[a] = (a == n) ? &mD[a] : &mP[a];
+                        P[a] = ((a == n) ? dualL[a] : lines[a])[p[a]];
+                    }
+                    int r = rk.winner_at(&(*m[0])[(size_t)p[0] * W], &(*m[1])[(size_t)p[1] * W],
+                                         &(*m[2])[(size_t)p[2] * W], P);
                     if (r < 0) continue;
                     const ExcBox* b = rk.box[r];
                     if (b->type != 0 && b->type != 1) continue;
-                    double amp = b->val[n] * EL(n, p[n], false);
+                    double amp = weighted(b->weight, n, b->val[n], P) * EL(n, p[n], false);
                     if (amp != 0) {
                         for (int a = 0; a < 3; ++a) exc[0].idx[a].push_back(p[a]);
                         exc[0].dir.push_back((unsigned short)n);
@@ -1108,12 +1183,17 @@ void emsop::build_excitation_boxes()
                 for (int n = 0; n < 3; ++n) {
                     if (p[0] >= N[0] - 1 || p[1] >= N[1] - 1 || p[2] >= N[2] - 1) continue;
                     const std::vector<uint64_t>* m[3];
-                    for (int a = 0; a < 3; ++a) m[a] = (a == n) ? &mP[a] : &mD[a];
-                    int r = rk.winner(&(*m[0])[(size_t)p[0] * W], &(*m[1])[(size_t)p[1] * W], &(*m[2])[(size_t)p[2] * W]);
+                    double P[3];
+                    for (int a = 0; a < 3; ++a) {
+                        m[a] = (a == n) ? &mP[a] : &mD[a];
+                        P[a] = ((a == n) ? lines[a] : dualL[a])[p[a]];
+                    }
+                    int r = rk.winner_at(&(*m[0])[(size_t)p[0] * W], &(*m[1])[(size_t)p[1] * W],
+                                         &(*m[2])[(size_t)p[2] * W], P);
                     if (r < 0) continue;
                     const ExcBox* b = rk.box[r];
                     if (b->type != 2 && b->type != 3) continue;
-                    double amp = b->val[n] * EL(n, p[n], true);
+                    double amp = weighted(b->weight, n, b->val[n], P) * EL(n, p[n], true);
                     if (amp != 0) {
                         for (int a = 0; a < 3; ++a) exc[1].idx[a].push_back(p[a]);
                         exc[1].dir.push_back((unsigned short)n);
@@ -1376,9 +1456,10 @@ struct Expr {
 
 struct ExprParser {
     const std::string& s;
+    const std::vector<std::string>& vars;   // variable names; VAR nodes hold the index in fn
     size_t i = 0;
     std::string err;
-    explicit ExprParser(const std::string& str) : s(str) {}
+    ExprParser(const std::string& str, const std::vector<std::string>& v) : s(str), vars(v) {}
     void ws() { while (i < s.size() && std::isspace((unsigned char)s[i])) ++i; }
     bool eat(const char* tok)
     {
@@ -1494,10 +1575,12 @@ struct ExprParser {
             while (i < s.size() && (std::isalnum((unsigned char)s[i]) || s[i] == '_')) ++i;
             const std::string name = s.substr(b, i - b);
             auto e = std::make_unique<Expr>();
-            if (name == "t") {
-                e->op = Expr::VAR;
-                return e;
-            }
+            for (size_t k = 0; k < vars.size(); ++k)
+                if (name == vars[k]) {
+                    e->op = Expr::VAR;
+                    e->fn = (int)k;
+                    return e;
+                }
             if (name == "pi" || name == "e") {
                 e->op = Expr::NUM;
                 e->v = name == "pi" ? 3.14159265358979323846 : 2.71828182845904523536;
@@ -1529,12 +1612,12 @@ struct ExprParser {
     }
 };
 
-double expr_eval(const Expr& e, double t)
+double expr_eval(const Expr& e, const double* var)
 {
-    auto A = [&](int k) { return expr_eval(*e.a[k], t); };
+    auto A = [&](int k) { return expr_eval(*e.a[k], var); };
     switch (e.op) {
     case Expr::NUM: return e.v;
-    case Expr::VAR: return t;
+    case Expr::VAR: return var[e.fn];
     case Expr::NEG: return -A(0);
     case Expr::NOT: return A(0) == 0 ? 1.0 : 0.0;
     case Expr::ADD: return A(0) + A(1);
@@ -1586,15 +1669,39 @@ double expr_eval(const Expr& e, double t)
     return 0;
 }
 
-// parse a custom signal; on failure returns null and sets err
-std::unique_ptr<Expr> parse_signal(const std::string& text, std::string& err)
+std::unique_ptr<Expr> parse_expr(const std::string& text, const std::vector<std::string>& vars, std::string& err)
 {
-    ExprParser P(text);
+    ExprParser P(text, vars);
     auto e = P.orx();
     P.ws();
     if (e && P.i != text.size()) e = P.fail_("unexpected text");
     err = P.err;
+    // a failed operand leaves a node with a missing child behind: any
+    // recorded error fails the whole parse
+    if (!err.empty()) e.reset();
     return e;
+}
+
+// parse a custom signal f(t); on failure returns null and sets err
+std::unique_ptr<Expr> parse_signal(const std::string& text, std::string& err)
+{
+    static const std::vector<std::string> vars = {"t"};
+    return parse_expr(text, vars, err);
+}
+
+// parse a weight function of x, y, z, rho, r, a
+std::unique_ptr<Expr> parse_weight(const std::string& text, std::string& err)
+{
+    static const std::vector<std::string> vars = {"x", "y", "z", "rho", "r", "a"};
+    return parse_expr(text, vars, err);
+}
+
+double weight_eval(const ExcWeight& w, int n, const double* P)
+{
+    if (!w.f[n]) return 1.0;
+    const double x = P[0], y = P[1], z = P[2];
+    const double v[6] = {x, y, z, std::sqrt(x * x + y * y), std::sqrt(x * x + y * y + z * z), std::atan2(y, x)};
+    return expr_eval(*w.f[n], v);
 }
 }  // namespace
 
@@ -1661,9 +1768,9 @@ int emsop::build_signal(unsigned nrts)
         sigC.resize(sigLen);
         for (unsigned n = 0; n < sigLen; ++n) {
             double t = n * dT;
-            sigV[n] = (float)expr_eval(*ex, t);
+            sigV[n] = (float)expr_eval(*ex, &t);
             t += 0.5 * dT;
-            sigC[n] = (float)expr_eval(*ex, t);
+            sigC[n] = (float)expr_eval(*ex, &t);
         }
         nyquist = nyquist_num(f0, dT);
         break;
@@ -1785,8 +1892,10 @@ void emsop::build_lorentz()
                         if (p[n] >= N[n] - 1) continue;                       // GetYeeCoords (operator.cpp:189-193)
                         const size_t q = idx(n, p[0], p[1], p[2]);
                         if (coeff[1][q] == 0) continue;
-                        const int r = rk.winner(&mv[n][0][(size_t)p[0] * W], &mv[n][1][(size_t)p[1] * W],
-                                                &mv[n][2][(size_t)p[2] * W]);
+                        double P[3];
+                        for (int a = 0; a < 3; ++a) P[a] = (a == n ? dualL[a] : lines[a])[p[a]];
+                        const int r = rk.winner_at(&mv[n][0][(size_t)p[0] * W], &mv[n][1][(size_t)p[1] * W],
+                                                   &mv[n][2][(size_t)p[2] * W], P);
                         if (r < 0 || rk.box[r]->disp < 0) continue;
                         const Dispersion& D = disps[rk.box[r]->disp];
                         if (D.debye) {
@@ -1823,8 +1932,10 @@ void emsop::build_lorentz()
                         if (p[nP] >= N[nP] - 1 || p[nPP] >= N[nPP] - 1) continue;   // operator.cpp:194-200
                         const size_t q = idx(n, p[0], p[1], p[2]);
                         if (coeff[3][q] == 0) continue;
-                        const int r = rk.winner(&mi[n][0][(size_t)p[0] * W], &mi[n][1][(size_t)p[1] * W],
-                                                &mi[n][2][(size_t)p[2] * W]);
+                        double P[3];
+                        for (int a = 0; a < 3; ++a) P[a] = (a == n ? lines[a] : dualL[a])[p[a]];
+                        const int r = rk.winner_at(&mi[n][0][(size_t)p[0] * W], &mi[n][1][(size_t)p[1] * W],
+                                                   &mi[n][2][(size_t)p[2] * W], P);
                         if (r < 0 || rk.box[r]->disp < 0) continue;
                         const Dispersion& D = disps[rk.box[r]->disp];
                         if (D.debye) continue;     // no current-side Debye term
@@ -2232,6 +2343,68 @@ int emsop_add_excitation(emsop* h, int type, const double val[3], double delay, 
     b.prio = priority;
     b.order = next_order(h);
     h->excs.push_back(b);
+    h->last_exc = 1;
+    return 0;
+}
+
+static int check_cylinder(const double start[3], const double stop[3], double radius, double shell_width)
+{
+    if (!(radius > 0) || !(shell_width >= 0)) return fail(EMSOP_E_ARG, "cylinder radius must be > 0 and shell width >= 0");
+    if (start[0] == stop[0] && start[1] == stop[1] && start[2] == stop[2])
+        return fail(EMSOP_E_ARG, "cylinder axis has zero length");
+    return 0;
+}
+
+int emsop_add_metal_cylinder(emsop* h, int priority, const double start[3], const double stop[3], double radius,
+                             double shell_width)
+{
+    if (!h || !start || !stop) return fail(EMSOP_E_ARG, "null argument");
+    if (int rc = check_cylinder(start, stop, radius, shell_width)) return rc;
+    MatBox b;
+    b.set_cylinder(start, stop, radius, shell_width);
+    for (int n = 0; n < 3; ++n) {
+        b.eps[n] = b.mue[n] = 1;
+        b.kappa[n] = b.sigma[n] = 0;
+    }
+    b.prio = priority;
+    b.order = next_order(h);
+    b.metal = true;
+    h->mats.push_back(b);
+    return 0;
+}
+
+int emsop_add_excitation_cylinder(emsop* h, int type, const double val[3], double delay, int priority,
+                                  const double start[3], const double stop[3], double radius, double shell_width)
+{
+    if (!h || !val || !start || !stop) return fail(EMSOP_E_ARG, "null argument");
+    if (type < 0 || type > 3) return fail(EMSOP_E_ARG, "excitation type must be 0..3");
+    if (int rc = check_cylinder(start, stop, radius, shell_width)) return rc;
+    ExcBox b;
+    b.set_cylinder(start, stop, radius, shell_width);
+    for (int n = 0; n < 3; ++n) b.val[n] = val[n];
+    b.type = type;
+    b.delay = delay;
+    b.prio = priority;
+    b.order = next_order(h);
+    h->excs.push_back(b);
+    h->last_exc = 1;
+    return 0;
+}
+
+int emsop_set_excitation_weight(emsop* h, const char* const weight[3])
+{
+    if (!h || !weight) return fail(EMSOP_E_ARG, "null argument");
+    if (h->last_exc == 0) return fail(EMSOP_E_STATE, "no excitation primitive to weight");
+    auto w = std::make_shared<ExcWeight>();
+    for (int n = 0; n < 3; ++n) {
+        if (!weight[n]) continue;   // no function: weight 1
+        std::string err;
+        auto e = parse_weight(weight[n], err);
+        if (!e) return fail(EMSOP_E_ARG, std::string("excitation weight ") + "xyz"[n] + ": " + err);
+        w->f[n] = std::shared_ptr<const Expr>(std::move(e));
+    }
+    if (h->last_exc == 1) h->excs.back().weight = w;
+    else h->curves.back().weight = w;
     return 0;
 }
 
@@ -2257,6 +2430,7 @@ int emsop_add_excitation_curve(emsop* h, int type, const double val[3], double d
     for (int n = 0; n < 3; ++n) c.val[n] = val[n];
     c.delay = delay;
     h->curves.push_back(c);
+    h->last_exc = 2;
     return 0;
 }
 

@@ -124,6 +124,11 @@ class Operator:
                 check_op(L.emsop_add_metal_curve(self.h, pts.shape[0], ptr(pts, C.c_double)))
                 continue
             st, sp = _v3(m["start"]), _v3(m["stop"])
+            if "radius" in m:
+                check_op(L.emsop_add_metal_cylinder(self.h, int(m.get("priority", 0)), ptr(st, C.c_double),
+                                                    ptr(sp, C.c_double), float(m["radius"]),
+                                                    float(m.get("shell_width", 0.0))))
+                continue
             check_op(L.emsop_add_metal(self.h, int(m.get("priority", 0)), ptr(st, C.c_double), ptr(sp, C.c_double)))
         for le in s.get("lumped", []):
             st, sp = _v3(le["start"]), _v3(le["stop"])
@@ -137,11 +142,20 @@ class Operator:
                 val = _v3(e.get("val", 0.0))
                 check_op(L.emsop_add_excitation_curve(self.h, int(e.get("type", 0)), ptr(val, C.c_double),
                                                       float(e.get("delay", 0.0)), pts.shape[0], ptr(pts, C.c_double)))
-                continue
-            st, sp, val = _v3(e["start"]), _v3(e["stop"]), _v3(e.get("val", 0.0))
-            check_op(L.emsop_add_excitation(self.h, int(e.get("type", 0)), ptr(val, C.c_double),
-                                            float(e.get("delay", 0.0)), int(e.get("priority", 0)),
-                                            ptr(st, C.c_double), ptr(sp, C.c_double)))
+            else:
+                st, sp, val = _v3(e["start"]), _v3(e["stop"]), _v3(e.get("val", 0.0))
+                if "radius" in e:
+                    check_op(L.emsop_add_excitation_cylinder(
+                        self.h, int(e.get("type", 0)), ptr(val, C.c_double), float(e.get("delay", 0.0)),
+                        int(e.get("priority", 0)), ptr(st, C.c_double), ptr(sp, C.c_double), float(e["radius"]),
+                        float(e.get("shell_width", 0.0))))
+                else:
+                    check_op(L.emsop_add_excitation(self.h, int(e.get("type", 0)), ptr(val, C.c_double),
+                                                    float(e.get("delay", 0.0)), int(e.get("priority", 0)),
+                                                    ptr(st, C.c_double), ptr(sp, C.c_double)))
+            if e.get("weight") is not None:
+                w = (C.c_char_p * 3)(*[None if f is None else str(f).encode() for f in e["weight"]])
+                check_op(L.emsop_set_excitation_weight(self.h, w))
         bc = np.asarray(s.get("bc", [0] * 6), np.int32)
         pml = np.asarray(s.get("pml_size", [8] * 6), np.uint32)
         vph = np.asarray(s.get("mur_vph", [0.0] * 6), np.float64)

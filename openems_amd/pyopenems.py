@@ -20,8 +20,9 @@ swapped::
     port.CalcPort(sim_path, freqs)
 
 The supported subset is what the operator builder takes: rectilinear
-Cartesian meshes, Box and Curve primitives, Material / Lorentz / Debye /
-Metal / Excitation / Probe / Dump / LumpedElement properties, lumped ports
+Cartesian meshes, Box and Curve primitives (and Cylinder / CylindricalShell
+for metals and excitations), Material / Lorentz / Debye / Metal / Excitation
+(with weight functions) / Probe / Dump / LumpedElement properties, lumped ports
 and Gaussian, sinusoidal, Dirac, step or custom excitation.  ``Run`` builds
 the operator and the engine, runs up to NrTS steps or the end criterion, and
 writes the probe files and HDF5 dumps into sim_path, as openEMS does.
@@ -105,7 +106,8 @@ class Property:
         self.kind = kind
         self.name = name
         self.params = params
-        self.prims = []          # ("box", priority, start, stop) or ("curve", priority, points)
+        self.prims = []          # ("box", priority, start, stop), ("curve", priority, points) or
+                                 # ("cylinder", priority, start, stop, radius, shell width)
 
     def GetName(self):
         return self.name
@@ -123,6 +125,24 @@ class Property:
             p = p.T
         self.prims.append(("curve", int(priority), [tuple(r) for r in p.tolist()]))
         return self
+
+    def AddCylinder(self, start, stop, radius, priority=0, **kw):
+        """CSPrimCylinder: axis start -> stop, radius (metal and excitation properties)."""
+        self.prims.append(("cylinder", int(priority), [float(v) for v in start], [float(v) for v in stop],
+                           float(radius), 0.0))
+        return self
+
+    def AddCylindricalShell(self, start, stop, radius, shell_width, priority=0, **kw):
+        """CSPrimCylindricalShell: mid radius and wall width (metal and excitation properties)."""
+        self.prims.append(("cylinder", int(priority), [float(v) for v in start], [float(v) for v in stop],
+                           float(radius), float(shell_width)))
+        return self
+
+    def SetWeightFunction(self, func):
+        """Excitation weights, one expression per direction in x, y, z, rho, r, a."""
+        if len(func) != 3:
+            raise ValueError("a weight function per direction (3) is needed")
+        self.params["weight"] = [str(f) for f in func]
 
     def SetMaterialProperty(self, **kw):
         self.params.update(kw)
@@ -193,8 +213,9 @@ class ContinuousStructure:
             P = p.params
             for prim in p.prims:
                 kind, prio = prim[0], prim[1]
-                if kind == "curve" and p.kind not in ("metal", "excitation"):
-                    raise ValueError(f"curve primitives of {p.kind} {p.name!r} are not supported")
+                if kind in ("curve", "cylinder") and p.kind not in ("metal", "excitation"):
+                    raise ValueError(f"{kind} primitives of {p.kind} {p.name!r} are not supported")
+                cyl = {"radius": prim[4], "shell_width": prim[5]} if kind == "cylinder" else {}
                 if p.kind in ("material", "lorentz", "debye"):
                     m = {"eps": P["epsilon"], "mue": P["mue"], "kappa": P["kappa"], "sigma": P["sigma"],
                          "priority": prio, "start": prim[2], "stop": prim[3]}
@@ -211,13 +232,15 @@ class ContinuousStructure:
                     mats.append(m)
                 elif p.kind == "metal":
                     metals.append({"curve": prim[2]} if kind == "curve" else
-                                  {"priority": prio, "start": prim[2], "stop": prim[3]})
+                                  dict(priority=prio, start=prim[2], stop=prim[3], **cyl))
                 elif p.kind == "excitation":
                     if not P.get("enabled", True):
                         continue
                     e = {"type": P["exc_type"], "val": tuple(P["exc_val"]), "delay": P["delay"]}
+                    if P.get("weight") is not None:
+                        e["weight"] = list(P["weight"])
                     excs.append(dict(e, curve=prim[2]) if kind == "curve" else
-                                dict(e, priority=prio, start=prim[2], stop=prim[3]))
+                                dict(e, priority=prio, start=prim[2], stop=prim[3], **cyl))
                 elif p.kind == "lumped":
                     lumped.append({"dir": P["ny"], "R": P["R"], "C": P["C"], "L": P["L"], "caps": P["caps"],
                                    "type": "series" if P["LEtype"] == 1 else "parallel", "priority": prio,

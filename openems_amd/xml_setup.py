@@ -19,7 +19,9 @@ X/Y/ZLines), ``BackgroundMaterial``, and the properties Material, Metal,
 Excitation, ProbeBox, DumpBox (time- and frequency-domain E / H dumps),
 LumpedElement, LorentzMaterial and
 DebyeMaterial with Box primitives (P1, P2, Priority) and, for Metal and
-Excitation, Curve primitives (Vertex points).  Properties are entered in file
+Excitation, Curve (Vertex points), Cylinder (P1, P2, Radius) and
+CylindricalShell (P1, P2, Radius, ShellWidth) primitives; an Excitation's
+Weight element (X, Y, Z expressions).  Properties are entered in file
 order, primitives in property order, as CSXCAD hands them to openEMS
 (openems.cpp:483-694 for probes and dumps).  Anything else (other primitive
 types, cylindrical meshes) is refused with a
@@ -80,6 +82,12 @@ def _primitives(prop):
         elif p.tag == "Curve":
             pts = [_point(v) for v in p.findall("Vertex")]
             out.append(("curve", prio, pts))
+        elif p.tag in ("Cylinder", "CylindricalShell"):
+            p1, p2 = p.find("P1"), p.find("P2")
+            if p1 is None or p2 is None or p.get("Radius") is None:
+                raise ValueError(f"{p.tag} of {prop.get('Name')!r} needs P1, P2 and Radius")
+            width = float(p.get("ShellWidth", 0.0)) if p.tag == "CylindricalShell" else 0.0
+            out.append(("cylinder", prio, (_point(p1), _point(p2), float(p.get("Radius")), width)))
         else:
             raise ValueError(f"primitive {p.tag!r} of {prop.get('Name')!r} is not supported")
     return out
@@ -222,8 +230,10 @@ def read_xml(source):
         prims = _primitives(prop)
         boxes = [(pr, d) for k, pr, d in prims if k == "box"]
         curves = [d for k, pr, d in prims if k == "curve"]
-        if curves and prop.tag not in ("Metal", "Excitation"):
-            raise ValueError(f"curve primitives of {prop.tag} {name!r} are not supported")
+        if prop.tag not in ("Metal", "Excitation"):
+            for k, pr, d in prims:
+                if k != "box":
+                    raise ValueError(f"{k} primitives of {prop.tag} {name!r} are not supported")
         if prop.tag in ("Material", "LorentzMaterial", "DebyeMaterial"):
             el = prop.find("Property")
             base = {"eps": _vec(el, "Epsilon", 1.0), "mue": _vec(el, "Mue", 1.0),
@@ -236,14 +246,23 @@ def read_xml(source):
                 mats.append(dict(base, priority=prio, start=p1, stop=p2))
         elif prop.tag == "Metal":
             for kind, prio, d in prims:
-                metals.append({"curve": d} if kind == "curve" else {"priority": prio, "start": d[0], "stop": d[1]})
+                m = {"curve": d} if kind == "curve" else {"priority": prio, "start": d[0], "stop": d[1]}
+                if kind == "cylinder":
+                    m.update(radius=d[2], shell_width=d[3])
+                metals.append(m)
         elif prop.tag == "Excitation":
             e = {"type": _int(prop, "Type", 0), "val": _vec(prop, "Excite", 0.0),
                  "delay": float(prop.get("Delay", 0.0))}
             if isinstance(e["val"], float):
                 e["val"] = (e["val"],) * 3
+            w = prop.find("Weight")
+            if w is not None:
+                e["weight"] = [w.get(k) for k in ("X", "Y", "Z")]
             for kind, prio, d in prims:
-                excs.append(dict(e, curve=d) if kind == "curve" else dict(e, priority=prio, start=d[0], stop=d[1]))
+                x = dict(e, curve=d) if kind == "curve" else dict(e, priority=prio, start=d[0], stop=d[1])
+                if kind == "cylinder":
+                    x.update(radius=d[2], shell_width=d[3])
+                excs.append(x)
         elif prop.tag == "LumpedElement":
             le = {"dir": _int(prop, "Direction", -1), "R": float(prop.get("R", -1.0) or -1.0),
                   "C": float(prop.get("C", 0.0) or 0.0), "L": float(prop.get("L", 0.0) or 0.0),
@@ -297,6 +316,20 @@ def _box(parent, prio, p1, p2):
     b = ET.SubElement(parent, "Box", Priority=str(int(prio)))
     ET.SubElement(b, "P1", X=repr(float(p1[0])), Y=repr(float(p1[1])), Z=repr(float(p1[2])))
     ET.SubElement(b, "P2", X=repr(float(p2[0])), Y=repr(float(p2[1])), Z=repr(float(p2[2])))
+
+
+def _prim(parent, d):
+    """A box, or a cylinder / cylindrical shell when d has a radius."""
+    if "radius" not in d:
+        _box(parent, d.get("priority", 0), d["start"], d["stop"])
+        return
+    w = float(d.get("shell_width", 0.0))
+    attrs = {"Priority": str(int(d.get("priority", 0))), "Radius": repr(float(d["radius"]))}
+    if w > 0:
+        attrs["ShellWidth"] = repr(w)
+    c = ET.SubElement(parent, "CylindricalShell" if w > 0 else "Cylinder", **attrs)
+    for tag, p in (("P1", d["start"]), ("P2", d["stop"])):
+        ET.SubElement(c, tag, X=repr(float(p[0])), Y=repr(float(p[1])), Z=repr(float(p[2])))
 
 
 def _curve(parent, pts):
@@ -374,7 +407,7 @@ def write_xml(scene, path=None):
         if "curve" in m:
             _curve(pr, m["curve"])
         else:
-            _box(pr, m.get("priority", 0), m["start"], m["stop"])
+            _prim(pr, m)
     for k, le in enumerate(scene.get("lumped", [])):
         el = prop("LumpedElement", f"lumped{k}", Direction=str(int(le["dir"])), Caps=str(int(bool(le.get("caps", True)))),
                   R=_fmt(le.get("R", -1.0)), C=_fmt(le.get("C", 0.0)), L=_fmt(le.get("L", 0.0)),
@@ -383,11 +416,13 @@ def write_xml(scene, path=None):
     for k, e in enumerate(scene.get("excitations", [])):
         el = prop("Excitation", f"excite{k}", Type=str(int(e.get("type", 0))), Excite=_fmt(e.get("val", (0, 0, 0))),
                   Delay=_fmt(e.get("delay", 0.0)))
+        if e.get("weight") is not None:
+            ET.SubElement(el, "Weight", **{k: ("1" if f is None else str(f)) for k, f in zip("XYZ", e["weight"])})
         pr = ET.SubElement(el, "Primitives")
         if "curve" in e:
             _curve(pr, e["curve"])
         else:
-            _box(pr, e.get("priority", 0), e["start"], e["stop"])
+            _prim(pr, e)
     for k, p in enumerate(scene.get("probes", [])):
         el = prop("ProbeBox", p.get("name") or f"probe{k}", Type=str(int(p["type"])),
                   Weight=_fmt(p.get("weight", 1.0)), NormDir=str(int(p.get("norm_dir", -1))))

@@ -198,31 +198,80 @@ class BoxPainter:
         # boxes: list of (priority, insertion_index, start3, stop3, payload)
         self.boxes = sorted(boxes, key=lambda b: (b[0], -b[1]))
 
+    @staticmethod
+    def _fill(out, coords, lo, hi, payload, value):
+        masks = []
+        for n in range(3):
+            a, b = min(lo[n], hi[n]), max(lo[n], hi[n])
+            masks.append(np.nonzero((coords[n] >= a) & (coords[n] <= b))[0])
+        if any(len(m) == 0 for m in masks):
+            return
+        sel = np.ix_(*masks)
+        if "cyl" not in payload:
+            out[sel] = value
+            return
+        P = np.meshgrid(*[coords[n][masks[n]] for n in range(3)], indexing="ij")
+        inside = cylinder_contains(P, *payload["cyl"])
+        sub = out[sel]
+        sub[inside] = value
+        out[sel] = sub
+
     def paint(self, coords, background, value_fn):
         """coords: three 1-D arrays; returns array of shape (len x, len y, len z)."""
         out = np.full(tuple(len(c) for c in coords), background, dtype=np.float64)
         for prio, _, lo, hi, payload in self.boxes:
-            masks = []
-            for n in range(3):
-                a, b = min(lo[n], hi[n]), max(lo[n], hi[n])
-                masks.append(np.nonzero((coords[n] >= a) & (coords[n] <= b))[0])
-            if any(len(m) == 0 for m in masks):
-                continue
-            out[np.ix_(*masks)] = value_fn(payload)
+            self._fill(out, coords, lo, hi, payload, value_fn(payload))
         return out
 
     def paint_index(self, coords):
         """Index (into the sorted box list) of the winning box, -1 for none."""
         out = np.full(tuple(len(c) for c in coords), -1, dtype=np.int64)
         for k, (prio, _, lo, hi, payload) in enumerate(self.boxes):
-            masks = []
-            for n in range(3):
-                a, b = min(lo[n], hi[n]), max(lo[n], hi[n])
-                masks.append(np.nonzero((coords[n] >= a) & (coords[n] <= b))[0])
-            if any(len(m) == 0 for m in masks):
-                continue
-            out[np.ix_(*masks)] = k
+            self._fill(out, coords, lo, hi, payload, k)
         return out
+
+
+def _weighted(wt, n, val, P):
+    """GetWeightedExcitation * value (operator_ext_excitation.cpp:176,212,267):
+    (w_n(P) * val), or val itself without a weight function."""
+    if wt is None or wt[n] is None:
+        return val
+    return wt[n](*P) * val
+
+
+def cylinder_bounds(start, stop, radius, width):
+    """Bounding box of a cylinder (width 0) or cylindrical shell."""
+    reach = radius + width / 2
+    return ([min(a, b) - reach for a, b in zip(start, stop)], [max(a, b) + reach for a, b in zip(start, stop)])
+
+
+def cylinder_contains(P, start, stop, radius, width):
+    """CSPrimCylinder / CSPrimCylindricalShell IsInside on a Cartesian mesh
+    (CSXCAD, absent from the tree; restated from its published behaviour): the
+    foot of the perpendicular of P on the axis start -> stop lies within the
+    segment, and the distance d from the axis is <= radius (cylinder, width 0)
+    or |d - radius| <= width / 2 (shell).  Same operation order as the
+    product's Point_Line_Distance so that ties fall the same way."""
+    d = [stop[n] - start[n] for n in range(3)]
+    LL = d[0] * d[0] + d[1] * d[1] + d[2] * d[2]
+    foot = ((P[0] - start[0]) * d[0] + (P[1] - start[1]) * d[1] + (P[2] - start[2]) * d[2]) / LL
+    q = [P[n] - (start[n] + foot * d[n]) for n in range(3)]
+    dist = np.sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2])
+    ok = (foot >= 0) & (foot <= 1)
+    if width > 0:
+        return ok & (np.abs(dist - radius) <= width / 2)
+    return ok & (dist <= radius)
+
+
+def weight_function(text):
+    """Excitation weight w(x, y, z) (CSPropExcitation::SetWeightFunction): an
+    expression in x, y, z, rho = sqrt(x^2 + y^2), r = sqrt(x^2 + y^2 + z^2)
+    and a = atan2(y, x), in the function-parser syntax of custom_function."""
+    f = custom_function(text, ("x", "y", "z", "rho", "r", "a"))
+
+    def w(x, y, z):
+        return f(x, y, z, math.sqrt(x * x + y * y), math.sqrt(x * x + y * y + z * z), math.atan2(y, x))
+    return w
 
 
 # ----------------------------------------------------------------------------
@@ -292,7 +341,7 @@ def excitation_signal(sig, dT, nrts):
     raise ValueError(kind)
 
 
-def custom_function(text):
+def custom_function(text, names=("t",)):
     """f(t) of a custom excitation written in the reference's function-parser
     syntax (fparser: ^ power, = equality, & | logical, pi, e), evaluated through
     Python's own expression parser: the text is rewritten into Python syntax
@@ -319,7 +368,7 @@ def custom_function(text):
         if isinstance(node, ast.Constant):
             return float(node.value)
         if isinstance(node, ast.Name):
-            return t if node.id == "t" else consts[node.id]
+            return t[names.index(node.id)] if node.id in names else consts[node.id]
         if isinstance(node, ast.BinOp):
             return binops[type(node.op)](ev(node.left, t), ev(node.right, t))
         if isinstance(node, ast.UnaryOp) and isinstance(node.op, ast.USub):
@@ -334,7 +383,7 @@ def custom_function(text):
                 return ev(node.args[1], t) if ev(node.args[0], t) != 0 else ev(node.args[2], t)
             return float(fns[node.func.id](*[ev(a, t) for a in node.args]))
         raise ValueError(f"unsupported expression part {ast.dump(node)}")
-    return lambda t: ev(tree, float(t))
+    return lambda *v: ev(tree, [float(x) for x in v])
 
 
 def signal_period(sig):
@@ -456,19 +505,32 @@ class RefOperator:
             if "curve" in m:
                 self.metal_curves.append(np.asarray(m["curve"], np.float64).reshape(-1, 3))
                 continue
+            if "radius" in m:
+                w = float(m.get("shell_width", 0.0))
+                lo, hi = cylinder_bounds(m["start"], m["stop"], float(m["radius"]), w)
+                metals.append((m.get("priority", 0), off + k, lo, hi,
+                               {"metal": True, "cyl": (list(m["start"]), list(m["stop"]), float(m["radius"]), w)}))
+                continue
             metals.append((m.get("priority", 0), off + k, m["start"], m["stop"], {"metal": True}))
         self.mat_painter = BoxPainter(mats)
         self.pec_painter = BoxPainter(mats + metals)
         excs = []
         self.exc_curves = []
         for k, e in enumerate(scene.get("excitations", [])):
+            wt = e.get("weight")
+            wt = None if wt is None else [None if f is None else weight_function(str(f)) for f in wt]
             if "curve" in e:
                 self.exc_curves.append((np.asarray(e["curve"], np.float64).reshape(-1, 3), int(e.get("type", 0)),
-                                        _vec3(e.get("val", (0, 0, 0))), float(e.get("delay", 0.0))))
+                                        _vec3(e.get("val", (0, 0, 0))), float(e.get("delay", 0.0)), wt))
                 continue
-            excs.append((e.get("priority", 0), k, e["start"], e["stop"],
-                         {"type": int(e.get("type", 0)), "val": _vec3(e.get("val", (0, 0, 0))),
-                          "delay": float(e.get("delay", 0.0)), "enabled": e.get("enabled", True)}))
+            payload = {"type": int(e.get("type", 0)), "val": _vec3(e.get("val", (0, 0, 0))),
+                       "delay": float(e.get("delay", 0.0)), "enabled": e.get("enabled", True), "weight": wt}
+            lo, hi = e["start"], e["stop"]
+            if "radius" in e:
+                w = float(e.get("shell_width", 0.0))
+                lo, hi = cylinder_bounds(e["start"], e["stop"], float(e["radius"]), w)
+                payload["cyl"] = (list(e["start"]), list(e["stop"]), float(e["radius"]), w)
+            excs.append((e.get("priority", 0), k, lo, hi, payload))
         self.exc_painter = BoxPainter(excs)
 
     # -- material lookup ----------------------------------------------------
@@ -1165,11 +1227,12 @@ class RefOperator:
             self._excitation_boxes(vv, vi, ii, iv, dT, volt, curr)
         # curves after the boxes (operator_ext_excitation.cpp:235-287)
         ELp = [m.edge_len(n, False) for n in range(3)]
-        for pts, typ, val, delay in self.exc_curves:
+        for pts, typ, val, delay, wt in self.exc_curves:
             for n, pos in self.curve_edges(pts):
                 if pos[n] >= N[n] - 1 or typ not in (0, 1):
                     continue
-                amp = val[n] * ELp[n][pos[n]]
+                co = self.yee_coords(n, False)
+                amp = _weighted(wt, n, val[n], [co[a][pos[a]] for a in range(3)]) * ELp[n][pos[n]]
                 if amp != 0:
                     volt.append((pos, n, f32(amp), int(delay / dT)))
                 if typ == 1:
@@ -1194,8 +1257,10 @@ class RefOperator:
         """Box primitives: scan order z (outer), y, x (inner), then n."""
         N = self.N
         m = self.mesh
-        win_v = [self.exc_painter.paint_index(self.yee_coords(n, False)) for n in range(3)]
-        win_c = [self.exc_painter.paint_index(self.yee_coords(n, True)) for n in range(3)]
+        cv = [self.yee_coords(n, False) for n in range(3)]
+        cc = [self.yee_coords(n, True) for n in range(3)]
+        win_v = [self.exc_painter.paint_index(cv[n]) for n in range(3)]
+        win_c = [self.exc_painter.paint_index(cc[n]) for n in range(3)]
         ELp = [m.edge_len(n, False) for n in range(3)]
         ELd = [m.edge_len(n, True) for n in range(3)]
         boxes = self.exc_painter.boxes
@@ -1215,7 +1280,7 @@ class RefOperator:
                 e = boxes[k][4]
                 if not e["enabled"] or e["type"] not in (0, 1):
                     continue
-                amp = e["val"][n] * ELp[n][pos[n]]
+                amp = _weighted(e["weight"], n, e["val"][n], [cv[n][a][pos[a]] for a in range(3)]) * ELp[n][pos[n]]
                 if amp != 0:
                     volt.append((pos, n, f32(amp), int(e["delay"] / dT)))
                 if e["type"] == 1:
@@ -1230,7 +1295,7 @@ class RefOperator:
                 e = boxes[k][4]
                 if not e["enabled"] or e["type"] not in (2, 3):
                     continue
-                amp = e["val"][n] * ELd[n][pos[n]]
+                amp = _weighted(e["weight"], n, e["val"][n], [cc[n][a][pos[a]] for a in range(3)]) * ELd[n][pos[n]]
                 if amp != 0:
                     curr.append((pos, n, f32(amp), int(e["delay"] / dT)))
                 if e["type"] == 3:

@@ -9,6 +9,9 @@
 // (12, 20, 4) to (14, 22, 24); a 50 Ohm parallel lumped port along z at
 // (14, 14, 10)-(14, 14, 12) with caps and a soft E_z source on the same box;
 // a Drude box (plasma 12 GHz, relaxation 30 ps) from (19, 9, 9) to (22, 14, 18);
+// a metal wire (curve) and a metal cylinder (radius 1.6, axis (20, 18, 3) to
+// (24, 24, 9)); a soft E_x source on a cylinder (radius 1.2, axis (5, 21, 21)
+// to (10, 21, 21)) weighted 1 + 0.1 x;
 // Gaussian pulse f0 = 0, fc = 15 GHz.
 #include <cstdio>
 #include <array>
@@ -46,10 +49,15 @@ int main()
     const double m0[3] = {12, 20, 4}, m1[3] = {14, 22, 24};
     op->AddMetalBox(2, m0, m1);
     op->AddMetalCurve({{{4.0, 4.0, 6.0}, {4.0, 4.0, 12.0}, {7.3, 5.6, 15.2}}});
+    const double c0[3] = {20, 18, 3}, c1[3] = {24, 24, 9};
+    op->AddMetalCylinder(1, c0, c1, 1.6);
     const double p0[3] = {14, 14, 10}, p1[3] = {14, 14, 12};
     op->AddLumpedElement(2, 50.0, 0.0, 0.0, true, true, 0, p0, p1);
     const double ez[3] = {0, 0, 1};
     op->AddExcitationBox(0, ez, 0.0, 0, p0, p1);
+    const double ex[3] = {1, 0, 0}, x0[3] = {5, 21, 21}, x1[3] = {10, 21, 21};
+    op->AddExcitationCylinder(0, ex, 0.0, 0, x0, x1, 1.2);
+    op->SetExcitationWeight("1+0.1*x", "", "");
     const int bc[6] = {3, 3, 0, 0, 0, 0};
     const unsigned pml[6] = {6, 6, 6, 6, 6, 6};
     const double vph[6] = {0, 0, 0, 0, 0, 0};

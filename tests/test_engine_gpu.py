@@ -455,10 +455,13 @@ def _mirror_scene():
              "lorentz": [{"eps_plasma": 12e9, "eps_relax": 30e-12}]},
         ],
         "metals": [{"priority": 2, "start": (12, 20, 4), "stop": (14, 22, 24)},
-                   {"curve": [(4.0, 4.0, 6.0), (4.0, 4.0, 12.0), (7.3, 5.6, 15.2)]}],
+                   {"curve": [(4.0, 4.0, 6.0), (4.0, 4.0, 12.0), (7.3, 5.6, 15.2)]},
+                   {"priority": 1, "start": (20, 18, 3), "stop": (24, 24, 9), "radius": 1.6}],
         "lumped": [{"dir": 2, "R": 50.0, "C": 0.0, "L": 0.0, "caps": True, "type": "parallel", "priority": 0,
                     "start": (14, 14, 10), "stop": (14, 14, 12)}],
-        "excitations": [{"type": 0, "val": (0.0, 0.0, 1.0), "start": (14, 14, 10), "stop": (14, 14, 12)}],
+        "excitations": [{"type": 0, "val": (0.0, 0.0, 1.0), "start": (14, 14, 10), "stop": (14, 14, 12)},
+                        {"type": 0, "val": (1.0, 0.0, 0.0), "start": (5, 21, 21), "stop": (10, 21, 21),
+                         "radius": 1.2, "weight": ["1+0.1*x", None, None]}],
         "signal": {"type": "gauss", "f0": 0.0, "fc": 15e9},
         "nrts": 400,
     }

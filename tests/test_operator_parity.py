@@ -294,11 +294,16 @@ def test_custom_excitation_signal(native, function):
     assert op.signal()[0].size == 300
 
 
-def test_custom_excitation_parse_error(native):
+@pytest.mark.parametrize("function,message", [("sin(2*pi*t", "expected at position"),
+                                               ("t+", "unexpected end"), ("2*t-", "unexpected end"),
+                                               ("t<", "unexpected end"), ("if(t,1,)", "unexpected")])
+def test_custom_excitation_parse_error(native, function, message):
+    """Malformed expressions, including a missing last operand, fail the build
+    with the position of the error."""
     from openems_amd._native import NativeError
     sc = scenes.pec_cavity(16, 30)
-    sc["signal"] = {"type": "custom", "function": "sin(2*pi*t", "f0": 5e9}
-    with pytest.raises(NativeError, match="expected at position"):
+    sc["signal"] = {"type": "custom", "function": function, "f0": 5e9}
+    with pytest.raises(NativeError, match=message):
         Operator(sc)
 
 
@@ -328,3 +333,31 @@ def test_processing_windows_match_oracle(native):
     # closed form of the window: floor / ceil, inverted -> whole run
     r = RefProbe(ref, 2, p0["start"], p0["start"], 1.0, [], -1, start_time=10.5 * dT, stop_time=20.2 * dT)
     assert (r.start_ts, r.stop_ts) == (10, 21)
+
+
+def test_tilted_cylinders_and_weighted_boxes(native):
+    """Cylinders and shells on oblique axes (metal, and hard / soft excitation
+    with weights in x, y, z, rho, r, a), priorities that overlap a dielectric
+    box and a metal box, on a graded mesh: coefficients and excitation lists
+    bit-identical to the oracle."""
+    sc = scenes.wire_antenna(28, 20)
+    sc["metals"] = [
+        {"priority": 2, "start": (-6.0, -4.0, -3.0), "stop": (5.0, 7.0, 9.0), "radius": 2.2},
+        {"priority": 1, "start": (4.0, -8.0, 0.5), "stop": (-3.0, 6.0, -2.0), "radius": 3.1, "shell_width": 1.3},
+        {"priority": 0, "start": (-2.0, -2.0, -2.0), "stop": (1.0, 3.0, 2.0)},
+    ]
+    sc["excitations"] = [
+        {"type": 0, "val": (1.0, -0.5, 0.25), "priority": 3, "start": (-7.0, 5.0, -6.0), "stop": (6.0, -5.0, 4.0),
+         "radius": 1.7, "shell_width": 0.9, "weight": ["x*y + 0.5", "cos(a)*rho", None]},
+        {"type": 1, "val": (0.0, 0.0, 2.0), "priority": 1, "start": (2.0, 2.0, -8.0), "stop": (2.5, 3.0, 8.0),
+         "radius": 1.2, "weight": [None, None, "exp(-r/4)*(z>0)"]},
+        {"type": 2, "val": (0.3, 0.3, 0.3), "priority": 0, "start": (-8.0, -8.0, 3.0), "stop": (8.0, 8.0, 5.0),
+         "weight": ["1", "y/3", "sqrt(x*x+1)"]},
+        {"type": 0, "val": (0.0, 0.0, 1.0), "weight": ["0", "0", "2+sin(z)"],
+         "curve": [(0.3, 0.4, -3.0), (0.3, 0.4, 3.0)]},
+    ]
+    op, ref = _compare(sc)
+    assert op.excitation(False)["dir"].size > 20 and op.excitation(True)["dir"].size > 20
+    bare = dict(sc, metals=[], excitations=[])
+    zeros = lambda o: np.count_nonzero(o.coeff("vv") == 0)
+    assert zeros(op) > zeros(Operator(bare, nthreads=2)) + 100

@@ -83,3 +83,78 @@ def test_port_dft_closed_form():
     assert abs(DFT_time2freq(t, v, f, "pulse")[0] - 3.0 * 400 * (t[1] - t[0])) < 1e-18
     with pytest.raises(ValueError):
         DFT_time2freq(t, v, f, "other")
+
+
+def coax_setup(res=5.0, length=1000.0, nrts=5000):
+    """The structure of the reference's TESTSUITE/combinedtests/Coax.m as a
+    script of this interface: a coaxial line along z (inner PEC cylinder of
+    radius 100, outer PEC shell from 230 to 240, drawing unit 1 mm), PEC on
+    five faces and PML_8 at z max, a soft E excitation weighted x / rho^2,
+    y / rho^2 on a shell between the conductors in the first half cell, a
+    voltage probe across the gap and a current probe around the inner
+    conductor at z = length / 2.  Returns (openEMS object, Z0 of the line)."""
+    eps0, mue0 = 8.85418781762e-12, 1.256637062e-6
+    ri, rai, raa = 100.0, 230.0, 240.0
+    FDTD = openEMS(NrTS=nrts, EndCriteria=1e-6)
+    FDTD.SetGaussExcite(0, 1e9)
+    FDTD.SetBoundaryCond(["PEC", "PEC", "PEC", "PEC", "PEC", "PML_8"])
+    CSX = ContinuousStructure()
+    FDTD.SetCSX(CSX)
+    mesh = CSX.GetGrid()
+    mesh.SetDeltaUnit(1e-3)
+    x = -2.5 * res - raa + res * np.arange(int(round((2 * raa + 5 * res) / res)) + 1)
+    mesh.SetLines("x", x)
+    mesh.SetLines("y", x.copy())
+    mesh.SetLines("z", np.linspace(0, length, int(round(length / res)) + 1))
+    pec = CSX.AddMetal("PEC")
+    pec.AddCylinder([0, 0, 0], [0, 0, length], ri, priority=1)
+    pec.AddCylindricalShell([0, 0, 0], [0, 0, length], 0.5 * (raa + rai), raa - rai, priority=0)
+    exc = CSX.AddExcitation("excite", 0, [1, 1, 0])
+    exc.SetWeightFunction(["(x)/(x*x+y*y)", "y/pow(rho,2)", "0"])
+    exc.AddCylindricalShell([0, 0, 0], [0, 0, res / 2], 0.5 * (ri + rai), rai - ri)
+    CSX.AddProbe("ut1", 0).AddBox([ri, 0, length / 2], [rai, 0, length / 2])
+    mid = ri + 3 * res
+    CSX.AddProbe("it1", 1).AddBox([-mid, -mid, length / 2], [mid, mid, length / 2])
+    Z0 = np.sqrt(mue0 / eps0) * np.log(rai / ri) / (2 * np.pi)
+    return FDTD, Z0
+
+
+def test_cylinders_and_weights_match_oracle(native):
+    """Cylinder and cylindrical-shell metals and a weighted excitation on a
+    shell (a coarse coax of the reference's Coax test): coefficients and
+    excitation lists bit-identical to the oracle, and the excitation
+    amplitudes follow the weight x / rho^2, y / rho^2 times the edge length."""
+    from tests.test_operator_parity import _compare
+    FDTD, _ = coax_setup(res=10.0, length=200.0, nrts=100)
+    scene = FDTD.scene()
+    op, ref = _compare(scene)
+    vv = op.coeff("vv")
+    L = [np.asarray(l, np.float64) for l in scene["lines"]]
+    # E_z edges inside the inner conductor (rho <= 100) are PEC, those in the gap are not
+    xi = int(np.argmin(np.abs(L[0] - 2.5)))
+    assert vv[2, xi, xi, 5] == 0
+    xg = int(np.argmin(np.abs(L[0] - 162.5)))
+    assert vv[2, xg, xi, 5] != 0
+    e = op.excitation(False)
+    assert e["dir"].size > 100 and set(np.unique(e["dir"])) == {0, 1}
+    idx, d, amp = e["index"], e["dir"], e["amp"]
+    pos = np.stack(np.unravel_index(idx, tuple(op.N)), 0) if idx.ndim == 1 else idx
+    dual = [(l[:-1] + l[1:]) / 2 for l in L]
+    for k in range(0, d.size, 7):
+        p = pos[:, k]
+        n = int(d[k])
+        P = [dual[a][p[a]] if a == n else L[a][p[a]] for a in range(3)]
+        rho2 = P[0] ** 2 + P[1] ** 2
+        w = P[n] / rho2
+        assert np.isclose(amp[k], w * (L[n][p[n] + 1] - L[n][p[n]]) * 1e-3, rtol=1e-6)
+
+
+def test_cylinder_arguments_are_checked(native):
+    with pytest.raises(Exception):
+        FDTD, _ = coax_setup(res=10.0, length=200.0, nrts=100)
+        FDTD.CSX.AddMetal("bad").AddCylinder([0, 0, 0], [0, 0, 0], 5.0)
+        Operator(FDTD.scene(), nthreads=1)
+    FDTD, _ = coax_setup(res=10.0, length=200.0, nrts=100)
+    FDTD.CSX.props[-3].SetWeightFunction(["x+", "0", "0"])
+    with pytest.raises(Exception, match="weight"):
+        Operator(FDTD.scene(), nthreads=1)

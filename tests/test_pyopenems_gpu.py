@@ -230,3 +230,28 @@ def test_cavity_resonances_against_closed_form(gpu, tmp_path):
     assert _check_frequency(f, uz, f_TM * (1 + 0), f_TM * (1 - 2.5e-3), 0.27, "inside")
     assert _check_frequency(f, uy, o2, o1, 0.17, "outside")
     assert _check_frequency(f, uz, o2m, o1m, 0.17, "outside")
+
+
+def test_coax_impedance_against_closed_form(gpu, tmp_path):
+    """The reference's physics test TESTSUITE/combinedtests/Coax.m on the
+    engine: a 1 m coaxial line (inner radius 100 mm, outer 230 mm, 5 mm mesh,
+    PML_8 at the far end) excited by a soft E source weighted x / rho^2,
+    y / rho^2 in the plane z = 0; |u / i| of the voltage and current probes
+    at the middle stays within -1 % / +3 % of the closed form
+    Z0 = sqrt(mu0 / eps0) ln(230 / 100) / (2 pi) from 0 to 1 GHz."""
+    import sys
+    import os
+    sys.path.insert(0, os.path.dirname(__file__))
+    from test_pyopenems import coax_setup
+    FDTD, Z0 = coax_setup()
+    print("case combinedtests Coax, 102 x 102 x 201 lines, up to 5000 steps", flush=True)
+    assert FDTD.Run(str(tmp_path)) == 0
+    tu = np.loadtxt(str(tmp_path / "ut1"), comments="%")
+    ti = np.loadtxt(str(tmp_path / "it1"), comments="%")
+    f, u = _fft_time2freq(tu[:, 0], tu[:, 1])
+    _, i = _fft_time2freq(ti[:, 0], ti[:, 1])
+    k1, k2 = int(np.argmin(np.abs(f - 0.0))), int(np.argmin(np.abs(f - 1e9)))
+    Z = np.abs(u[k1:k2 + 1] / i[k1:k2 + 1])
+    print("Z0 %.4f, |u/i| from %.4f to %.4f over %d frequencies" % (Z0, Z.min(), Z.max(), Z.size), flush=True)
+    assert Z.size > 10
+    assert np.all(Z <= Z0 * 1.03) and np.all(Z >= Z0 * 0.99)

@@ -100,6 +100,18 @@ def test_structure_subset():
     assert op.excitation()["dir"].size == 2
 
 
+def _coax_scene():
+    """Cylinder and cylindrical-shell metals and a weighted excitation shell."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(__file__))
+    from test_pyopenems import coax_setup
+    FDTD, _ = coax_setup(res=10.0, length=200.0, nrts=100)
+    scene = FDTD.scene()
+    assert any("radius" in m for m in scene["metals"]) and scene["excitations"][0]["weight"]
+    return scene
+
+
 def test_unsupported_parts_are_refused():
     with pytest.raises(ValueError, match="Cylinder|cylindrical"):
         read_xml(SETUP.replace('<FDTD NumberOfTimesteps="2e3"', '<FDTD CylinderCoords="1" NumberOfTimesteps="2e3"'))
@@ -117,6 +129,7 @@ def test_unsupported_parts_are_refused():
     lambda: scenes.dispersive_rlc(32, 100),
     lambda: scenes.wire_antenna(30, 100),
     lambda: scenes.mur_layers(32, 30, 28, 100),
+    lambda: _coax_scene(),
 ])
 def test_round_trip_rebuilds_the_same_operator(native, make, tmp_path):
     """write_xml -> read_xml reproduces every coefficient, timestep and
