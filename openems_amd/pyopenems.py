@@ -22,7 +22,7 @@ swapped::
 The supported subset is what the operator builder takes: rectilinear
 Cartesian meshes, Box and Curve primitives (and Cylinder / CylindricalShell
 for metals and excitations), Material / Lorentz / Debye / Metal / Excitation
-(with weight functions) / Probe / Dump / LumpedElement properties, lumped ports
+(with weight functions) / Probe / Dump / LumpedElement properties, lumped and microstrip ports
 and Gaussian, sinusoidal, Dirac, step or custom excitation.  ``Run`` builds
 the operator and the engine, runs up to NrTS steps or the end criterion, and
 writes the probe files and HDF5 dumps into sim_path, as openEMS does.
@@ -333,14 +333,34 @@ class LumpedPort:
         self.it_tot = sum(self.i_data.ui_val)
 
     def CalcPort(self, sim_path, freq, ref_impedance=None, ref_plane_shift=None, signal_type="pulse"):
+        """Incident / reflected split (ports.Port.CalcPort, python/openEMS/ports.py:108-150):
+        against ref_impedance, else the port's own Z_ref (R for a lumped port,
+        the measured line impedance for a microstrip port); ref_plane_shift
+        moves the reference plane along a line port by beta * shift."""
         self.ReadUIData(sim_path, freq, signal_type)
-        Z = self.R if ref_impedance is None else ref_impedance
-        self.Z_ref = Z
+        if ref_impedance is not None:
+            self.Z_ref = ref_impedance
+        if self.Z_ref is None:
+            self.Z_ref = getattr(self, "R", None)
+        if self.Z_ref is None:
+            raise ValueError("Port Z_ref should not be None!")
+        Z = self.Z_ref
+        if ref_plane_shift is not None:
+            if not hasattr(self, "beta"):
+                raise ValueError("Port has no beta attribute!")
+            shift = ref_plane_shift
+            if getattr(self, "measplane_shift", 0):
+                shift -= self.measplane_shift
+            shift *= self.CSX.GetGrid().GetDeltaUnit()
+            phase = np.real(self.beta) * shift
+            uf = self.uf_tot * np.cos(-phase) + 1j * self.if_tot * Z * np.sin(-phase)
+            itf = self.if_tot * np.cos(-phase) + 1j * self.uf_tot / Z * np.sin(-phase)
+            self.uf_tot, self.if_tot = uf, itf
         self.uf_inc = 0.5 * (self.uf_tot + self.if_tot * Z)
         self.if_inc = 0.5 * (self.if_tot + self.uf_tot / Z)
         self.uf_ref = self.uf_tot - self.uf_inc
         self.if_ref = self.if_inc - self.if_tot
-        if isinstance(Z, (int, float)):
+        if isinstance(Z, (int, float, np.floating)):
             self.ut_inc = 0.5 * (self.ut_tot + self.it_tot * Z)
             self.it_inc = 0.5 * (self.it_tot + self.ut_tot / Z)
             self.ut_ref = self.ut_tot - self.ut_inc
@@ -348,6 +368,103 @@ class LumpedPort:
         self.P_inc = 0.5 * np.real(self.uf_inc * np.conj(self.if_inc))
         self.P_ref = 0.5 * np.real(self.uf_ref * np.conj(self.if_ref))
         self.P_acc = 0.5 * np.real(self.uf_tot * np.conj(self.if_tot))
+
+
+class MSLPort(LumpedPort):
+    """A microstrip port (ports.MSLPort, python/openEMS/ports.py:190-340): the
+    strip as a metal box in the plane start[exc] of the port box, three
+    voltage probes across the substrate at the mesh lines around the
+    measurement plane (half the port length by default, MeasPlaneShift), two
+    current probes around the strip between them, an optional soft feed at
+    FeedShift and a feed resistance Feed_R at the port start.  ReadUIData
+    takes the middle voltage and the mean current and derives the
+    propagation constant beta and the line impedance Z_ref from the
+    differences (the method of the reference's port)."""
+
+    def __init__(self, CSX, port_nr, metal_prop, start, stop, prop_dir, exc_dir, excite=0, **kw):
+        self.CSX = CSX
+        self.number = port_nr
+        self.start = np.array(start, np.float64)
+        self.stop = np.array(stop, np.float64)
+        self.excite = excite
+        self.priority = kw.get("priority", 0)
+        self.delay = kw.get("delay", 0.0)
+        self.Z_ref = None
+        lbl = kw.get("PortNamePrefix", "") + "port_{}" + "_{}".format(port_nr)
+        ex, pr = _dir(exc_dir), _dir(prop_dir)
+        self.exc_ny, self.prop_ny = ex, pr
+        if not np.all(self.start != self.stop):
+            raise ValueError("Start coordinate must not be equal to stop coordinate")
+        if ex == pr:
+            raise ValueError("Excitation direction must not be equal to propagation direction")
+        self.direction = np.sign(self.stop[pr] - self.start[pr])
+        self.upside_down = np.sign(self.stop[ex] - self.start[ex])
+        self.feed_shift = kw.get("FeedShift", 0)
+        self.measplane_shift = kw.get("MeasPlaneShift", 0.5 * abs(self.start[pr] - self.stop[pr]))
+        self.measplane_pos = self.start[pr] + self.measplane_shift * self.direction
+        self.feed_R = kw.get("Feed_R", np.inf)
+        strip_stop = self.stop.copy()
+        strip_stop[ex] = self.start[ex]
+        metal_prop.AddBox(self.start, strip_stop, priority=self.priority)
+        lines = CSX.GetGrid().GetLines(pr)
+        if len(lines) <= 5:
+            raise ValueError("At least 5 lines in propagation direction required!")
+        k = int(np.clip(np.argmin(np.abs(lines - self.measplane_pos)), 1, len(lines) - 2))
+        self.measplane_shift = abs(self.start[pr] - lines[k])
+        idx = np.array([k - 1, k, k + 1])
+        if self.direction < 0:
+            idx = idx[::-1]
+        u_pos = lines[idx]
+        self.U_delta = np.diff(u_pos)
+        self.U_filenames = []
+        mid = 0.5 * (self.start + self.stop)
+        for n, sfx in enumerate("ABC"):
+            a, b = mid.copy(), mid.copy()
+            a[pr] = b[pr] = u_pos[n]
+            a[ex], b[ex] = self.start[ex], self.stop[ex]
+            self.U_filenames.append(lbl.format("ut") + sfx)
+            CSX.AddProbe(self.U_filenames[-1], p_type=0).AddBox(a, b)
+        i_pos = u_pos[:2] + np.diff(u_pos) / 2.0
+        self.I_delta = np.diff(i_pos)
+        self.I_filenames = []
+        a, b = self.start.copy(), self.stop.copy()
+        b[ex] = self.start[ex]
+        for n, sfx in enumerate("AB"):
+            a[pr] = b[pr] = i_pos[n]
+            self.I_filenames.append(lbl.format("it") + sfx)
+            CSX.AddProbe(self.I_filenames[-1], p_type=1, weight=self.direction, norm_dir=pr).AddBox(a.copy(), b.copy())
+        if excite != 0:
+            kf = int(np.argmin(np.abs(lines - (self.start[pr] + self.feed_shift * self.direction))))
+            a, b = self.start.copy(), self.stop.copy()
+            a[pr] = b[pr] = lines[kf]
+            v = np.zeros(3)
+            v[ex] = -1 * self.upside_down * excite
+            CSX.AddExcitation(lbl.format("excite"), exc_type=0, exc_val=v, delay=self.delay).AddBox(
+                a, b, priority=self.priority)
+        if self.feed_R >= 0 and not np.isinf(self.feed_R):
+            a, b = self.start.copy(), self.stop.copy()
+            b[pr] = a[pr]
+            if self.feed_R == 0:
+                metal_prop.AddBox(a, b)
+            else:
+                CSX.AddLumpedElement(lbl.format("resist"), ny=ex, caps=True, R=self.feed_R).AddBox(a, b)
+
+    def ReadUIData(self, sim_path, freq, signal_type="pulse"):
+        self.u_data = UI_data(self.U_filenames, sim_path, freq, signal_type)
+        self.uf_tot = self.u_data.ui_f_val[1]
+        self.ut_tot = self.u_data.ui_val[1]
+        self.i_data = UI_data(self.I_filenames, sim_path, freq, signal_type)
+        self.if_tot = 0.5 * (self.i_data.ui_f_val[0] + self.i_data.ui_f_val[1])
+        self.it_tot = 0.5 * (self.i_data.ui_val[0] + self.i_data.ui_val[1])
+        unit = self.CSX.GetGrid().GetDeltaUnit()
+        Et = self.u_data.ui_f_val[1]
+        dEt = (self.u_data.ui_f_val[2] - self.u_data.ui_f_val[0]) / (np.sum(np.abs(self.U_delta)) * unit)
+        Ht = self.if_tot
+        dHt = (self.i_data.ui_f_val[1] - self.i_data.ui_f_val[0]) / (np.abs(self.I_delta[0]) * unit)
+        beta = np.sqrt(-dEt * dHt / (Ht * Et))
+        beta[np.real(beta) < 0] *= -1
+        self.beta = beta
+        self.Z_ref = np.sqrt(Et * dEt / (Ht * dHt))
 
 
 # ----------------------------------------------------------------------------
@@ -429,6 +546,19 @@ class openEMS:
 
     def GetCSX(self):
         return self.CSX
+
+    def AddMSLPort(self, port_nr, metal_prop, start, stop, prop_dir, exc_dir, excite=0, **kw):
+        if self.CSX is None:
+            raise ValueError("AddMSLPort: CSX is not set!")
+        port = MSLPort(self.CSX, port_nr, metal_prop, start, stop, prop_dir, exc_dir, excite, **kw)
+        e2g = kw.get("edges2grid")
+        if e2g is not None:
+            grid = self.CSX.GetGrid()
+            for n in ([0, 1, 2] if e2g == "all" else [_dir(c) for c in e2g]):
+                grid.AddLine(n, start[n])
+                if start[n] != stop[n]:
+                    grid.AddLine(n, stop[n])
+        return port
 
     def AddLumpedPort(self, port_nr, R, start, stop, p_dir, excite=0, **kw):
         if self.CSX is None:

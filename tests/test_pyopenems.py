@@ -158,3 +158,80 @@ def test_cylinder_arguments_are_checked(native):
     FDTD.CSX.props[-3].SetWeightFunction(["x+", "0", "0"])
     with pytest.raises(Exception, match="weight"):
         Operator(FDTD.scene(), nthreads=1)
+
+
+def _msl_setup(nrts=3000):
+    """A 50 Ohm-class microstrip (eps_r 3.66, 0.254 mm substrate, 0.6 mm strip,
+    drawing unit 1 um) between two microstrip ports of 1 mm, port 1 fed,
+    both ends terminated with Feed_R = 50, 1.5 mm of substrate and an 8-cell
+    UPML beyond each end."""
+    from openems_amd.pyopenems import MSLPort  # noqa: F401
+    FDTD = openEMS(NrTS=nrts, EndCriteria=1e-4)
+    FDTD.SetGaussExcite(0, 20e9)
+    FDTD.SetBoundaryCond(["PML_8"] * 4 + ["PEC", "PML_8"])
+    CSX = ContinuousStructure()
+    FDTD.SetCSX(CSX)
+    mesh = CSX.GetGrid()
+    mesh.SetDeltaUnit(1e-6)
+    mesh.SetLines("x", np.arange(-7500, 7501, 100.0))
+    mesh.SetLines("y", np.concatenate([np.arange(-2000, -400, 100.0), np.arange(-400, 401, 50.0),
+                                       np.arange(500, 2001, 100.0)]))
+    mesh.SetLines("z", np.concatenate([np.linspace(0, 254, 5), np.arange(354, 2001, 100.0)]))
+    CSX.AddMaterial("RO4350B", epsilon=3.66).AddBox([-7500, -2000, 0], [7500, 2000, 254])
+    pec = CSX.AddMetal("PEC")
+    p1 = FDTD.AddMSLPort(1, pec, [-6000, -300, 254], [-5000, 300, 0], "x", "z", excite=-1, Feed_R=50, priority=10)
+    p2 = FDTD.AddMSLPort(2, pec, [6000, -300, 254], [5000, 300, 0], "x", "z", MeasPlaneShift=500, Feed_R=50,
+                         priority=10)
+    pec.AddBox([-5000, -300, 254], [5000, 300, 254], priority=10)
+    return FDTD, p1, p2
+
+
+def test_msl_port_structure():
+    """ports.MSLPort: three voltage probes across the substrate on the mesh
+    lines around the measurement plane, two current probes around the strip
+    half way between them, the feed on the port's first line, the strip in
+    the plane start[exc], and Feed_R as a lumped element at the port start."""
+    FDTD, p1, p2 = _msl_setup()
+    parts = FDTD.CSX.scene_parts()
+    probes = {p["name"]: p for p in parts["probes"]}
+    assert p1.U_filenames == ["port_ut_1A", "port_ut_1B", "port_ut_1C"]
+    assert [probes[n]["start"][0] for n in p1.U_filenames] == [-5600.0, -5500.0, -5400.0]
+    assert all(probes[n]["start"][2] == 254 and probes[n]["stop"][2] == 0 for n in p1.U_filenames)
+    assert [probes[n]["start"][0] for n in p1.I_filenames] == [-5550.0, -5450.0]
+    assert all(probes[n]["weight"] == 1 and probes[n]["norm_dir"] == 0 for n in p1.I_filenames)
+    # port 2 runs towards -x: probes in descending x, current weight -1
+    assert [probes[n]["start"][0] for n in p2.U_filenames] == [5600.0, 5500.0, 5400.0]
+    assert all(probes[n]["weight"] == -1 for n in p2.I_filenames)
+    assert p1.measplane_shift == 500 and p2.measplane_shift == 500
+    exc = [e for e in parts["excitations"]]
+    assert len(exc) == 1 and exc[0]["start"][0] == exc[0]["stop"][0] == -6000.0
+    assert exc[0]["val"] == (0.0, 0.0, -1.0)          # -upside_down * excite, upside_down = -1, excite = -1
+    assert len(parts["lumped"]) == 2 and all(le["R"] == 50 and le["dir"] == 2 for le in parts["lumped"])
+
+
+def test_msl_port_beta_and_impedance_of_a_travelling_wave(tmp_path):
+    """ReadUIData's finite-difference beta and Z_ref on probe files of an
+    ideal travelling wave u(x, t) = g(t - x / v), i = u / Z0, pinned against
+    the closed forms: beta -> 2 pi f / v and Z_ref -> Z0 as the probe spacing
+    shrinks against the wavelength; a plane shift of the reference plane by
+    the measurement-plane offset leaves the incident wave at the port start."""
+    from openems_amd.pyopenems import MSLPort
+    FDTD, p1, _ = _msl_setup()
+    v, Z0 = 1.6e8, 48.0
+    dt = 5e-14
+    t = np.arange(6000) * dt
+    g = lambda tt: np.exp(-((tt - 6e-11) / 1.5e-11) ** 2)
+    unit = 1e-6
+    probes = {p["name"]: p for p in FDTD.CSX.scene_parts()["probes"]}
+    for n in p1.U_filenames:
+        x = (probes[n]["start"][0] + 6000) * unit
+        np.savetxt(str(tmp_path / n), np.c_[t, g(t - x / v)], header="t u", comments="% ")
+    for n in p1.I_filenames:
+        x = (probes[n]["start"][0] + 6000) * unit
+        np.savetxt(str(tmp_path / n), np.c_[t + dt / 2, g(t + dt / 2 - x / v) / Z0], header="t i", comments="% ")
+    f = np.array([2e9, 5e9, 10e9])
+    p1.CalcPort(str(tmp_path), f)
+    assert np.allclose(np.real(p1.beta), 2 * np.pi * f / v, rtol=2e-2)
+    assert np.allclose(np.abs(p1.Z_ref), Z0, rtol=2e-2)
+    assert np.all(np.abs(p1.uf_ref) < 0.03 * np.abs(p1.uf_inc))
+    assert isinstance(p1, MSLPort)

@@ -255,3 +255,30 @@ def test_coax_impedance_against_closed_form(gpu, tmp_path):
     print("Z0 %.4f, |u/i| from %.4f to %.4f over %d frequencies" % (Z0, Z.min(), Z.max(), Z.size), flush=True)
     assert Z.size > 10
     assert np.all(Z <= Z0 * 1.03) and np.all(Z >= Z0 * 0.99)
+
+
+def test_msl_ports_through_line(gpu, tmp_path):
+    """Two microstrip ports (ports.MSLPort) on a 10 mm line of a 50 Ohm-class
+    strip: the measured line impedance lies within 35..60 Ohm (Hammerstad's
+    closed form gives 47 Ohm for w/h = 2.36, eps_r 3.66), beta between the
+    free-space and the substrate wave numbers, and the through line matched
+    (|S11| < 0.3, |S21| > 0.8) from 2 to 15 GHz."""
+    import sys
+    import os
+    sys.path.insert(0, os.path.dirname(__file__))
+    from test_pyopenems import _msl_setup
+    FDTD, p1, p2 = _msl_setup()
+    print("case microstrip ports, 151 x 48 x 22 lines, up to 3000 steps", flush=True)
+    assert FDTD.Run(str(tmp_path)) == 0
+    f = np.linspace(2e9, 15e9, 27)
+    p1.CalcPort(str(tmp_path), f)
+    p2.CalcPort(str(tmp_path), f, ref_impedance=p1.Z_ref)
+    k0 = 2 * np.pi * f / 299792458.0
+    s11 = p1.uf_ref / p1.uf_inc
+    s21 = p2.uf_ref / p1.uf_inc
+    print("Z_ref %.1f..%.1f Ohm, beta/k0 %.3f..%.3f, max|S11| %.3f, min|S21| %.3f" % (
+        np.real(p1.Z_ref).min(), np.real(p1.Z_ref).max(), (np.real(p1.beta) / k0).min(),
+        (np.real(p1.beta) / k0).max(), np.abs(s11).max(), np.abs(s21).min()), flush=True)
+    assert np.all((np.real(p1.Z_ref) > 35) & (np.real(p1.Z_ref) < 60))
+    assert np.all((np.real(p1.beta) > k0) & (np.real(p1.beta) < k0 * np.sqrt(3.66)))
+    assert np.all(np.abs(s11) < 0.3) and np.all(np.abs(s21) > 0.8)
