@@ -16,8 +16,8 @@
  *     [Operator::Calc_LumpedElements, FDTD/operator.cpp:1586-1763]
  *
  * Replaces Operator::SetGeometryCSX + Operator::CalcECOperator
- * (FDTD/operator.cpp:822-837, 986-1097) for scenes of boxes, material grids
- * and wires (polyline curves).
+ * (FDTD/operator.cpp:822-837, 986-1097) for scenes of boxes, cylinders,
+ * material grids and wires (polyline curves).
  * All functions return 0 on success and a negative EMSOP_E_* code on error;
  * emsop_last_error() describes the last failure of the calling thread.
  */
@@ -123,6 +123,12 @@ int emsop_add_excitation_cylinder(emsop* h, int type, const double val[3], doubl
  * (weight[n](Yee point) * val[n]) * edge length
  * (operator_ext_excitation.cpp:176,212,267). */
 int emsop_set_excitation_weight(emsop* h, const char* const weight[3]);
+/* Evaluates an expression of the syntax above at npts points: vars names the
+ * nvar variables, comma separated; values[k * nvar + i] is variable i at point
+ * k; out[k] the result.  Used for the mode functions of mode-matching probes
+ * (ProcessModeMatch::InitProcess, Common/processmodematch.cpp:78-147). */
+int emsop_eval_function(const char* text, const char* vars, unsigned nvar, const double* values, unsigned npts,
+                        double* out);
 /* bc: 0 PEC, 1 PMC, 2 Mur, 3 PML, -1 none (openEMS::m_BC_type) */
 int emsop_set_boundary(emsop* h, const int bc[6], const unsigned pml_size[6], const double mur_vph[6]);
 int emsop_set_signal(emsop* h, int type, double f0, double fc);

@@ -39,6 +39,7 @@ OPERATOR_API = [
     ("emsop_add_metal_cylinder", C.c_int, [vp, C.c_int, dp, dp, C.c_double, C.c_double]),
     ("emsop_add_excitation_cylinder", C.c_int, [vp, C.c_int, dp, C.c_double, C.c_int, dp, dp, C.c_double, C.c_double]),
     ("emsop_set_excitation_weight", C.c_int, [vp, C.POINTER(C.c_char_p)]),
+    ("emsop_eval_function", C.c_int, [C.c_char_p, C.c_char_p, C.c_uint, dp, C.c_uint, dp]),
     ("emsop_add_lumped", C.c_int, [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, dp, dp]),
     ("emsop_add_excitation", C.c_int, [vp, C.c_int, dp, C.c_double, C.c_int, dp, dp]),
     ("emsop_set_boundary", C.c_int, [vp, ip, up, dp]),

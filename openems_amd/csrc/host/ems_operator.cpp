@@ -2391,6 +2391,29 @@ int emsop_add_excitation_cylinder(emsop* h, int type, const double val[3], doubl
     return 0;
 }
 
+int emsop_eval_function(const char* text, const char* vars, unsigned nvar, const double* values, unsigned npts,
+                        double* out)
+{
+    if (!text || !vars || (npts && (!values || !out))) return fail(EMSOP_E_ARG, "null argument");
+    std::vector<std::string> names;
+    std::string cur;
+    for (const char* c = vars;; ++c) {
+        if (*c == ',' || *c == 0) {
+            names.push_back(cur);
+            cur.clear();
+            if (!*c) break;
+        } else if (!std::isspace((unsigned char)*c)) {
+            cur += *c;
+        }
+    }
+    if (names.size() != nvar) return fail(EMSOP_E_ARG, "variable list does not hold nvar names");
+    std::string err;
+    auto e = parse_expr(text, names, err);
+    if (!e) return fail(EMSOP_E_ARG, "function: " + err);
+    for (unsigned k = 0; k < npts; ++k) out[k] = expr_eval(*e, values + (size_t)k * nvar);
+    return 0;
+}
+
 int emsop_set_excitation_weight(emsop* h, const char* const weight[3])
 {
     if (!h || !weight) return fail(EMSOP_E_ARG, "null argument");

@@ -185,9 +185,10 @@ class ContinuousStructure:
         return self._add("excitation", name, exc_type=int(exc_type), exc_val=[float(v) for v in exc_val],
                          delay=float(delay))
 
-    def AddProbe(self, name, p_type, weight=1.0, norm_dir=-1, frequency=None, **kw):
+    def AddProbe(self, name, p_type, weight=1.0, norm_dir=-1, frequency=None, mode_function=None, **kw):
         return self._add("probe", name, p_type=int(p_type), weight=float(weight), norm_dir=int(norm_dir),
-                         frequency=list(np.atleast_1d(frequency)) if frequency is not None else [])
+                         frequency=list(np.atleast_1d(frequency)) if frequency is not None else [],
+                         mode_function=None if mode_function is None else [str(f) for f in mode_function])
 
     def AddDump(self, name, dump_type=0, dump_mode=1, file_type=1, frequency=None, **kw):
         return self._add("dump", name, dump_type=int(dump_type), dump_mode=int(dump_mode),
@@ -248,6 +249,8 @@ class ContinuousStructure:
                 elif p.kind == "probe":
                     probes.append({"type": P["p_type"], "name": p.name, "start": prim[2], "stop": prim[3],
                                    "weight": P["weight"], "freqs": P["frequency"], "norm_dir": P["norm_dir"]})
+                    if P.get("mode_function") is not None:
+                        probes[-1]["mode_function"] = [str(f) for f in P["mode_function"]]
                 elif p.kind == "dump":
                     dt = P["dump_type"]
                     if dt not in (0, 1, 10, 11):
@@ -266,6 +269,13 @@ class ContinuousStructure:
 # ----------------------------------------------------------------------------
 # ports
 # ----------------------------------------------------------------------------
+# python/openEMS/physical_constants.py
+C0 = 299792458.0
+MUE0 = 4e-7 * np.pi
+EPS0 = 1.0 / (MUE0 * C0 ** 2)
+Z0 = np.sqrt(MUE0 / EPS0)
+
+
 class UI_data:
     """Voltage / current probe files of a port read back (np.loadtxt with '%'
     comments, as the reference's port processing reads them)."""
@@ -368,6 +378,88 @@ class LumpedPort:
         self.P_inc = 0.5 * np.real(self.uf_inc * np.conj(self.if_inc))
         self.P_ref = 0.5 * np.real(self.uf_ref * np.conj(self.if_ref))
         self.P_acc = 0.5 * np.real(self.uf_tot * np.conj(self.if_tot))
+
+
+class WaveguidePort(LumpedPort):
+    """A waveguide port (ports.WaveguidePort, python/openEMS/ports.py:341-400):
+    a soft E excitation weighted by the mode's E pattern in the plane
+    start[exc] (when excited), and mode-matching probes (types 10 and 11) of
+    the E and H patterns in the plane stop[exc]; CalcPort takes the analytic
+    wave impedance k Z0 / beta (beta = sqrt(k^2 - kc^2)) as Z_ref."""
+
+    def __init__(self, CSX, port_nr, start, stop, exc_dir, E_WG_func, H_WG_func, kc, excite=0, **kw):
+        self.CSX = CSX
+        self.number = port_nr
+        self.start = np.array(start, np.float64)
+        self.stop = np.array(stop, np.float64)
+        self.excite = excite
+        self.priority = kw.get("priority", 0)
+        self.delay = kw.get("delay", 0.0)
+        self.Z_ref = None
+        lbl = kw.get("PortNamePrefix", "") + "port_{}" + "_{}".format(port_nr)
+        ny = _dir(exc_dir)
+        self.exc_ny, self.ny_P, self.ny_PP = ny, (ny + 1) % 3, (ny + 2) % 3
+        self.direction = np.sign(self.stop[ny] - self.start[ny])
+        self.ref_index = 1
+        if excite != 0 and self.stop[ny] == self.start[ny]:
+            raise ValueError("Port length in excitation direction may not be zero if port is excited!")
+        self.kc = kc
+        self.E_func, self.H_func = E_WG_func, H_WG_func
+        if excite != 0:
+            a, b = self.start.copy(), self.stop.copy()
+            b[ny] = a[ny]
+            v = np.ones(3)
+            v[ny] = 0
+            exc = CSX.AddExcitation(lbl.format("excite"), exc_type=0, exc_val=v, delay=self.delay)
+            exc.SetWeightFunction([str(x) for x in self.E_func])
+            exc.AddBox(a, b, priority=self.priority)
+        a, b = self.start.copy(), self.stop.copy()
+        a[ny] = b[ny]
+        self.measplane_shift = abs(self.stop[ny] - self.start[ny])
+        self.U_filenames = [lbl.format("ut")]
+        CSX.AddProbe(self.U_filenames[0], p_type=10, mode_function=self.E_func).AddBox(a, b)
+        self.I_filenames = [lbl.format("it")]
+        CSX.AddProbe(self.I_filenames[0], p_type=11, weight=self.direction, mode_function=self.H_func).AddBox(a, b)
+
+    def CalcPort(self, sim_path, freq, ref_impedance=None, ref_plane_shift=None, signal_type="pulse"):
+        k = 2.0 * np.pi * np.asarray(freq, np.float64) / C0 * self.ref_index
+        self.beta = np.sqrt(k ** 2 - self.kc ** 2 + 0j)
+        self.ZL = k * Z0 / self.beta
+        if ref_impedance is None:
+            self.Z_ref = self.ZL
+        super().CalcPort(sim_path, freq, ref_impedance, ref_plane_shift, signal_type)
+
+
+class RectWGPort(WaveguidePort):
+    """A rectangular waveguide port (ports.RectWGPort, ports.py:402-460): the
+    TE_mn patterns of an a x b guide (sizes in meters) written as mode
+    functions of the two transverse coordinates relative to start, in
+    drawing units; TE modes only."""
+
+    def __init__(self, CSX, port_nr, start, stop, exc_dir, a, b, mode_name, excite=0, **kw):
+        ny = _dir(exc_dir)
+        nP, nPP = (ny + 1) % 3, (ny + 2) % 3
+        if len(mode_name) != 4:
+            raise ValueError("Invalid mode definition")
+        if not mode_name.startswith("TE"):
+            raise ValueError("Currently only TE-modes are supported! Mode found: {}".format(mode_name))
+        self.WG_size, self.WG_mode = [a, b], mode_name
+        M, N = float(mode_name[2]), float(mode_name[3])
+        self.M, self.N = M, N
+        unit = CSX.GetGrid().GetDeltaUnit()
+        xyz = "xyz"
+        sP = "({}-{})".format(xyz[nP], float(start[nP]))
+        sPP = "({}-{})".format(xyz[nPP], float(start[nPP]))
+        kc = np.sqrt((M * np.pi / a) ** 2 + (N * np.pi / b) ** 2)
+        a, b = a / unit, b / unit
+        E, H = ["0", "0", "0"], ["0", "0", "0"]
+        if N > 0:
+            E[nP] = "{}*cos({}*{})*sin({}*{})".format(N / b, M * np.pi / a, sP, N * np.pi / b, sPP)
+            H[nPP] = "{}*cos({}*{})*sin({}*{})".format(N / b, M * np.pi / a, sP, N * np.pi / b, sPP)
+        if M > 0:
+            E[nPP] = "{}*sin({}*{})*cos({}*{})".format(-1 * M / a, M * np.pi / a, sP, N * np.pi / b, sPP)
+            H[nP] = "{}*sin({}*{})*cos({}*{})".format(M / a, M * np.pi / a, sP, N * np.pi / b, sPP)
+        super().__init__(CSX, port_nr, start, stop, exc_dir, E, H, kc, excite, **kw)
 
 
 class MSLPort(LumpedPort):
@@ -546,6 +638,11 @@ class openEMS:
 
     def GetCSX(self):
         return self.CSX
+
+    def AddRectWaveGuidePort(self, port_nr, start, stop, p_dir, a, b, mode_name, excite=0, **kw):
+        if self.CSX is None:
+            raise ValueError("AddRectWaveGuidePort: CSX is not set!")
+        return RectWGPort(self.CSX, port_nr, start, stop, p_dir, a, b, mode_name, excite, **kw)
 
     def AddMSLPort(self, port_nr, metal_prop, start, stop, prop_dir, exc_dir, excite=0, **kw):
         if self.CSX is None:

@@ -19,7 +19,7 @@ import numpy as np
 
 from .engine import EngineHIP
 from .operator import Operator
-from .processing import FieldDumpFD, FieldDumpTD, Probe
+from .processing import FieldDumpFD, FieldDumpTD, Probe, make_probe
 
 
 class Simulation:
@@ -38,7 +38,7 @@ class Simulation:
         if graph is not None:
             self.eng.set_graph(graph)
         os_ = int(scene.get("oversampling", 4))
-        self.probes = [Probe(self.op, p, self.nrts, os_) for p in scene.get("probes", [])]
+        self.probes = [make_probe(self.op, p, self.nrts, os_) for p in scene.get("probes", [])]
         self.dumps = [FieldDumpTD(self.op, d, self.nrts, os_) if d.get("domain") == "td"
                       else FieldDumpFD(self.op, d, self.nrts) for d in scene.get("dumps", [])]
         for p in self.probes:

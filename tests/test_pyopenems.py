@@ -235,3 +235,51 @@ def test_msl_port_beta_and_impedance_of_a_travelling_wave(tmp_path):
     assert np.allclose(np.abs(p1.Z_ref), Z0, rtol=2e-2)
     assert np.all(np.abs(p1.uf_ref) < 0.03 * np.abs(p1.uf_inc))
     assert isinstance(p1, MSLPort)
+
+
+def _wg_setup(nrts=12000):
+    """A WR-90 guide (22.86 x 10.16 mm, 100 mm long, PEC walls as the mesh
+    boundary, PML_8 at both ends) with TE10 ports at z = 10..15 (fed) and
+    z = 90..85, as the reference's rectangular waveguide tutorial sets them up."""
+    a, b, length = 22.86, 10.16, 100.0
+    FDTD = openEMS(NrTS=nrts, EndCriteria=1e-5)
+    FDTD.SetGaussExcite(10e9, 3e9)
+    FDTD.SetBoundaryCond([0, 0, 0, 0, 3, 3])
+    CSX = ContinuousStructure()
+    FDTD.SetCSX(CSX)
+    mesh = CSX.GetGrid()
+    mesh.SetDeltaUnit(1e-3)
+    mesh.SetLines("x", np.linspace(0, a, 24))
+    mesh.SetLines("y", np.linspace(0, b, 11))
+    mesh.SetLines("z", np.linspace(0, length, 101))
+    p1 = FDTD.AddRectWaveGuidePort(0, [0, 0, 10], [a, b, 15], "z", a * 1e-3, b * 1e-3, "TE10", 1)
+    p2 = FDTD.AddRectWaveGuidePort(1, [0, 0, 90], [a, b, 85], "z", a * 1e-3, b * 1e-3, "TE10")
+    return FDTD, p1, p2
+
+
+def test_rect_waveguide_port_structure_and_mode_template(native):
+    """ports.RectWGPort (TE10): an excitation weighted by the mode's E pattern
+    in the plane z = 10, mode-matching probes (types 10 and 11) in the plane
+    z = 15; the probe's template is the TE10 pattern sin(pi x / a) on E_y
+    (zero on E_x), normalised so that sum mode^2 * area = 1 over the plane
+    without its boundary lines."""
+    from openems_amd.operator import Operator
+    from openems_amd.processing import ModeMatchProbe, evaluate_function
+    FDTD, p1, p2 = _wg_setup()
+    sc = FDTD.scene()
+    exc = sc["excitations"]
+    assert len(exc) == 1 and exc[0]["val"] == (1.0, 1.0, 0.0) and exc[0]["start"][2] == exc[0]["stop"][2] == 10
+    assert exc[0]["weight"][0] == "0" and "sin(" in exc[0]["weight"][1]
+    pr = {p["name"]: p for p in sc["probes"]}
+    assert pr["port_ut_0"]["type"] == 10 and pr["port_it_0"]["type"] == 11
+    assert pr["port_it_1"]["weight"] == -1 and pr["port_ut_1"]["start"][2] == 85
+    assert np.isclose(evaluate_function("sin(pi*x/2)*y + rho", "x,y,rho", np.array([[1.0, 3.0, 2.0]]))[0], 5.0)
+    op = Operator(sc, nthreads=2)
+    mm = ModeMatchProbe(op, pr["port_ut_0"], 100)
+    assert mm.enabled and mm.ny == 2 and mm.start[0] == 1 and mm.stop[0] == 22
+    total = np.sum(mm.mode ** 2 * mm.area)
+    assert np.isclose(total, 1.0, rtol=1e-12)
+    assert np.all(mm.mode[0] == 0) and np.any(mm.mode[1] != 0)   # templates of E_x, E_y; TE10 has E_y only
+    x = np.array([op.disc_line(0, i) for i in range(1, 23)])
+    prof = np.abs(mm.mode).sum(axis=(0, 2))
+    assert np.allclose(prof / prof.max(), np.sin(np.pi * x / 22.86) / np.sin(np.pi * x / 22.86).max(), atol=1e-9)
