@@ -361,3 +361,20 @@ def test_tilted_cylinders_and_weighted_boxes(native):
     bare = dict(sc, metals=[], excitations=[])
     zeros = lambda o: np.count_nonzero(o.coeff("vv") == 0)
     assert zeros(op) > zeros(Operator(bare, nthreads=2)) + 100
+
+
+def test_eval_function_checks_its_arguments(native):
+    """emsop_eval_function: values at several points, NaN where the
+    expression has no value, and errors for a malformed expression, an
+    unknown variable and a variable list that does not match nvar."""
+    from openems_amd._native import NativeError
+    from openems_amd.processing import evaluate_function
+    v = np.array([[0.0, 2.0], [1.0, -1.0], [2.0, 0.5]])
+    got = evaluate_function("x^2 + y*pi - min(x, y) + if(x>1, 10, 0)", "x, y", v)
+    want = [x ** 2 + y * np.pi - min(x, y) + (10 if x > 1 else 0) for x, y in v]
+    assert np.array_equal(got, want)
+    assert np.isnan(evaluate_function("sqrt(x)", "x", np.array([[-1.0]]))[0])
+    for text, names, msg in (("x+", "x", "unexpected end"), ("q*2", "x", "unknown name"),
+                             ("x", "x,y,z", "variable list")):
+        with pytest.raises(NativeError, match=msg):
+            evaluate_function(text, names, np.zeros((1, 1)))
