@@ -310,6 +310,7 @@ class LumpedPort:
         prefix = kw.get("PortNamePrefix", "")
         lbl = prefix + "port_{}" + "_{}".format(port_nr)
         self.Z_ref = None
+        self.port_props = []
         ny = _dir(exc_dir)
         self.exc_ny = ny
         if self.start[ny] == self.stop[ny]:
@@ -323,7 +324,9 @@ class LumpedPort:
         if excite != 0:
             v = np.zeros(3)
             v[ny] = -1 * self.direction * excite
-            CSX.AddExcitation(lbl.format("excite"), exc_type=0, exc_val=v, delay=self.delay).AddBox(
+            self.port_props.append(CSX.AddExcitation(lbl.format("excite"), exc_type=0, exc_val=v,
+                                                     delay=self.delay))
+            self.port_props[-1].AddBox(
                 self.start, self.stop, priority=self.priority)
         self.U_filenames = [lbl.format("ut")]
         us, ue = 0.5 * (self.start + self.stop), 0.5 * (self.start + self.stop)
@@ -333,6 +336,17 @@ class LumpedPort:
         i_s, i_e = self.start.copy(), self.stop.copy()
         i_s[ny] = i_e[ny] = 0.5 * (self.start[ny] + self.stop[ny])
         CSX.AddProbe(self.I_filenames[0], p_type=1, weight=self.direction, norm_dir=ny).AddBox(i_s, i_e)
+
+    def SetEnabled(self, val):
+        """Switches the port's excitation on or off (ports.Port.SetEnabled,
+        python/openEMS/ports.py:86-96); enabling a port without one raises."""
+        found = False
+        for prop in self.port_props:
+            if prop.kind == "excitation":
+                prop.SetEnabled(val)
+                found = True
+        if not found and val:
+            raise ValueError("Unable to enable port! No excitation found!")
 
     def ReadUIData(self, sim_path, freq, signal_type="pulse"):
         self.u_data = UI_data(self.U_filenames, sim_path, freq, signal_type)
@@ -396,6 +410,7 @@ class WaveguidePort(LumpedPort):
         self.priority = kw.get("priority", 0)
         self.delay = kw.get("delay", 0.0)
         self.Z_ref = None
+        self.port_props = []
         lbl = kw.get("PortNamePrefix", "") + "port_{}" + "_{}".format(port_nr)
         ny = _dir(exc_dir)
         self.exc_ny, self.ny_P, self.ny_PP = ny, (ny + 1) % 3, (ny + 2) % 3
@@ -411,6 +426,7 @@ class WaveguidePort(LumpedPort):
             v = np.ones(3)
             v[ny] = 0
             exc = CSX.AddExcitation(lbl.format("excite"), exc_type=0, exc_val=v, delay=self.delay)
+            self.port_props.append(exc)
             exc.SetWeightFunction([str(x) for x in self.E_func])
             exc.AddBox(a, b, priority=self.priority)
         a, b = self.start.copy(), self.stop.copy()
@@ -482,6 +498,7 @@ class MSLPort(LumpedPort):
         self.priority = kw.get("priority", 0)
         self.delay = kw.get("delay", 0.0)
         self.Z_ref = None
+        self.port_props = []
         lbl = kw.get("PortNamePrefix", "") + "port_{}" + "_{}".format(port_nr)
         ex, pr = _dir(exc_dir), _dir(prop_dir)
         self.exc_ny, self.prop_ny = ex, pr
@@ -531,7 +548,9 @@ class MSLPort(LumpedPort):
             a[pr] = b[pr] = lines[kf]
             v = np.zeros(3)
             v[ex] = -1 * self.upside_down * excite
-            CSX.AddExcitation(lbl.format("excite"), exc_type=0, exc_val=v, delay=self.delay).AddBox(
+            self.port_props.append(CSX.AddExcitation(lbl.format("excite"), exc_type=0, exc_val=v,
+                                                     delay=self.delay))
+            self.port_props[-1].AddBox(
                 a, b, priority=self.priority)
         if self.feed_R >= 0 and not np.isinf(self.feed_R):
             a, b = self.start.copy(), self.stop.copy()

@@ -283,3 +283,20 @@ def test_rect_waveguide_port_structure_and_mode_template(native):
     x = np.array([op.disc_line(0, i) for i in range(1, 23)])
     prof = np.abs(mm.mode).sum(axis=(0, 2))
     assert np.allclose(prof / prof.max(), np.sin(np.pi * x / 22.86) / np.sin(np.pi * x / 22.86).max(), atol=1e-9)
+
+
+def test_port_set_enabled():
+    """Port.SetEnabled switches a port's excitation: a disabled fed port adds
+    no excitation to the scene; enabling a port without one raises."""
+    FDTD, p1, p2 = _msl_setup()
+    assert len(FDTD.scene()["excitations"]) == 1
+    p1.SetEnabled(False)
+    assert FDTD.scene()["excitations"] == []
+    p1.SetEnabled(True)
+    assert len(FDTD.scene()["excitations"]) == 1
+    with pytest.raises(ValueError, match="No excitation"):
+        p2.SetEnabled(True)
+    p2.SetEnabled(False)
+    FDTD, w1, _ = _wg_setup()
+    w1.SetEnabled(False)
+    assert FDTD.scene()["excitations"] == []
