@@ -126,6 +126,56 @@ def copy_calibration(reps=20, nbytes=1 << 30):
                                            "launches": reps}}), flush=True)
 
 
+def hbm_probe(reads, writes, nontemporal, bytes_per_array, reps=20):
+    """GB/s of a streaming kernel reading `reads` and writing `writes` float
+    arrays of `bytes_per_array` each, 16 B per lane (libems_hbm_probe.so)."""
+    import ctypes as C
+    from openems_amd import build
+    if not os.path.exists(build.PROBE_LIB):
+        raise RuntimeError(f"{build.PROBE_LIB} is not built (run __graft_entry__.build())")
+    lib = C.CDLL(build.PROBE_LIB)
+    lib.ems_hbm_probe.argtypes = [C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.c_int,
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    lib.ems_hbm_probe_error.restype = C.c_char_p
+    gbs, ms = C.c_double(), C.c_double()
+    rc = lib.ems_hbm_probe(reads, writes, int(nontemporal), int(bytes_per_array), reps, C.byref(gbs), C.byref(ms))
+    if rc != 0:
+        raise RuntimeError(lib.ems_hbm_probe_error().decode())
+    return gbs.value, ms.value
+
+
+HBM_PROBE_SWEEP = [  # (reads, writes): read-only, write-only, copy, and the update kernels' mixes
+    (1, 0), (4, 0), (0, 1), (0, 2), (1, 1), (2, 1), (7, 3), (4, 3)]
+
+
+def hbm_probe_sweep(sizes_gib=(1, 2, 4, 8)):
+    """Bandwidth of the streaming probe over array sizes and read/write mixes,
+    with and without non-temporal hints; one JSON line per case."""
+    rows = []
+    for gib in sizes_gib:
+        for r, w in HBM_PROBE_SWEEP:
+            for nt in (0, 1):
+                per = int(gib * (1 << 30) / (r + w)) & ~0xFFFF     # total footprint = gib GiB
+                gbs, ms = hbm_probe(r, w, nt, per)
+                row = {"hbm_probe": {"reads": r, "writes": w, "nontemporal": nt, "total_gib": gib,
+                                     "bytes_per_launch": per * (r + w), "ms": round(ms, 4), "GBs": round(gbs, 1)}}
+                print(json.dumps(row), flush=True)
+                rows.append(row["hbm_probe"])
+    return rows
+
+
+def achievable_bandwidth(total_bytes=3 << 30):
+    """Best streaming rate of the probe at the update kernels' read/write mix
+    (7 arrays read, 3 written: a half step reads 2.2-2.4 bytes per byte it
+    writes) and for a plain copy, with the footprint of the 256^3 fields."""
+    best = {}
+    for name, (r, w) in (("mix_7r3w", (7, 3)), ("copy", (1, 1))):
+        per = int(total_bytes / (r + w)) & ~0xFFFF
+        res = [hbm_probe(r, w, nt, per) for nt in (0, 1)]
+        best[name] = round(max(g for g, _ in res), 1)
+    return best
+
+
 def _field_digest(eng):
     import hashlib
     h = hashlib.blake2b(digest_size=16)
@@ -321,9 +371,14 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=8000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (for scaling HBM byte counters)")
+    ap.add_argument("--hbm-probe", action="store_true",
+                    help="only run the streaming bandwidth probe sweep (1-8 GiB footprints, read/write mixes)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.hbm_probe:
+        hbm_probe_sweep()
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "

@@ -56,7 +56,24 @@ def build(verbose=False):
     for o in objs:
         os.remove(o)
     build_mirror_check(verbose)
+    build_hbm_probe(verbose)
     return LIB
+
+
+PROBE_SRC = os.path.join(HERE, "csrc", "bench", "ems_hbm_probe.hip")
+PROBE_LIB = os.path.join(LIBDIR, "libems_hbm_probe.so")
+
+
+def build_hbm_probe(verbose=False):
+    """The HBM bandwidth probe bench.py reports beside the roofline (a library of
+    its own: measurement, not part of the engine's ABI)."""
+    tmp = PROBE_LIB + ".tmp"
+    cmd = [HIPCC] + device_flags() + ["-shared", PROBE_SRC, "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, stderr=None if verbose else subprocess.PIPE)
+    os.replace(tmp, PROBE_LIB)
+    return PROBE_LIB
 
 
 MIRROR_SRC = os.path.join(ROOT, "tests", "cpp", "engine_mirror.cpp")

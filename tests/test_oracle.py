@@ -662,3 +662,200 @@ def test_custom_signal_reduces_to_the_builtin_sinusoid():
     f = custom_function("-t^2^0.5 + (t>1)*(t<3) + (t>=2 & t<=2) + if(t=4, 10, 0)")
     assert f(4.0) == -(4.0 ** (2 ** 0.5)) + 0 + 0 + 10
     assert f(2.0) == -(2.0 ** (2 ** 0.5)) + 1 + 1
+
+
+# -- lumped R||C port: network identities (FDTD/operator.cpp:1586-1775) ---------------------
+def _port_scene(lines, start, stop, R=50.0, Cap=0.0, caps=True, with_port=True):
+    s = {"lines": lines, "delta_unit": 1e-3, "bc": [0] * 6,
+         "signal": {"type": "gauss", "f0": 0.0, "fc": 20e9}, "nrts": 10,
+         "excitations": [{"type": 0, "val": (0.0, 0.0, -1.0), "start": start, "stop": stop}]}
+    if with_port:
+        s["lumped"] = [{"dir": 2, "R": R, "C": Cap, "L": 0.0, "caps": caps, "type": "parallel",
+                        "start": start, "stop": stop}]
+    return s
+
+
+def _port_geometry(n=24, graded=True):
+    # graded: seeded random cell widths in [0.3, 0.9] mm, so neither the
+    # primary nor the dual edge lengths of neighbouring lines are in a common ratio
+    if graded:
+        rng = np.random.default_rng(7)
+        lines = [np.concatenate([[0.0], np.cumsum(rng.uniform(0.3, 0.9, n - 1))]) for _ in range(3)]
+    else:
+        lines = [np.arange(n) * 0.5 for _ in range(3)]
+    c = n // 2
+    start = (lines[0][c - 1], lines[1][c - 1], lines[2][c - 2])
+    stop = (lines[0][c], lines[1][c], lines[2][c + 2])
+    box = ((c - 1, c), (c - 1, c), (c - 2, c + 2))       # line ranges of the snapped box
+    return lines, start, stop, box
+
+
+def _network(edge_values, box, series):
+    """Series/parallel combination of the port's z edges: per layer (z edge) the
+    parallel sum over its (x, y) lines, then the layers in series; `series`
+    says how values combine in series (resistances add, conductances and
+    capacitances add reciprocally)."""
+    (x0, x1), (y0, y1), (z0, z1) = box
+    layers = [sum(edge_values[x, y, z] for x in range(x0, x1 + 1) for y in range(y0, y1 + 1))
+              for z in range(z0, z1)]
+    if series == "add":
+        return sum(layers)
+    return 1.0 / sum(1.0 / g for g in layers)
+
+
+@pytest.mark.parametrize("graded", [False, True])
+def test_lumped_port_network_conductance_equals_one_over_R(graded):
+    """The port's per-edge conductances G = kappa*A/delta, kappa = 1/(R*unitGC),
+    combined as the circuit they form (the four z edges of a layer in
+    parallel, the four layers in series) give exactly 1/R, on a uniform and on
+    a randomly graded mesh, where G*delta/A is one conductivity over the
+    block; on the uniform mesh every edge carries 1/R (4 in
+    parallel, 4 in series).  The same identity holds for the DC resistance
+    vi/(1-vv) of each edge's float32 update coefficients (the steady state of
+    V <- vv*V + vi*curl is V = curl*vi/(1-vv)), which is what the engine and
+    a voltage probe across the port see: R within float32 rounding."""
+    lines, start, stop, box = _port_geometry(graded=graded)
+    op = RefOperator(_port_scene(lines, start, stop)).build()
+    G = op.EC[1][2].astype(np.float64)
+    assert _network(G, box, "reciprocal") == pytest.approx(1 / 50.0, rel=1e-6)
+    # the port is a homogeneous resistive block: G*delta/A is one conductivity
+    # kappa over all its edges, with A the edge's dual face (half-way to the
+    # neighbouring lines on both sides, from the mesh lines) and delta its length
+    (x0, x1), (y0, y1), (z0, z1) = box
+    lm = [np.asarray(l) * 1e-3 for l in lines]
+    dual = [lambda i, l=l: 0.5 * (l[i + 1] - l[i - 1]) for l in lm]
+    kap = [G[x, y, z] * (lm[2][z + 1] - lm[2][z]) / (dual[0](x) * dual[1](y))
+           for x in range(x0, x1 + 1) for y in range(y0, y1 + 1) for z in range(z0, z1)]
+    np.testing.assert_allclose(kap, kap[0], rtol=1e-6)
+    if not graded:
+        (x0, x1), (y0, y1), (z0, z1) = box
+        np.testing.assert_allclose(G[x0:x1 + 1, y0:y1 + 1, z0:z1], 1 / 50.0, rtol=1e-6)
+    vv = op.vv[2].astype(np.float64)
+    vi = op.vi[2].astype(np.float64)
+    r_edge = vi / np.where(vv != 1.0, 1.0 - vv, 1.0)
+    # resistances: parallel combine reciprocally inside a layer, layers add in series
+    (x0, x1), (y0, y1), (z0, z1) = box
+    layers = [1.0 / sum(1.0 / r_edge[x, y, z] for x in range(x0, x1 + 1) for y in range(y0, y1 + 1))
+              for z in range(z0, z1)]
+    assert sum(layers) == pytest.approx(50.0, rel=1e-5)
+
+
+def test_lumped_port_capacitance_and_footprint():
+    """A port with a capacitance: the edges' EC capacitances form a network of
+    exactly C (replacing the material's); and the operator differs from the
+    same scene without the port only on the port's z edges and on the
+    tangential edges of its two end caps (x edges with x below the box's last
+    x line, y edges likewise, on the first and last z line), which are PEC."""
+    lines, start, stop, box = _port_geometry(graded=True)
+    Cap = 1e-12
+    op = RefOperator(_port_scene(lines, start, stop, Cap=Cap)).build()
+    Cz = op.EC[0][2].astype(np.float64)
+    assert _network(Cz, box, "reciprocal") == pytest.approx(Cap, rel=1e-6)
+    ref = RefOperator(_port_scene(lines, start, stop, with_port=False)).build()
+    diff = (op.vv != ref.vv) | (op.vi != ref.vi)
+    (x0, x1), (y0, y1), (z0, z1) = box
+    want = np.zeros_like(diff)
+    want[2, x0:x1 + 1, y0:y1 + 1, z0:z1] = True
+    caps = np.zeros_like(diff)
+    for z in (z0, z1):
+        caps[0, x0:x1, y0:y1 + 1, z] = True
+        caps[1, x0:x1 + 1, y0:y1, z] = True
+    assert np.array_equal(diff, want | caps)
+    assert not op.vv[caps].any() and not op.vi[caps].any()
+
+
+# -- frequency-domain dump: running DFT closed form (Common/processfields_fd.cpp:72-107) -----
+@pytest.mark.parametrize("dump_type", [0, 1])
+def test_fd_dump_running_dft_amplitude_and_phase(dump_type):
+    """A field A*cos(2 pi f0 t + phi) sampled at the dump's own times (every
+    FD_Interval steps, t = ts*dT for E and (ts + 1/2)*dT for H) over a whole
+    number of periods accumulates F(f0) = A*exp(i*phi)*T_window (the
+    single-sided spectrum: the x2 of the DFT weight times the dT*FD_Interval
+    integration step), and F(2 f0) = 0; a sign error in the exponent turns
+    the phase to -phi, a dropped x2 or interval halves or shrinks |F|."""
+    from oracle.oracle import RefFieldDumpFD
+    op = RefOperator(_uniform_scene(8)).build()
+    d = RefFieldDumpFD(op, dump_type, (1, 1, 4), (3, 3, 4), [10e9], interp=0)
+    assert d.fd_interval > 0
+    Ts = d.fd_interval * op.dT
+    M, P = 16, 8                          # samples per period, periods
+    f0 = 1.0 / (M * Ts)
+    d.freqs = [f0, 2 * f0]
+    shape = (2, 3) + tuple(len(p) for p in d.pos)
+    d.fd_re = np.zeros(shape, np.float32)
+    d.fd_im = np.zeros(shape, np.float32)
+    A, phi = 0.75, 0.6
+    half = 0.5 if dump_type == 1 else 0.0
+    cur = {}
+    d.field = lambda v, c: cur["F"]
+    K = M * P
+    for k in range(K):
+        ts = k * d.fd_interval
+        t = (ts + half) * op.dT
+        cur["F"] = np.full(shape[1:], A * math.cos(2 * math.pi * f0 * t + phi), np.float32)
+        d.process(ts, None, None)
+    T = K * Ts
+    F0 = complex(float(d.fd_re[0].mean()), float(d.fd_im[0].mean()))
+    assert abs(F0 - A * T * complex(math.cos(phi), math.sin(phi))) < 1e-5 * A * T
+    F1 = complex(float(d.fd_re[1].mean()), float(d.fd_im[1].mean()))
+    assert abs(F1) < 1e-5 * A * T
+
+
+# -- Mur ABC: normal-incidence reflection of the discrete plane wave ----------------------
+def _mur_reflection_closed_form(f, dT, delta):
+    """First-order Mur (engine_ext_mur_abc.cpp:68-178) applied to the 1-D Yee
+    wave: with z = exp(i w dT), p = exp(i k delta), coef a = (S-1)/(S+1),
+    S = c dT/delta, and k from the lattice dispersion sin(w dT/2) = S sin(k delta/2),
+    the face update V0' = V1 + a*(V1' - V0) taken on the incident z^n p^m and
+    reflected R z^n p^-m waves gives
+        R = (p + a z p - a - z) / (z - 1/p - a z/p + a)."""
+    S = C0 * dT / delta
+    a = (S - 1) / (S + 1)
+    w = 2 * math.pi * f
+    k = 2 * math.asin(math.sin(w * dT / 2) / S) / delta
+    z = complex(math.cos(w * dT), math.sin(w * dT))
+    p = complex(math.cos(k * delta), math.sin(k * delta))
+    return abs((p + a * z * p - a - z) / (z - 1 / p - a * z / p + a))
+
+
+def test_mur_normal_incidence_reflection_matches_the_discrete_closed_form(oracle_lib):
+    """A TEM plane wave (E_x, PEC on the x faces, PMC on the y faces, so the
+    interior rows reduce exactly to the 1-D Yee scheme) from a soft sheet
+    source hits a Mur face at z = 200 mm.  The reflected pulse (the probe
+    signal minus that of a 4x longer guide, which has no reflection in the
+    window) over the incident spectrum equals the closed-form |R(f)| of the
+    discrete scheme within 2 % from 1 to 20 GHz (|R| 2e-5 .. 7.5e-3), which
+    a wrong sign of the coefficient, a swapped face/shift line or a hook run
+    at the wrong half step would miss by orders of magnitude."""
+    nx, ny, zs, zp, steps = 4, 6, 20, 100, 900
+
+    def run(nz):
+        lx, ly, lz = np.arange(nx) * 1.0, np.arange(ny) * 1.0, np.arange(nz) * 1.0
+        s = {"lines": [lx, ly, lz], "delta_unit": 1e-3, "bc": [0, 0, 1, 1, 2, 2],
+             "signal": {"type": "gauss", "f0": 0.0, "fc": 20e9}, "nrts": steps,
+             "excitations": [{"type": 0, "val": (1.0, 0, 0), "start": (0, 0, zs), "stop": (nx - 1, ny - 1, zs)}]}
+        op = RefOperator(s).build()
+        e = RefEngine(op, "basic")
+        out = []
+        for _ in range(steps):
+            e.iterate(1)
+            v, _ = e.raw_views()
+            out.append(v[0, :nx - 1, 1:ny - 1, zp].copy())
+        e.close()
+        return op, np.array(out)
+
+    op, short = run(201)
+    _, long_ = run(801)
+    # the interior rows carry one uniform wave (exact 1-D reduction)
+    assert np.array_equal(short, np.broadcast_to(short[:, :1, :1], short.shape))
+    inc = long_[:, 0, 0].astype(np.float64)
+    refl = short[:, 0, 0].astype(np.float64) - inc
+    assert np.abs(inc[-50:]).max() < 1e-3 * np.abs(inc).max()       # both pulses inside the window
+    nf = 1 << 16
+    Fi, Fr = np.fft.rfft(inc, nf), np.fft.rfft(refl, nf)
+    f = np.fft.rfftfreq(nf, op.dT)
+    for fq in (1e9, 2e9, 5e9, 10e9, 15e9, 20e9):
+        i = int(np.argmin(np.abs(f - fq)))
+        want = _mur_reflection_closed_form(f[i], op.dT, 1e-3)
+        got = abs(Fr[i]) / abs(Fi[i])
+        assert abs(got / want - 1) < 0.02, (fq, got, want)
