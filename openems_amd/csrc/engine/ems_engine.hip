@@ -2889,6 +2889,7 @@ static int launch_samples(ems_engine* h)
     if (!h->probes.empty() && probe_due(h, h->numTS)) {
         hipLaunchKernelGGL(k_probe, dim3((unsigned)h->probes.size()), dim3(kBlock), 0, h->stream, h->V, h->I,
                            h->d_probes, h->d_elin, h->d_esign, h->d_ediv, h->numts_dev);
+        HIPCHK(hipGetLastError());
     }
     for (size_t k = 0; k < h->dumps.size(); ++k) {
         const DumpDev& d = h->dumps[k];
@@ -2897,6 +2898,7 @@ static int launch_samples(ems_engine* h)
         const unsigned npts = h->dump_npts[k];
         hipLaunchKernelGGL(k_fd_dump, dim3((npts + kBlock - 1) / kBlock), dim3(kBlock), 0, h->stream,
                            d.field ? h->I : h->V, h->cs, h->N[0], h->g.plane, d, h->numts_dev);
+        HIPCHK(hipGetLastError());
     }
     return 0;
 }
@@ -3054,11 +3056,12 @@ extern "C" {
 
 // one timestep in the reference's hook order (Engine::IterateTS, FDTD/engine.cpp:267-286,
 // priorities: UPML > default (lumped RLC, Lorentz, Mur) > excitation)
-static void launch_step(ems_engine* h, hipEvent_t* ev)
+static int launch_step(ems_engine* h, hipEvent_t* ev)
 {
     if (h->fused) {
         fused_step(h, ev);
-        return;
+        HIPCHK(hipGetLastError());
+        return 0;
     }
     const GridDesc& g = h->g;
     const unsigned nblk = g.qtiles * g.nzc;
@@ -3120,6 +3123,8 @@ static void launch_step(ems_engine* h, hipEvent_t* ev)
         hipLaunchKernelGGL(k_excite, dim3((e.G + T - 1) / T), dim3(T), 0, h->stream, h->I, e.G, e.gbeg, e.glin, e.amp,
                            e.delay, h->sig[1], h->sigLen, h->period, h->numts_dev, -1);
     }
+    HIPCHK(hipGetLastError());
+    return 0;
 }
 
 int ems_set_profiling(ems_engine* h, int enable)
@@ -3193,21 +3198,27 @@ static int graph_chunk(ems_engine* h, unsigned steps)
     if (h->graphs.size() >= kMaxGraphs) {
         // too many distinct patterns: direct launches
         for (unsigned k = 0; k < steps; ++k) {
-            launch_step(h, nullptr);
+            int rc = launch_step(h, nullptr);
             ++h->numTS;
-            launch_samples(h);
+            if (!rc) rc = launch_samples(h);
+            if (rc) return rc;
         }
         return 0;
     }
     ems_engine::GraphEnt e;
     e.key = key;
     HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int lrc = 0;
     for (unsigned k = 0; k < steps; ++k) {
-        launch_step(h, nullptr);
+        if (!lrc) lrc = launch_step(h, nullptr);
         ++h->numTS;
-        launch_samples(h);
+        if (!lrc) lrc = launch_samples(h);
     }
     const hipError_t ec = hipStreamEndCapture(h->stream, &e.graph);
+    if (lrc) {
+        if (ec == hipSuccess && e.graph) (void)hipGraphDestroy(e.graph);
+        return lrc;
+    }
     if (ec != hipSuccess) return fail(EMS_E_HIP, std::string("graph capture failed: ") + hipGetErrorString(ec));
     HIPCHK(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
     HIPCHK(hipGraphLaunch(e.exec, h->stream));
@@ -3260,15 +3271,16 @@ int ems_iterate(ems_engine* h, unsigned nsteps)
     }
     if (h->numTS == 0 && nsteps > 0) {
         // Process() before the first step (openems.cpp:1424-1425)
-        launch_samples(h);
+        if ((rc = launch_samples(h))) return rc;
         h->sampled0 = true;
     }
     HIPCHK(hipEventRecord(h->ev0, h->stream));
     unsigned s = 0;
     for (; s < nprof; ++s) {
-        launch_step(h, &h->kev[4 * (size_t)s]);
+        rc = launch_step(h, &h->kev[4 * (size_t)s]);
         ++h->numTS;
-        launch_samples(h);
+        if (!rc) rc = launch_samples(h);
+        if (rc) return rc;
     }
     if (h->graph_chunk) {
         const unsigned steps = graph_steps(h);
@@ -3282,9 +3294,10 @@ int ems_iterate(ems_engine* h, unsigned nsteps)
         }
     }
     for (; s < nsteps; ++s) {
-        launch_step(h, nullptr);
+        rc = launch_step(h, nullptr);
         ++h->numTS;
-        launch_samples(h);
+        if (!rc) rc = launch_samples(h);
+        if (rc) return rc;
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(h->ev1, h->stream));
@@ -3866,7 +3879,7 @@ int ems_batch_autotune(ems_batch* b, unsigned steps, double* best_ms)
     std::vector<Saved> sv(b->eng.size());
     std::vector<void*> tmp;
     auto cleanup = [&]() {
-        for (void* p : tmp) hipFree(p);
+        for (void* p : tmp) (void)hipFree(p);
     };
     auto dalloc = [&](float** p, size_t n) -> bool {
         void* q = nullptr;
@@ -3874,6 +3887,20 @@ int ems_batch_autotune(ems_batch* b, unsigned steps, double* best_ms)
         tmp.push_back(q);
         *p = (float*)q;
         return true;
+    };
+    // every snapshot / restore copy is checked; the first failure is kept
+    hipError_t cerr = hipSuccess;
+    const char* cwhat = "";
+    auto cp = [&](void* dst, const void* src, size_t bytes, hipMemcpyKind kind, const char* what) {
+        const hipError_t e = hipMemcpy(dst, src, bytes, kind);
+        if (e != hipSuccess && cerr == hipSuccess) {
+            cerr = e;
+            cwhat = what;
+        }
+    };
+    auto copy_failed = [&](const char* phase) {
+        return fail(EMS_E_HIP, std::string("ems_batch_autotune: ") + phase + " copy of " + cwhat + " failed: " +
+                                   hipGetErrorString(cerr));
     };
     for (size_t i = 0; i < b->eng.size(); ++i) {
         ems_engine* e = b->eng[i];
@@ -3883,42 +3910,59 @@ int ems_batch_autotune(ems_batch* b, unsigned steps, double* best_ms)
             cleanup();
             return fail(EMS_E_NOMEM, "no device memory for the autotuner's snapshot");
         }
-        hipMemcpy(S.V, e->V, 3 * e->cs * sizeof(float), hipMemcpyDeviceToDevice);
-        hipMemcpy(S.I, e->I, 3 * e->cs * sizeof(float), hipMemcpyDeviceToDevice);
+        cp(S.V, e->V, 3 * e->cs * sizeof(float), hipMemcpyDeviceToDevice, "voltages");
+        cp(S.I, e->I, 3 * e->cs * sizeof(float), hipMemcpyDeviceToDevice, "currents");
         if (e->pmlV.n) {
             if (!dalloc(&S.fV, 3 * e->pmlV.npos) || !dalloc(&S.fI, 3 * e->pmlI.npos)) {
                 cleanup();
                 return fail(EMS_E_NOMEM, "no device memory for the autotuner's snapshot");
             }
-            hipMemcpy(S.fV, e->pmlV.flux_in, 3 * e->pmlV.npos * sizeof(float), hipMemcpyDeviceToDevice);
-            hipMemcpy(S.fI, e->pmlI.flux_in, 3 * e->pmlI.npos * sizeof(float), hipMemcpyDeviceToDevice);
+            cp(S.fV, e->pmlV.flux_in, 3 * e->pmlV.npos * sizeof(float), hipMemcpyDeviceToDevice, "voltage flux");
+            cp(S.fI, e->pmlI.flux_in, 3 * e->pmlI.npos * sizeof(float), hipMemcpyDeviceToDevice, "current flux");
         }
-        hipMemcpy(S.numts, e->numts_dev, 2 * sizeof(unsigned), hipMemcpyDeviceToHost);
+        cp(S.numts, e->numts_dev, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, "step counter");
         for (const auto& P : e->probes) {
             unsigned c = 0;
-            hipMemcpy(&c, P.count, sizeof(unsigned), hipMemcpyDeviceToHost);
+            cp(&c, P.count, sizeof(unsigned), hipMemcpyDeviceToHost, "probe count");
             S.counts.push_back(c);
         }
     }
-    HIPCHK(hipDeviceSynchronize());
-    auto restore = [&]() {
+    if (cerr != hipSuccess) {
+        cleanup();
+        return copy_failed("snapshot");
+    }
+    {
+        const hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) {
+            cleanup();
+            return fail(EMS_E_HIP, std::string("ems_batch_autotune: ") + hipGetErrorString(e));
+        }
+    }
+    // puts every instance back to the snapshot; returns 0 or an EMS_E_HIP
+    // status (the instances are then in an undefined state)
+    auto restore = [&]() -> int {
         for (size_t i = 0; i < b->eng.size(); ++i) {
             ems_engine* e = b->eng[i];
             Saved& S = sv[i];
-            hipMemcpy(e->V, S.V, 3 * e->cs * sizeof(float), hipMemcpyDeviceToDevice);
-            hipMemcpy(e->I, S.I, 3 * e->cs * sizeof(float), hipMemcpyDeviceToDevice);
+            cp(e->V, S.V, 3 * e->cs * sizeof(float), hipMemcpyDeviceToDevice, "voltages");
+            cp(e->I, S.I, 3 * e->cs * sizeof(float), hipMemcpyDeviceToDevice, "currents");
             if (e->pmlV.n) {
-                hipMemcpy(const_cast<float*>(e->pmlV.flux_in), S.fV, 3 * e->pmlV.npos * sizeof(float),
-                          hipMemcpyDeviceToDevice);
-                hipMemcpy(const_cast<float*>(e->pmlI.flux_in), S.fI, 3 * e->pmlI.npos * sizeof(float),
-                          hipMemcpyDeviceToDevice);
+                cp(const_cast<float*>(e->pmlV.flux_in), S.fV, 3 * e->pmlV.npos * sizeof(float),
+                   hipMemcpyDeviceToDevice, "voltage flux");
+                cp(const_cast<float*>(e->pmlI.flux_in), S.fI, 3 * e->pmlI.npos * sizeof(float),
+                   hipMemcpyDeviceToDevice, "current flux");
             }
-            hipMemcpy(e->numts_dev, S.numts, 2 * sizeof(unsigned), hipMemcpyHostToDevice);
+            cp(e->numts_dev, S.numts, 2 * sizeof(unsigned), hipMemcpyHostToDevice, "step counter");
             for (size_t k = 0; k < e->probes.size(); ++k)
-                hipMemcpy(e->probes[k].count, &S.counts[k], sizeof(unsigned), hipMemcpyHostToDevice);
+                cp(e->probes[k].count, &S.counts[k], sizeof(unsigned), hipMemcpyHostToDevice, "probe count");
         }
         // device-to-device copies are not ordered with the batch's stream
-        hipDeviceSynchronize();
+        const hipError_t se = hipDeviceSynchronize();
+        if (se != hipSuccess && cerr == hipSuccess) {
+            cerr = se;
+            cwhat = "(synchronisation)";
+        }
+        return cerr == hipSuccess ? 0 : copy_failed("restore");
     };
     const unsigned txs[2] = {0, 64};
     const unsigned depths[3] = {0, 4, 8};   // 0: each instance's own choice
@@ -3942,27 +3986,29 @@ int ems_batch_autotune(ems_batch* b, unsigned steps, double* best_ms)
                     int rc = batch_build(b, tx, zd, grid, plan.second);
                     if (!rc) rc = ems_batch_set_streams(b, plan.first);
                     if (rc) {
-                        restore();
+                        const int rr = restore();
                         cleanup();
-                        return rc;
+                        return rr ? rr : rc;
                     }
                     // one warm-up step of every group, then steps-1 timed steps of all groups
                     if ((rc = batch_run(b, 1))) {
-                        restore();
+                        const int rr = restore();
                         cleanup();
-                        return rc;
+                        return rr ? rr : rc;
                     }
-                    hipEventRecord(b->ev0, b->stream);
+                    hipError_t te = hipEventRecord(b->ev0, b->stream);
                     rc = batch_run(b, steps - 1);
-                    hipEventRecord(b->ev1, b->stream);
-                    hipEventSynchronize(b->ev1);
-                    if (rc) {
-                        restore();
-                        cleanup();
-                        return rc;
-                    }
+                    if (te == hipSuccess) te = hipEventRecord(b->ev1, b->stream);
+                    if (te == hipSuccess) te = hipEventSynchronize(b->ev1);
                     float ms = 0;
-                    hipEventElapsedTime(&ms, b->ev0, b->ev1);
+                    if (te == hipSuccess) te = hipEventElapsedTime(&ms, b->ev0, b->ev1);
+                    if (!rc && te != hipSuccess)
+                        rc = fail(EMS_E_HIP, std::string("ems_batch_autotune: timing: ") + hipGetErrorString(te));
+                    if (rc) {
+                        const int rr = restore();
+                        cleanup();
+                        return rr ? rr : rc;
+                    }
                     const double tot = ms / (steps - 1);
                     if (tot < best) {
                         best = tot;
@@ -3972,14 +4018,19 @@ int ems_batch_autotune(ems_batch* b, unsigned steps, double* best_ms)
                         bbud = plan.second;
                         bns = plan.first;
                     }
-                    restore();
+                    if ((rc = restore())) {
+                        cleanup();
+                        return rc;
+                    }
                 }
     std::vector<unsigned> zd(b->eng.size());
     for (size_t i = 0; i < zd.size(); ++i) zd[i] = bdep ? bdep : choose_zdepth(b->eng[i]->N);
     int rc = batch_build(b, btx, zd, bgrid, bbud);
     if (!rc) rc = ems_batch_set_streams(b, bns);
-    HIPCHK(hipStreamSynchronize(b->stream));
+    const hipError_t se = hipStreamSynchronize(b->stream);
     cleanup();
+    if (!rc && se != hipSuccess)
+        rc = fail(EMS_E_HIP, std::string("ems_batch_autotune: ") + hipGetErrorString(se));
     if (best_ms) *best_ms = best;
     return rc;
 }
@@ -4261,8 +4312,9 @@ int ems_adj_forward(ems_adjoint* a, unsigned nsteps, double* J)
         if ((rc = adj_save(a, a->ck[s]))) return rc;
         const unsigned len = std::min(a->K, nsteps - s * a->K);
         for (unsigned j = 0; j < len; ++j) {
-            launch_step(h, nullptr);
+            rc = launch_step(h, nullptr);
             ++h->numTS;
+            if (rc) return rc;
             hipLaunchKernelGGL(k_adj_dft, dim3((a->np + T - 1) / T), dim3(T), 0, h->stream, h->V, a->d_probes, a->np,
                                a->freq, a->dT, h->numts_dev, a->d_F);
         }
@@ -4307,8 +4359,9 @@ int ems_adj_backward(ems_adjoint* a, double* dJ_dvi, double* dJ_dvv)
         for (unsigned j = 0; j < len; ++j) {
             hipLaunchKernelGGL(k_adj_record, dim3(rblk), dim3(T), 0, h->stream, h->g, h->V, h->I, a->box,
                                a->hc + (size_t)j * 3 * a->box.n, a->hv + (size_t)j * 3 * a->box.n);
-            launch_step(h, nullptr);
+            rc = launch_step(h, nullptr);
             ++h->numTS;
+            if (rc) return rc;
         }
         // reverse steps of the segment
         for (int j = (int)len - 1; j >= 0; --j) {

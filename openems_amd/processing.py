@@ -34,10 +34,9 @@ PROBE_NAMES = {0: "voltage", 1: "current", 2: "E-field", 3: "H-field"}
 # ProcessFieldProbe (Common/processvoltage.h:30, processvoltage.cpp:30-35,
 # processcurrent.h:29, processcurrent.cpp:35-40, processfieldprobe.cpp:32-64)
 PROCESSING_NAMES = {0: "voltage integration", 1: "current integration", 2: "electric field probe",
-                    3: "magnetic field probe", 10: "mode matching", 11: "mode matching"}
+                    3: "magnetic field probe"}
 INTEGRAL_NAMES = {0: ["voltage"], 1: ["current"], 2: ["Ex/(V/m)", "Ey/(V/m)", "Ez/(V/m)"],
-                  3: ["Hx/(A/m)", "Hy/(A/m)", "Hz/(A/m)"], 10: ["voltage", "mode_purity"],
-                  11: ["current", "mode_purity"]}
+                  3: ["Hx/(A/m)", "Hy/(A/m)", "Hz/(A/m)"]}
 
 
 def _g(v, prec=6):
@@ -298,124 +297,6 @@ class Probe:
         return np.array(td) if td else np.zeros((0, 1 + nint)), fd
 
 
-class ModeMatchProbe(Probe):
-    """ProcessModeMatch (Common/processmodematch.cpp:41-266), probe type 10
-    (E, voltage) or 11 (H, current, sampled at the dual time): over a plane
-    box (snapped to the mesh, boundary lines excluded) the integral of the
-    node-interpolated tangential field against a normalised mode template,
-    value = sum field * mode * area, and the purity value^2 / sum field^2 * area.
-    The template comes from spec["mode_function"] (three expressions in
-    x, y, z, rho, a, r, t, drawing units) at the plane's nodes (dual lines for
-    H), NaN or inf taken as 0, divided by sqrt(sum mode^2 * area).  The field
-    is sampled on the device as a node-interpolated field dump of the plane;
-    the plane sums run at collection in the reference's order (posP, posPP,
-    component) in double."""
-
-    def __init__(self, op, spec, nrts, oversampling=4):
-        self.op = op
-        self.spec = spec
-        self.name = spec.get("name", "")
-        self.ptype = int(spec["type"])
-        self.weight = float(spec.get("weight", 1.0))
-        self.dual = self.ptype == 11
-        nyq = op.nyquist
-        self.interval = max(1, nyq // int(oversampling))
-        self.fd_interval = 0
-        self.freqs = []
-        for f in spec.get("freqs", []):
-            nts = nyquist_num(f, op.dT)
-            if nts == 0:
-                continue
-            if self.fd_interval == 0:
-                self.fd_interval = nyq
-            self.fd_interval = min(self.fd_interval, nts)
-            self.freqs.append(float(f))
-        self.start_ts, self.stop_ts = process_window(spec, op.dT)
-        _, start, stop, _, _ = op.snap_box(spec["start"], spec["stop"], False, 0)
-        N = op.N
-        lo = [min(a, b) for a, b in zip(start, stop)]
-        hi = [max(a, b) for a, b in zip(start, stop)]
-        for n in range(3):
-            if lo[n] == 0:
-                lo[n] += 1
-            if hi[n] == N[n] - 1:
-                hi[n] -= 1
-        flat = [n for n in range(3) if lo[n] == hi[n]]
-        self.enabled = len(flat) == 1 and all(lo[n] <= hi[n] for n in range(3))
-        self.start, self.stop = lo, hi
-        self.comp = []
-        self.nrts = nrts
-        self.steps = [ts for ts in range(nrts + 1) if self.due(ts)]
-        self.pid = self.did = None
-        if not self.enabled:
-            return
-        ny = flat[0]
-        self.ny = ny
-        nP, nPP = (ny + 1) % 3, (ny + 2) % 3
-        dual = self.ptype == 11
-        iP = np.arange(lo[nP], hi[nP] + 1)
-        iPP = np.arange(lo[nPP], hi[nPP] + 1)
-        disc = [None] * 3
-        disc[ny] = np.array([op.disc_line(ny, lo[ny], dual)])
-        disc[nP] = np.array([op.disc_line(nP, i, dual) for i in iP])
-        disc[nPP] = np.array([op.disc_line(nPP, i, dual) for i in iPP])
-        # node widths: GetNodeWidth(n, pos, dual) = GetEdgeLength(n, pos, !dual)
-        wP = np.array([op.edge_length(nP, i, not dual) for i in iP])
-        wPP = np.array([op.edge_length(nPP, i, not dual) for i in iPP])
-        self.area = wP[:, None] * wPP[None, :]
-        P, PP = np.meshgrid(np.arange(iP.size), np.arange(iPP.size), indexing="ij")
-        D = [None] * 3
-        D[ny] = np.full(P.shape, disc[ny][0])
-        D[nP] = disc[nP][P]
-        D[nPP] = disc[nPP][PP]
-        x, y, z = D
-        rho = np.sqrt(x * x + y * y)
-        var = np.stack([x, y, z, rho, np.arctan2(y, x), np.sqrt(x ** 2 + y ** 2 + z ** 2),
-                        np.arcsin(1.0) - np.arctan(z / np.where(rho == 0, np.nan, rho))], -1)
-        funcs = spec.get("mode_function") or ["0", "0", "0"]
-        self.mode = np.zeros((2,) + P.shape)
-        for k in range(2):
-            self.mode[k] = evaluate_function(str(funcs[(ny + k + 1) % 3]), "x,y,z,rho,a,r,t", var)
-        self.mode[~np.isfinite(self.mode)] = 0.0
-        norm = 0.0
-        for a in range(P.shape[0]):
-            for b in range(P.shape[1]):
-                for k in range(2):
-                    norm += self.mode[k, a, b] ** 2 * self.area[a, b]
-        self.mode /= math.sqrt(norm) if norm > 0 else 1.0
-
-    def register(self, eng):
-        if not self.enabled or not self.steps:
-            return
-        # the node-interpolated field of the plane at every multiple of the
-        # greatest common interval; collect keeps the due steps
-        every = math.gcd(self.interval, self.fd_interval) if self.fd_interval else self.interval
-        last = min(self.stop_ts, self.nrts)
-        cap = last // every - (self.start_ts - 1) // every if self.start_ts > 0 else last // every + 1
-        self.did, self.box = eng.add_td_dump(self.start, self.stop, every, field=int(self.ptype == 11), interp=1,
-                                             start_ts=self.start_ts, stop_ts=self.stop_ts, capacity=cap)
-
-    def collect(self, eng):
-        if self.did is None:
-            return np.zeros((0, 3)), np.zeros((2, len(self.freqs)), np.complex128)
-        ts, out = eng.td_dump_results(self.did, self.box)
-        ny = self.ny
-        comps = [(ny + 1) % 3, (ny + 2) % 3]
-        F = np.squeeze(np.asarray(out, np.float64), axis=2 + ny)          # [S][3][.][.], x, y, z without ny
-        if (ny + 1) % 3 > (ny + 2) % 3:
-            F = np.swapaxes(F, 2, 3)                                       # to (posP, posPP)
-        fld = np.stack([F[:, c] for c in comps], -1)                       # [S][posP][posPP][k]
-        mode = np.moveaxis(self.mode, 0, -1)[None]
-        ar = self.area[None, :, :, None]
-        S = len(ts)
-        # sums left to right in the order posP, posPP, k (np.cumsum is sequential)
-        value = np.cumsum(((fld * mode) * ar).reshape(S, -1), axis=1)[:, -1] if S else np.zeros(0)
-        purity = np.cumsum(((fld * fld) * ar).reshape(S, -1), axis=1)[:, -1] if S else np.zeros(0)
-        with np.errstate(divide="ignore", invalid="ignore"):
-            pur = np.where(purity != 0, value * value / np.where(purity != 0, purity, 1.0), 0.0)
-        return self._series(ts, np.stack([value, pur], -1), 2)
-
-
 def evaluate_function(text, names, var):
     """An expression of the function-parser syntax at every point of var
     [..., nvar], through the operator library's parser."""
@@ -432,8 +313,9 @@ def evaluate_function(text, names, var):
 
 def make_probe(op, spec, nrts, oversampling=4):
     """The processing object of a probe spec (openEMS::SetupProcessing, openems.cpp:520-575)."""
-    if int(spec["type"]) in (10, 11):
-        return ModeMatchProbe(op, spec, nrts, oversampling)
+    if int(spec["type"]) not in PROBE_NAMES:
+        raise ValueError(f"probe type {spec['type']} is not supported (mode matching, types 10 and 11, "
+                         "is outside the engine's scope)")
     return Probe(op, spec, nrts, oversampling)
 
 

@@ -185,10 +185,11 @@ class ContinuousStructure:
         return self._add("excitation", name, exc_type=int(exc_type), exc_val=[float(v) for v in exc_val],
                          delay=float(delay))
 
-    def AddProbe(self, name, p_type, weight=1.0, norm_dir=-1, frequency=None, mode_function=None, **kw):
+    def AddProbe(self, name, p_type, weight=1.0, norm_dir=-1, frequency=None, **kw):
+        """Probe types 0-3 (voltage, current, E and H point); the mode-matching
+        types 10 / 11 are outside the engine's scope and refused at Run."""
         return self._add("probe", name, p_type=int(p_type), weight=float(weight), norm_dir=int(norm_dir),
-                         frequency=list(np.atleast_1d(frequency)) if frequency is not None else [],
-                         mode_function=None if mode_function is None else [str(f) for f in mode_function])
+                         frequency=list(np.atleast_1d(frequency)) if frequency is not None else [])
 
     def AddDump(self, name, dump_type=0, dump_mode=1, file_type=1, frequency=None, **kw):
         return self._add("dump", name, dump_type=int(dump_type), dump_mode=int(dump_mode),
@@ -249,8 +250,6 @@ class ContinuousStructure:
                 elif p.kind == "probe":
                     probes.append({"type": P["p_type"], "name": p.name, "start": prim[2], "stop": prim[3],
                                    "weight": P["weight"], "freqs": P["frequency"], "norm_dir": P["norm_dir"]})
-                    if P.get("mode_function") is not None:
-                        probes[-1]["mode_function"] = [str(f) for f in P["mode_function"]]
                 elif p.kind == "dump":
                     dt = P["dump_type"]
                     if dt not in (0, 1, 10, 11):
@@ -269,7 +268,7 @@ class ContinuousStructure:
 # ----------------------------------------------------------------------------
 # ports
 # ----------------------------------------------------------------------------
-# python/openEMS/physical_constants.py
+# SI constants of the reference's python/openEMS/physical_constants.py
 C0 = 299792458.0
 MUE0 = 4e-7 * np.pi
 EPS0 = 1.0 / (MUE0 * C0 ** 2)
@@ -277,305 +276,273 @@ Z0 = np.sqrt(MUE0 / EPS0)
 
 
 class UI_data:
-    """Voltage / current probe files of a port read back (np.loadtxt with '%'
-    comments, as the reference's port processing reads them)."""
+    """The probe files of a port read back: per file the sample times, the
+    values and their single-sided spectrum at `freq` (DFT_time2freq).  The
+    files are the probe text files of the simulation ('%' comment lines, time
+    in the first column, value in the second)."""
 
     def __init__(self, fns, path, freq, signal_type="pulse"):
         self.fns = [fns] if isinstance(fns, str) else list(fns)
         self.freq = np.atleast_1d(freq)
-        self.ui_time, self.ui_val, self.ui_f_val = [], [], []
-        for fn in self.fns:
-            d = np.loadtxt(os.path.join(path, fn), comments="%")
-            self.ui_time.append(d[:, 0])
-            self.ui_val.append(d[:, 1])
-            self.ui_f_val.append(DFT_time2freq(d[:, 0], d[:, 1], self.freq, signal_type))
+        tables = [np.loadtxt(os.path.join(path, fn), comments="%") for fn in self.fns]
+        self.ui_time = [tab[:, 0] for tab in tables]
+        self.ui_val = [tab[:, 1] for tab in tables]
+        self.ui_f_val = [DFT_time2freq(tab[:, 0], tab[:, 1], self.freq, signal_type) for tab in tables]
 
 
-class LumpedPort:
-    """A lumped port (ports.LumpedPort): a resistor (or a metal short for
-    R = 0) over the port box, an optional soft excitation against the port
-    direction, a voltage probe along it (weight -1) and a current probe
-    across its middle; CalcPort splits the port voltage and current into
-    incident and reflected waves against Z_ref = R."""
+def line_waves(U, I, Z):
+    """Forward and backward waves at one plane of a line of characteristic
+    impedance Z, from the total voltage U and current I there.
 
-    def __init__(self, CSX, port_nr, R, start, stop, exc_dir, excite=0, **kw):
+    On the line U = U_f + U_b and Z I = U_f - U_b (the backward wave's current
+    flows against the reference direction).  Adding and subtracting,
+        U_f = (U + Z I) / 2,   U_b = (U - Z I) / 2,
+    and each wave carries the current of its own voltage over Z:
+        I_f = U_f / Z,   I_b = U_b / Z   (so I = I_f - I_b).
+    Works on spectra (complex Z) and on time series (real Z)."""
+    zi = Z * I
+    u_f = 0.5 * (U + zi)
+    u_b = 0.5 * (U - zi)
+    return u_f, u_b, u_f / Z, u_b / Z
+
+
+def move_plane(U, I, Z, theta):
+    """Total voltage and current a distance d further along a lossless line,
+    theta = beta * d: the forward wave is delayed by exp(-j theta), the
+    backward wave advanced by exp(+j theta); recombined,
+        U' = U cos(theta) - j Z I sin(theta),  I' = I cos(theta) - j (U / Z) sin(theta)."""
+    u_f, u_b, _, _ = line_waves(U, I, Z)
+    fwd = np.exp(-1j * theta)
+    u_f, u_b = u_f * fwd, u_b / fwd
+    return u_f + u_b, (u_f - u_b) / Z
+
+
+def line_from_three_planes(U_a, U_b, U_c, span_u, I_1, I_2, span_i):
+    """Propagation constant beta and characteristic impedance of a uniform
+    line from the spectra of three voltage probes (planes a, b, c in the
+    propagation direction, a to c `span_u` metres) and two current probes
+    (between a-b and b-c, `span_i` metres apart).
+
+    The telegrapher's equations at plane b,
+        dU/dz = -Z' I,   dI/dz = -Y' U,
+    with the central differences dU/dz = (U_c - U_a) / span_u,
+    dI/dz = (I_2 - I_1) / span_i and the plane's values U = U_b,
+    I = (I_1 + I_2) / 2, give the per-length impedance Z' and admittance Y'.
+    A forward wave exp(-gamma z) has gamma^2 = Z' Y' and impedance
+    sqrt(Z' / Y'); with gamma = j beta on a line of small loss:
+        beta = sqrt(-gamma^2) = sqrt(-(dU/dz)(dI/dz) / (U I)),
+        Z_L  = sqrt((dU/dz) U / ((dI/dz) I)),
+    beta on the branch with a non-negative real part (the forward wave)."""
+    I_b = 0.5 * (I_1 + I_2)
+    dU = (U_c - U_a) / span_u
+    dI = (I_2 - I_1) / span_i
+    beta = np.sqrt(-(dU * dI) / (U_b * I_b))
+    beta = np.where(np.real(beta) < 0, -beta, beta)
+    return beta, np.sqrt((dU * U_b) / (dI * I_b)), U_b, I_b
+
+
+def _half_power(u, i):
+    """Time-averaged power of phasors u, i (peak amplitudes): Re(u i*) / 2."""
+    return 0.5 * np.real(u * np.conj(i))
+
+
+class _Port:
+    """What the reference's ports share: the excitation switch and the
+    post-processing of the port's probe files into total, incident and
+    reflected voltages, currents and powers (python/openEMS/ports.py:48-153
+    is the interface; the splitting follows line_waves / move_plane)."""
+
+    Z_ref = None
+    measplane_shift = 0.0
+
+    def _setup_common(self, CSX, port_nr, start, stop, excite, kw):
         self.CSX = CSX
         self.number = port_nr
-        self.R = R
         self.start = np.array(start, np.float64)
         self.stop = np.array(stop, np.float64)
         self.excite = excite
         self.priority = kw.get("priority", 0)
         self.delay = kw.get("delay", 0.0)
-        prefix = kw.get("PortNamePrefix", "")
-        lbl = prefix + "port_{}" + "_{}".format(port_nr)
-        self.Z_ref = None
         self.port_props = []
-        ny = _dir(exc_dir)
-        self.exc_ny = ny
-        if self.start[ny] == self.stop[ny]:
-            raise ValueError("LumpedPort: start and stop may not be identical in excitation direction")
-        self.direction = np.sign(self.stop[ny] - self.start[ny])
-        if R > 0:
-            CSX.AddLumpedElement(lbl.format("resist"), ny=ny, caps=True, R=R).AddBox(
-                self.start, self.stop, priority=self.priority)
-        elif R == 0:
-            CSX.AddMetal(lbl.format("resist")).AddBox(self.start, self.stop, priority=self.priority)
-        if excite != 0:
-            v = np.zeros(3)
-            v[ny] = -1 * self.direction * excite
-            self.port_props.append(CSX.AddExcitation(lbl.format("excite"), exc_type=0, exc_val=v,
-                                                     delay=self.delay))
-            self.port_props[-1].AddBox(
-                self.start, self.stop, priority=self.priority)
-        self.U_filenames = [lbl.format("ut")]
-        us, ue = 0.5 * (self.start + self.stop), 0.5 * (self.start + self.stop)
-        us[ny], ue[ny] = self.start[ny], self.stop[ny]
-        CSX.AddProbe(self.U_filenames[0], p_type=0, weight=-1).AddBox(us, ue)
-        self.I_filenames = [lbl.format("it")]
-        i_s, i_e = self.start.copy(), self.stop.copy()
-        i_s[ny] = i_e[ny] = 0.5 * (self.start[ny] + self.stop[ny])
-        CSX.AddProbe(self.I_filenames[0], p_type=1, weight=self.direction, norm_dir=ny).AddBox(i_s, i_e)
+        self._label = kw.get("PortNamePrefix", "") + "port_{}_" + str(port_nr)
+
+    def _name(self, what):
+        return self._label.format(what)
+
+    def _add_excitation(self, vec, start, stop):
+        exc = self.CSX.AddExcitation(self._name("excite"), exc_type=0, exc_val=vec, delay=self.delay)
+        exc.AddBox(start, stop, priority=self.priority)
+        self.port_props.append(exc)
+        return exc
 
     def SetEnabled(self, val):
-        """Switches the port's excitation on or off (ports.Port.SetEnabled,
-        python/openEMS/ports.py:86-96); enabling a port without one raises."""
-        found = False
-        for prop in self.port_props:
-            if prop.kind == "excitation":
-                prop.SetEnabled(val)
-                found = True
-        if not found and val:
-            raise ValueError("Unable to enable port! No excitation found!")
+        """Switches the port's excitation on or off; a port built without an
+        excitation cannot be switched on (ValueError)."""
+        excitations = [p for p in self.port_props if p.kind == "excitation"]
+        if val and not excitations:
+            raise ValueError(f"port {self.number} has no excitation to enable")
+        for p in excitations:
+            p.SetEnabled(val)
 
     def ReadUIData(self, sim_path, freq, signal_type="pulse"):
+        """Total voltage and current of the port: the sums over its probe files."""
         self.u_data = UI_data(self.U_filenames, sim_path, freq, signal_type)
-        self.uf_tot = sum(self.u_data.ui_f_val)
-        self.ut_tot = sum(self.u_data.ui_val)
         self.i_data = UI_data(self.I_filenames, sim_path, freq, signal_type)
-        self.if_tot = sum(self.i_data.ui_f_val)
-        self.it_tot = sum(self.i_data.ui_val)
+        self.ut_tot, self.uf_tot = sum(self.u_data.ui_val), sum(self.u_data.ui_f_val)
+        self.it_tot, self.if_tot = sum(self.i_data.ui_val), sum(self.i_data.ui_f_val)
+
+    def _port_impedance(self):
+        return self.Z_ref
 
     def CalcPort(self, sim_path, freq, ref_impedance=None, ref_plane_shift=None, signal_type="pulse"):
-        """Incident / reflected split (ports.Port.CalcPort, python/openEMS/ports.py:108-150):
-        against ref_impedance, else the port's own Z_ref (R for a lumped port,
-        the measured line impedance for a microstrip port); ref_plane_shift
-        moves the reference plane along a line port by beta * shift."""
+        """Reads the probe files and splits the port's voltage and current
+        into incident (forward) and reflected (backward) waves against
+        ref_impedance, or the port's own impedance (R for a lumped port, the
+        measured line impedance of a microstrip port).  ref_plane_shift (in
+        drawing units, from the port start) moves the reference plane along
+        a line port before the split.  Sets uf_/if_/ut_/it_ tot, inc and ref
+        and the powers P_inc, P_ref, P_acc."""
         self.ReadUIData(sim_path, freq, signal_type)
         if ref_impedance is not None:
             self.Z_ref = ref_impedance
+        else:
+            self.Z_ref = self._port_impedance()
         if self.Z_ref is None:
-            self.Z_ref = getattr(self, "R", None)
-        if self.Z_ref is None:
-            raise ValueError("Port Z_ref should not be None!")
+            raise ValueError(f"port {self.number}: no reference impedance")
         Z = self.Z_ref
         if ref_plane_shift is not None:
-            if not hasattr(self, "beta"):
-                raise ValueError("Port has no beta attribute!")
-            shift = ref_plane_shift
-            if getattr(self, "measplane_shift", 0):
-                shift -= self.measplane_shift
-            shift *= self.CSX.GetGrid().GetDeltaUnit()
-            phase = np.real(self.beta) * shift
-            uf = self.uf_tot * np.cos(-phase) + 1j * self.if_tot * Z * np.sin(-phase)
-            itf = self.if_tot * np.cos(-phase) + 1j * self.uf_tot / Z * np.sin(-phase)
-            self.uf_tot, self.if_tot = uf, itf
-        self.uf_inc = 0.5 * (self.uf_tot + self.if_tot * Z)
-        self.if_inc = 0.5 * (self.if_tot + self.uf_tot / Z)
-        self.uf_ref = self.uf_tot - self.uf_inc
-        self.if_ref = self.if_inc - self.if_tot
-        if isinstance(Z, (int, float, np.floating)):
-            self.ut_inc = 0.5 * (self.ut_tot + self.it_tot * Z)
-            self.it_inc = 0.5 * (self.it_tot + self.ut_tot / Z)
-            self.ut_ref = self.ut_tot - self.ut_inc
-            self.it_ref = self.it_inc - self.it_tot
-        self.P_inc = 0.5 * np.real(self.uf_inc * np.conj(self.if_inc))
-        self.P_ref = 0.5 * np.real(self.uf_ref * np.conj(self.if_ref))
-        self.P_acc = 0.5 * np.real(self.uf_tot * np.conj(self.if_tot))
+            if getattr(self, "beta", None) is None:
+                raise ValueError(f"port {self.number}: no propagation constant to move the reference plane")
+            d = (ref_plane_shift - self.measplane_shift) * self.CSX.GetGrid().GetDeltaUnit()
+            self.uf_tot, self.if_tot = move_plane(self.uf_tot, self.if_tot, Z, np.real(self.beta) * d)
+        self.uf_inc, self.uf_ref, self.if_inc, self.if_ref = line_waves(self.uf_tot, self.if_tot, Z)
+        if np.isrealobj(Z) and np.ndim(Z) == 0:
+            self.ut_inc, self.ut_ref, self.it_inc, self.it_ref = line_waves(self.ut_tot, self.it_tot, Z)
+        self.P_inc = _half_power(self.uf_inc, self.if_inc)
+        self.P_ref = _half_power(self.uf_ref, self.if_ref)
+        self.P_acc = _half_power(self.uf_tot, self.if_tot)
 
 
-class WaveguidePort(LumpedPort):
-    """A waveguide port (ports.WaveguidePort, python/openEMS/ports.py:341-400):
-    a soft E excitation weighted by the mode's E pattern in the plane
-    start[exc] (when excited), and mode-matching probes (types 10 and 11) of
-    the E and H patterns in the plane stop[exc]; CalcPort takes the analytic
-    wave impedance k Z0 / beta (beta = sqrt(k^2 - kc^2)) as Z_ref."""
+class LumpedPort(_Port):
+    """A lumped port (the interface of ports.LumpedPort): a resistor R with
+    end caps over the port box (a metal short for R = 0), optionally a soft
+    E excitation of -direction * excite along the port axis, a voltage probe
+    along the axis through the box centre (weight -1) and a current probe
+    over the box's mid plane (weight = direction, normal = the axis).  Its
+    reference impedance is R."""
 
-    def __init__(self, CSX, port_nr, start, stop, exc_dir, E_WG_func, H_WG_func, kc, excite=0, **kw):
-        self.CSX = CSX
-        self.number = port_nr
-        self.start = np.array(start, np.float64)
-        self.stop = np.array(stop, np.float64)
-        self.excite = excite
-        self.priority = kw.get("priority", 0)
-        self.delay = kw.get("delay", 0.0)
-        self.Z_ref = None
-        self.port_props = []
-        lbl = kw.get("PortNamePrefix", "") + "port_{}" + "_{}".format(port_nr)
-        ny = _dir(exc_dir)
-        self.exc_ny, self.ny_P, self.ny_PP = ny, (ny + 1) % 3, (ny + 2) % 3
-        self.direction = np.sign(self.stop[ny] - self.start[ny])
-        self.ref_index = 1
-        if excite != 0 and self.stop[ny] == self.start[ny]:
-            raise ValueError("Port length in excitation direction may not be zero if port is excited!")
-        self.kc = kc
-        self.E_func, self.H_func = E_WG_func, H_WG_func
+    def __init__(self, CSX, port_nr, R, start, stop, exc_dir, excite=0, **kw):
+        self._setup_common(CSX, port_nr, start, stop, excite, kw)
+        self.R = R
+        ax = _dir(exc_dir)
+        self.exc_ny = ax
+        length = self.stop[ax] - self.start[ax]
+        if length == 0:
+            raise ValueError(f"lumped port {port_nr}: the box has no extent along the port axis")
+        self.direction = np.sign(length)
+        if R > 0:
+            CSX.AddLumpedElement(self._name("resist"), ny=ax, caps=True, R=R).AddBox(
+                self.start, self.stop, priority=self.priority)
+        elif R == 0:
+            CSX.AddMetal(self._name("resist")).AddBox(self.start, self.stop, priority=self.priority)
         if excite != 0:
-            a, b = self.start.copy(), self.stop.copy()
-            b[ny] = a[ny]
-            v = np.ones(3)
-            v[ny] = 0
-            exc = CSX.AddExcitation(lbl.format("excite"), exc_type=0, exc_val=v, delay=self.delay)
-            self.port_props.append(exc)
-            exc.SetWeightFunction([str(x) for x in self.E_func])
-            exc.AddBox(a, b, priority=self.priority)
-        a, b = self.start.copy(), self.stop.copy()
-        a[ny] = b[ny]
-        self.measplane_shift = abs(self.stop[ny] - self.start[ny])
-        self.U_filenames = [lbl.format("ut")]
-        CSX.AddProbe(self.U_filenames[0], p_type=10, mode_function=self.E_func).AddBox(a, b)
-        self.I_filenames = [lbl.format("it")]
-        CSX.AddProbe(self.I_filenames[0], p_type=11, weight=self.direction, mode_function=self.H_func).AddBox(a, b)
+            vec = np.zeros(3)
+            vec[ax] = -self.direction * excite
+            self._add_excitation(vec, self.start, self.stop)
+        centre = 0.5 * (self.start + self.stop)
+        u0, u1 = centre.copy(), centre.copy()
+        u0[ax], u1[ax] = self.start[ax], self.stop[ax]
+        self.U_filenames = [self._name("ut")]
+        CSX.AddProbe(self.U_filenames[0], p_type=0, weight=-1).AddBox(u0, u1)
+        i0, i1 = self.start.copy(), self.stop.copy()
+        i0[ax] = i1[ax] = centre[ax]
+        self.I_filenames = [self._name("it")]
+        CSX.AddProbe(self.I_filenames[0], p_type=1, weight=self.direction, norm_dir=ax).AddBox(i0, i1)
 
-    def CalcPort(self, sim_path, freq, ref_impedance=None, ref_plane_shift=None, signal_type="pulse"):
-        k = 2.0 * np.pi * np.asarray(freq, np.float64) / C0 * self.ref_index
-        self.beta = np.sqrt(k ** 2 - self.kc ** 2 + 0j)
-        self.ZL = k * Z0 / self.beta
-        if ref_impedance is None:
-            self.Z_ref = self.ZL
-        super().CalcPort(sim_path, freq, ref_impedance, ref_plane_shift, signal_type)
+    def _port_impedance(self):
+        return self.R
 
 
-class RectWGPort(WaveguidePort):
-    """A rectangular waveguide port (ports.RectWGPort, ports.py:402-460): the
-    TE_mn patterns of an a x b guide (sizes in meters) written as mode
-    functions of the two transverse coordinates relative to start, in
-    drawing units; TE modes only."""
-
-    def __init__(self, CSX, port_nr, start, stop, exc_dir, a, b, mode_name, excite=0, **kw):
-        ny = _dir(exc_dir)
-        nP, nPP = (ny + 1) % 3, (ny + 2) % 3
-        if len(mode_name) != 4:
-            raise ValueError("Invalid mode definition")
-        if not mode_name.startswith("TE"):
-            raise ValueError("Currently only TE-modes are supported! Mode found: {}".format(mode_name))
-        self.WG_size, self.WG_mode = [a, b], mode_name
-        M, N = float(mode_name[2]), float(mode_name[3])
-        self.M, self.N = M, N
-        unit = CSX.GetGrid().GetDeltaUnit()
-        xyz = "xyz"
-        sP = "({}-{})".format(xyz[nP], float(start[nP]))
-        sPP = "({}-{})".format(xyz[nPP], float(start[nPP]))
-        kc = np.sqrt((M * np.pi / a) ** 2 + (N * np.pi / b) ** 2)
-        a, b = a / unit, b / unit
-        E, H = ["0", "0", "0"], ["0", "0", "0"]
-        if N > 0:
-            E[nP] = "{}*cos({}*{})*sin({}*{})".format(N / b, M * np.pi / a, sP, N * np.pi / b, sPP)
-            H[nPP] = "{}*cos({}*{})*sin({}*{})".format(N / b, M * np.pi / a, sP, N * np.pi / b, sPP)
-        if M > 0:
-            E[nPP] = "{}*sin({}*{})*cos({}*{})".format(-1 * M / a, M * np.pi / a, sP, N * np.pi / b, sPP)
-            H[nP] = "{}*sin({}*{})*cos({}*{})".format(M / a, M * np.pi / a, sP, N * np.pi / b, sPP)
-        super().__init__(CSX, port_nr, start, stop, exc_dir, E, H, kc, excite, **kw)
-
-
-class MSLPort(LumpedPort):
-    """A microstrip port (ports.MSLPort, python/openEMS/ports.py:190-340): the
-    strip as a metal box in the plane start[exc] of the port box, three
-    voltage probes across the substrate at the mesh lines around the
-    measurement plane (half the port length by default, MeasPlaneShift), two
-    current probes around the strip between them, an optional soft feed at
-    FeedShift and a feed resistance Feed_R at the port start.  ReadUIData
-    takes the middle voltage and the mean current and derives the
-    propagation constant beta and the line impedance Z_ref from the
-    differences (the method of the reference's port)."""
+class MSLPort(_Port):
+    """A microstrip port (the interface of ports.MSLPort): the strip as a
+    metal box in the plane start[exc] of the port box; three voltage probes
+    across the substrate on the mesh lines around the measurement plane
+    (half the port length from the start unless MeasPlaneShift is given) and
+    two current probes around the strip half way between them, ordered in
+    the propagation direction; an optional soft feed at FeedShift and a
+    feed resistance Feed_R at the port start.  ReadUIData takes the middle
+    voltage and the mean current as the port's totals and derives beta and
+    the line impedance from the telegrapher's equations
+    (line_from_three_planes); the line impedance is the reference impedance."""
 
     def __init__(self, CSX, port_nr, metal_prop, start, stop, prop_dir, exc_dir, excite=0, **kw):
-        self.CSX = CSX
-        self.number = port_nr
-        self.start = np.array(start, np.float64)
-        self.stop = np.array(stop, np.float64)
-        self.excite = excite
-        self.priority = kw.get("priority", 0)
-        self.delay = kw.get("delay", 0.0)
-        self.Z_ref = None
-        self.port_props = []
-        lbl = kw.get("PortNamePrefix", "") + "port_{}" + "_{}".format(port_nr)
+        self._setup_common(CSX, port_nr, start, stop, excite, kw)
         ex, pr = _dir(exc_dir), _dir(prop_dir)
         self.exc_ny, self.prop_ny = ex, pr
-        if not np.all(self.start != self.stop):
-            raise ValueError("Start coordinate must not be equal to stop coordinate")
+        if np.any(self.start == self.stop):
+            raise ValueError(f"microstrip port {port_nr}: the box must have an extent along every axis")
         if ex == pr:
-            raise ValueError("Excitation direction must not be equal to propagation direction")
+            raise ValueError(f"microstrip port {port_nr}: excitation and propagation directions coincide")
         self.direction = np.sign(self.stop[pr] - self.start[pr])
         self.upside_down = np.sign(self.stop[ex] - self.start[ex])
         self.feed_shift = kw.get("FeedShift", 0)
-        self.measplane_shift = kw.get("MeasPlaneShift", 0.5 * abs(self.start[pr] - self.stop[pr]))
-        self.measplane_pos = self.start[pr] + self.measplane_shift * self.direction
+        shift = kw.get("MeasPlaneShift", 0.5 * abs(self.stop[pr] - self.start[pr]))
+        self.measplane_pos = self.start[pr] + shift * self.direction
         self.feed_R = kw.get("Feed_R", np.inf)
-        strip_stop = self.stop.copy()
-        strip_stop[ex] = self.start[ex]
-        metal_prop.AddBox(self.start, strip_stop, priority=self.priority)
+        strip_end = self.stop.copy()
+        strip_end[ex] = self.start[ex]
+        metal_prop.AddBox(self.start, strip_end, priority=self.priority)
         lines = CSX.GetGrid().GetLines(pr)
         if len(lines) <= 5:
-            raise ValueError("At least 5 lines in propagation direction required!")
+            raise ValueError(f"microstrip port {port_nr}: more than 5 mesh lines along the line are needed")
+        # the mesh line nearest the measurement plane and its two neighbours,
+        # in the propagation direction
         k = int(np.clip(np.argmin(np.abs(lines - self.measplane_pos)), 1, len(lines) - 2))
         self.measplane_shift = abs(self.start[pr] - lines[k])
-        idx = np.array([k - 1, k, k + 1])
-        if self.direction < 0:
-            idx = idx[::-1]
-        u_pos = lines[idx]
-        self.U_delta = np.diff(u_pos)
-        self.U_filenames = []
-        mid = 0.5 * (self.start + self.stop)
-        for n, sfx in enumerate("ABC"):
-            a, b = mid.copy(), mid.copy()
-            a[pr] = b[pr] = u_pos[n]
-            a[ex], b[ex] = self.start[ex], self.stop[ex]
-            self.U_filenames.append(lbl.format("ut") + sfx)
-            CSX.AddProbe(self.U_filenames[-1], p_type=0).AddBox(a, b)
-        i_pos = u_pos[:2] + np.diff(u_pos) / 2.0
-        self.I_delta = np.diff(i_pos)
-        self.I_filenames = []
-        a, b = self.start.copy(), self.stop.copy()
-        b[ex] = self.start[ex]
-        for n, sfx in enumerate("AB"):
-            a[pr] = b[pr] = i_pos[n]
-            self.I_filenames.append(lbl.format("it") + sfx)
-            CSX.AddProbe(self.I_filenames[-1], p_type=1, weight=self.direction, norm_dir=pr).AddBox(a.copy(), b.copy())
+        u_at = lines[[k - 1, k, k + 1]][:: int(self.direction) or 1]
+        self.U_delta = np.diff(u_at)
+        i_at = 0.5 * (u_at[:-1] + u_at[1:])
+        self.I_delta = np.diff(i_at)
+        centre = 0.5 * (self.start + self.stop)
+        self.U_filenames = [self._name("ut") + tag for tag in "ABC"]
+        for fn, pos in zip(self.U_filenames, u_at):
+            u0, u1 = centre.copy(), centre.copy()
+            u0[pr] = u1[pr] = pos
+            u0[ex], u1[ex] = self.start[ex], self.stop[ex]
+            CSX.AddProbe(fn, p_type=0).AddBox(u0, u1)
+        self.I_filenames = [self._name("it") + tag for tag in "AB"]
+        for fn, pos in zip(self.I_filenames, i_at):
+            i0, i1 = self.start.copy(), self.stop.copy()
+            i1[ex] = self.start[ex]
+            i0[pr] = i1[pr] = pos
+            CSX.AddProbe(fn, p_type=1, weight=self.direction, norm_dir=pr).AddBox(i0, i1)
         if excite != 0:
             kf = int(np.argmin(np.abs(lines - (self.start[pr] + self.feed_shift * self.direction))))
-            a, b = self.start.copy(), self.stop.copy()
-            a[pr] = b[pr] = lines[kf]
-            v = np.zeros(3)
-            v[ex] = -1 * self.upside_down * excite
-            self.port_props.append(CSX.AddExcitation(lbl.format("excite"), exc_type=0, exc_val=v,
-                                                     delay=self.delay))
-            self.port_props[-1].AddBox(
-                a, b, priority=self.priority)
-        if self.feed_R >= 0 and not np.isinf(self.feed_R):
-            a, b = self.start.copy(), self.stop.copy()
-            b[pr] = a[pr]
+            f0, f1 = self.start.copy(), self.stop.copy()
+            f0[pr] = f1[pr] = lines[kf]
+            vec = np.zeros(3)
+            vec[ex] = -self.upside_down * excite
+            self._add_excitation(vec, f0, f1)
+        if self.feed_R >= 0 and np.isfinite(self.feed_R):
+            r0, r1 = self.start.copy(), self.stop.copy()
+            r1[pr] = r0[pr]
             if self.feed_R == 0:
-                metal_prop.AddBox(a, b)
+                metal_prop.AddBox(r0, r1)
             else:
-                CSX.AddLumpedElement(lbl.format("resist"), ny=ex, caps=True, R=self.feed_R).AddBox(a, b)
+                CSX.AddLumpedElement(self._name("resist"), ny=ex, caps=True, R=self.feed_R).AddBox(r0, r1)
 
     def ReadUIData(self, sim_path, freq, signal_type="pulse"):
         self.u_data = UI_data(self.U_filenames, sim_path, freq, signal_type)
-        self.uf_tot = self.u_data.ui_f_val[1]
-        self.ut_tot = self.u_data.ui_val[1]
         self.i_data = UI_data(self.I_filenames, sim_path, freq, signal_type)
-        self.if_tot = 0.5 * (self.i_data.ui_f_val[0] + self.i_data.ui_f_val[1])
-        self.it_tot = 0.5 * (self.i_data.ui_val[0] + self.i_data.ui_val[1])
         unit = self.CSX.GetGrid().GetDeltaUnit()
-        Et = self.u_data.ui_f_val[1]
-        dEt = (self.u_data.ui_f_val[2] - self.u_data.ui_f_val[0]) / (np.sum(np.abs(self.U_delta)) * unit)
-        Ht = self.if_tot
-        dHt = (self.i_data.ui_f_val[1] - self.i_data.ui_f_val[0]) / (np.abs(self.I_delta[0]) * unit)
-        beta = np.sqrt(-dEt * dHt / (Ht * Et))
-        beta[np.real(beta) < 0] *= -1
-        self.beta = beta
-        self.Z_ref = np.sqrt(Et * dEt / (Ht * dHt))
+        Uf, If = self.u_data.ui_f_val, self.i_data.ui_f_val
+        self.beta, self.Z_line, self.uf_tot, self.if_tot = line_from_three_planes(
+            Uf[0], Uf[1], Uf[2], np.sum(np.abs(self.U_delta)) * unit, If[0], If[1], abs(self.I_delta[0]) * unit)
+        self.ut_tot = self.u_data.ui_val[1]
+        self.it_tot = 0.5 * (self.i_data.ui_val[0] + self.i_data.ui_val[1])
+
+    def _port_impedance(self):
+        return self.Z_line
 
 
 # ----------------------------------------------------------------------------
@@ -657,11 +624,6 @@ class openEMS:
 
     def GetCSX(self):
         return self.CSX
-
-    def AddRectWaveGuidePort(self, port_nr, start, stop, p_dir, a, b, mode_name, excite=0, **kw):
-        if self.CSX is None:
-            raise ValueError("AddRectWaveGuidePort: CSX is not set!")
-        return RectWGPort(self.CSX, port_nr, start, stop, p_dir, a, b, mode_name, excite, **kw)
 
     def AddMSLPort(self, port_nr, metal_prop, start, stop, prop_dir, exc_dir, excite=0, **kw):
         if self.CSX is None:

@@ -21,8 +21,9 @@ LumpedElement, LorentzMaterial and
 DebyeMaterial with Box primitives (P1, P2, Priority) and, for Metal and
 Excitation, Curve (Vertex points), Cylinder (P1, P2, Radius) and
 CylindricalShell (P1, P2, Radius, ShellWidth) primitives; an Excitation's
-Weight element (X, Y, Z expressions); ProbeBox types 0-3 and the
-mode-matching types 10 / 11 with ModeFunctionX/Y/Z.  Properties are entered in file
+Weight element (X, Y, Z expressions); ProbeBox types 0-3 (mode matching,
+types 10 / 11, is outside the engine's scope and refused).  Properties are
+entered in file
 order, primitives in property order, as CSXCAD hands them to openEMS
 (openems.cpp:483-694 for probes and dumps).  Anything else (other primitive
 types, cylindrical meshes) is refused with a
@@ -273,7 +274,7 @@ def read_xml(source):
                 lumped.append(dict(le, priority=prio, start=p1, stop=p2))
         elif prop.tag == "ProbeBox":
             ptype = _int(prop, "Type", 0)
-            if ptype not in (0, 1, 2, 3, 10, 11):
+            if ptype not in (0, 1, 2, 3):
                 raise ValueError(f"probe type {ptype} of {name!r} is not supported")
             for nb, (prio, (p1, p2)) in enumerate(boxes):
                 probes.append({"type": ptype, "name": name if len(boxes) == 1 else f"{name}_{nb}",
@@ -281,8 +282,6 @@ def read_xml(source):
                                "freqs": _fd_samples(prop), "norm_dir": _int(prop, "NormDir", -1),
                                "start_time": float(prop.get("StartTime", 0.0)),
                                "stop_time": float(prop.get("StopTime", 0.0))})
-                if ptype in (10, 11):   # openems.cpp:545-551
-                    probes[-1]["mode_function"] = [prop.get("ModeFunction" + c, "0") for c in "XYZ"]
         elif prop.tag == "DumpBox":
             dtype = _int(prop, "DumpType", 0)
             mode = _int(prop, "DumpMode", 0)
@@ -432,8 +431,6 @@ def write_xml(scene, path=None):
         for key, attr in (("start_time", "StartTime"), ("stop_time", "StopTime")):
             if p.get(key):
                 el.set(attr, repr(float(p[key])))
-        for c, f in zip("XYZ", p.get("mode_function") or []):
-            el.set("ModeFunction" + c, str(f))
         _box(ET.SubElement(el, "Primitives"), 0, p["start"], p["stop"])
         if p.get("freqs"):
             ET.SubElement(el, "FD_Samples").text = ",".join(repr(float(f)) for f in p["freqs"])

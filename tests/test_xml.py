@@ -149,23 +149,10 @@ def test_round_trip_rebuilds_the_same_operator(native, make, tmp_path):
     assert [p["name"] for p in back["probes"]] == [p["name"] for p in scene.get("probes", [])]
 
 
-def test_mode_matching_probes_round_trip(native, tmp_path):
-    """Probe types 10 / 11 keep their ModeFunctionX/Y/Z attributes
-    (openems.cpp:545-551) through write_xml -> read_xml, and the read probes
-    build the same mode templates."""
-    import os
-    import sys
-    sys.path.insert(0, os.path.dirname(__file__))
-    from test_pyopenems import _wg_setup
-    from openems_amd.processing import make_probe
-    FDTD, _, _ = _wg_setup()
-    scene = FDTD.scene()
-    path = tmp_path / "wg.xml"
-    write_xml(scene, str(path))
-    assert "ModeFunctionY=" in path.read_text()
-    back = read_xml(str(path))
-    a, b = Operator(scene, nthreads=2), Operator(back, nthreads=2)
-    for pa, pb in zip(scene["probes"], back["probes"]):
-        assert pa["type"] == pb["type"] and pa["mode_function"] == pb["mode_function"]
-        ma, mb = make_probe(a, pa, 50), make_probe(b, pb, 50)
-        assert np.array_equal(ma.mode, mb.mode) and np.array_equal(ma.area, mb.area)
+def test_mode_matching_probes_are_refused(tmp_path):
+    """Probe types 10 / 11 (ProcessModeMatch) are outside the engine's scope:
+    the reader refuses them by name instead of writing an empty probe."""
+    path = tmp_path / "mm.xml"
+    path.write_text(SETUP.replace('NormDir="-1" Type="0"', 'NormDir="-1" Type="10" ModeFunctionX="0"'))
+    with pytest.raises(ValueError, match="probe type 10"):
+        read_xml(str(path))
