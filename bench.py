@@ -27,7 +27,14 @@ ceiling of the plain layout (8 TB/s / 143.57 B per cell-step).
 
 cpu_baseline: the oracle's restated multithreaded SSE-compressed engine
 (the reference's default "fastest" engine) on the same operator, on a bounded
-number of timesteps on the host cores.
+number of timesteps on the host's CPU share (its affinity set, capped by
+OMP_NUM_THREADS), plus the single-threaded SSE-compressed engine, with the CPU
+model.
+
+Beside the headline the line carries the other single-GPU workloads: the 64^3
+PEC cavity (plain_cavity), the 1024x1024x768 Mur layers (memory_scale), the
+64-instance batched sweep (batched) and the adjoint gradient (adjoint), and
+the achievable streaming bandwidth of the device (roofline.achievable).
 """
 import argparse
 import json
@@ -71,8 +78,45 @@ def traffic_from_profile(kernel, layout, size):
     return None
 
 
-def cpu_baseline(op, nthreads, budget_s=15.0):
-    """Time the oracle's multithreaded SSE-compressed engine on this operator."""
+def cpu_model():
+    """Host CPU model name (/proc/cpuinfo), or the platform string."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or platform.machine()
+
+
+def cpu_threads():
+    """Threads for the multithreaded CPU engine: the CPUs this process may run
+    on (its affinity set), capped by OMP_NUM_THREADS when that is set (the
+    host's CPU share for this job)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def _timed_steps(eng, budget_s):
+    eng.iterate(2)
+    t0 = time.perf_counter()
+    eng.iterate(4)
+    per = (time.perf_counter() - t0) / 4
+    steps = max(4, int(budget_s / max(per, 1e-6)))
+    t0 = time.perf_counter()
+    eng.iterate(steps)
+    return steps, time.perf_counter() - t0
+
+
+def cpu_baseline(op, nthreads, budget_s=15.0, single_budget_s=8.0):
+    """Time the oracle's restated reference engines on this operator: the
+    multithreaded SSE-compressed engine (the reference's default) on
+    `nthreads` threads, and the single-threaded SSE-compressed engine."""
     import numpy as np
     from oracle.operator_ref import ExcList, UPMLSlab
     from oracle.oracle import RefEngine
@@ -95,21 +139,23 @@ def cpu_baseline(op, nthreads, budget_s=15.0):
         for k in ("vv", "vvfo", "vvfn", "ii", "iifo", "iifn"):
             setattr(u, k, s[k])
         ro.upml.append(u)
-    eng = RefEngine(ro, "multithreaded", nthreads)
-    eng.iterate(2)
-    t0 = time.perf_counter()
-    eng.iterate(4)
-    per = (time.perf_counter() - t0) / 4
-    steps = max(4, int(budget_s / max(per, 1e-6)))
-    t0 = time.perf_counter()
-    eng.iterate(steps)
-    dt = time.perf_counter() - t0
-    eng.close()
     cells = float(np.prod(op.N))
-    return {"value": round(cells * steps / dt / 1e6, 2), "unit": "MCells/s", "cores": int(nthreads),
-            "kind": "port",
-            "sample": f"restated multithreaded sse-compressed engine (x-slab threads + barriers, "
-                      f"FTZ/DAZ, UPML hooks), {steps} timesteps of the same 256^3 UPML operator, {dt:.1f} s"}
+    eng = RefEngine(ro, "multithreaded", nthreads)
+    steps, dt = _timed_steps(eng, budget_s)
+    eng.close()
+    out = {"value": round(cells * steps / dt / 1e6, 2), "unit": "MCells/s", "cores": int(nthreads),
+           "kind": "port",
+           "sample": f"restated multithreaded sse-compressed engine (x-slab threads + barriers, "
+                     f"FTZ/DAZ, UPML hooks), {steps} timesteps of the same {op.N[0]}x{op.N[1]}x{op.N[2]} UPML "
+                     f"operator, {dt:.1f} s",
+           "cpu_model": cpu_model(), "nproc": os.cpu_count(), "threads": int(nthreads)}
+    if single_budget_s > 0:
+        eng = RefEngine(ro, "sse_compressed", 1)
+        s1, d1 = _timed_steps(eng, single_budget_s)
+        eng.close()
+        out["sse_compressed_1thread"] = {"value": round(cells * s1 / d1 / 1e6, 2), "unit": "MCells/s",
+                                         "cores": 1, "sample": f"{s1} timesteps, {d1:.1f} s"}
+    return out
 
 
 def copy_calibration(reps=20, nbytes=1 << 30):
@@ -316,21 +362,21 @@ def run_adjoint(nsteps=8000, K=256, seed=5):
 
 def run_memory_scale(steps=200, warmup=20):
     """Memory-scale workload: 1024x1024x768 lines (805 M cells), 12 seeded
-    dielectric layers, Mur on every face, sinusoidal 15 GHz E_x sheet source;
-    default engine (compressed layout).  Throughput over `steps` steps after
-    `warmup`, with the update kernels' average times (HIP events)."""
+    dielectric layers, Mur on every face, sinusoidal 15 GHz E_x sheet source,
+    16 E-point probes and a z-line voltage probe, steady-state end criterion
+    off; default engine (compressed layout).  Throughput over `steps` steps
+    after `warmup`, with the update kernels' average times (HIP events)."""
     from openems_amd import scenes
-    from openems_amd.engine import EngineHIP
-    from openems_amd.operator import Operator
+    from openems_amd.simulation import Simulation
     scene = scenes.mur_layers(nrts=steps + warmup)
+    scene["steady_state"] = False
     t0 = time.perf_counter()
-    op = Operator(scene)
+    sim = Simulation(scene)
     build_s = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    eng = EngineHIP(op)
-    create_s = time.perf_counter() - t0
+    eng, op = sim.eng, sim.op
     cells = float(op.num_cells)
     lay, nu, _, _ = eng.layout()
+    quad, qz = eng.quad()
     eng.IterateTS(warmup)
     eng.sync()
     eng.set_profiling(True)
@@ -339,17 +385,55 @@ def run_memory_scale(steps=200, warmup=20):
     eng.sync()
     wall = time.perf_counter() - t0
     _, vms, cms = eng.last_timing()
-    eng.close()
-    op.close()
+    eng.set_profiling(False)
+    sim.close()
     kms = max(vms, cms)
     per_launch = LAYOUT_BYTES[lay] * cells
     return {"workload": "mur_layers: 1024x1024x768 lines, 12 seeded layers, Mur x6, sinusoidal E_x sheet, "
-                        f"{steps} steps", "cells": int(cells), "layout": f"{lay} ({nu} unique)",
+                        f"16 E probes + 1 voltage probe, {steps} steps", "cells": int(cells),
+            "layout": f"{lay} ({nu} unique), two half steps, {'four x cells per lane' if quad else 'one cell per lane'}",
             "value": round(cells * steps / wall / 1e6, 1), "unit": "MCells/s", "ms_per_step": round(wall * 1e3 / steps, 3),
-            "operator_build_s": round(build_s, 1), "engine_create_s": round(create_s, 1),
-            "update_kernel_ms": round(kms, 4), "bytes_per_launch": int(per_launch),
+            "setup_s": round(build_s, 1),
+            "update_kernel": "k_volt4/k_curr4" if quad else "k_volt/k_curr",
+            "update_kernel_ms": round(kms, 4), "volt_ms": round(vms, 4), "curr_ms": round(cms, 4),
+            "bytes_per_launch": int(per_launch), "bytes_per_cell_update": 2 * LAYOUT_BYTES[lay],
             "achieved_GBs": round(per_launch / (kms * 1e-3) / 1e9, 1),
             "frac": round(per_launch / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
+def run_plain_cavity(steps=2000, cpu=True, nthreads=1):
+    """Plain EC-FDTD workload: 64^3 lines at 1 mm, vacuum, PEC on every face,
+    soft E_z Gaussian (f0 = 0, fc = 10 GHz) at the centre, E_z point probe and
+    z-line voltage probe, 2000 steps (default engine), with the restated CPU
+    engines on the same operator."""
+    from openems_amd import scenes
+    from openems_amd.simulation import Simulation
+    scene = scenes.pec_cavity(64, steps)
+    sim = Simulation(scene)
+    eng = sim.eng
+    cells = float(sim.op.num_cells)
+    eng.IterateTS(20)
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.IterateTS(steps)
+    eng.sync()
+    wall = time.perf_counter() - t0
+    quad, _ = eng.quad()
+    lay = eng.layout()[0]
+    out = {"workload": f"pec_cavity_64: 64^3 lines, 1 mm, vacuum, PEC x6, soft E_z Gaussian fc=10 GHz, "
+                       f"E point + voltage line probes, {steps} steps",
+           "cells": int(cells), "value": round(cells * steps / wall / 1e6, 1), "unit": "MCells/s",
+           "ms_per_step": round(wall * 1e3 / steps, 5),
+           "layout": f"{lay}, two half steps, {'four x cells per lane' if quad else 'one cell per lane'}, "
+                     f"timestep chunks replayed as HIP graphs",
+           "note": "3.1 MB per field: cache-resident and launch-bound, no HBM roofline"}
+    if cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(sim.op, nthreads, budget_s=4.0, single_budget_s=4.0)
+        except Exception as e:  # the baseline is reported, not required
+            out["cpu_baseline"] = {"value": None, "error": str(e)}
+    sim.close()
+    return out
 
 
 def main():
@@ -366,8 +450,12 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=4000)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--memory-scale", action="store_true",
-                    help="also run the 1024x1024x768 Mur workload (about 80 GB of host memory for its build)")
+    ap.add_argument("--no-memory-scale", action="store_true",
+                    help="skip the 1024x1024x768 Mur workload (its operator build holds about 80 GB of host memory)")
+    ap.add_argument("--memory-scale-steps", type=int, default=200)
+    ap.add_argument("--no-plain-cavity", action="store_true", help="skip the 64^3 PEC cavity workload")
+    ap.add_argument("--no-achievable", action="store_true",
+                    help="skip the streaming-probe measurement of the achievable HBM bandwidth")
     ap.add_argument("--adjoint-steps", type=int, default=8000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (for scaling HBM byte counters)")
@@ -413,6 +501,7 @@ def main():
     lay, n_unique, play, n_unique_pml = eng.layout()
     play = play or "plain"
     fused, zdepth = eng.fused()
+    quad, qzdepth = eng.quad()
     if fused:
         bytes_per_launch = FUSED_BYTES[lay] * cells + FUSED_PML_BYTES[play] * npml
         step_bytes = bytes_per_launch
@@ -424,10 +513,15 @@ def main():
         kname, kms = ("k_step", "k_step2")[fused - 1], volt_ms
     else:
         kname, kms = ("k_volt", volt_ms) if volt_ms >= curr_ms else ("k_curr", curr_ms)
+        if quad:
+            kname += "4"
     achieved = bytes_per_launch / (kms * 1e-3) / 1e9
     layout = f"{lay} coefficients ({n_unique} unique) + fused {play} UPML ({n_unique_pml} unique)" \
         if lay == "compressed" or play == "compressed" else "plain fp32 coefficients + fused UPML"
     form = ("two half steps", "single sweep, LDS planes (k_step)", "single sweep, independent columns (k_step2)")[fused]
+    if quad:
+        form += ", four x cells per lane"
+        zdepth = qzdepth
     layout += f", {form} (z depth {zdepth})"
     traffic = traffic_from_profile(kname, layout, n)
     plain_ceiling = HBM_PEAK_GBS * 1e9 / (plain_step_bytes / cells) / 1e6     # MCells/s
@@ -458,14 +552,30 @@ def main():
                      "plain_ceiling_mcells": round(plain_ceiling, 1),
                      "plain_ceiling_frac": round(value / plain_ceiling, 4)},
     }
-    print(f"# setup {setup_s:.1f}s, device time {dev_ms:.1f} ms for {args.steps} steps", file=sys.stderr)
+    print(f"# setup {setup_s:.1f}s, device time {dev_ms:.1f} ms for {args.steps} steps", file=sys.stderr, flush=True)
+    nthreads = cpu_threads()
     if not args.no_cpu_baseline:
         try:
-            nthreads = min(16, os.cpu_count() or 1)
             out["cpu_baseline"] = cpu_baseline(op, nthreads, args.cpu_seconds)
         except Exception as e:  # the baseline is reported, not required
             out["cpu_baseline"] = {"value": None, "error": str(e)}
+        print(f"# cpu baseline: {out['cpu_baseline']}", file=sys.stderr, flush=True)
     sim.close()
+    if not args.no_achievable:
+        try:
+            ach = achievable_bandwidth()
+            out["roofline"]["achievable"] = ach
+            out["roofline"]["achievable_unit"] = "GB/s (streaming probe, 16 B per lane, 3 GiB footprint)"
+            out["roofline"]["frac_of_achievable"] = round(achieved / ach["mix_7r3w"], 4)
+        except Exception as e:  # reported, the headline line still prints
+            out["roofline"]["achievable"] = {"error": str(e)}
+        print(f"# achievable: {out['roofline'].get('achievable')}", file=sys.stderr, flush=True)
+    if not args.no_plain_cavity:
+        try:
+            out["plain_cavity"] = run_plain_cavity(cpu=not args.no_cpu_baseline, nthreads=nthreads)
+        except Exception as e:  # reported, the headline line still prints
+            out["plain_cavity"] = {"error": str(e)}
+        print(f"# plain cavity: {out['plain_cavity']}", file=sys.stderr, flush=True)
     if not args.no_batched:
         try:
             out["batched"] = run_batched(args.batched_count, args.batched_steps)
@@ -478,9 +588,9 @@ def main():
         except Exception as e:  # reported, the headline line still prints
             out["adjoint"] = {"error": str(e)}
         print(f"# adjoint: {out['adjoint']}", file=sys.stderr)
-    if args.memory_scale:
+    if not args.no_memory_scale:
         try:
-            out["memory_scale"] = run_memory_scale()
+            out["memory_scale"] = run_memory_scale(args.memory_scale_steps)
         except Exception as e:  # reported, the headline line still prints
             out["memory_scale"] = {"error": str(e)}
         print(f"# memory scale: {out['memory_scale']}", file=sys.stderr)

@@ -30,6 +30,8 @@
  *   ems_set_graph         Engine::IterateTS loop replayed as HIP graphs per timestep chunk
  *   ems_set_fused         Engine::IterateTS step form (single sweep or two half steps)
  *   ems_get_fused                                                        FDTD/engine.cpp:267-286
+ *   ems_set_quad/get_quad Engine::UpdateVoltages / UpdateCurrents lane width (four x cells per lane)
+ *                                                    FDTD/engine.cpp:110-222, FDTD/engine_sse.cpp:79-274
  *   ems_get_numts         Engine::GetNumberOfTimesteps                   FDTD/engine.h:74
  *   ems_get_volt/curr     Engine::GetVolt / GetCurr (bulk)                FDTD/engine.h:77-93
  *   ems_set_volt/curr     Engine::SetVolt / SetCurr (bulk)                FDTD/engine.h:95-101
@@ -152,6 +154,19 @@ int ems_set_fused(ems_engine* h, int mode);
 /* fused: active form (0 two half steps, 1 LDS sweep, 2 column sweep);
  * zdepth: z planes per workgroup */
 int ems_get_fused(const ems_engine* h, int* fused, unsigned* zdepth);
+
+/* Half-step lane width (Engine::UpdateVoltages / UpdateCurrents,
+ * FDTD/engine.cpp:110-222; the reference's SSE engine packs four z cells per
+ * vector, FDTD/engine_sse.cpp:79-274).  With quads on, each lane of the two
+ * half steps updates four consecutive x cells with 16-byte accesses; results
+ * are bit-identical to one cell per lane.  Available when the number of x
+ * lines is a multiple of 4 and the engine runs two half steps.
+ * mode: -1 automatic (the default: quads where available on meshes without
+ * UPML), 0 one cell per lane, 1 quads (ems_init fails with EMS_E_ARG where
+ * unavailable). */
+int ems_set_quad(ems_engine* h, int mode);
+/* on: 1 when the half steps run quads; zdepth: their z planes per workgroup */
+int ems_get_quad(const ems_engine* h, int* on, unsigned* zdepth);
 
 /* Active layouts and table sizes (known after ems_init for the UPML part):
  * layout, pml_layout: EMS_LAYOUT_PLAIN or EMS_LAYOUT_COMPRESSED (pml_layout 0

@@ -63,7 +63,7 @@ STEP_FORMS = {"halves": 0, "lds": 1, "columns": 2}
 
 class EngineHIP:
     def __init__(self, op, device=0, layout="auto", sse_compressed=None, fused=None, steady_state=None,
-                 lorentz=None, rlc=None):
+                 lorentz=None, rlc=None, quad=None):
         """op: Operator; layout: "auto" | "plain" | "compressed";
         sse_compressed: optional (op_index, vv4, vi4, ii4, iv4), the reference's
         SSE-compressed form of the same operator, used instead of op's arrays;
@@ -79,7 +79,11 @@ class EngineHIP:
         fused: step form, None (automatic), False or "halves" (two half
         steps), True or "lds" (single sweep staging planes in LDS, k_step) or
         "columns" (single sweep of independent columns, k_step2); a required
-        single-sweep form fails when an active extension has none."""
+        single-sweep form fails when an active extension has none;
+        quad: half steps with four x cells per lane (k_volt4 / k_curr4), None
+        (automatic: used when the x lines are a multiple of 4, the step runs
+        two half steps and the mesh has no UPML), False (one cell per lane)
+        or True (required)."""
         L = lib()
         if L.ems_device_count() <= 0:
             raise RuntimeError("EngineHIP needs a HIP device; none is visible")
@@ -105,6 +109,8 @@ class EngineHIP:
         self._keep = []
         if fused is not None:
             check(L.ems_set_fused(h, STEP_FORMS[fused] if isinstance(fused, str) else (1 if fused else 0)))
+        if quad is not None:
+            check(L.ems_set_quad(h, 1 if quad else 0))
         if steady_state is not None:
             self.set_steady_state(*steady_state)
         # the operator's own Lorentz / lumped RLC tables unless tables are given
@@ -247,6 +253,12 @@ class EngineHIP:
         f, d = C.c_int(), C.c_uint()
         check(lib().ems_get_fused(self.h, C.byref(f), C.byref(d)))
         return f.value, d.value
+
+    def quad(self):
+        """(1 when the half steps run four x cells per lane, else 0; z planes per workgroup)"""
+        q, d = C.c_int(), C.c_uint()
+        check(lib().ems_get_quad(self.h, C.byref(q), C.byref(d)))
+        return q.value, d.value
 
     def last_timing(self):
         t, v, c = C.c_double(), C.c_double(), C.c_double()

@@ -112,3 +112,70 @@ def test_torch_autograd_function(gpu):
     assert J.item() == J2
     assert np.array_equal(e.grad.numpy(), ge) and np.array_equal(k.grad.numpy(), gk)
     prob.close()
+
+
+def test_gradient_32_cells_against_fp64_central_differences(gpu):
+    """The finite-difference check of the adjoint workload at the largest
+    size the CPU reference affords in a test: 64 x 64 x 48 lines at 0.25 mm,
+    8-cell UPML, 24 x 24 x 4 design cells with eps_r ~ U[1, 12], 2000 steps.
+    dJ/d eps_r from the GPU adjoint (float32 fields, checkpoints every 256
+    steps) against centred differences with step 1e-3 of the double-precision
+    reference forward (oracle.adjoint_ref) on 32 random design cells:
+    every cell within 1e-3 of the largest difference quotient, and the 32-cell
+    vectors within 1e-3 relative (2-norm)."""
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    from oracle.adjoint_ref import objective_f64_job
+    scene, probes, freq = scenes.adjoint_design(64, 64, 48, pml=8, design=(24, 24, 4), nrts=2000)
+    print("case adjoint 64x64x48: 32 cells vs FP64 centred differences (step 1e-3)", flush=True)
+    prob = DesignProblem(scene, probes, freq, 2000, checkpoint_every=256)
+    eps = scene["materials"][0]["grid"]["eps"].copy()
+    J, ge, _ = prob.value_and_grad(eps)
+    assert J > 0 and np.all(np.isfinite(ge))
+    rng = np.random.default_rng(32)
+    flat = rng.choice(eps.size, size=32, replace=False)
+    h = 1e-3
+    lo = tuple(int(v) for v in prob.box_lo - 1)
+    hi = tuple(int(v) for v in prob.box_hi + 1)
+    jobs = []
+    for f in flat:
+        idx = np.unravel_index(f, eps.shape)
+        for s in (+1, -1):
+            e = eps.copy()
+            e[idx] += s * h
+            jobs.append((prob.scene_with(e), probes, freq, 2000, lo, hi))
+    prob.close()
+    with ProcessPoolExecutor(max_workers=16, mp_context=mp.get_context("spawn")) as ex:
+        res = list(ex.map(objective_f64_job, jobs))
+    dts = {r[1] for r in res}
+    assert len(dts) == 1, "the design must not change the timestep"
+    fd = np.array([(res[2 * k][0] - res[2 * k + 1][0]) / (2 * h) for k in range(len(flat))])
+    ad = np.array([ge[np.unravel_index(f, eps.shape)] for f in flat])
+    scale = np.abs(fd).max()
+    for f, a, d in zip(flat, ad, fd):
+        print("cell", np.unravel_index(f, eps.shape), "adjoint", a, "fd64", d, flush=True)
+    assert scale > 0
+    assert np.max(np.abs(ad - fd)) <= 1e-3 * scale
+    assert np.linalg.norm(ad - fd) <= 1e-3 * np.linalg.norm(fd)
+
+
+def test_full_size_directional_derivative(gpu):
+    """The adjoint workload at full size (128 x 128 x 96 lines, 48 x 48 x 8
+    design cells, 8000 steps, checkpoints every 256): the gradient along a
+    random +-1 direction against a centred difference (step 1e-2) of two
+    float32 forward runs, within 1e-3 relative."""
+    scene, probes, freq = scenes.adjoint_design(nrts=8000)
+    print("case adjoint full size directional derivative", flush=True)
+    prob = DesignProblem(scene, probes, freq, 8000, checkpoint_every=256)
+    eps = scene["materials"][0]["grid"]["eps"].copy()
+    J, ge, _ = prob.value_and_grad(eps)
+    d = np.random.default_rng(5).choice([-1.0, 1.0], size=eps.shape)
+    h = 1e-2
+    Jp = prob.forward(eps + h * d)
+    Jm = prob.forward(eps - h * d)
+    prob.close()
+    fd = (Jp - Jm) / (2 * h)
+    ad = float(np.sum(ge * d))
+    print("J", J, "adjoint", ad, "fd", fd, flush=True)
+    assert J > 0 and fd != 0
+    assert abs(ad - fd) <= 1e-3 * abs(fd)

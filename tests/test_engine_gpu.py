@@ -647,3 +647,46 @@ def test_probe_and_dump_windows(gpu):
     want = rd.fd_re + 1j * rd.fd_im
     assert np.abs(want).max() > 0 and rel_l2(fd, want) <= 1e-5
     sim.close()
+
+
+def test_full_size_upml_1000_steps_probes_and_plane(gpu):
+    """The 256^3 UPML workload for 1000 steps with its lumped-port voltage and
+    current probes (with DFT at 5/10/15 GHz) and the 256 x 256 E-field DFT plane
+    at 5/10/15 GHz, against the oracle's multithreaded engine (bit-identical
+    to its basic engine) with the restated probe and dump processing: fields
+    and probe time series bit-identical, probe spectra and the plane within
+    1e-5 relative L2."""
+    import os as _os
+    scene = scenes.upml_bricks(256, 1000)
+    for p in scene["probes"]:
+        p["freqs"] = [5e9, 10e9, 15e9]
+    print("case upml_bricks_256, 1000 steps, port probes + FD plane vs oracle", flush=True)
+    sim = Simulation(scene)
+    sim.run()
+    res = sim.results()
+    v, c = sim.eng.get_volt(), sim.eng.get_curr()
+    sim.close()
+    print("gpu run done; building the oracle operator", flush=True)
+    ref = _ref_from_scene(scene)
+    rp = [RefProbe(ref, p["type"], p["start"], p["stop"], p["weight"], p["freqs"], p["norm_dir"],
+                   scene["oversampling"], p["name"]) for p in scene["probes"]]
+    d = scene["dumps"][0]
+    rd = RefFieldDumpFD(ref, 0, d["start"], d["stop"], d["freqs"])
+    nthreads = max(1, min(16, len(_os.sched_getaffinity(0))))
+    eng = run_reference(ref, 1000, probes=rp, dumps=[rd], kind="multithreaded", nthreads=nthreads)
+    rv, rc = eng.fields()
+    eng.close()
+    print("oracle run done", flush=True)
+    assert np.max(np.abs(rv)) > 0
+    assert np.array_equal(v.astype(np.float64), rv), f"voltage mismatches {np.count_nonzero(v != rv)}"
+    assert np.array_equal(c.astype(np.float64), rc), f"current mismatches {np.count_nonzero(c != rc)}"
+    for p in rp:
+        got = res["probes"][p.name]
+        want = p.td_array()
+        assert got["td"].shape == want.shape and np.abs(want[:, 1]).max() > 0
+        assert np.array_equal(got["td"], want)
+        assert rel_l2(got["fd"], p.fd) <= 1e-5
+    fd = res["dumps"][d["name"]]["fd"]
+    want = rd.fd_re + 1j * rd.fd_im
+    assert fd.shape == want.shape and np.abs(want).max() > 0
+    assert rel_l2(fd, want) <= 1e-5

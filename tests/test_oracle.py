@@ -2,6 +2,7 @@
 mathematics fixes and against the reference's own cross-engine invariant
 (TESTSUITE/enginetests/cavity.m: basic, sse, sse-compressed and
 multithreaded engines agree on the same setup).  CPU only."""
+import cmath
 import math
 
 import numpy as np
@@ -859,3 +860,44 @@ def test_mur_normal_incidence_reflection_matches_the_discrete_closed_form(oracle
         want = _mur_reflection_closed_form(f[i], op.dT, 1e-3)
         got = abs(Fr[i]) / abs(Fi[i])
         assert abs(got / want - 1) < 0.02, (fq, got, want)
+
+
+def test_adjoint_reference_objective_f64():
+    """oracle.adjoint_ref, the double-precision finite-difference reference of
+    the adjoint workload: (a) its design-box coefficients are Calc_ECOperatorPos
+    in double of C = eps_eff*A/delta, G = kappa_eff*A/delta, so rounding them
+    to float32 gives the reference's float32 operator to within 2 ulp (the
+    reference rounds C to float first), and outside the box they are the
+    float32 operator widened; (b) with an empty box its objective equals a
+    plain restatement: the float64 engine on the float32 operator and a
+    per-probe DFT sum."""
+    from oracle.adjoint_ref import coeffs64, objective_f64
+    scene, probes, freq = scenes.adjoint_design(24, 22, 20, pml=4, design=(4, 4, 2), nrts=120)
+    ref = RefOperator(scene).build()
+    g = scene["materials"][0]["grid"]
+    c0, nc = np.array(g["cell0"]), np.array(g["ncells"])
+    lo, hi = tuple(c0 - 1), tuple(c0 + nc + 1)
+    vv, vi, ii, iv = coeffs64(ref, lo, hi)
+    box = np.zeros(ref.vv.shape[1:], bool)
+    box[tuple(slice(lo[a], hi[a] + 1) for a in range(3))] = True
+    for n in range(3):
+        for c64, c32 in ((vv, ref.vv), (vi, ref.vi)):
+            r = c64[n][box].astype(np.float32)
+            ulp = np.spacing(np.abs(c32[n][box]))
+            assert np.all(np.abs(r.astype(np.float64) - c32[n][box]) <= 2 * ulp)
+            assert np.array_equal(c64[n][~box], c32[n][~box].astype(np.float64))
+        assert np.array_equal(ii[n], ref.ii[n].astype(np.float64))
+        assert np.array_equal(iv[n], ref.iv[n].astype(np.float64))
+    J, dT = objective_f64(scene, probes, freq, 120, (1, 1, 1), (0, 0, 0))
+    eng = RefEngine(ref, "basic_f64")
+    V, _ = eng.raw_views()
+    d = [np.diff(np.asarray(l) * scene["delta_unit"]) for l in scene["lines"]]
+    F = [0j] * len(probes)
+    for step in range(1, 121):
+        eng.iterate(1)
+        for k, (c, p) in enumerate(probes):
+            F[k] += complex(V[c][p] / d[c][p[c]]) * cmath.exp(-2j * math.pi * freq * step * ref.dT) * ref.dT
+    eng.close()
+    Jp = sum(abs(f) ** 2 for f in F)
+    assert dT == ref.dT and Jp > 0
+    assert J == pytest.approx(Jp, rel=1e-12)
