@@ -41,6 +41,7 @@
  *   ems_add_fd_dump       ProcessFieldsFD::Process                       Common/processfields_fd.cpp:72-107
  *   ems_add_td_dump       ProcessFieldsTD::Process                       Common/processfields_td.cpp:50-91
  *   ems_energy            Engine_Interface_FDTD::CalcFastEnergy          FDTD/engine_interface_fdtd.cpp:302-347
+ *   ems_energy_noalloc    (no counterpart: ems_energy for callers that cannot allocate)
  *   ems_set_steady_state  Operator_Ext_SteadyState -> Engine_Ext_SteadyState
  *   ems_steady_state                                 FDTD/extensions/engine_ext_steadystate.cpp:50-107
  *   ems_destroy           Engine::~Engine / Engine::Reset                FDTD/engine.cpp:45-108
@@ -295,6 +296,19 @@ int ems_td_dump_results(ems_engine* h, int id, unsigned* count, unsigned* ts, fl
 
 /* eps0 * sum(V^2) + mu0 * sum(I^2) over [0,N-1)^3 */
 int ems_energy(ems_engine* h, double* out);
+
+/* ems_energy without allocation: no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Enqueues the same
+ * reduction on the engine's stream and writes the same value (bit for bit)
+ * to *d_out, one double in device memory; it neither allocates nor
+ * synchronises: the value is ready once the engine's stream reaches it
+ * (ems_sync).  The partial sums live in a file-scope device array of 2048
+ * doubles (16 KiB, one reduction workgroup's pair each; the workgroup count
+ * is fixed, so every mesh size fits and none is refused), held by each
+ * device until the process ends and shared by every call on that device:
+ * calls that overlap on one GPU (engines on different streams) race. */
+int ems_energy_noalloc(ems_engine* h, double* d_out);
 
 /* device time of the last ems_iterate call in milliseconds (HIP events) and
  * the average duration of the voltage / current update kernels it launched */

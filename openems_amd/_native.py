@@ -110,6 +110,7 @@ ENGINE_API = [
     ("ems_set_graph", C.c_int, [vp, C.c_uint]),
     ("ems_get_graph", C.c_int, [vp, up, up, up]),
     ("ems_sync", C.c_int, [vp]),
+    ("ems_energy_noalloc", C.c_int, [vp, C.c_void_p]),
     ("ems_get_volt", C.c_int, [vp, fp]),
     ("ems_get_curr", C.c_int, [vp, fp]),
     ("ems_set_volt", C.c_int, [vp, fp]),

@@ -18,6 +18,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <thread>
@@ -1802,6 +1803,32 @@ __global__ __launch_bounds__(kBlock) void k_energy(const float* __restrict__ V, 
     }
 }
 
+// one thread: the energy from k_energy's partial sums, summed in workgroup
+// order (as ems_energy sums them on the host)
+__global__ void k_energy_total(const double* __restrict__ part, unsigned nb, double* __restrict__ out)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double e = 0.0, m = 0.0;
+    for (unsigned b = 0; b < nb; ++b) {
+        e += part[2 * b];
+        m += part[2 * b + 1];
+    }
+    *out = 8.85418781762e-12 * e + 1.256637062e-6 * m;
+}
+
+}  // namespace
+
+// Workgroups of the energy reduction, and the device scratch of the
+// allocation-free energy entry point (ems_energy_noalloc): the partial sums
+// of those workgroups, two doubles each (16 KiB; every device the library runs
+// on holds one copy until the process ends).  Host code never passes the
+// array's name to a kernel or a copy: it uses the per-device address from
+// hipGetSymbolAddress (scratch_setup).
+constexpr unsigned kEnergyBlocks = 1024;
+static __device__ __align__(256) double scratch[2 * kEnergyBlocks];
+
+namespace {
+
 // ---------------------------------------------------------------------------
 // Adjoint of the timestep (discrete adjoint of the two half steps with UPML),
 // for gradients of a DFT-of-probe objective with respect to the coefficients.
@@ -2358,6 +2385,29 @@ unsigned choose_zdepth(const unsigned* N)
 
 }  // namespace
 
+namespace {
+
+// per-device address of `scratch`, taken once per device with that device
+// current (each device holds its own copy of the array at its own address)
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_setup(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices) return fail(EMS_E_ARG, "device ordinal beyond the scratch table");
+    if (g_scratch_ptr[device]) return 0;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(EMS_E_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<double*>(p);
+    return 0;
+}
+
+}  // namespace
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
@@ -2387,6 +2437,7 @@ int ems_create_layout(ems_engine** out, const unsigned numLines[3], const float*
     if (ndev <= 0) return fail(EMS_E_NODEV, "no HIP device available");
     if (device < 0 || device >= ndev) return fail(EMS_E_ARG, "invalid device ordinal");
     HIPCHK(hipSetDevice(device));
+    if (int src = scratch_setup(device)) return src;
     const unsigned long long tot = plane * numLines[2];
     const unsigned long long cs = (tot + 63) / 64 * 64;
     // per-cell compression (host): first-seen order x -> y -> z
@@ -3814,7 +3865,7 @@ int ems_energy(ems_engine* h, double* out)
 {
     if (!h || !out) return fail(EMS_E_ARG, "null argument");
     HIPCHK(hipSetDevice(h->device));
-    const unsigned nb = 1024;
+    const unsigned nb = kEnergyBlocks;
     if (!h->energy_part) {
         int rc = h->alloc(&h->energy_part, 2 * nb);
         if (rc) return rc;
@@ -3830,6 +3881,22 @@ int ems_energy(ems_engine* h, double* out)
         m += hp[2 * b + 1];
     }
     *out = 8.85418781762e-12 * e + 1.256637062e-6 * m;
+    return 0;
+}
+
+// ems_energy without allocation or synchronisation: the partial sums go to
+// the device's file-scope scratch, the result to the caller's device memory
+int ems_energy_noalloc(ems_engine* h, double* d_out)
+{
+    if (!h || !d_out) return fail(EMS_E_ARG, "null argument");
+    if (h->device < 0 || h->device >= kMaxDevices || !g_scratch_ptr[h->device])
+        return fail(EMS_E_STATE, "the device scratch of this engine's device is not set up");
+    HIPCHK(hipSetDevice(h->device));
+    double* part = g_scratch_ptr[h->device];
+    hipLaunchKernelGGL(k_energy, dim3(kEnergyBlocks), dim3(kBlock), 0, h->stream, h->V, h->I, h->cs, h->N[0], h->N[1],
+                       h->N[2], part);
+    hipLaunchKernelGGL(k_energy_total, dim3(1), dim3(64), 0, h->stream, part, kEnergyBlocks, d_out);
+    HIPCHK(hipGetLastError());
     return 0;
 }
 
