@@ -443,7 +443,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--layout", default="auto", choices=["auto", "plain", "compressed"])
-    ap.add_argument("--step", default="auto", choices=["auto", "halves", "lds", "columns"])
+    ap.add_argument("--step", default="auto", choices=["auto", "halves", "lds", "columns", "quads"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--batched-count", type=int, default=64)
@@ -510,7 +510,7 @@ def main():
         step_bytes = 2 * bytes_per_launch
     plain_step_bytes = PLAIN_CELL_STEP_BYTES * (cells + npml)
     if fused:
-        kname, kms = ("k_step", "k_step2")[fused - 1], volt_ms
+        kname, kms = ("k_step", "k_step2", "k_step4")[fused - 1], volt_ms
     else:
         kname, kms = ("k_volt", volt_ms) if volt_ms >= curr_ms else ("k_curr", curr_ms)
         if quad:
@@ -518,7 +518,8 @@ def main():
     achieved = bytes_per_launch / (kms * 1e-3) / 1e9
     layout = f"{lay} coefficients ({n_unique} unique) + fused {play} UPML ({n_unique_pml} unique)" \
         if lay == "compressed" or play == "compressed" else "plain fp32 coefficients + fused UPML"
-    form = ("two half steps", "single sweep, LDS planes (k_step)", "single sweep, independent columns (k_step2)")[fused]
+    form = ("two half steps", "single sweep, LDS planes (k_step)", "single sweep, independent columns (k_step2)",
+            "single sweep, quad columns (k_step4)")[fused]
     if quad:
         form += ", four x cells per lane"
         zdepth = qzdepth

@@ -35,6 +35,9 @@
  *   ems_get_numts         Engine::GetNumberOfTimesteps                   FDTD/engine.h:74
  *   ems_get_volt/curr     Engine::GetVolt / GetCurr (bulk)                FDTD/engine.h:77-93
  *   ems_set_volt/curr     Engine::SetVolt / SetCurr (bulk)                FDTD/engine.h:95-101
+ *   ems_read/write_field  Engine::GetVolt / GetCurr / SetVolt / SetCurr over a box
+ *                                                                        FDTD/engine.h:77-101
+ *   ems_get_field_at      Engine_Interface_FDTD::GetEField / GetHField   FDTD/engine_interface_fdtd.cpp:63-204
  *   ems_add_probe         ProcessVoltage / ProcessCurrent / ProcessFieldProbe sampling
  *                                                    Common/processvoltage.cpp:37-47, Common/processcurrent.cpp:96-171,
  *                                                    Common/processfieldprobe.cpp:77-92
@@ -148,11 +151,13 @@ int ems_steady_state(ems_engine* h, double* last_max_diff, unsigned* evaluations
  * Lorentz / Drude, lumped RLC and steady-state detection keep two half steps,
  * as do current sources on the last line of a direction and more than 64
  * sources in one tile (form 1) or column (form 2).
+ * Form 3 sweeps columns of four x cells per lane (16-byte accesses) like
+ * form 2, on meshes without UPML whose x lines are a multiple of 4.
  * mode: -1 automatic (two half steps, the faster form on UPML meshes),
- * 0 two half steps, 1 LDS sweep, 2 column sweep (ems_init then fails with
- * EMS_E_ARG when the engine has no single-sweep form). */
+ * 0 two half steps, 1 LDS sweep, 2 column sweep, 3 quad column sweep
+ * (ems_init then fails with EMS_E_ARG when the engine has no such form). */
 int ems_set_fused(ems_engine* h, int mode);
-/* fused: active form (0 two half steps, 1 LDS sweep, 2 column sweep);
+/* fused: active form (0 two half steps, 1 LDS sweep, 2 column sweep, 3 quad sweep);
  * zdepth: z planes per workgroup */
 int ems_get_fused(const ems_engine* h, int* fused, unsigned* zdepth);
 
@@ -238,6 +243,23 @@ int ems_get_volt(ems_engine* h, float* out);
 int ems_get_curr(ems_engine* h, float* out);
 int ems_set_volt(ems_engine* h, const float* in);
 int ems_set_curr(ems_engine* h, const float* in);
+
+/* Engine::GetVolt / GetCurr and SetVolt / SetCurr (FDTD/engine.h:77-101) over
+ * the box start..stop (inclusive mesh indices): comp 0, 1 or 2 for one
+ * component, data [nx][ny][nz]; comp -1 for all three, data [3][nx][ny][nz]
+ * (N-I-J-K, z fastest).  Only the box travels over PCIe.  Writes reach both
+ * buffers of a single-sweep engine. */
+int ems_read_field(ems_engine* h, int is_curr, int comp, const unsigned start[3], const unsigned stop[3], float* out);
+int ems_write_field(ems_engine* h, int is_curr, int comp, const unsigned start[3], const unsigned stop[3],
+                    const float* in);
+/* Engine_Interface_FDTD::GetEField / GetHField (FDTD/engine_interface_fdtd.cpp:63-204,
+ * GetRawField :263-268): field 0 (E, from the voltages) or 1 (H, from the
+ * currents) at mesh node pos, interp 0 none, 1 node, 2 cell interpolation
+ * (Engine_Interface_Base::InterpolationType); edge_len[3]: the edge lengths
+ * per axis in metres (primary lines for E, dual for H); out[3] in V/m or A/m.
+ * The same formulas as the field dumps (ems_add_fd_dump / ems_add_td_dump). */
+int ems_get_field_at(ems_engine* h, int field, int interp, const unsigned pos[3], const double* const edge_len[3],
+                     double out[3]);
 
 /* ---- probes (evaluated on the device at their sampling steps) ---- */
 #define EMS_PROBE_VOLTAGE 0   /* line integral of voltages, double accumulation   */

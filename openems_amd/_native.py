@@ -95,6 +95,9 @@ ENGINE_API = [
     ("ems_set_steady_state", C.c_int, [vp, C.c_uint, C.c_uint, ip, up, C.c_uint]),
     ("ems_steady_state", C.c_int, [vp, dp, up, up, dp, C.c_uint]),
     ("ems_get_fused", C.c_int, [vp, ip, up]),
+    ("ems_read_field", C.c_int, [vp, C.c_int, C.c_int, up, up, fp]),
+    ("ems_write_field", C.c_int, [vp, C.c_int, C.c_int, up, up, fp]),
+    ("ems_get_field_at", C.c_int, [vp, C.c_int, C.c_int, up, C.POINTER(dp), dp]),
     ("ems_set_quad", C.c_int, [vp, C.c_int]),
     ("ems_get_quad", C.c_int, [vp, ip, up]),
     ("ems_set_excitation", C.c_int, [vp, fp, fp, C.c_uint, C.c_int, C.c_uint, up, usp, fp, up,

@@ -1322,6 +1322,266 @@ __global__ __launch_bounds__(kBlock) void k_step2(const Step2Args a, const float
 }
 
 // ---------------------------------------------------------------------------
+// Quad column sweep: one full timestep per launch, four x cells per lane
+// (16-byte accesses), over a box of cells without UPML (k_step2's scheme with
+// quads).  Each lane owns the quad x0 .. x0+3 of row y and sweeps a z chunk;
+// per plane z it loads state n once and computes
+//   * V^{n+1}(x0..x0+3, y, z), stored;
+//   * V^{n+1} z and x components of the quad of row y+1 and z and y
+//     components of the cell (x0+4, y): the neighbours its current update
+//     reads, recomputed (not stored) with their owners' arithmetic, sources
+//     included;
+//   * I^{n+1}(x0..x0+3, y, z-1) from those, stored.
+// The box is [4*qa, 4*(qa+nq)) x [ya, ya+ny) x [za, zb); voltages are stored
+// on all of it, currents where x + 1 < xb, y + 1 < yb and z + 1 < zb (the
+// rest of the box's current update, and every cell outside the box, belongs
+// to another pass or to no update at all).  Per element the arithmetic is
+// k_step2's (and so k_volt / k_curr's): results are bit-identical.
+// ---------------------------------------------------------------------------
+struct Step4Args {
+    GridDesc g;
+    CoefArgs kV, kI;
+    ExcArgs eV, eI;                // beg: [plane + 1] per-column ranges in list order
+    const unsigned* numts;
+    unsigned qa, nq, ya, ny, za, zb, xb, yb;
+    unsigned zdepth, ntiles;       // planes per chunk; tiles of kBlock quads over nq * ny
+};
+
+// coefficient pair (c, v) of component n of the quad at p (table entries u[])
+template <bool CC>
+__device__ __forceinline__ void quad_coef_n(const CoefArgs& k, unsigned long long cs, unsigned long long p,
+                                            const unsigned u[4], int n, float4& c, float4& v)
+{
+    if (CC) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float2 t = reinterpret_cast<const float2*>(k.tab)[4 * u[j] + n];
+            f4s(c, j, t.x);
+            f4s(v, j, t.y);
+        }
+    } else {
+        c = *reinterpret_cast<const float4*>(k.cc + n * cs + p);
+        v = *reinterpret_cast<const float4*>(k.cv + n * cs + p);
+    }
+}
+
+__device__ __forceinline__ void quad_idx(const unsigned short* idx, unsigned long long p, unsigned u[4])
+{
+    const uint2 w = *reinterpret_cast<const uint2*>(idx + p);
+    u[0] = w.x & 0xFFFFu;
+    u[1] = w.x >> 16;
+    u[2] = w.y & 0xFFFFu;
+    u[3] = w.y >> 16;
+}
+
+__device__ __forceinline__ float4 quad_math(float4 x0, float4 c, float4 v, float4 curl)
+{
+    float4 o;
+    o.x = x0.x * c.x; o.x = o.x + v.x * curl.x;
+    o.y = x0.y * c.y; o.y = o.y + v.y * curl.y;
+    o.z = x0.z * c.z; o.z = o.z + v.z * curl.z;
+    o.w = x0.w * c.w; o.w = o.w + v.w * curl.w;
+    return o;
+}
+
+// a - b - c + d per element, in that order
+__device__ __forceinline__ float4 quad_curl(float4 a, float4 b, float4 c, float4 d)
+{
+    float4 o;
+    o.x = a.x - b.x - c.x + d.x;
+    o.y = a.y - b.y - c.y + d.y;
+    o.z = a.z - b.z - c.z + d.z;
+    o.w = a.w - b.w - c.w + d.w;
+    return o;
+}
+
+// the sources of columns x0 .. x0+3 (entries [b, e), list order) at plane z
+// added to components `mask` (bit n: component n) of f
+__device__ __forceinline__ void quad_sources(const ExcArgs& ex, unsigned b, unsigned e, unsigned x0, unsigned z,
+                                             unsigned mask, int ts, float4 f[3])
+{
+    for (unsigned k = b; k < e; ++k) {
+        const ExcEnt E = ex.ent[k];
+        if (E.z != z || !(mask >> E.dir & 1u)) continue;
+        const float sv = E.amp * ex.sig[exc_sample(ts, E.delay, ex.period, ex.length)];
+        const unsigned j = (unsigned)E.x - x0;
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            if (E.dir != n) continue;
+            if (j == 0) f[n].x = f[n].x + sv;
+            else if (j == 1) f[n].y = f[n].y + sv;
+            else if (j == 2) f[n].z = f[n].z + sv;
+            else f[n].w = f[n].w + sv;
+        }
+    }
+}
+
+template <bool CC, bool NT>
+__global__ __launch_bounds__(kBlock) void k_step4(const Step4Args a, const float* __restrict__ V0,
+                                                  const float* __restrict__ I0, float* __restrict__ V1,
+                                                  float* __restrict__ I1)
+{
+    const GridDesc& g = a.g;
+    const unsigned long long cs = g.cs;
+    const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
+    const unsigned qt = t % a.ntiles, zc = t / a.ntiles;
+    const unsigned L = qt * kBlock + threadIdx.x;
+    if (L >= a.nq * a.ny) return;
+    const unsigned r = L / a.nq;
+    const unsigned x0 = 4u * (a.qa + (L - r * a.nq)), y = a.ya + r;
+    const int ts = (int)a.numts[0];
+    const unsigned Nx = g.Nx;
+    const unsigned long long plane = g.plane;
+    const unsigned z0 = a.za + zc * a.zdepth;
+    const unsigned z1 = min(z0 + a.zdepth, a.zb);
+    // the plane after the chunk is recomputed for the currents of its last plane
+    const unsigned zlast = z1 < a.zb ? z1 : z1 - 1;
+    const unsigned q = x0 + Nx * y;
+    const bool yupd = y + 1 < a.yb;                    // the row's current update runs here
+    const bool xr = x0 + 4 < a.xb;                     // element 3's current update runs here
+    const unsigned imask = !yupd ? 0u : xr ? 15u : (x0 + 3 < a.xb ? 7u : x0 + 2 < a.xb ? 3u : x0 + 1 < a.xb ? 1u : 0u);
+    const unsigned long long ym = y ? (unsigned long long)Nx : 0ull;
+    // neighbours read by the current update (the cell itself where none runs)
+    const unsigned long long du = yupd ? (unsigned long long)Nx : 0ull, dr = xr ? 4ull : 0ull;
+    const unsigned evb = a.eV.beg[q], eve = a.eV.beg[q + 4];
+    const unsigned eib = a.eI.beg[q], eie = a.eI.beg[q + 4];
+    const unsigned eub = a.eV.beg[q + du], eue = yupd ? a.eV.beg[q + du + 4] : eub;
+    const unsigned erb = a.eV.beg[q + dr], ere = xr ? a.eV.beg[q + dr + 1] : erb;
+    // I^n at z-1 (the cell itself at z = 0)
+    const unsigned long long pz = z0 ? (unsigned long long)(z0 - 1) * plane + q : (unsigned long long)q;
+    float4 ix_m = *reinterpret_cast<const float4*>(I0 + pz);
+    float4 iy_m = *reinterpret_cast<const float4*>(I0 + cs + pz);
+    float4 iz_m = *reinterpret_cast<const float4*>(I0 + 2 * cs + pz);
+    float4 iyu_m = *reinterpret_cast<const float4*>(I0 + cs + pz + du);
+    float ixr_m = I0[pz + dr];
+    // plane z-1 carried for the current update
+    float4 vm[3] = {}, vu2m = {}, vu0m = {};
+    float vr2m = 0.0f, vr1m = 0.0f;
+    unsigned um[4] = {0u, 0u, 0u, 0u};
+    for (unsigned z = z0; z <= zlast; ++z) {
+        const unsigned long long p = (unsigned long long)z * plane + q;
+        // ---- loads: state n around the quad
+        const float4 ix = *reinterpret_cast<const float4*>(I0 + p);
+        const float4 iy = *reinterpret_cast<const float4*>(I0 + cs + p);
+        const float4 iz = *reinterpret_cast<const float4*>(I0 + 2 * cs + p);
+        const float4 iz_ym = *reinterpret_cast<const float4*>(I0 + 2 * cs + p - ym);
+        const float4 ix_ym = *reinterpret_cast<const float4*>(I0 + p - ym);
+        float4 iz_xm, iy_xm;
+        if (x0) {
+            iz_xm = ld4u(I0 + 2 * cs + p - 1);
+            iy_xm = ld4u(I0 + cs + p - 1);
+        } else {
+            iz_xm = make_float4(iz.x, iz.x, iz.y, iz.z);
+            iy_xm = make_float4(iy.x, iy.x, iy.y, iy.z);
+        }
+        const unsigned long long pu = p + du;
+        const float4 ixu = *reinterpret_cast<const float4*>(I0 + pu);
+        const float4 iyu = *reinterpret_cast<const float4*>(I0 + cs + pu);
+        const float4 izu = *reinterpret_cast<const float4*>(I0 + 2 * cs + pu);
+        float4 iyu_xm;
+        if (x0) iyu_xm = ld4u(I0 + cs + pu - 1);
+        else iyu_xm = make_float4(iyu.x, iyu.x, iyu.y, iyu.z);
+        const unsigned long long pr = p + dr;
+        const float ixr = I0[pr], iyr = I0[cs + pr], izr = I0[2 * cs + pr], ixr_ym = I0[pr - ym];
+        float4 v0[3];
+        v0[0] = ld4<NT>(V0 + p);
+        v0[1] = ld4<NT>(V0 + cs + p);
+        v0[2] = ld4<NT>(V0 + 2 * cs + p);
+        const float4 u0z = *reinterpret_cast<const float4*>(V0 + 2 * cs + pu);
+        const float4 u0x = *reinterpret_cast<const float4*>(V0 + pu);
+        const float r0z = V0[2 * cs + pr], r0y = V0[cs + pr];
+        unsigned u[4] = {0u, 0u, 0u, 0u}, uu[4] = {0u, 0u, 0u, 0u};
+        unsigned ur = 0u;
+        if (CC) {
+            quad_idx(a.kV.idx, p, u);
+            quad_idx(a.kV.idx, pu, uu);
+            ur = a.kV.idx[pr];
+        }
+        // ---- own V^{n+1}(x0..x0+3, y, z)
+        float4 v[3];
+        {
+            float4 c, w;
+            quad_coef_n<CC>(a.kV, cs, p, u, 0, c, w);
+            v[0] = quad_math(v0[0], c, w, quad_curl(iz, iz_ym, iy, iy_m));
+            quad_coef_n<CC>(a.kV, cs, p, u, 1, c, w);
+            v[1] = quad_math(v0[1], c, w, quad_curl(ix, ix_m, iz, iz_xm));
+            quad_coef_n<CC>(a.kV, cs, p, u, 2, c, w);
+            v[2] = quad_math(v0[2], c, w, quad_curl(iy, iy_xm, ix, ix_ym));
+        }
+        if (evb != eve) quad_sources(a.eV, evb, eve, x0, z, 7u, ts, v);
+        // ---- neighbours: row y+1 (z and x components), cell (x0+4, y) (z and y)
+        float4 nu[3];
+        {
+            float4 c, w;
+            quad_coef_n<CC>(a.kV, cs, pu, uu, 2, c, w);
+            nu[2] = quad_math(u0z, c, w, quad_curl(iyu, iyu_xm, ixu, ix));
+            quad_coef_n<CC>(a.kV, cs, pu, uu, 0, c, w);
+            nu[0] = quad_math(u0x, c, w, quad_curl(izu, iz, iyu, iyu_m));
+            nu[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        if (eub != eue) quad_sources(a.eV, eub, eue, x0, z, 5u, ts, nu);
+        float nrz, nry;
+        {
+            const float2 c2 = coef_n<CC>(a.kV, cs, pr, ur, 2), c1 = coef_n<CC>(a.kV, cs, pr, ur, 1);
+            const float c2r = iyr - iy.w - ixr + ixr_ym;
+            const float c1r = ixr - ixr_m - izr + iz.w;
+            nrz = r0z * c2.x;
+            nrz = nrz + c2.y * c2r;
+            nry = r0y * c1.x;
+            nry = nry + c1.y * c1r;
+            if (erb != ere) {
+                nrz = col_sources(a.eV, erb, ere, z, 2, ts, nrz);
+                nry = col_sources(a.eV, erb, ere, z, 1, ts, nry);
+            }
+        }
+        // ---- I^{n+1}(x0..x0+3, y, z-1)
+        if (z > z0 && z < a.zb && imask) {
+            const unsigned long long pm = p - plane;
+            float4 io[3];
+            const float4 vzr = make_float4(vm[2].y, vm[2].z, vm[2].w, vr2m);
+            const float4 vyr = make_float4(vm[1].y, vm[1].z, vm[1].w, vr1m);
+            float4 c, w;
+            quad_coef_n<CC>(a.kI, cs, pm, um, 0, c, w);
+            io[0] = quad_math(ix_m, c, w, quad_curl(vm[2], vu2m, vm[1], v[1]));
+            quad_coef_n<CC>(a.kI, cs, pm, um, 1, c, w);
+            io[1] = quad_math(iy_m, c, w, quad_curl(vm[0], v[0], vm[2], vzr));
+            quad_coef_n<CC>(a.kI, cs, pm, um, 2, c, w);
+            io[2] = quad_math(iz_m, c, w, quad_curl(vm[1], vyr, vm[0], vu0m));
+            if (eib != eie) quad_sources(a.eI, eib, eie, x0, z - 1, 7u, ts, io);
+            if (imask == 15u) {
+#pragma unroll
+                for (int n = 0; n < 3; ++n) st4<NT>(I1 + n * cs + pm, io[n]);
+            } else {
+#pragma unroll
+                for (int n = 0; n < 3; ++n)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        if (imask >> j & 1u) I1[n * cs + pm + j] = f4g(io[n], j);
+            }
+        }
+        if (z < z1) {
+#pragma unroll
+            for (int n = 0; n < 3; ++n) st4<NT>(V1 + n * cs + p, v[n]);
+        }
+        // ---- carry plane z
+        vm[0] = v[0];
+        vm[1] = v[1];
+        vm[2] = v[2];
+        vu2m = nu[2];
+        vu0m = nu[0];
+        vr2m = nrz;
+        vr1m = nry;
+        ix_m = ix;
+        iy_m = iy;
+        iz_m = iz;
+        iyu_m = iyu;
+        ixr_m = ixr;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) um[j] = u[j];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // ++numTS after a single-sweep step (Engine::IterateTS, FDTD/engine.cpp:283).
 // Every workgroup of the step reads the counter, so the step itself cannot
 // advance it; a one-thread launch after it does, with no fence or atomic
@@ -1671,57 +1931,74 @@ __host__ __device__ inline bool dump_sample(const DumpDev& d, unsigned ts, unsig
     return *k < d.nsamples;
 }
 
-// GetRawField / GetRawDualField (engine_interface_fdtd.cpp:263-268,134-146):
-// the stored value over its edge length, zero for a zero length
-__device__ inline double dump_raw(const float* F, unsigned long long off, double delta)
-{
-    return delta ? (double)F[off] / delta : 0.0;
-}
-
 // Engine_Interface_FDTD::GetRawInterpolatedField (E, engine_interface_fdtd.cpp:63-124)
-// and GetRawInterpolatedDualField (H, :148-204), component n at node pos, in double
-__device__ inline double dump_value(const DumpDev& d, const float* F, unsigned long long cs, const unsigned* pos,
-                                    unsigned long long base, const unsigned long long* stride, int n)
+// and GetRawInterpolatedDualField (H, :148-204), component n at node pos, in
+// double; without interpolation GetRawField / GetRawDualField (:263-268,
+// :134-146), the stored value over its edge length, zero for a zero length.  raw(dx, dy, dz) is component n's stored value at pos + (dx, dy, dz)
+// (offsets -1, 0, +1); el: the component's edge lengths (primary for E, dual
+// for H).  Shared by the dump kernel and the host-side point read.
+template <class Raw>
+__host__ __device__ inline double interp_value(int field, int interp, const unsigned* N, const double* el,
+                                               const unsigned* pos, int n, Raw raw)
 {
     const int nP = (n + 1) % 3, nPP = (n + 2) % 3;
-    const float* C = F + (unsigned long long)n * cs;
+    // the value at pos + sa*e_a + sb*e_b over delta (zero for a zero length)
+    auto rv = [&](int a, int sa, int b, int sb, double delta) -> double {
+        int o[3] = {0, 0, 0};
+        if (a >= 0) o[a] += sa;
+        if (b >= 0) o[b] += sb;
+        return delta ? (double)raw(o[0], o[1], o[2]) / delta : 0.0;
+    };
     const unsigned pn = pos[n];
-    const double* el = d.el[n];
-    if (d.interp == 0) return dump_raw(C, base, el[pn]);
-    if (d.field == 0) {
-        if (d.interp == 1) {
-            if (pn == d.N[n] - 1) return dump_raw(C, base - stride[n], el[pn - 1]);
+    if (interp == 0) return rv(-1, 0, -1, 0, el[pn]);
+    if (field == 0) {
+        if (interp == 1) {
+            if (pn == N[n] - 1) return rv(n, -1, -1, 0, el[pn - 1]);
             const double delta = el[pn];
-            double out = dump_raw(C, base, delta);
+            double out = rv(-1, 0, -1, 0, delta);
             if (delta == 0.0) return 0.0;
             if (pn == 0) return out;
             const double deltaDown = el[pn - 1];
             const double deltaRel = delta / (delta + deltaDown);
-            return out * (1.0 - deltaRel) + dump_raw(C, base - stride[n], deltaDown) * deltaRel;
+            return out * (1.0 - deltaRel) + rv(n, -1, -1, 0, deltaDown) * deltaRel;
         }
-        if (pos[0] == d.N[0] - 1 || pos[1] == d.N[1] - 1 || pos[2] == d.N[2] - 1) return 0.0;
+        if (pos[0] == N[0] - 1 || pos[1] == N[1] - 1 || pos[2] == N[2] - 1) return 0.0;
         const double delta = el[pn];
-        double out = dump_raw(C, base, delta);
-        out += dump_raw(C, base + stride[nP], delta);
-        out += dump_raw(C, base + stride[nP] + stride[nPP], delta);
-        out += dump_raw(C, base + stride[nPP], delta);
+        double out = rv(-1, 0, -1, 0, delta);
+        out += rv(nP, 1, -1, 0, delta);
+        out += rv(nP, 1, nPP, 1, delta);
+        out += rv(nPP, 1, -1, 0, delta);
         return out / 4;
     }
-    if (d.interp == 1) {
-        if (pos[0] == d.N[0] - 1 || pos[1] == d.N[1] - 1 || pos[2] == d.N[2] - 1 || pos[nP] == 0 || pos[nPP] == 0)
+    if (interp == 1) {
+        if (pos[0] == N[0] - 1 || pos[1] == N[1] - 1 || pos[2] == N[2] - 1 || pos[nP] == 0 || pos[nPP] == 0)
             return 0.0;
         const double delta = el[pn];
-        double out = dump_raw(C, base, delta);
-        out += dump_raw(C, base - stride[nP], delta);
-        out += dump_raw(C, base - stride[nP] - stride[nPP], delta);
-        out += dump_raw(C, base - stride[nPP], delta);
+        double out = rv(-1, 0, -1, 0, delta);
+        out += rv(nP, -1, -1, 0, delta);
+        out += rv(nP, -1, nPP, -1, delta);
+        out += rv(nPP, -1, -1, 0, delta);
         return out / 4;
     }
-    if (pn >= d.N[n] - 1) return 0.0;
+    if (pn >= N[n] - 1) return 0.0;
     const double delta = el[pn];
     const double deltaUp = el[pn + 1];
     const double deltaRel = delta / (delta + deltaUp);
-    return dump_raw(C, base, delta) * (1.0 - deltaRel) + dump_raw(C, base + stride[n], deltaUp) * deltaRel;
+    return rv(-1, 0, -1, 0, delta) * (1.0 - deltaRel) + rv(n, 1, -1, 0, deltaUp) * deltaRel;
+}
+
+__device__ inline double dump_value(const DumpDev& d, const float* F, unsigned long long cs, const unsigned* pos,
+                                    unsigned long long base, const unsigned long long* stride, int n)
+{
+    const float* C = F + (unsigned long long)n * cs;
+    auto raw = [&](int dx, int dy, int dz) -> float {
+        long long off = (long long)base;
+        off += (long long)dx * (long long)stride[0];
+        off += (long long)dy * (long long)stride[1];
+        off += (long long)dz * (long long)stride[2];
+        return C[off];
+    };
+    return interp_value(d.field, d.interp, d.N, d.el[n], pos, n, raw);
 }
 
 // ProcessFieldsFD::Process (processfields_fd.cpp:72-107): every fd_interval
@@ -2229,7 +2506,7 @@ struct ems_engine {
     PmlArgs pmlV{}, pmlI{};
     float* flux_alt[2] = {nullptr, nullptr};   // second flux buffers of the fused step (V, I)
     // fused single-sweep step
-    int fused_req = -1;                   // -1 auto, 0 two half steps, 1 LDS sweep, 2 column sweep
+    int fused_req = -1;                   // -1 auto, 0 two half steps, 1 LDS sweep, 2 column sweep, 3 quad sweep
     int fused = 0;                        // active form: 0 two half steps, 1 k_step, 2 k_step2
     bool nt_fields = false;               // non-temporal own-field accesses in the half steps
     int quad_req = -1;                    // -1 auto, 0 one cell per lane, 1 quads (four x cells per lane)
@@ -2946,13 +3223,17 @@ static int setup_fused(ems_engine* h)
     // tile and its halo, column x0+kTX and row y0+kTY; current sources of the
     // tile) or per (x, y) column (form 2)
     const unsigned ntx = (Nx + kTX - 1) / kTX, nty = (Ny + kTY - 1) / kTY;
+    if (why.empty() && form == 3) {
+        if (h->pmlV.n) why = "the quad sweep has no UPML form";
+        else if (Nx % 4) why = "the quad sweep needs a multiple of 4 x lines";
+    }
     const unsigned nlists = form == 1 ? ntx * nty : h->g.plane;
     std::vector<std::vector<ExcEnt>> lists[2];
     size_t most = 0;
     for (int c = 0; c < 2 && why.empty(); ++c) {
         lists[c].assign(nlists, {});
         for (const auto& e : h->exc[c].host) {
-            if (form == 2) {
+            if (form >= 2) {
                 lists[c][e.x + Nx * (unsigned)e.y].push_back(e);
                 continue;
             }
@@ -3002,6 +3283,18 @@ static int setup_fused(ems_engine* h)
         h->nty = nty;
         h->fzdepth = choose_fused_zdepth(ntx * nty, Nz);
         h->fnzc = (Nz + h->fzdepth - 1) / h->fzdepth;
+    } else if (form == 3) {
+        // tiles of kBlock quads over the plane; 8 planes per chunk, shallower
+        // when the grid would give fewer than 2048 workgroups
+        h->ntx = (h->g.plane / 4 + kBlock - 1) / kBlock;
+        unsigned d = 8;
+        while (d > 2 && (unsigned long long)h->ntx * ((Nz + d - 1) / d) < 2048ull) d /= 2;
+        if (const char* zd = std::getenv("EMS_Q4ZDEPTH")) {
+            const int v = std::atoi(zd);
+            if (v >= 1 && v <= 1024) d = (unsigned)v;
+        }
+        h->fzdepth = d;
+        h->fnzc = (Nz + d - 1) / d;
     } else {
         // the two-half-step tiles (kBlock consecutive columns) and z depth
         h->fzdepth = h->g.zdepth;
@@ -3015,7 +3308,7 @@ int ems_set_fused(ems_engine* h, int mode)
 {
     if (!h) return fail(EMS_E_ARG, "null handle");
     if (h->inited) return fail(EMS_E_STATE, "engine already initialised");
-    if (mode < -1 || mode > 2)
+    if (mode < -1 || mode > 3)
         return fail(EMS_E_ARG, "step form is -1 (auto), 0 (two half steps), 1 (LDS sweep) or 2 (column sweep)");
     h->fused_req = mode;
     return 0;
@@ -3487,12 +3780,49 @@ void column_step(ems_engine* h)
     }
 }
 
+template <bool CC>
+void launch_step4_t(ems_engine* h, const Step4Args& a, unsigned nblk)
+{
+    if (h->nt_fields)
+        hipLaunchKernelGGL((k_step4<CC, true>), dim3(nblk), dim3(kBlock), 0, h->stream, a, h->V, h->I, h->V_alt,
+                           h->I_alt);
+    else
+        hipLaunchKernelGGL((k_step4<CC, false>), dim3(nblk), dim3(kBlock), 0, h->stream, a, h->V, h->I, h->V_alt,
+                           h->I_alt);
+}
+
+// the quad column sweep over the whole mesh (no UPML)
+void quad_step(ems_engine* h)
+{
+    Step4Args a{};
+    a.g = h->g;
+    a.kV = h->kV;
+    a.kI = h->kI;
+    a.eV = h->eV;
+    a.eI = h->eI;
+    a.numts = h->numts_dev;
+    a.qa = 0;
+    a.nq = h->N[0] / 4;
+    a.ya = 0;
+    a.ny = h->N[1];
+    a.za = 0;
+    a.zb = h->N[2];
+    a.xb = h->N[0];
+    a.yb = h->N[1];
+    a.zdepth = h->fzdepth;
+    a.ntiles = h->ntx;
+    const unsigned nblk = h->ntx * h->fnzc;
+    if (h->cmp) launch_step4_t<true>(h, a, nblk);
+    else launch_step4_t<false>(h, a, nblk);
+}
+
 // one fused step from state n (h->V, h->I, current fluxes) into the second
 // buffers, which then become the current state
 void fused_step(ems_engine* h, hipEvent_t* ev)
 {
     if (ev) (void)hipEventRecord(ev[0], h->stream);
-    if (h->fused == 2) column_step(h);
+    if (h->fused == 3) quad_step(h);
+    else if (h->fused == 2) column_step(h);
     else lds_step(h);
     if (ev) {
         (void)hipEventRecord(ev[1], h->stream);
@@ -3859,6 +4189,110 @@ int ems_set_volt(ems_engine* h, const float* in)
 int ems_set_curr(ems_engine* h, const float* in)
 {
     return h ? set_field(h, h->I, h->I_alt, in) : fail(EMS_E_ARG, "null handle");
+}
+
+// one component's box [s, s+n) as x-fastest planes, through pitched copies
+// of the device rows (dir: hipMemcpyDeviceToHost or HostToDevice)
+static int box_copy(ems_engine* h, float* dev, float* host_xfast, const unsigned s[3], const unsigned n[3],
+                    hipMemcpyKind dir)
+{
+    const size_t Nx = h->N[0], Ny = h->N[1];
+    for (unsigned z = 0; z < n[2]; ++z) {
+        float* d = dev + s[0] + Nx * (s[1] + Ny * (size_t)(s[2] + z));
+        float* q = host_xfast + (size_t)z * n[0] * n[1];
+        if (dir == hipMemcpyDeviceToHost)
+            HIPCHK(hipMemcpy2D(q, n[0] * sizeof(float), d, Nx * sizeof(float), n[0] * sizeof(float), n[1], dir));
+        else
+            HIPCHK(hipMemcpy2D(d, Nx * sizeof(float), q, n[0] * sizeof(float), n[0] * sizeof(float), n[1], dir));
+    }
+    return 0;
+}
+
+static int box_check(const ems_engine* h, int comp, const unsigned start[3], const unsigned stop[3], unsigned n[3])
+{
+    if (comp < -1 || comp > 2) return fail(EMS_E_ARG, "field component must be 0, 1, 2 or -1 (all three)");
+    for (int a = 0; a < 3; ++a) {
+        if (start[a] > stop[a] || stop[a] >= h->N[a]) return fail(EMS_E_ARG, "field box outside the mesh");
+        n[a] = stop[a] - start[a] + 1;
+    }
+    return 0;
+}
+
+int ems_read_field(ems_engine* h, int is_curr, int comp, const unsigned start[3], const unsigned stop[3], float* out)
+{
+    if (!h || !start || !stop || !out) return fail(EMS_E_ARG, "null argument");
+    unsigned n[3];
+    int rc = box_check(h, comp, start, stop, n);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const size_t cnt = (size_t)n[0] * n[1] * n[2];
+    std::vector<float> stage(cnt);
+    const int c0 = comp < 0 ? 0 : comp, c1 = comp < 0 ? 2 : comp;
+    float* F = is_curr ? h->I : h->V;
+    for (int c = c0; c <= c1; ++c) {
+        if ((rc = box_copy(h, F + (size_t)c * h->cs, stage.data(), start, n, hipMemcpyDeviceToHost))) return rc;
+        float* o = out + (size_t)(c - c0) * cnt;
+        for (unsigned x = 0; x < n[0]; ++x)
+            for (unsigned y = 0; y < n[1]; ++y)
+                for (unsigned z = 0; z < n[2]; ++z)
+                    o[((size_t)x * n[1] + y) * n[2] + z] = stage[x + (size_t)n[0] * (y + (size_t)n[1] * z)];
+    }
+    return 0;
+}
+
+int ems_write_field(ems_engine* h, int is_curr, int comp, const unsigned start[3], const unsigned stop[3],
+                    const float* in)
+{
+    if (!h || !start || !stop || !in) return fail(EMS_E_ARG, "null argument");
+    unsigned n[3];
+    int rc = box_check(h, comp, start, stop, n);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const size_t cnt = (size_t)n[0] * n[1] * n[2];
+    std::vector<float> stage(cnt);
+    const int c0 = comp < 0 ? 0 : comp, c1 = comp < 0 ? 2 : comp;
+    // the fused step's second buffer must agree on the cells a step leaves alone
+    float* bufs[2] = {is_curr ? h->I : h->V, is_curr ? h->I_alt : h->V_alt};
+    for (int c = c0; c <= c1; ++c) {
+        const float* src = in + (size_t)(c - c0) * cnt;
+        for (unsigned x = 0; x < n[0]; ++x)
+            for (unsigned y = 0; y < n[1]; ++y)
+                for (unsigned z = 0; z < n[2]; ++z)
+                    stage[x + (size_t)n[0] * (y + (size_t)n[1] * z)] = src[((size_t)x * n[1] + y) * n[2] + z];
+        for (float* F : bufs)
+            if (F && (rc = box_copy(h, F + (size_t)c * h->cs, stage.data(), start, n, hipMemcpyHostToDevice))) return rc;
+    }
+    return 0;
+}
+
+int ems_get_field_at(ems_engine* h, int field, int interp, const unsigned pos[3], const double* const edge_len[3],
+                     double out[3])
+{
+    if (!h || !pos || !edge_len || !out) return fail(EMS_E_ARG, "null argument");
+    if (field != 0 && field != 1) return fail(EMS_E_ARG, "field must be 0 (E) or 1 (H)");
+    if (interp < 0 || interp > 2) return fail(EMS_E_ARG, "interpolation must be 0 (none), 1 (node) or 2 (cell)");
+    unsigned lo[3], hi[3], n[3];
+    for (int a = 0; a < 3; ++a) {
+        if (pos[a] >= h->N[a]) return fail(EMS_E_ARG, "position outside the mesh");
+        if (!edge_len[a]) return fail(EMS_E_ARG, "null edge lengths");
+        lo[a] = pos[a] ? pos[a] - 1 : 0;
+        hi[a] = std::min(pos[a] + 1, h->N[a] - 1);
+        n[a] = hi[a] - lo[a] + 1;
+    }
+    std::vector<float> box(3 * (size_t)n[0] * n[1] * n[2]);
+    int rc = ems_read_field(h, field, -1, lo, hi, box.data());
+    if (rc) return rc;
+    for (int c = 0; c < 3; ++c) {
+        const float* C = box.data() + (size_t)c * n[0] * n[1] * n[2];
+        auto raw = [&](int dx, int dy, int dz) -> float {
+            const unsigned x = pos[0] + dx - lo[0], y = pos[1] + dy - lo[1], z = pos[2] + dz - lo[2];
+            return C[((size_t)x * n[1] + y) * n[2] + z];
+        };
+        out[c] = interp_value(field, interp, h->N, edge_len[c], pos, c, raw);
+    }
+    return 0;
 }
 
 int ems_energy(ems_engine* h, double* out)

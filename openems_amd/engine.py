@@ -58,7 +58,7 @@ def expand_sse_compressed(N, op_index, vv4, vi4, ii4, iv4):
     return out
 
 
-STEP_FORMS = {"halves": 0, "lds": 1, "columns": 2}
+STEP_FORMS = {"halves": 0, "lds": 1, "columns": 2, "quads": 3}
 
 
 class EngineHIP:
@@ -198,6 +198,32 @@ class EngineHIP:
     def set_curr(self, a):
         a = np.ascontiguousarray(a, np.float32)
         check(lib().ems_set_curr(self.h, ptr(a, C.c_float)))
+
+    def read_field(self, curr, start, stop, comp=-1):
+        """Voltages (curr False) or currents of the box start..stop (inclusive
+        mesh indices): [3][nx][ny][nz] for comp -1, else [nx][ny][nz]."""
+        st = (C.c_uint * 3)(*[int(v) for v in start])
+        sp = (C.c_uint * 3)(*[int(v) for v in stop])
+        n = tuple(int(b) - int(a) + 1 for a, b in zip(start, stop))
+        out = np.empty(((3,) if comp < 0 else ()) + n, np.float32)
+        check(lib().ems_read_field(self.h, int(bool(curr)), int(comp), st, sp, ptr(out, C.c_float)))
+        return out
+
+    def write_field(self, curr, start, stop, values, comp=-1):
+        st = (C.c_uint * 3)(*[int(v) for v in start])
+        sp = (C.c_uint * 3)(*[int(v) for v in stop])
+        a = np.ascontiguousarray(values, np.float32)
+        check(lib().ems_write_field(self.h, int(bool(curr)), int(comp), st, sp, ptr(a, C.c_float)))
+
+    def field_at(self, field, pos, interp=0):
+        """E (field 0) or H (field 1) at mesh node pos with interpolation 0
+        (none), 1 (node) or 2 (cell): Engine_Interface_FDTD::GetEField / GetHField."""
+        el = [np.ascontiguousarray(self.op.edge_lengths(n, dual=bool(field)), np.float64) for n in range(3)]
+        arr = (dp * 3)(*[ptr(e, C.c_double) for e in el])
+        p = (C.c_uint * 3)(*[int(v) for v in pos])
+        out = np.zeros(3, np.float64)
+        check(lib().ems_get_field_at(self.h, int(field), int(interp), p, arr, ptr(out, C.c_double)))
+        return out
 
     def energy(self):
         e = C.c_double()
