@@ -230,7 +230,7 @@ def _field_digest(eng):
     return h.hexdigest()
 
 
-def run_batched(count, steps, tune_steps=6):
+def run_batched(count, steps, tune_steps=6, sequential=True):
     """Batched parameter sweep: `count` independent instances of uneven size
     (scenes.batched_sweep, U{48..160}^3 lines, 10 bricks, 6-cell UPML, Gaussian
     soft source, voltage + E-point probes).  Times the instances one after
@@ -245,7 +245,7 @@ def run_batched(count, steps, tune_steps=6):
     cells = sum(int(np.prod([len(l) for l in s["lines"]])) for s in scs)
     seq_s = 0.0
     digests, tds = [], []
-    for sc in scs:
+    for sc in (scs if sequential else []):
         sim = Simulation(sc)
         sim.eng.sync()
         t0 = time.perf_counter()
@@ -258,7 +258,7 @@ def run_batched(count, steps, tune_steps=6):
     print(f"# batched leg: sequential {seq_s:.2f} s for {count} instances", file=sys.stderr, flush=True)
     # the same instances one after another on the plain-coefficient engine
     seq_plain_s = 0.0
-    for sc in scs:
+    for sc in (scs if sequential else []):
         sim = Simulation(sc, layout="plain")
         sim.eng.sync()
         t0 = time.perf_counter()
@@ -273,7 +273,7 @@ def run_batched(count, steps, tune_steps=6):
     t0 = time.perf_counter()
     b.iterate(steps)
     bat_s = time.perf_counter() - t0
-    same = True
+    same = bool(sequential)
     for sim, d, td in zip(b.sims, digests, tds):
         if _field_digest(sim.eng) != d:
             same = False
@@ -284,12 +284,12 @@ def run_batched(count, steps, tune_steps=6):
     out = {"workload": f"batched_sweep: {count} instances, U{{48..160}}^3 lines (total {cells} cells), "
                        f"10 bricks, 6-cell UPML, Gaussian soft source, V + E probes, {steps} steps",
            "instances": count, "cells": cells, "steps": steps,
-           "sequential_s": round(seq_s, 3), "batched_s": round(bat_s, 3),
-           "speedup": round(seq_s / bat_s, 3),
-           "sequential_plain_s": round(seq_plain_s, 3),
-           "speedup_vs_plain_engine": round(seq_plain_s / bat_s, 3),
+           "sequential_s": round(seq_s, 3) if sequential else None, "batched_s": round(bat_s, 3),
+           "speedup": round(seq_s / bat_s, 3) if sequential else None,
+           "sequential_plain_s": round(seq_plain_s, 3) if sequential else None,
+           "speedup_vs_plain_engine": round(seq_plain_s / bat_s, 3) if sequential else None,
            "batched_mcells": round(cells * steps / bat_s / 1e6, 1),
-           "sequential_mcells": round(cells * steps / seq_s / 1e6, 1),
+           "sequential_mcells": round(cells * steps / seq_s / 1e6, 1) if sequential else None,
            "bit_identical": bool(same), "layout": b.layout,
            "autotuned": {"tile_x": cfg["tile_x"], "grid": cfg["grid"], "work_items": cfg["work_items"],
                          "groups": cfg["groups"], "group_bytes": cfg["group_bytes"], "streams": b.streams(),
@@ -449,6 +449,8 @@ def main():
     ap.add_argument("--batched-count", type=int, default=64)
     ap.add_argument("--batched-steps", type=int, default=4000)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="batched leg without the one-after-another reference runs (for profiling the batch)")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-memory-scale", action="store_true",
                     help="skip the 1024x1024x768 Mur workload (its operator build holds about 80 GB of host memory)")
@@ -579,7 +581,7 @@ def main():
         print(f"# plain cavity: {out['plain_cavity']}", file=sys.stderr, flush=True)
     if not args.no_batched:
         try:
-            out["batched"] = run_batched(args.batched_count, args.batched_steps)
+            out["batched"] = run_batched(args.batched_count, args.batched_steps, sequential=not args.no_sequential)
         except Exception as e:  # reported, the headline line still prints
             out["batched"] = {"error": str(e)}
         print(f"# batched: {out['batched']}", file=sys.stderr)

@@ -298,42 +298,44 @@ public:
     {
         if (h_) ems_destroy(h_);
         h_ = nullptr;
-        volt_.clear();
-        curr_.clear();
-        have_v_ = have_c_ = dirty_v_ = dirty_c_ = false;
+        row_valid_ = false;
     }
 
-    // Engine::IterateTS: fields changed through SetVolt / SetCurr go back first
+    // Engine::IterateTS
     virtual bool IterateTS(unsigned int iterTS)
     {
-        flush();
         check_engine(ems_iterate(h_, iterTS));
-        have_v_ = have_c_ = false;
+        row_valid_ = false;
         return true;
     }
     virtual unsigned int GetNumberOfTimesteps() { return ems_get_numts(h_); }
 
-    FDTD_FLOAT GetVolt(unsigned n, unsigned x, unsigned y, unsigned z) { return field(false)[idx(n, x, y, z)]; }
+    // Engine::GetVolt / GetCurr: the x row holding the element is read from
+    // the device once (ems_read_field) and kept until the next step or write
+    FDTD_FLOAT GetVolt(unsigned n, unsigned x, unsigned y, unsigned z) { return row(false, n, y, z)[x]; }
     FDTD_FLOAT GetVolt(unsigned n, const unsigned pos[3]) { return GetVolt(n, pos[0], pos[1], pos[2]); }
-    FDTD_FLOAT GetCurr(unsigned n, unsigned x, unsigned y, unsigned z) { return field(true)[idx(n, x, y, z)]; }
+    FDTD_FLOAT GetCurr(unsigned n, unsigned x, unsigned y, unsigned z) { return row(true, n, y, z)[x]; }
     FDTD_FLOAT GetCurr(unsigned n, const unsigned pos[3]) { return GetCurr(n, pos[0], pos[1], pos[2]); }
-    void SetVolt(unsigned n, unsigned x, unsigned y, unsigned z, FDTD_FLOAT value)
-    {
-        field(false)[idx(n, x, y, z)] = value;
-        dirty_v_ = true;
-    }
+    // Engine::SetVolt / SetCurr: written through to the device (one element)
+    void SetVolt(unsigned n, unsigned x, unsigned y, unsigned z, FDTD_FLOAT value) { set(false, n, x, y, z, value); }
     void SetVolt(unsigned n, const unsigned pos[3], FDTD_FLOAT value) { SetVolt(n, pos[0], pos[1], pos[2], value); }
-    void SetCurr(unsigned n, unsigned x, unsigned y, unsigned z, FDTD_FLOAT value)
-    {
-        field(true)[idx(n, x, y, z)] = value;
-        dirty_c_ = true;
-    }
+    void SetCurr(unsigned n, unsigned x, unsigned y, unsigned z, FDTD_FLOAT value) { set(true, n, x, y, z, value); }
     void SetCurr(unsigned n, const unsigned pos[3], FDTD_FLOAT value) { SetCurr(n, pos[0], pos[1], pos[2], value); }
+
+    // a box of voltages or currents, [3][nx][ny][nz] (comp -1) or [nx][ny][nz]
+    void ReadBox(bool curr, int comp, const unsigned start[3], const unsigned stop[3], FDTD_FLOAT* out)
+    {
+        check_engine(ems_read_field(h_, curr ? 1 : 0, comp, start, stop, out));
+    }
+    void WriteBox(bool curr, int comp, const unsigned start[3], const unsigned stop[3], const FDTD_FLOAT* in)
+    {
+        check_engine(ems_write_field(h_, curr ? 1 : 0, comp, start, stop, in));
+        row_valid_ = false;
+    }
 
     // Engine_Interface_FDTD::CalcFastEnergy
     double CalcFastEnergy()
     {
-        flush();
         double e = 0;
         check_engine(ems_energy(h_, &e));
         return e;
@@ -345,34 +347,35 @@ protected:
     Engine_HIP(const Operator_HIP* op, int device) : op_(op), device_(device) {}
 
 private:
-    size_t idx(unsigned n, unsigned x, unsigned y, unsigned z) const
+    const std::vector<float>& row(bool curr, unsigned n, unsigned y, unsigned z)
     {
-        return (((size_t)n * N_[0] + x) * N_[1] + y) * N_[2] + z;
-    }
-    std::vector<float>& field(bool curr)
-    {
-        std::vector<float>& f = curr ? curr_ : volt_;
-        bool& have = curr ? have_c_ : have_v_;
-        if (!have) {
-            f.resize(3 * (size_t)N_[0] * N_[1] * N_[2]);
-            check_engine(curr ? ems_get_curr(h_, f.data()) : ems_get_volt(h_, f.data()));
-            have = true;
+        if (!row_valid_ || row_key_[0] != (unsigned)curr || row_key_[1] != n || row_key_[2] != y || row_key_[3] != z) {
+            const unsigned a[3] = {0, y, z}, b[3] = {N_[0] - 1, y, z};
+            row_.resize(N_[0]);
+            check_engine(ems_read_field(h_, curr ? 1 : 0, (int)n, a, b, row_.data()));
+            row_key_[0] = curr;
+            row_key_[1] = n;
+            row_key_[2] = y;
+            row_key_[3] = z;
+            row_valid_ = true;
         }
-        return f;
+        return row_;
     }
-    void flush()
+    void set(bool curr, unsigned n, unsigned x, unsigned y, unsigned z, float v)
     {
-        if (dirty_v_) check_engine(ems_set_volt(h_, volt_.data()));
-        if (dirty_c_) check_engine(ems_set_curr(h_, curr_.data()));
-        dirty_v_ = dirty_c_ = false;
+        const unsigned p[3] = {x, y, z};
+        check_engine(ems_write_field(h_, curr ? 1 : 0, (int)n, p, p, &v));
+        if (row_valid_ && row_key_[0] == (unsigned)curr && row_key_[1] == n && row_key_[2] == y && row_key_[3] == z)
+            row_[x] = v;
     }
 
     const Operator_HIP* op_;
     int device_;
     ems_engine* h_ = nullptr;
     unsigned N_[3] = {0, 0, 0};
-    std::vector<float> volt_, curr_;
-    bool have_v_ = false, have_c_ = false, dirty_v_ = false, dirty_c_ = false;
+    std::vector<float> row_;
+    unsigned row_key_[4] = {0, 0, 0, 0};
+    bool row_valid_ = false;
 };
 
 inline Engine_HIP* Operator_HIP::CreateEngine(int device) const { return Engine_HIP::New(this, device); }
@@ -418,6 +421,55 @@ public:
         return add(h_field ? EMS_PROBE_HFIELD : EMS_PROBE_EFIELD, comp, P, std::vector<int>(3, 1), div, interval,
                    capacity);
     }
+    // ProcessCurrent::CalcIntegral (Common/processcurrent.cpp:96-171): the
+    // currents around the rectangle start..stop (snapped mesh indices, equal
+    // along normDir), in the reference's order and orientation; returns the id
+    int AddCurrentProbe(const unsigned start[3], const unsigned stop[3], int normDir, unsigned interval,
+                        unsigned capacity)
+    {
+        if (normDir < 0 || normDir > 2 || start[normDir] != stop[normDir])
+            throw Error("current probe: start and stop must lie in one plane normal to normDir");
+        std::vector<unsigned> comp, pos[3];
+        std::vector<int> sign;
+        // the edges lo+1 .. hi along `axis` from the fixed point f, component c
+        auto seg = [&](unsigned c, unsigned fx, unsigned fy, unsigned fz, int axis, unsigned lo, unsigned hi, int sg) {
+            for (unsigned i = lo + 1; i <= hi; ++i) {
+                unsigned p[3] = {fx, fy, fz};
+                p[axis] = i;
+                comp.push_back(c);
+                for (int a = 0; a < 3; ++a) pos[a].push_back(p[a]);
+                sign.push_back(sg);
+            }
+        };
+        const unsigned* st = start;
+        const unsigned* sp = stop;
+        if (normDir == 0) {
+            seg(1, sp[0], 0, st[2], 1, st[1], sp[1], 1);
+            seg(2, sp[0], sp[1], 0, 2, st[2], sp[2], 1);
+            seg(1, st[0], 0, sp[2], 1, st[1], sp[1], -1);
+            seg(2, st[0], st[1], 0, 2, st[2], sp[2], -1);
+        } else if (normDir == 1) {
+            seg(2, st[0], st[1], 0, 2, st[2], sp[2], 1);
+            seg(0, 0, sp[1], sp[2], 0, st[0], sp[0], 1);
+            seg(2, sp[0], sp[1], 0, 2, st[2], sp[2], -1);
+            seg(0, 0, st[1], st[2], 0, st[0], sp[0], -1);
+        } else {
+            seg(0, 0, st[1], st[2], 0, st[0], sp[0], 1);
+            seg(1, sp[0], 0, st[2], 1, st[1], sp[1], 1);
+            seg(0, 0, sp[1], sp[2], 0, st[0], sp[0], -1);
+            seg(1, st[0], 0, sp[2], 1, st[1], sp[1], -1);
+        }
+        std::vector<unsigned> P;
+        for (int a = 0; a < 3; ++a) P.insert(P.end(), pos[a].begin(), pos[a].end());
+        return add(EMS_PROBE_CURRENT, comp, P, sign, std::vector<double>(), interval, capacity);
+    }
+
+    // Engine_Interface_Base::SetInterpolationType: 0 none, 1 node, 2 cell
+    void SetInterpolationType(int type) { interp_ = type; }
+    // Engine_Interface_FDTD::GetEField / GetHField at a mesh node (V/m, A/m)
+    void GetEField(const unsigned pos[3], double out[3]) { field_at(0, pos, out); }
+    void GetHField(const unsigned pos[3], double out[3]) { field_at(1, pos, out); }
+
     // samples so far: steps and values[count][3] (voltage probes use slot 0)
     std::vector<double> GetSamples(int id, std::vector<unsigned>* steps = nullptr)
     {
@@ -451,8 +503,25 @@ private:
         return id;
     }
 
+    void field_at(int field, const unsigned pos[3], double out[3])
+    {
+        std::vector<double>* el = field ? dual_ : prim_;
+        if (el[0].empty()) {
+            for (int n = 0; n < 3; ++n)
+                for (unsigned i = 0; i < op_->GetNumberOfLines(n); ++i) {
+                    unsigned p[3] = {0, 0, 0};
+                    p[n] = i;
+                    el[n].push_back(op_->GetEdgeLength(n, p, field == 1));
+                }
+        }
+        const double* e[3] = {el[0].data(), el[1].data(), el[2].data()};
+        check_engine(ems_get_field_at(eng_->handle(), field, interp_, pos, e, out));
+    }
+
     Engine_HIP* eng_;
     const Operator_HIP* op_;
+    int interp_ = 0;                       // no interpolation, the reference's default (engine_interface_base.cpp:24)
+    std::vector<double> prim_[3], dual_[3];
 };
 
 }  // namespace openems_hip

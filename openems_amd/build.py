@@ -48,7 +48,8 @@ def build(verbose=False):
         subprocess.run(cmd, check=True, stderr=None if verbose else subprocess.PIPE)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-pthread"]
+    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-pthread", "-L/opt/rocm/lib",
+                                                                          "-lrocprofiler-sdk-roctx"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -12,6 +12,7 @@
 // (no contraction: -ffp-contract=off and the pragma below), so results are
 // bit-identical to the reference's basic engine on the same operator.
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <algorithm>
 #include <cstdint>
@@ -39,6 +40,15 @@ int fail(int code, const std::string& msg)
     g_err = msg;
     return code;
 }
+
+// a roctx range over a scope (shown by rocprofv3 --marker-trace; free when
+// no profiler is attached)
+struct TxRange {
+    explicit TxRange(const char* m) { roctxRangePushA(m); }
+    ~TxRange() { roctxRangePop(); }
+    TxRange(const TxRange&) = delete;
+    TxRange& operator=(const TxRange&) = delete;
+};
 
 #define HIPCHK(expr)                                                                          \
     do {                                                                                      \
@@ -102,7 +112,18 @@ struct CoefArgs {
     const float* cv;
     const unsigned short* idx;
     const float4* tab;
+    unsigned ntab;                 // float4 entries of tab (2 per unique tuple)
 };
+
+// Kernels that stage a small compressed table in LDS first copy it there
+// cooperatively (every thread of the workgroup takes part, then a barrier)
+// and then read it instead of the global table.
+constexpr unsigned kLdsTabBytes = 32768;    // both half steps' tables of the quad sweep
+
+__device__ __forceinline__ void stage_table(float4* dst, const float4* src, unsigned n)
+{
+    for (unsigned i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
 
 // One UPML slab (Operator_Ext_UPML range).  The positions of all slabs are
 // numbered consecutively (slab after slab in creation order, x fastest inside
@@ -171,6 +192,43 @@ __device__ __forceinline__ bool pml_at(const PmlArgs& P, int x, int y, int z, un
         }
     }
     li = off + (unsigned)(x - bx) + nx * ((unsigned)(y - by) + ny * (unsigned)(z - bz));
+    return sel >= 0;
+}
+
+// slabs whose (x, y) range holds column (x, y), one bit per slab
+__device__ __forceinline__ unsigned pml_colmask(const PmlArgs& P, int x, int y)
+{
+    unsigned m = 0;
+#pragma unroll
+    for (int s = 0; s < (int)kMaxSlabs; ++s) {
+        const SlabTab& S = P.tab[s];
+        if (s < P.n && x >= S.x0 && x <= S.x1 && y >= S.y0 && y <= S.y1) m |= 1u << s;
+    }
+    return m;
+}
+
+// UPML position of (x, y, z) for a column whose slab mask is m; 0 (a valid
+// position) when the cell lies in no slab
+__device__ __forceinline__ bool pml_at_col(const PmlArgs& P, unsigned m, int x, int y, int z, unsigned& li)
+{
+    int sel = -1;
+#pragma unroll
+    for (int s = 0; s < (int)kMaxSlabs; ++s) {
+        const SlabTab& S = P.tab[s];
+        if ((m >> s & 1u) && z >= S.z0 && z <= S.z1) sel = s;
+    }
+    int bx = 0, by = 0, bz = 0;
+    unsigned nx = 0, ny = 0, off = 0;
+#pragma unroll
+    for (int s = 0; s < (int)kMaxSlabs; ++s) {
+        if (sel == s) {
+            const SlabTab& S = P.tab[s];
+            bx = S.xo; by = S.y0; bz = S.z0;
+            nx = S.nx; ny = S.ny; off = (unsigned)S.off;
+        }
+    }
+    const unsigned l = off + (unsigned)(x - bx) + nx * ((unsigned)(y - by) + ny * (unsigned)(z - bz));
+    li = sel >= 0 ? l : 0u;
     return sel >= 0;
 }
 
@@ -327,6 +385,8 @@ __device__ __forceinline__ void volt_tile(const GridDesc& g, unsigned t, float* 
     // z-1 values carried in registers along the sweep (self at z = 0)
     const unsigned long long pz = z0 ? p - g.plane : p;
     float Ix_zm = Ix[pz], Iy_zm = Iy[pz];
+    // UPML slabs that hold this column (the per-plane test is then along z only)
+    const unsigned pcm = PM != kNoPml ? pml_colmask(pml, (int)x, (int)y) : 0u;
 
     for (unsigned z = z0; z < z1; ++z, p += g.plane) {
         const float ix = Ix[p], iy = Iy[p], iz = Iz[p];
@@ -342,8 +402,8 @@ __device__ __forceinline__ void volt_tile(const GridDesc& g, unsigned t, float* 
 #pragma unroll
         for (int n = 0; n < 3; ++n) x0[n] = ld_own<NT>(&V[n * cs + p]);
         load_coef<CC>(k, cs, p, CC ? k.idx[p] : 0u, a, b);
-        unsigned long long li = 0;
-        const bool in_pml = PM != kNoPml && pml_at(pml, (int)x, (int)y, (int)z, li);
+        unsigned li = 0;
+        const bool in_pml = PM != kNoPml && pcm && pml_at_col(pml, pcm, (int)x, (int)y, (int)z, li);
         cell_math<PM>(x0, a, b, curl, in_pml, pml, li, true, out);
 #pragma unroll
         for (int n = 0; n < 3; ++n) st_own<NT>(&V[n * cs + p], out[n]);
@@ -391,6 +451,8 @@ __device__ __forceinline__ void curr_tile(const GridDesc& g, unsigned t, float* 
     const float* __restrict__ Vz = V + 2 * cs;
     // values at z carried from the previous iteration's z+1 loads
     float vx = Vx[p], vy = Vy[p];
+    // UPML slabs that hold this column (the per-plane test is then along z only)
+    const unsigned pcm = PM != kNoPml ? pml_colmask(pml, (int)x, (int)y) : 0u;
 
     for (unsigned z = z0; z < z1; ++z, p += g.plane) {
         const bool main = col_upd && z + 1 < g.Nz;
@@ -409,8 +471,8 @@ __device__ __forceinline__ void curr_tile(const GridDesc& g, unsigned t, float* 
 #pragma unroll
         for (int n = 0; n < 3; ++n) x0[n] = ld_own<NT>(&I[n * cs + p]);
         load_coef<CC>(k, cs, p, CC ? k.idx[p] : 0u, a, b);
-        unsigned long long li = 0;
-        const bool in_pml = PM != kNoPml && pml_at(pml, (int)x, (int)y, (int)z, li);
+        unsigned li = 0;
+        const bool in_pml = PM != kNoPml && pcm && pml_at_col(pml, pcm, (int)x, (int)y, (int)z, li);
         cell_math<PM>(x0, a, b, curl, in_pml, pml, li, main, out, main);
         if (main || in_pml) {
 #pragma unroll
@@ -590,10 +652,17 @@ __device__ __forceinline__ void quad_coef(const CoefArgs& k, unsigned long long 
     }
 }
 
-template <int PM, bool CC, bool NT>
+template <int PM, bool CC, bool NT, bool LT>
 __global__ __launch_bounds__(kBlock) void k_volt4(const GridDesc g, float* V, const float* __restrict__ I,
-                                                  const CoefArgs k, const PmlArgs pml)
+                                                  const CoefArgs kg, const PmlArgs pml)
 {
+    extern __shared__ float4 s_tab[];
+    CoefArgs k = kg;
+    if (LT) {
+        stage_table(s_tab, kg.tab, kg.ntab);
+        __syncthreads();
+        k.tab = s_tab;
+    }
     const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
     const unsigned qt = t % g.qtiles, zc = t / g.qtiles;
     unsigned x0, y;
@@ -656,10 +725,17 @@ __global__ __launch_bounds__(kBlock) void k_volt4(const GridDesc g, float* V, co
     }
 }
 
-template <int PM, bool CC, bool NT>
+template <int PM, bool CC, bool NT, bool LT>
 __global__ __launch_bounds__(kBlock) void k_curr4(const GridDesc g, float* I, const float* __restrict__ V,
-                                                  const CoefArgs k, const PmlArgs pml, unsigned* __restrict__ numts)
+                                                  const CoefArgs kg, const PmlArgs pml, unsigned* __restrict__ numts)
 {
+    extern __shared__ float4 s_tab[];
+    CoefArgs k = kg;
+    if (LT) {
+        stage_table(s_tab, kg.tab, kg.ntab);
+        __syncthreads();
+        k.tab = s_tab;
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(numts, 1u);
     const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
     const unsigned qt = t % g.qtiles, zc = t / g.qtiles;
@@ -1023,43 +1099,6 @@ struct Step2Args {
     const unsigned* numts;         // [0] timestep counter (advanced by k_tick after the step)
 };
 
-// slabs whose (x, y) range holds column (x, y), one bit per slab
-__device__ __forceinline__ unsigned pml_colmask(const PmlArgs& P, int x, int y)
-{
-    unsigned m = 0;
-#pragma unroll
-    for (int s = 0; s < (int)kMaxSlabs; ++s) {
-        const SlabTab& S = P.tab[s];
-        if (s < P.n && x >= S.x0 && x <= S.x1 && y >= S.y0 && y <= S.y1) m |= 1u << s;
-    }
-    return m;
-}
-
-// UPML position of (x, y, z) for a column whose slab mask is m; 0 (a valid
-// position) when the cell lies in no slab
-__device__ __forceinline__ bool pml_at_col(const PmlArgs& P, unsigned m, int x, int y, int z, unsigned& li)
-{
-    int sel = -1;
-#pragma unroll
-    for (int s = 0; s < (int)kMaxSlabs; ++s) {
-        const SlabTab& S = P.tab[s];
-        if ((m >> s & 1u) && z >= S.z0 && z <= S.z1) sel = s;
-    }
-    int bx = 0, by = 0, bz = 0;
-    unsigned nx = 0, ny = 0, off = 0;
-#pragma unroll
-    for (int s = 0; s < (int)kMaxSlabs; ++s) {
-        if (sel == s) {
-            const SlabTab& S = P.tab[s];
-            bx = S.xo; by = S.y0; bz = S.z0;
-            nx = S.nx; ny = S.ny; off = (unsigned)S.off;
-        }
-    }
-    const unsigned l = off + (unsigned)(x - bx) + nx * ((unsigned)(y - by) + ny * (unsigned)(z - bz));
-    li = sel >= 0 ? l : 0u;
-    return sel >= 0;
-}
-
 // main coefficients (c, v) of component n at cell p: plain arrays or table
 // entry u, whose float2 n is {c_n, v_n}
 template <bool CC>
@@ -1416,11 +1455,20 @@ __device__ __forceinline__ void quad_sources(const ExcArgs& ex, unsigned b, unsi
     }
 }
 
-template <bool CC, bool NT>
-__global__ __launch_bounds__(kBlock) void k_step4(const Step4Args a, const float* __restrict__ V0,
+template <bool CC, bool NT, bool LT>
+__global__ __launch_bounds__(kBlock) void k_step4(const Step4Args ag, const float* __restrict__ V0,
                                                   const float* __restrict__ I0, float* __restrict__ V1,
                                                   float* __restrict__ I1)
 {
+    extern __shared__ float4 s_tab[];
+    Step4Args a = ag;
+    if (LT) {
+        stage_table(s_tab, ag.kV.tab, ag.kV.ntab);
+        stage_table(s_tab + ag.kV.ntab, ag.kI.tab, ag.kI.ntab);
+        __syncthreads();
+        a.kV.tab = s_tab;
+        a.kI.tab = s_tab + ag.kV.ntab;
+    }
     const GridDesc& g = a.g;
     const unsigned long long cs = g.cs;
     const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
@@ -1447,59 +1495,38 @@ __global__ __launch_bounds__(kBlock) void k_step4(const Step4Args a, const float
     const unsigned eib = a.eI.beg[q], eie = a.eI.beg[q + 4];
     const unsigned eub = a.eV.beg[q + du], eue = yupd ? a.eV.beg[q + du + 4] : eub;
     const unsigned erb = a.eV.beg[q + dr], ere = xr ? a.eV.beg[q + dr + 1] : erb;
-    // I^n at z-1 (the cell itself at z = 0)
-    const unsigned long long pz = z0 ? (unsigned long long)(z0 - 1) * plane + q : (unsigned long long)q;
-    float4 ix_m = *reinterpret_cast<const float4*>(I0 + pz);
-    float4 iy_m = *reinterpret_cast<const float4*>(I0 + cs + pz);
-    float4 iz_m = *reinterpret_cast<const float4*>(I0 + 2 * cs + pz);
-    float4 iyu_m = *reinterpret_cast<const float4*>(I0 + cs + pz + du);
-    float ixr_m = I0[pz + dr];
-    // plane z-1 carried for the current update
+    // V^{n+1} of plane z-1 carried for the current update: the own quad, the
+    // z and x components of row y+1, the z and y components of (x0+4, y)
     float4 vm[3] = {}, vu2m = {}, vu0m = {};
     float vr2m = 0.0f, vr1m = 0.0f;
-    unsigned um[4] = {0u, 0u, 0u, 0u};
     for (unsigned z = z0; z <= zlast; ++z) {
         const unsigned long long p = (unsigned long long)z * plane + q;
-        // ---- loads: state n around the quad
+        // I^n at z-1 (the cell itself at z = 0); read again where needed
+        const unsigned long long pz = z ? p - plane : p;
+        // ---- own V^{n+1}(x0..x0+3, y, z)
         const float4 ix = *reinterpret_cast<const float4*>(I0 + p);
         const float4 iy = *reinterpret_cast<const float4*>(I0 + cs + p);
         const float4 iz = *reinterpret_cast<const float4*>(I0 + 2 * cs + p);
-        const float4 iz_ym = *reinterpret_cast<const float4*>(I0 + 2 * cs + p - ym);
-        const float4 ix_ym = *reinterpret_cast<const float4*>(I0 + p - ym);
-        float4 iz_xm, iy_xm;
-        if (x0) {
-            iz_xm = ld4u(I0 + 2 * cs + p - 1);
-            iy_xm = ld4u(I0 + cs + p - 1);
-        } else {
-            iz_xm = make_float4(iz.x, iz.x, iz.y, iz.z);
-            iy_xm = make_float4(iy.x, iy.x, iy.y, iy.z);
-        }
-        const unsigned long long pu = p + du;
-        const float4 ixu = *reinterpret_cast<const float4*>(I0 + pu);
-        const float4 iyu = *reinterpret_cast<const float4*>(I0 + cs + pu);
-        const float4 izu = *reinterpret_cast<const float4*>(I0 + 2 * cs + pu);
-        float4 iyu_xm;
-        if (x0) iyu_xm = ld4u(I0 + cs + pu - 1);
-        else iyu_xm = make_float4(iyu.x, iyu.x, iyu.y, iyu.z);
-        const unsigned long long pr = p + dr;
-        const float ixr = I0[pr], iyr = I0[cs + pr], izr = I0[2 * cs + pr], ixr_ym = I0[pr - ym];
-        float4 v0[3];
-        v0[0] = ld4<NT>(V0 + p);
-        v0[1] = ld4<NT>(V0 + cs + p);
-        v0[2] = ld4<NT>(V0 + 2 * cs + p);
-        const float4 u0z = *reinterpret_cast<const float4*>(V0 + 2 * cs + pu);
-        const float4 u0x = *reinterpret_cast<const float4*>(V0 + pu);
-        const float r0z = V0[2 * cs + pr], r0y = V0[cs + pr];
-        unsigned u[4] = {0u, 0u, 0u, 0u}, uu[4] = {0u, 0u, 0u, 0u};
-        unsigned ur = 0u;
-        if (CC) {
-            quad_idx(a.kV.idx, p, u);
-            quad_idx(a.kV.idx, pu, uu);
-            ur = a.kV.idx[pr];
-        }
-        // ---- own V^{n+1}(x0..x0+3, y, z)
         float4 v[3];
+        unsigned u[4] = {0u, 0u, 0u, 0u};
         {
+            const float4 iz_ym = *reinterpret_cast<const float4*>(I0 + 2 * cs + p - ym);
+            const float4 ix_ym = *reinterpret_cast<const float4*>(I0 + p - ym);
+            const float4 ix_m = *reinterpret_cast<const float4*>(I0 + pz);
+            const float4 iy_m = *reinterpret_cast<const float4*>(I0 + cs + pz);
+            float4 iz_xm, iy_xm;
+            if (x0) {
+                iz_xm = ld4u(I0 + 2 * cs + p - 1);
+                iy_xm = ld4u(I0 + cs + p - 1);
+            } else {
+                iz_xm = make_float4(iz.x, iz.x, iz.y, iz.z);
+                iy_xm = make_float4(iy.x, iy.x, iy.y, iy.z);
+            }
+            float4 v0[3];
+            v0[0] = ld4<NT>(V0 + p);
+            v0[1] = ld4<NT>(V0 + cs + p);
+            v0[2] = ld4<NT>(V0 + 2 * cs + p);
+            if (CC) quad_idx(a.kV.idx, p, u);
             float4 c, w;
             quad_coef_n<CC>(a.kV, cs, p, u, 0, c, w);
             v[0] = quad_math(v0[0], c, w, quad_curl(iz, iz_ym, iy, iy_m));
@@ -1507,46 +1534,28 @@ __global__ __launch_bounds__(kBlock) void k_step4(const Step4Args a, const float
             v[1] = quad_math(v0[1], c, w, quad_curl(ix, ix_m, iz, iz_xm));
             quad_coef_n<CC>(a.kV, cs, p, u, 2, c, w);
             v[2] = quad_math(v0[2], c, w, quad_curl(iy, iy_xm, ix, ix_ym));
+            if (evb != eve) quad_sources(a.eV, evb, eve, x0, z, 7u, ts, v);
         }
-        if (evb != eve) quad_sources(a.eV, evb, eve, x0, z, 7u, ts, v);
-        // ---- neighbours: row y+1 (z and x components), cell (x0+4, y) (z and y)
-        float4 nu[3];
-        {
-            float4 c, w;
-            quad_coef_n<CC>(a.kV, cs, pu, uu, 2, c, w);
-            nu[2] = quad_math(u0z, c, w, quad_curl(iyu, iyu_xm, ixu, ix));
-            quad_coef_n<CC>(a.kV, cs, pu, uu, 0, c, w);
-            nu[0] = quad_math(u0x, c, w, quad_curl(izu, iz, iyu, iyu_m));
-            nu[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (z < z1) {
+#pragma unroll
+            for (int n = 0; n < 3; ++n) st4<NT>(V1 + n * cs + p, v[n]);
         }
-        if (eub != eue) quad_sources(a.eV, eub, eue, x0, z, 5u, ts, nu);
-        float nrz, nry;
-        {
-            const float2 c2 = coef_n<CC>(a.kV, cs, pr, ur, 2), c1 = coef_n<CC>(a.kV, cs, pr, ur, 1);
-            const float c2r = iyr - iy.w - ixr + ixr_ym;
-            const float c1r = ixr - ixr_m - izr + iz.w;
-            nrz = r0z * c2.x;
-            nrz = nrz + c2.y * c2r;
-            nry = r0y * c1.x;
-            nry = nry + c1.y * c1r;
-            if (erb != ere) {
-                nrz = col_sources(a.eV, erb, ere, z, 2, ts, nrz);
-                nry = col_sources(a.eV, erb, ere, z, 1, ts, nry);
-            }
-        }
-        // ---- I^{n+1}(x0..x0+3, y, z-1)
+        // ---- I^{n+1}(x0..x0+3, y, z-1) from the carried plane and v
         if (z > z0 && z < a.zb && imask) {
-            const unsigned long long pm = p - plane;
+            const unsigned long long pm = pz;
+            unsigned um[4] = {0u, 0u, 0u, 0u};
+            if (CC) quad_idx(a.kI.idx, pm, um);
             float4 io[3];
             const float4 vzr = make_float4(vm[2].y, vm[2].z, vm[2].w, vr2m);
             const float4 vyr = make_float4(vm[1].y, vm[1].z, vm[1].w, vr1m);
             float4 c, w;
             quad_coef_n<CC>(a.kI, cs, pm, um, 0, c, w);
-            io[0] = quad_math(ix_m, c, w, quad_curl(vm[2], vu2m, vm[1], v[1]));
+            io[0] = quad_math(*reinterpret_cast<const float4*>(I0 + pm), c, w, quad_curl(vm[2], vu2m, vm[1], v[1]));
             quad_coef_n<CC>(a.kI, cs, pm, um, 1, c, w);
-            io[1] = quad_math(iy_m, c, w, quad_curl(vm[0], v[0], vm[2], vzr));
+            io[1] = quad_math(*reinterpret_cast<const float4*>(I0 + cs + pm), c, w, quad_curl(vm[0], v[0], vm[2], vzr));
             quad_coef_n<CC>(a.kI, cs, pm, um, 2, c, w);
-            io[2] = quad_math(iz_m, c, w, quad_curl(vm[1], vyr, vm[0], vu0m));
+            io[2] = quad_math(*reinterpret_cast<const float4*>(I0 + 2 * cs + pm), c, w,
+                              quad_curl(vm[1], vyr, vm[0], vu0m));
             if (eib != eie) quad_sources(a.eI, eib, eie, x0, z - 1, 7u, ts, io);
             if (imask == 15u) {
 #pragma unroll
@@ -1559,25 +1568,53 @@ __global__ __launch_bounds__(kBlock) void k_step4(const Step4Args a, const float
                         if (imask >> j & 1u) I1[n * cs + pm + j] = f4g(io[n], j);
             }
         }
-        if (z < z1) {
-#pragma unroll
-            for (int n = 0; n < 3; ++n) st4<NT>(V1 + n * cs + p, v[n]);
-        }
-        // ---- carry plane z
         vm[0] = v[0];
         vm[1] = v[1];
         vm[2] = v[2];
-        vu2m = nu[2];
-        vu0m = nu[0];
-        vr2m = nrz;
-        vr1m = nry;
-        ix_m = ix;
-        iy_m = iy;
-        iz_m = iz;
-        iyu_m = iyu;
-        ixr_m = ixr;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) um[j] = u[j];
+        // ---- neighbours at z: row y+1 (z and x components), cell (x0+4, y) (z
+        // and y), for the current update of plane z in the next iteration
+        if (z < zlast) {
+            const unsigned long long pu = p + du;
+            const float4 ixu = *reinterpret_cast<const float4*>(I0 + pu);
+            const float4 iyu = *reinterpret_cast<const float4*>(I0 + cs + pu);
+            const float4 izu = *reinterpret_cast<const float4*>(I0 + 2 * cs + pu);
+            const float4 iyu_m = *reinterpret_cast<const float4*>(I0 + cs + pz + du);
+            float4 iyu_xm;
+            if (x0) iyu_xm = ld4u(I0 + cs + pu - 1);
+            else iyu_xm = make_float4(iyu.x, iyu.x, iyu.y, iyu.z);
+            const float4 u0z = *reinterpret_cast<const float4*>(V0 + 2 * cs + pu);
+            const float4 u0x = *reinterpret_cast<const float4*>(V0 + pu);
+            unsigned uu[4] = {0u, 0u, 0u, 0u};
+            if (CC) quad_idx(a.kV.idx, pu, uu);
+            float4 nu[3];
+            float4 c, w;
+            quad_coef_n<CC>(a.kV, cs, pu, uu, 2, c, w);
+            nu[2] = quad_math(u0z, c, w, quad_curl(iyu, iyu_xm, ixu, ix));
+            quad_coef_n<CC>(a.kV, cs, pu, uu, 0, c, w);
+            nu[0] = quad_math(u0x, c, w, quad_curl(izu, iz, iyu, iyu_m));
+            nu[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (eub != eue) quad_sources(a.eV, eub, eue, x0, z, 5u, ts, nu);
+            vu2m = nu[2];
+            vu0m = nu[0];
+            const unsigned long long pr = p + dr;
+            const float ixr = I0[pr], iyr = I0[cs + pr], izr = I0[2 * cs + pr], ixr_ym = I0[pr - ym];
+            const float ixr_m = I0[pz + dr];
+            const float r0z = V0[2 * cs + pr], r0y = V0[cs + pr];
+            const unsigned ur = CC ? a.kV.idx[pr] : 0u;
+            const float2 c2 = coef_n<CC>(a.kV, cs, pr, ur, 2), c1 = coef_n<CC>(a.kV, cs, pr, ur, 1);
+            const float c2r = iyr - iy.w - ixr + ixr_ym;
+            const float c1r = ixr - ixr_m - izr + iz.w;
+            float nrz = r0z * c2.x;
+            nrz = nrz + c2.y * c2r;
+            float nry = r0y * c1.x;
+            nry = nry + c1.y * c1r;
+            if (erb != ere) {
+                nrz = col_sources(a.eV, erb, ere, z, 2, ts, nrz);
+                nry = col_sources(a.eV, erb, ere, z, 1, ts, nry);
+            }
+            vr2m = nrz;
+            vr1m = nry;
+        }
     }
 }
 
@@ -2510,6 +2547,7 @@ struct ems_engine {
     int fused = 0;                        // active form: 0 two half steps, 1 k_step, 2 k_step2
     bool nt_fields = false;               // non-temporal own-field accesses in the half steps
     int quad_req = -1;                    // -1 auto, 0 one cell per lane, 1 quads (four x cells per lane)
+    bool lds_tab = false;                 // quad kernels read the compressed tables from LDS
     bool quad = false;                    // the half steps run k_volt4 / k_curr4
     GridDesc gq{};                        // tiles of the quad half steps
     unsigned ntx = 0, nty = 0, fzdepth = 0, fnzc = 0;
@@ -2779,8 +2817,8 @@ int ems_create_layout(ems_engine** out, const unsigned numLines[3], const float*
             (rc = upload_nijk(h, h->iv, iv)))
             return bail(rc);
     }
-    h->kV = CoefArgs{h->vv, h->vi, h->cidx, (const float4*)h->ctab[0]};
-    h->kI = CoefArgs{h->ii, h->iv, h->cidx, (const float4*)h->ctab[1]};
+    h->kV = CoefArgs{h->vv, h->vi, h->cidx, (const float4*)h->ctab[0], 2u * h->n_unique};
+    h->kI = CoefArgs{h->ii, h->iv, h->cidx, (const float4*)h->ctab[1], 2u * h->n_unique};
     hipEventCreate(&h->ev0);
     hipEventCreate(&h->ev1);
     *out = h;
@@ -3433,6 +3471,9 @@ int ems_init(ems_engine* h)
     if (rc) return rc;
     if ((rc = setup_fused(h))) return rc;
     if ((rc = setup_quad(h))) return rc;
+    // small compressed tables (both half steps' within 32 KiB) go to LDS in the quad kernels
+    h->lds_tab = h->cmp && (size_t)4 * h->n_unique * sizeof(float4) <= kLdsTabBytes;
+    if (const char* lt = std::getenv("EMS_LDS_TAB")) h->lds_tab = h->lds_tab && std::atoi(lt) != 0;
     h->inited = true;
     return 0;
 }
@@ -3681,25 +3722,33 @@ void launch_update(ems_engine* h, bool curr, unsigned nblk)
     else curr ? launch_curr_t<kNoPml, CC>(h, nblk) : launch_volt_t<kNoPml, CC>(h, nblk);
 }
 
-template <int PM, bool CC>
-void launch_quad_t(ems_engine* h, bool curr)
+template <int PM, bool CC, bool LT>
+void launch_quad_lt(ems_engine* h, bool curr)
 {
     const unsigned nblk = h->gq.qtiles * h->gq.nzc;
+    const size_t lds = LT ? (size_t)h->kV.ntab * sizeof(float4) : 0;
     if (curr) {
         if (h->nt_fields)
-            hipLaunchKernelGGL((k_curr4<PM, CC, true>), dim3(nblk), dim3(kBlock), 0, h->stream, h->gq, h->I, h->V, h->kI,
-                               h->pmlI, h->numts_dev);
+            hipLaunchKernelGGL((k_curr4<PM, CC, true, LT>), dim3(nblk), dim3(kBlock), lds, h->stream, h->gq, h->I, h->V,
+                               h->kI, h->pmlI, h->numts_dev);
         else
-            hipLaunchKernelGGL((k_curr4<PM, CC, false>), dim3(nblk), dim3(kBlock), 0, h->stream, h->gq, h->I, h->V,
+            hipLaunchKernelGGL((k_curr4<PM, CC, false, LT>), dim3(nblk), dim3(kBlock), lds, h->stream, h->gq, h->I, h->V,
                                h->kI, h->pmlI, h->numts_dev);
     } else {
         if (h->nt_fields)
-            hipLaunchKernelGGL((k_volt4<PM, CC, true>), dim3(nblk), dim3(kBlock), 0, h->stream, h->gq, h->V, h->I, h->kV,
-                               h->pmlV);
+            hipLaunchKernelGGL((k_volt4<PM, CC, true, LT>), dim3(nblk), dim3(kBlock), lds, h->stream, h->gq, h->V, h->I,
+                               h->kV, h->pmlV);
         else
-            hipLaunchKernelGGL((k_volt4<PM, CC, false>), dim3(nblk), dim3(kBlock), 0, h->stream, h->gq, h->V, h->I,
+            hipLaunchKernelGGL((k_volt4<PM, CC, false, LT>), dim3(nblk), dim3(kBlock), lds, h->stream, h->gq, h->V, h->I,
                                h->kV, h->pmlV);
     }
+}
+
+template <int PM, bool CC>
+void launch_quad_t(ems_engine* h, bool curr)
+{
+    if (CC && h->lds_tab) launch_quad_lt<PM, CC, true>(h, curr);
+    else launch_quad_lt<PM, CC, false>(h, curr);
 }
 
 template <bool CC>
@@ -3780,15 +3829,23 @@ void column_step(ems_engine* h)
     }
 }
 
+template <bool CC, bool LT>
+void launch_step4_lt(ems_engine* h, const Step4Args& a, unsigned nblk)
+{
+    const size_t lds = LT ? (size_t)(h->kV.ntab + h->kI.ntab) * sizeof(float4) : 0;
+    if (h->nt_fields)
+        hipLaunchKernelGGL((k_step4<CC, true, LT>), dim3(nblk), dim3(kBlock), lds, h->stream, a, h->V, h->I, h->V_alt,
+                           h->I_alt);
+    else
+        hipLaunchKernelGGL((k_step4<CC, false, LT>), dim3(nblk), dim3(kBlock), lds, h->stream, a, h->V, h->I, h->V_alt,
+                           h->I_alt);
+}
+
 template <bool CC>
 void launch_step4_t(ems_engine* h, const Step4Args& a, unsigned nblk)
 {
-    if (h->nt_fields)
-        hipLaunchKernelGGL((k_step4<CC, true>), dim3(nblk), dim3(kBlock), 0, h->stream, a, h->V, h->I, h->V_alt,
-                           h->I_alt);
-    else
-        hipLaunchKernelGGL((k_step4<CC, false>), dim3(nblk), dim3(kBlock), 0, h->stream, a, h->V, h->I, h->V_alt,
-                           h->I_alt);
+    if (CC && h->lds_tab) launch_step4_lt<CC, true>(h, a, nblk);
+    else launch_step4_lt<CC, false>(h, a, nblk);
 }
 
 // the quad column sweep over the whole mesh (no UPML)
@@ -4075,6 +4132,7 @@ int ems_get_graph(const ems_engine* h, unsigned* chunk_steps, unsigned* captures
 int ems_iterate(ems_engine* h, unsigned nsteps)
 {
     if (!h) return fail(EMS_E_ARG, "null handle");
+    TxRange tx("ems_iterate");
     int rc;
     if (!h->inited && (rc = ems_init(h))) return rc;
     HIPCHK(hipSetDevice(h->device));
@@ -4106,6 +4164,7 @@ int ems_iterate(ems_engine* h, unsigned nsteps)
             h->graph_len = steps;
         }
         while (nsteps - s >= steps) {
+            TxRange txc("ems_iterate chunk");
             if ((rc = graph_chunk(h, steps))) return rc;
             s += steps;
         }
@@ -4773,6 +4832,7 @@ int ems_batch_get_grouping(const ems_batch* b, unsigned long long* group_bytes, 
 int ems_batch_iterate(ems_batch* b, unsigned nsteps)
 {
     if (!b) return fail(EMS_E_ARG, "null batch");
+    TxRange tx("ems_batch_iterate");
     HIPCHK(hipSetDevice(b->device));
     for (auto* e : b->eng) HIPCHK(hipStreamSynchronize(e->stream));
     if (b->eng[0]->numTS == 0 && nsteps > 0)
@@ -5233,6 +5293,7 @@ int ems_adj_create(ems_adjoint** out, ems_engine* h, unsigned nprobe, const unsi
 
 int ems_adj_forward(ems_adjoint* a, unsigned nsteps, double* J)
 {
+    TxRange tx("ems_adj_forward");
     if (!a || nsteps == 0) return fail(EMS_E_ARG, "null adjoint or no steps");
     ems_engine* h = a->h;
     HIPCHK(hipSetDevice(h->device));
@@ -5270,6 +5331,7 @@ int ems_adj_forward(ems_adjoint* a, unsigned nsteps, double* J)
 
 int ems_adj_backward(ems_adjoint* a, double* dJ_dvi, double* dJ_dvv)
 {
+    TxRange tx("ems_adj_backward");
     if (!a) return fail(EMS_E_ARG, "null adjoint");
     if (!a->forward_done) return fail(EMS_E_STATE, "run ems_adj_forward first");
     ems_engine* h = a->h;

@@ -690,11 +690,22 @@ class openEMS:
         sim = Simulation(self.scene(), **{k: v for k, v in kw.items() if k in ("device", "layout", "fused", "graph")})
         if setup_only:
             return sim
+        self._sim = sim
         try:
+            sim.SetAbort(getattr(self, "_abort", False))
             sim.run()
             self.results = sim.results()
             sim.write_probes(sim_path)
             sim.write_dumps(sim_path)
         finally:
+            self._sim = None
             sim.close()
         return 0
+
+    def SetAbort(self, val):
+        """openEMS.SetAbort (python/openEMS/openEMS.pyx:566-567): a run in
+        progress (from another thread) or the next run stops gracefully."""
+        self._abort = bool(val)
+        sim = getattr(self, "_sim", None)
+        if sim is not None:
+            sim.SetAbort(val)

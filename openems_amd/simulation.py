@@ -13,6 +13,8 @@ fixed step interval.
 """
 from __future__ import annotations
 
+import os
+import sys
 import time
 
 import numpy as np
@@ -46,20 +48,50 @@ class Simulation:
         for d in self.dumps:
             d.register(self.eng)
 
+    def SetAbort(self, val=True):
+        """openEMS::SetAbort (openems.h:103): a running or later run stops
+        gracefully at its next abort check."""
+        self._abort = bool(val)
+
+    def check_abort(self):
+        """openEMS::CheckAbortCond (openems.cpp:1364-1385): the abort flag, or a
+        file named ABORT in the working directory."""
+        if getattr(self, "_abort", False):
+            return True
+        if os.path.exists("ABORT"):
+            print('Simulation.check_abort: found file "ABORT", aborting simulation gracefully...', file=sys.stderr)
+            return True
+        return False
+
     def run(self, nsteps=None, chunk=0):
+        """Advance nsteps (default NrTS) in chunks, checking the end criterion
+        and the abort conditions (CheckAbortCond: SetAbort, an ABORT file,
+        SIGINT) between chunks of at most scene["abort_check"] steps (default
+        1000) as RunFDTD's loop does (openems.cpp:1427-1480).  self.aborted
+        tells whether the run stopped early on an abort."""
         n = self.nrts if nsteps is None else int(nsteps)
         end = float(self.scene.get("end_criteria", 0.0))
+        self.aborted = False
         if end > 0 and not self.ss_period:
             return self._run_energy(n, end)
         if end > 0 and self.ss_period:
             chunk = self.ss_period if chunk <= 0 else min(chunk, self.ss_period)
+        every = max(1, int(self.scene.get("abort_check", 1000)))
+        chunk = every if chunk <= 0 else min(chunk, every)
         done = 0
-        while done < n:
-            k = n - done if chunk <= 0 else min(chunk, n - done)
-            self.eng.IterateTS(k)
-            done += k
-            if end > 0 and self.ss_period and self.eng.steady_state(1)[0] <= end:
-                break
+        try:
+            while done < n:
+                if self.check_abort():
+                    self.aborted = True
+                    break
+                k = min(chunk, n - done)
+                self.eng.IterateTS(k)
+                done += k
+                if end > 0 and self.ss_period and self.eng.steady_state(1)[0] <= end:
+                    break
+        except KeyboardInterrupt:
+            print("Simulation.run: received SIGINT, aborting simulation gracefully...", file=sys.stderr)
+            self.aborted = True
         return self
 
     def _run_energy(self, n, end):
@@ -77,6 +109,9 @@ class Simulation:
         max_e, change = 0.0, 1.0
         self.stopped_at = None
         while self.eng.numTS - start < n and change > end:
+            if self.check_abort():
+                self.aborted = True
+                break
             ts = self.eng.numTS
             nxt = min(start + n, (ts // check + 1) * check)
             if ts < max_step:

@@ -12,7 +12,9 @@
 // a metal wire (curve) and a metal cylinder (radius 1.6, axis (20, 18, 3) to
 // (24, 24, 9)); a soft E_x source on a cylinder (radius 1.2, axis (5, 21, 21)
 // to (10, 21, 21)) weighted 1 + 0.1 x;
-// Gaussian pulse f0 = 0, fc = 15 GHz.
+// Gaussian pulse f0 = 0, fc = 15 GHz.  Probes: a voltage line across the
+// port, E and H points, a current loop around the port; after the last step
+// the E and H fields at three nodes with each interpolation type.
 #include <cstdio>
 #include <array>
 #include <vector>
@@ -76,6 +78,8 @@ int main()
         const int pu = iface.AddVoltageProbe(ua, ub, every, 512);
         const int pe = iface.AddFieldProbe(false, en, every, 512);
         const int ph = iface.AddFieldProbe(true, hn, every, 512);
+        const unsigned ia[3] = {13, 13, 11}, ib[3] = {14, 14, 11};
+        const int pi = iface.AddCurrentProbe(ia, ib, 2, every, 512);
         eng->IterateTS(120);
         std::printf("{");
         std::printf("\"dT\": \"%a\", ", op->GetTimestep());
@@ -94,8 +98,24 @@ int main()
         hexf("v_d", eng->GetVolt(0, 5, 5, 5));
         hexf("i_b", eng->GetCurr(1, q));
         hexf("v_e", eng->GetVolt(1, 21, 12, 13));
+        // interpolated point fields (Engine_Interface_FDTD::GetEField / GetHField)
+        std::printf("\"fields\": [");
+        const unsigned fpos[3][3] = {{20, 11, 12}, {10, 12, 14}, {27, 0, 5}};
+        bool first = true;
+        for (int it = 0; it < 3; ++it) {
+            iface.SetInterpolationType(it);
+            for (const auto& fp : fpos) {
+                double e[3], hh[3];
+                iface.GetEField(fp, e);
+                iface.GetHField(fp, hh);
+                for (double v : e) std::printf("%s\"%a\"", first ? "" : ", ", v), first = false;
+                for (double v : hh) std::printf(", \"%a\"", v);
+            }
+        }
+        std::printf("], ");
         std::vector<unsigned> ts;
         const std::vector<double> su = iface.GetSamples(pu, &ts), se = iface.GetSamples(pe), sh = iface.GetSamples(ph);
+        const std::vector<double> si = iface.GetSamples(pi);
         std::printf("\"probe_every\": %u, \"probe_count\": %u, ", every, (unsigned)ts.size());
         std::printf("\"probe_u\": [");
         for (size_t k = 0; k < ts.size(); ++k) std::printf("%s\"%a\"", k ? ", " : "", su[3 * k]);
@@ -103,6 +123,8 @@ int main()
         for (size_t k = 0; k < 3 * ts.size(); ++k) std::printf("%s\"%a\"", k ? ", " : "", se[k]);
         std::printf("], \"probe_h\": [");
         for (size_t k = 0; k < 3 * ts.size(); ++k) std::printf("%s\"%a\"", k ? ", " : "", sh[k]);
+        std::printf("], \"probe_i\": [");
+        for (size_t k = 0; k < ts.size(); ++k) std::printf("%s\"%a\"", k ? ", " : "", si[3 * k]);
         std::printf("], ");
         std::printf("\"energy_b\": \"%a\"}\n", eng->CalcFastEnergy());
         delete eng;

@@ -486,7 +486,8 @@ def test_cpp_host_classes_match_python(gpu):
     eng = EngineHIP(op)
     specs = [{"type": 0, "name": "u", "start": (14, 14, 10), "stop": (14, 14, 12)},
              {"type": 2, "name": "e", "start": (20, 11, 12), "stop": (20, 11, 12)},
-             {"type": 3, "name": "h", "start": (10.5, 12.5, 14.5), "stop": (10.5, 12.5, 14.5)}]
+             {"type": 3, "name": "h", "start": (10.5, 12.5, 14.5), "stop": (10.5, 12.5, 14.5)},
+             {"type": 1, "name": "i", "start": (13.5, 13.5, 11.5), "stop": (14.5, 14.5, 11.5), "norm_dir": 2}]
     probes = [Probe(op, sp, 200) for sp in specs]
     for p in probes:
         p.register(eng)
@@ -520,6 +521,16 @@ def test_cpp_host_classes_match_python(gpu):
     assert [fx(x) for x in got["probe_e"]] == list(tds[1][:, 1:].ravel())
     assert [fx(x) for x in got["probe_h"]] == list(tds[2][:, 1:].ravel())
     assert np.abs(tds[0][:, 1]).max() > 0 and np.abs(tds[2][:, 1:]).max() > 0
+    # the current loop (snapped as ProcessCurrent snaps it) and point fields
+    assert probes[3].start == [13, 13, 11] and probes[3].stop == [14, 14, 11]
+    assert [fx(x) for x in got["probe_i"]] == list(tds[3][:, 1])
+    assert np.abs(tds[3][:, 1]).max() > 0
+    vals = []
+    for interp in (0, 1, 2):
+        for pos in ((20, 11, 12), (10, 12, 14), (27, 0, 5)):
+            vals += list(eng.field_at(0, pos, interp)) + list(eng.field_at(1, pos, interp))
+    assert [fx(x) for x in got["fields"]] == vals
+    assert max(abs(v) for v in vals) > 0
     eng.close()
     op.close()
 
@@ -699,3 +710,25 @@ def test_full_size_upml_1000_steps_probes_and_plane(gpu):
     mt.close()
     print("oracle multithreaded engine done", flush=True)
     assert rel_l2(v, mv) <= 1e-5 and rel_l2(c, mc) <= 1e-5
+
+
+def test_run_stops_on_abort(gpu, tmp_path, monkeypatch):
+    """RunFDTD's abort conditions (openems.cpp:1364-1385) between chunks: an
+    ABORT file in the working directory, or SetAbort."""
+    monkeypatch.chdir(tmp_path)
+    scene = scenes.pec_cavity(16, 400)
+    scene["abort_check"] = 100
+    sim = Simulation(scene)
+    sim.run(200)
+    assert sim.eng.numTS == 200 and not sim.aborted
+    (tmp_path / "ABORT").write_text("")
+    sim.run(200)
+    assert sim.eng.numTS == 200 and sim.aborted
+    (tmp_path / "ABORT").unlink()
+    sim.SetAbort(True)
+    sim.run(200)
+    assert sim.eng.numTS == 200 and sim.aborted
+    sim.SetAbort(False)
+    sim.run(200)
+    assert sim.eng.numTS == 400 and not sim.aborted
+    sim.close()

@@ -295,3 +295,19 @@ def test_line_from_three_planes_on_a_lossy_line():
         beta, Z, Ub, Ib = line_from_three_planes(U(za), U(zb), U(zc), 2 * h, U(z1) / ZL, U(z2) / ZL, h)
         assert abs(beta - gamma.imag + 1j * gamma.real) / abs(gamma) < tol   # beta = -j gamma
         assert abs(Z - ZL) / abs(ZL) < tol
+
+
+def test_abort_conditions(tmp_path, monkeypatch):
+    """openEMS::CheckAbortCond restated in Simulation.check_abort: the abort
+    flag and a file named ABORT in the working directory (CPU: the check
+    alone, no engine)."""
+    from openems_amd.simulation import Simulation
+    sim = Simulation.__new__(Simulation)
+    monkeypatch.chdir(tmp_path)
+    assert not sim.check_abort()
+    sim.SetAbort(True)
+    assert sim.check_abort()
+    sim.SetAbort(False)
+    assert not sim.check_abort()
+    (tmp_path / "ABORT").write_text("")
+    assert sim.check_abort()
