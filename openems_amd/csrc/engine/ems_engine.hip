@@ -1692,6 +1692,57 @@ __global__ void k_mur_apply(float* __restrict__ V, const MurDev m, const unsigne
     V[m.offPP + f] = m.mPP[k];
 }
 
+// The current update (curr_tile's arithmetic, FDTD/engine.cpp:170-222) redone
+// on the planes lo..hi along `axis`, from the final voltages V1 and the
+// currents of state n, into I1: the quad sweep's currents next to a Mur face,
+// whose voltages the Mur hooks and the deferred sources set after the sweep.
+__global__ void k_curr_band(const GridDesc g, const float* __restrict__ V1, const float* __restrict__ I0,
+                            float* __restrict__ I1, const CoefArgs k, int axis, unsigned lo, unsigned hi)
+{
+    const unsigned n[3] = {axis == 0 ? hi - lo + 1 : g.Nx, axis == 1 ? hi - lo + 1 : g.Ny,
+                           axis == 2 ? hi - lo + 1 : g.Nz};
+    const unsigned long long cnt = (unsigned long long)n[0] * n[1] * n[2];
+    const unsigned long long t = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    unsigned c[3];
+    c[0] = (unsigned)(t % n[0]);
+    c[1] = (unsigned)((t / n[0]) % n[1]);
+    c[2] = (unsigned)(t / ((unsigned long long)n[0] * n[1]));
+    c[axis] += lo;
+    const unsigned x = c[0], y = c[1], z = c[2];
+    if (x + 1 >= g.Nx || y + 1 >= g.Ny || z + 1 >= g.Nz) return;
+    const unsigned long long cs = g.cs;
+    const unsigned long long p = x + (unsigned long long)g.Nx * y + (unsigned long long)g.plane * z;
+    const unsigned long long dx = 1, dy = g.Nx, dz = g.plane;
+    const float* Vx = V1;
+    const float* Vy = V1 + cs;
+    const float* Vz = V1 + 2 * cs;
+    float curl[3];
+    curl[0] = Vz[p] - Vz[p + dy] - Vy[p] + Vy[p + dz];
+    curl[1] = Vx[p] - Vx[p + dz] - Vz[p] + Vz[p + dx];
+    curl[2] = Vy[p] - Vy[p + dx] - Vx[p] + Vx[p + dy];
+    float a[3], b[3];
+    if (k.idx) {
+        const unsigned u = k.idx[p];
+        const float4 t0 = k.tab[2 * u], t1 = k.tab[2 * u + 1];
+        a[0] = t0.x; b[0] = t0.y;
+        a[1] = t0.z; b[1] = t0.w;
+        a[2] = t1.x; b[2] = t1.y;
+    } else {
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            a[m] = k.cc[m * cs + p];
+            b[m] = k.cv[m * cs + p];
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        float f = I0[m * cs + p] * a[m];
+        f = f + b[m] * curl[m];
+        I1[m * cs + p] = f;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // lumped RLC (Engine_Ext_LumpedRLC, engine_ext_lumpedRLC.cpp:83-147).  The
 // reference rotates its three history buffers once per step (Vd in the pre
@@ -2556,6 +2607,22 @@ struct ems_engine {
     unsigned n_unique_pml = 0;
     // Mur faces in creation order
     std::vector<MurDev> mur;
+    // Mur face geometry (normal axis, face line, neighbour line), creation order
+    struct MurGeo {
+        int ny;
+        unsigned line, shift;
+    };
+    std::vector<MurGeo> mur_geo;
+    // quad sweep with Mur faces: sources on face / neighbour lines added after
+    // the Mur hooks (exc_def[0]), the current update redone on the planes next
+    // to each face (bands: axis, first and last plane) with the current sources
+    // of those planes (exc_def[1])
+    Exc exc_def[2];
+    struct Band {
+        int axis;
+        unsigned lo, hi;
+    };
+    std::vector<Band> bands;
     // lumped RLC and dispersive lists
     std::vector<RlcDev> rlc;
     std::vector<LorDev> lor;
@@ -2891,6 +2958,50 @@ int ems_get_layout(const ems_engine* h, int* layout, unsigned* n_unique, int* pm
     return 0;
 }
 
+}  // extern "C"
+
+// entries (list order) grouped per (component, cell) in order of first
+// appearance, as k_excite applies them, uploaded into E (device lists only)
+static int upload_exc_groups(ems_engine* h, const std::vector<ExcEnt>& ent, ems_engine::Exc& E)
+{
+    const unsigned K = (unsigned)ent.size();
+    E.K = K;
+    E.G = 0;
+    if (!K) return 0;
+    std::unordered_map<unsigned long long, unsigned> gid;
+    std::vector<unsigned long long> glin;
+    std::vector<unsigned> grp(K);
+    for (unsigned k = 0; k < K; ++k) {
+        const unsigned long long l = ent[k].dir * h->cs + h->lin(ent[k].x, ent[k].y, ent[k].z);
+        auto it = gid.find(l);
+        if (it == gid.end()) {
+            it = gid.emplace(l, (unsigned)glin.size()).first;
+            glin.push_back(l);
+        }
+        grp[k] = it->second;
+    }
+    const unsigned G = (unsigned)glin.size();
+    std::vector<unsigned> gbeg(G + 1, 0), order(K);
+    for (unsigned k = 0; k < K; ++k) ++gbeg[grp[k] + 1];
+    for (unsigned g = 0; g < G; ++g) gbeg[g + 1] += gbeg[g];
+    std::vector<unsigned> fill(gbeg.begin(), gbeg.end() - 1);
+    for (unsigned k = 0; k < K; ++k) order[fill[grp[k]]++] = k;
+    std::vector<float> ga(K);
+    std::vector<unsigned> gd(K);
+    for (unsigned k = 0; k < K; ++k) {
+        ga[k] = ent[order[k]].amp;
+        gd[k] = ent[order[k]].delay;
+    }
+    E.G = G;
+    int rc;
+    if ((rc = upload(h, &E.gbeg, gbeg.data(), G + 1)) || (rc = upload(h, &E.glin, glin.data(), G)) ||
+        (rc = upload(h, &E.amp, ga.data(), K)) || (rc = upload(h, &E.delay, gd.data(), K)))
+        return rc;
+    return 0;
+}
+
+extern "C" {
+
 int ems_set_excitation(ems_engine* h, const float* sig_volt, const float* sig_curr, unsigned length, int period_steps,
                        unsigned nvolt, const unsigned* vidx, const unsigned short* vdir, const float* vamp,
                        const unsigned* vdelay, unsigned ncurr, const unsigned* cidx, const unsigned short* cdir,
@@ -3142,6 +3253,7 @@ int ems_add_mur(ems_engine* h, int ny, unsigned line, unsigned line_shift, const
     m.cP = dcp;
     m.cPP = dcpp;
     h->mur.push_back(m);
+    h->mur_geo.push_back(ems_engine::MurGeo{ny, line, line_shift});
     return 0;
 }
 
@@ -3249,7 +3361,7 @@ static int setup_fused(ems_engine* h)
     if (form <= 0) return 0;
     std::string why;
     const unsigned Nx = h->N[0], Ny = h->N[1], Nz = h->N[2];
-    if (!h->mur.empty()) why = "Mur ABC has no fused form";
+    if (!h->mur.empty() && form != 3) why = "Mur ABC has no fused form but the quad sweep's";
     else if (!h->lor.empty()) why = "Lorentz / Drude materials have no fused form";
     else if (!h->rlc.empty()) why = "lumped RLC elements have no fused form";
     else if (h->has_ss) why = "steady-state detection has no fused form";
@@ -3265,12 +3377,43 @@ static int setup_fused(ems_engine* h)
         if (h->pmlV.n) why = "the quad sweep has no UPML form";
         else if (Nx % 4) why = "the quad sweep needs a multiple of 4 x lines";
     }
+    // quad sweep with Mur faces: voltage sources on a face or neighbour line
+    // wait for the Mur hooks (exc_def[0]); the current update of the planes
+    // whose voltages the hooks and those sources change is redone (bands)
+    std::vector<ExcEnt> deferred[2];
+    auto on_face = [&](const ExcEnt& e) {
+        const unsigned p[3] = {e.x, e.y, e.z};
+        for (const auto& f : h->mur_geo)
+            if (p[f.ny] == f.line || p[f.ny] == f.shift) return true;
+        return false;
+    };
+    h->bands.clear();
+    if (why.empty() && form == 3) {
+        for (const auto& f : h->mur_geo) {
+            // I at plane k reads V at planes k and k+1 along the face normal
+            const unsigned a = std::min(f.line, f.shift), b = std::max(f.line, f.shift);
+            const unsigned lo = a ? a - 1 : 0, hi = std::min(b, h->N[f.ny] - 2);
+            if (lo <= hi) h->bands.push_back(ems_engine::Band{f.ny, lo, hi});
+        }
+        for (const auto& e : h->exc[1].host) {
+            const unsigned p[3] = {e.x, e.y, e.z};
+            for (const auto& b : h->bands)
+                if (p[b.axis] >= b.lo && p[b.axis] <= b.hi) {
+                    deferred[1].push_back(e);
+                    break;
+                }
+        }
+    }
     const unsigned nlists = form == 1 ? ntx * nty : h->g.plane;
     std::vector<std::vector<ExcEnt>> lists[2];
     size_t most = 0;
     for (int c = 0; c < 2 && why.empty(); ++c) {
         lists[c].assign(nlists, {});
         for (const auto& e : h->exc[c].host) {
+            if (form == 3 && c == 0 && !h->mur_geo.empty() && on_face(e)) {
+                deferred[0].push_back(e);
+                continue;
+            }
             if (form >= 2) {
                 lists[c][e.x + Nx * (unsigned)e.y].push_back(e);
                 continue;
@@ -3286,6 +3429,8 @@ static int setup_fused(ems_engine* h)
     if (why.empty() && most > 64) why = form == 1 ? "more than 64 sources in one tile" : "more than 64 sources in one column";
     if (!why.empty()) return fail(EMS_E_ARG, "fused step unavailable: " + why);
     int rc;
+    for (int c = 0; c < 2; ++c)
+        if ((rc = upload_exc_groups(h, deferred[c], h->exc_def[c]))) return rc;
     for (int c = 0; c < 2; ++c) {
         std::vector<unsigned> beg(nlists + 1, 0);
         std::vector<ExcEnt> ent;
@@ -3877,6 +4022,10 @@ void quad_step(ems_engine* h)
 // buffers, which then become the current state
 void fused_step(ems_engine* h, hipEvent_t* ev)
 {
+    const unsigned T = 256;
+    // Mur pre hooks read state n (FDTD/engine.cpp:224-265 order; only the quad sweep has Mur)
+    for (const auto& m : h->mur)
+        hipLaunchKernelGGL(k_mur_pre, dim3((m.na * m.nb + T - 1) / T), dim3(T), 0, h->stream, h->V, m, h->numts_dev);
     if (ev) (void)hipEventRecord(ev[0], h->stream);
     if (h->fused == 3) quad_step(h);
     else if (h->fused == 2) column_step(h);
@@ -3884,6 +4033,35 @@ void fused_step(ems_engine* h, hipEvent_t* ev)
     if (ev) {
         (void)hipEventRecord(ev[1], h->stream);
         (void)hipEventRecord(ev[2], h->stream);
+    }
+    if (!h->mur.empty()) {
+        // state n+1: Mur post (reverse creation order) on the swept voltages,
+        // Mur apply, the deferred voltage sources, then the current update of
+        // the planes next to each face again, with their current sources
+        for (int k = (int)h->mur.size() - 1; k >= 0; --k) {
+            const auto& m = h->mur[k];
+            hipLaunchKernelGGL(k_mur_post, dim3((m.na * m.nb + T - 1) / T), dim3(T), 0, h->stream, h->V_alt, m,
+                               h->numts_dev);
+        }
+        for (int k = (int)h->mur.size() - 1; k >= 0; --k) {
+            const auto& m = h->mur[k];
+            hipLaunchKernelGGL(k_mur_apply, dim3((m.na * m.nb + T - 1) / T), dim3(T), 0, h->stream, h->V_alt, m,
+                               h->numts_dev);
+        }
+        const auto& e0 = h->exc_def[0];
+        if (e0.G)
+            hipLaunchKernelGGL(k_excite, dim3((e0.G + T - 1) / T), dim3(T), 0, h->stream, h->V_alt, e0.G, e0.gbeg,
+                               e0.glin, e0.amp, e0.delay, h->sig[0], h->sigLen, h->period, h->numts_dev, 0);
+        for (const auto& b : h->bands) {
+            const unsigned long long cnt = (unsigned long long)(b.hi - b.lo + 1) * h->N[0] * h->N[1] * h->N[2] /
+                                           h->N[b.axis];
+            hipLaunchKernelGGL(k_curr_band, dim3((unsigned)((cnt + T - 1) / T)), dim3(T), 0, h->stream, h->g,
+                               h->V_alt, h->I, h->I_alt, h->kI, b.axis, b.lo, b.hi);
+        }
+        const auto& e1 = h->exc_def[1];
+        if (e1.G)
+            hipLaunchKernelGGL(k_excite, dim3((e1.G + T - 1) / T), dim3(T), 0, h->stream, h->I_alt, e1.G, e1.gbeg,
+                               e1.glin, e1.amp, e1.delay, h->sig[1], h->sigLen, h->period, h->numts_dev, 0);
     }
     hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, h->stream, h->numts_dev);
     if (ev) (void)hipEventRecord(ev[3], h->stream);

@@ -525,6 +525,9 @@ struct BasicEngine : EngineBase {
 // I-J-K-N storage (component fastest)
 // ---------------------------------------------------------------------------
 struct SSEEngine : EngineBase {
+    // flush-to-zero / denormals-are-zero as the reference's SSE engines run
+    // (FDTD/engine_sse.cpp:43); off, the engine keeps denormals like the basic one
+    bool ftz = true;
     unsigned nV;
     std::vector<f4vector> volt, curr;
     // plain coefficients (sse engine) or compressed table + index (sse-compressed)
@@ -744,8 +747,10 @@ struct SSEEngine : EngineBase {
 
     void worker(unsigned tid)
     {
-        _MM_SET_FLUSH_ZERO_MODE(_MM_FLUSH_ZERO_ON);          // Denormal::Disable (tools/denormal.h)
-        _MM_SET_DENORMALS_ZERO_MODE(_MM_DENORMALS_ZERO_ON);
+        if (ftz) {
+            _MM_SET_FLUSH_ZERO_MODE(_MM_FLUSH_ZERO_ON);      // Denormal::Disable (tools/denormal.h)
+            _MM_SET_DENORMALS_ZERO_MODE(_MM_DENORMALS_ZERO_ON);
+        }
         unsigned start = tstart[tid], stopx = tstop[tid];
         unsigned stop_h = (tid == nthreads - 1) ? stopx - 1 : stopx;
         while (true) {
@@ -795,8 +800,10 @@ struct SSEEngine : EngineBase {
             return;
         }
         unsigned csr = _mm_getcsr();
-        _MM_SET_FLUSH_ZERO_MODE(_MM_FLUSH_ZERO_ON);
-        _MM_SET_DENORMALS_ZERO_MODE(_MM_DENORMALS_ZERO_ON);
+        if (ftz) {
+            _MM_SET_FLUSH_ZERO_MODE(_MM_FLUSH_ZERO_ON);
+            _MM_SET_DENORMALS_ZERO_MODE(_MM_DENORMALS_ZERO_ON);
+        }
         for (unsigned it = 0; it < steps; ++it) {
             hooks(0, 0);
             UpdateVoltages(0, N[0]);
@@ -944,6 +951,13 @@ void* ref_create(int kind, const unsigned* N, const float* vv, const float* vi, 
         h->nthreads = (kind == K_MT) ? std::max(1u, nthreads) : 1;
     }
     return h;
+}
+
+// the SSE engines' flush-to-zero mode (set before the first iterate)
+void ref_set_ftz(void* hh, int on)
+{
+    Handle* h = (Handle*)hh;
+    if (h->se) h->se->ftz = on != 0;
 }
 
 // double-precision coefficients for the fp64 basic engine

@@ -51,6 +51,7 @@ def lib():
         up = C.POINTER(C.c_uint)
         L.ref_create.restype = vp
         L.ref_create.argtypes = [C.c_int, up, fp, fp, fp, fp, C.c_uint]
+        L.ref_set_ftz.argtypes = [vp, C.c_int]
         L.ref_set_coeffs_f64.argtypes = [vp] + [C.POINTER(C.c_double)] * 4
         L.ref_add_excitation.argtypes = [vp, fp, fp, C.c_uint, C.c_int,
                                          C.c_uint, up, C.POINTER(C.c_ushort), fp, up,
@@ -92,7 +93,7 @@ class RefEngine:
     """One restated reference engine instance built from a RefOperator."""
 
     def __init__(self, op: RefOperator, kind="basic", nthreads=1, coeffs64=None, steady_state=None,
-                 lorentz=None, rlc=None):
+                 lorentz=None, rlc=None, ftz=True):
         """steady_state: optional (period_steps, dirs, positions) for the
         restated Engine_Ext_SteadyState; lorentz / rlc: ADE tables in the
         form openems_amd.engine.EngineHIP takes them (created after Mur and
@@ -105,6 +106,9 @@ class RefEngine:
         self._keep = []
         arrs = [np.ascontiguousarray(a, np.float32) for a in (op.vv, op.vi, op.ii, op.iv)]
         self.h = L.ref_create(KIND[kind], _p(N, C.c_uint), *[_p(a, C.c_float) for a in arrs], int(nthreads))
+        # the SSE engines flush denormals as the reference's do; ftz=False keeps
+        # them, which makes those engines bit-identical to the basic one
+        L.ref_set_ftz(self.h, int(bool(ftz)))
         if kind == "basic_f64" and coeffs64 is not None:
             c64 = [np.ascontiguousarray(a, np.float64) for a in coeffs64]
             L.ref_set_coeffs_f64(self.h, *[_p(a, C.c_double) for a in c64])
@@ -640,11 +644,11 @@ class RefFieldDumpTD(RefFieldDumpFD):
         self.samples.append((ts, T, self.field(volt, curr)))
 
 
-def run_reference(op: RefOperator, nrts, probes=(), dumps=(), kind="basic", nthreads=1, eng=None):
+def run_reference(op: RefOperator, nrts, probes=(), dumps=(), kind="basic", nthreads=1, eng=None, ftz=True):
     """RunFDTD (openems.cpp:1388-1494) with the end criterion disabled:
     Process() at numTS=0 and after every timestep up to NrTS."""
     if eng is None:
-        eng = RefEngine(op, kind, nthreads)
+        eng = RefEngine(op, kind, nthreads, ftz=ftz)
     basic = kind in ("basic", "basic_f64")
 
     def snap():

@@ -901,3 +901,29 @@ def test_adjoint_reference_objective_f64():
     Jp = sum(abs(f) ** 2 for f in F)
     assert dT == ref.dT and Jp > 0
     assert J == pytest.approx(Jp, rel=1e-12)
+
+
+def test_sse_engines_flush_denormals_unless_asked_not_to():
+    """The reference's SSE engines run with flush-to-zero and denormals-are-
+    zero (FDTD/engine_sse.cpp:43), its basic engine keeps denormals.  A source
+    of denormal amplitude: the multithreaded engine without FTZ equals the
+    basic engine bit for bit (denormal fields), with FTZ it does not."""
+    scene = scenes.pec_cavity(16, 60)
+    scene["excitations"][0]["val"] = (0.0, 0.0, 1e-36)
+    ref = RefOperator(scene).build()
+    assert 0 < abs(float(ref.vexc.amp[0])) < np.finfo(np.float32).tiny
+    basic = RefEngine(ref, "basic")
+    basic.iterate(60)
+    bv, bc = basic.fields()
+    basic.close()
+    assert np.any((bv != 0) & (np.abs(bv) < np.finfo(np.float32).tiny))
+    mt = RefEngine(ref, "multithreaded", 2, ftz=False)
+    mt.iterate(60)
+    mv, mc = mt.fields()
+    mt.close()
+    assert np.array_equal(bv, mv) and np.array_equal(bc, mc)
+    mf = RefEngine(ref, "multithreaded", 2)
+    mf.iterate(60)
+    fv, _ = mf.fields()
+    mf.close()
+    assert not np.array_equal(bv, fv)

@@ -150,6 +150,11 @@ SWEEP_CASES = [
     ("pec_cavity_32", lambda: scenes.pec_cavity(32, 120)),
     ("pec_cavity_128_tile_edge_source", lambda: scenes.pec_cavity(128, 60)),
     ("wire_antenna_36_pec", lambda: dict(scenes.wire_antenna(36, 120), bc=[0] * 6)),
+    # Mur on every face, a sheet source across the whole mesh (on the face and
+    # neighbour lines of the x and y faces too), 300 steps (past the Mur
+    # start step), and Mur next to PEC and PMC faces
+    ("mur_layers_40", lambda: scenes.mur_layers(40, 36, 32, 300)),
+    ("mur_pec_pmc_mix", lambda: dict(scenes.mur_layers(36, 28, 44, 250), bc=[2, 0, 1, 2, 2, 0])),
 ]
 
 
@@ -204,6 +209,6 @@ def test_quad_sweep_random_initial_fields(gpu):
 def test_quad_sweep_refused_where_unavailable(gpu):
     for make, why in ((lambda: scenes.upml_bricks(40, 10, nbricks=4, dump=False), "UPML"),
                       (lambda: scenes.pec_cavity(30, 10), "multiple of 4"),
-                      (lambda: scenes.mur_layers(24, 20, 16, 10), "Mur")):
+                      (lambda: scenes.dispersive_rlc(40, 10), "fused form")):
         with pytest.raises(NativeError, match=why):
             EngineHIP(Operator(make()), fused="quads")
