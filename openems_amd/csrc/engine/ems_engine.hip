@@ -541,13 +541,40 @@ __device__ __forceinline__ void f4s(float4& v, int j, float a)
     else v.w = a;
 }
 
-// quad of plane positions of lane threadIdx.x in tile qt: row y, first x0
-__device__ __forceinline__ bool quad_column(const GridDesc& g, unsigned qt, unsigned& x0, unsigned& y)
+// Boxes of quads a quad half-step launch covers: quads qa .. qa+nq-1 of rows
+// ya .. ya+ny-1, planes za .. zb-1, in tiles of kBlock quads (ntiles per z
+// chunk of zd planes); `first` is the box's first workgroup.  One box is the whole mesh; a
+// UPML mesh splits into the box without UPML cells and the boxes around it.
+constexpr int kMaxQBoxes = 8;
+struct QBox {
+    unsigned qa, nq, ya, ny, za, zb, zd, ntiles, first;   // zd: planes per workgroup
+};
+// the boxes of one launch in device memory (read with a workgroup-uniform
+// index: a by-value array indexed at run time would go to per-lane scratch)
+struct QBoxes {
+    const QBox* b;
+    unsigned n;
+};
+
+// quad (x0..x0+3, y) and z range [z0, z1) of lane threadIdx.x in workgroup t;
+// false for a lane past its box
+__device__ __forceinline__ bool qbox_cell(const QBoxes B, unsigned t, unsigned& x0, unsigned& y, unsigned& z0,
+                                          unsigned& z1)
 {
-    const unsigned q = (qt * kBlock + threadIdx.x) * 4u;
-    y = q / g.Nx;
-    x0 = q - y * g.Nx;
-    return q < g.plane;
+    unsigned k = 0;
+    for (unsigned i = 1; i < B.n; ++i)
+        if (t >= B.b[i].first) k = i;
+    const QBox q = B.b[k];
+    const unsigned local = t - q.first;
+    const unsigned qt = local % q.ntiles, zc = local / q.ntiles;
+    const unsigned L = qt * kBlock + threadIdx.x;
+    if (L >= q.nq * q.ny) return false;
+    const unsigned r = L / q.nq;
+    x0 = 4u * (q.qa + (L - r * q.nq));
+    y = q.ya + r;
+    z0 = q.za + zc * q.zd;
+    z1 = min(z0 + q.zd, q.zb);
+    return true;
 }
 
 // per slab s, bits 4s..4s+3: the elements of quad (x0..x0+3, y) inside the
@@ -653,8 +680,8 @@ __device__ __forceinline__ void quad_coef(const CoefArgs& k, unsigned long long 
 }
 
 template <int PM, bool CC, bool NT, bool LT>
-__global__ __launch_bounds__(kBlock) void k_volt4(const GridDesc g, float* V, const float* __restrict__ I,
-                                                  const CoefArgs kg, const PmlArgs pml)
+__global__ __launch_bounds__(kBlock) void k_volt4(const GridDesc g, const QBoxes boxes, float* V,
+                                                  const float* __restrict__ I, const CoefArgs kg, const PmlArgs pml)
 {
     extern __shared__ float4 s_tab[];
     CoefArgs k = kg;
@@ -663,12 +690,8 @@ __global__ __launch_bounds__(kBlock) void k_volt4(const GridDesc g, float* V, co
         __syncthreads();
         k.tab = s_tab;
     }
-    const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
-    const unsigned qt = t % g.qtiles, zc = t / g.qtiles;
-    unsigned x0, y;
-    if (!quad_column(g, qt, x0, y)) return;
-    const unsigned z0 = zc * g.zdepth;
-    const unsigned z1 = min(z0 + g.zdepth, g.Nz);
+    unsigned x0, y, z0, z1;
+    if (!qbox_cell(boxes, xcd_tile(blockIdx.x, gridDim.x), x0, y, z0, z1)) return;
     const unsigned long long cs = g.cs;
     unsigned long long p = x0 + (unsigned long long)g.Nx * y + (unsigned long long)g.plane * z0;
     const unsigned ym = y ? g.Nx : 0u;
@@ -726,8 +749,9 @@ __global__ __launch_bounds__(kBlock) void k_volt4(const GridDesc g, float* V, co
 }
 
 template <int PM, bool CC, bool NT, bool LT>
-__global__ __launch_bounds__(kBlock) void k_curr4(const GridDesc g, float* I, const float* __restrict__ V,
-                                                  const CoefArgs kg, const PmlArgs pml, unsigned* __restrict__ numts)
+__global__ __launch_bounds__(kBlock) void k_curr4(const GridDesc g, const QBoxes boxes, float* I,
+                                                  const float* __restrict__ V, const CoefArgs kg, const PmlArgs pml,
+                                                  unsigned* __restrict__ numts)
 {
     extern __shared__ float4 s_tab[];
     CoefArgs k = kg;
@@ -736,17 +760,16 @@ __global__ __launch_bounds__(kBlock) void k_curr4(const GridDesc g, float* I, co
         __syncthreads();
         k.tab = s_tab;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(numts, 1u);
-    const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
-    const unsigned qt = t % g.qtiles, zc = t / g.qtiles;
-    unsigned x0, y;
-    if (!quad_column(g, qt, x0, y)) return;
+    unsigned x0, y, z0, z1;
+    const bool lane = qbox_cell(boxes, xcd_tile(blockIdx.x, gridDim.x), x0, y, z0, z1);
+    // one launch of the half step advances the timestep counter
+    if (numts && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(numts, 1u);
+    if (!lane) return;
     const bool yupd = y + 1 < g.Ny;
     // elements the main update covers along x (x < Nx-1)
     const unsigned xmain = x0 + 4 < g.Nx ? 15u : 7u;
     if (PM == kNoPml && !yupd) return;
-    const unsigned z0 = zc * g.zdepth;
-    const unsigned z1 = min(z0 + g.zdepth, PM == kNoPml ? g.Nz - 1 : g.Nz);
+    if (PM == kNoPml) z1 = min(z1, g.Nz - 1);
     if (z0 >= z1) return;
     const unsigned long long cs = g.cs;
     const unsigned colbits = PM != kNoPml ? quad_colbits(pml, (int)x0, (int)y) : 0u;
@@ -2600,7 +2623,13 @@ struct ems_engine {
     int quad_req = -1;                    // -1 auto, 0 one cell per lane, 1 quads (four x cells per lane)
     bool lds_tab = false;                 // quad kernels read the compressed tables from LDS
     bool quad = false;                    // the half steps run k_volt4 / k_curr4
-    GridDesc gq{};                        // tiles of the quad half steps
+    GridDesc gq{};                        // z depth of the quad half steps
+    // quad boxes: the whole mesh, or on a UPML mesh the box without UPML
+    // cells (qbox_in, launched without UPML code) and the boxes around it
+    QBoxes qbox_all{}, qbox_in{}, qbox_out{};
+    unsigned qblk_all = 0, qblk_in = 0, qblk_out = 0;
+    QBox* qbox_dev = nullptr;             // the three box lists, kMaxQBoxes each
+    bool qsplit = false;
     unsigned ntx = 0, nty = 0, fzdepth = 0, fnzc = 0;
     ExcArgs eV{}, eI{};
     int pml_mode = kNoPml;
@@ -2659,6 +2688,7 @@ struct ems_engine {
         std::vector<unsigned char> key;
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
+        size_t nalloc = 0;                // device allocations of the engine when captured
     };
     unsigned graph_chunk = 32;            // requested steps per graph (0: direct launches)
     unsigned graph_len = 0;               // steps per graph of the captured set
@@ -3228,7 +3258,12 @@ int ems_add_mur(ems_engine* h, int ny, unsigned line, unsigned line_shift, const
     m.offPP = nyPP * h->cs;
     m.start_ts = start_ts;
     const size_t cnt = (size_t)m.na * m.nb;
+    // device order of the face points: the lower tangential axis fastest (x
+    // for y and z faces, y for x faces), so that neighbouring threads touch
+    // neighbouring field elements; the caller's arrays are [nyP][nyPP]
+    const bool tr = nyP < nyPP;
     std::vector<unsigned> face(cnt), shift(cnt);
+    std::vector<float> cP(cnt), cPP(cnt);
     for (unsigned i = 0; i < m.na; ++i)
         for (unsigned j = 0; j < m.nb; ++j) {
             unsigned p[3], s[3];
@@ -3236,14 +3271,17 @@ int ems_add_mur(ems_engine* h, int ny, unsigned line, unsigned line_shift, const
             s[ny] = line_shift;
             p[nyP] = s[nyP] = i;
             p[nyPP] = s[nyPP] = j;
-            face[(size_t)i * m.nb + j] = (unsigned)h->lin(p[0], p[1], p[2]);
-            shift[(size_t)i * m.nb + j] = (unsigned)h->lin(s[0], s[1], s[2]);
+            const size_t src = (size_t)i * m.nb + j, k = tr ? (size_t)j * m.na + i : src;
+            face[k] = (unsigned)h->lin(p[0], p[1], p[2]);
+            shift[k] = (unsigned)h->lin(s[0], s[1], s[2]);
+            cP[k] = coeff_nyP[src];
+            cPP[k] = coeff_nyPP[src];
         }
     int rc;
     unsigned *df, *dsh;
     float *dcp, *dcpp;
     if ((rc = upload(h, &df, face.data(), cnt)) || (rc = upload(h, &dsh, shift.data(), cnt)) ||
-        (rc = upload(h, &dcp, coeff_nyP, cnt)) || (rc = upload(h, &dcpp, coeff_nyPP, cnt)) ||
+        (rc = upload(h, &dcp, cP.data(), cnt)) || (rc = upload(h, &dcpp, cPP.data(), cnt)) ||
         (rc = h->alloc(&m.mP, cnt)) || (rc = h->alloc(&m.mPP, cnt)))
         return rc;
     HIPCHK(hipMemset(m.mP, 0, cnt * sizeof(float)));
@@ -3559,10 +3597,13 @@ int ems_steady_state(ems_engine* h, double* last_max_diff, unsigned* evaluations
 
 // Quad half steps (k_volt4 / k_curr4): two-half-step engines whose rows hold
 // a whole number of four-cell groups.  Automatic mode takes them on meshes
-// without UPML (1024 x 1024 x 768 Mur layers: 6.37 -> 5.90 ms per half
-// step); UPML meshes keep one cell per lane, where the quads' UPML path
-// costs more than the 16-byte accesses gain (256^3 UPML bricks: 0.161 ->
-// 0.222 ms).  z depth: 4 planes per workgroup, shallower when the grid would
+// without UPML (1024 x 1024 x 768 Mur layers: 6.37 -> 5.79 ms per half
+// step).  On a UPML mesh, quads run as two launches: the box without UPML
+// cells without UPML code, the six boxes around it with it.  Automatic mode
+// keeps one cell per lane there: on the 256^3 UPML bricks the UPML-free box
+// (82 % of the cells) takes 119 us, but the quads' UPML path over the
+// remaining 18 % takes 75 us, against 161 us for the whole mesh at one cell
+// per lane.  z depth: 4 planes per workgroup, shallower when the grid would
 // give fewer than 2048 workgroups.
 static int setup_quad(ems_engine* h)
 {
@@ -3572,7 +3613,6 @@ static int setup_quad(ems_engine* h)
     if (h->fused) why = "the single-sweep step forms have no quad form";
     else if (h->N[0] % 4) why = "the number of x lines is not a multiple of 4";
     if (!why.empty()) return h->quad_req == 1 ? fail(EMS_E_ARG, "quad half steps unavailable: " + why) : 0;
-    if (h->quad_req == -1 && h->pmlV.n) return 0;
     GridDesc& q = h->gq;
     q = h->g;
     q.tx = 0;
@@ -3586,6 +3626,91 @@ static int setup_quad(ems_engine* h)
     }
     q.zdepth = d;
     q.nzc = (q.Nz + d - 1) / d;
+    const unsigned NQ = h->N[0] / 4;
+    // boxes in quads x rows x planes; workgroups counted box after box
+    QBox hb[3][kMaxQBoxes] = {};          // all, in, out
+    unsigned nb[3] = {0, 0, 0}, blk[3] = {0, 0, 0};
+    // the boxes around the UPML-free box hold few cells and the longer UPML
+    // update: one plane per workgroup, for enough waves to hide its latency
+    unsigned dout = 1;
+    if (const char* zo = std::getenv("EMS_QZOUT")) {
+        const int v = std::atoi(zo);
+        if (v >= 1 && v <= 1024) dout = (unsigned)v;
+    }
+    auto add = [&](int set, unsigned qa, unsigned nq, unsigned ya, unsigned ny, unsigned za, unsigned zb) {
+        if (!nq || !ny || za >= zb) return;
+        QBox& b = hb[set][nb[set]++];
+        b.qa = qa;
+        b.nq = nq;
+        b.ya = ya;
+        b.ny = ny;
+        b.za = za;
+        b.zb = zb;
+        b.zd = set == 2 ? dout : d;
+        b.ntiles = (nq * ny + kBlock - 1) / kBlock;
+        b.first = blk[set];
+        blk[set] += b.ntiles * ((zb - za + b.zd - 1) / b.zd);
+    };
+    add(0, 0, NQ, 0, h->N[1], 0, h->N[2]);
+    // UPML mesh: the largest box of whole quads holding no UPML cell, found
+    // along the mesh's centre lines, and the six boxes around it
+    h->qsplit = false;
+    if (h->pmlV.n) {
+        const PmlArgs& P = h->pmlV;
+        const unsigned c[3] = {h->N[0] / 2, h->N[1] / 2, h->N[2] / 2};
+        unsigned lo[3], hi[3];                  // interior [lo, hi) per axis
+        for (int a = 0; a < 3; ++a) {
+            auto in_slab = [&](unsigned v) {
+                unsigned p[3] = {c[0], c[1], c[2]};
+                p[a] = v;
+                for (int s = 0; s < P.n; ++s) {
+                    const SlabTab& S = P.tab[s];
+                    if ((int)p[0] >= S.x0 && (int)p[0] <= S.x1 && (int)p[1] >= S.y0 && (int)p[1] <= S.y1 &&
+                        (int)p[2] >= S.z0 && (int)p[2] <= S.z1)
+                        return true;
+                }
+                return false;
+            };
+            lo[a] = 0;
+            while (lo[a] < h->N[a] && in_slab(lo[a])) ++lo[a];
+            hi[a] = h->N[a];
+            while (hi[a] > lo[a] && in_slab(hi[a] - 1)) --hi[a];
+        }
+        const unsigned qa = (lo[0] + 3) / 4, qb = hi[0] / 4;
+        bool ok = qa < qb && lo[1] < hi[1] && lo[2] < hi[2];
+        // no slab may reach into the box
+        for (int s = 0; s < P.n && ok; ++s) {
+            const SlabTab& S = P.tab[s];
+            const bool meet = S.x0 < (int)(4 * qb) && S.x1 >= (int)(4 * qa) && S.y0 < (int)hi[1] &&
+                              S.y1 >= (int)lo[1] && S.z0 < (int)hi[2] && S.z1 >= (int)lo[2];
+            if (meet) ok = false;
+        }
+        if (ok) {
+            add(1, qa, qb - qa, lo[1], hi[1] - lo[1], lo[2], hi[2]);
+            const unsigned Ny = h->N[1], Nz = h->N[2];
+            add(2, 0, NQ, 0, Ny, 0, lo[2]);                        // planes below
+            add(2, 0, NQ, 0, Ny, hi[2], Nz);                       // planes above
+            add(2, 0, NQ, 0, lo[1], lo[2], hi[2]);                 // rows in front
+            add(2, 0, NQ, hi[1], Ny - hi[1], lo[2], hi[2]);        // rows behind
+            add(2, 0, qa, lo[1], hi[1] - lo[1], lo[2], hi[2]);     // quads left
+            add(2, qb, NQ - qb, lo[1], hi[1] - lo[1], lo[2], hi[2]);  // quads right
+            h->qsplit = true;
+        }
+    }
+    if (!h->qbox_dev) {
+        int rc;
+        if ((rc = h->alloc(&h->qbox_dev, 3 * kMaxQBoxes))) return rc;
+    }
+    HIPCHK(hipMemcpy(h->qbox_dev, hb, sizeof(hb), hipMemcpyHostToDevice));
+    h->qbox_all = QBoxes{h->qbox_dev, nb[0]};
+    h->qbox_in = QBoxes{h->qbox_dev + kMaxQBoxes, nb[1]};
+    h->qbox_out = QBoxes{h->qbox_dev + 2 * kMaxQBoxes, nb[2]};
+    h->qblk_all = blk[0];
+    h->qblk_in = blk[1];
+    h->qblk_out = blk[2];
+    if (const char* sp = std::getenv("EMS_QSPLIT")) h->qsplit = h->qsplit && std::atoi(sp) != 0;
+    // automatic mode: quads on meshes without UPML
+    if (h->quad_req == -1 && h->pmlV.n) return 0;
     h->quad = true;
     return 0;
 }
@@ -3868,25 +3993,38 @@ void launch_update(ems_engine* h, bool curr, unsigned nblk)
 }
 
 template <int PM, bool CC, bool LT>
-void launch_quad_lt(ems_engine* h, bool curr)
+void launch_quad_box(ems_engine* h, bool curr, const QBoxes& B, unsigned nblk, unsigned* numts)
 {
-    const unsigned nblk = h->gq.qtiles * h->gq.nzc;
+    if (!nblk) return;
     const size_t lds = LT ? (size_t)h->kV.ntab * sizeof(float4) : 0;
     if (curr) {
         if (h->nt_fields)
-            hipLaunchKernelGGL((k_curr4<PM, CC, true, LT>), dim3(nblk), dim3(kBlock), lds, h->stream, h->gq, h->I, h->V,
-                               h->kI, h->pmlI, h->numts_dev);
+            hipLaunchKernelGGL((k_curr4<PM, CC, true, LT>), dim3(nblk), dim3(kBlock), lds, h->stream, h->gq, B, h->I,
+                               h->V, h->kI, h->pmlI, numts);
         else
-            hipLaunchKernelGGL((k_curr4<PM, CC, false, LT>), dim3(nblk), dim3(kBlock), lds, h->stream, h->gq, h->I, h->V,
-                               h->kI, h->pmlI, h->numts_dev);
+            hipLaunchKernelGGL((k_curr4<PM, CC, false, LT>), dim3(nblk), dim3(kBlock), lds, h->stream, h->gq, B, h->I,
+                               h->V, h->kI, h->pmlI, numts);
     } else {
         if (h->nt_fields)
-            hipLaunchKernelGGL((k_volt4<PM, CC, true, LT>), dim3(nblk), dim3(kBlock), lds, h->stream, h->gq, h->V, h->I,
-                               h->kV, h->pmlV);
+            hipLaunchKernelGGL((k_volt4<PM, CC, true, LT>), dim3(nblk), dim3(kBlock), lds, h->stream, h->gq, B, h->V,
+                               h->I, h->kV, h->pmlV);
         else
-            hipLaunchKernelGGL((k_volt4<PM, CC, false, LT>), dim3(nblk), dim3(kBlock), lds, h->stream, h->gq, h->V, h->I,
-                               h->kV, h->pmlV);
+            hipLaunchKernelGGL((k_volt4<PM, CC, false, LT>), dim3(nblk), dim3(kBlock), lds, h->stream, h->gq, B, h->V,
+                               h->I, h->kV, h->pmlV);
     }
+}
+
+template <int PM, bool CC, bool LT>
+void launch_quad_lt(ems_engine* h, bool curr)
+{
+    if (PM != kNoPml && h->qsplit) {
+        // the box without UPML cells runs without UPML code; the boxes around
+        // it (and the timestep counter) take the UPML path
+        launch_quad_box<kNoPml, CC, LT>(h, curr, h->qbox_in, h->qblk_in, nullptr);
+        launch_quad_box<PM, CC, LT>(h, curr, h->qbox_out, h->qblk_out, h->numts_dev);
+        return;
+    }
+    launch_quad_box<PM, CC, LT>(h, curr, h->qbox_all, h->qblk_all, h->numts_dev);
 }
 
 template <int PM, bool CC>
@@ -4233,6 +4371,11 @@ static std::vector<unsigned char> chunk_key(const ems_engine* h, unsigned steps)
 // captured from the direct launches the first time the pattern occurs
 static int graph_chunk(ems_engine* h, unsigned steps)
 {
+    // liveness of what the captured launches reference: every buffer a graph
+    // names is an allocation of this engine, kept until ems_destroy; one made
+    // after a capture (a table uploaded again) means the graphs may name the
+    // previous one, so they are captured anew
+    if (!h->graphs.empty() && h->graphs.front().nalloc != h->allocs.size()) drop_graphs(h);
     const std::vector<unsigned char> key = chunk_key(h, steps);
     ems_engine::GraphEnt* G = nullptr;
     for (auto& e : h->graphs)
@@ -4274,6 +4417,7 @@ static int graph_chunk(ems_engine* h, unsigned steps)
     if (ec != hipSuccess) return fail(EMS_E_HIP, std::string("graph capture failed: ") + hipGetErrorString(ec));
     HIPCHK(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
     HIPCHK(hipGraphLaunch(e.exec, h->stream));
+    e.nalloc = h->allocs.size();
     h->graphs.push_back(std::move(e));
     ++h->graph_captures;
     return 0;

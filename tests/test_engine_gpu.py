@@ -733,3 +733,31 @@ def test_run_stops_on_abort(gpu, tmp_path, monkeypatch):
     sim.run(200)
     assert sim.eng.numTS == 400 and not sim.aborted
     sim.close()
+
+
+def test_graphs_recaptured_after_probe_tables_change(gpu):
+    """A probe added after chunks were captured uploads new probe tables:
+    the captured chunks are dropped and captured anew, and the fields and
+    the new probe's series equal direct launches."""
+    scene = scenes.upml_bricks(40, 200, nbricks=4, dump=False)
+    op = Operator(scene)
+    print("case upml_bricks_40 graph recapture", flush=True)
+    engs = [EngineHIP(op), EngineHIP(op)]
+    engs[0].set_graph(0)
+    engs[1].set_graph(8)
+    try:
+        for e in engs:
+            e.IterateTS(40)
+        assert engs[1].graph()[1] >= 1
+        caps = engs[1].graph()[1]
+        pos = np.array([[20, 21], [20, 20], [20, 20]], np.uint32)   # two E_x edges
+        for e in engs:
+            e.add_probe(0, [0, 0], pos, None, None, 2, 0, 0, 10 ** 6, 200)
+            e.IterateTS(64)
+        assert engs[1].graph()[1] > caps
+        assert np.array_equal(engs[0].get_volt(), engs[1].get_volt())
+        a, b = engs[0].probe_results(0), engs[1].probe_results(0)
+        assert len(a[0]) == 32 and np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    finally:
+        for e in engs:
+            e.close()

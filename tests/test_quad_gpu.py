@@ -122,10 +122,11 @@ def test_quad_unavailable_for_odd_rows(gpu):
 
 def test_quad_default_policy(gpu):
     """Automatic mode: quads on meshes without UPML, one cell per lane on
-    UPML meshes."""
+    UPML meshes and where the x lines are not a multiple of 4."""
     for make, want in ((lambda: scenes.mur_layers(40, 36, 32, 10), 1),
                        (lambda: scenes.pec_cavity(32, 10), 1),
-                       (lambda: scenes.upml_bricks(48, 10, nbricks=4, dump=False), 0)):
+                       (lambda: scenes.upml_bricks(48, 10, nbricks=4, dump=False), 0),
+                       (lambda: scenes.upml_bricks(46, 10, nbricks=4, dump=False), 0)):
         op = Operator(make())
         eng = EngineHIP(op)
         assert eng.quad()[0] == want
@@ -212,3 +213,25 @@ def test_quad_sweep_refused_where_unavailable(gpu):
                       (lambda: scenes.dispersive_rlc(40, 10), "fused form")):
         with pytest.raises(NativeError, match=why):
             EngineHIP(Operator(make()), fused="quads")
+
+
+@pytest.mark.parametrize("pml", [6, 8])
+def test_quad_split_boxes_match_single_box(gpu, pml, monkeypatch):
+    """A UPML mesh split into the box without UPML cells (run without UPML
+    code) and the boxes around it gives the same fields as one box over the
+    whole mesh and as one cell per lane, from random initial fields (6-cell
+    UPML: the box's x faces fall inside quads, whose other cells go to the
+    outer boxes; the outer boxes at one plane and at 3 planes per
+    workgroup)."""
+    scene = scenes.upml_bricks(52, 25, nbricks=6, pml=pml, dump=False)
+    op = Operator(scene)
+    rng = np.random.default_rng(31 + pml)
+    shape = (3,) + tuple(op.N)
+    init = (rng.standard_normal(shape).astype(np.float32), rng.standard_normal(shape).astype(np.float32))
+    out = []
+    for split, zout, quad in (("1", "1", True), ("1", "3", True), ("0", "1", True), ("1", "1", False)):
+        monkeypatch.setenv("EMS_QSPLIT", split)
+        monkeypatch.setenv("EMS_QZOUT", zout)
+        out.append(_run(op, 25, "compressed", quad, init))
+    for v, c, _ in out[1:]:
+        assert np.array_equal(out[0][0], v) and np.array_equal(out[0][1], c)
