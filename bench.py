@@ -60,11 +60,12 @@ def pml_positions(op):
     return sum(s["num"][0] * s["num"][1] * s["num"][2] for s in op.upml_slabs())
 
 
-def traffic_from_profile(kernel, layout, size):
+def traffic_from_profile(kernel, layout, size=None, pattern=None):
     """HBM bytes per launch of `kernel` measured by rocprofv3 FETCH_SIZE / WRITE_SIZE
     runs of this workload and coefficient layout (committed summary), or None."""
     import glob
-    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", f"upml_bricks_{size}_traffic*.json"))):
+    pattern = pattern or f"upml_bricks_{size}_traffic*.json"
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", pattern))):
         try:
             with open(path) as f:
                 d = json.load(f)
@@ -388,17 +389,21 @@ def run_memory_scale(steps=200, warmup=20):
     eng.set_profiling(False)
     sim.close()
     kms = max(vms, cms)
+    kname = ("k_curr4" if quad else "k_curr") if cms >= vms else ("k_volt4" if quad else "k_volt")
     per_launch = LAYOUT_BYTES[lay] * cells
+    layout = f"{lay} ({nu} unique), two half steps, {'four x cells per lane' if quad else 'one cell per lane'}"
     return {"workload": "mur_layers: 1024x1024x768 lines, 12 seeded layers, Mur x6, sinusoidal E_x sheet, "
                         f"16 E probes + 1 voltage probe, {steps} steps", "cells": int(cells),
-            "layout": f"{lay} ({nu} unique), two half steps, {'four x cells per lane' if quad else 'one cell per lane'}",
+            "layout": layout,
             "value": round(cells * steps / wall / 1e6, 1), "unit": "MCells/s", "ms_per_step": round(wall * 1e3 / steps, 3),
             "setup_s": round(build_s, 1),
             "update_kernel": "k_volt4/k_curr4" if quad else "k_volt/k_curr",
             "update_kernel_ms": round(kms, 4), "volt_ms": round(vms, 4), "curr_ms": round(cms, 4),
             "bytes_per_launch": int(per_launch), "bytes_per_cell_update": 2 * LAYOUT_BYTES[lay],
             "achieved_GBs": round(per_launch / (kms * 1e-3) / 1e9, 1),
-            "frac": round(per_launch / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            "frac": round(per_launch / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "dominant_kernel": kname,
+            "traffic": traffic_from_profile(kname, layout, pattern="mur_layers_1024x1024x768_traffic*.json")}
 
 
 def run_plain_cavity(steps=2000, cpu=True, nthreads=1):
