@@ -378,6 +378,7 @@ def run_memory_scale(steps=200, warmup=20):
     cells = float(op.num_cells)
     lay, nu, _, _ = eng.layout()
     quad, qz = eng.quad()
+    form, fz = eng.fused()
     eng.IterateTS(warmup)
     eng.sync()
     eng.set_profiling(True)
@@ -388,18 +389,28 @@ def run_memory_scale(steps=200, warmup=20):
     _, vms, cms = eng.last_timing()
     eng.set_profiling(False)
     sim.close()
-    kms = max(vms, cms)
-    kname = ("k_curr4" if quad else "k_curr") if cms >= vms else ("k_volt4" if quad else "k_volt")
-    per_launch = LAYOUT_BYTES[lay] * cells
-    layout = f"{lay} ({nu} unique), two half steps, {'four x cells per lane' if quad else 'one cell per lane'}"
+    if form == 3:
+        # one sweep per step (volt_ms: the sweep; curr_ms: the Mur hooks and
+        # the face-band currents after it)
+        kms = vms
+        kname = "k_step4t" if op.N[0] // 4 >= 64 else "k_step4"
+        per_launch = FUSED_BYTES[lay] * cells
+        layout = f"{lay} ({nu} unique), quad column sweep in 64 x 4 quad tiles, z depth {fz}"
+        cell_bytes = FUSED_BYTES[lay]
+    else:
+        kms = max(vms, cms)
+        kname = ("k_curr4" if quad else "k_curr") if cms >= vms else ("k_volt4" if quad else "k_volt")
+        per_launch = LAYOUT_BYTES[lay] * cells
+        layout = f"{lay} ({nu} unique), two half steps, {'four x cells per lane' if quad else 'one cell per lane'}"
+        cell_bytes = 2 * LAYOUT_BYTES[lay]
     return {"workload": "mur_layers: 1024x1024x768 lines, 12 seeded layers, Mur x6, sinusoidal E_x sheet, "
                         f"16 E probes + 1 voltage probe, {steps} steps", "cells": int(cells),
             "layout": layout,
             "value": round(cells * steps / wall / 1e6, 1), "unit": "MCells/s", "ms_per_step": round(wall * 1e3 / steps, 3),
             "setup_s": round(build_s, 1),
-            "update_kernel": "k_volt4/k_curr4" if quad else "k_volt/k_curr",
+            "update_kernel": kname if form == 3 else ("k_volt4/k_curr4" if quad else "k_volt/k_curr"),
             "update_kernel_ms": round(kms, 4), "volt_ms": round(vms, 4), "curr_ms": round(cms, 4),
-            "bytes_per_launch": int(per_launch), "bytes_per_cell_update": 2 * LAYOUT_BYTES[lay],
+            "bytes_per_launch": int(per_launch), "bytes_per_cell_update": cell_bytes,
             "achieved_GBs": round(per_launch / (kms * 1e-3) / 1e9, 1),
             "frac": round(per_launch / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "dominant_kernel": kname,

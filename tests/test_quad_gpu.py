@@ -156,18 +156,24 @@ SWEEP_CASES = [
     # start step), and Mur next to PEC and PMC faces
     ("mur_layers_40", lambda: scenes.mur_layers(40, 36, 32, 300)),
     ("mur_pec_pmc_mix", lambda: dict(scenes.mur_layers(36, 28, 44, 250), bc=[2, 0, 1, 2, 2, 0])),
+    # 66 quads per row and 22 rows: two 64 x 4 tiles along x (the second with
+    # two quads) and a last row of tiles with two rows
+    ("mur_layers_264x22x18", lambda: scenes.mur_layers(264, 22, 18, 200)),
 ]
 
 
+@pytest.mark.parametrize("tile", ["0", "1"])
 @pytest.mark.parametrize("layout", ["plain", "compressed"])
 @pytest.mark.parametrize("name,make", SWEEP_CASES, ids=[c[0] for c in SWEEP_CASES])
-def test_quad_sweep_matches_oracle(gpu, name, make, layout):
+def test_quad_sweep_matches_oracle(gpu, name, make, layout, tile, monkeypatch):
     """Fields after the steps of the quad column sweep equal the oracle's bit
     for bit (sources at the mesh centre, on quad and tile boundaries, and on
-    curves)."""
+    curves), one column per lane (k_step4) and in 64 x 4 quad tiles
+    (k_step4t; rows of fewer than 64 quads leave lanes idle)."""
+    monkeypatch.setenv("EMS_Q4TILE", tile)
     scene = make()
     nrts = scene["nrts"]
-    print("case quad sweep", name, "layout", layout, "steps", nrts, flush=True)
+    print("case quad sweep", name, "layout", layout, "tile", tile, "steps", nrts, flush=True)
     op = Operator(scene)
     eng = EngineHIP(op, layout=layout, fused="quads")
     assert eng.fused()[0] == 3
@@ -183,9 +189,11 @@ def test_quad_sweep_matches_oracle(gpu, name, make, layout):
     ref.close()
 
 
-def test_quad_sweep_random_initial_fields(gpu):
+@pytest.mark.parametrize("tile", ["0", "1"])
+def test_quad_sweep_random_initial_fields(gpu, tile, monkeypatch):
     """Random fields everywhere: the last line of each direction keeps its
     currents, every other cell matches the oracle bit for bit."""
+    monkeypatch.setenv("EMS_Q4TILE", tile)
     scene = scenes.pec_cavity(40, 30)
     op = Operator(scene)
     ref = RefEngine(RefOperator(scene).build(), "basic")
