@@ -1981,26 +1981,14 @@ __global__ void k_mur_apply(float* __restrict__ V, const MurDev m, const unsigne
 }
 
 // The current update (curr_tile's arithmetic, FDTD/engine.cpp:170-222) redone
-// on the planes lo..hi along `axis`, from the final voltages V1 and the
-// currents of state n, into I1: the quad sweep's currents next to a Mur face,
-// whose voltages the Mur hooks and the deferred sources set after the sweep.
-__global__ void k_curr_band(const GridDesc g, const float* __restrict__ V1, const float* __restrict__ I0,
-                            float* __restrict__ I1, const CoefArgs k, int axis, unsigned lo, unsigned hi)
+// from the final voltages V1 and the currents of state n, into I1, for the
+// quad sweep's currents that read voltages set after the sweep: the planes
+// lo..hi along `axis` next to a Mur face (k_curr_band), and the cells next to
+// a deferred voltage source (k_curr_cells).
+__device__ __forceinline__ void curr_cell(const GridDesc& g, const float* __restrict__ V1, const float* __restrict__ I0,
+                                          float* __restrict__ I1, const CoefArgs& k, unsigned long long p)
 {
-    const unsigned n[3] = {axis == 0 ? hi - lo + 1 : g.Nx, axis == 1 ? hi - lo + 1 : g.Ny,
-                           axis == 2 ? hi - lo + 1 : g.Nz};
-    const unsigned long long cnt = (unsigned long long)n[0] * n[1] * n[2];
-    const unsigned long long t = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= cnt) return;
-    unsigned c[3];
-    c[0] = (unsigned)(t % n[0]);
-    c[1] = (unsigned)((t / n[0]) % n[1]);
-    c[2] = (unsigned)(t / ((unsigned long long)n[0] * n[1]));
-    c[axis] += lo;
-    const unsigned x = c[0], y = c[1], z = c[2];
-    if (x + 1 >= g.Nx || y + 1 >= g.Ny || z + 1 >= g.Nz) return;
     const unsigned long long cs = g.cs;
-    const unsigned long long p = x + (unsigned long long)g.Nx * y + (unsigned long long)g.plane * z;
     const unsigned long long dx = 1, dy = g.Nx, dz = g.plane;
     const float* Vx = V1;
     const float* Vy = V1 + cs;
@@ -2029,6 +2017,33 @@ __global__ void k_curr_band(const GridDesc g, const float* __restrict__ V1, cons
         f = f + b[m] * curl[m];
         I1[m * cs + p] = f;
     }
+}
+
+// the same at listed cells (each with x, y, z below the last line)
+__global__ void k_curr_cells(const GridDesc g, const float* __restrict__ V1, const float* __restrict__ I0,
+                             float* __restrict__ I1, const CoefArgs k, const unsigned long long* __restrict__ cells,
+                             unsigned n)
+{
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) curr_cell(g, V1, I0, I1, k, cells[t]);
+}
+
+__global__ void k_curr_band(const GridDesc g, const float* __restrict__ V1, const float* __restrict__ I0,
+                            float* __restrict__ I1, const CoefArgs k, int axis, unsigned lo, unsigned hi)
+{
+    const unsigned n[3] = {axis == 0 ? hi - lo + 1 : g.Nx, axis == 1 ? hi - lo + 1 : g.Ny,
+                           axis == 2 ? hi - lo + 1 : g.Nz};
+    const unsigned long long cnt = (unsigned long long)n[0] * n[1] * n[2];
+    const unsigned long long t = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    unsigned c[3];
+    c[0] = (unsigned)(t % n[0]);
+    c[1] = (unsigned)((t / n[0]) % n[1]);
+    c[2] = (unsigned)(t / ((unsigned long long)n[0] * n[1]));
+    c[axis] += lo;
+    const unsigned x = c[0], y = c[1], z = c[2];
+    if (x + 1 >= g.Nx || y + 1 >= g.Ny || z + 1 >= g.Nz) return;
+    curr_cell(g, V1, I0, I1, k, x + (unsigned long long)g.Nx * y + (unsigned long long)g.plane * z);
 }
 
 // ---------------------------------------------------------------------------
@@ -2919,6 +2934,10 @@ struct ems_engine {
         unsigned lo, hi;
     };
     std::vector<Band> bands;
+    // and at single cells: the currents reading a cell whose deferred source
+    // changes it after the sweep, outside the bands (linear cell indices)
+    unsigned long long* cell_list = nullptr;
+    unsigned ncells = 0;
     // lumped RLC and dispersive lists
     std::vector<RlcDev> rlc;
     std::vector<LorDev> lor;
@@ -3704,20 +3723,40 @@ static int setup_fused(ems_engine* h)
         return false;
     };
     h->bands.clear();
+    h->ncells = 0;
+    std::vector<unsigned long long> cells;
     if (why.empty() && form == 3) {
+        // the Mur apply sets the voltages of the face line only; the current
+        // at plane k reads the voltages of planes k and k+1 along the normal
         for (const auto& f : h->mur_geo) {
-            // I at plane k reads V at planes k and k+1 along the face normal
-            const unsigned a = std::min(f.line, f.shift), b = std::max(f.line, f.shift);
-            const unsigned lo = a ? a - 1 : 0, hi = std::min(b, h->N[f.ny] - 2);
+            const unsigned lo = f.line ? f.line - 1 : 0, hi = std::min(f.line, h->N[f.ny] - 2);
             if (lo <= hi) h->bands.push_back(ems_engine::Band{f.ny, lo, hi});
         }
+        auto in_band = [&](const unsigned p[3]) {
+            for (const auto& b : h->bands)
+                if (p[b.axis] >= b.lo && p[b.axis] <= b.hi) return true;
+            return false;
+        };
+        // a deferred voltage source changes its cell after the sweep: the
+        // currents of the cell and of its -x, -y, -z neighbours read it
+        for (const auto& e : h->exc[0].host) {
+            if (!on_face(e)) continue;
+            for (int d = 0; d < 4; ++d) {
+                unsigned p[3] = {e.x, e.y, e.z};
+                if (d < 3) {
+                    if (!p[d]) continue;
+                    --p[d];
+                }
+                if (p[0] + 1 >= Nx || p[1] + 1 >= Ny || p[2] + 1 >= Nz || in_band(p)) continue;
+                cells.push_back(h->lin(p[0], p[1], p[2]));
+            }
+        }
+        std::sort(cells.begin(), cells.end());
+        cells.erase(std::unique(cells.begin(), cells.end()), cells.end());
         for (const auto& e : h->exc[1].host) {
             const unsigned p[3] = {e.x, e.y, e.z};
-            for (const auto& b : h->bands)
-                if (p[b.axis] >= b.lo && p[b.axis] <= b.hi) {
-                    deferred[1].push_back(e);
-                    break;
-                }
+            if (in_band(p) || std::binary_search(cells.begin(), cells.end(), h->lin(e.x, e.y, e.z)))
+                deferred[1].push_back(e);
         }
     }
     const unsigned nlists = form == 1 ? ntx * nty : h->g.plane;
@@ -3750,6 +3789,10 @@ static int setup_fused(ems_engine* h)
     int rc;
     for (int c = 0; c < 2; ++c)
         if ((rc = upload_exc_groups(h, deferred[c], h->exc_def[c]))) return rc;
+    if (!cells.empty()) {
+        if ((rc = upload(h, &h->cell_list, cells.data(), cells.size()))) return rc;
+        h->ncells = (unsigned)cells.size();
+    }
     for (int c = 0; c < 2; ++c) {
         std::vector<unsigned> beg(nlists + 1, 0);
         std::vector<ExcEnt> ent;
@@ -4498,6 +4541,9 @@ void fused_step(ems_engine* h, hipEvent_t* ev)
             hipLaunchKernelGGL(k_curr_band, dim3((unsigned)((cnt + T - 1) / T)), dim3(T), 0, h->stream, h->g,
                                h->V_alt, h->I, h->I_alt, h->kI, b.axis, b.lo, b.hi);
         }
+        if (h->ncells)
+            hipLaunchKernelGGL(k_curr_cells, dim3((h->ncells + T - 1) / T), dim3(T), 0, h->stream, h->g, h->V_alt,
+                               h->I, h->I_alt, h->kI, h->cell_list, h->ncells);
         const auto& e1 = h->exc_def[1];
         if (e1.G)
             hipLaunchKernelGGL(k_excite, dim3((e1.G + T - 1) / T), dim3(T), 0, h->stream, h->I_alt, e1.G, e1.gbeg,
