@@ -1763,6 +1763,10 @@ __global__ __launch_bounds__(kBlock) void k_step4t(const Step4Args ag, const flo
     float vr2m = 0.0f, vr1m = 0.0f;
     // I^n of the own quad at plane z-1, carried from the previous plane
     float4 ixp = {}, iyp = {}, izp = {};
+    // and its packed table entries (the engine's two half steps share one
+    // index array; read again otherwise)
+    uint2 wp = make_uint2(0u, 0u);
+    const bool same_idx = a.kI.idx == a.kV.idx;
     if (z0) {
         const unsigned long long p0 = (unsigned long long)(z0 - 1) * plane + q;
         ixp = *reinterpret_cast<const float4*>(I0 + p0);
@@ -1786,6 +1790,7 @@ __global__ __launch_bounds__(kBlock) void k_step4t(const Step4Args ag, const flo
         }
         float4 v[3];
         unsigned u[4] = {0u, 0u, 0u, 0u};
+        uint2 wv = make_uint2(0u, 0u);
         {
             const float4 iz_ym = *reinterpret_cast<const float4*>(I0 + 2 * cs + p - ym);
             const float4 ix_ym = *reinterpret_cast<const float4*>(I0 + p - ym);
@@ -1797,7 +1802,13 @@ __global__ __launch_bounds__(kBlock) void k_step4t(const Step4Args ag, const flo
             v0[0] = ld4<NT>(V0 + p);
             v0[1] = ld4<NT>(V0 + cs + p);
             v0[2] = ld4<NT>(V0 + 2 * cs + p);
-            if (CC) quad_idx(a.kV.idx, p, u);
+            if (CC) {
+                wv = *reinterpret_cast<const uint2*>(a.kV.idx + p);
+                u[0] = wv.x & 0xFFFFu;
+                u[1] = wv.x >> 16;
+                u[2] = wv.y & 0xFFFFu;
+                u[3] = wv.y >> 16;
+            }
             float4 c, w;
             quad_coef_n<CC>(a.kV, cs, p, u, 0, c, w);
             v[0] = quad_math(v0[0], c, w, quad_curl(iz, iz_ym, iy, iy_m));
@@ -1821,7 +1832,13 @@ __global__ __launch_bounds__(kBlock) void k_step4t(const Step4Args ag, const flo
         if (z > z0 && z < a.zb && imask) {
             const unsigned long long pm = pz;
             unsigned um[4] = {0u, 0u, 0u, 0u};
-            if (CC) quad_idx(a.kI.idx, pm, um);
+            if (CC) {
+                const uint2 wi = same_idx ? wp : *reinterpret_cast<const uint2*>(a.kI.idx + pm);
+                um[0] = wi.x & 0xFFFFu;
+                um[1] = wi.x >> 16;
+                um[2] = wi.y & 0xFFFFu;
+                um[3] = wi.y >> 16;
+            }
             float4 u2 = vu2m, u0 = vu0m;
             if (!top) {
                 const unsigned bm = (z + 2u) % 3u;     // plane z-1
@@ -1857,6 +1874,7 @@ __global__ __launch_bounds__(kBlock) void k_step4t(const Step4Args ag, const flo
         ixp = ix;
         iyp = iy;
         izp = iz;
+        wp = wv;
         if (z < zlast && top && yupd) {
             // row y+1 at plane z (z and x components), outside the tile
             const unsigned long long pu = p + du;
@@ -3835,15 +3853,16 @@ static int setup_fused(ems_engine* h)
         h->fnzc = (Nz + h->fzdepth - 1) / h->fzdepth;
     } else if (form == 3) {
         // tiles of kBlock quads over the plane (k_step4), or of 64 quads x 4
-        // rows (k_step4t, rows of at least 64 quads); 8 planes per chunk (32
-        // for the tiles: 10.72 -> 10.61 ms per sweep on the Mur layers),
-        // shallower when the grid would give fewer than 2048 workgroups
+        // rows (k_step4t, rows of at least 64 quads); 8 planes per chunk (64
+        // for the tiles: 10.72, 10.61, 10.51 ms per sweep on the Mur layers
+        // at 16, 32, 64), shallower when the grid would give fewer than 2048
+        // workgroups
         const unsigned NQ = Nx / 4;
         h->q4tile = NQ >= 64;
         if (const char* te = std::getenv("EMS_Q4TILE")) h->q4tile = std::atoi(te) != 0;
         h->q4ntq = (NQ + 63) / 64;
         h->ntx = h->q4tile ? h->q4ntq * ((Ny + kQTileRows - 1) / kQTileRows) : (h->g.plane / 4 + kBlock - 1) / kBlock;
-        unsigned d = h->q4tile ? 32 : 8;
+        unsigned d = h->q4tile ? 64 : 8;
         while (d > 2 && (unsigned long long)h->ntx * ((Nz + d - 1) / d) < 2048ull) d /= 2;
         if (const char* zd = std::getenv("EMS_Q4ZDEPTH")) {
             const int v = std::atoi(zd);
