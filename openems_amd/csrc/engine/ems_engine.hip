@@ -1453,6 +1453,17 @@ __global__ __launch_bounds__(kBlock) void k_step2(const Step2Args a, const float
 // to another pass or to no update at all).  Per element the arithmetic is
 // k_step2's (and so k_volt / k_curr's): results are bit-identical.
 // ---------------------------------------------------------------------------
+// A Mur face on x = 0 (neighbour line 1) or x = Nx-1 (neighbour Nx-2) whose
+// pre and post hooks k_step4t runs (its points are numbered z * Ny + y)
+struct MurFold {
+    const float* cP;               // coefficients of the y and z components
+    const float* cPP;
+    float* mP;                     // the hooks' values, read by k_mur_apply
+    float* mPP;
+    unsigned start_ts;
+    unsigned on;
+};
+
 struct Step4Args {
     GridDesc g;
     CoefArgs kV, kI;
@@ -1461,6 +1472,7 @@ struct Step4Args {
     unsigned qa, nq, ya, ny, za, zb, xb, yb;
     unsigned zdepth, ntiles;       // planes per chunk; tiles of kBlock quads over nq * ny
     unsigned ntq;                  // k_step4t: tiles of 64 quads along x (ntiles = ntq x row tiles)
+    MurFold mf[2];                 // k_step4t: the x = 0 and x = Nx-1 Mur faces
 };
 
 // coefficient pair (c, v) of component n of the quad at p (table entries u[])
@@ -1711,6 +1723,25 @@ __global__ __launch_bounds__(kBlock) void k_step4(const Step4Args ag, const floa
 // ---------------------------------------------------------------------------
 constexpr unsigned kQTileRows = kBlock / 64;
 
+// Mur pre and post hooks of point k (coefficients cp, cpp) for face element
+// F and neighbour element S of a quad (k_mur_pre / k_mur_post arithmetic):
+// y and z from V^n (v0) and V^{n+1} (v); stored when `keep`
+template <int F, int S>
+__device__ __forceinline__ void mur_fold_quad(const MurFold& M, unsigned k, float cp, float cpp, const float4 v0[3],
+                                              const float4 v[3], bool keep, float& fy, float& fz)
+{
+    float my = f4g(v0[1], S) - cp * f4g(v0[1], F);
+    my = my + cp * f4g(v[1], S);
+    float mz = f4g(v0[2], S) - cpp * f4g(v0[2], F);
+    mz = mz + cpp * f4g(v[2], S);
+    if (keep) {
+        M.mP[k] = my;
+        M.mPP[k] = mz;
+    }
+    fy = my;
+    fz = mz;
+}
+
 template <bool CC, bool NT, bool LT>
 __global__ __launch_bounds__(kBlock) void k_step4t(const Step4Args ag, const float* __restrict__ V0,
                                                    const float* __restrict__ I0, float* __restrict__ V1,
@@ -1757,6 +1788,14 @@ __global__ __launch_bounds__(kBlock) void k_step4t(const Step4Args ag, const flo
     const unsigned eub = a.eV.beg[q + du], eue = yupd ? a.eV.beg[q + du + 4] : eub;
     // cell x0+4: the next quad's list (ordered by plane), entries of x0+4 only
     const unsigned erb = a.eV.beg[q + dr], ere = xr ? a.eV.beg[q + dr + 4] : erb;
+    // Mur faces on x = 0 and x = Nx-1: the quad holding the face runs their
+    // pre and post hooks (Engine_Ext_Mur_ABC::DoPreVoltageUpdates /
+    // DoPostVoltageUpdates, engine_ext_mur_abc.cpp:68-142) and uses the value
+    // the apply hook will set for the face's y and z components in its own
+    // current update; V^{n+1} itself is stored as swept, for the other faces'
+    // hooks, and k_mur_apply writes the face afterwards
+    const bool flo = a.mf[0].on && x0 == 0 && (unsigned)ts >= a.mf[0].start_ts;
+    const bool fhi = a.mf[1].on && x0 + 4 == Nx && (unsigned)ts >= a.mf[1].start_ts;
     SrcCursor cv = src_cursor(a.eV, evb, eve, z0), ci = src_cursor(a.eI, eib, eie, z0);
     SrcCursor cu = src_cursor(a.eV, eub, eue, z0);
     float4 vm[3] = {}, vu2m = {}, vu0m = {};
@@ -1777,6 +1816,16 @@ __global__ __launch_bounds__(kBlock) void k_step4t(const Step4Args ag, const flo
         const unsigned long long p = (unsigned long long)z * plane + q;
         const unsigned long long pz = z ? p - plane : p;
         const unsigned b = z % 3u;
+        // Mur coefficients of this plane's face points, issued with the field loads
+        float mcp = 0.0f, mcpp = 0.0f, mcpp1 = 0.0f;
+        if (flo) {
+            mcp = a.mf[0].cP[z * a.ny + y];
+            mcpp = a.mf[0].cPP[z * a.ny + y];
+            if (top && yupd && z < zlast) mcpp1 = a.mf[0].cPP[z * a.ny + y + 1];
+        } else if (fhi) {
+            mcp = a.mf[1].cP[z * a.ny + y];
+            mcpp = a.mf[1].cPP[z * a.ny + y];
+        }
         // ---- own V^{n+1}(x0..x0+3, y, z)
         const float4 ix = *reinterpret_cast<const float4*>(I0 + p);
         const float4 iy = *reinterpret_cast<const float4*>(I0 + cs + p);
@@ -1791,6 +1840,7 @@ __global__ __launch_bounds__(kBlock) void k_step4t(const Step4Args ag, const flo
         float4 v[3];
         unsigned u[4] = {0u, 0u, 0u, 0u};
         uint2 wv = make_uint2(0u, 0u);
+        float fy = 0.0f, fz = 0.0f;        // the face element's y and z after the Mur apply
         {
             const float4 iz_ym = *reinterpret_cast<const float4*>(I0 + 2 * cs + p - ym);
             const float4 ix_ym = *reinterpret_cast<const float4*>(I0 + p - ym);
@@ -1817,12 +1867,23 @@ __global__ __launch_bounds__(kBlock) void k_step4t(const Step4Args ag, const flo
             quad_coef_n<CC>(a.kV, cs, p, u, 2, c, w);
             v[2] = quad_math(v0[2], c, w, quad_curl(iy, iy_xm, ix, ix_ym));
             quad_sources_at(a.eV, cv, x0, z, 7u, ts, v);
+            if (flo) mur_fold_quad<0, 1>(a.mf[0], z * a.ny + y, mcp, mcpp, v0, v, active && z < z1, fy, fz);
+            else if (fhi) mur_fold_quad<3, 2>(a.mf[1], z * a.ny + y, mcp, mcpp, v0, v, active && z < z1, fy, fz);
         }
         if (active && z < z1) {
 #pragma unroll
             for (int n = 0; n < 3; ++n) st4<NT>(V1 + n * cs + p, v[n]);
         }
-        s_vz[b][threadIdx.x] = v[2];
+        // V^{n+1} as the currents read it: the face element after the Mur apply
+        float4 vq1 = v[1], vq2 = v[2];
+        if (flo) {
+            vq1.x = fy;
+            vq2.x = fz;
+        } else if (fhi) {
+            vq1.w = fy;
+            vq2.w = fz;
+        }
+        s_vz[b][threadIdx.x] = vq2;
         s_vx[b][threadIdx.x] = v[0];
         // element 0 of the next lane's quad at plane z-1 (cell x0+4)
         const float nr2 = __shfl_down(vm[2].x, 1u, 64);
@@ -1851,7 +1912,7 @@ __global__ __launch_bounds__(kBlock) void k_step4t(const Step4Args ag, const flo
             const float4 vyr = make_float4(vm[1].y, vm[1].z, vm[1].w, r1);
             float4 c, w;
             quad_coef_n<CC>(a.kI, cs, pm, um, 0, c, w);
-            io[0] = quad_math(ixp, c, w, quad_curl(vm[2], u2, vm[1], v[1]));
+            io[0] = quad_math(ixp, c, w, quad_curl(vm[2], u2, vm[1], vq1));
             quad_coef_n<CC>(a.kI, cs, pm, um, 1, c, w);
             io[1] = quad_math(iyp, c, w, quad_curl(vm[0], v[0], vm[2], vzr));
             quad_coef_n<CC>(a.kI, cs, pm, um, 2, c, w);
@@ -1869,8 +1930,8 @@ __global__ __launch_bounds__(kBlock) void k_step4t(const Step4Args ag, const flo
             }
         }
         vm[0] = v[0];
-        vm[1] = v[1];
-        vm[2] = v[2];
+        vm[1] = vq1;
+        vm[2] = vq2;
         ixp = ix;
         iyp = iy;
         izp = iz;
@@ -1897,6 +1958,12 @@ __global__ __launch_bounds__(kBlock) void k_step4t(const Step4Args ag, const flo
             nu[0] = quad_math(u0x, c, w, quad_curl(izu, iz, iyu, iyu_m));
             nu[1] = make_float4(0.f, 0.f, 0.f, 0.f);
             quad_sources_at(a.eV, cu, x0, z, 5u, ts, nu);
+            if (flo) {
+                // the Mur value of row y+1's face cell (z component)
+                float mz1 = u0z.y - mcpp1 * u0z.x;
+                mz1 = mz1 + mcpp1 * nu[2].y;
+                nu[2].x = mz1;
+            }
             vu2m = nu[2];
             vu0m = nu[0];
         }
@@ -2931,6 +2998,7 @@ struct ems_engine {
     unsigned ntx = 0, nty = 0, fzdepth = 0, fnzc = 0;
     bool q4tile = false;                  // form 3 runs k_step4t (64 x 4 quad tiles)
     unsigned q4ntq = 0;                   // its tiles along x
+    int mur_fold[2] = {-1, -1};           // Mur faces (index) on x = 0 / x = Nx-1 whose pre and post hooks k_step4t runs
     ExcArgs eV{}, eI{};
     int pml_mode = kNoPml;
     unsigned n_unique_pml = 0;
@@ -3742,11 +3810,27 @@ static int setup_fused(ems_engine* h)
     };
     h->bands.clear();
     h->ncells = 0;
+    h->mur_fold[0] = h->mur_fold[1] = -1;
     std::vector<unsigned long long> cells;
     if (why.empty() && form == 3) {
+        // k_step4t on rows of at least 64 quads (EMS_Q4TILE forces it on or off)
+        h->q4tile = Nx / 4 >= 64;
+        if (const char* te = std::getenv("EMS_Q4TILE")) h->q4tile = std::atoi(te) != 0;
+        // it runs the pre and post hooks of the Mur faces on x = 0 and
+        // x = Nx-1 and updates the currents next to them itself
+        bool fold = h->q4tile && Nx >= 8;
+        if (const char* mf = std::getenv("EMS_MUR_FOLD")) fold = fold && std::atoi(mf) != 0;
+        for (size_t k = 0; k < h->mur_geo.size() && fold; ++k) {
+            const auto& f = h->mur_geo[k];
+            if (f.ny != 0) continue;
+            if (f.line == 0 && f.shift == 1 && h->mur_fold[0] < 0) h->mur_fold[0] = (int)k;
+            else if (f.line == Nx - 1 && f.shift == Nx - 2 && h->mur_fold[1] < 0) h->mur_fold[1] = (int)k;
+        }
         // the Mur apply sets the voltages of the face line only; the current
         // at plane k reads the voltages of planes k and k+1 along the normal
-        for (const auto& f : h->mur_geo) {
+        for (size_t k = 0; k < h->mur_geo.size(); ++k) {
+            const auto& f = h->mur_geo[k];
+            if ((int)k == h->mur_fold[0] || (int)k == h->mur_fold[1]) continue;
             const unsigned lo = f.line ? f.line - 1 : 0, hi = std::min(f.line, h->N[f.ny] - 2);
             if (lo <= hi) h->bands.push_back(ems_engine::Band{f.ny, lo, hi});
         }
@@ -3858,8 +3942,6 @@ static int setup_fused(ems_engine* h)
         // at 16, 32, 64), shallower when the grid would give fewer than 2048
         // workgroups
         const unsigned NQ = Nx / 4;
-        h->q4tile = NQ >= 64;
-        if (const char* te = std::getenv("EMS_Q4TILE")) h->q4tile = std::atoi(te) != 0;
         h->q4ntq = (NQ + 63) / 64;
         h->ntx = h->q4tile ? h->q4ntq * ((Ny + kQTileRows - 1) / kQTileRows) : (h->g.plane / 4 + kBlock - 1) / kBlock;
         unsigned d = h->q4tile ? 64 : 8;
@@ -4515,6 +4597,11 @@ void quad_step(ems_engine* h)
     a.zdepth = h->fzdepth;
     a.ntiles = h->ntx;
     a.ntq = h->q4ntq;
+    for (int side = 0; side < 2; ++side) {
+        if (h->mur_fold[side] < 0) continue;
+        const MurDev& m = h->mur[h->mur_fold[side]];
+        a.mf[side] = MurFold{m.cP, m.cPP, m.mP, m.mPP, m.start_ts, 1u};
+    }
     const unsigned nblk = h->ntx * h->fnzc;
     if (h->cmp) launch_step4_t<true>(h, a, nblk);
     else launch_step4_t<false>(h, a, nblk);
@@ -4526,8 +4613,11 @@ void fused_step(ems_engine* h, hipEvent_t* ev)
 {
     const unsigned T = 256;
     // Mur pre hooks read state n (FDTD/engine.cpp:224-265 order; only the quad sweep has Mur)
-    for (const auto& m : h->mur)
+    for (size_t k = 0; k < h->mur.size(); ++k) {
+        if ((int)k == h->mur_fold[0] || (int)k == h->mur_fold[1]) continue;     // in k_step4t
+        const auto& m = h->mur[k];
         hipLaunchKernelGGL(k_mur_pre, dim3((m.na * m.nb + T - 1) / T), dim3(T), 0, h->stream, h->V, m, h->numts_dev);
+    }
     if (ev) (void)hipEventRecord(ev[0], h->stream);
     if (h->fused == 3) quad_step(h);
     else if (h->fused == 2) column_step(h);
@@ -4541,6 +4631,7 @@ void fused_step(ems_engine* h, hipEvent_t* ev)
         // Mur apply, the deferred voltage sources, then the current update of
         // the planes next to each face again, with their current sources
         for (int k = (int)h->mur.size() - 1; k >= 0; --k) {
+            if (k == h->mur_fold[0] || k == h->mur_fold[1]) continue;          // in k_step4t
             const auto& m = h->mur[k];
             hipLaunchKernelGGL(k_mur_post, dim3((m.na * m.nb + T - 1) / T), dim3(T), 0, h->stream, h->V_alt, m,
                                h->numts_dev);
@@ -4624,8 +4715,11 @@ static int launch_step(ems_engine* h, hipEvent_t* ev)
     const unsigned nblk = g.qtiles * g.nzc;
     const unsigned T = 256;
     // -- voltage pre hooks: Mur (creation order), Lorentz, lumped RLC; UPML pre is fused
-    for (const auto& m : h->mur)
+    for (size_t k = 0; k < h->mur.size(); ++k) {
+        if ((int)k == h->mur_fold[0] || (int)k == h->mur_fold[1]) continue;     // in k_step4t
+        const auto& m = h->mur[k];
         hipLaunchKernelGGL(k_mur_pre, dim3((m.na * m.nb + T - 1) / T), dim3(T), 0, h->stream, h->V, m, h->numts_dev);
+    }
     for (size_t k = 0; k < h->lor.size(); k += 2)
         if (h->lor_v[k / 2])
             hipLaunchKernelGGL(k_lor_pre, dim3((h->lor[k].count + T - 1) / T), dim3(T), 0, h->stream, h->V, h->cs, h->lor[k]);
