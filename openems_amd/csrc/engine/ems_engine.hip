@@ -3776,6 +3776,13 @@ static int setup_fused(ems_engine* h)
     if (autom) {
         bool sweep = !h->pmlV.n && h->quad_req != 0 && h->N[0] % 4 == 0 && h->N[0] / 4 >= 64;
         if (const char* as = std::getenv("EMS_AUTO_SWEEP")) sweep = sweep && std::atoi(as) != 0;
+        if (sweep) {
+            // the sweep keeps a second copy of both fields: only where it fits
+            // with 1 GiB to spare
+            size_t freeb = 0, totalb = 0;
+            const size_t need = 6 * h->cs * sizeof(float) + (1ull << 30);
+            if (hipMemGetInfo(&freeb, &totalb) != hipSuccess || freeb < need) sweep = false;
+        }
         if (!sweep) return 0;
         form = 3;
     }
