@@ -152,8 +152,13 @@ int ems_steady_state(ems_engine* h, double* last_max_diff, unsigned* evaluations
  * as do current sources on the last line of a direction and more than 64
  * sources in one tile (form 1) or column (form 2).
  * Form 3 sweeps columns of four x cells per lane (16-byte accesses) like
- * form 2, on meshes without UPML whose x lines are a multiple of 4.
- * mode: -1 automatic (two half steps, the faster form on UPML meshes),
+ * form 2, on meshes without UPML whose x lines are a multiple of 4, Mur
+ * faces included; on rows of at least 64 quads it runs in 64 x 4 quad tiles
+ * that take neighbour voltages from LDS and adjacent lanes, and runs the
+ * pre and post hooks of Mur faces on x = 0 and x = Nx-1 itself.
+ * mode: -1 automatic (form 3 on meshes without UPML whose rows hold at
+ * least 64 quads, where every active extension has a sweep form and the
+ * second field copy fits in device memory; two half steps otherwise),
  * 0 two half steps, 1 LDS sweep, 2 column sweep, 3 quad column sweep
  * (ems_init then fails with EMS_E_ARG when the engine has no such form). */
 int ems_set_fused(ems_engine* h, int mode);
