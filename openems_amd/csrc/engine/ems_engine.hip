@@ -541,6 +541,75 @@ __device__ __forceinline__ void f4s(float4& v, int j, float a)
     else v.w = a;
 }
 
+// Excitation entry as the fused step applies it: cell, component, amplitude
+// and delay of one Operator_Ext_Excitation list element.
+struct ExcEnt {
+    unsigned short x, y, z, dir;
+    float amp;
+    unsigned delay;
+};
+
+// sample of the excitation signal at timestep ts for a delay
+// (Engine_Ext_Excitation::Apply2Voltages, engine_ext_excitation.cpp:33-61)
+__device__ __forceinline__ int exc_sample(int ts, unsigned delay, int period, unsigned length)
+{
+    const int p = period > 0 ? period : ts + 1;
+    int e = ts - (int)delay;
+    e *= (e > 0);
+    e %= p;
+    e *= (e < (int)length);
+    return e;
+}
+
+struct ExcArgs {
+    const unsigned* beg;           // [ntiles + 1] entry ranges per xy tile, each sorted by z
+    const ExcEnt* ent;
+    const float* sig;
+    unsigned length;
+    int period;
+};
+
+// A quad's source entries ordered by plane (list order within a plane):
+// [k, e) are the entries not yet applied, nz the plane of entry k (~0u when
+// none is left); a sweep that visits planes in increasing order applies each
+// plane's entries once and otherwise only compares nz.
+struct SrcCursor {
+    unsigned k, e, nz;
+};
+
+__device__ __forceinline__ SrcCursor src_cursor(const ExcArgs& ex, unsigned b, unsigned e, unsigned z0)
+{
+    SrcCursor c{b, e, ~0u};
+    while (c.k < e && ex.ent[c.k].z < z0) ++c.k;
+    if (c.k < e) c.nz = ex.ent[c.k].z;
+    return c;
+}
+
+// the entries of plane z added to components `mask` of f, in list order
+__device__ __forceinline__ void quad_sources_at(const ExcArgs& ex, SrcCursor& c, unsigned x0, unsigned z,
+                                                unsigned mask, int ts, float4 f[3])
+{
+    if (c.nz != z) return;
+    unsigned k = c.k;
+    for (; k < c.e; ++k) {
+        const ExcEnt E = ex.ent[k];
+        if (E.z != z) break;
+        if (!(mask >> E.dir & 1u)) continue;
+        const float sv = E.amp * ex.sig[exc_sample(ts, E.delay, ex.period, ex.length)];
+        const unsigned j = (unsigned)E.x - x0;
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            if (E.dir != n) continue;
+            if (j == 0) f[n].x = f[n].x + sv;
+            else if (j == 1) f[n].y = f[n].y + sv;
+            else if (j == 2) f[n].z = f[n].z + sv;
+            else f[n].w = f[n].w + sv;
+        }
+    }
+    c.k = k;
+    c.nz = k < c.e ? ex.ent[k].z : ~0u;
+}
+
 // Boxes of quads a quad half-step launch covers: quads qa .. qa+nq-1 of rows
 // ya .. ya+ny-1, planes za .. zb-1, in tiles of kBlock quads (ntiles per z
 // chunk of zd planes); `first` is the box's first workgroup.  One box is the whole mesh; a
@@ -681,7 +750,8 @@ __device__ __forceinline__ void quad_coef(const CoefArgs& k, unsigned long long 
 
 template <int PM, bool CC, bool NT, bool LT>
 __global__ __launch_bounds__(kBlock) void k_volt4(const GridDesc g, const QBoxes boxes, float* V,
-                                                  const float* __restrict__ I, const CoefArgs kg, const PmlArgs pml)
+                                                  const float* __restrict__ I, const CoefArgs kg, const PmlArgs pml,
+                                                  const ExcArgs ex, const unsigned* __restrict__ numts)
 {
     extern __shared__ float4 s_tab[];
     CoefArgs k = kg;
@@ -701,6 +771,15 @@ __global__ __launch_bounds__(kBlock) void k_volt4(const GridDesc g, const QBoxes
     const float* __restrict__ Iz = I + 2 * cs;
     const unsigned long long pz = z0 ? p - g.plane : p;
     float4 Ix_zm = *reinterpret_cast<const float4*>(Ix + pz), Iy_zm = *reinterpret_cast<const float4*>(Iy + pz);
+    // voltage sources added here (per-quad lists ordered by plane) where no
+    // hook sits between the update and the excitation; else by k_excite
+    SrcCursor cv{0u, 0u, ~0u};
+    int ts = 0;
+    if (ex.beg) {
+        const unsigned qi = x0 / 4 + (g.Nx / 4) * y;
+        cv = src_cursor(ex, ex.beg[qi], ex.beg[qi + 1], z0);
+        ts = (int)numts[0];
+    }
     for (unsigned z = z0; z < z1; ++z, p += g.plane) {
         const float4 ix = *reinterpret_cast<const float4*>(Ix + p);
         const float4 iy = *reinterpret_cast<const float4*>(Iy + p);
@@ -743,6 +822,7 @@ __global__ __launch_bounds__(kBlock) void k_volt4(const GridDesc g, const QBoxes
             out[n].w = xv[n].w * a[n].w; out[n].w = out[n].w + b[n].w * curl[n].w;
         }
         if (PM != kNoPml) quad_pml<PM, true>(pml, colbits, (int)x0, (int)y, (int)z, xv, a, b, curl, 15u, out);
+        quad_sources_at(ex, cv, x0, z, 7u, ts, out);
 #pragma unroll
         for (int n = 0; n < 3; ++n) st4<NT>(V + n * cs + p, out[n]);
     }
@@ -839,34 +919,6 @@ __global__ __launch_bounds__(kBlock) void k_curr4(const GridDesc g, const QBoxes
         }
     }
 }
-
-// Excitation entry as the fused step applies it: cell, component, amplitude
-// and delay of one Operator_Ext_Excitation list element.
-struct ExcEnt {
-    unsigned short x, y, z, dir;
-    float amp;
-    unsigned delay;
-};
-
-// sample of the excitation signal at timestep ts for a delay
-// (Engine_Ext_Excitation::Apply2Voltages, engine_ext_excitation.cpp:33-61)
-__device__ __forceinline__ int exc_sample(int ts, unsigned delay, int period, unsigned length)
-{
-    const int p = period > 0 ? period : ts + 1;
-    int e = ts - (int)delay;
-    e *= (e > 0);
-    e %= p;
-    e *= (e < (int)length);
-    return e;
-}
-
-struct ExcArgs {
-    const unsigned* beg;           // [ntiles + 1] entry ranges per xy tile, each sorted by z
-    const ExcEnt* ent;
-    const float* sig;
-    unsigned length;
-    int period;
-};
 
 // ---------------------------------------------------------------------------
 // One full timestep in one z-sweep (voltage half step, voltage excitation,
@@ -1187,47 +1239,6 @@ __device__ __forceinline__ float cell_sources(const ExcArgs& e, unsigned b, unsi
         if (E.x == x && E.z == z && E.dir == n) v = v + E.amp * e.sig[exc_sample(ts, E.delay, e.period, e.length)];
     }
     return v;
-}
-
-// A quad's source entries ordered by plane (list order within a plane):
-// [k, e) are the entries not yet applied, nz the plane of entry k (~0u when
-// none is left); a sweep that visits planes in increasing order applies each
-// plane's entries once and otherwise only compares nz.
-struct SrcCursor {
-    unsigned k, e, nz;
-};
-
-__device__ __forceinline__ SrcCursor src_cursor(const ExcArgs& ex, unsigned b, unsigned e, unsigned z0)
-{
-    SrcCursor c{b, e, ~0u};
-    while (c.k < e && ex.ent[c.k].z < z0) ++c.k;
-    if (c.k < e) c.nz = ex.ent[c.k].z;
-    return c;
-}
-
-// the entries of plane z added to components `mask` of f, in list order
-__device__ __forceinline__ void quad_sources_at(const ExcArgs& ex, SrcCursor& c, unsigned x0, unsigned z,
-                                                unsigned mask, int ts, float4 f[3])
-{
-    if (c.nz != z) return;
-    unsigned k = c.k;
-    for (; k < c.e; ++k) {
-        const ExcEnt E = ex.ent[k];
-        if (E.z != z) break;
-        if (!(mask >> E.dir & 1u)) continue;
-        const float sv = E.amp * ex.sig[exc_sample(ts, E.delay, ex.period, ex.length)];
-        const unsigned j = (unsigned)E.x - x0;
-#pragma unroll
-        for (int n = 0; n < 3; ++n) {
-            if (E.dir != n) continue;
-            if (j == 0) f[n].x = f[n].x + sv;
-            else if (j == 1) f[n].y = f[n].y + sv;
-            else if (j == 2) f[n].z = f[n].z + sv;
-            else f[n].w = f[n].w + sv;
-        }
-    }
-    c.k = k;
-    c.nz = k < c.e ? ex.ent[k].z : ~0u;
 }
 
 template <int PM, bool CC>
@@ -2994,6 +3005,8 @@ struct ems_engine {
     QBoxes qbox_all{}, qbox_in{}, qbox_out{};
     unsigned qblk_all = 0, qblk_in = 0, qblk_out = 0;
     QBox* qbox_dev = nullptr;             // the three box lists, kMaxQBoxes each
+    // voltage sources added by k_volt4 (per-quad lists; beg null: k_excite adds them)
+    ExcArgs qsrc{};
     bool qsplit = false;
     unsigned ntx = 0, nty = 0, fzdepth = 0, fnzc = 0;
     bool q4tile = false;                  // form 3 runs k_step4t (64 x 4 quad tiles)
@@ -4155,6 +4168,31 @@ static int setup_quad(ems_engine* h)
     // automatic mode: quads on meshes without UPML
     if (h->quad_req == -1 && h->pmlV.n) return 0;
     h->quad = true;
+    // voltage sources inside k_volt4 when no hook runs between the voltage
+    // update and the excitation (Mur post and apply, lumped RLC, Lorentz,
+    // the steady-state record): one launch less per step
+    bool qsrc = h->exc[0].G && h->exc[0].host.size() == h->exc[0].K && h->mur.empty() && h->lor.empty() &&
+                h->rlc.empty() && !h->has_ss && h->N[0] < 65535 && h->N[1] < 65535 && h->N[2] < 65535;
+    if (const char* qs = std::getenv("EMS_QSRC")) qsrc = qsrc && std::atoi(qs) != 0;
+    if (qsrc) {
+        const unsigned nq = (unsigned)(h->g.plane / 4);
+        std::vector<std::vector<ExcEnt>> lists(nq);
+        for (const auto& e : h->exc[0].host) lists[e.x / 4 + NQ * (unsigned)e.y].push_back(e);
+        std::vector<unsigned> beg(nq + 1, 0);
+        std::vector<ExcEnt> ent;
+        for (unsigned i = 0; i < nq; ++i) {
+            // ordered by plane, list order kept within a plane
+            std::stable_sort(lists[i].begin(), lists[i].end(),
+                             [](const ExcEnt& l, const ExcEnt& r) { return l.z < r.z; });
+            ent.insert(ent.end(), lists[i].begin(), lists[i].end());
+            beg[i + 1] = (unsigned)ent.size();
+        }
+        unsigned* db;
+        ExcEnt* de;
+        int rc;
+        if ((rc = upload(h, &db, beg.data(), beg.size())) || (rc = upload(h, &de, ent.data(), ent.size()))) return rc;
+        h->qsrc = ExcArgs{db, de, h->sig[0], h->sigLen, h->period};
+    }
     return 0;
 }
 
@@ -4450,10 +4488,10 @@ void launch_quad_box(ems_engine* h, bool curr, const QBoxes& B, unsigned nblk, u
     } else {
         if (h->nt_fields)
             hipLaunchKernelGGL((k_volt4<PM, CC, true, LT>), dim3(nblk), dim3(kBlock), lds, h->stream, h->gq, B, h->V,
-                               h->I, h->kV, h->pmlV);
+                               h->I, h->kV, h->pmlV, h->qsrc, h->numts_dev);
         else
             hipLaunchKernelGGL((k_volt4<PM, CC, false, LT>), dim3(nblk), dim3(kBlock), lds, h->stream, h->gq, B, h->V,
-                               h->I, h->kV, h->pmlV);
+                               h->I, h->kV, h->pmlV, h->qsrc, h->numts_dev);
     }
 }
 
@@ -4761,7 +4799,7 @@ static int launch_step(ems_engine* h, hipEvent_t* ev)
         const auto& m = h->mur[k];
         hipLaunchKernelGGL(k_mur_apply, dim3((m.na * m.nb + T - 1) / T), dim3(T), 0, h->stream, h->V, m, h->numts_dev);
     }
-    if (h->exc[0].G) {
+    if (h->exc[0].G && !(h->quad && h->qsrc.beg)) {
         const auto& e = h->exc[0];
         hipLaunchKernelGGL(k_excite, dim3((e.G + T - 1) / T), dim3(T), 0, h->stream, h->V, e.G, e.gbeg, e.glin, e.amp,
                            e.delay, h->sig[0], h->sigLen, h->period, h->numts_dev, 0);
