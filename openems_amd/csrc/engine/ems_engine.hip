@@ -356,6 +356,93 @@ __device__ __forceinline__ void st_own(float* q, float v)
     else *q = v;
 }
 
+// Excitation entry as the fused step applies it: cell, component, amplitude
+// and delay of one Operator_Ext_Excitation list element.
+struct ExcEnt {
+    unsigned short x, y, z, dir;
+    float amp;
+    unsigned delay;
+};
+
+// sample of the excitation signal at timestep ts for a delay
+// (Engine_Ext_Excitation::Apply2Voltages, engine_ext_excitation.cpp:33-61)
+__device__ __forceinline__ int exc_sample(int ts, unsigned delay, int period, unsigned length)
+{
+    const int p = period > 0 ? period : ts + 1;
+    int e = ts - (int)delay;
+    e *= (e > 0);
+    e %= p;
+    e *= (e < (int)length);
+    return e;
+}
+
+struct ExcArgs {
+    const unsigned* beg;           // [ntiles + 1] entry ranges per xy tile, each sorted by z
+    const ExcEnt* ent;
+    const float* sig;
+    unsigned length;
+    int period;
+    const unsigned* tmask;         // k_volt: bit per tile of kBlock columns holding a source (null: every lane looks)
+};
+
+// A quad's source entries ordered by plane (list order within a plane):
+// [k, e) are the entries not yet applied, nz the plane of entry k (~0u when
+// none is left); a sweep that visits planes in increasing order applies each
+// plane's entries once and otherwise only compares nz.
+struct SrcCursor {
+    unsigned k, e, nz;
+};
+
+__device__ __forceinline__ SrcCursor src_cursor(const ExcArgs& ex, unsigned b, unsigned e, unsigned z0)
+{
+    SrcCursor c{b, e, ~0u};
+    while (c.k < e && ex.ent[c.k].z < z0) ++c.k;
+    if (c.k < e) c.nz = ex.ent[c.k].z;
+    return c;
+}
+
+// the entries of plane z added to components `mask` of f, in list order
+__device__ __forceinline__ void quad_sources_at(const ExcArgs& ex, SrcCursor& c, unsigned x0, unsigned z,
+                                                unsigned mask, int ts, float4 f[3])
+{
+    if (c.nz != z) return;
+    unsigned k = c.k;
+    for (; k < c.e; ++k) {
+        const ExcEnt E = ex.ent[k];
+        if (E.z != z) break;
+        if (!(mask >> E.dir & 1u)) continue;
+        const float sv = E.amp * ex.sig[exc_sample(ts, E.delay, ex.period, ex.length)];
+        const unsigned j = (unsigned)E.x - x0;
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            if (E.dir != n) continue;
+            if (j == 0) f[n].x = f[n].x + sv;
+            else if (j == 1) f[n].y = f[n].y + sv;
+            else if (j == 2) f[n].z = f[n].z + sv;
+            else f[n].w = f[n].w + sv;
+        }
+    }
+    c.k = k;
+    c.nz = k < c.e ? ex.ent[k].z : ~0u;
+}
+
+// the same for one cell's column (f: its three components)
+__device__ __forceinline__ void col_sources_at(const ExcArgs& ex, SrcCursor& c, unsigned z, int ts, float f[3])
+{
+    if (c.nz != z) return;
+    unsigned k = c.k;
+    for (; k < c.e; ++k) {
+        const ExcEnt E = ex.ent[k];
+        if (E.z != z) break;
+        const float sv = E.amp * ex.sig[exc_sample(ts, E.delay, ex.period, ex.length)];
+        if (E.dir == 0) f[0] = f[0] + sv;
+        else if (E.dir == 1) f[1] = f[1] + sv;
+        else f[2] = f[2] + sv;
+    }
+    c.k = k;
+    c.nz = k < c.e ? ex.ent[k].z : ~0u;
+}
+
 // ---------------------------------------------------------------------------
 // voltage half step (Engine::UpdateVoltages, FDTD/engine.cpp:110-168) with the
 // UPML pre/post hooks of Engine_Ext_UPML (engine_ext_upml.cpp:52-124) fused in:
@@ -368,7 +455,8 @@ __device__ __forceinline__ void st_own(float* q, float v)
 // ---------------------------------------------------------------------------
 template <int PM, bool CC, bool NT>
 __device__ __forceinline__ void volt_tile(const GridDesc& g, unsigned t, float* V, const float* __restrict__ I,
-                                          const CoefArgs& k, const PmlArgs& pml)
+                                          const CoefArgs& k, const PmlArgs& pml, const ExcArgs& ex,
+                                          const unsigned* __restrict__ numts)
 {
     const unsigned qt = t % g.qtiles, zc = t / g.qtiles;
     unsigned x, y;
@@ -387,6 +475,13 @@ __device__ __forceinline__ void volt_tile(const GridDesc& g, unsigned t, float* 
     float Ix_zm = Ix[pz], Iy_zm = Iy[pz];
     // UPML slabs that hold this column (the per-plane test is then along z only)
     const unsigned pcm = PM != kNoPml ? pml_colmask(pml, (int)x, (int)y) : 0u;
+    // the column's voltage sources (ordered by plane) where this kernel adds them
+    SrcCursor cv{0u, 0u, ~0u};
+    int ts = 0;
+    if (ex.beg && (!ex.tmask || (ex.tmask[qt >> 5] >> (qt & 31u) & 1u))) {
+        cv = src_cursor(ex, ex.beg[x + g.Nx * y], ex.beg[x + g.Nx * y + 1], z0);
+        ts = (int)numts[0];
+    }
 
     for (unsigned z = z0; z < z1; ++z, p += g.plane) {
         const float ix = Ix[p], iy = Iy[p], iz = Iz[p];
@@ -405,6 +500,7 @@ __device__ __forceinline__ void volt_tile(const GridDesc& g, unsigned t, float* 
         unsigned li = 0;
         const bool in_pml = PM != kNoPml && pcm && pml_at_col(pml, pcm, (int)x, (int)y, (int)z, li);
         cell_math<PM>(x0, a, b, curl, in_pml, pml, li, true, out);
+        col_sources_at(ex, cv, z, ts, out);
 #pragma unroll
         for (int n = 0; n < 3; ++n) st_own<NT>(&V[n * cs + p], out[n]);
     }
@@ -412,9 +508,10 @@ __device__ __forceinline__ void volt_tile(const GridDesc& g, unsigned t, float* 
 
 template <int PM, bool CC, bool NT>
 __global__ __launch_bounds__(kBlock) void k_volt(const GridDesc g, float* V, const float* __restrict__ I,
-                                                 const CoefArgs k, const PmlArgs pml)
+                                                 const CoefArgs k, const PmlArgs pml, const ExcArgs ex,
+                                                 const unsigned* __restrict__ numts)
 {
-    volt_tile<PM, CC, NT>(g, xcd_tile(blockIdx.x, gridDim.x), V, I, k, pml);
+    volt_tile<PM, CC, NT>(g, xcd_tile(blockIdx.x, gridDim.x), V, I, k, pml, ex, numts);
 }
 
 // ---------------------------------------------------------------------------
@@ -539,75 +636,6 @@ __device__ __forceinline__ void f4s(float4& v, int j, float a)
     else if (j == 1) v.y = a;
     else if (j == 2) v.z = a;
     else v.w = a;
-}
-
-// Excitation entry as the fused step applies it: cell, component, amplitude
-// and delay of one Operator_Ext_Excitation list element.
-struct ExcEnt {
-    unsigned short x, y, z, dir;
-    float amp;
-    unsigned delay;
-};
-
-// sample of the excitation signal at timestep ts for a delay
-// (Engine_Ext_Excitation::Apply2Voltages, engine_ext_excitation.cpp:33-61)
-__device__ __forceinline__ int exc_sample(int ts, unsigned delay, int period, unsigned length)
-{
-    const int p = period > 0 ? period : ts + 1;
-    int e = ts - (int)delay;
-    e *= (e > 0);
-    e %= p;
-    e *= (e < (int)length);
-    return e;
-}
-
-struct ExcArgs {
-    const unsigned* beg;           // [ntiles + 1] entry ranges per xy tile, each sorted by z
-    const ExcEnt* ent;
-    const float* sig;
-    unsigned length;
-    int period;
-};
-
-// A quad's source entries ordered by plane (list order within a plane):
-// [k, e) are the entries not yet applied, nz the plane of entry k (~0u when
-// none is left); a sweep that visits planes in increasing order applies each
-// plane's entries once and otherwise only compares nz.
-struct SrcCursor {
-    unsigned k, e, nz;
-};
-
-__device__ __forceinline__ SrcCursor src_cursor(const ExcArgs& ex, unsigned b, unsigned e, unsigned z0)
-{
-    SrcCursor c{b, e, ~0u};
-    while (c.k < e && ex.ent[c.k].z < z0) ++c.k;
-    if (c.k < e) c.nz = ex.ent[c.k].z;
-    return c;
-}
-
-// the entries of plane z added to components `mask` of f, in list order
-__device__ __forceinline__ void quad_sources_at(const ExcArgs& ex, SrcCursor& c, unsigned x0, unsigned z,
-                                                unsigned mask, int ts, float4 f[3])
-{
-    if (c.nz != z) return;
-    unsigned k = c.k;
-    for (; k < c.e; ++k) {
-        const ExcEnt E = ex.ent[k];
-        if (E.z != z) break;
-        if (!(mask >> E.dir & 1u)) continue;
-        const float sv = E.amp * ex.sig[exc_sample(ts, E.delay, ex.period, ex.length)];
-        const unsigned j = (unsigned)E.x - x0;
-#pragma unroll
-        for (int n = 0; n < 3; ++n) {
-            if (E.dir != n) continue;
-            if (j == 0) f[n].x = f[n].x + sv;
-            else if (j == 1) f[n].y = f[n].y + sv;
-            else if (j == 2) f[n].z = f[n].z + sv;
-            else f[n].w = f[n].w + sv;
-        }
-    }
-    c.k = k;
-    c.nz = k < c.e ? ex.ent[k].z : ~0u;
 }
 
 // Boxes of quads a quad half-step launch covers: quads qa .. qa+nq-1 of rows
@@ -2332,6 +2360,7 @@ struct BatchInst {
     const float* sig[2];
     unsigned sigLen;
     int period;
+    ExcArgs vsrc;                  // voltage sources added by the voltage half step (beg null: by k_batch_excite)
 };
 
 template <int PM, bool CC, bool CURR>
@@ -2345,7 +2374,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_half(const BatchInst* __restri
             if (it.y == 0 && threadIdx.x == 0) atomicAdd(B.numts, 1u);
             curr_tile<PM, CC, false>(B.g, it.y, B.I, B.V, B.kI, B.pI);
         } else {
-            volt_tile<PM, CC, false>(B.g, it.y, B.V, B.I, B.kV, B.pV);
+            volt_tile<PM, CC, false>(B.g, it.y, B.V, B.I, B.kV, B.pV, B.vsrc, B.numts);
         }
     }
 }
@@ -3005,7 +3034,8 @@ struct ems_engine {
     QBoxes qbox_all{}, qbox_in{}, qbox_out{};
     unsigned qblk_all = 0, qblk_in = 0, qblk_out = 0;
     QBox* qbox_dev = nullptr;             // the three box lists, kMaxQBoxes each
-    // voltage sources added by k_volt4 (per-quad lists; beg null: k_excite adds them)
+    // voltage sources added by k_volt4 / k_volt (per-quad or per-column lists;
+    // beg null: k_excite adds them)
     ExcArgs qsrc{};
     bool qsplit = false;
     unsigned ntx = 0, nty = 0, fzdepth = 0, fnzc = 0;
@@ -4168,31 +4198,51 @@ static int setup_quad(ems_engine* h)
     // automatic mode: quads on meshes without UPML
     if (h->quad_req == -1 && h->pmlV.n) return 0;
     h->quad = true;
-    // voltage sources inside k_volt4 when no hook runs between the voltage
-    // update and the excitation (Mur post and apply, lumped RLC, Lorentz,
-    // the steady-state record): one launch less per step
-    bool qsrc = h->exc[0].G && h->exc[0].host.size() == h->exc[0].K && h->mur.empty() && h->lor.empty() &&
-                h->rlc.empty() && !h->has_ss && h->N[0] < 65535 && h->N[1] < 65535 && h->N[2] < 65535;
-    if (const char* qs = std::getenv("EMS_QSRC")) qsrc = qsrc && std::atoi(qs) != 0;
-    if (qsrc) {
-        const unsigned nq = (unsigned)(h->g.plane / 4);
-        std::vector<std::vector<ExcEnt>> lists(nq);
-        for (const auto& e : h->exc[0].host) lists[e.x / 4 + NQ * (unsigned)e.y].push_back(e);
-        std::vector<unsigned> beg(nq + 1, 0);
-        std::vector<ExcEnt> ent;
-        for (unsigned i = 0; i < nq; ++i) {
-            // ordered by plane, list order kept within a plane
-            std::stable_sort(lists[i].begin(), lists[i].end(),
-                             [](const ExcEnt& l, const ExcEnt& r) { return l.z < r.z; });
-            ent.insert(ent.end(), lists[i].begin(), lists[i].end());
-            beg[i + 1] = (unsigned)ent.size();
-        }
-        unsigned* db;
-        ExcEnt* de;
-        int rc;
-        if ((rc = upload(h, &db, beg.data(), beg.size())) || (rc = upload(h, &de, ent.data(), ent.size()))) return rc;
-        h->qsrc = ExcArgs{db, de, h->sig[0], h->sigLen, h->period};
+    return 0;
+}
+
+// Voltage sources added by the half-step voltage kernel itself (k_volt4 per
+// quad, k_volt per column) when no hook runs between the voltage update and
+// the excitation (Mur post and apply, lumped RLC, Lorentz, the steady-state
+// record): one launch less per step.  Lists per quad or column, ordered by
+// plane, list order kept within a plane.
+static int setup_vsrc(ems_engine* h)
+{
+    h->qsrc = ExcArgs{};
+    bool on = !h->fused && h->exc[0].G && h->exc[0].host.size() == h->exc[0].K && h->mur.empty() &&
+              h->lor.empty() && h->rlc.empty() && !h->has_ss && h->N[0] < 65535 && h->N[1] < 65535 &&
+              h->N[2] < 65535;
+    if (const char* qs = std::getenv("EMS_QSRC")) on = on && std::atoi(qs) != 0;
+    if (!on) return 0;
+    const unsigned w = h->quad ? 4u : 1u;            // cells per list along x
+    const unsigned nx = h->N[0] / w;
+    const unsigned nl = (unsigned)(h->g.plane / w);
+    std::vector<std::vector<ExcEnt>> lists(nl);
+    for (const auto& e : h->exc[0].host) lists[e.x / w + nx * (unsigned)e.y].push_back(e);
+    std::vector<unsigned> beg(nl + 1, 0);
+    std::vector<ExcEnt> ent;
+    for (unsigned i = 0; i < nl; ++i) {
+        std::stable_sort(lists[i].begin(), lists[i].end(), [](const ExcEnt& l, const ExcEnt& r) { return l.z < r.z; });
+        ent.insert(ent.end(), lists[i].begin(), lists[i].end());
+        beg[i + 1] = (unsigned)ent.size();
     }
+    unsigned* db;
+    ExcEnt* de;
+    unsigned* dm = nullptr;
+    int rc;
+    if ((rc = upload(h, &db, beg.data(), beg.size())) || (rc = upload(h, &de, ent.data(), ent.size()))) return rc;
+    if (w == 1 && h->g.tx == 0) {
+        // k_volt: tiles of kBlock consecutive columns (the single engine's tile
+        // shape); only tiles holding a source read the lists
+        const unsigned nt = (unsigned)((h->g.plane + kBlock - 1) / kBlock);
+        std::vector<unsigned> mask((nt + 31) / 32, 0u);
+        for (const auto& e : h->exc[0].host) {
+            const unsigned t = (unsigned)(((unsigned long long)e.x + (unsigned long long)h->N[0] * e.y) / kBlock);
+            mask[t >> 5] |= 1u << (t & 31);
+        }
+        if ((rc = upload(h, &dm, mask.data(), mask.size()))) return rc;
+    }
+    h->qsrc = ExcArgs{db, de, h->sig[0], h->sigLen, h->period, dm};
     return 0;
 }
 
@@ -4222,6 +4272,7 @@ int ems_init(ems_engine* h)
     if (rc) return rc;
     if ((rc = setup_fused(h))) return rc;
     if ((rc = setup_quad(h))) return rc;
+    if ((rc = setup_vsrc(h))) return rc;
     // small compressed tables (both half steps' within 32 KiB) go to LDS in the quad kernels
     h->lds_tab = h->cmp && (size_t)4 * h->n_unique * sizeof(float4) <= kLdsTabBytes;
     if (const char* lt = std::getenv("EMS_LDS_TAB")) h->lds_tab = h->lds_tab && std::atoi(lt) != 0;
@@ -4448,9 +4499,11 @@ template <int PM, bool CC>
 void launch_volt_t(ems_engine* h, unsigned nblk)
 {
     if (h->nt_fields)
-        hipLaunchKernelGGL((k_volt<PM, CC, true>), dim3(nblk), dim3(kBlock), 0, h->stream, h->g, h->V, h->I, h->kV, h->pmlV);
+        hipLaunchKernelGGL((k_volt<PM, CC, true>), dim3(nblk), dim3(kBlock), 0, h->stream, h->g, h->V, h->I, h->kV, h->pmlV,
+                           h->qsrc, h->numts_dev);
     else
-        hipLaunchKernelGGL((k_volt<PM, CC, false>), dim3(nblk), dim3(kBlock), 0, h->stream, h->g, h->V, h->I, h->kV, h->pmlV);
+        hipLaunchKernelGGL((k_volt<PM, CC, false>), dim3(nblk), dim3(kBlock), 0, h->stream, h->g, h->V, h->I, h->kV,
+                           h->pmlV, h->qsrc, h->numts_dev);
 }
 
 template <int PM, bool CC>
@@ -4799,7 +4852,7 @@ static int launch_step(ems_engine* h, hipEvent_t* ev)
         const auto& m = h->mur[k];
         hipLaunchKernelGGL(k_mur_apply, dim3((m.na * m.nb + T - 1) / T), dim3(T), 0, h->stream, h->V, m, h->numts_dev);
     }
-    if (h->exc[0].G && !(h->quad && h->qsrc.beg)) {
+    if (h->exc[0].G && !h->qsrc.beg) {
         const auto& e = h->exc[0];
         hipLaunchKernelGGL(k_excite, dim3((e.G + T - 1) / T), dim3(T), 0, h->stream, h->V, e.G, e.gbeg, e.glin, e.amp,
                            e.delay, h->sig[0], h->sigLen, h->period, h->numts_dev, 0);
@@ -5261,6 +5314,10 @@ struct ems_batch {
     bool cmp = false;
     std::vector<BatchInst> h_inst;
     BatchInst* d_inst = nullptr;
+    // voltage sources in the voltage half step (every instance has its
+    // lists); their tile masks hold for linear tiles only
+    bool vsrc = false;
+    std::vector<const unsigned*> vmask;
     // per-instance host lists, assembled into per-group device tables
     std::vector<std::vector<BatchExcGroup>> igrp[2];   // beg/end local to the instance's amp list
     std::vector<std::vector<float>> iamp[2];
@@ -5374,6 +5431,7 @@ int batch_build(ems_batch* b, unsigned tx, const std::vector<unsigned>& zd, unsi
         BatchInst& B = b->h_inst[i];
         B.g.zdepth = zd[i];
         set_tiles(B.g, tx);
+        if (b->vsrc) B.vsrc.tmask = tx == 0 ? b->vmask[i] : nullptr;
         if ((unsigned long long)B.g.qtiles * B.g.nzc > 0xFFFFFFFFull)
             return fail(EMS_E_ARG, "instance too large for the batch work table");
     }
@@ -5449,7 +5507,7 @@ void batch_step(ems_batch* b, const ems_batch::Group& g, hipStream_t st)
 {
     const unsigned T = 256;
     batch_launch<false>(b, g, st);
-    if (g.G[0])
+    if (g.G[0] && !b->vsrc)
         hipLaunchKernelGGL(k_batch_excite, dim3((g.G[0] + T - 1) / T), dim3(T), 0, st, b->d_inst, g.d_grp[0],
                            g.G[0], g.d_amp[0], g.d_delay[0], 0, 0);
     batch_launch<true>(b, g, st);
@@ -5564,6 +5622,7 @@ int ems_batch_create(ems_batch** out, ems_engine* const* engines, unsigned count
         B.sig[1] = e->sig[1];
         B.sigLen = e->sigLen;
         B.period = e->period;
+        B.vsrc = e->quad ? ExcArgs{} : e->qsrc;
         b->h_inst.push_back(B);
         b->zdepth.push_back(e->g.zdepth);
         // device working set: fields, coefficients, UPML
@@ -5598,6 +5657,12 @@ int ems_batch_create(ems_batch** out, ems_engine* const* engines, unsigned count
             q.ediv = e->d_ediv;
             b->iprobes[i].push_back(q);
         }
+    }
+    b->vsrc = true;
+    for (const auto& B : b->h_inst) b->vsrc = b->vsrc && B.vsrc.beg;
+    for (auto& B : b->h_inst) {
+        if (!b->vsrc) B.vsrc = ExcArgs{};
+        b->vmask.push_back(B.vsrc.tmask);
     }
     if ((rc = b->alloc(&b->d_inst, count))) return bail(rc);
     hipDeviceProp_t prop;
