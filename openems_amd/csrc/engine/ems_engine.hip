@@ -1782,7 +1782,7 @@ __device__ __forceinline__ void mur_fold_quad(const MurFold& M, unsigned k, floa
 }
 
 template <bool CC, bool NT, bool LT>
-__global__ __launch_bounds__(kBlock) void k_step4t(const Step4Args ag, const float* __restrict__ V0,
+__global__ __launch_bounds__(kBlock, 3) void k_step4t(const Step4Args ag, const float* __restrict__ V0,
                                                    const float* __restrict__ I0, float* __restrict__ V1,
                                                    float* __restrict__ I1)
 {
@@ -1831,8 +1831,9 @@ __global__ __launch_bounds__(kBlock) void k_step4t(const Step4Args ag, const flo
     // pre and post hooks (Engine_Ext_Mur_ABC::DoPreVoltageUpdates /
     // DoPostVoltageUpdates, engine_ext_mur_abc.cpp:68-142) and uses the value
     // the apply hook will set for the face's y and z components in its own
-    // current update; V^{n+1} itself is stored as swept, for the other faces'
-    // hooks, and k_mur_apply writes the face afterwards
+    // current update; it stores that value too, except on the face's rim,
+    // where the other faces' hooks still read or write, and which
+    // k_mur_apply_rim sets afterwards
     const bool flo = a.mf[0].on && x0 == 0 && (unsigned)ts >= a.mf[0].start_ts;
     const bool fhi = a.mf[1].on && x0 + 4 == Nx && (unsigned)ts >= a.mf[1].start_ts;
     SrcCursor cv = src_cursor(a.eV, evb, eve, z0), ci = src_cursor(a.eI, eib, eie, z0);
@@ -1906,12 +1907,10 @@ __global__ __launch_bounds__(kBlock) void k_step4t(const Step4Args ag, const flo
             quad_coef_n<CC>(a.kV, cs, p, u, 2, c, w);
             v[2] = quad_math(v0[2], c, w, quad_curl(iy, iy_xm, ix, ix_ym));
             quad_sources_at(a.eV, cv, x0, z, 7u, ts, v);
-            if (flo) mur_fold_quad<0, 1>(a.mf[0], z * a.ny + y, mcp, mcpp, v0, v, active && z < z1, fy, fz);
-            else if (fhi) mur_fold_quad<3, 2>(a.mf[1], z * a.ny + y, mcp, mcpp, v0, v, active && z < z1, fy, fz);
-        }
-        if (active && z < z1) {
-#pragma unroll
-            for (int n = 0; n < 3; ++n) st4<NT>(V1 + n * cs + p, v[n]);
+            // the hook values are kept for k_mur_apply_rim on the face's rim
+            const bool keep = active && z < z1 && (y < 2 || y + 2 >= a.ny || z < 2 || z + 2 >= a.zb);
+            if (flo) mur_fold_quad<0, 1>(a.mf[0], z * a.ny + y, mcp, mcpp, v0, v, keep, fy, fz);
+            else if (fhi) mur_fold_quad<3, 2>(a.mf[1], z * a.ny + y, mcp, mcpp, v0, v, keep, fy, fz);
         }
         // V^{n+1} as the currents read it: the face element after the Mur apply
         float4 vq1 = v[1], vq2 = v[2];
@@ -1921,6 +1920,18 @@ __global__ __launch_bounds__(kBlock) void k_step4t(const Step4Args ag, const flo
         } else if (fhi) {
             vq1.w = fy;
             vq2.w = fz;
+        }
+        if (active && z < z1) {
+            // face points off the face's rim (no other face's hook reads or
+            // writes them) are stored with the apply's value; k_mur_apply_rim
+            // sets the rim after the other faces' post hooks
+#pragma unroll
+            for (int n = 0; n < 3; ++n) st4<NT>(V1 + n * cs + p, v[n]);
+            if ((flo || fhi) && y >= 2 && y + 2 < a.ny && z >= 2 && z + 2 < a.zb) {
+                const unsigned long long pf = p + (flo ? 0u : 3u);
+                V1[cs + pf] = fy;
+                V1[2 * cs + pf] = fz;
+            }
         }
         s_vz[b][threadIdx.x] = vq2;
         s_vx[b][threadIdx.x] = v[0];
@@ -2099,6 +2110,19 @@ __global__ void k_mur_apply(float* __restrict__ V, const MurDev m, const unsigne
 {
     const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= m.na * m.nb || *numts < m.start_ts) return;
+    const unsigned f = m.face[k];
+    V[m.offP + f] = m.mP[k];
+    V[m.offPP + f] = m.mPP[k];
+}
+
+// k_mur_apply at the listed points only (the rim of a face whose other
+// points the tiled quad sweep sets)
+__global__ void k_mur_apply_rim(float* __restrict__ V, const MurDev m, const unsigned* __restrict__ rim, unsigned n,
+                                const unsigned* __restrict__ numts)
+{
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n || *numts < m.start_ts) return;
+    const unsigned k = rim[t];
     const unsigned f = m.face[k];
     V[m.offP + f] = m.mP[k];
     V[m.offPP + f] = m.mPP[k];
@@ -3042,6 +3066,8 @@ struct ems_engine {
     bool q4tile = false;                  // form 3 runs k_step4t (64 x 4 quad tiles)
     unsigned q4ntq = 0;                   // its tiles along x
     int mur_fold[2] = {-1, -1};           // Mur faces (index) on x = 0 / x = Nx-1 whose pre and post hooks k_step4t runs
+    unsigned* mur_rim[2] = {nullptr, nullptr};    // their rim points (y or z within 2 lines of the mesh edge)
+    unsigned nrim[2] = {0, 0};
     ExcArgs eV{}, eI{};
     int pml_mode = kNoPml;
     unsigned n_unique_pml = 0;
@@ -3945,6 +3971,17 @@ static int setup_fused(ems_engine* h)
         if ((rc = upload(h, &h->cell_list, cells.data(), cells.size()))) return rc;
         h->ncells = (unsigned)cells.size();
     }
+    for (int side = 0; side < 2; ++side) {
+        h->nrim[side] = 0;
+        if (h->mur_fold[side] < 0) continue;
+        // points z * Ny + y of an x face with y or z within 2 lines of the edge
+        std::vector<unsigned> rim;
+        for (unsigned z = 0; z < Nz; ++z)
+            for (unsigned y = 0; y < Ny; ++y)
+                if (y < 2 || y + 2 >= Ny || z < 2 || z + 2 >= Nz) rim.push_back(z * Ny + y);
+        if ((rc = upload(h, &h->mur_rim[side], rim.data(), rim.size()))) return rc;
+        h->nrim[side] = (unsigned)rim.size();
+    }
     for (int c = 0; c < 2; ++c) {
         std::vector<unsigned> beg(nlists + 1, 0);
         std::vector<ExcEnt> ent;
@@ -4736,6 +4773,13 @@ void fused_step(ems_engine* h, hipEvent_t* ev)
         }
         for (int k = (int)h->mur.size() - 1; k >= 0; --k) {
             const auto& m = h->mur[k];
+            const int side = k == h->mur_fold[0] ? 0 : k == h->mur_fold[1] ? 1 : -1;
+            if (side >= 0) {
+                if (h->nrim[side])
+                    hipLaunchKernelGGL(k_mur_apply_rim, dim3((h->nrim[side] + T - 1) / T), dim3(T), 0, h->stream,
+                                       h->V_alt, m, h->mur_rim[side], h->nrim[side], h->numts_dev);
+                continue;
+            }
             hipLaunchKernelGGL(k_mur_apply, dim3((m.na * m.nb + T - 1) / T), dim3(T), 0, h->stream, h->V_alt, m,
                                h->numts_dev);
         }
