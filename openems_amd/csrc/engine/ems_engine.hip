@@ -2312,23 +2312,24 @@ __device__ void probe_block(const ProbeDev P, const float* __restrict__ V, const
     double acc_d = 0.0;
     float acc_f = 0.0f;
     double out[3] = {0.0, 0.0, 0.0};
+    const bool integral = P.type == EMS_PROBE_VOLTAGE || P.type == EMS_PROBE_CURRENT;
     for (unsigned base = 0; base < P.nelem; base += kProbeChunk) {
         const unsigned m = min(kProbeChunk, P.nelem - base);
-        for (unsigned e = threadIdx.x; e < m; e += blockDim.x) vals[e] = F[elin[P.off + base + e]];
+        // gathered with the term's sign (a - b and a + (-b) round alike), so
+        // the ordered sum below reads nothing from global memory
+        for (unsigned e = threadIdx.x; e < m; e += blockDim.x) {
+            const unsigned j = P.off + base + e;
+            const float v = F[elin[j]];
+            vals[e] = integral && esign[j] <= 0 ? -v : v;
+        }
         __syncthreads();
         if (threadIdx.x == 0) {
             for (unsigned e = 0; e < m; ++e) {
                 const unsigned j = P.off + base + e;
                 if (P.type == EMS_PROBE_VOLTAGE) {
-                    if (esign[j] > 0)
-                        acc_d = acc_d + (double)vals[e];
-                    else
-                        acc_d = acc_d - (double)vals[e];
+                    acc_d = acc_d + (double)vals[e];
                 } else if (P.type == EMS_PROBE_CURRENT) {
-                    if (esign[j] > 0)
-                        acc_f = acc_f + vals[e];
-                    else
-                        acc_f = acc_f - vals[e];
+                    acc_f = acc_f + vals[e];
                 } else if (base + e < 3) {
                     const double d = ediv[j];
                     const double r = d != 0.0 ? (double)vals[e] / d : 0.0;
@@ -2344,7 +2345,6 @@ __device__ void probe_block(const ProbeDev P, const float* __restrict__ V, const
         const unsigned slot = *P.count;
         if (slot < P.capacity) {
             // the three results as plain selects, stored unconditionally
-            const bool integral = P.type == EMS_PROBE_VOLTAGE || P.type == EMS_PROBE_CURRENT;
             const double r0 = P.type == EMS_PROBE_VOLTAGE ? acc_d : P.type == EMS_PROBE_CURRENT ? (double)acc_f : out[0];
             const double r1 = integral ? 0.0 : out[1];
             const double r2 = integral ? 0.0 : out[2];
