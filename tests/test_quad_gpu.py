@@ -243,3 +243,31 @@ def test_quad_split_boxes_match_single_box(gpu, pml, monkeypatch):
         out.append(_run(op, 25, "compressed", quad, init))
     for v, c, _ in out[1:]:
         assert np.array_equal(out[0][0], v) and np.array_equal(out[0][1], c)
+
+
+@pytest.mark.parametrize("chunk", [8, 7])
+def test_tiled_sweep_graph_chunks_match_direct_launches(gpu, chunk):
+    """The automatic tiled quad sweep with Mur faces (x-face hooks inside the
+    sweep), E-point and voltage probes: timestep chunks replayed as HIP graphs
+    give the same fields and probe series as direct launches, over several
+    iterate calls (odd and even step counts, so both buffer parities are
+    captured)."""
+    from openems_amd.simulation import Simulation
+    scene = scenes.mur_layers(264, 22, 18, 420)
+    scene["steady_state"] = False
+    print("case mur_layers_264x22x18 sweep graphs, chunk", chunk, flush=True)
+    ref = Simulation(scene, graph=0)
+    sim = Simulation(scene, graph=chunk)
+    assert sim.eng.fused()[0] == 3 and ref.eng.fused()[0] == 3
+    for k in (37, 100, 83, 200):
+        ref.eng.IterateTS(k)
+        sim.eng.IterateTS(k)
+        assert np.array_equal(sim.eng.get_volt(), ref.eng.get_volt())
+        assert np.array_equal(sim.eng.get_curr(), ref.eng.get_curr())
+    assert sim.eng.graph()[2] > 0
+    a, b = sim.results(), ref.results()
+    for name, pr in b["probes"].items():
+        assert np.array_equal(a["probes"][name]["td"], pr["td"])
+    assert max(np.abs(pr["td"][:, 1:]).max() for pr in b["probes"].values()) > 0
+    sim.close()
+    ref.close()
