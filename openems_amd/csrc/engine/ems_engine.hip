@@ -1756,9 +1756,12 @@ __global__ __launch_bounds__(kBlock) void k_step4(const Step4Args ag, const floa
 // barrier per plane separates every write from the reads of the same
 // buffer), element 0 of quad x0+4 from the next lane of the wave.  Only the
 // tile's top row (wave 3) recomputes row y+1 as k_step4 does, and only lane
-// 63 recomputes cell (x0+4, y).  Arithmetic, sources and the Mur contract
-// are k_step4's; lanes past the mesh run the loop on clamped positions and
-// store nothing.
+// 63 recomputes cell (x0+4, y); the x-1 neighbours of the voltage update
+// come from the previous lane and I^n of plane z-1 is carried.  Sources are
+// per-quad lists ordered by plane, consumed through cursors.  Arithmetic
+// is k_step4's; the Mur faces on x = 0 and x = Nx-1 run their pre and post
+// hooks here (below), the other faces' hooks around the launch.  Lanes past
+// the mesh run the loop on clamped positions and store nothing.
 // ---------------------------------------------------------------------------
 constexpr unsigned kQTileRows = kBlock / 64;
 
