@@ -4027,14 +4027,14 @@ static int setup_fused(ems_engine* h)
         h->fnzc = (Nz + h->fzdepth - 1) / h->fzdepth;
     } else if (form == 3) {
         // tiles of kBlock quads over the plane (k_step4), or of 64 quads x 4
-        // rows (k_step4t, rows of at least 64 quads); 8 planes per chunk (64
+        // rows (k_step4t, rows of at least 64 quads); 8 planes per chunk (128
         // for the tiles: 10.72, 10.61, 10.51 ms per sweep on the Mur layers
-        // at 16, 32, 64), shallower when the grid would give fewer than 2048
-        // workgroups
+        // at 16, 32, 64, and 10.47 against 10.53 at 128), shallower when the
+        // grid would give fewer than 2048 workgroups
         const unsigned NQ = Nx / 4;
         h->q4ntq = (NQ + 63) / 64;
         h->ntx = h->q4tile ? h->q4ntq * ((Ny + kQTileRows - 1) / kQTileRows) : (h->g.plane / 4 + kBlock - 1) / kBlock;
-        unsigned d = h->q4tile ? 64 : 8;
+        unsigned d = h->q4tile ? 128 : 8;
         while (d > 2 && (unsigned long long)h->ntx * ((Nz + d - 1) / d) < 2048ull) d /= 2;
         if (const char* zd = std::getenv("EMS_Q4ZDEPTH")) {
             const int v = std::atoi(zd);
