@@ -349,6 +349,15 @@ __device__ __forceinline__ float ld_own(const float* q)
     return *q;
 }
 
+// a cell's coefficient index, read once per half step: non-temporal with
+// the own field (256^3 UPML: 0.3361 -> 0.3350 ms per step)
+template <bool NT>
+__device__ __forceinline__ unsigned cidx(const unsigned short* q)
+{
+    if (NT) return (unsigned)__builtin_nontemporal_load(q);
+    return (unsigned)*q;
+}
+
 template <bool NT>
 __device__ __forceinline__ void st_own(float* q, float v)
 {
@@ -496,7 +505,7 @@ __device__ __forceinline__ void volt_tile(const GridDesc& g, unsigned t, float* 
         float x0[3], a[3], b[3], out[3];
 #pragma unroll
         for (int n = 0; n < 3; ++n) x0[n] = ld_own<NT>(&V[n * cs + p]);
-        load_coef<CC>(k, cs, p, CC ? k.idx[p] : 0u, a, b);
+        load_coef<CC>(k, cs, p, CC ? cidx<NT>(k.idx + p) : 0u, a, b);
         unsigned li = 0;
         const bool in_pml = PM != kNoPml && pcm && pml_at_col(pml, pcm, (int)x, (int)y, (int)z, li);
         cell_math<PM>(x0, a, b, curl, in_pml, pml, li, true, out);
@@ -567,7 +576,7 @@ __device__ __forceinline__ void curr_tile(const GridDesc& g, unsigned t, float* 
         float x0[3], a[3], b[3], out[3];
 #pragma unroll
         for (int n = 0; n < 3; ++n) x0[n] = ld_own<NT>(&I[n * cs + p]);
-        load_coef<CC>(k, cs, p, CC ? k.idx[p] : 0u, a, b);
+        load_coef<CC>(k, cs, p, CC ? cidx<NT>(k.idx + p) : 0u, a, b);
         unsigned li = 0;
         const bool in_pml = PM != kNoPml && pcm && pml_at_col(pml, pcm, (int)x, (int)y, (int)z, li);
         cell_math<PM>(x0, a, b, curl, in_pml, pml, li, main, out, main);
